@@ -1,0 +1,725 @@
+/* ipm_oracle.c - TEST INFRASTRUCTURE ONLY. A plain-C, single-threaded, FP64 restatement of the
+ * reference's primal-dual interior-point iteration for block-diagonal SDP/LP (the infeasible-
+ * start method with the HKM direction and Mehrotra's predictor-corrector), used as the checker
+ * of the GPU path. Only tests/, the smoke entry and bench.py's CPU-baseline leg load it; the
+ * product library never does.
+ *
+ * What it follows (reference file:line):
+ *   scaling            problem.c:1866-1909 (d_i = 1/||A_i||_F, bs, cs, du)
+ *   start point        solver.c:13940-14210 (least-norm X0, least-squares y0 on A A', shifts)
+ *   R0 / theta         solver.c:14316-14329, re-base solver.c:14614-14636
+ *   residuals/scores   solver.c:14426-14482, res_score solver.c:8089-8097
+ *   stopping rules     solver.c:14567-14600 (tolerance, infeasibility, maxit, stall windows)
+ *   Schur route 0      solver.c:2626-2660 (M_ab = sum v_p v_q X[c_p,r_q] Zi[c_q,r_p])
+ *   factor             solver.c:5266-5420 (equilibration, FP64 shift ladder, pivot test)
+ *   PCG solve          solver.c:5483-5622
+ *   predictor          solver.c:14787-14818, sigma/gamma solver.c:14829-14872
+ *   corrector          solver.c:14997-15055, refinement solver.c:15058-15105
+ *   step lengths       solver.c:5947-6027 (tridiagonalisation + Sturm bisection, used here for
+ *                      every block size: the exact lambda_min the Lanczos bound approximates)
+ *   catastrophe guard  solver.c:15143-15160, try_step solver.c:6393-6446, update 15185-15188
+ *   DIMACS errors      postsolve.c:3037-3112, status rule main.c:2349-2359, brisk.h:187-189
+ *
+ * Deliberately absent (documented as out of scope): presolve, HSD embedding, NT direction,
+ * correctors, polish/restoration and the double-double endgame.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+typedef struct { int n, lp; size_t off; } OBlk;          /* off into the packed block arrays */
+typedef struct { int nnz; int *r, *c; double *v; } OEnt;  /* full symmetric expansion, 0-based */
+typedef struct {
+    int m, nb;
+    OBlk *B;
+    size_t tot;               /* packed length of all blocks (n*n SDP, n LP) */
+    double Ntot;
+    OEnt *A;                  /* A[k * m + i]: entries of constraint i in block k */
+    double *C;                /* packed dense C (scaled) */
+    double *b, *b0, *d, *du;
+    double normb1, normb2, normC1, normC2, bs, cs;
+} OProb;
+
+/* ------------------------------------------------------------------------------------- */
+static double dot(size_t n, const double *a, const double *b) { double s = 0; for (size_t i = 0; i < n; i++) s += a[i] * b[i]; return s; }
+
+/* lower Cholesky in place (column-major); returns 0 or the failing column + 1 */
+static int chol(int n, double *A) {
+    for (int j = 0; j < n; j++) {
+        double d = A[j + (size_t)j * n];
+        for (int k = 0; k < j; k++) d -= A[j + (size_t)k * n] * A[j + (size_t)k * n];
+        if (!(d > 0)) return j + 1;
+        d = sqrt(d);
+        A[j + (size_t)j * n] = d;
+        for (int i = j + 1; i < n; i++) {
+            double s = A[i + (size_t)j * n];
+            for (int k = 0; k < j; k++) s -= A[i + (size_t)k * n] * A[j + (size_t)k * n];
+            A[i + (size_t)j * n] = s / d;
+        }
+    }
+    for (int j = 0; j < n; j++) for (int i = 0; i < j; i++) A[i + (size_t)j * n] = 0;
+    return 0;
+}
+/* x <- L^{-1} x, x <- L^{-T} x */
+static void lsolve(int n, const double *L, double *x) {
+    for (int i = 0; i < n; i++) { double s = x[i]; for (int k = 0; k < i; k++) s -= L[i + (size_t)k * n] * x[k]; x[i] = s / L[i + (size_t)i * n]; }
+}
+static void ltsolve(int n, const double *L, double *x) {
+    for (int i = n - 1; i >= 0; i--) { double s = x[i]; for (int k = i + 1; k < n; k++) s -= L[k + (size_t)i * n] * x[k]; x[i] = s / L[i + (size_t)i * n]; }
+}
+/* Vi = (L L')^{-1} (full) */
+static void inv_from_chol(int n, const double *L, double *Vi) {
+    memset(Vi, 0, sizeof(double) * (size_t)n * n);
+    for (int j = 0; j < n; j++) { double *x = Vi + (size_t)j * n; x[j] = 1; lsolve(n, L, x); ltsolve(n, L, x); }
+    for (int j = 0; j < n; j++) for (int i = j + 1; i < n; i++) { double v = 0.5 * (Vi[i + (size_t)j * n] + Vi[j + (size_t)i * n]); Vi[i + (size_t)j * n] = Vi[j + (size_t)i * n] = v; }
+}
+static void gemm_nn(int n, const double *A, const double *B, double *C) {     /* C = A B */
+    for (int j = 0; j < n; j++) {
+        double *c = C + (size_t)j * n;
+        for (int i = 0; i < n; i++) c[i] = 0;
+        for (int k = 0; k < n; k++) { const double bk = B[k + (size_t)j * n]; const double *a = A + (size_t)k * n; for (int i = 0; i < n; i++) c[i] += a[i] * bk; }
+    }
+}
+static void symmetrize(int n, double *G) {
+    for (int j = 0; j < n; j++) for (int i = j + 1; i < n; i++) { double v = 0.5 * (G[i + (size_t)j * n] + G[j + (size_t)i * n]); G[i + (size_t)j * n] = G[j + (size_t)i * n] = v; }
+}
+/* smallest eigenvalue of a symmetric matrix: Householder tridiagonalisation, then Sturm-count
+ * bisection (the method of the reference's small_lammin, solver.c:5947-6027); B is destroyed.
+ * The lower end of the final bracket is returned, less a rounding margin. */
+double oracle_lammin(int n, double *B) {
+    if (n == 1) return B[0];
+    double *d = malloc(sizeof(double) * n), *e = calloc(n, sizeof(double)), *v = malloc(sizeof(double) * n), *p = malloc(sizeof(double) * n);
+    for (int k = 0; k + 2 < n; k++) {
+        const int r = n - k - 1;
+        double *x = &B[(k + 1) + (size_t)k * n];
+        double sig = 0;
+        for (int i = 1; i < r; i++) sig += x[i] * x[i];
+        if (sig == 0) { e[k] = x[0]; continue; }
+        const double a = sqrt(x[0] * x[0] + sig), alpha = x[0] > 0 ? -a : a;
+        v[0] = x[0] - alpha;
+        for (int i = 1; i < r; i++) v[i] = x[i];
+        const double beta = 2.0 / (v[0] * v[0] + sig);
+        e[k] = alpha;
+        double pv = 0;
+        for (int i = 0; i < r; i++) {
+            double t = 0;
+            for (int j = 0; j < r; j++) t += B[(k + 1 + i) + (size_t)(k + 1 + j) * n] * v[j];
+            p[i] = beta * t; pv += p[i] * v[i];
+        }
+        const double hb = 0.5 * beta * pv;
+        for (int i = 0; i < r; i++) p[i] -= hb * v[i];
+        for (int j = 0; j < r; j++)
+            for (int i = j; i < r; i++) {
+                double t = B[(k + 1 + i) + (size_t)(k + 1 + j) * n] - v[i] * p[j] - p[i] * v[j];
+                B[(k + 1 + i) + (size_t)(k + 1 + j) * n] = t; B[(k + 1 + j) + (size_t)(k + 1 + i) * n] = t;
+            }
+    }
+    for (int k = 0; k < n; k++) d[k] = B[k + (size_t)k * n];
+    e[n - 2] = B[(n - 1) + (size_t)(n - 2) * n];
+    double lo = INFINITY, hi = INFINITY, sc = 0;
+    for (int k = 0; k < n; k++) {
+        const double rad = (k > 0 ? fabs(e[k - 1]) : 0) + (k + 1 < n ? fabs(e[k]) : 0);
+        lo = fmin(lo, d[k] - rad); hi = fmin(hi, d[k]); sc = fmax(sc, fabs(d[k]) + rad);
+    }
+    for (int it = 0; it < 200; it++) {
+        const double w = hi - lo;
+        if (w <= 1e-13 * fmax(fabs(lo), fabs(hi)) || w <= 1e-15 * sc) break;
+        const double x = 0.5 * (lo + hi);
+        int cnt = 0;
+        double q = d[0] - x;
+        if (q < 0) cnt++;
+        for (int k = 1; k < n; k++) {
+            if (fabs(q) < 1e-300) q = q < 0 ? -1e-300 : 1e-300;
+            q = d[k] - x - e[k - 1] * e[k - 1] / q;
+            if (q < 0) cnt++;
+        }
+        if (cnt >= 1) hi = x; else lo = x;
+    }
+    free(d); free(e); free(v); free(p);
+    return lo - 1e-14 * sc;
+}
+
+/* ------------------------------------------------------------------------------------- */
+static int prob_build(const OracleData *D, OProb *P) {
+    memset(P, 0, sizeof(*P));
+    const int m = D->m, nb = D->nblk;
+    P->m = m; P->nb = nb;
+    P->B = calloc(nb, sizeof(OBlk));
+    size_t off = 0;
+    for (int k = 0; k < nb; k++) {
+        const int n = abs(D->bs[k]);
+        P->B[k].n = n; P->B[k].lp = D->bs[k] < 0; P->B[k].off = off;
+        off += P->B[k].lp ? (size_t)n : (size_t)n * n;
+        P->Ntot += n;
+    }
+    P->tot = off;
+    P->C = calloc(off + 1, sizeof(double));
+    P->A = calloc((size_t)nb * m, sizeof(OEnt));
+    P->b = malloc(sizeof(double) * (m + 1)); P->b0 = malloc(sizeof(double) * (m + 1));
+    for (int i = 0; i < m; i++) P->b[i] = P->b0[i] = D->c[i];
+    int *cnt = calloc((size_t)nb * m, sizeof(int));
+    for (int64_t q = 0; q < D->nnz; q++) {
+        const int t = D->mat[q], k = D->blk[q] - 1;
+        int i = D->i[q] - 1, j = D->j[q] - 1;
+        if (t < 0 || t > m || k < 0 || k >= nb || i < 0 || j < 0 || i >= P->B[k].n || j >= P->B[k].n) return 2;
+        if (D->v[q] == 0) continue;
+        if (P->B[k].lp && i != j) return 2;
+        if (t == 0) continue;
+        cnt[(size_t)k * m + t - 1] += (i == j || P->B[k].lp) ? 1 : 2;
+    }
+    for (size_t q = 0; q < (size_t)nb * m; q++)
+        if (cnt[q]) { P->A[q].r = malloc(sizeof(int) * cnt[q]); P->A[q].c = malloc(sizeof(int) * cnt[q]); P->A[q].v = malloc(sizeof(double) * cnt[q]); }
+    for (int64_t q = 0; q < D->nnz; q++) {
+        const int t = D->mat[q], k = D->blk[q] - 1;
+        int i = D->i[q] - 1, j = D->j[q] - 1;
+        const double v = D->v[q];
+        if (v == 0) continue;
+        if (i > j) { int s = i; i = j; j = s; }
+        const OBlk *B = &P->B[k];
+        if (t == 0) {   /* C = -F0 */
+            if (B->lp) P->C[B->off + i] += -v;
+            else { P->C[B->off + i + (size_t)j * B->n] += -v; if (i != j) P->C[B->off + j + (size_t)i * B->n] += -v; }
+            continue;
+        }
+        OEnt *E = &P->A[(size_t)k * m + t - 1];
+        E->r[E->nnz] = i; E->c[E->nnz] = j; E->v[E->nnz] = v; E->nnz++;
+        if (i != j && !B->lp) { E->r[E->nnz] = j; E->c[E->nnz] = i; E->v[E->nnz] = v; E->nnz++; }
+    }
+    free(cnt);
+    /* norms on the data as read (problem.c:1866-1881) */
+    double c2 = 0, c1 = 0;
+    for (size_t q = 0; q < off; q++) { c2 += P->C[q] * P->C[q]; c1 += fabs(P->C[q]); }
+    P->normC2 = sqrt(c2); P->normC1 = c1;
+    for (int i = 0; i < m; i++) { P->normb1 += fabs(P->b0[i]); P->normb2 += P->b0[i] * P->b0[i]; }
+    P->normb2 = sqrt(P->normb2);
+    /* row equilibration and scaling (problem.c:1884-1909) */
+    P->d = malloc(sizeof(double) * (m + 1)); P->du = malloc(sizeof(double) * (m + 1));
+    for (int i = 0; i < m; i++) {
+        double s = 0;
+        for (int k = 0; k < nb; k++) { const OEnt *E = &P->A[(size_t)k * m + i]; for (int q = 0; q < E->nnz; q++) s += E->v[q] * E->v[q]; }
+        P->d[i] = s > 0 ? 1.0 / sqrt(s) : 1.0;
+    }
+    double nbv = 0;
+    for (int i = 0; i < m; i++) nbv += P->d[i] * P->b[i] * P->d[i] * P->b[i];
+    nbv = sqrt(nbv);
+    P->bs = nbv > 1 ? nbv : 1.0;
+    for (int k = 0; k < nb; k++) for (int i = 0; i < m; i++) { OEnt *E = &P->A[(size_t)k * m + i]; for (int q = 0; q < E->nnz; q++) E->v[q] *= P->d[i]; }
+    for (int i = 0; i < m; i++) { P->b[i] *= P->d[i]; P->du[i] = P->bs / P->d[i]; }
+    P->cs = P->normC2 > 1 ? P->normC2 : 1.0;
+    for (int i = 0; i < m; i++) P->b[i] /= P->bs;
+    for (size_t q = 0; q < off; q++) P->C[q] /= P->cs;
+    return 0;
+}
+static void prob_free(OProb *P) {
+    for (size_t q = 0; q < (size_t)P->nb * P->m; q++) { free(P->A[q].r); free(P->A[q].c); free(P->A[q].v); }
+    free(P->A); free(P->B); free(P->C); free(P->b); free(P->b0); free(P->d); free(P->du);
+}
+
+/* out[i] += alpha * <A_i, D> (D packed over blocks; nonsymmetric D is read as sym(D)) */
+static void Aop(const OProb *P, const double *D, double *out, double alpha) {
+    for (int i = 0; i < P->m; i++) {
+        double s = 0;
+        for (int k = 0; k < P->nb; k++) {
+            const OEnt *E = &P->A[(size_t)k * P->m + i];
+            const OBlk *B = &P->B[k];
+            const double *Dk = D + B->off;
+            for (int q = 0; q < E->nnz; q++) s += E->v[q] * (B->lp ? Dk[E->r[q]] : Dk[E->r[q] + (size_t)E->c[q] * B->n]);
+        }
+        out[i] += alpha * s;
+    }
+}
+/* D = sum_i y_i A_i (packed) */
+static void ATy(const OProb *P, const double *y, double *D) {
+    memset(D, 0, sizeof(double) * P->tot);
+    for (int k = 0; k < P->nb; k++) {
+        const OBlk *B = &P->B[k];
+        double *Dk = D + B->off;
+        for (int i = 0; i < P->m; i++) {
+            const OEnt *E = &P->A[(size_t)k * P->m + i];
+            for (int q = 0; q < E->nnz; q++) {
+                if (B->lp) Dk[E->r[q]] += y[i] * E->v[q];
+                else Dk[E->r[q] + (size_t)E->c[q] * B->n] += y[i] * E->v[q];
+            }
+        }
+    }
+}
+/* M (m x m, full) = sum_blocks route-0 products; LP: sum_p a_ip a_jp x_p zi_p */
+void oracle_schur_packed(const void *Pv, const double *X, const double *Zi, double *M);
+static void schur(const OProb *P, const double *X, const double *Zi, double *M) {
+    const int m = P->m;
+    memset(M, 0, sizeof(double) * (size_t)m * m);
+    for (int k = 0; k < P->nb; k++) {
+        const OBlk *B = &P->B[k];
+        const double *Xk = X + B->off, *Zk = Zi + B->off;
+        const int n = B->n;
+        for (int a = 0; a < m; a++) {
+            const OEnt *Ea = &P->A[(size_t)k * m + a];
+            if (!Ea->nnz) continue;
+            for (int b = 0; b <= a; b++) {
+                const OEnt *Eb = &P->A[(size_t)k * m + b];
+                if (!Eb->nnz) continue;
+                double s = 0;
+                for (int p = 0; p < Ea->nnz; p++)
+                    for (int q = 0; q < Eb->nnz; q++) {
+                        if (B->lp) { if (Ea->r[p] == Eb->r[q]) s += Ea->v[p] * Eb->v[q] * Xk[Ea->r[p]] * Zk[Ea->r[p]]; }
+                        else s += Ea->v[p] * Eb->v[q] * Xk[Ea->c[p] + (size_t)Eb->r[q] * n] * Zk[Eb->c[q] + (size_t)Ea->r[p] * n];
+                    }
+                M[a + (size_t)b * m] += s;
+            }
+        }
+    }
+    for (int a = 0; a < m; a++) for (int b = 0; b < a; b++) M[b + (size_t)a * m] = M[a + (size_t)b * m];
+}
+
+/* ---- the Schur factor and solve (solver.c:5266-5420, 5483-5622) ---- */
+typedef struct { int m; double *M, *L, *D; double reg; int reg_hint; double reg_ok; } OSchur;
+static int sc_factor(OSchur *S, int *nfail) {
+    const int m = S->m;
+    for (int i = 0; i < m; i++) { const double d = S->M[i + (size_t)i * m]; S->D[i] = d > 0 ? 1.0 / sqrt(d) : 1.0; }
+    const double floor_ = fmax(1e-15, m * 2.220446049250313e-16);
+    double reg = S->reg_hint ? fmax(floor_, 0.01 * S->reg_ok) : 0;
+    for (int attempt = 0; attempt < 10; attempt++) {
+        for (int j = 0; j < m; j++) for (int i = 0; i < m; i++) S->L[i + (size_t)j * m] = S->M[i + (size_t)j * m] * S->D[i] * S->D[j];
+        for (int i = 0; i < m; i++) S->L[i + (size_t)i * m] += reg;
+        if (chol(m, S->L) == 0) {
+            double pmin = 1.0;
+            for (int i = 0; i < m; i++) { double p = S->L[i + (size_t)i * m]; if (p * p < pmin) pmin = p * p; }
+            if (pmin >= 1e-14 || reg >= floor_) { S->reg = reg; S->reg_hint = reg > 0; S->reg_ok = reg; return 0; }
+            reg = fmax(floor_, reg * 100);
+            continue;
+        }
+        if (nfail) (*nfail)++;
+        reg = reg == 0 ? floor_ : reg * 100;
+    }
+    return 1;
+}
+static void sc_prec(const OSchur *S, const double *r, double *z) {
+    for (int i = 0; i < S->m; i++) z[i] = r[i] * S->D[i];
+    lsolve(S->m, S->L, z); ltsolve(S->m, S->L, z);
+    for (int i = 0; i < S->m; i++) z[i] *= S->D[i];
+}
+static void sc_mv(const OSchur *S, const double *x, double *y) {
+    const int m = S->m;
+    for (int i = 0; i < m; i++) y[i] = 0;
+    for (int j = 0; j < m; j++) { const double xj = x[j]; const double *c = S->M + (size_t)j * m; for (int i = 0; i < m; i++) y[i] += c[i] * xj; }
+}
+static double sc_solve(const OSchur *S, const double *rhs, double *x, double tol_rel) {
+    const int m = S->m;
+    double nb = sqrt(dot(m, rhs, rhs));
+    if (nb == 0) { memset(x, 0, sizeof(double) * m); return 0; }
+    double *r = malloc(sizeof(double) * m * 5), *z = r + m, *p = z + m, *q = p + m, *xb = q + m;
+    sc_prec(S, rhs, x);
+    sc_mv(S, x, q);
+    for (int i = 0; i < m; i++) r[i] = rhs[i] - q[i];
+    double nr = sqrt(dot(m, r, r));
+    const double tol = fmax(tol_rel, 1e-13) * nb;
+    if (nr <= tol) { free(r); return nr / nb; }
+    double best = nr;
+    memcpy(xb, x, sizeof(double) * m);
+    sc_prec(S, r, z);
+    memcpy(p, z, sizeof(double) * m);
+    double rz = dot(m, r, z);
+    const int stag = S->reg == 0;
+    const int maxit = S->reg > 0 ? 50 : 10, smax = stag ? 1 : 4;
+    const double sgain = stag ? 0.5 : 0.9;
+    int stall = 0;
+    for (int k = 0; k < maxit; k++) {
+        sc_mv(S, p, q);
+        const double pq = dot(m, p, q);
+        if (!(pq > 0)) break;
+        const double a = rz / pq;
+        for (int i = 0; i < m; i++) { x[i] += a * p[i]; r[i] -= a * q[i]; }
+        nr = sqrt(dot(m, r, r));
+        if (nr < best) { stall = nr > sgain * best ? stall + 1 : 0; best = nr; memcpy(xb, x, sizeof(double) * m); }
+        else stall++;
+        if (nr <= tol || stall >= smax) break;
+        sc_prec(S, r, z);
+        const double rz2 = dot(m, r, z), beta = rz2 / rz;
+        rz = rz2;
+        for (int i = 0; i < m; i++) p[i] = z[i] + beta * p[i];
+    }
+    memcpy(x, xb, sizeof(double) * m);
+    sc_mv(S, x, q);
+    for (int i = 0; i < m; i++) r[i] = rhs[i] - q[i];
+    const double rel = sqrt(dot(m, r, r)) / nb;
+    free(r);
+    return rel;
+}
+
+/* ---- step lengths (solver.c:6300-6388 with the exact lambda_min) ---- */
+static double blk_maxstep(const OBlk *B, const double *V, const double *L, const double *dV, double *work) {
+    const int n = B->n;
+    if (B->lp) {
+        double a = 1e30;
+        for (int i = 0; i < n; i++) { if (!isfinite(dV[i])) return 0; if (dV[i] < 0) a = fmin(a, -V[i] / dV[i]); }
+        return a;
+    }
+    /* T = L^{-1} dV L^{-T} */
+    double *T = work;
+    memcpy(T, dV, sizeof(double) * (size_t)n * n);
+    for (int j = 0; j < n; j++) lsolve(n, L, T + (size_t)j * n);
+    for (int j = 0; j < n; j++) for (int i = j + 1; i < n; i++) { double s = T[i + (size_t)j * n]; T[i + (size_t)j * n] = T[j + (size_t)i * n]; T[j + (size_t)i * n] = s; }
+    for (int j = 0; j < n; j++) lsolve(n, L, T + (size_t)j * n);
+    symmetrize(n, T);
+    const double lam = oracle_lammin(n, T);
+    if (!isfinite(lam)) return 0;
+    return lam >= 0 ? 1e30 : -1.0 / lam;
+}
+static double maxstep(const OProb *P, const double *V, const double *L, const double *dV, double *work) {
+    double a = 1e30;
+    for (int k = 0; k < P->nb; k++) {
+        const OBlk *B = &P->B[k];
+        const double ak = blk_maxstep(B, V + B->off, L + B->off, dV + B->off, work);
+        if (ak < a) a = ak;
+    }
+    return a;
+}
+/* V += (a - stepped) dV, L = chol(V); x0.8 on failure (solver.c:6393-6446) */
+static int try_step(const OProb *P, double *V, double *L, const double *dV, double *alpha) {
+    double a = *alpha, stepped = 0;
+    for (int tries = 0; tries < 40; tries++) {
+        const double inc = a - stepped;
+        stepped = a;
+        int ok = 1;
+        for (size_t q = 0; q < P->tot; q++) V[q] += inc * dV[q];
+        for (int k = 0; k < P->nb; k++) {
+            const OBlk *B = &P->B[k];
+            if (B->lp) { for (int i = 0; i < B->n; i++) { L[B->off + i] = V[B->off + i]; if (!(V[B->off + i] > 0)) ok = 0; } continue; }
+            memcpy(L + B->off, V + B->off, sizeof(double) * (size_t)B->n * B->n);
+            if (chol(B->n, L + B->off)) ok = 0;
+        }
+        if (ok) { *alpha = a; return 1; }
+        a *= 0.8;
+    }
+    return 0;
+}
+/* G = Zi F R per block (LP: elementwise) */
+static void prod3(const OProb *P, const double *Zi, const double *F, const double *R, double *G, double *W) {
+    for (int k = 0; k < P->nb; k++) {
+        const OBlk *B = &P->B[k];
+        if (B->lp) { for (int i = 0; i < B->n; i++) G[B->off + i] = Zi[B->off + i] * F[B->off + i] * R[B->off + i]; continue; }
+        gemm_nn(B->n, Zi + B->off, F + B->off, W);
+        gemm_nn(B->n, W, R + B->off, G + B->off);
+    }
+}
+static void sym_all(const OProb *P, double *G) { for (int k = 0; k < P->nb; k++) if (!P->B[k].lp) symmetrize(P->B[k].n, G + P->B[k].off); }
+static void add_identity(const OProb *P, double *V, double s) {
+    for (int k = 0; k < P->nb; k++) { const OBlk *B = &P->B[k]; for (int i = 0; i < B->n; i++) V[B->off + (B->lp ? (size_t)i : (size_t)i * B->n + i)] += s; }
+}
+static double lammin_all(const OProb *P, const double *V, double *work) {
+    double lam = INFINITY;
+    for (int k = 0; k < P->nb; k++) {
+        const OBlk *B = &P->B[k];
+        if (B->lp) { for (int i = 0; i < B->n; i++) lam = fmin(lam, V[B->off + i]); continue; }
+        memcpy(work, V + B->off, sizeof(double) * (size_t)B->n * B->n);
+        lam = fmin(lam, oracle_lammin(B->n, work));
+    }
+    return lam;
+}
+
+/* ---- DIMACS errors on the data as read (postsolve.c:3037-3112) ---- */
+void oracle_dimacs_packed(const OProb *P, const double *Xo, const double *yo, double *err, double *pobj, double *dobj);
+static void dimacs(const OProb *P, const double *Xo, const double *yo, double *err, double *po, double *dob, double *work) {
+    const int m = P->m;
+    /* unscaled data: A_i = Ascaled_i / d_i, C = Cscaled * cs */
+    double *Zo = malloc(sizeof(double) * (P->tot + 1)), *ys = malloc(sizeof(double) * (m + 1)), *rp = calloc(m + 1, sizeof(double));
+    for (int i = 0; i < m; i++) ys[i] = yo[i] / P->d[i];          /* sum yo_i A_i = sum (yo_i / d_i) Ascaled_i */
+    ATy(P, ys, Zo);
+    for (size_t q = 0; q < P->tot; q++) Zo[q] = P->C[q] * P->cs - Zo[q];
+    Aop(P, Xo, rp, 1.0);
+    double pobj = 0, dobj = 0, xz = 0, r2 = 0, zres = 0;
+    for (size_t q = 0; q < P->tot; q++) { pobj += P->C[q] * P->cs * Xo[q]; xz += Xo[q] * Zo[q]; }
+    for (int i = 0; i < m; i++) { dobj += P->b0[i] * yo[i]; const double v = rp[i] / P->d[i] - P->b0[i]; r2 += v * v; }
+    for (int k = 0; k < P->nb; k++) {
+        const OBlk *B = &P->B[k];
+        if (B->lp) continue;
+        double f2 = 0; for (size_t q = 0; q < (size_t)B->n * B->n; q++) f2 += Zo[B->off + q] * Zo[B->off + q];
+        zres = fmax(zres, 2.220446049250313e-16 * sqrt(f2));
+    }
+    const double lamx = lammin_all(P, Xo, work), lamz = lammin_all(P, Zo, work);
+    const double den = 1 + fabs(pobj) + fabs(dobj);
+    err[0] = sqrt(r2) / (1 + P->normb1);
+    err[1] = fmax(0.0, -fmin(lamx, 0.0)) / (1 + P->normb1);
+    err[2] = 0;
+    err[3] = fmax(fmax(0.0, -fmin(lamz, 0.0)), zres > 1e4 * 2.220446049250313e-16 * (1 + P->normC1) ? zres : 0) / (1 + P->normC1);
+    err[4] = (pobj - dobj) / den;
+    err[5] = xz / den;
+    *po = pobj; *dob = dobj;
+    free(Zo); free(ys); free(rp);
+}
+
+/* ------------------------------------------------------------------------------------- */
+int oracle_solve(const OracleData *Dd, const OracleOptions *opt, OracleResult *res) {
+    memset(res, 0, sizeof(*res));
+    res->status = -1;
+    OProb P;
+    int rc = prob_build(Dd, &P);
+    if (rc) return rc;
+    const int m = P.m;
+    const size_t T = P.tot;
+    const double tol = opt && opt->tol > 0 ? opt->tol : 1e-8;
+    const int maxit = opt && opt->maxit > 0 ? opt->maxit : 100;
+    int maxn = 1;
+    for (int k = 0; k < P.nb; k++) if (P.B[k].n > maxn) maxn = P.B[k].n;
+    double *X = calloc(T + 1, sizeof(double)), *Z = calloc(T + 1, sizeof(double)), *Zi = calloc(T + 1, sizeof(double));
+    double *LX = calloc(T + 1, sizeof(double)), *LZ = calloc(T + 1, sizeof(double)), *dX = calloc(T + 1, sizeof(double));
+    double *dZ = calloc(T + 1, sizeof(double)), *F = calloc(T + 1, sizeof(double)), *G = calloc(T + 1, sizeof(double));
+    double *Q = calloc(T + 1, sizeof(double)), *R0 = calloc(T + 1, sizeof(double)), *Xb = calloc(T + 1, sizeof(double));
+    double *W = malloc(sizeof(double) * (size_t)maxn * maxn + 8), *work = malloc(sizeof(double) * (size_t)maxn * maxn + 8);
+    double *y = calloc(m + 1, sizeof(double)), *dy = calloc(m + 1, sizeof(double)), *rp = calloc(m + 1, sizeof(double));
+    double *rhs = calloc(m + 1, sizeof(double)), *a0 = calloc(m + 1, sizeof(double)), *h = calloc(m + 1, sizeof(double));
+    double *qv = calloc(m + 1, sizeof(double)), *ev = calloc(m + 1, sizeof(double)), *wv = calloc(m + 1, sizeof(double)), *yb = calloc(m + 1, sizeof(double));
+    OSchur S = { m, calloc((size_t)m * m + 1, sizeof(double)), calloc((size_t)m * m + 1, sizeof(double)), calloc(m + 1, sizeof(double)), 0, 0, 0 };
+    int status = 3, it = 0, stall = 0, it_best = 0, nfail = 0;
+    double best_score = 1e300;
+    double best_err[7] = { 0 };
+
+    /* ---- start: least-norm X0 = A'(AA')^{-1} b, least-squares y0 = (AA')^{-1} A(C) ---- */
+    memset(X, 0, sizeof(double) * T);
+    add_identity(&P, X, 1.0);
+    memcpy(Zi, X, sizeof(double) * T);
+    memset(rhs, 0, sizeof(double) * m);
+    Aop(&P, P.C, rhs, 1.0);
+    schur(&P, X, Zi, S.M);
+    if (sc_factor(&S, &nfail)) { status = 4; goto done; }
+    sc_solve(&S, P.b, dy, 1e-8);
+    sc_solve(&S, rhs, y, 1e-8);
+    ATy(&P, dy, X);
+    ATy(&P, y, F);
+    for (size_t q = 0; q < T; q++) Z[q] = P.C[q] - F[q];
+    {
+        const double ax = -lammin_all(&P, X, work), az = -lammin_all(&P, Z, work);
+        const double shx = ax >= -1e-8 ? 1.0 + ax : 0.0;
+        const double shz = fmax(0.0, 1.0 + az);
+        add_identity(&P, X, shx);
+        add_identity(&P, Z, shz);
+        for (int k = 0; k < P.nb; k++) {
+            const OBlk *B = &P.B[k];
+            const size_t len = B->lp ? (size_t)B->n : (size_t)B->n * B->n;
+            if (B->lp) { memcpy(LX + B->off, X + B->off, sizeof(double) * len); memcpy(LZ + B->off, Z + B->off, sizeof(double) * len); continue; }
+            for (double extra = 1e-8; ; extra *= 10) {
+                memcpy(LX + B->off, X + B->off, sizeof(double) * len);
+                if (chol(B->n, LX + B->off) == 0) break;
+                for (int i = 0; i < B->n; i++) X[B->off + i + (size_t)i * B->n] += extra;
+            }
+            for (double extra = 1e-8; ; extra *= 10) {
+                memcpy(LZ + B->off, Z + B->off, sizeof(double) * len);
+                if (chol(B->n, LZ + B->off) == 0) break;
+                for (int i = 0; i < B->n; i++) Z[B->off + i + (size_t)i * B->n] += extra;
+            }
+        }
+    }
+    /* ---- R0, theta ---- */
+    double theta = 1.0, r0norm = 0;
+    ATy(&P, y, F);
+    for (size_t q = 0; q < T; q++) { R0[q] = P.C[q] - Z[q] - F[q]; r0norm += R0[q] * R0[q]; }
+    r0norm = sqrt(r0norm);
+    const double nbs = sqrt(dot(m, P.b, P.b));
+    double alp = 0, ald = 0;
+    memcpy(yb, y, sizeof(double) * m); memcpy(Xb, X, sizeof(double) * T);
+
+    for (it = 0; it <= maxit; it++) {
+        /* Zi */
+        for (int k = 0; k < P.nb; k++) {
+            const OBlk *B = &P.B[k];
+            if (B->lp) { for (int i = 0; i < B->n; i++) Zi[B->off + i] = 1.0 / Z[B->off + i]; }
+            else inv_from_chol(B->n, LZ + B->off, Zi + B->off);
+        }
+        /* residuals and objectives (solver.c:14426-14482) */
+        memcpy(rp, P.b, sizeof(double) * m);
+        Aop(&P, X, rp, -1.0);
+        ATy(&P, y, F);
+        double ncr2 = 0, pobj = 0, xz = 0, rdn2 = 0, dev2 = 0;
+        for (size_t q = 0; q < T; q++) {
+            const double f = F[q] + Z[q];
+            ncr2 += f * f;
+            F[q] = f - P.C[q];
+            pobj += P.C[q] * X[q]; xz += X[q] * Z[q]; rdn2 += F[q] * F[q];
+            const double w = F[q] + theta * R0[q]; dev2 += w * w;
+        }
+        const double dobj = dot(m, P.b, y), mu = xz / P.Ntot;
+        double rpo = 0;
+        for (int i = 0; i < m; i++) { const double v = rp[i] * P.du[i]; rpo += v * v; }
+        rpo = sqrt(rpo);
+        const double rdo = P.cs * sqrt(rdn2), sc = P.bs * P.cs;
+        const double po = sc * pobj, dob = sc * dobj;
+        const double pinf = rpo / (1 + P.normb2), dinf = rdo / (1 + P.normC2);
+        const double oden = 1 + fabs(po) + fabs(dob);
+        const double relgap = fabs(po - dob) / oden, relcomp = sc * xz / oden;
+        double score = fmax(fmax(relgap, relcomp), fmax(pinf, dinf));
+        if (!isfinite(score)) score = INFINITY;
+        if (it < ORACLE_TRACE) { res->trace_pobj[it] = po; res->trace_dobj[it] = dob; res->trace_mu[it] = sc * mu; res->trace_score[it] = score; }
+        res->iters = it;
+        if (!isfinite(po) || !isfinite(dob)) { status = 4; break; }
+        if (score < best_score) {
+            best_score = score; it_best = it;
+            memcpy(yb, y, sizeof(double) * m); memcpy(Xb, X, sizeof(double) * T);
+            best_err[1] = pinf; best_err[3] = dinf;
+        }
+        if (score < tol) { status = 0; break; }
+        const double nCR = sqrt(ncr2);
+        if (dobj > 0 && nCR / dobj < 1e-8 && it > 5) { status = 1; break; }
+        double nAX = 0;
+        for (int i = 0; i < m; i++) { const double v = P.b[i] - rp[i]; nAX += v * v; }
+        nAX = sqrt(nAX);
+        if (pobj < 0 && nAX / (-pobj) < 1e-8 && it > 5) { status = 2; break; }
+        if (it == maxit) { status = 3; break; }
+        if (stall >= 3) { status = 4; break; }
+        if (it - it_best >= 5 && best_score < 1e-6) { status = 4; break; }
+        if (it - it_best >= 8 && best_score < 1e-4) { status = 4; break; }
+        if (it - it_best >= 12) { status = 4; break; }
+        /* re-base the dual residual (solver.c:14614-14636) */
+        {
+            const double dev = sqrt(dev2);
+            if (dev > 1e-10 * (1.0 + theta * r0norm) && dev > 1e-12) {
+                theta = 1.0; r0norm = sqrt(rdn2);
+                for (size_t q = 0; q < T; q++) R0[q] = -F[q];
+            }
+            if (theta * r0norm <= 1e-13) theta = 0.0;
+        }
+        /* Schur complement, factor */
+        schur(&P, X, Zi, S.M);
+        if (sc_factor(&S, &nfail)) { status = 4; break; }
+        /* a0 = A(Zi), h = A(Zi theta R0 X) */
+        memset(a0, 0, sizeof(double) * m); memset(h, 0, sizeof(double) * m);
+        Aop(&P, Zi, a0, 1.0);
+        const int hasH = theta > 0;
+        if (hasH) {
+            for (size_t q = 0; q < T; q++) dZ[q] = theta * R0[q];
+            prod3(&P, Zi, dZ, X, G, W);
+            Aop(&P, G, h, 1.0);
+        }
+        /* predictor */
+        for (int i = 0; i < m; i++) rhs[i] = P.b[i] + h[i];
+        sc_solve(&S, rhs, dy, 1e-7);
+        ATy(&P, dy, F);
+        for (size_t q = 0; q < T; q++) dZ[q] = -F[q] + theta * R0[q];
+        prod3(&P, Zi, dZ, X, G, W);
+        sym_all(&P, G);
+        for (size_t q = 0; q < T; q++) dX[q] = -G[q] - X[q];
+        double ap = fmin(1.0, maxstep(&P, X, LX, dX, work));
+        double ad = fmin(1.0, maxstep(&P, Z, LZ, dZ, work));
+        const double xdz = dot(T, X, dZ), dxz = dot(T, dX, Z), dxdz = dot(T, dX, dZ);
+        const double mu_a = (xz + ap * dxz + ad * xdz + ap * ad * dxdz) / P.Ntot;
+        const double frac = fmax(0.0, mu_a / mu), mina = fmin(ap, ad);
+        int anysdp = 0;
+        for (int k = 0; k < P.nb; k++) if (!P.B[k].lp) anysdp = 1;
+        const double expon = !anysdp ? 3.0 : (relgap > 1e-6 ? fmax(1.0, 3 * mina * mina) : 3.0);
+        const double sigma = fmin(1.0, pow(frac, expon));
+        const double gamma = 0.9 + (0.99 - 0.9) * mina;
+        const double smu = sigma * mu;
+        /* corrector (solver.c:14997-15055) */
+        prod3(&P, Zi, dZ, dX, Q, W);
+        memset(qv, 0, sizeof(double) * m);
+        Aop(&P, Q, qv, 1.0);
+        for (int i = 0; i < m; i++) rhs[i] = P.b[i] - smu * a0[i] + qv[i] + h[i];
+        sc_solve(&S, rhs, dy, 0);
+        ATy(&P, dy, F);
+        for (size_t q = 0; q < T; q++) dZ[q] = -F[q] + theta * R0[q];
+        prod3(&P, Zi, dZ, X, G, W);
+        for (int k = 0; k < P.nb; k++) {
+            const OBlk *B = &P.B[k];
+            const size_t len = B->lp ? (size_t)B->n : (size_t)B->n * B->n;
+            double *dXk = dX + B->off;
+            const double *Gk = G + B->off, *Qk = Q + B->off, *Zik = Zi + B->off, *Xk = X + B->off;
+            if (B->lp) { for (int i = 0; i < B->n; i++) dXk[i] = smu * Zik[i] - Zik[i] * dZ[B->off + i] * Xk[i] - Qk[i] - Xk[i]; continue; }
+            for (size_t q = 0; q < len; q++) dXk[q] = -Gk[q] - Qk[q];
+            symmetrize(B->n, dXk);
+            for (size_t q = 0; q < len; q++) dXk[q] += smu * Zik[q] - Xk[q];
+        }
+        /* refinement enforcing A(dX) = rp (solver.c:15058-15105) */
+        for (int pass = 0; pass < 3; pass++) {
+            memcpy(ev, rp, sizeof(double) * m);
+            Aop(&P, dX, ev, -1.0);
+            const double ne = sqrt(dot(m, ev, ev)), nrp = sqrt(dot(m, rp, rp));
+            if (!(ne > 1e-2 * nrp && ne > 1e-3 * tol * (1 + nbs))) break;
+            sc_solve(&S, ev, wv, 0);
+            memcpy(qv, ev, sizeof(double) * m);
+            ATy(&P, wv, Q);
+            prod3(&P, Zi, Q, X, G, W);
+            sym_all(&P, G);
+            Aop(&P, G, qv, -1.0);
+            const double ne2 = sqrt(dot(m, qv, qv));
+            if (!(ne2 < 0.5 * ne)) break;
+            for (int i = 0; i < m; i++) dy[i] += wv[i];
+            for (size_t q = 0; q < T; q++) { dZ[q] -= Q[q]; dX[q] += G[q]; }
+        }
+        const double apr = fmin(1.0, maxstep(&P, X, LX, dX, work));
+        const double adr = fmin(1.0, maxstep(&P, Z, LZ, dZ, work));
+        ap = fmin(1.0, gamma * apr);
+        ad = fmin(1.0, gamma * adr);
+        if (pinf < 1e-6) {   /* catastrophe guard (solver.c:15143-15160) */
+            memcpy(ev, rp, sizeof(double) * m);
+            Aop(&P, dX, ev, -1.0);
+            const double nrp = sqrt(dot(m, rp, rp)), nev = sqrt(dot(m, ev, ev));
+            const double g = fmax(1e3, 1e-6 / fmax(pinf, 1e-300)), lim = g * nrp;
+            if (nev > lim && nrp > 0) { const double cap = lim / nev; if (cap < ap) ap = cap; }
+        }
+        if (!try_step(&P, X, LX, dX, &ap) || !try_step(&P, Z, LZ, dZ, &ad)) { status = 4; break; }
+        for (int i = 0; i < m; i++) y[i] += ad * dy[i];
+        theta *= (1.0 - ad);
+        alp = ap; ald = ad;
+        res->trace_ap[it] = ap; res->trace_ad[it] = ad;
+        stall = (ap < 1e-6 && ad < 1e-6) ? stall + 1 : 0;
+    }
+    (void)alp; (void)ald; (void)best_err;
+    if (status != 0) { memcpy(y, yb, sizeof(double) * m); memcpy(X, Xb, sizeof(double) * T); }
+done:
+    /* unscale: X = bs X~, y_i = cs d_i y~_i (Z follows from y on the data as read) */
+    res->m = m; res->nblk = P.nb;
+    res->y = malloc(sizeof(double) * (m + 1));
+    res->X = malloc(sizeof(double) * (T + 1));
+    for (int i = 0; i < m; i++) res->y[i] = P.cs * P.d[i] * y[i];
+    for (size_t q = 0; q < T; q++) res->X[q] = P.bs * X[q];
+    res->xlen = T;
+    {
+        double pobj, dobj;
+        dimacs(&P, res->X, res->y, res->err, &pobj, &dobj, work);
+        res->pobj = pobj; res->dobj = dobj;
+        double acc = fmax(fmax(res->err[0], res->err[1]), fmax(res->err[3], fmax(fabs(res->err[4]), fabs(res->err[5]))));
+        if (!isfinite(acc)) acc = INFINITY;
+        if (status == 0 || status == 3 || status == 4 || status == 5) {
+            if (acc <= tol) status = 0;
+            else if (acc <= 1e-6) status = 5;
+            else if (status == 0) status = 4;
+        }
+        res->acc = acc;
+    }
+    res->status = status;
+    free(X); free(Z); free(Zi); free(LX); free(LZ); free(dX); free(dZ); free(F); free(G); free(Q); free(R0); free(Xb);
+    free(W); free(work); free(y); free(dy); free(rp); free(rhs); free(a0); free(h); free(qv); free(ev); free(wv); free(yb);
+    free(S.M); free(S.L); free(S.D);
+    prob_free(&P);
+    return 0;
+}
+
+void oracle_result_free(OracleResult *r) { if (r) { free(r->y); free(r->X); r->y = NULL; r->X = NULL; } }
+
+/* DIMACS errors of a given (X, y) on the data as read - the same measurement the solve ends
+ * with, exposed so a test can measure the GPU path's returned point independently */
+int oracle_dimacs(const OracleData *Dd, const double *X, const double *y, double *err, double *pobj, double *dobj) {
+    OProb P;
+    int rc = prob_build(Dd, &P);
+    if (rc) return rc;
+    int maxn = 1;
+    for (int k = 0; k < P.nb; k++) if (P.B[k].n > maxn) maxn = P.B[k].n;
+    double *work = malloc(sizeof(double) * (size_t)maxn * maxn + 8);
+    dimacs(&P, X, y, err, pobj, dobj, work);
+    free(work);
+    prob_free(&P);
+    return 0;
+}
+
+/* the route-0 Schur complement of the SCALED problem at given packed X, Zi (tests of the
+ * assembly kernel): M_ab = <A~_a, X A~_b Zi> */
+int oracle_schur(const OracleData *Dd, const double *X, const double *Zi, double *M) {
+    OProb P;
+    int rc = prob_build(Dd, &P);
+    if (rc) return rc;
+    schur(&P, X, Zi, M);
+    prob_free(&P);
+    return 0;
+}

@@ -1,0 +1,147 @@
+"""ctypes plumbing shared by the tests: the instance generators (product side), the oracle
+(test infrastructure) and, where it was built in this container, the reference library."""
+import ctypes, os
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+I, D = ctypes.c_int, ctypes.c_double
+PI, PD = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
+
+
+class Inst(ctypes.Structure):
+    _fields_ = [("m", I), ("nblk", I), ("bs", PI), ("c", PD), ("nnz", ctypes.c_int64),
+                ("mat", PI), ("blk", PI), ("i", PI), ("j", PI), ("v", PD)]
+
+
+class OracleData(ctypes.Structure):
+    _fields_ = [("m", I), ("nblk", I), ("bs", PI), ("c", PD), ("nnz", ctypes.c_int64),
+                ("mat", PI), ("blk", PI), ("i", PI), ("j", PI), ("v", PD)]
+
+
+TR = 128
+
+
+class OracleOptions(ctypes.Structure):
+    _fields_ = [("tol", D), ("maxit", I)]
+
+
+class OracleResult(ctypes.Structure):
+    _fields_ = [("status", I), ("iters", I), ("m", I), ("nblk", I), ("pobj", D), ("dobj", D), ("acc", D),
+                ("err", D * 6), ("y", PD), ("X", PD), ("xlen", ctypes.c_size_t),
+                ("trace_pobj", D * TR), ("trace_dobj", D * TR), ("trace_mu", D * TR), ("trace_score", D * TR),
+                ("trace_ap", D * TR), ("trace_ad", D * TR)]
+
+
+_gen = None
+_orc = None
+
+
+def gen_lib():
+    global _gen
+    if _gen is None:
+        _gen = ctypes.CDLL(os.path.join(ROOT, "brisk_amd", "libbrisk_gen.so"))
+        pI = ctypes.POINTER(Inst)
+        u64 = ctypes.c_uint64
+        _gen.brisk_gen_theta.argtypes = [I, D, u64, pI]
+        _gen.brisk_gen_theta_edges.argtypes = [I, ctypes.c_int64, PI, pI]
+        _gen.brisk_gen_maxcut.argtypes = [I, D, u64, pI]
+        _gen.brisk_gen_maxcut_cycle.argtypes = [I, pI]
+        _gen.brisk_gen_dense.argtypes = [I, I, u64, pI]
+        _gen.brisk_gen_multi.argtypes = [I, I, I, I, u64, pI]
+        _gen.brisk_gen_free.argtypes = [pI]
+        _gen.brisk_gen_u01.argtypes = [u64, u64, u64]
+        _gen.brisk_gen_u01.restype = D
+    return _gen
+
+
+class Instance:
+    """A generated instance; frees its arrays when collected."""
+
+    def __init__(self, kind, *args):
+        self.s = Inst()
+        g = gen_lib()
+        if kind == "theta_edges":
+            n, edges = args
+            e = np.ascontiguousarray(np.asarray(edges, dtype=np.int32).reshape(-1))
+            rc = g.brisk_gen_theta_edges(n, len(e) // 2, e.ctypes.data_as(PI), ctypes.byref(self.s))
+        else:
+            rc = getattr(g, "brisk_gen_" + kind)(*args, ctypes.byref(self.s))
+        if rc != 0:
+            raise ValueError("generator %s%r failed" % (kind, args))
+        self.kind = kind
+        self.args = args
+
+    def __del__(self):
+        try:
+            gen_lib().brisk_gen_free(ctypes.byref(self.s))
+        except Exception:
+            pass
+
+    @property
+    def m(self):
+        return self.s.m
+
+    @property
+    def bs(self):
+        return [self.s.bs[k] for k in range(self.s.nblk)]
+
+    def arrays(self):
+        s = self.s
+        nnz = s.nnz
+        return dict(m=s.m, nblk=s.nblk, bs=np.ctypeslib.as_array(s.bs, (s.nblk,)).copy(),
+                    c=np.ctypeslib.as_array(s.c, (s.m,)).copy(),
+                    mat=np.ctypeslib.as_array(s.mat, (nnz,)).copy() if nnz else np.zeros(0, np.int32),
+                    blk=np.ctypeslib.as_array(s.blk, (nnz,)).copy() if nnz else np.zeros(0, np.int32),
+                    i=np.ctypeslib.as_array(s.i, (nnz,)).copy() if nnz else np.zeros(0, np.int32),
+                    j=np.ctypeslib.as_array(s.j, (nnz,)).copy() if nnz else np.zeros(0, np.int32),
+                    v=np.ctypeslib.as_array(s.v, (nnz,)).copy() if nnz else np.zeros(0))
+
+    def data_args(self):
+        s = self.s
+        return (s.m, s.nblk, s.bs, s.c, s.nnz, s.mat, s.blk, s.i, s.j, s.v)
+
+
+def oracle_lib():
+    global _orc
+    if _orc is None:
+        _orc = ctypes.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+        pO = ctypes.POINTER(OracleData)
+        _orc.oracle_solve.argtypes = [pO, ctypes.POINTER(OracleOptions), ctypes.POINTER(OracleResult)]
+        _orc.oracle_result_free.argtypes = [ctypes.POINTER(OracleResult)]
+        _orc.oracle_dimacs.argtypes = [pO, PD, PD, PD, PD, PD]
+        _orc.oracle_schur.argtypes = [pO, PD, PD, PD]
+        _orc.oracle_lammin.argtypes = [I, PD]
+        _orc.oracle_lammin.restype = D
+    return _orc
+
+
+def oracle_data(inst):
+    s = inst.s
+    return OracleData(s.m, s.nblk, s.bs, s.c, s.nnz, s.mat, s.blk, s.i, s.j, s.v)
+
+
+def oracle_solve(inst, tol=1e-8, maxit=100):
+    lib = oracle_lib()
+    d = oracle_data(inst)
+    r = OracleResult()
+    rc = lib.oracle_solve(ctypes.byref(d), ctypes.byref(OracleOptions(tol, maxit)), ctypes.byref(r))
+    if rc != 0:
+        raise ValueError("oracle_solve: bad data (%d)" % rc)
+    out = dict(status=r.status, iters=r.iters, pobj=r.pobj, dobj=r.dobj, err=list(r.err), acc=r.acc,
+               y=np.ctypeslib.as_array(r.y, (r.m,)).copy(), X=np.ctypeslib.as_array(r.X, (r.xlen,)).copy(),
+               trace_pobj=list(r.trace_pobj[:r.iters + 1]), trace_dobj=list(r.trace_dobj[:r.iters + 1]),
+               trace_mu=list(r.trace_mu[:r.iters + 1]))
+    lib.oracle_result_free(ctypes.byref(r))
+    return out
+
+
+def oracle_dimacs(inst, X, y):
+    lib = oracle_lib()
+    d = oracle_data(inst)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    err = np.zeros(6)
+    po, do = D(), D()
+    lib.oracle_dimacs(ctypes.byref(d), X.ctypes.data_as(PD), y.ctypes.data_as(PD), err.ctypes.data_as(PD),
+                      ctypes.byref(po), ctypes.byref(do))
+    return err, po.value, do.value
