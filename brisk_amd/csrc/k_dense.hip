@@ -1,0 +1,434 @@
+// k_dense.hip - dense batched linear algebra on CDNA4 matrix cores (gfx950):
+//   * batched GEMM on MFMA (f64: v_mfma_f64_16x16x4f64, f32: v_mfma_f32_16x16x4f32), LDS-staged
+//     64x64x16 tiles, 4 waves per workgroup each owning a 32x32 quadrant, triangular K-range
+//     trimming and a lower-only epilogue for SYRK-shaped updates;
+//   * blocked right-looking Cholesky of a set of matrices (64-wide panels: an LDS kernel factors
+//     and inverts the diagonal block, MFMA GEMMs do the panel and the trailing update);
+//   * triangular inverse by recursive doubling (log2(n/64) levels of batched GEMMs);
+//   * batched matrix-vector products (one wave per output element for the transposed form).
+// The dense Cholesky replaces the reference's task-DAG potrf (sparsechol.c:3759-3859) and its
+// triangular solves (sparsechol.c:4058-4093); M's solves become products with the explicit
+// inverse factor, which are HBM-bound streams instead of latency-bound substitutions.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <vector>
+#include "dev.h"
+#include "rt_internal.h"
+
+#define GBM 64
+#define GBN 64
+#define GBK 16
+
+template <typename T> struct MF;
+// out(g, r): the row of the 16x16 result held by register r of a lane in lane group g = lane/16
+// (the column is lane % 16). The f64 instruction interleaves the rows of the four lane groups.
+template <> struct MF<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int out(int g, int r) { return g + 4 * r; }
+};
+template <> struct MF<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int out(int g, int r) { return 4 * g + r; }
+};
+
+// C = alpha op(A) op(B) + beta C for one 64x64 tile of one item. The MFMA is fed with the B
+// tile as its A operand and the A tile as its B operand, so each output register row is a run
+// of 16 consecutive rows of column-major C (coalesced stores).
+template <typename T, int TA, int TB>
+__global__ __launch_bounds__(256) void k_gemm(const GemmItem *__restrict__ items, int nitems, const T *__restrict__ A,
+                                              const T *__restrict__ B, T *__restrict__ C, T alpha, T beta,
+                                              const int *__restrict__ info) {
+    typedef typename MF<T>::v4 v4;
+    const int item = blockIdx.y + blockIdx.z * 65535;
+    if (item >= nitems) return;
+    const GemmItem it = items[item];
+    if (it.guard >= 0 && info && info[it.guard] != 0) return;
+    const int tilesM = (it.M + GBM - 1) / GBM, tilesN = (it.N + GBN - 1) / GBN;
+    const int t = blockIdx.x;
+    if (t >= tilesM * tilesN) return;
+    const int tm = t % tilesM, tn = t / tilesM;
+    const int r0 = tm * GBM, c0 = tn * GBN;
+    if ((it.tri & 1) && c0 > r0 + GBM - 1) return;
+    int kb = 0, ke = it.K;
+    if (it.tri & 2) kb = max(kb, r0);            // op(A) upper triangular: A(i,k) = 0 for k < i
+    if (it.tri & 4) ke = min(ke, r0 + GBM);      // op(A) lower triangular: A(i,k) = 0 for k > i
+    if (it.tri & 8) kb = max(kb, c0);            // op(B) lower triangular: B(k,j) = 0 for k < j
+    if (it.tri & 16) ke = min(ke, c0 + GBN);     // op(B) upper triangular: B(k,j) = 0 for k > j
+    kb = (kb / GBK) * GBK;
+    __shared__ T As[GBK][GBM + 4];
+    __shared__ T Bs[GBK][GBN + 4];
+    const T *a = A + it.a, *b = B + it.b;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave & 1, wn = wave >> 1;
+    const int M = it.M, N = it.N, lda = it.lda, ldb = it.ldb;
+    v4 acc[2][2];
+    for (int i = 0; i < 2; i++) for (int j = 0; j < 2; j++) for (int r = 0; r < 4; r++) acc[i][j][r] = 0;
+    for (int k0 = kb; k0 < ke; k0 += GBK) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            if (TA == 0) {
+                const int i = tid & 63, k = (tid >> 6) + 4 * r, gi = r0 + i, gk = k0 + k;
+                As[k][i] = (gi < M && gk < ke) ? a[gi + (int64_t)gk * lda] : (T)0;
+            } else {
+                const int k = tid & 15, i = (tid >> 4) + 16 * r, gi = r0 + i, gk = k0 + k;
+                As[k][i] = (gi < M && gk < ke) ? a[gk + (int64_t)gi * lda] : (T)0;
+            }
+            if (TB == 0) {
+                const int k = tid & 15, j = (tid >> 4) + 16 * r, gj = c0 + j, gk = k0 + k;
+                Bs[k][j] = (gj < N && gk < ke) ? b[gk + (int64_t)gj * ldb] : (T)0;
+            } else {
+                const int j = tid & 63, k = (tid >> 6) + 4 * r, gj = c0 + j, gk = k0 + k;
+                Bs[k][j] = (gj < N && gk < ke) ? b[gj + (int64_t)gk * ldb] : (T)0;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GBK; kk += 4) {
+            T av[2], bv[2];
+            const int kr = kk + (lane >> 4), l16 = lane & 15;
+#pragma unroll
+            for (int s = 0; s < 2; s++) { av[s] = As[kr][wm * 32 + s * 16 + l16]; bv[s] = Bs[kr][wn * 32 + s * 16 + l16]; }
+#pragma unroll
+            for (int si = 0; si < 2; si++)
+#pragma unroll
+                for (int sj = 0; sj < 2; sj++) acc[si][sj] = MF<T>::mma(bv[sj], av[si], acc[si][sj]);
+        }
+        __syncthreads();
+    }
+    T *c = C + it.c;
+    const int ldc = it.ldc;
+#pragma unroll
+    for (int si = 0; si < 2; si++)
+#pragma unroll
+        for (int sj = 0; sj < 2; sj++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int gi = r0 + wm * 32 + si * 16 + (lane & 15);
+                const int gj = c0 + wn * 32 + sj * 16 + MF<T>::out(lane >> 4, r);
+                if (gi < M && gj < N && (!(it.tri & 1) || gi >= gj)) {
+                    T *cp = c + gi + (int64_t)gj * ldc;
+                    T v = alpha * acc[si][sj][r];
+                    if (beta != (T)0) v += beta * *cp;
+                    *cp = v;
+                }
+            }
+}
+
+template <typename T>
+static int gemm_launch(int ta, int tb, T alpha, T beta, const T *A, const T *B, T *C, const GemmItem *items, int nitems, const int *info) {
+    if (nitems <= 0) return 0;
+    int maxt = 0;
+    for (int q = 0; q < nitems; q++) {
+        const int t = ((items[q].M + GBM - 1) / GBM) * ((items[q].N + GBN - 1) / GBN);
+        if (items[q].M > 0 && items[q].N > 0 && t > maxt) maxt = t;
+    }
+    if (maxt == 0) return 0;
+    const GemmItem *d = (const GemmItem *)bk_stage(items, sizeof(GemmItem) * nitems);
+    if (!d) return -1;
+    dim3 grid(maxt, nitems < 65535 ? nitems : 65535, (nitems + 65534) / 65535);
+    if (!ta && !tb) hipLaunchKernelGGL((k_gemm<T, 0, 0>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info);
+    else if (!ta && tb) hipLaunchKernelGGL((k_gemm<T, 0, 1>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info);
+    else if (ta && !tb) hipLaunchKernelGGL((k_gemm<T, 1, 0>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info);
+    else hipLaunchKernelGGL((k_gemm<T, 1, 1>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info);
+    return BK_LAUNCH_CHECK("k_gemm");
+}
+extern "C" int bk_gemm_f64(int ta, int tb, double alpha, double beta, const double *A, const double *B, double *C,
+                           const GemmItem *items, int nitems, const int *info) {
+    return gemm_launch<double>(ta, tb, alpha, beta, A, B, C, items, nitems, info);
+}
+extern "C" int bk_gemm_f32(int ta, int tb, float alpha, float beta, const float *A, const float *B, float *C,
+                           const GemmItem *items, int nitems, const int *info) {
+    return gemm_launch<float>(ta, tb, alpha, beta, A, B, C, items, nitems, info);
+}
+
+// ---- Cholesky: the diagonal block ------------------------------------------------------------
+// One workgroup per matrix: factors the nb x nb diagonal block at (k0, k0) in LDS (right-looking,
+// unblocked), records the first failing column and the smallest squared pivot, writes L_kk back
+// (strict upper part of the block zeroed) and its inverse to Dinv.
+template <typename T>
+__global__ __launch_bounds__(256) void k_potrf_diag(const MItem *__restrict__ items, int nit, int k0, T *__restrict__ A,
+                                                    T *__restrict__ Dinv, int *__restrict__ info, T *__restrict__ pmin) {
+    const int q = blockIdx.x;
+    if (q >= nit) return;
+    const MItem it = items[q];
+    const int n = it.n;
+    if (k0 >= n || info[it.slot] != 0) return;
+    const int nb = min(64, n - k0);
+    __shared__ T a[64][65];      // a[col][row]
+    __shared__ T li[64][65];
+    __shared__ int fail;
+    __shared__ T piv;
+    T *Ab = A + it.off + k0 + (int64_t)k0 * n;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int r = e & 63, c = e >> 6;
+        a[c][r] = (r < nb && c < nb && r >= c) ? Ab[r + (int64_t)c * n] : (T)0;
+    }
+    if (tid == 0) fail = 0;
+    T pm = (T)1e30;
+    __syncthreads();
+    for (int j = 0; j < nb; j++) {
+        if (tid == 0) {
+            const T d = a[j][j];
+            if (!(d > (T)0)) fail = j + 1;
+            else { piv = sqrt(d); a[j][j] = piv; if (d < pm) pm = d; }
+        }
+        __syncthreads();
+        if (fail) break;
+        const T p = piv;
+        if (tid > j && tid < nb) a[j][tid] /= p;
+        __syncthreads();
+        const int rem = nb - j - 1;
+        for (int e = tid; e < rem * rem; e += 256) {
+            const int i = j + 1 + e % rem, l = j + 1 + e / rem;
+            if (i >= l) a[l][i] -= a[j][i] * a[j][l];
+        }
+        __syncthreads();
+    }
+    if (fail) {
+        if (tid == 0) info[it.slot] = k0 + fail;
+        return;
+    }
+    if (tid < nb) {        // inverse of the block: column c by forward substitution
+        const int c = tid;
+        for (int i = 0; i < 64; i++) li[c][i] = 0;
+        li[c][c] = (T)1 / a[c][c];
+        for (int i = c + 1; i < nb; i++) {
+            T s = 0;
+            for (int k = c; k < i; k++) s += a[k][i] * li[c][k];
+            li[c][i] = -s / a[i][i];
+        }
+    }
+    __syncthreads();
+    T *Di = Dinv + it.off2 + (int64_t)(k0 / 64) * 64 * 64;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int r = e & 63, c = e >> 6;
+        if (r < nb && c < nb) Ab[r + (int64_t)c * n] = r >= c ? a[c][r] : (T)0;
+        Di[r + 64 * c] = (r < nb && c < nb) ? li[c][r] : (T)0;
+    }
+    if (tid == 0) pmin[it.slot] = (k0 == 0) ? pm : min(pmin[it.slot], pm);
+}
+
+// the strict upper triangle of every matrix set to zero, and the panel (rows k0+64.., cols
+// k0..k0+63) copied from the scratch W back into A
+template <typename T>
+__global__ void k_copy_panel(const MItem *__restrict__ items, int nit, int k0, const T *__restrict__ W, T *__restrict__ A,
+                             const int *__restrict__ info) {
+    const int q = blockIdx.y;
+    if (q >= nit) return;
+    const MItem it = items[q];
+    const int n = it.n;
+    if (k0 + 64 >= n || info[it.slot] != 0) return;
+    const int rows = n - k0 - 64;
+    const int64_t tot = (int64_t)rows * 64;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e % rows), c = (int)(e / rows);
+        A[it.off + (k0 + 64 + r) + (int64_t)(k0 + c) * n] = W[it.off3 + r + (int64_t)c * n];
+    }
+}
+template <typename T>
+__global__ void k_zero_upper(const MItem *__restrict__ items, int nit, T *__restrict__ A) {
+    const int q = blockIdx.y;
+    if (q >= nit) return;
+    const MItem it = items[q];
+    const int n = it.n;
+    const int64_t tot = (int64_t)n * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e % n), c = (int)(e / n);
+        if (r < c) A[it.off + e] = (T)0;
+    }
+}
+
+template <typename T>
+using GemmFn = int (*)(int, int, T, T, const T *, const T *, T *, const GemmItem *, int, const int *);
+
+template <typename T>
+static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, T *pmin, GemmFn<T> gemm) {
+    if (n_it <= 0) return 0;
+    int nmax = 0;
+    for (int q = 0; q < n_it; q++) nmax = it[q].n > nmax ? it[q].n : nmax;
+    const MItem *d_it = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
+    if (!d_it) return -1;
+    {
+        hipLaunchKernelGGL((k_zero_upper<T>), dim3(bk_grid1((int64_t)nmax * nmax, 256, 1024), n_it), dim3(256), 0, g_bk_stream, d_it, n_it, A);
+        if (BK_LAUNCH_CHECK("k_zero_upper")) return -1;
+    }
+    std::vector<GemmItem> gi, tr;
+    for (int k0 = 0; k0 < nmax; k0 += 64) {
+        hipLaunchKernelGGL((k_potrf_diag<T>), dim3(n_it), dim3(256), 0, g_bk_stream, d_it, n_it, k0, A, Dinv, info, pmin);
+        if (BK_LAUNCH_CHECK("k_potrf_diag")) return -1;
+        if (k0 + 64 >= nmax) break;
+        gi.clear(); tr.clear();
+        for (int q = 0; q < n_it; q++) {
+            const int n = it[q].n;
+            if (k0 + 64 >= n) continue;
+            GemmItem g;
+            // panel: W = A[k0+64:, k0:k0+64] Dinv_k'   (Dinv_k' upper triangular)
+            g.M = n - k0 - 64; g.N = 64; g.K = 64;
+            g.lda = n; g.ldb = 64; g.ldc = n;
+            g.a = it[q].off + (k0 + 64) + (int64_t)k0 * n;
+            g.b = it[q].off2 + (int64_t)(k0 / 64) * 64 * 64;
+            g.c = it[q].off3;
+            g.guard = it[q].slot; g.tri = 16;
+            gi.push_back(g);
+            // trailing: A22 -= W W' (lower triangle)
+            g.M = n - k0 - 64; g.N = n - k0 - 64; g.K = 64;
+            g.lda = n; g.ldb = n; g.ldc = n;
+            g.a = it[q].off3; g.b = it[q].off3;
+            g.c = it[q].off + (k0 + 64) + (int64_t)(k0 + 64) * n;
+            g.guard = it[q].slot; g.tri = 1;
+            tr.push_back(g);
+        }
+        if (gi.empty()) continue;
+        if (gemm(0, 1, (T)1, (T)0, A, Dinv, W, gi.data(), (int)gi.size(), info)) return -1;
+        if (gemm(0, 1, (T)-1, (T)1, W, W, A, tr.data(), (int)tr.size(), info)) return -1;
+        hipLaunchKernelGGL((k_copy_panel<T>), dim3(64, n_it), dim3(256), 0, g_bk_stream, d_it, n_it, k0, W, A, info);
+        if (BK_LAUNCH_CHECK("k_copy_panel")) return -1;
+    }
+    return 0;
+}
+extern "C" int bk_potrf_f64(const MItem *it, int n_it, double *A, double *Dinv, double *W, int *info, double *pmin) {
+    return potrf_run<double>(it, n_it, A, Dinv, W, info, pmin, bk_gemm_f64);
+}
+extern "C" int bk_potrf_f32(const MItem *it, int n_it, float *A, float *Dinv, float *W, int *info, float *pmin) {
+    return potrf_run<float>(it, n_it, A, Dinv, W, info, pmin, bk_gemm_f32);
+}
+
+// ---- triangular inverse by recursive doubling ----------------------------------------------
+// Linv starts as the block diagonal of the 64 x 64 inverses; at level s the lower-left block of
+// every 2s x 2s diagonal block is -C^{-1} B A^{-1} (A, C the inverted s x s diagonal blocks, B the
+// block of L between them): two batched GEMMs per level, T = B A^{-1} into scratch, then the
+// product with C^{-1}.
+template <typename T>
+__global__ void k_trtri_init(const MItem *__restrict__ items, int nit, const T *__restrict__ Dinv, T *__restrict__ Linv,
+                             const int *__restrict__ info) {
+    const int q = blockIdx.y;
+    if (q >= nit) return;
+    const MItem it = items[q];
+    if (info && info[it.slot] != 0) return;
+    const int n = it.n;
+    const int64_t tot = (int64_t)n * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e % n), c = (int)(e / n);
+        T v = 0;
+        if ((r >> 6) == (c >> 6) && r >= c) v = Dinv[it.off2 + (int64_t)(c >> 6) * 4096 + (r & 63) + 64 * (c & 63)];
+        Linv[it.off + e] = v;
+    }
+}
+template <typename T>
+static int trtri_run(const MItem *it, int n_it, const T *L, const T *Dinv, T *Linv, T *W, const int *info, GemmFn<T> gemm) {
+    if (n_it <= 0) return 0;
+    int nmax = 0;
+    for (int q = 0; q < n_it; q++) nmax = it[q].n > nmax ? it[q].n : nmax;
+    const MItem *d_it = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
+    if (!d_it) return -1;
+    hipLaunchKernelGGL((k_trtri_init<T>), dim3(bk_grid1((int64_t)nmax * nmax, 256, 1024), n_it), dim3(256), 0, g_bk_stream, d_it, n_it, Dinv, Linv, info);
+    if (BK_LAUNCH_CHECK("k_trtri_init")) return -1;
+    std::vector<GemmItem> g1, g2;
+    for (int s = 64; s < nmax; s *= 2) {
+        g1.clear(); g2.clear();
+        for (int q = 0; q < n_it; q++) {
+            const int n = it[q].n;
+            for (int p0 = 0; p0 + s < n; p0 += 2 * s) {
+                const int s2 = min(s, n - p0 - s);          // order of the second diagonal block
+                GemmItem g;
+                // T = B * A^{-1}: B = L[p0+s : p0+s+s2, p0 : p0+s], A^{-1} = Linv[p0:p0+s, p0:p0+s] (lower)
+                g.M = s2; g.N = s; g.K = s;
+                g.lda = n; g.ldb = n; g.ldc = n;
+                g.a = it[q].off + (p0 + s) + (int64_t)p0 * n;
+                g.b = it[q].off + p0 + (int64_t)p0 * n;
+                g.c = it[q].off3 + (p0 + s) + (int64_t)p0 * n;
+                g.guard = it[q].slot; g.tri = 8;
+                g1.push_back(g);
+                // Linv[p0+s.., p0..] = -C^{-1} T, C^{-1} = Linv[p0+s : p0+s+s2, same] (lower)
+                g.M = s2; g.N = s; g.K = s2;
+                g.lda = n; g.ldb = n; g.ldc = n;
+                g.a = it[q].off + (p0 + s) + (int64_t)(p0 + s) * n;
+                g.b = it[q].off3 + (p0 + s) + (int64_t)p0 * n;
+                g.c = it[q].off + (p0 + s) + (int64_t)p0 * n;
+                g.guard = it[q].slot; g.tri = 4;
+                g2.push_back(g);
+            }
+        }
+        if (g1.empty()) continue;
+        // first GEMM: A-operand from L, B-operand from Linv; second: both from Linv / W
+        if (gemm(0, 0, (T)1, (T)0, L, Linv, W, g1.data(), (int)g1.size(), info)) return -1;
+        if (gemm(0, 0, (T)-1, (T)0, Linv, W, Linv, g2.data(), (int)g2.size(), info)) return -1;
+    }
+    return 0;
+}
+extern "C" int bk_trtri_f64(const MItem *it, int n_it, const double *L, const double *Dinv, double *Linv, double *W, const int *info) {
+    return trtri_run<double>(it, n_it, L, Dinv, Linv, W, info, bk_gemm_f64);
+}
+extern "C" int bk_trtri_f32(const MItem *it, int n_it, const float *L, const float *Dinv, float *Linv, float *W, const int *info) {
+    return trtri_run<float>(it, n_it, L, Dinv, Linv, W, info, bk_gemm_f32);
+}
+
+// ---- batched matrix-vector products -------------------------------------------------------
+// trans = 1: y_j = alpha sum_i A[i, j] x_i + beta y_j, one wave per output (column of A read
+// contiguously), fixed-order lane partials + butterfly: deterministic.
+// trans = 0: y_i = alpha sum_j A[i, j] x_j + beta y_i, one thread per row with the columns in
+// order (rows read coalesced across the wave).
+template <typename TA, typename TX, typename TY, typename TACC>
+__global__ __launch_bounds__(256) void k_gemv_t(const GemvItem *__restrict__ items, int nit, const TA *__restrict__ A,
+                                                const TX *__restrict__ x, TY *__restrict__ y, TACC alpha, TACC beta) {
+    const int q = blockIdx.y + blockIdx.z * 65535;
+    if (q >= nit) return;
+    const GemvItem it = items[q];
+    const int lane = threadIdx.x & 63;
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= it.N) return;
+    const TA *col = A + it.a + (int64_t)j * it.lda;
+    const TX *xv = x + it.x;
+    TACC s = 0;
+    for (int i = lane; i < it.M; i += 64) s += (TACC)col[i] * (TACC)xv[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) {
+        TY *yp = y + it.y + j;
+        TACC v = alpha * s;
+        if (beta != (TACC)0) v += beta * (TACC)*yp;
+        *yp = (TY)v;
+    }
+}
+template <typename TA, typename TX, typename TY, typename TACC>
+__global__ __launch_bounds__(256) void k_gemv_n(const GemvItem *__restrict__ items, int nit, const TA *__restrict__ A,
+                                                const TX *__restrict__ x, TY *__restrict__ y, TACC alpha, TACC beta) {
+    const int q = blockIdx.y + blockIdx.z * 65535;
+    if (q >= nit) return;
+    const GemvItem it = items[q];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= it.M) return;
+    const TA *row = A + it.a + i;
+    const TX *xv = x + it.x;
+    TACC s = 0;
+    for (int j = 0; j < it.N; j++) s += (TACC)row[(int64_t)j * it.lda] * (TACC)xv[j];
+    TY *yp = y + it.y + i;
+    TACC v = alpha * s;
+    if (beta != (TACC)0) v += beta * (TACC)*yp;
+    *yp = (TY)v;
+}
+template <typename TA, typename TX, typename TY, typename TACC>
+static int gemv_run(int trans, TACC alpha, TACC beta, const TA *A, const TX *x, TY *y, const GemvItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    int mx = 0;
+    for (int q = 0; q < n_it; q++) { const int v = trans ? (it[q].N + 3) / 4 : (it[q].M + 255) / 256; mx = v > mx ? v : mx; }
+    if (mx == 0) return 0;
+    const GemvItem *d = (const GemvItem *)bk_stage(it, sizeof(GemvItem) * n_it);
+    if (!d) return -1;
+    dim3 grid(mx, n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    if (trans) hipLaunchKernelGGL((k_gemv_t<TA, TX, TY, TACC>), grid, dim3(256), 0, g_bk_stream, d, n_it, A, x, y, alpha, beta);
+    else hipLaunchKernelGGL((k_gemv_n<TA, TX, TY, TACC>), grid, dim3(256), 0, g_bk_stream, d, n_it, A, x, y, alpha, beta);
+    return BK_LAUNCH_CHECK("k_gemv");
+}
+extern "C" int bk_gemv_f64(int trans, double alpha, double beta, const double *A, const double *x, double *y, const GemvItem *it, int n_it) {
+    return gemv_run<double, double, double, double>(trans, alpha, beta, A, x, y, it, n_it);
+}
+extern "C" int bk_gemv_f32(int trans, float alpha, float beta, const float *A, const float *x, float *y, const GemvItem *it, int n_it) {
+    return gemv_run<float, float, float, float>(trans, alpha, beta, A, x, y, it, n_it);
+}
+extern "C" int bk_gemv_mixed(int trans, double alpha, double beta, const float *A, const double *x, double *y, const GemvItem *it, int n_it) {
+    return gemv_run<float, double, double, double>(trans, alpha, beta, A, x, y, it, n_it);
+}

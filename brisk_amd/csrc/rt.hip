@@ -1,0 +1,134 @@
+// rt.hip - device runtime of the solver library: one stream, device allocations, a pinned
+// staging ring for the small item arrays the batched launches read, events and errors.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+#include <time.h>
+#include "dev.h"
+#include "rt_internal.h"
+
+static int g_init = 0;
+static int g_failed = 0;
+static char g_err[256] = "";
+hipStream_t g_bk_stream = nullptr;
+
+// staging ring: host pinned + device, recycled at every synchronisation
+static char *g_ring_h = nullptr, *g_ring_d = nullptr;
+static size_t g_ring_cap = 0, g_ring_pos = 0;
+
+int bk_check(hipError_t e, const char *what) {
+    if (e == hipSuccess) return 0;
+    g_failed = 1;
+    snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
+    fprintf(stderr, "brisk gpu: %s\n", g_err);
+    return -1;
+}
+
+extern "C" int bk_init(void) {
+    if (g_init) return g_failed ? -1 : 0;
+    int n = 0;
+    if (bk_check(hipGetDeviceCount(&n), "hipGetDeviceCount")) return -1;
+    if (n < 1) { snprintf(g_err, sizeof(g_err), "no GPU"); g_failed = 1; return -1; }
+    if (bk_check(hipSetDevice(0), "hipSetDevice")) return -1;
+    if (bk_check(hipStreamCreateWithFlags(&g_bk_stream, hipStreamNonBlocking), "hipStreamCreate")) return -1;
+    g_ring_cap = (size_t)64 << 20;
+    if (bk_check(hipHostMalloc((void **)&g_ring_h, g_ring_cap, hipHostMallocDefault), "hipHostMalloc ring")) return -1;
+    if (bk_check(hipMalloc((void **)&g_ring_d, g_ring_cap), "hipMalloc ring")) return -1;
+    g_ring_pos = 0;
+    g_init = 1;
+    return 0;
+}
+
+extern "C" const char *bk_error(void) { return g_err; }
+extern "C" int bk_failed(void) { return g_failed; }
+
+extern "C" void *bk_malloc(size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    if (bk_check(hipMalloc(&p, bytes), "hipMalloc")) return nullptr;
+    return p;
+}
+extern "C" void bk_free(void *p) { if (p) (void)hipFree(p); }
+
+extern "C" int bk_sync(void) {
+    int r = bk_check(hipStreamSynchronize(g_bk_stream), "hipStreamSynchronize");
+    g_ring_pos = 0;
+    return r;
+}
+extern "C" int bk_h2d(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return 0;
+    if (bk_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_bk_stream), "hipMemcpy H2D")) return -1;
+    return bk_sync();
+}
+extern "C" int bk_d2h(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return bk_sync();
+    if (bk_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_bk_stream), "hipMemcpy D2H")) return -1;
+    return bk_sync();
+}
+extern "C" int bk_d2d(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return 0;
+    return bk_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_bk_stream), "hipMemcpy D2D");
+}
+
+__global__ void k_zero(unsigned long long *p, size_t nw) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < nw; q += (size_t)gridDim.x * blockDim.x) p[q] = 0ull;
+}
+__global__ void k_zero_bytes(unsigned char *p, size_t nb) {
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < nb; q += (size_t)gridDim.x * blockDim.x) p[q] = 0;
+}
+extern "C" int bk_zero(void *dst, size_t bytes) {
+    if (!bytes) return 0;
+    if (((uintptr_t)dst & 7) == 0 && (bytes & 7) == 0) {
+        size_t nw = bytes / 8;
+        unsigned grid = (unsigned)((nw + 255) / 256 < 4096 ? (nw + 255) / 256 : 4096);
+        hipLaunchKernelGGL(k_zero, dim3(grid), dim3(256), 0, g_bk_stream, (unsigned long long *)dst, nw);
+    } else {
+        unsigned grid = (unsigned)((bytes + 255) / 256 < 4096 ? (bytes + 255) / 256 : 4096);
+        hipLaunchKernelGGL(k_zero_bytes, dim3(grid), dim3(256), 0, g_bk_stream, (unsigned char *)dst, bytes);
+    }
+    return bk_check(hipGetLastError(), "k_zero");
+}
+
+extern "C" void *bk_stage(const void *host, size_t bytes) {
+    size_t need = (bytes + 255) & ~(size_t)255;
+    if (need == 0) need = 256;
+    if (need > g_ring_cap) { snprintf(g_err, sizeof(g_err), "staging request of %zu bytes too large", bytes); g_failed = 1; return nullptr; }
+    if (g_ring_pos + need > g_ring_cap) bk_sync();            // every staged copy has completed: recycle
+    char *h = g_ring_h + g_ring_pos, *d = g_ring_d + g_ring_pos;
+    memcpy(h, host, bytes);
+    if (bk_check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, g_bk_stream), "stage")) return nullptr;
+    g_ring_pos += need;
+    return d;
+}
+
+extern "C" double bk_now(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+extern "C" size_t bk_mem_free(void) {
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) return 0;
+    return f;
+}
+extern "C" void *bk_event_new(void) {
+    hipEvent_t e;
+    if (bk_check(hipEventCreate(&e), "hipEventCreate")) return nullptr;
+    return (void *)e;
+}
+extern "C" void bk_event_record(void *e) { bk_check(hipEventRecord((hipEvent_t)e, g_bk_stream), "hipEventRecord"); }
+extern "C" float bk_event_ms(void *a, void *b) {
+    float ms = 0;
+    bk_check(hipEventSynchronize((hipEvent_t)b), "hipEventSynchronize");
+    bk_check(hipEventElapsedTime(&ms, (hipEvent_t)a, (hipEvent_t)b), "hipEventElapsedTime");
+    return ms;
+}
+extern "C" void bk_event_free(void *e) { if (e) (void)hipEventDestroy((hipEvent_t)e); }
+static char g_devname[256] = "";
+extern "C" const char *bk_device_name(void) {
+    if (!g_devname[0]) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, 0) == hipSuccess) snprintf(g_devname, sizeof(g_devname), "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+    }
+    return g_devname;
+}

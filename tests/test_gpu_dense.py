@@ -1,0 +1,143 @@
+"""Kernel-level parity of the dense batched linear algebra (MFMA GEMM, blocked Cholesky,
+triangular inverse, matrix-vector products) against numpy/LAPACK on the same inputs."""
+import numpy as np
+import pytest
+from _gpu_util import lib, DBuf, GemmItem, MItem, GemvItem, items_array
+
+pytestmark = pytest.mark.gpu
+
+
+def _spd(n, rng, cond=1e3):
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    e = np.logspace(0, np.log10(cond), n)
+    return (Q * e) @ Q.T
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_gemm_batched_ragged(dt, ta, tb):
+    rng = np.random.default_rng(1 + ta + 2 * tb)
+    shapes = [(70, 45, 33), (1, 1, 1), (64, 64, 64), (130, 17, 200), (5, 129, 7)]
+    A_parts, B_parts, C_parts, rows, refs = [], [], [], [], []
+    oa = ob = oc = 0
+    for (M, N, K) in shapes:
+        A = rng.standard_normal((K, M) if ta else (M, K))
+        B = rng.standard_normal((N, K) if tb else (K, N))
+        C = rng.standard_normal((M, N))
+        alpha, beta = 0.7, -0.3
+        ref = alpha * (A.T if ta else A) @ (B.T if tb else B) + beta * C
+        lda = A.shape[0]; ldb = B.shape[0]
+        rows.append((M, N, K, lda, ldb, M, oa, ob, oc, -1, 0))
+        A_parts.append(A.flatten(order="F")); B_parts.append(B.flatten(order="F")); C_parts.append(C.flatten(order="F"))
+        refs.append((oc, M, N, ref))
+        oa += A.size; ob += B.size; oc += C.size
+    dA = DBuf(np.concatenate(A_parts).astype(dt)); dB = DBuf(np.concatenate(B_parts).astype(dt))
+    dC = DBuf(np.concatenate(C_parts).astype(dt))
+    it = items_array(GemmItem, rows)
+    L = lib()
+    f = L.bk_gemm_f64 if dt == np.float64 else L.bk_gemm_f32
+    assert f(ta, tb, 0.7, -0.3, dA.p, dB.p, dC.p, it, len(rows), None) == 0
+    out = dC.get()
+    tol = 1e-12 if dt == np.float64 else 2e-4
+    for (oc, M, N, ref) in refs:
+        got = out[oc:oc + M * N].reshape((M, N), order="F")
+        assert np.allclose(got, ref, rtol=tol, atol=tol * np.abs(ref).max()), (M, N)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_gemm_triangular_flags(dt):
+    """Zi = Linv' Linv with the K range trimmed (2|8) and a lower-only update (1)."""
+    rng = np.random.default_rng(7)
+    n = 150
+    Li = np.tril(rng.standard_normal((n, n)))
+    dA = DBuf(Li.flatten(order="F").astype(dt))
+    dC = DBuf(np.zeros(n * n, dt))
+    it = items_array(GemmItem, [(n, n, n, n, n, n, 0, 0, 0, -1, 2 | 8)])
+    f = lib().bk_gemm_f64 if dt == np.float64 else lib().bk_gemm_f32
+    assert f(1, 0, 1.0, 0.0, dA.p, dA.p, dC.p, it, 1, None) == 0
+    got = dC.get().reshape((n, n), order="F")
+    ref = Li.T @ Li
+    tol = 1e-12 if dt == np.float64 else 1e-4
+    assert np.allclose(got, ref, rtol=tol, atol=tol * np.abs(ref).max())
+    # lower-only: C = -W W' + C, upper triangle left untouched
+    W = rng.standard_normal((n, 64))
+    C0 = rng.standard_normal((n, n))
+    dW = DBuf(W.flatten(order="F").astype(dt)); dC2 = DBuf(C0.flatten(order="F").astype(dt))
+    it = items_array(GemmItem, [(n, n, 64, n, n, n, 0, 0, 0, -1, 1)])
+    assert f(0, 1, -1.0, 1.0, dW.p, dW.p, dC2.p, it, 1, None) == 0
+    got = dC2.get().reshape((n, n), order="F")
+    ref = C0 - W @ W.T
+    low = np.tril(np.ones((n, n), bool))
+    assert np.allclose(got[low], ref[low], rtol=tol, atol=tol * 50)
+    assert np.array_equal(got[~low], C0.astype(dt)[~low])
+
+
+def _pack(mats, dt):
+    offs, tot = [], 0
+    for M in mats:
+        offs.append(tot); tot += M.size
+    buf = np.zeros(tot, dt)
+    for M, o in zip(mats, offs):
+        buf[o:o + M.size] = M.flatten(order="F")
+    return buf, offs
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_potrf_trtri_batched(dt):
+    rng = np.random.default_rng(3)
+    sizes = [1, 5, 63, 64, 65, 130, 200, 300]
+    mats = [_spd(n, rng, 1e2) for n in sizes]
+    mats.append(-np.eye(7))                                # not positive definite: fails at column 1
+    sizes.append(7)
+    buf, offs = _pack(mats, dt)
+    nb = [((n + 63) // 64) * 4096 for n in sizes]
+    doff = np.concatenate([[0], np.cumsum(nb)[:-1]]).astype(np.int64)
+    woff = np.array(offs, np.int64)
+    rows = [(n, k, offs[k], int(doff[k]), int(woff[k])) for k, n in enumerate(sizes)]
+    it = items_array(MItem, rows)
+    dA = DBuf(buf); dD = DBuf(nbytes=int(sum(nb)) * np.dtype(dt).itemsize, dtype=dt)
+    dW = DBuf(nbytes=buf.nbytes, dtype=dt)
+    dinfo = DBuf(np.zeros(len(sizes), np.int32)); dpmin = DBuf(np.zeros(len(sizes), dt))
+    L = lib()
+    potrf = L.bk_potrf_f64 if dt == np.float64 else L.bk_potrf_f32
+    trtri = L.bk_trtri_f64 if dt == np.float64 else L.bk_trtri_f32
+    assert potrf(it, len(rows), dA.p, dD.p, dW.p, dinfo.p, dpmin.p) == 0
+    info = dinfo.get(); pmin = dpmin.get()
+    assert list(info[:-1]) == [0] * (len(sizes) - 1)
+    assert info[-1] == 1
+    out = dA.get()
+    tol = 1e-11 if dt == np.float64 else 2e-4
+    for k, n in enumerate(sizes[:-1]):
+        Lg = out[offs[k]:offs[k] + n * n].reshape((n, n), order="F")
+        Lr = np.linalg.cholesky(mats[k])
+        assert np.allclose(Lg, Lr, rtol=tol, atol=tol * np.abs(Lr).max()), n
+        assert abs(pmin[k] - np.min(np.diag(Lr)) ** 2) <= tol * 10 * max(1, pmin[k])
+    dLi = DBuf(nbytes=buf.nbytes, dtype=dt)
+    assert trtri(it, len(rows), dA.p, dD.p, dLi.p, dW.p, dinfo.p) == 0
+    li = dLi.get()
+    for k, n in enumerate(sizes[:-1]):
+        Lg = out[offs[k]:offs[k] + n * n].reshape((n, n), order="F").astype(np.float64)
+        Lig = li[offs[k]:offs[k] + n * n].reshape((n, n), order="F").astype(np.float64)
+        assert np.allclose(np.triu(Lig, 1), 0)
+        err = np.abs(Lig @ Lg - np.eye(n)).max()
+        assert err < (1e-10 if dt == np.float64 else 5e-3), (n, err)
+
+
+@pytest.mark.parametrize("trans", [0, 1])
+def test_gemv_batched(trans):
+    rng = np.random.default_rng(5)
+    shapes = [(1, 1), (100, 37), (257, 513), (64, 64)]
+    As, xs, ys, rows, refs = [], [], [], [], []
+    oa = ox = oy = 0
+    for (M, N) in shapes:
+        A = rng.standard_normal((M, N)); x = rng.standard_normal(M if trans else N); y = rng.standard_normal(N if trans else M)
+        ref = 1.5 * (A.T @ x if trans else A @ x) - 0.5 * y
+        rows.append((M, N, M, oa, ox, oy)); refs.append((oy, ref))
+        As.append(A.flatten(order="F")); xs.append(x); ys.append(y)
+        oa += A.size; ox += x.size; oy += y.size
+    dA = DBuf(np.concatenate(As)); dx = DBuf(np.concatenate(xs)); dy = DBuf(np.concatenate(ys))
+    it = items_array(GemvItem, rows)
+    assert lib().bk_gemv_f64(trans, 1.5, -0.5, dA.p, dx.p, dy.p, it, len(rows)) == 0
+    out = dy.get()
+    for (o, ref) in refs:
+        assert np.allclose(out[o:o + ref.size], ref, rtol=1e-12, atol=1e-12)
