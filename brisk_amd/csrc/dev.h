@@ -78,34 +78,66 @@ int bk_gemv_f32(int trans, float alpha, float beta, const float *A, const float 
 /* mixed: y (f64) = alpha op(A) x + beta y with A in f32, x in f64 (products and sums in f64) */
 int bk_gemv_mixed(int trans, double alpha, double beta, const float *A, const double *x, double *y, const GemvItem *it, int n_it);
 
-/* ---- segmented deterministic reductions ---- */
-typedef struct { int64_t off, len; } Seg;
-/* out[s] = sum_q a[off+q] * b[off+q] over each segment (b == NULL: sum of a^2 ... a == b allowed);
- * the order of summation is fixed by the segment length alone */
-int bk_seg_dot(const Seg *segs, int nseg, const double *a, const double *b, double *out);
-/* out[s] = sum of a[off+q] */
-int bk_seg_sum(const Seg *segs, int nseg, const double *a, double *out);
+/* ---- the device heap: one allocation; every array is an offset from its base ---- */
+int bk_heap_init(size_t bytes);                  /* (re)allocate; 0 on success */
+void *bk_heap(void);                             /* base pointer */
+size_t bk_heap_size(void);
 
-/* ---- elementwise over a flat range ---- */
-/* y = a*x + b*y + c*z  (any of x, z may be NULL) */
-int bk_axpbypcz(int64_t n, double a, const double *x, double b, double *y, double c, const double *z);
-/* per-segment scalars: y[q] = x[q] + s[seg] * d[q] (y may alias x); V update of try_step */
-int bk_seg_axpy(const Seg *segs, int nseg, const double *s, const double *x, const double *d, double *y);
-/* per-segment scalar a[s]: y[q] += a[s] * x[q] */
-int bk_seg_addscal(const Seg *segs, int nseg, const double *a, const double *x, double *y);
-/* symmetrise square blocks: G = (G + G') / 2 */
-int bk_symmetrize(const MItem *it, int n_it, double *G);
-/* copy the lower triangle of src into dst and zero dst's strict upper triangle */
-int bk_copy_lower(const MItem *it, int n_it, const double *src, double *dst);
-/* add s * I to every matrix */
-int bk_add_diag(const MItem *it, int n_it, const double *s_per_slot, double *A);
-/* equilibrate and convert: F = D A D (lower from A's full storage), D = 1/sqrt(diag A) (or 1) */
-int bk_equil_f32(const MItem *it, int n_it, const double *A, float *F, double *D, const int64_t *doff);
-int bk_equil_f64(const MItem *it, int n_it, const double *A, double *F, double *D, const int64_t *doff, const double *reg_per_slot);
-/* v[q] = v[q] * s[q] (f64) and conversions */
-int bk_vmul(int64_t n, const double *a, const double *b, double *out);
-int bk_d2f(int64_t n, const double *a, float *out);
-int bk_f2d(int64_t n, const float *a, double *out);
+/* ---- segmented deterministic reductions ---- */
+/* out[s] = sum_q x[xo + q] * y[yo + q], q < len, for each item; chunked partial sums (4096
+ * elements per chunk, fixed-order tree in a chunk) summed in chunk order: the result depends on
+ * the data and len only. All offsets are elements of double from the heap base. */
+typedef struct { int64_t len, xo, yo, out; } DotItem;
+int bk_dots(const DotItem *it, int n_it);
+
+/* out[q] = c0 in0[q] + c1 in1[q] + c2 in2[q] + c3 in3[q] for q < len (an offset < 0 means the
+ * term is absent; out may alias any input) */
+typedef struct { int64_t len, out, in[4]; double c[4]; } LinItem;
+int bk_lincomb(const LinItem *it, int n_it);
+/* out[q] = c0 in0[q] in1[q] (+ c1 in2[q] when in[2] >= 0) */
+int bk_vmul(const LinItem *it, int n_it);
+
+/* per square matrix at heap offset off (n = order): */
+int bk_symmetrize(const MItem *it, int n_it);                       /* G = (G + G')/2 */
+int bk_transpose_f64(const MItem *it, int n_it, int64_t dst_delta); /* dst = off + delta: src' */
+int bk_transpose_f32(const MItem *it, int n_it, int64_t dst_delta);
+/* A += s[slot] I, with s read from the heap at offset s_off (doubles) */
+int bk_add_diag(const MItem *it, int n_it, int64_t s_off);
+/* equilibration of M (full f64 at off): D[off2 + i] = 1/sqrt(M_ii) (1 when M_ii <= 0);
+ * F (at off3, lower triangle, strict upper zero) = D M D + reg[slot] I in f32 (prec 0) or f64 (1);
+ * reg read from the heap at reg_off (doubles, per slot); entries below 1e-30 flushed to 0 */
+int bk_equil(const MItem *it, int n_it, int prec, int64_t reg_off);
+
+/* ---- constraint operators (route 0: sparse entries) ---- */
+typedef struct { int k, r, c, pad; double v; } SEnt;   /* upper entry (r <= c) of block k */
+typedef struct { int n; int pad; int64_t off; } BlkRef; /* block k: order and offset in the solve's block arena */
+/* one item per solve: m constraints; CSR cptr (m+1 ints) into SEnt entries (ent); block table
+ * (blk); cptr, ent, blk, pptr, ppos, pcon are BYTE offsets from the heap base, the rest doubles;
+ * aop: out[i] = beta out[i] + alpha <A_i, D> (D read as sym(D): both triangles);
+ * aty: D[pos] += alpha sum_i y_i A_i over the positions of the position CSR (pptr, pcon, pval);
+ * schur: M (m x m full) = sum over common blocks of sum_{p,q} v_p v_q X[c_p, r_q] Zi[c_q, r_p]
+ * (full symmetric expansion of each upper entry) */
+typedef struct {
+    int m, nblk;
+    int64_t cptr, ent, blk;          /* constraint CSR (int), SEnt, BlkRef */
+    int64_t pptr, ppos, pcon, pval;  /* position CSR: npos+1 ints, positions (2 x int: o1, o2 or -1), con (int), val (double) */
+    int npos, diag;                  /* diag: every constraint one diagonal entry of one block (Hadamard Schur) */
+    int64_t dpos, dval;              /* diag: per constraint its position in the arena (int64) and value */
+    int64_t D, out, y, X, Zi, M;     /* operand offsets (doubles) for this call */
+    double alpha, beta;
+} ConItem;
+int bk_aop(const ConItem *it, int n_it);
+int bk_aty(const ConItem *it, int n_it);
+int bk_schur(const ConItem *it, int n_it);
+
+/* ---- Lanczos on explicit symmetric T (per block) ---- */
+/* step j for every item: given w = T q_j (at w), alpha_j = q_j.w; w -= alpha q_j + beta_{j-1} q_{j-1};
+ * two passes of full reorthogonalisation against q_0..q_j; beta_j = |w|; q_{j+1} = w / beta_j.
+ * Q holds the basis (K x n per item), ab the coefficients (alpha at ab, beta at ab + 32) */
+typedef struct { int n, K; int64_t Q, w, ab; } LzItem;
+int bk_lanczos_step(const LzItem *it, int n_it, int j);
+/* q_0 = the fixed start vector of the reference (1 + 0.5 sin(1.7 i + 0.3), normalised) */
+int bk_lanczos_start(const LzItem *it, int n_it);
 
 #ifdef __cplusplus
 }

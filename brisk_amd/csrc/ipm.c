@@ -1,0 +1,1347 @@
+/* ipm.c - the primal-dual interior-point iteration on the GPU, batched over solves (host C).
+ *
+ * The method is the reference's infeasible-start Mehrotra predictor-corrector with the HKM
+ * direction (solver.c:13507-15189); each phase below cites the lines it follows. All matrix and
+ * vector work runs as batched HIP launches over every active solve; each solve keeps its own
+ * scalars on the host, read back from a single batch scalar buffer once per phase.
+ *
+ * Precision: in FP64 mode everything is FP64. In mixed mode the Schur complement is factored in
+ * FP32 (equilibrated, on MFMA) while max(relgap, pinf, dinf) exceeds mixed_until, and every
+ * solve with it is refined by PCG against the FP64 product M v (reference solver.c:5266-5354,
+ * 5483-5622); a factor whose PCG needs more than 4 steps or misses its tolerance switches the
+ * solve to FP64 for good (reference solver.c:5620). */
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "solver.h"
+
+#define NSC 64                  /* scalars per solve in the batch scalar buffer */
+#define FLD(s, f) (((Solve *)(s))->f)
+
+void solve_log(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vfprintf(stderr, fmt, ap);
+    va_end(ap);
+}
+
+/* ---- growable item lists --------------------------------------------------------------------- */
+#define LIST(T, name) typedef struct { T *v; int n, cap; } name; \
+    static void name##_add(name *L, T x) { if (L->n == L->cap) { L->cap = L->cap ? 2 * L->cap : 64; L->v = realloc(L->v, sizeof(T) * L->cap); } L->v[L->n++] = x; } \
+    static void name##_clear(name *L) { L->n = 0; }
+LIST(DotItem, DotL)
+LIST(LinItem, LinL)
+LIST(GemmItem, GemmL)
+LIST(MItem, MItL)
+LIST(GemvItem, GemvL)
+LIST(ConItem, ConL)
+LIST(LzItem, LzL)
+
+static DotL g_dl; static LinL g_ll; static GemmL g_gl; static MItL g_ml; static GemvL g_vl; static ConL g_cl; static LzL g_zl;
+
+/* ---- batch scratch (device) ----------------------------------------------------------------- */
+static int64_t g_bsc = -1, g_bint = -1, g_bdbl = -1;   /* byte offsets */
+static int g_bsc_cap = 0, g_bint_cap = 0;
+static double *g_hsc = NULL;                            /* host mirror of the batch scalars */
+static int *g_hint = NULL;
+static double *g_hdbl = NULL;
+
+static int batch_scratch(int ns, int nslots) {
+    if (ns > g_bsc_cap) {
+        if (g_bsc >= 0) heap_free(g_bsc);
+        g_bsc_cap = ns + 16;
+        g_bsc = heap_alloc(sizeof(double) * NSC * g_bsc_cap);
+        g_hsc = realloc(g_hsc, sizeof(double) * NSC * g_bsc_cap);
+        if (g_bsc < 0) return -1;
+    }
+    if (nslots > g_bint_cap) {
+        if (g_bint >= 0) heap_free(g_bint);
+        if (g_bdbl >= 0) heap_free(g_bdbl);
+        g_bint_cap = nslots + 64;
+        g_bint = heap_alloc(sizeof(int) * g_bint_cap);
+        g_bdbl = heap_alloc(sizeof(double) * g_bint_cap);
+        g_hint = realloc(g_hint, sizeof(int) * g_bint_cap);
+        g_hdbl = realloc(g_hdbl, sizeof(double) * g_bint_cap);
+        if (g_bint < 0 || g_bdbl < 0) return -1;
+    }
+    return 0;
+}
+void ipm_release_scratch(void) {
+    bk_sync();
+    if (g_bsc >= 0) heap_free(g_bsc);
+    if (g_bint >= 0) heap_free(g_bint);
+    if (g_bdbl >= 0) heap_free(g_bdbl);
+    g_bsc = g_bint = g_bdbl = -1;
+    g_bsc_cap = g_bint_cap = 0;
+}
+static inline int64_t SC(int q, int k) { return g_bsc / 8 + (int64_t)q * NSC + k; }  /* double offset of scalar k of batch slot q */
+static inline double HS(int q, int k) { return g_hsc[(size_t)q * NSC + k]; }
+static void *heapp(void) { return bk_heap(); }
+static int *dint(void) { return (int *)((char *)bk_heap() + g_bint); }
+static double *ddbl(void) { return (double *)((char *)bk_heap() + g_bdbl); }
+static int read_scalars(int ns) { return bk_d2h(g_hsc, (char *)bk_heap() + g_bsc, sizeof(double) * NSC * ns); }
+
+/* ---- solve setup ----------------------------------------------------------------------------- */
+typedef struct { int64_t dinv_tot, lz_tot; int64_t *dboff, *lzoff; } BlkAux;
+
+static int64_t arr(Solve *S, size_t bytes, int *fail) {
+    const int64_t o = heap_alloc(bytes);
+    if (o < 0) { *fail = 1; return 0; }
+    return o;
+}
+/* per-solve block auxiliaries are kept in a side table keyed by the solve */
+typedef struct { Solve *S; int64_t *dboff, *lzoff; int64_t allocs[96]; int nalloc; } SAux;
+static SAux *g_aux = NULL;
+static int g_naux = 0, g_capaux = 0;
+static SAux *aux_of(Solve *S) {
+    for (int i = 0; i < g_naux; i++) if (g_aux[i].S == S) return &g_aux[i];
+    return NULL;
+}
+
+int64_t solve_heap_bytes(const HProb *P) {
+    const int64_t T = P->T, m = P->m;
+    int64_t dinv = 0, lz = 0;
+    for (int k = 0; k < P->nb; k++) { dinv += (int64_t)((P->bn[k] + 63) / 64) * 4096; lz += P->bn[k]; }
+    int64_t b = 0;
+    b += 17 * (T * 8 + 256);                              /* block arena arrays */
+    b += 2 * (dinv * 8 + 256);
+    b += (BK_LZ_K + 1) * lz * 8 + 64 * 8 * (int64_t)P->nb + 768;
+    b += 5 * (m * m * 8 + 256) + (((m + 63) / 64) * 4096 * 8 + 256);
+    b += 20 * (m * 8 + 256) + NSC * 8 + 256;
+    b += (m + 1) * 4 + (int64_t)P->nent * (sizeof(SEnt) + 12) + (int64_t)P->nb * sizeof(BlkRef) + (P->npos + 1) * 12 + 2048;
+    return b;
+}
+
+int solve_setup(Solve *S) {
+    HProb *P = &S->P;
+    if (g_naux == g_capaux) { g_capaux = g_capaux ? 2 * g_capaux : 16; g_aux = realloc(g_aux, sizeof(SAux) * g_capaux); }
+    SAux *A = &g_aux[g_naux++];
+    memset(A, 0, sizeof(*A));
+    A->S = S;
+    A->dboff = malloc(sizeof(int64_t) * (P->nb + 1));
+    A->lzoff = malloc(sizeof(int64_t) * (P->nb + 1));
+    int64_t dinv = 0, lz = 0;
+    for (int k = 0; k < P->nb; k++) { A->dboff[k] = dinv; A->lzoff[k] = lz; dinv += (int64_t)((P->bn[k] + 63) / 64) * 4096; lz += P->bn[k]; }
+    int fail = 0;
+    const int64_t T = P->T, m = P->m;
+#define ALLOC(field, bytes) do { const int64_t o_ = heap_alloc(bytes); if (o_ < 0) fail = 1; else { A->allocs[A->nalloc++] = o_; S->field = o_ / 8; } } while (0)
+#define ALLOCB(field, bytes) do { const int64_t o_ = heap_alloc(bytes); if (o_ < 0) fail = 1; else { A->allocs[A->nalloc++] = o_; S->field = o_; } } while (0)
+    ALLOC(X, T * 8); ALLOC(Z, T * 8); ALLOC(Zi, T * 8); ALLOC(LX, T * 8); ALLOC(LZ, T * 8); ALLOC(LXi, T * 8); ALLOC(LZi, T * 8);
+    ALLOC(dX, T * 8); ALLOC(dZ, T * 8); ALLOC(G, T * 8); ALLOC(Q, T * 8); ALLOC(R0, T * 8); ALLOC(F, T * 8); ALLOC(W1, T * 8);
+    ALLOC(W2, T * 8); ALLOC(C, T * 8); ALLOC(Xb, T * 8);
+    ALLOC(DinvX, dinv * 8); ALLOC(DinvZ, dinv * 8);
+    ALLOC(lzQ, (int64_t)BK_LZ_K * lz * 8); ALLOC(lzw, lz * 8); ALLOC(lzab, (int64_t)64 * P->nb * 8);
+    ALLOC(M, m * m * 8); ALLOC(MF, m * m * 8); ALLOC(MLi, m * m * 8); ALLOC(MLiT, m * m * 8); ALLOC(Mwork, m * m * 8);
+    ALLOC(MDinv, ((m + 63) / 64) * 4096 * 8); ALLOC(Dm, m * 8);
+    ALLOC(y, m * 8); ALLOC(dy, m * 8); ALLOC(rp, m * 8); ALLOC(rhs, m * 8); ALLOC(a0, m * 8); ALLOC(h, m * 8); ALLOC(qv, m * 8);
+    ALLOC(ev, m * 8); ALLOC(wv, m * 8); ALLOC(bvec, m * 8); ALLOC(pr, m * 8); ALLOC(pz, m * 8); ALLOC(pp, m * 8); ALLOC(pq, m * 8);
+    ALLOC(pxb, m * 8); ALLOC(t1, m * 8); ALLOC(t2, m * 8); ALLOC(yb, m * 8);
+    ALLOCB(cptr, (m + 1) * 4); ALLOCB(ent, (int64_t)(P->nent + 1) * sizeof(SEnt)); ALLOCB(blk, (int64_t)P->nb * sizeof(BlkRef));
+    ALLOCB(pptr, (int64_t)(P->npos + 1) * 4); ALLOCB(ppos, (int64_t)(P->npos + 1) * 8); ALLOCB(pcon, (int64_t)(P->nent + 1) * 4);
+    ALLOC(pval, (int64_t)(P->nent + 1) * 8);
+#undef ALLOC
+#undef ALLOCB
+    if (fail) { solve_release(S); return -1; }
+    char *H = heapp();
+    int rc = 0;
+    rc |= bk_h2d(H + S->C * 8, P->C, T * 8);
+    rc |= bk_h2d(H + S->bvec * 8, P->b, m * 8);
+    rc |= bk_h2d(H + S->cptr, P->cptr, (m + 1) * 4);
+    if (P->nent) rc |= bk_h2d(H + S->ent, P->ent, (int64_t)P->nent * sizeof(SEnt));
+    BlkRef *br = malloc(sizeof(BlkRef) * P->nb);
+    for (int k = 0; k < P->nb; k++) { br[k].n = P->bn[k]; br[k].pad = 0; br[k].off = P->boff[k]; }
+    rc |= bk_h2d(H + S->blk, br, (int64_t)P->nb * sizeof(BlkRef));
+    free(br);
+    rc |= bk_h2d(H + S->pptr, P->pptr, (int64_t)(P->npos + 1) * 4);
+    if (P->npos) rc |= bk_h2d(H + S->ppos, P->ppos, (int64_t)P->npos * 8);
+    if (P->nent) { rc |= bk_h2d(H + S->pcon, P->pcon, (int64_t)P->nent * 4); rc |= bk_h2d(H + S->pval * 8, P->pval, (int64_t)P->nent * 8); }
+    if (rc) { solve_release(S); return -1; }
+    S->status = -1;
+    return 0;
+}
+
+void solve_release(Solve *S) {
+    SAux *A = aux_of(S);
+    if (!A) return;
+    bk_sync();
+    for (int i = 0; i < A->nalloc; i++) heap_free(A->allocs[i]);
+    free(A->dboff); free(A->lzoff);
+    *A = g_aux[--g_naux];
+}
+
+/* ---- batched building blocks ------------------------------------------------------------- */
+static ConItem con_item(Solve *S) {
+    ConItem c;
+    memset(&c, 0, sizeof(c));
+    c.m = S->P.m; c.nblk = S->P.nb;
+    c.cptr = S->cptr; c.ent = S->ent; c.blk = S->blk;
+    c.pptr = S->pptr; c.ppos = S->ppos; c.pcon = S->pcon; c.pval = S->pval;
+    c.npos = S->P.npos; c.diag = S->P.diag;
+    return c;
+}
+/* out = alpha A(D) + beta out, D and out given as double offsets per solve */
+static int aop(Solve **S, int ns, const int *act, size_t fD, size_t fout, double alpha, double beta) {
+    ConL_clear(&g_cl);
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        ConItem c = con_item(S[q]);
+        c.D = *(int64_t *)((char *)S[q] + fD); c.out = *(int64_t *)((char *)S[q] + fout); c.alpha = alpha; c.beta = beta;
+        ConL_add(&g_cl, c);
+    }
+    return bk_aop(g_cl.v, g_cl.n);
+}
+/* D += alpha A*(y) at the touched positions */
+static int aty(Solve **S, int ns, const int *act, size_t fy, size_t fD, double alpha) {
+    ConL_clear(&g_cl);
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        ConItem c = con_item(S[q]);
+        c.y = *(int64_t *)((char *)S[q] + fy); c.D = *(int64_t *)((char *)S[q] + fD); c.alpha = alpha;
+        ConL_add(&g_cl, c);
+    }
+    return bk_aty(g_cl.v, g_cl.n);
+}
+#define OFF(f) offsetof(Solve, f)
+static inline int64_t F_(Solve *S, size_t f) { return *(int64_t *)((char *)S + f); }
+
+/* lincomb over the block arena (len T) or vectors (len m): out = sum c_t in_t */
+static void lin_add(int64_t len, int64_t out, int64_t i0, double c0, int64_t i1, double c1, int64_t i2, double c2, int64_t i3, double c3) {
+    LinItem L;
+    L.len = len; L.out = out; L.in[0] = i0; L.in[1] = i1; L.in[2] = i2; L.in[3] = i3;
+    L.c[0] = c0; L.c[1] = c1; L.c[2] = c2; L.c[3] = c3;
+    LinL_add(&g_ll, L);
+}
+static void dot_add(int64_t len, int64_t x, int64_t y, int64_t out) {
+    DotItem d = { len, x, y, out };
+    DotL_add(&g_dl, d);
+}
+/* per-block products: out_k = op(A_k) op(B_k) over every block of the active solves */
+static int blk_gemm(Solve **S, int ns, const int *act, int ta, int tb, int tri, double alpha, double beta, size_t fA, size_t fB, size_t fC) {
+    GemmL_clear(&g_gl);
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        const HProb *P = &S[q]->P;
+        const int64_t a = F_(S[q], fA), b = F_(S[q], fB), c = F_(S[q], fC);
+        for (int k = 0; k < P->nb; k++) {
+            const int n = P->bn[k];
+            GemmItem g = { n, n, n, n, n, n, a + P->boff[k], b + P->boff[k], c + P->boff[k], -1, tri };
+            GemmL_add(&g_gl, g);
+        }
+    }
+    double *H = heapp();
+    return bk_gemm_f64(ta, tb, alpha, beta, H, H, H, g_gl.v, g_gl.n, NULL);
+}
+static int blk_sym(Solve **S, int ns, const int *act, size_t f) {
+    MItL_clear(&g_ml);
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        const HProb *P = &S[q]->P;
+        for (int k = 0; k < P->nb; k++) {
+            if (P->bn[k] < 2) continue;
+            MItem it = { P->bn[k], 0, F_(S[q], f) + P->boff[k], 0, 0 };
+            MItL_add(&g_ml, it);
+        }
+    }
+    return bk_symmetrize(g_ml.v, g_ml.n);
+}
+/* Cholesky of every block of the active solves: L = V (copied), factored; ok[q] = 1 when every
+ * block of solve q is positive definite */
+static int blk_chol(Solve **S, int ns, const int *act, size_t fV, size_t fL, size_t fDinv, int *ok) {
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (!act || act[q]) lin_add(S[q]->P.T, F_(S[q], fL), F_(S[q], fV), 1.0, -1, 0, -1, 0, -1, 0);
+    if (bk_lincomb(g_ll.v, g_ll.n)) return -1;
+    MItL_clear(&g_ml);
+    int slot = 0;
+    int *first = malloc(sizeof(int) * (ns + 1));
+    for (int q = 0; q < ns; q++) {
+        first[q] = slot;
+        if (act && !act[q]) continue;
+        const HProb *P = &S[q]->P;
+        SAux *A = aux_of(S[q]);
+        for (int k = 0; k < P->nb; k++) {
+            MItem it = { P->bn[k], slot++, F_(S[q], fL) + P->boff[k], F_(S[q], fDinv) + A->dboff[k], S[q]->W2 + P->boff[k] };
+            MItL_add(&g_ml, it);
+        }
+    }
+    first[ns] = slot;
+    if (batch_scratch(ns, slot + 1)) { free(first); return -1; }
+    bk_zero(dint(), sizeof(int) * (slot + 1));
+    double *H = heapp();
+    if (bk_potrf_f64(g_ml.v, g_ml.n, H, H, H, dint(), ddbl())) { free(first); return -1; }
+    if (bk_d2h(g_hint, dint(), sizeof(int) * (slot + 1))) { free(first); return -1; }
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) { ok[q] = 1; continue; }
+        ok[q] = 1;
+        for (int s = first[q]; s < (q + 1 < ns ? first[q + 1] : slot); s++) if (g_hint[s]) ok[q] = 0;
+    }
+    free(first);
+    return 0;
+}
+/* Li = L^{-1} per block (after a successful blk_chol) */
+static int blk_inv(Solve **S, int ns, const int *act, size_t fL, size_t fDinv, size_t fLi) {
+    MItL_clear(&g_ml);
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        const HProb *P = &S[q]->P;
+        SAux *A = aux_of(S[q]);
+        for (int k = 0; k < P->nb; k++) {
+            MItem it = { P->bn[k], 0, F_(S[q], fL) + P->boff[k], F_(S[q], fDinv) + A->dboff[k], S[q]->W2 + P->boff[k] };
+            MItL_add(&g_ml, it);
+        }
+    }
+    if (batch_scratch(ns, 1)) return -1;
+    bk_zero(dint(), sizeof(int));
+    double *H = heapp();
+    int c = 0;
+    /* trtri reads L and writes Linv at the same item offsets of two base pointers: L and Linv of
+     * a solve differ by a per-solve shift, so one launch sequence per solve */
+    int rc = 0;
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        const HProb *P = &S[q]->P;
+        const int64_t shift = F_(S[q], fLi) - F_(S[q], fL);
+        rc |= bk_trtri_f64(g_ml.v + c, P->nb, H, H, H + shift, H, dint());
+        c += P->nb;
+    }
+    return rc;
+}
+
+/* ---- Lanczos step-length bound (reference solver.c:6083-6225) ----------------------------- */
+/* smallest eigenvalue of the k x k tridiagonal (a, b) and the last component of its eigenvector,
+ * by cyclic Jacobi on the dense tridiagonal (k <= 30) */
+static void tri_min(int k, const double *a, const double *b, double *theta, double *zlast) {
+    double Am[BK_LZ_K * BK_LZ_K], V[BK_LZ_K * BK_LZ_K];
+    memset(Am, 0, sizeof(Am)); memset(V, 0, sizeof(V));
+    for (int i = 0; i < k; i++) { Am[i + i * k] = a[i]; V[i + i * k] = 1; if (i + 1 < k) Am[i + (i + 1) * k] = Am[(i + 1) + i * k] = b[i]; }
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0, tot = 0;
+        for (int j = 0; j < k; j++) for (int i = 0; i < k; i++) { const double x = Am[i + j * k] * Am[i + j * k]; tot += x; if (i != j) off += x; }
+        if (off <= 1e-30 * tot || off == 0) break;
+        for (int p = 0; p < k; p++)
+            for (int r = p + 1; r < k; r++) {
+                const double apr = Am[p + r * k];
+                if (apr == 0) continue;
+                const double tau = (Am[r + r * k] - Am[p + p * k]) / (2 * apr);
+                const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
+                const double c = 1 / sqrt(1 + t * t), s = t * c;
+                for (int i = 0; i < k; i++) {
+                    const double x = Am[i + p * k], y = Am[i + r * k];
+                    Am[i + p * k] = c * x - s * y; Am[i + r * k] = s * x + c * y;
+                }
+                for (int i = 0; i < k; i++) {
+                    const double x = Am[p + i * k], y = Am[r + i * k];
+                    Am[p + i * k] = c * x - s * y; Am[r + i * k] = s * x + c * y;
+                }
+                for (int i = 0; i < k; i++) {
+                    const double x = V[i + p * k], y = V[i + r * k];
+                    V[i + p * k] = c * x - s * y; V[i + r * k] = s * x + c * y;
+                }
+            }
+    }
+    int im = 0;
+    for (int i = 1; i < k; i++) if (Am[i + i * k] < Am[im + im * k]) im = i;
+    *theta = Am[im + im * k];
+    *zlast = V[(k - 1) + im * k];
+}
+/* alpha_max per solve for V + alpha dV >= 0 given Li = chol(V)^{-1}: Lanczos on Li dV Li' */
+static int maxstep(Solve **S, int ns, const int *act, size_t fLi, size_t fdV, double *amax) {
+    /* T = Li dV Li' into W2 (W1 = Li dV first) */
+    if (blk_gemm(S, ns, act, 0, 0, 4, 1.0, 0.0, fLi, fdV, OFF(W1))) return -1;
+    if (blk_gemm(S, ns, act, 0, 1, 16, 1.0, 0.0, OFF(W1), fLi, OFF(W2))) return -1;
+    if (blk_sym(S, ns, act, OFF(W2))) return -1;
+    LzL_clear(&g_zl);
+    int kmax = 0;
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        const HProb *P = &S[q]->P;
+        SAux *A = aux_of(S[q]);
+        for (int k = 0; k < P->nb; k++) {
+            const int n = P->bn[k], K = n < BK_LZ_K ? n : BK_LZ_K;
+            LzItem z = { n, K, S[q]->lzQ + (int64_t)BK_LZ_K * A->lzoff[k], S[q]->lzw + A->lzoff[k], S[q]->lzab + 64 * k };
+            LzL_add(&g_zl, z);
+            if (K > kmax) kmax = K;
+        }
+    }
+    if (bk_lanczos_start(g_zl.v, g_zl.n)) return -1;
+    double *H = heapp();
+    for (int j = 0; j < kmax; j++) {
+        GemvL_clear(&g_vl);
+        int c = 0;
+        for (int q = 0; q < ns; q++) {
+            if (act && !act[q]) continue;
+            const HProb *P = &S[q]->P;
+            for (int k = 0; k < P->nb; k++, c++) {
+                const LzItem *z = &g_zl.v[c];
+                if (j >= z->K) continue;
+                GemvItem g = { z->n, z->n, z->n, S[q]->W2 + P->boff[k], z->Q + (int64_t)j * z->n, z->w };
+                GemvL_add(&g_vl, g);
+            }
+        }
+        if (bk_gemv_f64(1, 1.0, 0.0, H, H, H, g_vl.v, g_vl.n)) return -1;
+        if (bk_lanczos_step(g_zl.v, g_zl.n, j)) return -1;
+    }
+    /* coefficients back: per solve its lzab block (64 per block) */
+    int c = 0;
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) { c += 0; continue; }
+        const HProb *P = &S[q]->P;
+        double *ab = malloc(sizeof(double) * 64 * P->nb);
+        if (bk_d2h(ab, H + S[q]->lzab, sizeof(double) * 64 * P->nb)) { free(ab); return -1; }
+        double a = 1e30;
+        for (int k = 0; k < P->nb; k++, c++) {
+            const int K = g_zl.v[c].K;
+            const double *al = ab + 64 * k, *be = al + 32;
+            int kk = K;
+            double scale = 0;
+            for (int j = 0; j < K; j++) {
+                if (fabs(al[j]) > scale) scale = fabs(al[j]);
+                if (!(be[j] > 1e-12 * (scale + 1e-300))) { kk = j + 1; break; }
+            }
+            int bad = 0;
+            for (int j = 0; j < kk; j++) if (!isfinite(al[j]) || !isfinite(be[j])) bad = 1;
+            double ak;
+            if (bad) ak = 0;
+            else {
+                double th, zl;
+                tri_min(kk, al, be, &th, &zl);
+                const double res = (kk == K && kk < P->bn[k]) || (kk < K && be[kk - 1] > 1e-12 * (scale + 1e-300)) ? fabs(be[kk - 1] * zl) : 0.0;
+                const double lam = th - res;
+                ak = lam >= 0 ? 1e30 : -1.0 / lam;
+            }
+            if (ak < a) a = ak;
+        }
+        free(ab);
+        amax[q] = a;
+    }
+    return 0;
+}
+
+/* ---- the Schur complement: assemble, factor, solve ------------------------------------------ */
+static int schur_assemble(Solve **S, int ns, const int *act) {
+    ConL_clear(&g_cl);
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        ConItem c = con_item(S[q]);
+        c.X = S[q]->X; c.Zi = S[q]->Zi; c.M = S[q]->M;
+        ConL_add(&g_cl, c);
+    }
+    return bk_schur(g_cl.v, g_cl.n);
+}
+
+/* factor M of the active solves (reference solver.c:5266-5420): f32 when allowed (accepted if
+ * the smallest squared pivot is >= 1e-6), else f64 with the shift ladder and pivot test.
+ * fail[q] = 1 when no factorisation was found. */
+static int schur_factor(Solve **S, int ns, const int *act, int force64, int *fail) {
+    int *todo = calloc(ns, sizeof(int)), *f64 = calloc(ns, sizeof(int));
+    double *reg = calloc(ns, sizeof(double));
+    int nslot = ns;
+    if (batch_scratch(ns, nslot + 1)) { free(todo); free(f64); free(reg); return -1; }
+    double *H = heapp();
+    for (int q = 0; q < ns; q++) {
+        fail[q] = 0;
+        if (act && !act[q]) continue;
+        todo[q] = 1;
+        Solve *s = S[q];
+        f64[q] = force64 || s->o.prec == PREC_F64 || s->want_double;
+        const double floor_ = fmax(1e-15, s->P.m * 2.220446049250313e-16);
+        reg[q] = f64[q] && s->reg_hint ? fmax(floor_, 0.01 * s->reg_ok) : 0;
+    }
+    for (int attempt = 0; attempt < 12; attempt++) {
+        int any = 0;
+        for (int q = 0; q < ns; q++) any |= todo[q];
+        if (!any) break;
+        /* equilibrate into MF (f32 view of the same bytes for the f32 factor) */
+        for (int prec = 0; prec < 2; prec++) {
+            MItL_clear(&g_ml);
+            for (int q = 0; q < ns; q++) {
+                if (!todo[q] || f64[q] != prec) continue;
+                Solve *s = S[q];
+                g_hdbl[q] = reg[q];
+                const int64_t f = prec ? s->MF : 2 * s->MF;      /* element offset in the factor's type */
+                MItem it = { s->P.m, q, s->M, s->Dm, f };
+                MItL_add(&g_ml, it);
+            }
+            if (!g_ml.n) continue;
+            if (bk_h2d(ddbl(), g_hdbl, sizeof(double) * ns)) goto err;
+            if (bk_equil(g_ml.v, g_ml.n, prec, g_bdbl / 8)) goto err;
+            /* factor in place: items now point to MF in the factor's element type */
+            MItem *fi = malloc(sizeof(MItem) * g_ml.n);
+            for (int t = 0; t < g_ml.n; t++) {
+                const int q = g_ml.v[t].slot;
+                Solve *s = S[q];
+                const int64_t mul = prec ? 1 : 2;
+                fi[t].n = s->P.m; fi[t].slot = q; fi[t].off = mul * s->MF; fi[t].off2 = mul * s->MDinv; fi[t].off3 = mul * s->Mwork;
+            }
+            bk_zero(dint(), sizeof(int) * ns);
+            int rc = prec ? bk_potrf_f64(fi, g_ml.n, H, H, H, dint(), ddbl())
+                          : bk_potrf_f32(fi, g_ml.n, (float *)H, (float *)H, (float *)H, dint(), (float *)ddbl());
+            free(fi);
+            if (rc) goto err;
+            if (bk_d2h(g_hint, dint(), sizeof(int) * ns)) goto err;
+            if (bk_d2h(g_hdbl, ddbl(), sizeof(double) * ns)) goto err;
+            for (int t = 0; t < g_ml.n; t++) {
+                const int q = g_ml.v[t].slot;
+                Solve *s = S[q];
+                const double pmin = prec ? g_hdbl[q] : (double)((float *)g_hdbl)[q];
+                const double floor_ = fmax(1e-15, s->P.m * 2.220446049250313e-16);
+                if (!prec) {
+                    if (g_hint[q] == 0 && pmin >= 1e-6) { todo[q] = 0; s->f32 = 1; s->reg = 0; s->nfactor_f32++; }
+                    else { f64[q] = 1; s->want_double = 1; reg[q] = s->reg_hint ? fmax(floor_, 0.01 * s->reg_ok) : 0; }
+                    continue;
+                }
+                if (g_hint[q] == 0) {
+                    if (pmin >= 1e-14 || reg[q] >= floor_) {
+                        todo[q] = 0; s->f32 = 0; s->reg = reg[q]; s->reg_hint = reg[q] > 0; s->reg_ok = reg[q]; s->nfactor_f64++;
+                        continue;
+                    }
+                    reg[q] = fmax(floor_, reg[q] * 100);
+                } else reg[q] = reg[q] == 0 ? floor_ : reg[q] * 100;
+                if (attempt >= 10) { todo[q] = 0; fail[q] = 1; }
+            }
+        }
+    }
+    for (int q = 0; q < ns; q++) if (todo[q]) fail[q] = 1;
+    /* explicit inverse of the factor and its transpose: every later solve is two products */
+    for (int prec = 0; prec < 2; prec++) {
+        int c = 0;
+        for (int q = 0; q < ns; q++) {
+            if ((act && !act[q]) || fail[q] || S[q]->f32 == prec) continue;
+            Solve *s = S[q];
+            const int64_t mul = prec ? 1 : 2;
+            MItem it = { s->P.m, q, mul * s->MF, mul * s->MDinv, mul * s->Mwork };
+            const int64_t shift = mul * (s->MLi - s->MF);
+            bk_zero(dint(), sizeof(int) * ns);
+            int rc = prec ? bk_trtri_f64(&it, 1, H, H, H + shift, H, dint())
+                          : bk_trtri_f32(&it, 1, (float *)H, (float *)H, (float *)H + shift, (float *)H, dint());
+            if (rc) goto err;
+            MItem tt = { s->P.m, q, mul * s->MLi, 0, 0 };
+            rc = prec ? bk_transpose_f64(&tt, 1, mul * (s->MLiT - s->MLi)) : bk_transpose_f32(&tt, 1, mul * (s->MLiT - s->MLi));
+            if (rc) goto err;
+            c++;
+        }
+        (void)c;
+    }
+    free(todo); free(f64); free(reg);
+    return 0;
+err:
+    free(todo); free(f64); free(reg);
+    return -1;
+}
+
+/* z = M^{-1} r through the factor: z = D Li' Li D r */
+static int schur_prec(Solve **S, int ns, const int *act, size_t fr, size_t fz) {
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (!act || act[q]) lin_add(S[q]->P.m, S[q]->t1, S[q]->Dm, 1.0, F_(S[q], fr), 0, -1, 0, -1, 0);
+    if (bk_vmul(g_ll.v, g_ll.n)) return -1;
+    double *H = heapp();
+    for (int pass = 0; pass < 2; pass++) {
+        for (int prec = 0; prec < 2; prec++) {
+            GemvL_clear(&g_vl);
+            for (int q = 0; q < ns; q++) {
+                if ((act && !act[q]) || S[q]->f32 == prec) continue;
+                Solve *s = S[q];
+                const int64_t mul = prec ? 1 : 2;
+                /* pass 0: t2 = Li t1 (columns of Li' = rows of Li); pass 1: t1 = Li' t2 */
+                GemvItem g = { s->P.m, s->P.m, s->P.m, mul * (pass == 0 ? s->MLiT : s->MLi), pass == 0 ? s->t1 : s->t2, pass == 0 ? s->t2 : s->t1 };
+                GemvL_add(&g_vl, g);
+            }
+            if (!g_vl.n) continue;
+            if (prec ? bk_gemv_f64(1, 1.0, 0.0, H, H, H, g_vl.v, g_vl.n) : bk_gemv_mixed(1, 1.0, 0.0, (float *)H, H, H, g_vl.v, g_vl.n)) return -1;
+        }
+    }
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (!act || act[q]) lin_add(S[q]->P.m, F_(S[q], fz), S[q]->Dm, 1.0, S[q]->t1, 0, -1, 0, -1, 0);
+    return bk_vmul(g_ll.v, g_ll.n);
+}
+/* y = alpha M x + beta y in FP64 (M stored full) */
+static int schur_mv(Solve **S, int ns, const int *act, size_t fx, size_t fy, double alpha, double beta) {
+    GemvL_clear(&g_vl);
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        Solve *s = S[q];
+        GemvItem g = { s->P.m, s->P.m, s->P.m, s->M, F_(s, fx), F_(s, fy) };
+        GemvL_add(&g_vl, g);
+    }
+    double *H = heapp();
+    return bk_gemv_f64(1, alpha, beta, H, H, H, g_vl.v, g_vl.n);
+}
+/* batched PCG on M x = rhs (reference schur_solve_, solver.c:5483-5622); relres[q] out */
+static int schur_solve(Solve **S, int ns, const int *act0, size_t frhs, size_t fx, double tol_rel, double *relres) {
+    int *act = malloc(sizeof(int) * ns), *used = calloc(ns, sizeof(int)), *stall = calloc(ns, sizeof(int)), *maxit = malloc(sizeof(int) * ns);
+    double *nb = malloc(sizeof(double) * ns), *tol = malloc(sizeof(double) * ns), *best = malloc(sizeof(double) * ns), *rz = malloc(sizeof(double) * ns);
+    double *sgain = malloc(sizeof(double) * ns);
+    int *smax = malloc(sizeof(int) * ns);
+    int rc = -1;
+    if (batch_scratch(ns, 1)) goto out;
+    for (int q = 0; q < ns; q++) act[q] = !act0 || act0[q];
+    DotL_clear(&g_dl);
+    for (int q = 0; q < ns; q++) if (act[q]) dot_add(S[q]->P.m, F_(S[q], frhs), F_(S[q], frhs), SC(q, 0));
+    if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) {
+        if (!act[q]) continue;
+        nb[q] = sqrt(HS(q, 0));
+        relres[q] = 0;
+        if (nb[q] == 0) { act[q] = 0; lin_add(S[q]->P.m, F_(S[q], fx), -1, 0, -1, 0, -1, 0, -1, 0); }
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (schur_prec(S, ns, act, frhs, fx)) goto out;
+    /* r = rhs - M x */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->pr, F_(S[q], frhs), 1.0, -1, 0, -1, 0, -1, 0);
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (schur_mv(S, ns, act, fx, OFF(pr), -1.0, 1.0)) goto out;
+    DotL_clear(&g_dl);
+    for (int q = 0; q < ns; q++) if (act[q]) dot_add(S[q]->P.m, S[q]->pr, S[q]->pr, SC(q, 0));
+    if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+    for (int q = 0; q < ns; q++) {
+        if (!act[q]) continue;
+        Solve *s = S[q];
+        const double nr = sqrt(HS(q, 0));
+        tol[q] = fmax(tol_rel, s->f32 ? 1e-10 : 1e-13) * nb[q];
+        if (nr <= tol[q]) { act[q] = 0; relres[q] = nr / nb[q]; continue; }
+        best[q] = nr;
+        const int stag = s->reg == 0 && !s->f32;
+        maxit[q] = (s->reg > 0 || s->f32) ? 50 : 10;
+        sgain[q] = stag ? 0.5 : 0.9; smax[q] = stag ? 1 : 4;
+    }
+    {
+        int any = 0;
+        for (int q = 0; q < ns; q++) any |= act[q];
+        int *was = malloc(sizeof(int) * ns);
+        memcpy(was, act, sizeof(int) * ns);
+        if (any) {
+            /* xb = x; z = prec(r); p = z; rz = r.z */
+            LinL_clear(&g_ll);
+            for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->pxb, F_(S[q], fx), 1.0, -1, 0, -1, 0, -1, 0);
+            if (bk_lincomb(g_ll.v, g_ll.n)) { free(was); goto out; }
+            if (schur_prec(S, ns, act, OFF(pr), OFF(pz))) { free(was); goto out; }
+            LinL_clear(&g_ll); DotL_clear(&g_dl);
+            for (int q = 0; q < ns; q++) if (act[q]) { lin_add(S[q]->P.m, S[q]->pp, S[q]->pz, 1.0, -1, 0, -1, 0, -1, 0); dot_add(S[q]->P.m, S[q]->pr, S[q]->pz, SC(q, 0)); }
+            if (bk_lincomb(g_ll.v, g_ll.n) || bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) { free(was); goto out; }
+            for (int q = 0; q < ns; q++) if (act[q]) rz[q] = HS(q, 0);
+        }
+        for (int k = 0; any; k++) {
+            /* q = M p; pq = p.q */
+            if (schur_mv(S, ns, act, OFF(pp), OFF(pq), 1.0, 0.0)) { free(was); goto out; }
+            DotL_clear(&g_dl);
+            for (int q = 0; q < ns; q++) if (act[q]) dot_add(S[q]->P.m, S[q]->pp, S[q]->pq, SC(q, 0));
+            if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) { free(was); goto out; }
+            LinL_clear(&g_ll);
+            for (int q = 0; q < ns; q++) {
+                if (!act[q]) continue;
+                const double pq = HS(q, 0);
+                if (!(pq > 0)) { act[q] = 0; continue; }
+                const double a = rz[q] / pq;
+                lin_add(S[q]->P.m, F_(S[q], fx), F_(S[q], fx), 1.0, S[q]->pp, a, -1, 0, -1, 0);
+                lin_add(S[q]->P.m, S[q]->pr, S[q]->pr, 1.0, S[q]->pq, -a, -1, 0, -1, 0);
+                used[q]++;
+                S[q]->pcg_total++;
+            }
+            if (bk_lincomb(g_ll.v, g_ll.n)) { free(was); goto out; }
+            DotL_clear(&g_dl);
+            for (int q = 0; q < ns; q++) if (act[q]) dot_add(S[q]->P.m, S[q]->pr, S[q]->pr, SC(q, 0));
+            if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) { free(was); goto out; }
+            LinL_clear(&g_ll);
+            int *cont = calloc(ns, sizeof(int));
+            for (int q = 0; q < ns; q++) {
+                if (!act[q]) continue;
+                const double nr = sqrt(HS(q, 0));
+                if (nr < best[q]) {
+                    stall[q] = nr > sgain[q] * best[q] ? stall[q] + 1 : 0;
+                    best[q] = nr;
+                    lin_add(S[q]->P.m, S[q]->pxb, F_(S[q], fx), 1.0, -1, 0, -1, 0, -1, 0);
+                } else stall[q]++;
+                if (nr <= tol[q] || stall[q] >= smax[q] || used[q] >= maxit[q]) act[q] = 0;
+                else cont[q] = 1;
+            }
+            if (bk_lincomb(g_ll.v, g_ll.n)) { free(cont); free(was); goto out; }
+            any = 0;
+            for (int q = 0; q < ns; q++) any |= cont[q];
+            if (any) {
+                if (schur_prec(S, ns, cont, OFF(pr), OFF(pz))) { free(cont); free(was); goto out; }
+                DotL_clear(&g_dl);
+                for (int q = 0; q < ns; q++) if (cont[q]) dot_add(S[q]->P.m, S[q]->pr, S[q]->pz, SC(q, 0));
+                if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) { free(cont); free(was); goto out; }
+                LinL_clear(&g_ll);
+                for (int q = 0; q < ns; q++) {
+                    if (!cont[q]) continue;
+                    const double rz2 = HS(q, 0), beta = rz2 / rz[q];
+                    rz[q] = rz2;
+                    lin_add(S[q]->P.m, S[q]->pp, S[q]->pz, 1.0, S[q]->pp, beta, -1, 0, -1, 0);
+                }
+                if (bk_lincomb(g_ll.v, g_ll.n)) { free(cont); free(was); goto out; }
+            }
+            free(cont);
+        }
+        /* x = xb; true residual */
+        any = 0;
+        for (int q = 0; q < ns; q++) any |= was[q];
+        if (any) {
+            LinL_clear(&g_ll);
+            for (int q = 0; q < ns; q++) if (was[q]) {
+                lin_add(S[q]->P.m, F_(S[q], fx), S[q]->pxb, 1.0, -1, 0, -1, 0, -1, 0);
+                lin_add(S[q]->P.m, S[q]->pr, F_(S[q], frhs), 1.0, -1, 0, -1, 0, -1, 0);
+            }
+            if (bk_lincomb(g_ll.v, g_ll.n)) { free(was); goto out; }
+            if (schur_mv(S, ns, was, fx, OFF(pr), -1.0, 1.0)) { free(was); goto out; }
+            DotL_clear(&g_dl);
+            for (int q = 0; q < ns; q++) if (was[q]) dot_add(S[q]->P.m, S[q]->pr, S[q]->pr, SC(q, 0));
+            if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) { free(was); goto out; }
+            for (int q = 0; q < ns; q++) {
+                if (!was[q]) continue;
+                const double rel = sqrt(HS(q, 0)) / nb[q];
+                relres[q] = rel;
+                Solve *s = S[q];
+                if (s->f32 && (used[q] > 4 || best[q] > tol[q] || rel > fmax(1e-9, 10 * tol_rel))) s->want_double = 1;
+            }
+        }
+        free(was);
+    }
+    rc = 0;
+out:
+    free(act); free(used); free(stall); free(maxit); free(nb); free(tol); free(best); free(rz); free(sgain); free(smax);
+    return rc;
+}
+
+/* ---- start point (reference solver.c:13940-14210) ----------------------------------------- */
+static int set_identity(Solve **S, int ns, const int *act, size_t f) {
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (!act || act[q]) lin_add(S[q]->P.T, F_(S[q], f), -1, 0, -1, 0, -1, 0, -1, 0);
+    if (bk_lincomb(g_ll.v, g_ll.n)) return -1;
+    MItL_clear(&g_ml);
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        for (int k = 0; k < S[q]->P.nb; k++) { MItem it = { S[q]->P.bn[k], q, F_(S[q], f) + S[q]->P.boff[k], 0, 0 }; MItL_add(&g_ml, it); }
+    }
+    for (int q = 0; q < ns; q++) g_hdbl[q] = 1.0;
+    if (bk_h2d(ddbl(), g_hdbl, sizeof(double) * ns)) return -1;
+    return bk_add_diag(g_ml.v, g_ml.n, g_bdbl / 8);
+}
+/* lambda_min of V per solve (Lanczos on V itself, conservative: theta - residual bound) */
+static int lammin_all(Solve **S, int ns, const int *act, size_t fV, double *lam) {
+    /* W2 = V; Lanczos as in maxstep without the triangular products */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (!act || act[q]) lin_add(S[q]->P.T, S[q]->W2, F_(S[q], fV), 1.0, -1, 0, -1, 0, -1, 0);
+    if (bk_lincomb(g_ll.v, g_ll.n)) return -1;
+    LzL_clear(&g_zl);
+    int kmax = 0;
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        SAux *A = aux_of(S[q]);
+        for (int k = 0; k < S[q]->P.nb; k++) {
+            const int n = S[q]->P.bn[k], K = n < BK_LZ_K ? n : BK_LZ_K;
+            LzItem z = { n, K, S[q]->lzQ + (int64_t)BK_LZ_K * A->lzoff[k], S[q]->lzw + A->lzoff[k], S[q]->lzab + 64 * k };
+            LzL_add(&g_zl, z);
+            if (K > kmax) kmax = K;
+        }
+    }
+    if (bk_lanczos_start(g_zl.v, g_zl.n)) return -1;
+    double *H = heapp();
+    for (int j = 0; j < kmax; j++) {
+        GemvL_clear(&g_vl);
+        int c = 0;
+        for (int q = 0; q < ns; q++) {
+            if (act && !act[q]) continue;
+            for (int k = 0; k < S[q]->P.nb; k++, c++) {
+                const LzItem *z = &g_zl.v[c];
+                if (j >= z->K) continue;
+                GemvItem g = { z->n, z->n, z->n, S[q]->W2 + S[q]->P.boff[k], z->Q + (int64_t)j * z->n, z->w };
+                GemvL_add(&g_vl, g);
+            }
+        }
+        if (bk_gemv_f64(1, 1.0, 0.0, H, H, H, g_vl.v, g_vl.n)) return -1;
+        if (bk_lanczos_step(g_zl.v, g_zl.n, j)) return -1;
+    }
+    int c = 0;
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        const HProb *P = &S[q]->P;
+        double *ab = malloc(sizeof(double) * 64 * P->nb);
+        if (bk_d2h(ab, H + S[q]->lzab, sizeof(double) * 64 * P->nb)) { free(ab); return -1; }
+        double lmin = INFINITY;
+        for (int k = 0; k < P->nb; k++, c++) {
+            const int K = g_zl.v[c].K;
+            const double *al = ab + 64 * k, *be = al + 32;
+            int kk = K;
+            double scale = 0;
+            for (int j = 0; j < K; j++) {
+                if (fabs(al[j]) > scale) scale = fabs(al[j]);
+                if (!(be[j] > 1e-12 * (scale + 1e-300))) { kk = j + 1; break; }
+            }
+            double th, zl;
+            tri_min(kk, al, be, &th, &zl);
+            const double res = (kk == K && kk < P->bn[k]) ? fabs(be[kk - 1] * zl) : 0.0;
+            const double l = th - res;
+            if (l < lmin) lmin = l;
+        }
+        free(ab);
+        lam[q] = lmin;
+    }
+    return 0;
+}
+
+int ipm_start_batch(Solve **S, int ns) {
+    int rc = -1;
+    int *fail = calloc(ns, sizeof(int)), *ok = calloc(ns, sizeof(int)), *act = calloc(ns, sizeof(int));
+    double *rel = calloc(ns, sizeof(double)), *lx = calloc(ns, sizeof(double)), *lz = calloc(ns, sizeof(double));
+    double *extra = calloc(ns, sizeof(double));
+    int nsl = 1;
+    for (int q = 0; q < ns; q++) nsl += S[q]->P.nb;
+    if (batch_scratch(ns, nsl + ns)) goto out;
+    for (int q = 0; q < ns; q++) {
+        Solve *s = S[q];
+        s->t_start = bk_now();
+        s->status = -1; s->done = 0; s->it = 0; s->stall = 0; s->it_best = 0;
+        s->best_score = 1e300; s->theta = 1.0; s->alp = s->ald = 0;
+        s->want_double = 0; s->reg_hint = 0; s->reg_ok = 0; s->reg = 0; s->f32 = 0;
+        s->pcg_total = 0; s->nfactor_f32 = s->nfactor_f64 = 0;
+    }
+    /* X = I, Zi = I; rhs = A(C); M = A A' (route at X = Zi = I) */
+    if (set_identity(S, ns, NULL, OFF(X)) || set_identity(S, ns, NULL, OFF(Zi))) goto out;
+    if (aop(S, ns, NULL, OFF(C), OFF(rhs), 1.0, 0.0)) goto out;
+    if (schur_assemble(S, ns, NULL)) goto out;
+    if (schur_factor(S, ns, NULL, 1, fail)) goto out;
+    for (int q = 0; q < ns; q++) if (fail[q]) { S[q]->status = 4; S[q]->done = 1; }
+    for (int q = 0; q < ns; q++) act[q] = !S[q]->done;
+    if (schur_solve(S, ns, act, OFF(bvec), OFF(dy), 1e-8, rel)) goto out;
+    if (schur_solve(S, ns, act, OFF(rhs), OFF(y), 1e-8, rel)) goto out;
+    /* X = A*(dy); Z = C - A*(y) */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        lin_add(S[q]->P.T, S[q]->X, -1, 0, -1, 0, -1, 0, -1, 0);
+        lin_add(S[q]->P.T, S[q]->Z, S[q]->C, 1.0, -1, 0, -1, 0, -1, 0);
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (aty(S, ns, act, OFF(dy), OFF(X), 1.0) || aty(S, ns, act, OFF(y), OFF(Z), -1.0)) goto out;
+    if (lammin_all(S, ns, act, OFF(X), lx) || lammin_all(S, ns, act, OFF(Z), lz)) goto out;
+    MItL_clear(&g_ml);
+    for (int q = 0; q < ns; q++) {
+        if (!act[q]) continue;
+        const double ax = -lx[q], az = -lz[q];
+        const double shx = ax >= -1e-8 ? 1.0 + ax : 0.0;
+        const double shz = fmax(0.0, 1.0 + az);
+        g_hdbl[2 * q] = shx; g_hdbl[2 * q + 1] = shz;
+        for (int k = 0; k < S[q]->P.nb; k++) {
+            MItem a = { S[q]->P.bn[k], 2 * q, S[q]->X + S[q]->P.boff[k], 0, 0 }; MItL_add(&g_ml, a);
+            MItem b = { S[q]->P.bn[k], 2 * q + 1, S[q]->Z + S[q]->P.boff[k], 0, 0 }; MItL_add(&g_ml, b);
+        }
+    }
+    if (bk_h2d(ddbl(), g_hdbl, sizeof(double) * 2 * ns)) goto out;
+    if (bk_add_diag(g_ml.v, g_ml.n, g_bdbl / 8)) goto out;
+    /* Cholesky of X and Z; a block that fails gets 1e-8, 1e-7, ... added to its diagonal */
+    for (int side = 0; side < 2; side++) {
+        const size_t fV = side ? OFF(Z) : OFF(X), fL = side ? OFF(LZ) : OFF(LX), fD = side ? OFF(DinvZ) : OFF(DinvX);
+        int *need = calloc(ns, sizeof(int));
+        for (int q = 0; q < ns; q++) { need[q] = act[q]; extra[q] = 1e-8; }
+        for (int tries = 0; tries < 30; tries++) {
+            int any = 0;
+            for (int q = 0; q < ns; q++) any |= need[q];
+            if (!any) break;
+            if (blk_chol(S, ns, need, fV, fL, fD, ok)) { free(need); goto out; }
+            MItL_clear(&g_ml);
+            for (int q = 0; q < ns; q++) {
+                if (!need[q]) continue;
+                if (ok[q]) { need[q] = 0; continue; }
+                g_hdbl[q] = extra[q]; extra[q] *= 10;
+                for (int k = 0; k < S[q]->P.nb; k++) { MItem a = { S[q]->P.bn[k], q, F_(S[q], fV) + S[q]->P.boff[k], 0, 0 }; MItL_add(&g_ml, a); }
+            }
+            if (g_ml.n) {
+                if (bk_h2d(ddbl(), g_hdbl, sizeof(double) * ns) || bk_add_diag(g_ml.v, g_ml.n, g_bdbl / 8)) { free(need); goto out; }
+            }
+        }
+        for (int q = 0; q < ns; q++) if (need[q]) { S[q]->status = 4; S[q]->done = 1; act[q] = 0; }
+        free(need);
+    }
+    if (blk_inv(S, ns, act, OFF(LX), OFF(DinvX), OFF(LXi)) || blk_inv(S, ns, act, OFF(LZ), OFF(DinvZ), OFF(LZi))) goto out;
+    /* R0 = C - Z - A*(y); theta = 1 */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.T, S[q]->R0, S[q]->C, 1.0, S[q]->Z, -1.0, -1, 0, -1, 0);
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (aty(S, ns, act, OFF(y), OFF(R0), -1.0)) goto out;
+    DotL_clear(&g_dl);
+    for (int q = 0; q < ns; q++) if (act[q]) dot_add(S[q]->P.T, S[q]->R0, S[q]->R0, SC(q, 0));
+    if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+    for (int q = 0; q < ns; q++) if (act[q]) { S[q]->r0norm = sqrt(HS(q, 0)); S[q]->theta = 1.0; }
+    /* best iterate starts at the start point */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        lin_add(S[q]->P.T, S[q]->Xb, S[q]->X, 1.0, -1, 0, -1, 0, -1, 0);
+        lin_add(S[q]->P.m, S[q]->yb, S[q]->y, 1.0, -1, 0, -1, 0, -1, 0);
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    rc = 0;
+out:
+    free(fail); free(ok); free(act); free(rel); free(lx); free(lz); free(extra);
+    return rc;
+}
+
+/* ---- one iteration (reference solver.c:14394-15189) --------------------------------------- */
+int ipm_iterate_batch(Solve **S, int ns) {
+    int rc = -1;
+    int *act = calloc(ns, sizeof(int)), *fail = calloc(ns, sizeof(int)), *okp = calloc(ns, sizeof(int)), *okd = calloc(ns, sizeof(int));
+    int *hasH = calloc(ns, sizeof(int)), *ract = calloc(ns, sizeof(int)), *tact = calloc(ns, sizeof(int));
+    double *rel = calloc(ns, sizeof(double)), *ap = calloc(ns, sizeof(double)), *ad = calloc(ns, sizeof(double));
+    double *smu = calloc(ns, sizeof(double)), *gam = calloc(ns, sizeof(double)), *apr = calloc(ns, sizeof(double)), *adr = calloc(ns, sizeof(double));
+    double *ne = calloc(ns, sizeof(double)), *nrp = calloc(ns, sizeof(double)), *stepped = calloc(ns, sizeof(double));
+    double *hrp = NULL, *hy = NULL;
+    int mmax = 1;
+    for (int q = 0; q < ns; q++) { act[q] = !S[q]->done; if (S[q]->P.m > mmax) mmax = S[q]->P.m; }
+    hrp = malloc(sizeof(double) * mmax); hy = malloc(sizeof(double) * mmax);
+    int nsl = 1;
+    for (int q = 0; q < ns; q++) nsl += S[q]->P.nb;
+    if (batch_scratch(ns, nsl + ns)) goto out;
+    double *H = heapp();
+
+    /* Zi = LZi' LZi (K range trimmed to k >= max(i, j)), symmetrised */
+    if (blk_gemm(S, ns, act, 1, 0, 2 | 8, 1.0, 0.0, OFF(LZi), OFF(LZi), OFF(Zi))) goto out;
+    if (blk_sym(S, ns, act, OFF(Zi))) goto out;
+    /* residuals and objectives (solver.c:14426-14482) */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        lin_add(S[q]->P.m, S[q]->rp, S[q]->bvec, 1.0, -1, 0, -1, 0, -1, 0);
+        lin_add(S[q]->P.T, S[q]->F, S[q]->Z, 1.0, S[q]->C, -1.0, -1, 0, -1, 0);
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (aop(S, ns, act, OFF(X), OFF(rp), -1.0, 1.0)) goto out;
+    if (aty(S, ns, act, OFF(y), OFF(F), 1.0)) goto out;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        lin_add(S[q]->P.T, S[q]->W1, S[q]->F, 1.0, S[q]->C, 1.0, -1, 0, -1, 0);                 /* A'y + Z */
+        lin_add(S[q]->P.T, S[q]->W2, S[q]->F, 1.0, S[q]->R0, S[q]->theta, -1, 0, -1, 0);        /* F + theta R0 */
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    DotL_clear(&g_dl);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        Solve *s = S[q];
+        const int64_t T = s->P.T;
+        dot_add(T, s->W1, s->W1, SC(q, 0));
+        dot_add(T, s->C, s->X, SC(q, 1));
+        dot_add(T, s->X, s->Z, SC(q, 2));
+        dot_add(T, s->F, s->F, SC(q, 3));
+        dot_add(T, s->W2, s->W2, SC(q, 4));
+    }
+    if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+    for (int q = 0; q < ns; q++) {
+        if (!act[q]) continue;
+        Solve *s = S[q];
+        const HProb *P = &s->P;
+        const int m = P->m;
+        if (bk_d2h(hrp, H + s->rp, sizeof(double) * m) || bk_d2h(hy, H + s->y, sizeof(double) * m)) goto out;
+        const double ncr2 = HS(q, 0), pobj = HS(q, 1), xz = HS(q, 2), rdn2 = HS(q, 3), dev2 = HS(q, 4);
+        double dobj = 0, rpo = 0, nAX = 0;
+        for (int i = 0; i < m; i++) {
+            dobj += P->b[i] * hy[i];
+            const double v = hrp[i] * P->du[i]; rpo += v * v;
+            const double w = P->b[i] - hrp[i]; nAX += w * w;
+        }
+        rpo = sqrt(rpo); nAX = sqrt(nAX);
+        const double mu = xz / P->Ntot;
+        const double rdo = P->cs * sqrt(rdn2), sc = P->bs * P->cs;
+        const double po = sc * pobj, dob = sc * dobj;
+        s->pinf = rpo / (1 + P->normb2);
+        s->dinf = rdo / (1 + P->normC2);
+        const double oden = 1 + fabs(po) + fabs(dob);
+        s->relgap = fabs(po - dob) / oden;
+        s->relcomp = sc * xz / oden;
+        s->pobj = po; s->dobj = dob; s->mu = mu;
+        double score = fmax(fmax(s->relgap, s->relcomp), fmax(s->pinf, s->dinf));
+        if (!isfinite(score)) score = INFINITY;
+        s->score = score;
+        const int it = s->it;
+        if (it < 128) { s->tr_pobj[it] = po; s->tr_dobj[it] = dob; s->tr_mu[it] = sc * mu; }
+        if (s->o.verbose)
+            solve_log("%4d %+.10e %+.10e %.1e %.1e %.1e %.1e %.3f %.3f %s %.3fs\n", it, po, dob, s->pinf, s->dinf, s->relgap, sc * mu,
+                      s->alp, s->ald, s->f32 ? "f32" : "f64", bk_now() - s->t_start);
+        int st = -1;
+        if (!isfinite(po) || !isfinite(dob)) st = 4;
+        else {
+            if (score < s->best_score) {
+                s->best_score = score; s->it_best = it;
+                LinL_clear(&g_ll);
+                lin_add(P->T, s->Xb, s->X, 1.0, -1, 0, -1, 0, -1, 0);
+                lin_add(m, s->yb, s->y, 1.0, -1, 0, -1, 0, -1, 0);
+                if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+            }
+            if (score < s->o.tol) st = 0;
+            else if (dobj > 0 && sqrt(ncr2) / dobj < 1e-8 && it > 5) st = 1;
+            else if (pobj < 0 && nAX / (-pobj) < 1e-8 && it > 5) st = 2;
+            else if (it >= s->o.maxit) st = 3;
+            else if (s->stall >= 3) st = 4;
+            else if (it - s->it_best >= 5 && s->best_score < 1e-6) st = 4;
+            else if (it - s->it_best >= 8 && s->best_score < 1e-4) st = 4;
+            else if (it - s->it_best >= 12) st = 4;
+            else if (s->o.time_limit > 0 && bk_now() - s->t_start > s->o.time_limit) st = 6;
+        }
+        if (st >= 0) { s->status = st; s->done = 1; act[q] = 0; continue; }
+        /* re-base the dual residual (solver.c:14614-14636) */
+        const double dev = sqrt(dev2);
+        if (dev > 1e-10 * (1.0 + s->theta * s->r0norm) && dev > 1e-12) {
+            s->theta = 1.0; s->r0norm = sqrt(rdn2);
+            LinL_clear(&g_ll);
+            lin_add(P->T, s->R0, s->F, -1.0, -1, 0, -1, 0, -1, 0);
+            if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+        }
+        if (s->theta * s->r0norm <= 1e-13) s->theta = 0.0;
+        hasH[q] = s->theta > 0;
+        /* mixed precision: the f32 factor only far from the optimum */
+        if (s->o.prec == PREC_MIXED && fmax(s->relgap, fmax(s->pinf, s->dinf)) <= s->o.mixed_until) s->want_double = 1;
+    }
+    {
+        int any = 0;
+        for (int q = 0; q < ns; q++) any |= act[q];
+        if (!any) { rc = 0; goto out; }
+    }
+    /* Schur complement and its factor */
+    if (schur_assemble(S, ns, act)) goto out;
+    if (schur_factor(S, ns, act, 0, fail)) goto out;
+    for (int q = 0; q < ns; q++) if (act[q] && fail[q]) { S[q]->status = 4; S[q]->done = 1; act[q] = 0; }
+    /* a0 = A(Zi); h = A(Zi theta R0 X) */
+    if (aop(S, ns, act, OFF(Zi), OFF(a0), 1.0, 0.0)) goto out;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) {
+        if (!act[q]) continue;
+        if (hasH[q]) lin_add(S[q]->P.T, S[q]->dZ, S[q]->R0, S[q]->theta, -1, 0, -1, 0, -1, 0);
+        else lin_add(S[q]->P.m, S[q]->h, -1, 0, -1, 0, -1, 0, -1, 0);
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (blk_gemm(S, ns, hasH, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(dZ), OFF(W1))) goto out;
+    if (blk_gemm(S, ns, hasH, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G))) goto out;
+    if (aop(S, ns, hasH, OFF(G), OFF(h), 1.0, 0.0)) goto out;
+    /* predictor: M dy = b + h (tol 1e-7); dZ = theta R0 - A*(dy); dX = -X - sym(Zi dZ X) */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        lin_add(S[q]->P.m, S[q]->rhs, S[q]->bvec, 1.0, S[q]->h, 1.0, -1, 0, -1, 0);
+        lin_add(S[q]->P.T, S[q]->dZ, S[q]->R0, S[q]->theta, -1, 0, -1, 0, -1, 0);
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (schur_solve(S, ns, act, OFF(rhs), OFF(dy), 1e-7, rel)) goto out;
+    if (aty(S, ns, act, OFF(dy), OFF(dZ), -1.0)) goto out;
+    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(dZ), OFF(W1))) goto out;
+    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G))) goto out;
+    if (blk_sym(S, ns, act, OFF(G))) goto out;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.T, S[q]->dX, S[q]->G, -1.0, S[q]->X, -1.0, -1, 0, -1, 0);
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (maxstep(S, ns, act, OFF(LXi), OFF(dX), ap) || maxstep(S, ns, act, OFF(LZi), OFF(dZ), ad)) goto out;
+    DotL_clear(&g_dl);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        Solve *s = S[q];
+        dot_add(s->P.T, s->X, s->dZ, SC(q, 0));
+        dot_add(s->P.T, s->dX, s->Z, SC(q, 1));
+        dot_add(s->P.T, s->dX, s->dZ, SC(q, 2));
+        dot_add(s->P.T, s->X, s->Z, SC(q, 3));
+    }
+    if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+    for (int q = 0; q < ns; q++) {
+        if (!act[q]) continue;
+        Solve *s = S[q];
+        const double a_p = fmin(1.0, ap[q]), a_d = fmin(1.0, ad[q]);
+        const double xdz = HS(q, 0), dxz = HS(q, 1), dxdz = HS(q, 2), xz = HS(q, 3);
+        const double mu = xz / s->P.Ntot;
+        const double mu_a = (xz + a_p * dxz + a_d * xdz + a_p * a_d * dxdz) / s->P.Ntot;
+        const double frac = fmax(0.0, mu_a / mu), mina = fmin(a_p, a_d);
+        const double expon = !s->P.any_sdp ? 3.0 : (s->relgap > 1e-6 ? fmax(1.0, 3 * mina * mina) : 3.0);
+        const double sigma = fmin(1.0, pow(frac, expon));
+        gam[q] = 0.9 + (0.99 - 0.9) * mina;
+        smu[q] = sigma * mu;
+    }
+    /* corrector (solver.c:14997-15055): Q = Zi dZ dX; rhs = b - smu a0 + A(Q) + h */
+    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(dZ), OFF(W1))) goto out;
+    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(dX), OFF(Q))) goto out;
+    if (aop(S, ns, act, OFF(Q), OFF(qv), 1.0, 0.0)) goto out;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        lin_add(S[q]->P.m, S[q]->rhs, S[q]->bvec, 1.0, S[q]->a0, -smu[q], S[q]->qv, 1.0, S[q]->h, 1.0);
+        lin_add(S[q]->P.T, S[q]->dZ, S[q]->R0, S[q]->theta, -1, 0, -1, 0, -1, 0);
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (schur_solve(S, ns, act, OFF(rhs), OFF(dy), 0.0, rel)) goto out;
+    if (aty(S, ns, act, OFF(dy), OFF(dZ), -1.0)) goto out;
+    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(dZ), OFF(W1))) goto out;
+    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G))) goto out;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.T, S[q]->dX, S[q]->G, -1.0, S[q]->Q, -1.0, -1, 0, -1, 0);
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (blk_sym(S, ns, act, OFF(dX))) goto out;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.T, S[q]->dX, S[q]->dX, 1.0, S[q]->Zi, smu[q], S[q]->X, -1.0, -1, 0);
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    /* refinement enforcing A(dX) = rp (solver.c:15058-15105) */
+    for (int q = 0; q < ns; q++) ract[q] = act[q];
+    for (int pass = 0; pass < 3; pass++) {
+        int any = 0;
+        for (int q = 0; q < ns; q++) any |= ract[q];
+        if (!any) break;
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) if (ract[q]) lin_add(S[q]->P.m, S[q]->ev, S[q]->rp, 1.0, -1, 0, -1, 0, -1, 0);
+        if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+        if (aop(S, ns, ract, OFF(dX), OFF(ev), -1.0, 1.0)) goto out;
+        DotL_clear(&g_dl);
+        for (int q = 0; q < ns; q++) if (ract[q]) { dot_add(S[q]->P.m, S[q]->ev, S[q]->ev, SC(q, 0)); dot_add(S[q]->P.m, S[q]->rp, S[q]->rp, SC(q, 1)); }
+        if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+        for (int q = 0; q < ns; q++) {
+            if (!ract[q]) continue;
+            ne[q] = sqrt(HS(q, 0)); nrp[q] = sqrt(HS(q, 1));
+            if (!(ne[q] > 1e-2 * nrp[q] && ne[q] > 1e-3 * S[q]->o.tol * (1 + S[q]->P.nbs))) ract[q] = 0;
+        }
+        any = 0;
+        for (int q = 0; q < ns; q++) any |= ract[q];
+        if (!any) break;
+        if (schur_solve(S, ns, ract, OFF(ev), OFF(wv), 0.0, rel)) goto out;
+        /* Q = A*(wv); G = sym(Zi Q X); qv = ev - A(G) */
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) if (ract[q]) {
+            lin_add(S[q]->P.T, S[q]->Q, -1, 0, -1, 0, -1, 0, -1, 0);
+            lin_add(S[q]->P.m, S[q]->qv, S[q]->ev, 1.0, -1, 0, -1, 0, -1, 0);
+        }
+        if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+        if (aty(S, ns, ract, OFF(wv), OFF(Q), 1.0)) goto out;
+        if (blk_gemm(S, ns, ract, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(Q), OFF(W1))) goto out;
+        if (blk_gemm(S, ns, ract, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G))) goto out;
+        if (blk_sym(S, ns, ract, OFF(G))) goto out;
+        if (aop(S, ns, ract, OFF(G), OFF(qv), -1.0, 1.0)) goto out;
+        DotL_clear(&g_dl);
+        for (int q = 0; q < ns; q++) if (ract[q]) dot_add(S[q]->P.m, S[q]->qv, S[q]->qv, SC(q, 0));
+        if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) {
+            if (!ract[q]) continue;
+            const double ne2 = sqrt(HS(q, 0));
+            if (!(ne2 < 0.5 * ne[q])) { ract[q] = 0; continue; }
+            Solve *s = S[q];
+            lin_add(s->P.m, s->dy, s->dy, 1.0, s->wv, 1.0, -1, 0, -1, 0);
+            lin_add(s->P.T, s->dZ, s->dZ, 1.0, s->Q, -1.0, -1, 0, -1, 0);
+            lin_add(s->P.T, s->dX, s->dX, 1.0, s->G, 1.0, -1, 0, -1, 0);
+        }
+        if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    }
+    /* step lengths (solver.c:15108-15130) */
+    if (maxstep(S, ns, act, OFF(LXi), OFF(dX), apr) || maxstep(S, ns, act, OFF(LZi), OFF(dZ), adr)) goto out;
+    for (int q = 0; q < ns; q++) if (act[q]) { ap[q] = fmin(1.0, gam[q] * fmin(1.0, apr[q])); ad[q] = fmin(1.0, gam[q] * fmin(1.0, adr[q])); }
+    /* catastrophe guard (solver.c:15143-15160) */
+    {
+        int *g = calloc(ns, sizeof(int)), any = 0;
+        for (int q = 0; q < ns; q++) if (act[q] && S[q]->pinf < 1e-6) { g[q] = 1; any = 1; }
+        if (any) {
+            LinL_clear(&g_ll);
+            for (int q = 0; q < ns; q++) if (g[q]) lin_add(S[q]->P.m, S[q]->ev, S[q]->rp, 1.0, -1, 0, -1, 0, -1, 0);
+            if (bk_lincomb(g_ll.v, g_ll.n) || aop(S, ns, g, OFF(dX), OFF(ev), -1.0, 1.0)) { free(g); goto out; }
+            DotL_clear(&g_dl);
+            for (int q = 0; q < ns; q++) if (g[q]) { dot_add(S[q]->P.m, S[q]->ev, S[q]->ev, SC(q, 0)); dot_add(S[q]->P.m, S[q]->rp, S[q]->rp, SC(q, 1)); }
+            if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) { free(g); goto out; }
+            for (int q = 0; q < ns; q++) {
+                if (!g[q]) continue;
+                const double nev = sqrt(HS(q, 0)), nr = sqrt(HS(q, 1));
+                const double gg = fmax(1e3, 1e-6 / fmax(S[q]->pinf, 1e-300)), lim = gg * nr;
+                if (nev > lim && nr > 0) { const double cap = lim / nev; if (cap < ap[q]) ap[q] = cap; if (S[q]->o.prec == PREC_MIXED) S[q]->want_double = 1; }
+            }
+        }
+        free(g);
+    }
+    /* try_step: V += (a - stepped) dV, L = chol(V); x0.8 on failure (solver.c:6393-6446) */
+    for (int side = 0; side < 2; side++) {
+        double *a = side ? ad : ap;
+        int *okv = side ? okd : okp;
+        const size_t fV = side ? OFF(Z) : OFF(X), fdV = side ? OFF(dZ) : OFF(dX), fL = side ? OFF(LZ) : OFF(LX), fD = side ? OFF(DinvZ) : OFF(DinvX);
+        for (int q = 0; q < ns; q++) { tact[q] = act[q]; stepped[q] = 0; okv[q] = 0; }
+        for (int tries = 0; tries < 40; tries++) {
+            int any = 0;
+            for (int q = 0; q < ns; q++) any |= tact[q];
+            if (!any) break;
+            LinL_clear(&g_ll);
+            for (int q = 0; q < ns; q++) {
+                if (!tact[q]) continue;
+                const double inc = a[q] - stepped[q];
+                stepped[q] = a[q];
+                lin_add(S[q]->P.T, F_(S[q], fV), F_(S[q], fV), 1.0, F_(S[q], fdV), inc, -1, 0, -1, 0);
+            }
+            if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+            int *ok = calloc(ns, sizeof(int));
+            if (blk_chol(S, ns, tact, fV, fL, fD, ok)) { free(ok); goto out; }
+            for (int q = 0; q < ns; q++) {
+                if (!tact[q]) continue;
+                if (ok[q]) { tact[q] = 0; okv[q] = 1; }
+                else a[q] *= 0.8;
+            }
+            free(ok);
+        }
+        if (blk_inv(S, ns, okv, fL, fD, side ? OFF(LZi) : OFF(LXi))) goto out;
+    }
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) {
+        if (!act[q]) continue;
+        Solve *s = S[q];
+        if (!okp[q] || !okd[q]) { s->status = 4; s->done = 1; continue; }
+        lin_add(s->P.m, s->y, s->y, 1.0, s->dy, ad[q], -1, 0, -1, 0);
+        s->theta *= (1.0 - ad[q]);
+        s->alp = ap[q]; s->ald = ad[q];
+        s->stall = (ap[q] < 1e-6 && ad[q] < 1e-6) ? s->stall + 1 : 0;
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    for (int q = 0; q < ns; q++) if (act[q] && !S[q]->done) S[q]->it++;
+    rc = 0;
+out:
+    free(act); free(fail); free(okp); free(okd); free(hasH); free(ract); free(tact); free(rel); free(ap); free(ad); free(smu); free(gam);
+    free(apr); free(adr); free(ne); free(nrp); free(stepped); free(hrp); free(hy);
+    return rc;
+}
+
+/* ---- DIMACS errors of the returned point on the data as read (postsolve.c:3037-3112), FP64 on
+ * the device. With X = bs X~, y_i = cs d_i y~_i and Z = C - A'y = cs (C~ - A~'y~):
+ *   A(X) - b = du o (A~(X~) - b~), <C,X> = bs cs <C~,X~>, b'y = bs cs b~'y~, <X,Z> = bs cs <X~, Z~>.
+ * lambda_min(X) and lambda_min(Z) are tested by Cholesky: a factor that exists proves the block
+ * positive definite (error 0); otherwise the Lanczos bound gives the violation. */
+static int ipm_dimacs(Solve **S, int ns) {
+    int rc = -1;
+    int *act = calloc(ns, sizeof(int)), *okx = calloc(ns, sizeof(int)), *okz = calloc(ns, sizeof(int));
+    double *lam = calloc(ns, sizeof(double));
+    int mmax = 1, nsl = 1;
+    for (int q = 0; q < ns; q++) { act[q] = 1; if (S[q]->P.m > mmax) mmax = S[q]->P.m; nsl += S[q]->P.nb; }
+    double *hrp = malloc(sizeof(double) * mmax);
+    if (batch_scratch(ns, nsl + ns)) goto out;
+    double *H = heapp();
+    /* the returned point into X, y (the best iterate unless the run ended optimal) */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (S[q]->status != 0) {
+        lin_add(S[q]->P.T, S[q]->X, S[q]->Xb, 1.0, -1, 0, -1, 0, -1, 0);
+        lin_add(S[q]->P.m, S[q]->y, S[q]->yb, 1.0, -1, 0, -1, 0, -1, 0);
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    /* rp = A(X) - b (scaled); W1 = C - A'y (scaled Z on the data) */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) {
+        lin_add(S[q]->P.m, S[q]->rp, S[q]->bvec, -1.0, -1, 0, -1, 0, -1, 0);
+        lin_add(S[q]->P.T, S[q]->W1, S[q]->C, 1.0, -1, 0, -1, 0, -1, 0);
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (aop(S, ns, act, OFF(X), OFF(rp), 1.0, 1.0) || aty(S, ns, act, OFF(y), OFF(W1), -1.0)) goto out;
+    DotL_clear(&g_dl);
+    for (int q = 0; q < ns; q++) {
+        Solve *s = S[q];
+        dot_add(s->P.T, s->C, s->X, SC(q, 0));
+        dot_add(s->P.T, s->X, s->W1, SC(q, 1));
+        dot_add(s->P.m, s->bvec, s->y, SC(q, 2));
+        /* largest block Frobenius norm of Z: per block */
+        for (int k = 0; k < s->P.nb && k < NSC - 8; k++)
+            if (s->P.bn[k] > 1) dot_add((int64_t)s->P.bn[k] * s->P.bn[k], s->W1 + s->P.boff[k], s->W1 + s->P.boff[k], SC(q, 8 + k));
+    }
+    if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+    for (int q = 0; q < ns; q++) {
+        Solve *s = S[q];
+        const HProb *P = &s->P;
+        const double sc = P->bs * P->cs;
+        s->res_pobj = sc * HS(q, 0);
+        s->res_dobj = sc * HS(q, 2);
+        const double xz = sc * HS(q, 1);
+        if (bk_d2h(hrp, H + s->rp, sizeof(double) * P->m)) goto out;
+        double r2 = 0;
+        for (int i = 0; i < P->m; i++) { const double v = hrp[i] * P->du[i]; r2 += v * v; }
+        double zres = 0;
+        for (int k = 0; k < P->nb && k < NSC - 8; k++) if (P->bn[k] > 1) zres = fmax(zres, 2.220446049250313e-16 * P->cs * sqrt(HS(q, 8 + k)));
+        const double den = 1 + fabs(s->res_pobj) + fabs(s->res_dobj);
+        s->res_err[0] = sqrt(r2) / (1 + P->normb1);
+        s->res_err[1] = 0; s->res_err[2] = 0;
+        s->res_err[3] = zres > 1e4 * 2.220446049250313e-16 * (1 + P->normC1) ? zres / (1 + P->normC1) : 0;
+        s->res_err[4] = (s->res_pobj - s->res_dobj) / den;
+        s->res_err[5] = xz / den;
+    }
+    /* cone membership: X (its factor is recomputed; the iterate's may be stale for the best point)
+     * and Z on the data as read (W1) */
+    if (blk_chol(S, ns, act, OFF(X), OFF(LX), OFF(DinvX), okx)) goto out;
+    if (blk_chol(S, ns, act, OFF(W1), OFF(LZ), OFF(DinvZ), okz)) goto out;
+    {
+        int *need = calloc(ns, sizeof(int)), any = 0;
+        for (int q = 0; q < ns; q++) if (!okx[q]) { need[q] = 1; any = 1; }
+        if (any) {
+            if (lammin_all(S, ns, need, OFF(X), lam)) { free(need); goto out; }
+            for (int q = 0; q < ns; q++) if (need[q]) S[q]->res_err[1] = fmax(0.0, -S[q]->P.bs * lam[q]) / (1 + S[q]->P.normb1);
+        }
+        any = 0;
+        for (int q = 0; q < ns; q++) { need[q] = !okz[q]; any |= need[q]; }
+        if (any) {
+            if (lammin_all(S, ns, need, OFF(W1), lam)) { free(need); goto out; }
+            for (int q = 0; q < ns; q++) if (need[q]) S[q]->res_err[3] = fmax(S[q]->res_err[3], fmax(0.0, -S[q]->P.cs * lam[q]) / (1 + S[q]->P.normC1));
+        }
+        free(need);
+    }
+    /* status from the measured errors (main.c:2349-2359, brisk.h:187-189) */
+    for (int q = 0; q < ns; q++) {
+        Solve *s = S[q];
+        double acc = fmax(fmax(s->res_err[0], s->res_err[1]), fmax(s->res_err[3], fmax(fabs(s->res_err[4]), fabs(s->res_err[5]))));
+        if (!isfinite(acc)) acc = INFINITY;
+        int st = s->status;
+        if (st == 0 || st == 3 || st == 4 || st == 5 || st == 6) {
+            if (acc <= s->o.tol) st = 0;
+            else if (acc <= 1e-6) st = 5;
+            else if (st == 0 || st == 5) st = 4;
+        }
+        s->status = st;
+    }
+    rc = 0;
+out:
+    free(act); free(okx); free(okz); free(lam); free(hrp);
+    return rc;
+}
+
+/* ---- end: the returned point on the data as read ------------------------------------------ */
+int ipm_finish(Solve *s) {
+    const HProb *P = &s->P;
+    const int m = P->m;
+    double *H = heapp();
+    double *X = malloc(sizeof(double) * (P->T + 1)), *y = malloc(sizeof(double) * (m + 1));
+    if (bk_d2h(X, H + s->X, sizeof(double) * P->T) || bk_d2h(y, H + s->y, sizeof(double) * m)) {
+        free(X); free(y); return -1;
+    }
+    for (int64_t q = 0; q < P->T; q++) X[q] *= P->bs;                  /* X = bs X~ */
+    for (int i = 0; i < m; i++) y[i] *= P->cs * P->d[i];                 /* y_i = cs d_i y~_i */
+    /* results per input block; Z = C - A'y on the data as read (host, FP64) */
+    int64_t tot = 0;
+    for (int k = 0; k < P->nblk_in; k++) tot += P->bs_in[k] > 0 ? (int64_t)P->bs_in[k] * P->bs_in[k] : -P->bs_in[k];
+    free(s->res_X); free(s->res_Z); free(s->res_y);
+    s->res_X = malloc(sizeof(double) * (tot + 1));
+    s->res_Z = malloc(sizeof(double) * (tot + 1));
+    s->res_y = y;
+    double *Zint = malloc(sizeof(double) * (P->T + 1));
+    for (int64_t q = 0; q < P->T; q++) Zint[q] = P->C[q] * P->cs;
+    for (int p = 0; p < P->npos; p++) {
+        double sum = 0;
+        for (int t = P->pptr[p]; t < P->pptr[p + 1]; t++) sum += y[P->pcon[t]] * P->pval[t] / P->d[P->pcon[t]];
+        Zint[P->ppos[2 * p]] -= sum;
+        if (P->ppos[2 * p + 1] >= 0) Zint[P->ppos[2 * p + 1]] -= sum;
+    }
+    int64_t o = 0;
+    for (int k = 0; k < P->nblk_in; k++) {
+        const int kb = P->in_first[k];
+        if (P->bs_in[k] > 0) {
+            const int64_t len = (int64_t)P->bs_in[k] * P->bs_in[k];
+            memcpy(s->res_X + o, X + P->boff[kb], sizeof(double) * len);
+            memcpy(s->res_Z + o, Zint + P->boff[kb], sizeof(double) * len);
+            o += len;
+        } else {
+            for (int i = 0; i < -P->bs_in[k]; i++) { s->res_X[o + i] = X[P->boff[kb + i]]; s->res_Z[o + i] = Zint[P->boff[kb + i]]; }
+            o += -P->bs_in[k];
+        }
+    }
+    free(X); free(Zint);
+    s->t_solve = bk_now() - s->t_start;
+    return 0;
+}
+
+int ipm_run_batch(Solve **S, int ns) {
+    if (ipm_start_batch(S, ns)) return -1;
+    for (int guard = 0; guard < 1000; guard++) {
+        int any = 0;
+        for (int q = 0; q < ns; q++) any |= !S[q]->done;
+        if (!any) break;
+        if (ipm_iterate_batch(S, ns)) return -1;
+    }
+    return ipm_end_batch(S, ns);
+}
+int ipm_end_batch(Solve **S, int ns) {
+    if (ipm_dimacs(S, ns)) return -1;
+    for (int q = 0; q < ns; q++) if (ipm_finish(S[q])) return -1;
+    return 0;
+}

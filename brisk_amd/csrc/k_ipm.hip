@@ -1,0 +1,497 @@
+// k_ipm.hip - the interior-point method's own device kernels (CDNA4, wave64):
+//   * constraint operators A(D) (gather, one wave per constraint) and A*(y) (scatter by position,
+//     one thread per position, the constraints of a position summed in a fixed order);
+//   * Schur complement assembly, route 0 (reference solver.c:2626-2660): one thread per pair of
+//     constraints over their common blocks, and the Hadamard form M = D_v (X o Zi) D_v for
+//     constraints that are one diagonal entry each (max-cut; reference solver.c:2631-2634);
+//   * segmented dot products with a fixed summation order (the result does not depend on how
+//     many other solves share the launch) and fused linear combinations;
+//   * equilibration of M into the f32 or f64 factor input, symmetrisation, transposes;
+//   * Lanczos steps with full reorthogonalisation for the step-length bound (reference
+//     solver.c:6083-6225).
+// All operands are offsets from the one device heap (doubles unless stated).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <vector>
+#include "dev.h"
+#include "rt_internal.h"
+
+static char *g_heap = nullptr;
+static size_t g_heap_size = 0;
+static double *g_partial = nullptr;        // dot-product chunk partials
+static size_t g_partial_cap = 0;
+
+extern "C" int bk_heap_init(size_t bytes) {
+    if (g_heap && g_heap_size >= bytes) return 0;
+    if (g_heap) { bk_sync(); (void)hipFree(g_heap); g_heap = nullptr; g_heap_size = 0; }
+    if (bk_check(hipMalloc((void **)&g_heap, bytes), "hipMalloc heap")) return -1;
+    g_heap_size = bytes;
+    return 0;
+}
+extern "C" void *bk_heap(void) { return g_heap; }
+extern "C" size_t bk_heap_size(void) { return g_heap_size; }
+
+#define H64(off) (((double *)g_heapd) + (off))
+
+// ---- dot products ---------------------------------------------------------------------------
+#define DCH 4096
+__global__ __launch_bounds__(256) void k_dot_chunks(const DotItem *__restrict__ it, const int64_t *__restrict__ cbase, int nit,
+                                                    const double *__restrict__ H, double *__restrict__ part) {
+    const int q = blockIdx.y + blockIdx.z * 65535;
+    if (q >= nit) return;
+    const DotItem d = it[q];
+    const int64_t c = blockIdx.x;
+    const int64_t nch = (d.len + DCH - 1) / DCH;
+    if (c >= nch) return;
+    __shared__ double sh[256];
+    const int t = threadIdx.x;
+    double s = 0;
+    const double *x = H + d.xo, *y = H + d.yo;
+#pragma unroll 4
+    for (int u = 0; u < DCH / 256; u++) {
+        const int64_t e = c * DCH + t + 256 * u;
+        if (e < d.len) s += x[e] * y[e];
+    }
+    sh[t] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    if (t == 0) part[cbase[q] + c] = sh[0];
+}
+__global__ void k_dot_final(const DotItem *__restrict__ it, const int64_t *__restrict__ cbase, int nit, double *__restrict__ H,
+                            const double *__restrict__ part) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nit) return;
+    const DotItem d = it[q];
+    const int64_t nch = (d.len + DCH - 1) / DCH;
+    double s = 0;
+    for (int64_t c = 0; c < nch; c++) s += part[cbase[q] + c];
+    H[d.out] = s;
+}
+extern "C" int bk_dots(const DotItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    std::vector<int64_t> cb(n_it);
+    int64_t tot = 0, mx = 1;
+    for (int q = 0; q < n_it; q++) {
+        cb[q] = tot;
+        const int64_t nch = (it[q].len + DCH - 1) / DCH;
+        tot += nch;
+        if (nch > mx) mx = nch;
+    }
+    if ((size_t)tot > g_partial_cap) {
+        bk_sync();
+        if (g_partial) (void)hipFree(g_partial);
+        g_partial_cap = (size_t)tot * 2 + 4096;
+        if (bk_check(hipMalloc((void **)&g_partial, g_partial_cap * sizeof(double)), "hipMalloc partials")) return -1;
+    }
+    const DotItem *d = (const DotItem *)bk_stage(it, sizeof(DotItem) * n_it);
+    const int64_t *dcb = (const int64_t *)bk_stage(cb.data(), sizeof(int64_t) * n_it);
+    if (!d || !dcb) return -1;
+    dim3 grid((unsigned)mx, n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    hipLaunchKernelGGL(k_dot_chunks, grid, dim3(256), 0, g_bk_stream, d, dcb, n_it, (const double *)g_heap, g_partial);
+    if (BK_LAUNCH_CHECK("k_dot_chunks")) return -1;
+    hipLaunchKernelGGL(k_dot_final, dim3((n_it + 255) / 256), dim3(256), 0, g_bk_stream, d, dcb, n_it, (double *)g_heap, g_partial);
+    return BK_LAUNCH_CHECK("k_dot_final");
+}
+
+// ---- linear combinations ----------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_lincomb(const LinItem *__restrict__ it, int nit, double *__restrict__ H) {
+    const int q = blockIdx.y + blockIdx.z * 65535;
+    if (q >= nit) return;
+    const LinItem L = it[q];
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < L.len; e += (int64_t)gridDim.x * 256) {
+        double v = 0;
+#pragma unroll
+        for (int t = 0; t < 4; t++)
+            if (L.in[t] >= 0) v += L.c[t] * H[L.in[t] + e];
+        H[L.out + e] = v;
+    }
+}
+extern "C" int bk_lincomb(const LinItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    int64_t mx = 1;
+    for (int q = 0; q < n_it; q++) mx = it[q].len > mx ? it[q].len : mx;
+    const LinItem *d = (const LinItem *)bk_stage(it, sizeof(LinItem) * n_it);
+    if (!d) return -1;
+    dim3 grid(bk_grid1(mx, 256, 2048), n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    hipLaunchKernelGGL(k_lincomb, grid, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    return BK_LAUNCH_CHECK("k_lincomb");
+}
+
+// out[q] = c0 in0[q] in1[q] (+ c1 in2[q] when in[2] >= 0)
+__global__ __launch_bounds__(256) void k_vmul(const LinItem *__restrict__ it, int nit, double *__restrict__ H) {
+    const int q = blockIdx.y + blockIdx.z * 65535;
+    if (q >= nit) return;
+    const LinItem L = it[q];
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < L.len; e += (int64_t)gridDim.x * 256) {
+        double v = L.c[0] * H[L.in[0] + e] * H[L.in[1] + e];
+        if (L.in[2] >= 0) v += L.c[1] * H[L.in[2] + e];
+        H[L.out + e] = v;
+    }
+}
+extern "C" int bk_vmul(const LinItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    int64_t mx = 1;
+    for (int q = 0; q < n_it; q++) mx = it[q].len > mx ? it[q].len : mx;
+    const LinItem *d = (const LinItem *)bk_stage(it, sizeof(LinItem) * n_it);
+    if (!d) return -1;
+    dim3 grid(bk_grid1(mx, 256, 2048), n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    hipLaunchKernelGGL(k_vmul, grid, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    return BK_LAUNCH_CHECK("k_vmul");
+}
+
+// ---- per-matrix helpers ---------------------------------------------------------------------
+__global__ void k_symmetrize(const MItem *__restrict__ it, int nit, double *__restrict__ H) {
+    const int q = blockIdx.y + blockIdx.z * 65535;
+    if (q >= nit) return;
+    const MItem m = it[q];
+    const int n = m.n;
+    double *G = H + m.off;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)n * n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e % n), c = (int)(e / n);
+        if (r > c) {
+            const double v = 0.5 * (G[r + (int64_t)c * n] + G[c + (int64_t)r * n]);
+            G[r + (int64_t)c * n] = v;
+            G[c + (int64_t)r * n] = v;
+        }
+    }
+}
+template <typename T>
+__global__ void k_transpose(const MItem *__restrict__ it, int nit, T *__restrict__ H, int64_t delta) {
+    const int q = blockIdx.y + blockIdx.z * 65535;
+    if (q >= nit) return;
+    const MItem m = it[q];
+    const int n = m.n;
+    // 32 x 32 tiles through LDS: coalesced reads and writes
+    __shared__ T tile[32][33];
+    const int tiles = (n + 31) / 32;
+    for (int64_t tt = blockIdx.x; tt < (int64_t)tiles * tiles; tt += gridDim.x) {
+        const int tr = (int)(tt % tiles) * 32, tc = (int)(tt / tiles) * 32;
+        const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;     // 256 threads: 32 x 8
+        for (int y = ly; y < 32; y += 8) {
+            const int r = tr + lx, c = tc + y;
+            if (r < n && c < n) tile[y][lx] = H[m.off + r + (int64_t)c * n];
+        }
+        __syncthreads();
+        for (int y = ly; y < 32; y += 8) {
+            const int r = tc + lx, c = tr + y;                       // dst(r, c) = src(c, r)
+            if (r < n && c < n) H[m.off + delta + r + (int64_t)c * n] = tile[lx][y];
+        }
+        __syncthreads();
+    }
+}
+__global__ void k_add_diag(const MItem *__restrict__ it, int nit, double *__restrict__ H, int64_t s_off) {
+    const int q = blockIdx.y + blockIdx.z * 65535;
+    if (q >= nit) return;
+    const MItem m = it[q];
+    const double s = H[s_off + m.slot];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.n; i += gridDim.x * blockDim.x) H[m.off + i + (int64_t)i * m.n] += s;
+}
+__global__ void k_equil_d(const MItem *__restrict__ it, int nit, double *__restrict__ H) {
+    const int q = blockIdx.y + blockIdx.z * 65535;
+    if (q >= nit) return;
+    const MItem m = it[q];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.n; i += gridDim.x * blockDim.x) {
+        const double d = H[m.off + i + (int64_t)i * m.n];
+        H[m.off2 + i] = d > 0 ? 1.0 / sqrt(d) : 1.0;
+    }
+}
+template <typename T>
+__global__ void k_equil_f(const MItem *__restrict__ it, int nit, const double *__restrict__ H, T *__restrict__ F, int64_t reg_off) {
+    const int q = blockIdx.y + blockIdx.z * 65535;
+    if (q >= nit) return;
+    const MItem m = it[q];
+    const int n = m.n;
+    const double reg = H[reg_off + m.slot];
+    const double *D = H + m.off2;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)n * n; e += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e % n), c = (int)(e / n);
+        T v = 0;
+        if (r >= c) {
+            double x = H[m.off + e] * D[r] * D[c];
+            if (r == c) x += reg;
+            v = fabs(x) < 1e-30 ? (T)0 : (T)x;
+        }
+        F[m.off3 + e] = v;
+    }
+}
+static int mitem_launch_dims(const MItem *it, int n_it, int64_t per, dim3 &grid) {
+    int64_t mx = 1;
+    for (int q = 0; q < n_it; q++) { const int64_t v = per == 1 ? it[q].n : (int64_t)it[q].n * it[q].n; mx = v > mx ? v : mx; }
+    grid = dim3(bk_grid1(mx, 256, 1024), n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    return 0;
+}
+extern "C" int bk_symmetrize(const MItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    dim3 g; mitem_launch_dims(it, n_it, 2, g);
+    const MItem *d = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
+    hipLaunchKernelGGL(k_symmetrize, g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    return BK_LAUNCH_CHECK("k_symmetrize");
+}
+extern "C" int bk_transpose_f64(const MItem *it, int n_it, int64_t delta) {
+    if (n_it <= 0) return 0;
+    dim3 g; mitem_launch_dims(it, n_it, 2, g);
+    g.x = g.x > 512 ? 512 : g.x;
+    const MItem *d = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
+    hipLaunchKernelGGL((k_transpose<double>), g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap, delta);
+    return BK_LAUNCH_CHECK("k_transpose");
+}
+extern "C" int bk_transpose_f32(const MItem *it, int n_it, int64_t delta) {
+    if (n_it <= 0) return 0;
+    dim3 g; mitem_launch_dims(it, n_it, 2, g);
+    g.x = g.x > 512 ? 512 : g.x;
+    const MItem *d = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
+    hipLaunchKernelGGL((k_transpose<float>), g, dim3(256), 0, g_bk_stream, d, n_it, (float *)g_heap, delta);
+    return BK_LAUNCH_CHECK("k_transpose");
+}
+extern "C" int bk_add_diag(const MItem *it, int n_it, int64_t s_off) {
+    if (n_it <= 0) return 0;
+    dim3 g; mitem_launch_dims(it, n_it, 1, g);
+    const MItem *d = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
+    hipLaunchKernelGGL(k_add_diag, g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap, s_off);
+    return BK_LAUNCH_CHECK("k_add_diag");
+}
+extern "C" int bk_equil(const MItem *it, int n_it, int prec, int64_t reg_off) {
+    if (n_it <= 0) return 0;
+    dim3 g1; mitem_launch_dims(it, n_it, 1, g1);
+    dim3 g2; mitem_launch_dims(it, n_it, 2, g2);
+    const MItem *d = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
+    hipLaunchKernelGGL(k_equil_d, g1, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    if (BK_LAUNCH_CHECK("k_equil_d")) return -1;
+    if (prec == 0) hipLaunchKernelGGL((k_equil_f<float>), g2, dim3(256), 0, g_bk_stream, d, n_it, (const double *)g_heap, (float *)g_heap, reg_off);
+    else hipLaunchKernelGGL((k_equil_f<double>), g2, dim3(256), 0, g_bk_stream, d, n_it, (const double *)g_heap, (double *)g_heap, reg_off);
+    return BK_LAUNCH_CHECK("k_equil_f");
+}
+
+// ---- constraint operators -------------------------------------------------------------------
+__device__ __forceinline__ const int *HI(const double *H, int64_t off) { return (const int *)(((const char *)H) + off); }
+
+// A(D): one wave per constraint, lanes stride its entries, butterfly in a fixed pattern
+__global__ __launch_bounds__(256) void k_aop(const ConItem *__restrict__ it, int nit, double *__restrict__ H) {
+    const int q = blockIdx.y + blockIdx.z * 65535;
+    if (q >= nit) return;
+    const ConItem C = it[q];
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= C.m) return;
+    const int *cptr = HI(H, C.cptr);
+    const SEnt *ent = (const SEnt *)(((const char *)H) + C.ent);
+    const BlkRef *blk = (const BlkRef *)(((const char *)H) + C.blk);
+    const double *D = H + C.D;
+    double s = 0;
+    for (int e = cptr[i] + lane; e < cptr[i + 1]; e += 64) {
+        const SEnt E = ent[e];
+        const BlkRef B = blk[E.k];
+        double x = D[B.off + E.r + (int64_t)E.c * B.n];
+        if (E.r != E.c) x += D[B.off + E.c + (int64_t)E.r * B.n];
+        s += E.v * x;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) {
+        double *o = H + C.out + i;
+        *o = C.beta != 0 ? C.beta * *o + C.alpha * s : C.alpha * s;
+    }
+}
+extern "C" int bk_aop(const ConItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    int mx = 1;
+    for (int q = 0; q < n_it; q++) mx = it[q].m > mx ? it[q].m : mx;
+    const ConItem *d = (const ConItem *)bk_stage(it, sizeof(ConItem) * n_it);
+    dim3 g((mx + 3) / 4, n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    hipLaunchKernelGGL(k_aop, g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    return BK_LAUNCH_CHECK("k_aop");
+}
+
+// A*(y) added at the positions the constraints touch: D[pos] += alpha sum_i y_i v_i
+__global__ __launch_bounds__(256) void k_aty(const ConItem *__restrict__ it, int nit, double *__restrict__ H) {
+    const int q = blockIdx.y + blockIdx.z * 65535;
+    if (q >= nit) return;
+    const ConItem C = it[q];
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= C.npos) return;
+    const int *pptr = HI(H, C.pptr), *ppos = HI(H, C.ppos), *pcon = HI(H, C.pcon);
+    const double *pval = H + C.pval, *y = H + C.y;
+    double s = 0;
+    for (int t = pptr[p]; t < pptr[p + 1]; t++) s += y[pcon[t]] * pval[t];
+    double *D = H + C.D;
+    const int o1 = ppos[2 * p], o2 = ppos[2 * p + 1];
+    D[o1] += C.alpha * s;
+    if (o2 >= 0) D[o2] += C.alpha * s;
+}
+extern "C" int bk_aty(const ConItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    int mx = 1;
+    for (int q = 0; q < n_it; q++) mx = it[q].npos > mx ? it[q].npos : mx;
+    const ConItem *d = (const ConItem *)bk_stage(it, sizeof(ConItem) * n_it);
+    dim3 g((mx + 255) / 256, n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    hipLaunchKernelGGL(k_aty, g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    return BK_LAUNCH_CHECK("k_aty");
+}
+
+// Schur complement, Hadamard form: every constraint a single diagonal entry v_a at (r_a, r_a) of
+// block k_a: M_ab = v_a v_b X[r_a, r_b] Zi[r_a, r_b] when k_a == k_b (else 0). One column b per
+// blockIdx.y, rows a across the workgroup: X and Zi columns streamed, M written coalesced.
+__global__ __launch_bounds__(256) void k_schur_diag(const ConItem *__restrict__ it, int nit, double *__restrict__ H) {
+    const int q = blockIdx.z;
+    if (q >= nit) return;
+    const ConItem C = it[q];
+    if (!C.diag) return;
+    const int b = blockIdx.y, a = blockIdx.x * 256 + threadIdx.x;
+    if (b >= C.m || a >= C.m) return;
+    const SEnt *ent = (const SEnt *)(((const char *)H) + C.ent);
+    const BlkRef *blk = (const BlkRef *)(((const char *)H) + C.blk);
+    const SEnt Ea = ent[a], Eb = ent[b];
+    double v = 0;
+    if (Ea.k == Eb.k) {
+        const BlkRef B = blk[Ea.k];
+        const int64_t o = B.off + Ea.r + (int64_t)Eb.r * B.n;
+        v = Ea.v * Eb.v * H[C.X + o] * H[C.Zi + o];
+    }
+    H[C.M + a + (int64_t)b * C.m] = v;
+}
+// Schur complement, route 0: one thread per pair a >= b; both constraints' entries are grouped
+// by block in ascending order, so the common blocks are found by a merge
+__global__ __launch_bounds__(256) void k_schur_r0(const ConItem *__restrict__ it, int nit, double *__restrict__ H) {
+    const int q = blockIdx.y + blockIdx.z * 65535;
+    if (q >= nit) return;
+    const ConItem C = it[q];
+    if (C.diag) return;
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t npair = (int64_t)C.m * (C.m + 1) / 2;
+    if (t >= npair) return;
+    int64_t a = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while (a * (a + 1) / 2 > t) a--;
+    while ((a + 1) * (a + 2) / 2 <= t) a++;
+    const int64_t b = t - a * (a + 1) / 2;
+    const int *cptr = HI(H, C.cptr);
+    const SEnt *ent = (const SEnt *)(((const char *)H) + C.ent);
+    const BlkRef *blk = (const BlkRef *)(((const char *)H) + C.blk);
+    const double *X = H + C.X, *Zi = H + C.Zi;
+    int pa = cptr[a], ea = cptr[a + 1], pb = cptr[b], eb = cptr[b + 1];
+    double s = 0;
+    while (pa < ea && pb < eb) {
+        const int ka = ent[pa].k, kb = ent[pb].k;
+        if (ka < kb) { pa++; continue; }
+        if (kb < ka) { pb++; continue; }
+        int qa = pa; while (qa < ea && ent[qa].k == ka) qa++;
+        int qb = pb; while (qb < eb && ent[qb].k == ka) qb++;
+        const BlkRef B = blk[ka];
+        const int n = B.n;
+        const double *Xk = X + B.off, *Zk = Zi + B.off;
+        for (int u = pa; u < qa; u++) {
+            const SEnt P = ent[u];
+            for (int w = pb; w < qb; w++) {
+                const SEnt Q = ent[w];
+                // expansions (r, c) and, off the diagonal, (c, r) of both entries
+                double acc = Xk[P.c + (int64_t)Q.r * n] * Zk[Q.c + (int64_t)P.r * n];
+                if (Q.r != Q.c) acc += Xk[P.c + (int64_t)Q.c * n] * Zk[Q.r + (int64_t)P.r * n];
+                if (P.r != P.c) {
+                    acc += Xk[P.r + (int64_t)Q.r * n] * Zk[Q.c + (int64_t)P.c * n];
+                    if (Q.r != Q.c) acc += Xk[P.r + (int64_t)Q.c * n] * Zk[Q.r + (int64_t)P.c * n];
+                }
+                s += P.v * Q.v * acc;
+            }
+        }
+        pa = qa; pb = qb;
+    }
+    H[C.M + a + b * C.m] = s;
+    H[C.M + b + a * C.m] = s;
+}
+extern "C" int bk_schur(const ConItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    int mxd = 0; int64_t mxp = 0; int anyd = 0, anyr = 0;
+    for (int q = 0; q < n_it; q++) {
+        if (it[q].diag) { anyd = 1; mxd = it[q].m > mxd ? it[q].m : mxd; }
+        else { anyr = 1; const int64_t np = (int64_t)it[q].m * (it[q].m + 1) / 2; mxp = np > mxp ? np : mxp; }
+    }
+    const ConItem *d = (const ConItem *)bk_stage(it, sizeof(ConItem) * n_it);
+    if (anyd) {
+        if (n_it > 65535 || mxd > 65535) return -1;
+        hipLaunchKernelGGL(k_schur_diag, dim3((mxd + 255) / 256, mxd, n_it), dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+        if (BK_LAUNCH_CHECK("k_schur_diag")) return -1;
+    }
+    if (anyr) {
+        dim3 g((unsigned)((mxp + 255) / 256), n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+        hipLaunchKernelGGL(k_schur_r0, g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+        if (BK_LAUNCH_CHECK("k_schur_r0")) return -1;
+    }
+    return 0;
+}
+
+// ---- Lanczos --------------------------------------------------------------------------------
+__device__ double block_sum(double v, double *sh) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+__global__ __launch_bounds__(256) void k_lz_start(const LzItem *__restrict__ it, int nit, double *__restrict__ H) {
+    const int q = blockIdx.x;
+    if (q >= nit) return;
+    const LzItem L = it[q];
+    __shared__ double sh[256];
+    double *Q = H + L.Q;
+    double s = 0;
+    for (int i = threadIdx.x; i < L.n; i += 256) { const double v = 1.0 + 0.5 * sin(1.7 * i + 0.3); Q[i] = v; s += v * v; }
+    const double nrm = 1.0 / sqrt(block_sum(s, sh));
+    for (int i = threadIdx.x; i < L.n; i += 256) Q[i] *= nrm;
+}
+__global__ __launch_bounds__(256) void k_lz_step(const LzItem *__restrict__ it, int nit, int j, double *__restrict__ H) {
+    const int q = blockIdx.x;
+    if (q >= nit) return;
+    const LzItem L = it[q];
+    if (j >= L.K) return;
+    __shared__ double sh[256];
+    const int n = L.n;
+    double *Q = H + L.Q, *w = H + L.w, *ab = H + L.ab;
+    const double *qj = Q + (int64_t)j * n;
+    double s = 0;
+    for (int i = threadIdx.x; i < n; i += 256) s += qj[i] * w[i];
+    const double alpha = block_sum(s, sh);
+    const double bprev = j > 0 ? ab[32 + j - 1] : 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        double v = w[i] - alpha * qj[i];
+        if (j > 0) v -= bprev * Q[(int64_t)(j - 1) * n + i];
+        w[i] = v;
+    }
+    __syncthreads();
+    for (int pass = 0; pass < 2; pass++)
+        for (int p = 0; p <= j; p++) {
+            const double *qp = Q + (int64_t)p * n;
+            double c = 0;
+            for (int i = threadIdx.x; i < n; i += 256) c += qp[i] * w[i];
+            c = block_sum(c, sh);
+            for (int i = threadIdx.x; i < n; i += 256) w[i] -= c * qp[i];
+            __syncthreads();
+        }
+    double b2 = 0;
+    for (int i = threadIdx.x; i < n; i += 256) b2 += w[i] * w[i];
+    const double beta = sqrt(block_sum(b2, sh));
+    if (threadIdx.x == 0) { ab[j] = alpha; ab[32 + j] = beta; }
+    if (j + 1 < L.K) {
+        const double inv = beta > 1e-300 ? 1.0 / beta : 0.0;
+        double *qn = Q + (int64_t)(j + 1) * n;
+        for (int i = threadIdx.x; i < n; i += 256) qn[i] = w[i] * inv;
+    }
+}
+extern "C" int bk_lanczos_start(const LzItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    const LzItem *d = (const LzItem *)bk_stage(it, sizeof(LzItem) * n_it);
+    hipLaunchKernelGGL(k_lz_start, dim3(n_it), dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    return BK_LAUNCH_CHECK("k_lz_start");
+}
+extern "C" int bk_lanczos_step(const LzItem *it, int n_it, int j) {
+    if (n_it <= 0) return 0;
+    const LzItem *d = (const LzItem *)bk_stage(it, sizeof(LzItem) * n_it);
+    hipLaunchKernelGGL(k_lz_step, dim3(n_it), dim3(256), 0, g_bk_stream, d, n_it, j, (double *)g_heap);
+    return BK_LAUNCH_CHECK("k_lz_step");
+}

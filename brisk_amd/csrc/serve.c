@@ -1,0 +1,179 @@
+/* serve.c - the persistent request loop (continuous batching of SDP solves on one GPU).
+ *
+ * The reference has no counterpart (its library is not reentrant, capi.c:19-20). Requests queue
+ * in arrival order; every step admits waiting requests while solve slots and device memory
+ * allow (each solve's projected bytes: its block arrays, its Schur complement and factor, its
+ * vectors - solve_heap_bytes), starts the newcomers, advances every solve in flight by one
+ * interior-point iteration in one set of batched launches, and finishes the solves that ended.
+ * A solve's arithmetic does not depend on its batch-mates: every reduction is per solve in a
+ * fixed order, so a result is the same alone and in the mix. */
+#include <stdlib.h>
+#include <string.h>
+#include "brisk_amd.h"
+#include "solver.h"
+
+#define API __attribute__((visibility("default")))
+
+int brisk_parse_options(int nopt, const char *const *opts, SolveOpts *o, int m, int *slots, double *heapgb);
+int brisk_heap_ensure(size_t need);
+void brisk_fill_result(BriskResult *r, Solve *s, double t_setup, double t_iter, double t_end);
+
+typedef struct {
+    int state;                  /* 0 waiting, 1 in flight, 2 finished, 3 taken */
+    Solve *s;
+    double t_submit, t_admit, t_done;
+    BriskResult *res;
+} Req;
+
+struct BriskServe {
+    int nopt;
+    char **opts;
+    int slots;
+    Req *req;
+    int nreq, capreq;
+    int *queue, qhead, qtail, qcap;   /* waiting request ids, FIFO */
+    Solve **fly;                      /* solves in flight */
+    int *fly_id;
+    int nfly;
+};
+
+API BriskServe *brisk_serve_new(int nopt, const char *const *opts) {
+    SolveOpts o;
+    int slots = 64;
+    double heapgb = 0;
+    if (brisk_parse_options(nopt, opts, &o, 0, &slots, &heapgb)) return NULL;
+    if (slots < 1) slots = 1;
+    if (bk_init()) return NULL;
+    if (heapgb > 0 && brisk_heap_ensure((size_t)(heapgb * (double)(1ull << 30)))) return NULL;
+    BriskServe *sv = calloc(1, sizeof(BriskServe));
+    sv->nopt = nopt;
+    sv->opts = malloc(sizeof(char *) * (nopt + 1));
+    for (int k = 0; k < nopt; k++) sv->opts[k] = strdup(opts[k]);
+    sv->slots = slots;
+    sv->fly = malloc(sizeof(Solve *) * slots);
+    sv->fly_id = malloc(sizeof(int) * slots);
+    if (heapgb <= 0) {
+        /* default: most of the free device memory */
+        const size_t f = bk_mem_free() + bk_heap_size();
+        size_t want = f > ((size_t)4 << 30) ? f - ((size_t)4 << 30) : f / 2;
+        if (want > ((size_t)96 << 30)) want = (size_t)96 << 30;
+        if (brisk_heap_ensure(want)) { brisk_serve_delete(sv); return NULL; }
+    }
+    return sv;
+}
+
+API int brisk_serve_submit(BriskServe *sv, const BriskProblem *p) {
+    if (!sv || !p) return -1;
+    Solve *s = calloc(1, sizeof(Solve));
+    if (brisk_parse_options(sv->nopt, (const char *const *)sv->opts, &s->o, p->m, NULL, NULL)) { free(s); return -1; }
+    if (hprob_build(&s->P, p->m, p->nblk, p->bs, p->c, p->nnz, p->mat, p->blk, p->i, p->j, p->v)) { hprob_free(&s->P); free(s); return -2; }
+    if (sv->nreq == sv->capreq) { sv->capreq = sv->capreq ? 2 * sv->capreq : 256; sv->req = realloc(sv->req, sizeof(Req) * sv->capreq); }
+    const int id = sv->nreq++;
+    Req *r = &sv->req[id];
+    memset(r, 0, sizeof(*r));
+    r->s = s;
+    s->id = id;
+    r->t_submit = bk_now();
+    if (sv->qtail == sv->qcap) {
+        if (sv->qhead > 0) { memmove(sv->queue, sv->queue + sv->qhead, sizeof(int) * (sv->qtail - sv->qhead)); sv->qtail -= sv->qhead; sv->qhead = 0; }
+        if (sv->qtail == sv->qcap) { sv->qcap = sv->qcap ? 2 * sv->qcap : 256; sv->queue = realloc(sv->queue, sizeof(int) * sv->qcap); }
+    }
+    sv->queue[sv->qtail++] = id;
+    return id;
+}
+
+static void finish_done(BriskServe *sv) {
+    int nd = 0;
+    for (int q = 0; q < sv->nfly; q++) nd += sv->fly[q]->done;
+    if (!nd) return;
+    Solve **D = malloc(sizeof(Solve *) * nd);
+    int k = 0;
+    for (int q = 0; q < sv->nfly; q++) if (sv->fly[q]->done) D[k++] = sv->fly[q];
+    const double t0 = bk_now();
+    const int bad = ipm_end_batch(D, nd);
+    const double t1 = bk_now();
+    for (int q = 0; q < nd; q++) {
+        Req *r = &sv->req[D[q]->id];
+        r->res = brisk_result_new();
+        if (bad) D[q]->status = 4;
+        brisk_fill_result(r->res, D[q], 0.0, t0 - r->t_admit, (t1 - t0) / nd);
+        r->t_done = bk_now();
+        r->state = 2;
+        solve_release(D[q]);
+        hprob_free(&D[q]->P);
+        free(D[q]->res_X); free(D[q]->res_Z); free(D[q]->res_y);
+        free(D[q]);
+        r->s = NULL;
+    }
+    int w = 0;
+    for (int q = 0; q < sv->nfly; q++) {
+        if (sv->req[sv->fly_id[q]].state == 2) continue;
+        sv->fly[w] = sv->fly[q]; sv->fly_id[w] = sv->fly_id[q]; w++;
+    }
+    sv->nfly = w;
+    free(D);
+}
+
+API int brisk_serve_step(BriskServe *sv) {
+    if (!sv) return -1;
+    /* admission: in arrival order while a slot and the solve's projected memory are free */
+    Solve **nw = malloc(sizeof(Solve *) * sv->slots);
+    int nn = 0;
+    while (sv->qhead < sv->qtail && sv->nfly + nn < sv->slots) {
+        const int id = sv->queue[sv->qhead];
+        Solve *s = sv->req[id].s;
+        const size_t need = (size_t)solve_heap_bytes(&s->P) + ((size_t)1 << 20);
+        if (heap_largest_free() < need) {
+            if (sv->nfly + nn == 0 && brisk_heap_ensure(need)) { sv->req[id].state = 2; sv->qhead++; s->status = 4; continue; }
+            if (heap_largest_free() < need) break;
+        }
+        if (solve_setup(s)) break;
+        sv->qhead++;
+        sv->req[id].state = 1;
+        sv->req[id].t_admit = bk_now();
+        nw[nn++] = s;
+    }
+    if (nn) {
+        if (ipm_start_batch(nw, nn)) { free(nw); return -1; }
+        for (int q = 0; q < nn; q++) { sv->fly[sv->nfly] = nw[q]; sv->fly_id[sv->nfly] = nw[q]->id; sv->nfly++; }
+    }
+    free(nw);
+    finish_done(sv);
+    if (sv->nfly) {
+        if (ipm_iterate_batch(sv->fly, sv->nfly)) return -1;
+        finish_done(sv);
+    }
+    return (sv->qtail - sv->qhead) + sv->nfly;
+}
+
+API int brisk_serve_take(BriskServe *sv, int id, BriskResult *res, double *latency) {
+    if (!sv || id < 0 || id >= sv->nreq) return -1;
+    Req *r = &sv->req[id];
+    if (r->state < 2) return 0;
+    if (r->state == 3) return -1;
+    if (r->res && res) {
+        /* move: swap the contents */
+        BriskResult *tmp = r->res;
+        extern void brisk_result_move(BriskResult *dst, BriskResult *src);
+        brisk_result_move(res, tmp);
+        brisk_result_delete(tmp);
+    } else if (r->res) brisk_result_delete(r->res);
+    r->res = NULL;
+    if (latency) *latency = r->t_done - r->t_submit;
+    r->state = 3;
+    return 1;
+}
+API int brisk_serve_inflight(const BriskServe *sv) { return sv ? sv->nfly : 0; }
+
+API void brisk_serve_delete(BriskServe *sv) {
+    if (!sv) return;
+    for (int q = 0; q < sv->nfly; q++) { solve_release(sv->fly[q]); }
+    for (int i = 0; i < sv->nreq; i++) {
+        Req *r = &sv->req[i];
+        if (r->s) { hprob_free(&r->s->P); free(r->s->res_X); free(r->s->res_Z); free(r->s->res_y); free(r->s); }
+        if (r->res) brisk_result_delete(r->res);
+    }
+    for (int k = 0; k < sv->nopt; k++) free(sv->opts[k]);
+    free(sv->opts); free(sv->req); free(sv->queue); free(sv->fly); free(sv->fly_id);
+    free(sv);
+}

@@ -1,0 +1,107 @@
+/* solver.h - host-side structures of the GPU interior-point solver (C).
+ *
+ * A solve owns a problem (host copy of the scaled data, constraint entries and position lists)
+ * and device arrays carved from the one device heap. The iteration runs on a batch of solves at
+ * once: every kernel launch covers all solves of the batch, and each solve keeps its own scalars
+ * (step lengths, sigma, theta, convergence state) on the host. */
+#ifndef BRISK_SOLVER_H
+#define BRISK_SOLVER_H
+#include <stdint.h>
+#include <stddef.h>
+#include "dev.h"
+
+#define BK_LZ_K 30          /* Lanczos steps (reference Params.lanczos_k, solver.c:119) */
+
+typedef struct {
+    /* the problem as read */
+    int m, nblk_in;
+    int *bs_in;
+    int *in_first;          /* internal block of input block k (LP: its first variable) */
+    /* internal blocks: every SDP block, and every LP variable as a 1 x 1 block */
+    int nb;
+    int *bn;
+    int64_t *boff;          /* element offset of block k in a block arena */
+    int64_t T;              /* arena length: sum of n^2 */
+    int nmax, any_sdp;
+    double Ntot;            /* sum of block orders (the barrier's dimension) */
+    /* scaled data (problem.c:1866-1909) */
+    double *b, *b0, *d, *du, *C;
+    double normb1, normb2, normC1, normC2, bs, cs, nbs;
+    /* constraint entries, upper (r <= c), CSR by constraint, grouped by block */
+    int *cptr;
+    SEnt *ent;
+    int nent;
+    /* positions touched by constraints, each with its (constraint, value) list */
+    int npos;
+    int *pptr, *ppos, *pcon;
+    double *pval;
+    int diag;               /* every constraint is one diagonal entry: Hadamard Schur form */
+} HProb;
+
+enum { PREC_F64 = 0, PREC_MIXED = 1 };
+
+typedef struct {
+    double tol;             /* 1e-8 */
+    int maxit;              /* 100 */
+    int prec;               /* PREC_F64 or PREC_MIXED */
+    int verbose;
+    double mixed_until;     /* mixed: f32 factor while max(relgap, pinf, dinf) exceeds this (1e-6) */
+    double time_limit;      /* s, <= 0 none */
+} SolveOpts;
+
+typedef struct Solve {
+    HProb P;
+    SolveOpts o;
+    int id;
+    /* device heap offsets in doubles (block arena arrays have P.T elements) */
+    int64_t hbase, hsize;   /* the solve's heap region (bytes) */
+    int64_t X, Z, Zi, LX, LZ, LXi, LZi, dX, dZ, G, Q, R0, F, W1, W2, C, DinvX, DinvZ, Xb;
+    int64_t lzQ, lzw, lzab;
+    int64_t M, MF, MLi, MLiT, MDinv, Mwork, Dm;
+    int64_t y, dy, rp, rhs, a0, h, qv, ev, wv, bvec, pr, pz, pp, pq, pxb, t1, t2, yb;
+    int64_t sc;             /* 64 scalars */
+    int64_t cptr, ent, blk, pptr, ppos, pcon, pval, dinvoff; /* problem data on the device */
+    /* factor state of M */
+    int f32;                /* current factor is f32 */
+    int want_double;        /* sticky: f32 no longer used */
+    double reg, reg_ok;
+    int reg_hint;
+    /* iteration state */
+    int it, status, done, stall, it_best;
+    double theta, r0norm, best_score, alp, ald;
+    double pobj, dobj, pinf, dinf, relgap, relcomp, mu, score;
+    double err_it[7];
+    double t_start, t_solve;
+    int pcg_total, nfactor_f32, nfactor_f64;
+    /* results (data as read) */
+    double res_pobj, res_dobj, res_err[6];
+    double *res_y, *res_X, *res_Z;  /* y (m), X and Z packed per input block */
+    /* trace */
+    double tr_pobj[128], tr_dobj[128], tr_mu[128];
+} Solve;
+
+/* problem.c */
+int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64_t nnz, const int *mat, const int *blk,
+                const int *ii, const int *jj, const double *v);
+void hprob_free(HProb *P);
+int64_t solve_heap_bytes(const HProb *P);   /* device memory a solve needs */
+
+/* heap allocator (host-managed, first fit) */
+int heap_setup(size_t bytes);
+int64_t heap_alloc(size_t bytes);           /* byte offset, 256-aligned; -1 when full */
+void heap_free(int64_t off);
+size_t heap_largest_free(void);
+
+/* ipm.c */
+int solve_setup(Solve *S);                  /* allocate device arrays and upload the problem */
+void solve_release(Solve *S);
+int ipm_run_batch(Solve **S, int ns);       /* runs every solve to its end */
+/* stepwise interface for the serving loop: start, one iteration for a batch, finish */
+int ipm_start_batch(Solve **S, int ns);
+int ipm_iterate_batch(Solve **S, int ns);   /* one iteration; sets done on solves that end */
+int ipm_end_batch(Solve **S, int ns);       /* DIMACS on the device, status, results to the host */
+int ipm_finish(Solve *S);                   /* results of one solve to the host (after ipm_end_batch's DIMACS) */
+void solve_log(const char *fmt, ...);
+void ipm_release_scratch(void);           /* frees the batch scratch (before the heap is re-sized) */
+
+#endif
