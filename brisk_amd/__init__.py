@@ -69,6 +69,10 @@ def load_library(path=None):
     L.brisk_serve_take.argtypes, L.brisk_serve_take.restype = [P, I, P, PD], I
     L.brisk_serve_inflight.argtypes, L.brisk_serve_inflight.restype = [P], I
     L.brisk_serve_delete.argtypes, L.brisk_serve_delete.restype = [P], None
+    L.bk_prof_enable.argtypes, L.bk_prof_enable.restype = [I], None
+    L.bk_prof_reset.argtypes, L.bk_prof_reset.restype = [], None
+    L.bk_prof_read.argtypes, L.bk_prof_read.restype = [I, PD], None
+    L.bk_device_name.restype = ctypes.c_char_p
     _LIB = L
     return L
 
@@ -94,7 +98,7 @@ class Result:
     X: Optional[List[np.ndarray]] = None
     Z: Optional[List[np.ndarray]] = None
     cause: str = ""
-    phases: Optional[np.ndarray] = None      # host setup + upload, iterations, DIMACS + download (s)
+    phases: Optional[np.ndarray] = None      # host setup + upload, iterations, DIMACS, download (s)
     counts: Optional[List[int]] = None       # iterations with an f32 factor, with an f64 factor, PCG steps
     trace: Optional[dict] = None             # pobj, dobj, mu per iteration
 
@@ -127,7 +131,7 @@ def _collect(L, r):
     bs = [L.brisk_result_blocksize(r, k) for k in range(nb)]
     err = np.zeros(6)
     L.brisk_result_dimacs(r, err.ctypes.data_as(PD))
-    ph = np.zeros(3)
+    ph = np.zeros(4)
     L.brisk_result_phases(r, ph.ctypes.data_as(PD))
     cn = (I * 3)()
     L.brisk_result_counts(r, cn)
@@ -230,6 +234,24 @@ def solve_batch(problems, options=None, verbose=False):
     finally:
         for k in range(len(arrs)):
             L.brisk_result_delete(res[k])
+
+
+def profile(enable=True):
+    """Per-kernel-class event timing on the library stream (for measurement)."""
+    load_library().bk_prof_enable(1 if enable else 0)
+
+
+def profile_reset():
+    load_library().bk_prof_reset()
+
+
+def profile_read(cls):
+    """{ms, launches, flops, bytes} of a kernel class (see dev.h: 0-3 f64 GEMM by transposes,
+    4-7 f32 GEMM, 8 Cholesky diagonal blocks, 9 GEMV, 10 Schur assembly, 11 A(.), 12 Lanczos,
+    13 dot products, 14 linear combinations)."""
+    out = np.zeros(4)
+    load_library().bk_prof_read(cls, out.ctypes.data_as(PD))
+    return dict(ms=out[0], launches=out[1], flops=out[2], bytes=out[3])
 
 
 class Server:

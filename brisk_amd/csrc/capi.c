@@ -22,7 +22,7 @@ struct BriskResult {
     double pobj, dobj, err[6], time;
     double *y, *Xp, *Zp;
     double **X, **Z;
-    double phases[3];
+    double phases[4];
     int counts[3];
     int ntrace;
     double tr[3][128];
@@ -93,7 +93,7 @@ API const double *brisk_result_X(const BriskResult *r, int k) { return r && r->h
 API const double *brisk_result_Z(const BriskResult *r, int k) { return r && r->Z && k >= 0 && k < r->nblk ? r->Z[k] : NULL; }
 API const char *brisk_result_cause(const BriskResult *r) { return r ? r->cause : ""; }
 API int brisk_result_resolves(const BriskResult *r, double *t) { (void)r; if (t) *t = 0; return 0; }
-API void brisk_result_phases(const BriskResult *r, double *out) { for (int k = 0; k < 3; k++) out[k] = r ? r->phases[k] : 0; }
+API void brisk_result_phases(const BriskResult *r, double *out) { for (int k = 0; k < 4; k++) out[k] = r ? r->phases[k] : 0; }
 API void brisk_result_counts(const BriskResult *r, int *out) { for (int k = 0; k < 3; k++) out[k] = r ? r->counts[k] : 0; }
 API int brisk_result_trace(const BriskResult *r, double *po, double *dob, double *mu, int n) {
     if (!r) return 0;
@@ -143,7 +143,7 @@ int brisk_heap_ensure(size_t need) {
     return heap_setup(want);
 }
 
-void brisk_fill_result(BriskResult *r, Solve *s, double t_setup, double t_iter, double t_end) {
+void brisk_fill_result(BriskResult *r, Solve *s, double t_setup, double t_iter, double t_dimacs, double t_down) {
     result_clear(r);
     const HProb *P = &s->P;
     r->status = s->status;
@@ -155,8 +155,8 @@ void brisk_fill_result(BriskResult *r, Solve *s, double t_setup, double t_iter, 
     memcpy(r->bs, P->bs_in, sizeof(int) * P->nblk_in);
     r->pobj = s->res_pobj; r->dobj = s->res_dobj;
     for (int e = 0; e < 6; e++) r->err[e] = s->res_err[e];
-    r->phases[0] = t_setup; r->phases[1] = t_iter; r->phases[2] = t_end;
-    r->time = t_setup + t_iter + t_end;
+    r->phases[0] = t_setup; r->phases[1] = t_iter; r->phases[2] = t_dimacs; r->phases[3] = t_down;
+    r->time = t_setup + t_iter + t_dimacs + t_down;
     r->counts[0] = s->nfactor_f32; r->counts[1] = s->nfactor_f64; r->counts[2] = s->pcg_total;
     r->ntrace = s->it + 1 < 128 ? s->it + 1 : 128;
     for (int k = 0; k < r->ntrace; k++) { r->tr[0][k] = s->tr_pobj[k]; r->tr[1][k] = s->tr_dobj[k]; r->tr[2][k] = s->tr_mu[k]; }
@@ -216,9 +216,12 @@ static int solve_many(int n, const BriskProblem *pr, int nopt, const char *const
     }
     bk_sync();
     const double t2 = bk_now();
-    if (ipm_end_batch(SP, n)) { rc = -4; goto out; }
+    if (ipm_dimacs(SP, n)) { rc = -4; goto out; }
+    bk_sync();
     const double t3 = bk_now();
-    for (int q = 0; q < n; q++) if (res[q]) brisk_fill_result(res[q], &S[q], t1 - t0, t2 - t1, t3 - t2);
+    for (int q = 0; q < n; q++) if (ipm_finish(SP[q])) { rc = -4; goto out; }
+    const double t4 = bk_now();
+    for (int q = 0; q < n; q++) if (res[q]) brisk_fill_result(res[q], &S[q], t1 - t0, t2 - t1, t3 - t2, (t4 - t3) / n);
     if (bk_failed()) rc = -4;
 out:
     if (rc < 0 && bk_failed()) say(1, "brisk: GPU error: %s\n", bk_error());

@@ -34,6 +34,16 @@ void bk_event_record(void *e);
 float bk_event_ms(void *a, void *b);
 void bk_event_free(void *e);
 const char *bk_device_name(void);
+/* per-class kernel timing (events around instrumented launches on the library stream):
+ * out4 = { total ms, launches, algorithmic flops, algorithmic bytes } of the class */
+void bk_prof_enable(int on);
+void bk_prof_reset(void);
+int bk_prof_begin(void);
+void bk_prof_end(int tok, int cls, double flops, double bytes);
+void bk_prof_read(int cls, double *out4);
+/* classes: 0-3 k_gemm<double, ta, tb> (2 ta + tb), 4-7 k_gemm<float, ta, tb>, 8 k_potrf_diag,
+ * 9 k_gemv, 10 Schur assembly, 11 constraint operators, 12 Lanczos steps, 13 dot products,
+ * 14 linear combinations */
 
 /* ---- batched GEMM: C_it = alpha op(A_it) op(B_it) + beta C_it ---- */
 typedef struct {

@@ -1194,7 +1194,7 @@ out:
  *   A(X) - b = du o (A~(X~) - b~), <C,X> = bs cs <C~,X~>, b'y = bs cs b~'y~, <X,Z> = bs cs <X~, Z~>.
  * lambda_min(X) and lambda_min(Z) are tested by Cholesky: a factor that exists proves the block
  * positive definite (error 0); otherwise the Lanczos bound gives the violation. */
-static int ipm_dimacs(Solve **S, int ns) {
+int ipm_dimacs(Solve **S, int ns) {
     int rc = -1;
     int *act = calloc(ns, sizeof(int)), *okx = calloc(ns, sizeof(int)), *okz = calloc(ns, sizeof(int));
     double *lam = calloc(ns, sizeof(double));

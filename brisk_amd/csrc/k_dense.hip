@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <math.h>
 #include <vector>
 #include "dev.h"
 #include "rt_internal.h"
@@ -116,6 +117,17 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmItem *__restrict__ items
             }
 }
 
+// algorithmic flops of one item: 2 per multiply-add where both operands can be nonzero
+static double gemm_flops(const GemmItem &g) {
+    const double M = g.M, N = g.N, K = g.K;
+    if (g.tri & 1) return M * (M + 1) * K;                                   // C lower only (M == N)
+    if ((g.tri & 2) && (g.tri & 8)) return 2.0 * K * K * K / 3.0 + K * K;   // k >= max(i, j), M == N == K
+    if (g.tri & 4) return N * M * (M + 1);                                   // op(A) lower, K == M
+    if (g.tri & 16) return M * N * (N + 1);                                  // op(B) upper, K == N
+    if (g.tri & 8) return 2.0 * M * N * K - M * N * (N - 1);                 // op(B) lower
+    if (g.tri & 2) return 2.0 * M * N * K - M * N * (M - 1);                 // op(A) upper
+    return 2.0 * M * N * K;
+}
 template <typename T>
 static int gemm_launch(int ta, int tb, T alpha, T beta, const T *A, const T *B, T *C, const GemmItem *items, int nitems, const int *info) {
     if (nitems <= 0) return 0;
@@ -128,10 +140,16 @@ static int gemm_launch(int ta, int tb, T alpha, T beta, const T *A, const T *B, 
     const GemmItem *d = (const GemmItem *)bk_stage(items, sizeof(GemmItem) * nitems);
     if (!d) return -1;
     dim3 grid(maxt, nitems < 65535 ? nitems : 65535, (nitems + 65534) / 65535);
+    const int tok = bk_prof_begin();
     if (!ta && !tb) hipLaunchKernelGGL((k_gemm<T, 0, 0>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info);
     else if (!ta && tb) hipLaunchKernelGGL((k_gemm<T, 0, 1>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info);
     else if (ta && !tb) hipLaunchKernelGGL((k_gemm<T, 1, 0>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info);
     else hipLaunchKernelGGL((k_gemm<T, 1, 1>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info);
+    if (tok >= 0) {
+        double fl = 0, by = 0;
+        for (int q = 0; q < nitems; q++) { fl += gemm_flops(items[q]); by += (double)sizeof(T) * ((double)items[q].M * items[q].K + (double)items[q].K * items[q].N + 2.0 * items[q].M * items[q].N); }
+        bk_prof_end(tok, (sizeof(T) == 8 ? 0 : 4) + 2 * ta + tb, fl, by);
+    }
     return BK_LAUNCH_CHECK("k_gemm");
 }
 extern "C" int bk_gemm_f64(int ta, int tb, double alpha, double beta, const double *A, const double *B, double *C,
@@ -257,7 +275,9 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
     }
     std::vector<GemmItem> gi, tr;
     for (int k0 = 0; k0 < nmax; k0 += 64) {
+        const int tok = bk_prof_begin();
         hipLaunchKernelGGL((k_potrf_diag<T>), dim3(n_it), dim3(256), 0, g_bk_stream, d_it, n_it, k0, A, Dinv, info, pmin);
+        if (tok >= 0) { double fl = 0; for (int q = 0; q < n_it; q++) { const double b = it[q].n > k0 ? fmin(64.0, it[q].n - k0) : 0; fl += b * b * b / 3.0 + b * b * b / 3.0; } bk_prof_end(tok, 8, fl, 0); }
         if (BK_LAUNCH_CHECK("k_potrf_diag")) return -1;
         if (k0 + 64 >= nmax) break;
         gi.clear(); tr.clear();
@@ -419,8 +439,14 @@ static int gemv_run(int trans, TACC alpha, TACC beta, const TA *A, const TX *x, 
     const GemvItem *d = (const GemvItem *)bk_stage(it, sizeof(GemvItem) * n_it);
     if (!d) return -1;
     dim3 grid(mx, n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    const int tok = bk_prof_begin();
     if (trans) hipLaunchKernelGGL((k_gemv_t<TA, TX, TY, TACC>), grid, dim3(256), 0, g_bk_stream, d, n_it, A, x, y, alpha, beta);
     else hipLaunchKernelGGL((k_gemv_n<TA, TX, TY, TACC>), grid, dim3(256), 0, g_bk_stream, d, n_it, A, x, y, alpha, beta);
+    if (tok >= 0) {
+        double fl = 0, by = 0;
+        for (int q = 0; q < n_it; q++) { fl += 2.0 * it[q].M * it[q].N; by += (double)sizeof(TA) * it[q].M * it[q].N + sizeof(TX) * (double)(trans ? it[q].M : it[q].N) + sizeof(TY) * (double)(trans ? it[q].N : it[q].M) * (beta != (TACC)0 ? 2 : 1); }
+        bk_prof_end(tok, 9, fl, by);
+    }
     return BK_LAUNCH_CHECK("k_gemv");
 }
 extern "C" int bk_gemv_f64(int trans, double alpha, double beta, const double *A, const double *x, double *y, const GemvItem *it, int n_it) {

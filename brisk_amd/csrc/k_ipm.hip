@@ -91,7 +91,9 @@ extern "C" int bk_dots(const DotItem *it, int n_it) {
     const int64_t *dcb = (const int64_t *)bk_stage(cb.data(), sizeof(int64_t) * n_it);
     if (!d || !dcb) return -1;
     dim3 grid((unsigned)mx, n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    const int tok = bk_prof_begin();
     hipLaunchKernelGGL(k_dot_chunks, grid, dim3(256), 0, g_bk_stream, d, dcb, n_it, (const double *)g_heap, g_partial);
+    if (tok >= 0) { double by = 0; for (int q = 0; q < n_it; q++) by += 16.0 * it[q].len; bk_prof_end(tok, 13, 0, by); }
     if (BK_LAUNCH_CHECK("k_dot_chunks")) return -1;
     hipLaunchKernelGGL(k_dot_final, dim3((n_it + 255) / 256), dim3(256), 0, g_bk_stream, d, dcb, n_it, (double *)g_heap, g_partial);
     return BK_LAUNCH_CHECK("k_dot_final");
@@ -117,7 +119,9 @@ extern "C" int bk_lincomb(const LinItem *it, int n_it) {
     const LinItem *d = (const LinItem *)bk_stage(it, sizeof(LinItem) * n_it);
     if (!d) return -1;
     dim3 grid(bk_grid1(mx, 256, 2048), n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    const int tok = bk_prof_begin();
     hipLaunchKernelGGL(k_lincomb, grid, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    if (tok >= 0) { double by = 0; for (int q = 0; q < n_it; q++) { int t = 1; for (int u = 0; u < 4; u++) t += it[q].in[u] >= 0; by += 8.0 * t * it[q].len; } bk_prof_end(tok, 14, 0, by); }
     return BK_LAUNCH_CHECK("k_lincomb");
 }
 
@@ -301,7 +305,9 @@ extern "C" int bk_aop(const ConItem *it, int n_it) {
     for (int q = 0; q < n_it; q++) mx = it[q].m > mx ? it[q].m : mx;
     const ConItem *d = (const ConItem *)bk_stage(it, sizeof(ConItem) * n_it);
     dim3 g((mx + 3) / 4, n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    const int tok = bk_prof_begin();
     hipLaunchKernelGGL(k_aop, g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    if (tok >= 0) bk_prof_end(tok, 11, 0, 0);
     return BK_LAUNCH_CHECK("k_aop");
 }
 
@@ -408,6 +414,7 @@ extern "C" int bk_schur(const ConItem *it, int n_it) {
         else { anyr = 1; const int64_t np = (int64_t)it[q].m * (it[q].m + 1) / 2; mxp = np > mxp ? np : mxp; }
     }
     const ConItem *d = (const ConItem *)bk_stage(it, sizeof(ConItem) * n_it);
+    const int tok = bk_prof_begin();
     if (anyd) {
         if (n_it > 65535 || mxd > 65535) return -1;
         hipLaunchKernelGGL(k_schur_diag, dim3((mxd + 255) / 256, mxd, n_it), dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
@@ -417,6 +424,15 @@ extern "C" int bk_schur(const ConItem *it, int n_it) {
         dim3 g((unsigned)((mxp + 255) / 256), n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
         hipLaunchKernelGGL(k_schur_r0, g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
         if (BK_LAUNCH_CHECK("k_schur_r0")) return -1;
+    }
+    if (tok >= 0) {
+        double by = 0, fl = 0;
+        for (int q = 0; q < n_it; q++) {
+            const double m = it[q].m;
+            if (it[q].diag) { by += 8.0 * m * m + 2.0 * 8.0 * m * m; fl += 3.0 * m * m; }   // X, Zi entries read once, M written
+            else { by += 8.0 * m * m; fl += 0; }
+        }
+        bk_prof_end(tok, 10, fl, by);
     }
     return 0;
 }
@@ -492,6 +508,8 @@ extern "C" int bk_lanczos_start(const LzItem *it, int n_it) {
 extern "C" int bk_lanczos_step(const LzItem *it, int n_it, int j) {
     if (n_it <= 0) return 0;
     const LzItem *d = (const LzItem *)bk_stage(it, sizeof(LzItem) * n_it);
+    const int tok = bk_prof_begin();
     hipLaunchKernelGGL(k_lz_step, dim3(n_it), dim3(256), 0, g_bk_stream, d, n_it, j, (double *)g_heap);
+    if (tok >= 0) bk_prof_end(tok, 12, 0, 0);
     return BK_LAUNCH_CHECK("k_lz_step");
 }

@@ -124,6 +124,58 @@ extern "C" float bk_event_ms(void *a, void *b) {
     return ms;
 }
 extern "C" void bk_event_free(void *e) { if (e) (void)hipEventDestroy((hipEvent_t)e); }
+// ---- optional per-class kernel timing: an event pair around each instrumented launch on the
+// library stream; read back (and summed per class) by bk_prof_read
+#define PROF_MAX 32768
+#define PROF_CLS 64
+static int g_prof_on = 0, g_prof_n = 0;
+static hipEvent_t g_prof_ev[2 * PROF_MAX];
+static int g_prof_cls[PROF_MAX];
+static double g_prof_flops[PROF_MAX], g_prof_bytes[PROF_MAX];
+static double g_acc_ms[PROF_CLS], g_acc_n[PROF_CLS], g_acc_flops[PROF_CLS], g_acc_bytes[PROF_CLS];
+static int g_prof_evinit = 0;
+static void prof_flush(void) {
+    if (!g_prof_n) return;
+    (void)hipStreamSynchronize(g_bk_stream);
+    for (int i = 0; i < g_prof_n; i++) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, g_prof_ev[2 * i], g_prof_ev[2 * i + 1]) == hipSuccess) {
+            const int c = g_prof_cls[i];
+            g_acc_ms[c] += ms; g_acc_n[c] += 1; g_acc_flops[c] += g_prof_flops[i]; g_acc_bytes[c] += g_prof_bytes[i];
+        }
+    }
+    g_prof_n = 0;
+}
+extern "C" void bk_prof_enable(int on) {
+    if (on && !g_prof_evinit) {
+        for (int i = 0; i < 2 * PROF_MAX; i++) (void)hipEventCreate(&g_prof_ev[i]);
+        g_prof_evinit = 1;
+    }
+    g_prof_on = on;
+}
+extern "C" void bk_prof_reset(void) {
+    prof_flush();
+    memset(g_acc_ms, 0, sizeof(g_acc_ms)); memset(g_acc_n, 0, sizeof(g_acc_n));
+    memset(g_acc_flops, 0, sizeof(g_acc_flops)); memset(g_acc_bytes, 0, sizeof(g_acc_bytes));
+}
+extern "C" int bk_prof_begin(void) {
+    if (!g_prof_on) return -1;
+    if (g_prof_n == PROF_MAX) prof_flush();
+    const int i = g_prof_n++;
+    (void)hipEventRecord(g_prof_ev[2 * i], g_bk_stream);
+    return i;
+}
+extern "C" void bk_prof_end(int tok, int cls, double flops, double bytes) {
+    if (tok < 0 || cls < 0 || cls >= PROF_CLS) return;
+    (void)hipEventRecord(g_prof_ev[2 * tok + 1], g_bk_stream);
+    g_prof_cls[tok] = cls; g_prof_flops[tok] = flops; g_prof_bytes[tok] = bytes;
+}
+extern "C" void bk_prof_read(int cls, double *out4) {
+    prof_flush();
+    if (cls < 0 || cls >= PROF_CLS) { out4[0] = out4[1] = out4[2] = out4[3] = 0; return; }
+    out4[0] = g_acc_ms[cls]; out4[1] = g_acc_n[cls]; out4[2] = g_acc_flops[cls]; out4[3] = g_acc_bytes[cls];
+}
+
 static char g_devname[256] = "";
 extern "C" const char *bk_device_name(void) {
     if (!g_devname[0]) {

@@ -16,7 +16,7 @@
 
 int brisk_parse_options(int nopt, const char *const *opts, SolveOpts *o, int m, int *slots, double *heapgb);
 int brisk_heap_ensure(size_t need);
-void brisk_fill_result(BriskResult *r, Solve *s, double t_setup, double t_iter, double t_end);
+void brisk_fill_result(BriskResult *r, Solve *s, double t_setup, double t_iter, double t_dimacs, double t_down);
 
 typedef struct {
     int state;                  /* 0 waiting, 1 in flight, 2 finished, 3 taken */
@@ -96,7 +96,7 @@ static void finish_done(BriskServe *sv) {
         Req *r = &sv->req[D[q]->id];
         r->res = brisk_result_new();
         if (bad) D[q]->status = 4;
-        brisk_fill_result(r->res, D[q], 0.0, t0 - r->t_admit, (t1 - t0) / nd);
+        brisk_fill_result(r->res, D[q], 0.0, t0 - r->t_admit, (t1 - t0) / nd, 0.0);
         r->t_done = bk_now();
         r->state = 2;
         solve_release(D[q]);

@@ -99,6 +99,7 @@ int ipm_run_batch(Solve **S, int ns);       /* runs every solve to its end */
 /* stepwise interface for the serving loop: start, one iteration for a batch, finish */
 int ipm_start_batch(Solve **S, int ns);
 int ipm_iterate_batch(Solve **S, int ns);   /* one iteration; sets done on solves that end */
+int ipm_dimacs(Solve **S, int ns);          /* DIMACS errors and status of the returned points (device) */
 int ipm_end_batch(Solve **S, int ns);       /* DIMACS on the device, status, results to the host */
 int ipm_finish(Solve *S);                   /* results of one solve to the host (after ipm_end_batch's DIMACS) */
 void solve_log(const char *fmt, ...);

@@ -64,8 +64,8 @@ int brisk_solve_file(const char *fname, int nopt, const char *const *opts, Brisk
  *   -prec double|mixed   FP64 throughout, or FP32-MFMA Schur factor refined in FP64 (default:
  *                        mixed when m >= 1000, the reference's own threshold, solver.c:13814)
  *   -tol t, -maxit k, -timelimit s, -v, -q; -threads k and -bm k are accepted and ignored. */
-/* time split of the last solve: out[0] host setup and upload, out[1] the iterations,
- * out[2] the DIMACS evaluation and the result download (s) */
+/* time split of the solve (s): out[0] host setup and upload, out[1] start point and iterations,
+ * out[2] DIMACS evaluation on the device, out[3] result download */
 void brisk_result_phases(const BriskResult *r, double *out);
 /* iterations that used the f32 / f64 factor, PCG steps */
 void brisk_result_counts(const BriskResult *r, int *out3);
