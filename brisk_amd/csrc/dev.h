@@ -68,6 +68,7 @@ typedef struct {
     int64_t off;             /* the matrix at base + off */
     int64_t off2;            /* potrf/trtri: diagonal-block inverses (64*64 per 64-wide panel) */
     int64_t off3;            /* potrf/trtri: scratch of n*n elements */
+    int64_t off4;            /* trtri: the inverse; transpose: the destination */
 } MItem;
 
 /* Cholesky of a set of matrices, blocked right-looking with 64-wide panels, in place on A (its
@@ -76,13 +77,13 @@ typedef struct {
  * info[slot] = first failing column + 1 (0: factored); pmin[slot] = smallest squared pivot. */
 int bk_potrf_f64(const MItem *it, int n_it, double *A, double *Dinv, double *W, int *info, double *pmin);
 int bk_potrf_f32(const MItem *it, int n_it, float *A, float *Dinv, float *W, int *info, float *pmin);
-/* Linv + off = L^{-1} (lower, strict upper zero) from the factor L + off and its diagonal-block
+/* Linv + off4 = L^{-1} (lower, strict upper zero) from the factor L + off and its diagonal-block
  * inverses (Dinv + off2); W + off3 scratch (n*n). Items with info[slot] != 0 are skipped. */
 int bk_trtri_f64(const MItem *it, int n_it, const double *L, const double *Dinv, double *Linv, double *W, const int *info);
 int bk_trtri_f32(const MItem *it, int n_it, const float *L, const float *Dinv, float *Linv, float *W, const int *info);
 
 /* ---- batched matrix-vector: y_it = alpha op(A_it) x_it + beta y_it ---- */
-typedef struct { int M, N, lda; int64_t a, x, y; } GemvItem;
+typedef struct { int M, N, lda; int64_t a, x, y; int tri; } GemvItem;   /* tri (trans = 1 only): 1 A lower (rows >= j), 2 A upper (rows <= j) */
 int bk_gemv_f64(int trans, double alpha, double beta, const double *A, const double *x, double *y, const GemvItem *it, int n_it);
 int bk_gemv_f32(int trans, float alpha, float beta, const float *A, const float *x, float *y, const GemvItem *it, int n_it);
 /* mixed: y (f64) = alpha op(A) x + beta y with A in f32, x in f64 (products and sums in f64) */
@@ -109,8 +110,8 @@ int bk_vmul(const LinItem *it, int n_it);
 
 /* per square matrix at heap offset off (n = order): */
 int bk_symmetrize(const MItem *it, int n_it);                       /* G = (G + G')/2 */
-int bk_transpose_f64(const MItem *it, int n_it, int64_t dst_delta); /* dst = off + delta: src' */
-int bk_transpose_f32(const MItem *it, int n_it, int64_t dst_delta);
+int bk_transpose_f64(const MItem *it, int n_it);                    /* matrix at off4 = (matrix at off)' */
+int bk_transpose_f32(const MItem *it, int n_it);
 /* A += s[slot] I, with s read from the heap at offset s_off (doubles) */
 int bk_add_diag(const MItem *it, int n_it, int64_t s_off);
 /* equilibration of M (full f64 at off): D[off2 + i] = 1/sqrt(M_ii) (1 when M_ii <= 0);
@@ -141,9 +142,10 @@ int bk_aty(const ConItem *it, int n_it);
 int bk_schur(const ConItem *it, int n_it);
 
 /* ---- Lanczos on explicit symmetric T (per block) ---- */
-/* step j for every item: given w = T q_j (at w), alpha_j = q_j.w; w -= alpha q_j + beta_{j-1} q_{j-1};
- * two passes of full reorthogonalisation against q_0..q_j; beta_j = |w|; q_{j+1} = w / beta_j.
- * Q holds the basis (K x n per item), ab the coefficients (alpha at ab, beta at ab + 32) */
+/* step j for every item: given w = T q_j (at w), two passes of classical Gram-Schmidt against
+ * q_0..q_j (all coefficients of a pass in one reduction; the first pass's coefficient of q_j is
+ * alpha_j); beta_j = |w|; q_{j+1} = w / beta_j. Q holds the basis (K x n per item), ab the
+ * coefficients (alpha at ab, beta at ab + 32). Items with j >= K are skipped. */
 typedef struct { int n, K; int64_t Q, w, ab; } LzItem;
 int bk_lanczos_step(const LzItem *it, int n_it, int j);
 /* q_0 = the fixed start vector of the reference (1 + 0.5 sin(1.7 i + 0.3), normalised) */

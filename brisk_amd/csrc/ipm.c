@@ -42,8 +42,9 @@ LIST(LzItem, LzL)
 static DotL g_dl; static LinL g_ll; static GemmL g_gl; static MItL g_ml; static GemvL g_vl; static ConL g_cl; static LzL g_zl;
 
 /* ---- batch scratch (device) ----------------------------------------------------------------- */
-static int64_t g_bsc = -1, g_bint = -1, g_bdbl = -1;   /* byte offsets */
-static int g_bsc_cap = 0, g_bint_cap = 0;
+static int64_t g_bsc = -1, g_bint = -1, g_bdbl = -1, g_bab = -1;   /* byte offsets */
+static int g_bsc_cap = 0, g_bint_cap = 0, g_bab_cap = 0;
+static double *g_hab = NULL;
 static double *g_hsc = NULL;                            /* host mirror of the batch scalars */
 static int *g_hint = NULL;
 static double *g_hdbl = NULL;
@@ -73,8 +74,18 @@ void ipm_release_scratch(void) {
     if (g_bsc >= 0) heap_free(g_bsc);
     if (g_bint >= 0) heap_free(g_bint);
     if (g_bdbl >= 0) heap_free(g_bdbl);
-    g_bsc = g_bint = g_bdbl = -1;
-    g_bsc_cap = g_bint_cap = 0;
+    if (g_bab >= 0) heap_free(g_bab);
+    g_bsc = g_bint = g_bdbl = g_bab = -1;
+    g_bsc_cap = g_bint_cap = g_bab_cap = 0;
+}
+/* Lanczos coefficients of every block of a batch: 64 doubles per block */
+static int batch_ab(int nblocks) {
+    if (nblocks <= g_bab_cap) return 0;
+    if (g_bab >= 0) heap_free(g_bab);
+    g_bab_cap = nblocks + 256;
+    g_bab = heap_alloc(sizeof(double) * 64 * g_bab_cap);
+    g_hab = realloc(g_hab, sizeof(double) * 64 * g_bab_cap);
+    return g_bab < 0 ? -1 : 0;
 }
 static inline int64_t SC(int q, int k) { return g_bsc / 8 + (int64_t)q * NSC + k; }  /* double offset of scalar k of batch slot q */
 static inline double HS(int q, int k) { return g_hsc[(size_t)q * NSC + k]; }
@@ -105,9 +116,9 @@ int64_t solve_heap_bytes(const HProb *P) {
     int64_t dinv = 0, lz = 0;
     for (int k = 0; k < P->nb; k++) { dinv += (int64_t)((P->bn[k] + 63) / 64) * 4096; lz += P->bn[k]; }
     int64_t b = 0;
-    b += 17 * (T * 8 + 256);                              /* block arena arrays */
+    b += 19 * (T * 8 + 256);                              /* block arena arrays */
     b += 2 * (dinv * 8 + 256);
-    b += (BK_LZ_K + 1) * lz * 8 + 64 * 8 * (int64_t)P->nb + 768;
+    b += 2 * (BK_LZ_K + 3) * lz * 8 + 1024;
     b += 5 * (m * m * 8 + 256) + (((m + 63) / 64) * 4096 * 8 + 256);
     b += 20 * (m * 8 + 256) + NSC * 8 + 256;
     b += (m + 1) * 4 + (int64_t)P->nent * (sizeof(SEnt) + 12) + (int64_t)P->nb * sizeof(BlkRef) + (P->npos + 1) * 12 + 2048;
@@ -128,11 +139,11 @@ int solve_setup(Solve *S) {
     const int64_t T = P->T, m = P->m;
 #define ALLOC(field, bytes) do { const int64_t o_ = heap_alloc(bytes); if (o_ < 0) fail = 1; else { A->allocs[A->nalloc++] = o_; S->field = o_ / 8; } } while (0)
 #define ALLOCB(field, bytes) do { const int64_t o_ = heap_alloc(bytes); if (o_ < 0) fail = 1; else { A->allocs[A->nalloc++] = o_; S->field = o_; } } while (0)
-    ALLOC(X, T * 8); ALLOC(Z, T * 8); ALLOC(Zi, T * 8); ALLOC(LX, T * 8); ALLOC(LZ, T * 8); ALLOC(LXi, T * 8); ALLOC(LZi, T * 8);
+    ALLOC(X, T * 8); ALLOC(Z, T * 8); ALLOC(Zi, T * 8); ALLOC(LX, T * 8); ALLOC(LZ, T * 8); ALLOC(LXi, T * 8); ALLOC(LZi, T * 8); ALLOC(LXiT, T * 8); ALLOC(LZiT, T * 8);
     ALLOC(dX, T * 8); ALLOC(dZ, T * 8); ALLOC(G, T * 8); ALLOC(Q, T * 8); ALLOC(R0, T * 8); ALLOC(F, T * 8); ALLOC(W1, T * 8);
     ALLOC(W2, T * 8); ALLOC(C, T * 8); ALLOC(Xb, T * 8);
     ALLOC(DinvX, dinv * 8); ALLOC(DinvZ, dinv * 8);
-    ALLOC(lzQ, (int64_t)BK_LZ_K * lz * 8); ALLOC(lzw, lz * 8); ALLOC(lzab, (int64_t)64 * P->nb * 8);
+    ALLOC(lzQ, 2 * (int64_t)BK_LZ_K * lz * 8); ALLOC(lzw, 2 * lz * 8); ALLOC(lzt1, 2 * lz * 8); ALLOC(lzt2, 2 * lz * 8);
     ALLOC(M, m * m * 8); ALLOC(MF, m * m * 8); ALLOC(MLi, m * m * 8); ALLOC(MLiT, m * m * 8); ALLOC(Mwork, m * m * 8);
     ALLOC(MDinv, ((m + 63) / 64) * 4096 * 8); ALLOC(Dm, m * 8);
     ALLOC(y, m * 8); ALLOC(dy, m * 8); ALLOC(rp, m * 8); ALLOC(rhs, m * 8); ALLOC(a0, m * 8); ALLOC(h, m * 8); ALLOC(qv, m * 8);
@@ -246,67 +257,80 @@ static int blk_sym(Solve **S, int ns, const int *act, size_t f) {
     }
     return bk_symmetrize(g_ml.v, g_ml.n);
 }
-/* Cholesky of every block of the active solves: L = V (copied), factored; ok[q] = 1 when every
- * block of solve q is positive definite */
-static int blk_chol(Solve **S, int ns, const int *act, size_t fV, size_t fL, size_t fDinv, int *ok) {
+/* Cholesky of every block of the active (solve, side) pairs in one batched factorisation:
+ * L = V (copied), factored in place; side s uses the fields fV[s], fL[s], fD[s] and the scratch
+ * fW[s]; act and ok are indexed side * ns + q (ok = 1 when every block is positive definite) */
+static int chol_sides(Solve **S, int ns, int nside, const int *act, const size_t *fV, const size_t *fL, const size_t *fD,
+                      const size_t *fW, int *ok) {
     LinL_clear(&g_ll);
-    for (int q = 0; q < ns; q++) if (!act || act[q]) lin_add(S[q]->P.T, F_(S[q], fL), F_(S[q], fV), 1.0, -1, 0, -1, 0, -1, 0);
+    for (int sd = 0; sd < nside; sd++)
+        for (int q = 0; q < ns; q++) if (!act || act[sd * ns + q]) lin_add(S[q]->P.T, F_(S[q], fL[sd]), F_(S[q], fV[sd]), 1.0, -1, 0, -1, 0, -1, 0);
     if (bk_lincomb(g_ll.v, g_ll.n)) return -1;
     MItL_clear(&g_ml);
     int slot = 0;
-    int *first = malloc(sizeof(int) * (ns + 1));
-    for (int q = 0; q < ns; q++) {
-        first[q] = slot;
-        if (act && !act[q]) continue;
-        const HProb *P = &S[q]->P;
-        SAux *A = aux_of(S[q]);
-        for (int k = 0; k < P->nb; k++) {
-            MItem it = { P->bn[k], slot++, F_(S[q], fL) + P->boff[k], F_(S[q], fDinv) + A->dboff[k], S[q]->W2 + P->boff[k] };
-            MItL_add(&g_ml, it);
+    int *first = malloc(sizeof(int) * (nside * ns + 1));
+    for (int sd = 0; sd < nside; sd++)
+        for (int q = 0; q < ns; q++) {
+            first[sd * ns + q] = slot;
+            if (act && !act[sd * ns + q]) continue;
+            const HProb *P = &S[q]->P;
+            SAux *A = aux_of(S[q]);
+            for (int k = 0; k < P->nb; k++) {
+                MItem it = { P->bn[k], slot++, F_(S[q], fL[sd]) + P->boff[k], F_(S[q], fD[sd]) + A->dboff[k], F_(S[q], fW[sd]) + P->boff[k], 0 };
+                MItL_add(&g_ml, it);
+            }
         }
-    }
-    first[ns] = slot;
+    first[nside * ns] = slot;
     if (batch_scratch(ns, slot + 1)) { free(first); return -1; }
     bk_zero(dint(), sizeof(int) * (slot + 1));
     double *H = heapp();
     if (bk_potrf_f64(g_ml.v, g_ml.n, H, H, H, dint(), ddbl())) { free(first); return -1; }
     if (bk_d2h(g_hint, dint(), sizeof(int) * (slot + 1))) { free(first); return -1; }
-    for (int q = 0; q < ns; q++) {
-        if (act && !act[q]) { ok[q] = 1; continue; }
-        ok[q] = 1;
-        for (int s = first[q]; s < (q + 1 < ns ? first[q + 1] : slot); s++) if (g_hint[s]) ok[q] = 0;
+    for (int u = 0; u < nside * ns; u++) {
+        ok[u] = 1;
+        if (act && !act[u]) continue;
+        for (int t = first[u]; t < first[u + 1]; t++) if (g_hint[t]) ok[u] = 0;
     }
     free(first);
     return 0;
 }
-/* Li = L^{-1} per block (after a successful blk_chol) */
-static int blk_inv(Solve **S, int ns, const int *act, size_t fL, size_t fDinv, size_t fLi) {
+static int blk_chol(Solve **S, int ns, const int *act, size_t fV, size_t fL, size_t fDinv, int *ok) {
+    const size_t a[1] = { fV }, b[1] = { fL }, c[1] = { fDinv }, w[1] = { OFF(W2) };
+    return chol_sides(S, ns, 1, act, a, b, c, w, ok);
+}
+/* Li = L^{-1} and LiT = Li' per block for the active (solve, side) pairs (after a successful
+ * factorisation), all in one batched triangular inversion and one transpose */
+static int inv_sides(Solve **S, int ns, int nside, const int *act, const size_t *fL, const size_t *fD, const size_t *fLi,
+                     const size_t *fLiT, const size_t *fW) {
     MItL_clear(&g_ml);
-    for (int q = 0; q < ns; q++) {
-        if (act && !act[q]) continue;
-        const HProb *P = &S[q]->P;
-        SAux *A = aux_of(S[q]);
-        for (int k = 0; k < P->nb; k++) {
-            MItem it = { P->bn[k], 0, F_(S[q], fL) + P->boff[k], F_(S[q], fDinv) + A->dboff[k], S[q]->W2 + P->boff[k] };
-            MItL_add(&g_ml, it);
+    for (int sd = 0; sd < nside; sd++)
+        for (int q = 0; q < ns; q++) {
+            if (act && !act[sd * ns + q]) continue;
+            const HProb *P = &S[q]->P;
+            SAux *A = aux_of(S[q]);
+            for (int k = 0; k < P->nb; k++) {
+                MItem it = { P->bn[k], 0, F_(S[q], fL[sd]) + P->boff[k], F_(S[q], fD[sd]) + A->dboff[k], F_(S[q], fW[sd]) + P->boff[k],
+                             F_(S[q], fLi[sd]) + P->boff[k] };
+                MItL_add(&g_ml, it);
+            }
         }
-    }
+    if (!g_ml.n) return 0;
     if (batch_scratch(ns, 1)) return -1;
     bk_zero(dint(), sizeof(int));
     double *H = heapp();
-    int c = 0;
-    /* trtri reads L and writes Linv at the same item offsets of two base pointers: L and Linv of
-     * a solve differ by a per-solve shift, so one launch sequence per solve */
-    int rc = 0;
-    for (int q = 0; q < ns; q++) {
-        if (act && !act[q]) continue;
-        const HProb *P = &S[q]->P;
-        const int64_t shift = F_(S[q], fLi) - F_(S[q], fL);
-        rc |= bk_trtri_f64(g_ml.v + c, P->nb, H, H, H + shift, H, dint());
-        c += P->nb;
+    if (bk_trtri_f64(g_ml.v, g_ml.n, H, H, H, H, dint())) return -1;
+    /* the transposes: every product with Li then reads columns (one wave per output) */
+    for (int t = 0, c = 0; t < nside * ns; t++) {
+        const int sd = t / ns, q = t % ns;
+        if (act && !act[t]) continue;
+        for (int k = 0; k < S[q]->P.nb; k++, c++) { g_ml.v[c].off = F_(S[q], fLi[sd]) + S[q]->P.boff[k]; g_ml.v[c].off4 = F_(S[q], fLiT[sd]) + S[q]->P.boff[k]; }
     }
-    return rc;
+    return bk_transpose_f64(g_ml.v, g_ml.n);
 }
+static const size_t F_V[2] = { offsetof(Solve, X), offsetof(Solve, Z) }, F_L[2] = { offsetof(Solve, LX), offsetof(Solve, LZ) },
+                    F_D[2] = { offsetof(Solve, DinvX), offsetof(Solve, DinvZ) }, F_Li[2] = { offsetof(Solve, LXi), offsetof(Solve, LZi) },
+                    F_LiT[2] = { offsetof(Solve, LXiT), offsetof(Solve, LZiT) }, F_W[2] = { offsetof(Solve, W2), offsetof(Solve, G) },
+                    F_dV[2] = { offsetof(Solve, dX), offsetof(Solve, dZ) };
 
 /* ---- Lanczos step-length bound (reference solver.c:6083-6225) ----------------------------- */
 /* smallest eigenvalue of the k x k tridiagonal (a, b) and the last component of its eigenvector,
@@ -345,77 +369,119 @@ static void tri_min(int k, const double *a, const double *b, double *theta, doub
     *theta = Am[im + im * k];
     *zlast = V[(k - 1) + im * k];
 }
-/* alpha_max per solve for V + alpha dV >= 0 given Li = chol(V)^{-1}: Lanczos on Li dV Li' */
-static int maxstep(Solve **S, int ns, const int *act, size_t fLi, size_t fdV, double *amax) {
-    /* T = Li dV Li' into W2 (W1 = Li dV first) */
-    if (blk_gemm(S, ns, act, 0, 0, 4, 1.0, 0.0, fLi, fdV, OFF(W1))) return -1;
-    if (blk_gemm(S, ns, act, 0, 1, 16, 1.0, 0.0, OFF(W1), fLi, OFF(W2))) return -1;
-    if (blk_sym(S, ns, act, OFF(W2))) return -1;
+/* Lanczos on T = Li V Li' (operator form: three products per step, Li and its transpose
+ * stored) or on T = V (fLi == 0), per block of every active solve, for nside independent sides
+ * (the primal and dual steps run in the same launches); every 3 steps from the 6th the
+ * coefficients come back and a block whose extreme Ritz value is resolved to lz_tol stops
+ * (reference solver.c:6083-6225). out[side * ns + q] = min over the blocks of solve q of the
+ * bound theta - res of lambda_min(T) (want_step == 0) or the step 1/max(0, -lambda) (1). */
+static int lanczos_min(Solve **S, int ns, const int *act, int nside, const size_t *fLi, const size_t *fLiT, const size_t *fV,
+                       int want_step, double lz_tol, double *out) {
     LzL_clear(&g_zl);
-    int kmax = 0;
-    for (int q = 0; q < ns; q++) {
-        if (act && !act[q]) continue;
-        const HProb *P = &S[q]->P;
-        SAux *A = aux_of(S[q]);
-        for (int k = 0; k < P->nb; k++) {
-            const int n = P->bn[k], K = n < BK_LZ_K ? n : BK_LZ_K;
-            LzItem z = { n, K, S[q]->lzQ + (int64_t)BK_LZ_K * A->lzoff[k], S[q]->lzw + A->lzoff[k], S[q]->lzab + 64 * k };
-            LzL_add(&g_zl, z);
-            if (K > kmax) kmax = K;
-        }
-    }
-    if (bk_lanczos_start(g_zl.v, g_zl.n)) return -1;
-    double *H = heapp();
-    for (int j = 0; j < kmax; j++) {
-        GemvL_clear(&g_vl);
-        int c = 0;
+    int kmax = 0, nbt = 0;
+    for (int q = 0; q < ns; q++) if (!act || act[q]) nbt += nside * S[q]->P.nb;
+    if (batch_ab(nbt + 1)) return -1;
+    int *kq = malloc(sizeof(int) * (nbt + 1)), *kb = malloc(sizeof(int) * (nbt + 1)), *ks = malloc(sizeof(int) * (nbt + 1));
+    int *live = malloc(sizeof(int) * (nbt + 1));
+    double *lam = malloc(sizeof(double) * (nbt + 1));
+    for (int side = 0; side < nside; side++)
         for (int q = 0; q < ns; q++) {
             if (act && !act[q]) continue;
             const HProb *P = &S[q]->P;
-            for (int k = 0; k < P->nb; k++, c++) {
+            SAux *A = aux_of(S[q]);
+            int64_t lzn = 0;
+            for (int k = 0; k < P->nb; k++) lzn += P->bn[k];
+            for (int k = 0; k < P->nb; k++) {
+                const int n = P->bn[k], K = n < BK_LZ_K ? n : BK_LZ_K;
+                LzItem z = { n, K, S[q]->lzQ + (int64_t)BK_LZ_K * (side * lzn + A->lzoff[k]), S[q]->lzw + side * lzn + A->lzoff[k],
+                             g_bab / 8 + 64 * (int64_t)g_zl.n };
+                kq[g_zl.n] = q; kb[g_zl.n] = k; ks[g_zl.n] = side; live[g_zl.n] = 1; lam[g_zl.n] = 0;
+                LzL_add(&g_zl, z);
+                if (K > kmax) kmax = K;
+            }
+        }
+    const int nz = g_zl.n;
+    int rc = -1;
+    double *H = heapp();
+    LzItem *zz = malloc(sizeof(LzItem) * (nz + 1));
+    if (bk_lanczos_start(g_zl.v, nz)) goto out;
+    for (int j = 0; j < kmax; j++) {
+        /* w = T q_j for the live blocks */
+        const int nstage = fLi[0] ? 3 : 1;
+        for (int stage = 0; stage < nstage; stage++) {
+            GemvL_clear(&g_vl);
+            for (int c = 0; c < nz; c++) {
                 const LzItem *z = &g_zl.v[c];
-                if (j >= z->K) continue;
-                GemvItem g = { z->n, z->n, z->n, S[q]->W2 + P->boff[k], z->Q + (int64_t)j * z->n, z->w };
+                if (!live[c] || j >= z->K) continue;
+                Solve *s = S[kq[c]];
+                SAux *A = aux_of(s);
+                const int k = kb[c], sd = ks[c];
+                int64_t lzn = 0;
+                for (int b = 0; b < s->P.nb; b++) lzn += s->P.bn[b];
+                const int64_t bo = s->P.boff[k], qj = z->Q + (int64_t)j * z->n;
+                const int64_t t1 = s->lzt1 + sd * lzn + A->lzoff[k], t2 = s->lzt2 + sd * lzn + A->lzoff[k];
+                GemvItem g;
+                g.M = g.N = g.lda = z->n;
+                if (!fLi[sd]) { g.a = F_(s, fV[sd]) + bo; g.x = qj; g.y = z->w; g.tri = 0; }
+                else if (stage == 0) { g.a = F_(s, fLi[sd]) + bo; g.x = qj; g.y = t1; g.tri = 1; }     /* t1 = Li' q */
+                else if (stage == 1) { g.a = F_(s, fV[sd]) + bo; g.x = t1; g.y = t2; g.tri = 0; }      /* t2 = V t1 */
+                else { g.a = F_(s, fLiT[sd]) + bo; g.x = t2; g.y = z->w; g.tri = 2; }                  /* w = Li t2 */
                 GemvL_add(&g_vl, g);
             }
+            if (bk_gemv_f64(1, 1.0, 0.0, H, H, H, g_vl.v, g_vl.n)) goto out;
         }
-        if (bk_gemv_f64(1, 1.0, 0.0, H, H, H, g_vl.v, g_vl.n)) return -1;
-        if (bk_lanczos_step(g_zl.v, g_zl.n, j)) return -1;
-    }
-    /* coefficients back: per solve its lzab block (64 per block) */
-    int c = 0;
-    for (int q = 0; q < ns; q++) {
-        if (act && !act[q]) { c += 0; continue; }
-        const HProb *P = &S[q]->P;
-        double *ab = malloc(sizeof(double) * 64 * P->nb);
-        if (bk_d2h(ab, H + S[q]->lzab, sizeof(double) * 64 * P->nb)) { free(ab); return -1; }
-        double a = 1e30;
-        for (int k = 0; k < P->nb; k++, c++) {
-            const int K = g_zl.v[c].K;
-            const double *al = ab + 64 * k, *be = al + 32;
-            int kk = K;
+        int nl = 0;
+        for (int c = 0; c < nz; c++) if (live[c] && j < g_zl.v[c].K) zz[nl++] = g_zl.v[c];
+        if (bk_lanczos_step(zz, nl, j)) goto out;
+        const int kk = j + 1;
+        const int check = kk == kmax || (kk >= 6 && kk % 3 == 0);
+        if (!check) continue;
+        if (bk_d2h(g_hab, H + g_bab / 8, sizeof(double) * 64 * nz)) goto out;
+        int any = 0;
+        for (int c = 0; c < nz; c++) {
+            if (!live[c]) continue;
+            const LzItem *z = &g_zl.v[c];
+            const int K = z->K, kc = kk < K ? kk : K;
+            const double *al = g_hab + 64 * c, *be = al + 32;
+            int m = kc, brk = 0;
             double scale = 0;
-            for (int j = 0; j < K; j++) {
-                if (fabs(al[j]) > scale) scale = fabs(al[j]);
-                if (!(be[j] > 1e-12 * (scale + 1e-300))) { kk = j + 1; break; }
+            for (int t = 0; t < kc; t++) {
+                if (fabs(al[t]) > scale) scale = fabs(al[t]);
+                if (!(be[t] > 1e-12 * (scale + 1e-300))) { m = t + 1; brk = 1; break; }
             }
             int bad = 0;
-            for (int j = 0; j < kk; j++) if (!isfinite(al[j]) || !isfinite(be[j])) bad = 1;
-            double ak;
-            if (bad) ak = 0;
-            else {
-                double th, zl;
-                tri_min(kk, al, be, &th, &zl);
-                const double res = (kk == K && kk < P->bn[k]) || (kk < K && be[kk - 1] > 1e-12 * (scale + 1e-300)) ? fabs(be[kk - 1] * zl) : 0.0;
-                const double lam = th - res;
-                ak = lam >= 0 ? 1e30 : -1.0 / lam;
-            }
-            if (ak < a) a = ak;
+            for (int t = 0; t < m; t++) if (!isfinite(al[t]) || !isfinite(be[t])) bad = 1;
+            if (bad) { lam[c] = -1e300; live[c] = 0; continue; }
+            double th, zl;
+            tri_min(m, al, be, &th, &zl);
+            const double res = (brk || m >= z->n) ? 0.0 : fabs(be[m - 1] * zl);
+            lam[c] = th - res;
+            if (brk || kc >= K || res <= lz_tol * fabs(th) + 1e-14 * scale) live[c] = 0;
+            else any = 1;
         }
-        free(ab);
-        amax[q] = a;
+        if (!any) break;
     }
-    return 0;
+    for (int side = 0; side < nside; side++)
+        for (int q = 0; q < ns; q++) if (!act || act[q]) out[side * ns + q] = want_step ? 1e30 : INFINITY;
+    for (int c = 0; c < nz; c++) {
+        double *o = &out[ks[c] * ns + kq[c]];
+        if (want_step) { const double a = lam[c] >= 0 ? 1e30 : (lam[c] <= -1e299 ? 0.0 : -1.0 / lam[c]); if (a < *o) *o = a; }
+        else if (lam[c] < *o) *o = lam[c];
+    }
+    rc = 0;
+out:
+    free(kq); free(kb); free(ks); free(live); free(lam); free(zz);
+    return rc;
+}
+/* alpha_max per solve for X + ap dX >= 0 and Z + ad dZ >= 0 given the inverse Cholesky factors
+ * (reference maxstep, solver.c:6300-6388): both sides in the same launches */
+static int maxstep2(Solve **S, int ns, const int *act, double *ap, double *ad) {
+    const size_t fLi[2] = { OFF(LXi), OFF(LZi) }, fLiT[2] = { OFF(LXiT), OFF(LZiT) }, fV[2] = { OFF(dX), OFF(dZ) };
+    double *o = malloc(sizeof(double) * 2 * ns);
+    const int rc = lanczos_min(S, ns, act, 2, fLi, fLiT, fV, 1, 0.02, o);
+    for (int q = 0; q < ns; q++) { ap[q] = o[q]; ad[q] = o[ns + q]; }
+    free(o);
+    return rc;
 }
 
 /* ---- the Schur complement: assemble, factor, solve ------------------------------------------ */
@@ -506,22 +572,27 @@ static int schur_factor(Solve **S, int ns, const int *act, int force64, int *fai
     /* explicit inverse of the factor and its transpose: every later solve is two products */
     for (int prec = 0; prec < 2; prec++) {
         int c = 0;
+        MItL_clear(&g_ml);
         for (int q = 0; q < ns; q++) {
             if ((act && !act[q]) || fail[q] || S[q]->f32 == prec) continue;
             Solve *s = S[q];
             const int64_t mul = prec ? 1 : 2;
-            MItem it = { s->P.m, q, mul * s->MF, mul * s->MDinv, mul * s->Mwork };
-            const int64_t shift = mul * (s->MLi - s->MF);
-            bk_zero(dint(), sizeof(int) * ns);
-            int rc = prec ? bk_trtri_f64(&it, 1, H, H, H + shift, H, dint())
-                          : bk_trtri_f32(&it, 1, (float *)H, (float *)H, (float *)H + shift, (float *)H, dint());
-            if (rc) goto err;
-            MItem tt = { s->P.m, q, mul * s->MLi, 0, 0 };
-            rc = prec ? bk_transpose_f64(&tt, 1, mul * (s->MLiT - s->MLi)) : bk_transpose_f32(&tt, 1, mul * (s->MLiT - s->MLi));
-            if (rc) goto err;
+            MItem it = { s->P.m, q, mul * s->MF, mul * s->MDinv, mul * s->Mwork, mul * s->MLi };
+            MItL_add(&g_ml, it);
             c++;
         }
-        (void)c;
+        if (!c) continue;
+        bk_zero(dint(), sizeof(int) * ns);
+        int rc = prec ? bk_trtri_f64(g_ml.v, g_ml.n, H, H, H, H, dint())
+                      : bk_trtri_f32(g_ml.v, g_ml.n, (float *)H, (float *)H, (float *)H, (float *)H, dint());
+        if (rc) goto err;
+        for (int t = 0; t < g_ml.n; t++) {
+            const Solve *s = S[g_ml.v[t].slot];
+            const int64_t mul = prec ? 1 : 2;
+            g_ml.v[t].off = mul * s->MLi; g_ml.v[t].off4 = mul * s->MLiT;
+        }
+        rc = prec ? bk_transpose_f64(g_ml.v, g_ml.n) : bk_transpose_f32(g_ml.v, g_ml.n);
+        if (rc) goto err;
     }
     free(todo); free(f64); free(reg);
     return 0;
@@ -544,7 +615,7 @@ static int schur_prec(Solve **S, int ns, const int *act, size_t fr, size_t fz) {
                 Solve *s = S[q];
                 const int64_t mul = prec ? 1 : 2;
                 /* pass 0: t2 = Li t1 (columns of Li' = rows of Li); pass 1: t1 = Li' t2 */
-                GemvItem g = { s->P.m, s->P.m, s->P.m, mul * (pass == 0 ? s->MLiT : s->MLi), pass == 0 ? s->t1 : s->t2, pass == 0 ? s->t2 : s->t1 };
+                GemvItem g = { s->P.m, s->P.m, s->P.m, mul * (pass == 0 ? s->MLiT : s->MLi), pass == 0 ? s->t1 : s->t2, pass == 0 ? s->t2 : s->t1, pass == 0 ? 2 : 1 };
                 GemvL_add(&g_vl, g);
             }
             if (!g_vl.n) continue;
@@ -561,7 +632,7 @@ static int schur_mv(Solve **S, int ns, const int *act, size_t fx, size_t fy, dou
     for (int q = 0; q < ns; q++) {
         if (act && !act[q]) continue;
         Solve *s = S[q];
-        GemvItem g = { s->P.m, s->P.m, s->P.m, s->M, F_(s, fx), F_(s, fy) };
+        GemvItem g = { s->P.m, s->P.m, s->P.m, s->M, F_(s, fx), F_(s, fy), 0 };
         GemvL_add(&g_vl, g);
     }
     double *H = heapp();
@@ -722,65 +793,8 @@ static int set_identity(Solve **S, int ns, const int *act, size_t f) {
 }
 /* lambda_min of V per solve (Lanczos on V itself, conservative: theta - residual bound) */
 static int lammin_all(Solve **S, int ns, const int *act, size_t fV, double *lam) {
-    /* W2 = V; Lanczos as in maxstep without the triangular products */
-    LinL_clear(&g_ll);
-    for (int q = 0; q < ns; q++) if (!act || act[q]) lin_add(S[q]->P.T, S[q]->W2, F_(S[q], fV), 1.0, -1, 0, -1, 0, -1, 0);
-    if (bk_lincomb(g_ll.v, g_ll.n)) return -1;
-    LzL_clear(&g_zl);
-    int kmax = 0;
-    for (int q = 0; q < ns; q++) {
-        if (act && !act[q]) continue;
-        SAux *A = aux_of(S[q]);
-        for (int k = 0; k < S[q]->P.nb; k++) {
-            const int n = S[q]->P.bn[k], K = n < BK_LZ_K ? n : BK_LZ_K;
-            LzItem z = { n, K, S[q]->lzQ + (int64_t)BK_LZ_K * A->lzoff[k], S[q]->lzw + A->lzoff[k], S[q]->lzab + 64 * k };
-            LzL_add(&g_zl, z);
-            if (K > kmax) kmax = K;
-        }
-    }
-    if (bk_lanczos_start(g_zl.v, g_zl.n)) return -1;
-    double *H = heapp();
-    for (int j = 0; j < kmax; j++) {
-        GemvL_clear(&g_vl);
-        int c = 0;
-        for (int q = 0; q < ns; q++) {
-            if (act && !act[q]) continue;
-            for (int k = 0; k < S[q]->P.nb; k++, c++) {
-                const LzItem *z = &g_zl.v[c];
-                if (j >= z->K) continue;
-                GemvItem g = { z->n, z->n, z->n, S[q]->W2 + S[q]->P.boff[k], z->Q + (int64_t)j * z->n, z->w };
-                GemvL_add(&g_vl, g);
-            }
-        }
-        if (bk_gemv_f64(1, 1.0, 0.0, H, H, H, g_vl.v, g_vl.n)) return -1;
-        if (bk_lanczos_step(g_zl.v, g_zl.n, j)) return -1;
-    }
-    int c = 0;
-    for (int q = 0; q < ns; q++) {
-        if (act && !act[q]) continue;
-        const HProb *P = &S[q]->P;
-        double *ab = malloc(sizeof(double) * 64 * P->nb);
-        if (bk_d2h(ab, H + S[q]->lzab, sizeof(double) * 64 * P->nb)) { free(ab); return -1; }
-        double lmin = INFINITY;
-        for (int k = 0; k < P->nb; k++, c++) {
-            const int K = g_zl.v[c].K;
-            const double *al = ab + 64 * k, *be = al + 32;
-            int kk = K;
-            double scale = 0;
-            for (int j = 0; j < K; j++) {
-                if (fabs(al[j]) > scale) scale = fabs(al[j]);
-                if (!(be[j] > 1e-12 * (scale + 1e-300))) { kk = j + 1; break; }
-            }
-            double th, zl;
-            tri_min(kk, al, be, &th, &zl);
-            const double res = (kk == K && kk < P->bn[k]) ? fabs(be[kk - 1] * zl) : 0.0;
-            const double l = th - res;
-            if (l < lmin) lmin = l;
-        }
-        free(ab);
-        lam[q] = lmin;
-    }
-    return 0;
+    const size_t z[1] = { 0 }, v[1] = { fV };
+    return lanczos_min(S, ns, act, 1, z, z, v, 0, 0.02, lam);
 }
 
 int ipm_start_batch(Solve **S, int ns) {
@@ -831,31 +845,37 @@ int ipm_start_batch(Solve **S, int ns) {
     }
     if (bk_h2d(ddbl(), g_hdbl, sizeof(double) * 2 * ns)) goto out;
     if (bk_add_diag(g_ml.v, g_ml.n, g_bdbl / 8)) goto out;
-    /* Cholesky of X and Z; a block that fails gets 1e-8, 1e-7, ... added to its diagonal */
-    for (int side = 0; side < 2; side++) {
-        const size_t fV = side ? OFF(Z) : OFF(X), fL = side ? OFF(LZ) : OFF(LX), fD = side ? OFF(DinvZ) : OFF(DinvX);
-        int *need = calloc(ns, sizeof(int));
-        for (int q = 0; q < ns; q++) { need[q] = act[q]; extra[q] = 1e-8; }
+    /* Cholesky of X and Z; a solve whose factorisation fails gets 1e-8, 1e-7, ... added to the
+     * diagonal of that side */
+    {
+        int *need = calloc(2 * ns, sizeof(int)), *ok2 = calloc(2 * ns, sizeof(int));
+        double *ex = calloc(2 * ns, sizeof(double));
+        for (int t = 0; t < 2 * ns; t++) { need[t] = act[t % ns]; ex[t] = 1e-8; }
         for (int tries = 0; tries < 30; tries++) {
             int any = 0;
-            for (int q = 0; q < ns; q++) any |= need[q];
+            for (int t = 0; t < 2 * ns; t++) any |= need[t];
             if (!any) break;
-            if (blk_chol(S, ns, need, fV, fL, fD, ok)) { free(need); goto out; }
+            if (chol_sides(S, ns, 2, need, F_V, F_L, F_D, F_W, ok2)) { free(need); free(ok2); free(ex); goto out; }
             MItL_clear(&g_ml);
-            for (int q = 0; q < ns; q++) {
-                if (!need[q]) continue;
-                if (ok[q]) { need[q] = 0; continue; }
-                g_hdbl[q] = extra[q]; extra[q] *= 10;
-                for (int k = 0; k < S[q]->P.nb; k++) { MItem a = { S[q]->P.bn[k], q, F_(S[q], fV) + S[q]->P.boff[k], 0, 0 }; MItL_add(&g_ml, a); }
+            for (int t = 0; t < 2 * ns; t++) {
+                if (!need[t]) continue;
+                if (ok2[t]) { need[t] = 0; continue; }
+                const int sd = t / ns, q = t % ns;
+                g_hdbl[t] = ex[t]; ex[t] *= 10;
+                for (int k = 0; k < S[q]->P.nb; k++) { MItem a = { S[q]->P.bn[k], t, F_(S[q], F_V[sd]) + S[q]->P.boff[k], 0, 0, 0 }; MItL_add(&g_ml, a); }
             }
-            if (g_ml.n) {
-                if (bk_h2d(ddbl(), g_hdbl, sizeof(double) * ns) || bk_add_diag(g_ml.v, g_ml.n, g_bdbl / 8)) { free(need); goto out; }
-            }
+            if (g_ml.n && (bk_h2d(ddbl(), g_hdbl, sizeof(double) * 2 * ns) || bk_add_diag(g_ml.v, g_ml.n, g_bdbl / 8))) { free(need); free(ok2); free(ex); goto out; }
         }
-        for (int q = 0; q < ns; q++) if (need[q]) { S[q]->status = 4; S[q]->done = 1; act[q] = 0; }
-        free(need);
+        for (int t = 0; t < 2 * ns; t++) if (need[t]) { S[t % ns]->status = 4; S[t % ns]->done = 1; act[t % ns] = 0; }
+        free(need); free(ok2); free(ex);
     }
-    if (blk_inv(S, ns, act, OFF(LX), OFF(DinvX), OFF(LXi)) || blk_inv(S, ns, act, OFF(LZ), OFF(DinvZ), OFF(LZi))) goto out;
+    {
+        int *a2 = malloc(sizeof(int) * 2 * ns);
+        for (int t = 0; t < 2 * ns; t++) a2[t] = act[t % ns];
+        const int e = inv_sides(S, ns, 2, a2, F_L, F_D, F_Li, F_LiT, F_W);
+        free(a2);
+        if (e) goto out;
+    }
     /* R0 = C - Z - A*(y); theta = 1 */
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.T, S[q]->R0, S[q]->C, 1.0, S[q]->Z, -1.0, -1, 0, -1, 0);
@@ -1025,7 +1045,7 @@ int ipm_iterate_batch(Solve **S, int ns) {
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.T, S[q]->dX, S[q]->G, -1.0, S[q]->X, -1.0, -1, 0, -1, 0);
     if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
-    if (maxstep(S, ns, act, OFF(LXi), OFF(dX), ap) || maxstep(S, ns, act, OFF(LZi), OFF(dZ), ad)) goto out;
+    if (maxstep2(S, ns, act, ap, ad)) goto out;
     DotL_clear(&g_dl);
     for (int q = 0; q < ns; q++) if (act[q]) {
         Solve *s = S[q];
@@ -1048,8 +1068,8 @@ int ipm_iterate_batch(Solve **S, int ns) {
         gam[q] = 0.9 + (0.99 - 0.9) * mina;
         smu[q] = sigma * mu;
     }
-    /* corrector (solver.c:14997-15055): Q = Zi dZ dX; rhs = b - smu a0 + A(Q) + h */
-    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(dZ), OFF(W1))) goto out;
+    /* corrector (solver.c:14997-15055): Q = Zi dZ dX (W1 = Zi dZ still holds from the predictor);
+     * rhs = b - smu a0 + A(Q) + h */
     if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(dX), OFF(Q))) goto out;
     if (aop(S, ns, act, OFF(Q), OFF(qv), 1.0, 0.0)) goto out;
     LinL_clear(&g_ll);
@@ -1119,7 +1139,7 @@ int ipm_iterate_batch(Solve **S, int ns) {
         if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
     }
     /* step lengths (solver.c:15108-15130) */
-    if (maxstep(S, ns, act, OFF(LXi), OFF(dX), apr) || maxstep(S, ns, act, OFF(LZi), OFF(dZ), adr)) goto out;
+    if (maxstep2(S, ns, act, apr, adr)) goto out;
     for (int q = 0; q < ns; q++) if (act[q]) { ap[q] = fmin(1.0, gam[q] * fmin(1.0, apr[q])); ad[q] = fmin(1.0, gam[q] * fmin(1.0, adr[q])); }
     /* catastrophe guard (solver.c:15143-15160) */
     {
@@ -1141,34 +1161,37 @@ int ipm_iterate_batch(Solve **S, int ns) {
         }
         free(g);
     }
-    /* try_step: V += (a - stepped) dV, L = chol(V); x0.8 on failure (solver.c:6393-6446) */
-    for (int side = 0; side < 2; side++) {
-        double *a = side ? ad : ap;
-        int *okv = side ? okd : okp;
-        const size_t fV = side ? OFF(Z) : OFF(X), fdV = side ? OFF(dZ) : OFF(dX), fL = side ? OFF(LZ) : OFF(LX), fD = side ? OFF(DinvZ) : OFF(DinvX);
-        for (int q = 0; q < ns; q++) { tact[q] = act[q]; stepped[q] = 0; okv[q] = 0; }
+    /* try_step: V += (a - stepped) dV, L = chol(V); x0.8 on failure (solver.c:6393-6446); the
+     * primal and dual sides factor in the same launches */
+    {
+        int *ta = calloc(2 * ns, sizeof(int)), *okv = calloc(2 * ns, sizeof(int)), *ok = calloc(2 * ns, sizeof(int));
+        double *st = calloc(2 * ns, sizeof(double));
+        for (int t = 0; t < 2 * ns; t++) ta[t] = act[t % ns];
         for (int tries = 0; tries < 40; tries++) {
             int any = 0;
-            for (int q = 0; q < ns; q++) any |= tact[q];
+            for (int t = 0; t < 2 * ns; t++) any |= ta[t];
             if (!any) break;
             LinL_clear(&g_ll);
-            for (int q = 0; q < ns; q++) {
-                if (!tact[q]) continue;
-                const double inc = a[q] - stepped[q];
-                stepped[q] = a[q];
-                lin_add(S[q]->P.T, F_(S[q], fV), F_(S[q], fV), 1.0, F_(S[q], fdV), inc, -1, 0, -1, 0);
+            for (int t = 0; t < 2 * ns; t++) {
+                if (!ta[t]) continue;
+                const int sd = t / ns, q = t % ns;
+                double *a = sd ? &ad[q] : &ap[q];
+                const double inc = *a - st[t];
+                st[t] = *a;
+                lin_add(S[q]->P.T, F_(S[q], F_V[sd]), F_(S[q], F_V[sd]), 1.0, F_(S[q], F_dV[sd]), inc, -1, 0, -1, 0);
             }
-            if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
-            int *ok = calloc(ns, sizeof(int));
-            if (blk_chol(S, ns, tact, fV, fL, fD, ok)) { free(ok); goto out; }
-            for (int q = 0; q < ns; q++) {
-                if (!tact[q]) continue;
-                if (ok[q]) { tact[q] = 0; okv[q] = 1; }
-                else a[q] *= 0.8;
+            if (bk_lincomb(g_ll.v, g_ll.n)) { free(ta); free(okv); free(ok); free(st); goto out; }
+            if (chol_sides(S, ns, 2, ta, F_V, F_L, F_D, F_W, ok)) { free(ta); free(okv); free(ok); free(st); goto out; }
+            for (int t = 0; t < 2 * ns; t++) {
+                if (!ta[t]) continue;
+                if (ok[t]) { ta[t] = 0; okv[t] = 1; }
+                else { if (t / ns) ad[t % ns] *= 0.8; else ap[t % ns] *= 0.8; }
             }
-            free(ok);
         }
-        if (blk_inv(S, ns, okv, fL, fD, side ? OFF(LZi) : OFF(LXi))) goto out;
+        for (int q = 0; q < ns; q++) { okp[q] = okv[q]; okd[q] = okv[ns + q]; }
+        const int e = inv_sides(S, ns, 2, okv, F_L, F_D, F_Li, F_LiT, F_W);
+        free(ta); free(okv); free(ok); free(st);
+        if (e) goto out;
     }
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) {

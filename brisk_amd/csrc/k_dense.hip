@@ -16,6 +16,7 @@
 #include <vector>
 #include "dev.h"
 #include "rt_internal.h"
+#include "chol_diag.h"
 
 #define GBM 64
 #define GBN 64
@@ -35,79 +36,104 @@ template <> struct MF<float> {
     static __device__ __forceinline__ int out(int g, int r) { return 4 * g + r; }
 };
 
-// C = alpha op(A) op(B) + beta C for one 64x64 tile of one item. The MFMA is fed with the B
-// tile as its A operand and the A tile as its B operand, so each output register row is a run
-// of 16 consecutive rows of column-major C (coalesced stores).
-template <typename T, int TA, int TB>
+// C = alpha op(A) op(B) + beta C for one BM x BN tile of one item. Four waves in a 2 x 2 grid,
+// each owning a (BM/2) x (BN/2) quadrant of 16 x 16 MFMA tiles. The MFMA is fed with the B tile
+// as its A operand and the A tile as its B operand, so each output register row is a run of 16
+// consecutive rows of column-major C (coalesced stores). The next K slab is fetched into
+// registers while the current one is multiplied from LDS.
+template <typename T, int TA, int TB, int BM, int BN>
 __global__ __launch_bounds__(256) void k_gemm(const GemmItem *__restrict__ items, int nitems, const T *__restrict__ A,
                                               const T *__restrict__ B, T *__restrict__ C, T alpha, T beta,
                                               const int *__restrict__ info) {
     typedef typename MF<T>::v4 v4;
+    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
+    constexpr int LA = BM * GBK / 256, LB = BN * GBK / 256;
     const int item = blockIdx.y + blockIdx.z * 65535;
     if (item >= nitems) return;
     const GemmItem it = items[item];
     if (it.guard >= 0 && info && info[it.guard] != 0) return;
-    const int tilesM = (it.M + GBM - 1) / GBM, tilesN = (it.N + GBN - 1) / GBN;
+    const int tilesM = (it.M + BM - 1) / BM, tilesN = (it.N + BN - 1) / BN;
     const int t = blockIdx.x;
     if (t >= tilesM * tilesN) return;
     const int tm = t % tilesM, tn = t / tilesM;
-    const int r0 = tm * GBM, c0 = tn * GBN;
-    if ((it.tri & 1) && c0 > r0 + GBM - 1) return;
+    const int r0 = tm * BM, c0 = tn * BN;
+    if ((it.tri & 1) && c0 > r0 + BM - 1) return;
     int kb = 0, ke = it.K;
     if (it.tri & 2) kb = max(kb, r0);            // op(A) upper triangular: A(i,k) = 0 for k < i
-    if (it.tri & 4) ke = min(ke, r0 + GBM);      // op(A) lower triangular: A(i,k) = 0 for k > i
+    if (it.tri & 4) ke = min(ke, r0 + BM);       // op(A) lower triangular: A(i,k) = 0 for k > i
     if (it.tri & 8) kb = max(kb, c0);            // op(B) lower triangular: B(k,j) = 0 for k < j
-    if (it.tri & 16) ke = min(ke, c0 + GBN);     // op(B) upper triangular: B(k,j) = 0 for k > j
+    if (it.tri & 16) ke = min(ke, c0 + BN);      // op(B) upper triangular: B(k,j) = 0 for k > j
     kb = (kb / GBK) * GBK;
-    __shared__ T As[GBK][GBM + 4];
-    __shared__ T Bs[GBK][GBN + 4];
+    __shared__ T As[GBK][BM + 4];
+    __shared__ T Bs[GBK][BN + 4];
     const T *a = A + it.a, *b = B + it.b;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave & 1, wn = wave >> 1;
     const int M = it.M, N = it.N, lda = it.lda, ldb = it.ldb;
-    v4 acc[2][2];
-    for (int i = 0; i < 2; i++) for (int j = 0; j < 2; j++) for (int r = 0; r < 4; r++) acc[i][j][r] = 0;
+    v4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; i++)
+#pragma unroll
+        for (int j = 0; j < FN; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[i][j][r] = 0;
+    T ra[LA], rb[LB];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int r = 0; r < LA; r++) {
+            const int e = tid + 256 * r;
+            int i, k;
+            if (TA == 0) { i = e % BM; k = e / BM; } else { k = e % GBK; i = e / GBK; }
+            const int gi = r0 + i, gk = k0 + k;
+            ra[r] = (gi < M && gk < ke) ? (TA == 0 ? a[gi + (int64_t)gk * lda] : a[gk + (int64_t)gi * lda]) : (T)0;
+        }
+#pragma unroll
+        for (int r = 0; r < LB; r++) {
+            const int e = tid + 256 * r;
+            int j, k;
+            if (TB == 0) { k = e % GBK; j = e / GBK; } else { j = e % BN; k = e / BN; }
+            const int gj = c0 + j, gk = k0 + k;
+            rb[r] = (gj < N && gk < ke) ? (TB == 0 ? b[gk + (int64_t)gj * ldb] : b[gj + (int64_t)gk * ldb]) : (T)0;
+        }
+    };
+    if (kb < ke) fetch(kb);
     for (int k0 = kb; k0 < ke; k0 += GBK) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            if (TA == 0) {
-                const int i = tid & 63, k = (tid >> 6) + 4 * r, gi = r0 + i, gk = k0 + k;
-                As[k][i] = (gi < M && gk < ke) ? a[gi + (int64_t)gk * lda] : (T)0;
-            } else {
-                const int k = tid & 15, i = (tid >> 4) + 16 * r, gi = r0 + i, gk = k0 + k;
-                As[k][i] = (gi < M && gk < ke) ? a[gk + (int64_t)gi * lda] : (T)0;
-            }
-            if (TB == 0) {
-                const int k = tid & 15, j = (tid >> 4) + 16 * r, gj = c0 + j, gk = k0 + k;
-                Bs[k][j] = (gj < N && gk < ke) ? b[gk + (int64_t)gj * ldb] : (T)0;
-            } else {
-                const int j = tid & 63, k = (tid >> 6) + 4 * r, gj = c0 + j, gk = k0 + k;
-                Bs[k][j] = (gj < N && gk < ke) ? b[gj + (int64_t)gk * ldb] : (T)0;
-            }
+        for (int r = 0; r < LA; r++) {
+            const int e = tid + 256 * r;
+            if (TA == 0) As[e / BM][e % BM] = ra[r]; else As[e % GBK][e / GBK] = ra[r];
+        }
+#pragma unroll
+        for (int r = 0; r < LB; r++) {
+            const int e = tid + 256 * r;
+            if (TB == 0) Bs[e % GBK][e / GBK] = rb[r]; else Bs[e / BN][e % BN] = rb[r];
         }
         __syncthreads();
+        if (k0 + GBK < ke) fetch(k0 + GBK);
 #pragma unroll
         for (int kk = 0; kk < GBK; kk += 4) {
-            T av[2], bv[2];
+            T av[FM], bv[FN];
             const int kr = kk + (lane >> 4), l16 = lane & 15;
 #pragma unroll
-            for (int s = 0; s < 2; s++) { av[s] = As[kr][wm * 32 + s * 16 + l16]; bv[s] = Bs[kr][wn * 32 + s * 16 + l16]; }
+            for (int s = 0; s < FM; s++) av[s] = As[kr][wm * WM + s * 16 + l16];
 #pragma unroll
-            for (int si = 0; si < 2; si++)
+            for (int s = 0; s < FN; s++) bv[s] = Bs[kr][wn * WN + s * 16 + l16];
 #pragma unroll
-                for (int sj = 0; sj < 2; sj++) acc[si][sj] = MF<T>::mma(bv[sj], av[si], acc[si][sj]);
+            for (int si = 0; si < FM; si++)
+#pragma unroll
+                for (int sj = 0; sj < FN; sj++) acc[si][sj] = MF<T>::mma(bv[sj], av[si], acc[si][sj]);
         }
         __syncthreads();
     }
     T *c = C + it.c;
     const int ldc = it.ldc;
 #pragma unroll
-    for (int si = 0; si < 2; si++)
+    for (int si = 0; si < FM; si++)
 #pragma unroll
-        for (int sj = 0; sj < 2; sj++)
+        for (int sj = 0; sj < FN; sj++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int gi = r0 + wm * 32 + si * 16 + (lane & 15);
-                const int gj = c0 + wn * 32 + sj * 16 + MF<T>::out(lane >> 4, r);
+                const int gi = r0 + wm * WM + si * 16 + (lane & 15);
+                const int gj = c0 + wn * WN + sj * 16 + MF<T>::out(lane >> 4, r);
                 if (gi < M && gj < N && (!(it.tri & 1) || gi >= gj)) {
                     T *cp = c + gi + (int64_t)gj * ldc;
                     T v = alpha * acc[si][sj][r];
@@ -131,9 +157,16 @@ static double gemm_flops(const GemmItem &g) {
 template <typename T>
 static int gemm_launch(int ta, int tb, T alpha, T beta, const T *A, const T *B, T *C, const GemmItem *items, int nitems, const int *info) {
     if (nitems <= 0) return 0;
+    // 128 x 128 tiles when every item is large enough to fill them, else 64 x 64
+    int big = 1, maxM = 0, maxN = 0;
+    for (int q = 0; q < nitems; q++) {
+        if (items[q].M < 192 || items[q].N < 192) big = 0;
+        maxM = items[q].M > maxM ? items[q].M : maxM; maxN = items[q].N > maxN ? items[q].N : maxN;
+    }
+    const int TBM = big ? 128 : 64, TBN = big ? 128 : 64;
     int maxt = 0;
     for (int q = 0; q < nitems; q++) {
-        const int t = ((items[q].M + GBM - 1) / GBM) * ((items[q].N + GBN - 1) / GBN);
+        const int t = ((items[q].M + TBM - 1) / TBM) * ((items[q].N + TBN - 1) / TBN);
         if (items[q].M > 0 && items[q].N > 0 && t > maxt) maxt = t;
     }
     if (maxt == 0) return 0;
@@ -141,10 +174,13 @@ static int gemm_launch(int ta, int tb, T alpha, T beta, const T *A, const T *B, 
     if (!d) return -1;
     dim3 grid(maxt, nitems < 65535 ? nitems : 65535, (nitems + 65534) / 65535);
     const int tok = bk_prof_begin();
-    if (!ta && !tb) hipLaunchKernelGGL((k_gemm<T, 0, 0>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info);
-    else if (!ta && tb) hipLaunchKernelGGL((k_gemm<T, 0, 1>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info);
-    else if (ta && !tb) hipLaunchKernelGGL((k_gemm<T, 1, 0>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info);
-    else hipLaunchKernelGGL((k_gemm<T, 1, 1>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info);
+#define GL(TA_, TB_, BM_) hipLaunchKernelGGL((k_gemm<T, TA_, TB_, BM_, BM_>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info)
+    if (big) {
+        if (!ta && !tb) GL(0, 0, 128); else if (!ta && tb) GL(0, 1, 128); else if (ta && !tb) GL(1, 0, 128); else GL(1, 1, 128);
+    } else {
+        if (!ta && !tb) GL(0, 0, 64); else if (!ta && tb) GL(0, 1, 64); else if (ta && !tb) GL(1, 0, 64); else GL(1, 1, 64);
+    }
+#undef GL
     if (tok >= 0) {
         double fl = 0, by = 0;
         for (int q = 0; q < nitems; q++) { fl += gemm_flops(items[q]); by += (double)sizeof(T) * ((double)items[q].M * items[q].K + (double)items[q].K * items[q].N + 2.0 * items[q].M * items[q].N); }
@@ -174,78 +210,17 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const MItem *__restrict__ it
     const int n = it.n;
     if (k0 >= n || info[it.slot] != 0) return;
     const int nb = min(64, n - k0);
-    __shared__ T a[64][65];      // a[col][row]
-    __shared__ T li[64][65];
-    __shared__ int fail;
-    __shared__ T piv;
     T *Ab = A + it.off + k0 + (int64_t)k0 * n;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int r = e & 63, c = e >> 6;
-        a[c][r] = (r < nb && c < nb && r >= c) ? Ab[r + (int64_t)c * n] : (T)0;
-    }
-    if (tid == 0) fail = 0;
-    T pm = (T)1e30;
-    __syncthreads();
-    for (int j = 0; j < nb; j++) {
-        if (tid == 0) {
-            const T d = a[j][j];
-            if (!(d > (T)0)) fail = j + 1;
-            else { piv = sqrt(d); a[j][j] = piv; if (d < pm) pm = d; }
-        }
-        __syncthreads();
-        if (fail) break;
-        const T p = piv;
-        if (tid > j && tid < nb) a[j][tid] /= p;
-        __syncthreads();
-        const int rem = nb - j - 1;
-        for (int e = tid; e < rem * rem; e += 256) {
-            const int i = j + 1 + e % rem, l = j + 1 + e / rem;
-            if (i >= l) a[l][i] -= a[j][i] * a[j][l];
-        }
-        __syncthreads();
-    }
-    if (fail) {
-        if (tid == 0) info[it.slot] = k0 + fail;
-        return;
-    }
-    if (tid < nb) {        // inverse of the block: column c by forward substitution
-        const int c = tid;
-        for (int i = 0; i < 64; i++) li[c][i] = 0;
-        li[c][c] = (T)1 / a[c][c];
-        for (int i = c + 1; i < nb; i++) {
-            T s = 0;
-            for (int k = c; k < i; k++) s += a[k][i] * li[c][k];
-            li[c][i] = -s / a[i][i];
-        }
-    }
-    __syncthreads();
     T *Di = Dinv + it.off2 + (int64_t)(k0 / 64) * 64 * 64;
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int r = e & 63, c = e >> 6;
-        if (r < nb && c < nb) Ab[r + (int64_t)c * n] = r >= c ? a[c][r] : (T)0;
-        Di[r + 64 * c] = (r < nb && c < nb) ? li[c][r] : (T)0;
+    T pm;
+    const int f = diag_factor_inverse<T>(Ab, n, nb, Di, pm);
+    if (threadIdx.x == 0) {
+        if (f) info[it.slot] = k0 + f;
+        else pmin[it.slot] = (k0 == 0) ? pm : min(pmin[it.slot], pm);
     }
-    if (tid == 0) pmin[it.slot] = (k0 == 0) ? pm : min(pmin[it.slot], pm);
 }
 
-// the strict upper triangle of every matrix set to zero, and the panel (rows k0+64.., cols
-// k0..k0+63) copied from the scratch W back into A
-template <typename T>
-__global__ void k_copy_panel(const MItem *__restrict__ items, int nit, int k0, const T *__restrict__ W, T *__restrict__ A,
-                             const int *__restrict__ info) {
-    const int q = blockIdx.y;
-    if (q >= nit) return;
-    const MItem it = items[q];
-    const int n = it.n;
-    if (k0 + 64 >= n || info[it.slot] != 0) return;
-    const int rows = n - k0 - 64;
-    const int64_t tot = (int64_t)rows * 64;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
-        const int r = (int)(e % rows), c = (int)(e / rows);
-        A[it.off + (k0 + 64 + r) + (int64_t)(k0 + c) * n] = W[it.off3 + r + (int64_t)c * n];
-    }
-}
+// the strict upper triangle of every matrix set to zero
 template <typename T>
 __global__ void k_zero_upper(const MItem *__restrict__ items, int nit, T *__restrict__ A) {
     const int q = blockIdx.y;
@@ -285,27 +260,27 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
             const int n = it[q].n;
             if (k0 + 64 >= n) continue;
             GemmItem g;
-            // panel: W = A[k0+64:, k0:k0+64] Dinv_k'   (Dinv_k' upper triangular)
+            // panel, in place: A[k0+64:, k0:k0+64] := A[k0+64:, k0:k0+64] Dinv_k' (Dinv_k' upper
+            // triangular). The 64 x 64 tiles span the panel's full width, so every workgroup reads
+            // its rows completely before it writes them.
             g.M = n - k0 - 64; g.N = 64; g.K = 64;
             g.lda = n; g.ldb = 64; g.ldc = n;
             g.a = it[q].off + (k0 + 64) + (int64_t)k0 * n;
             g.b = it[q].off2 + (int64_t)(k0 / 64) * 64 * 64;
-            g.c = it[q].off3;
+            g.c = g.a;
             g.guard = it[q].slot; g.tri = 16;
             gi.push_back(g);
-            // trailing: A22 -= W W' (lower triangle)
+            // trailing: A22 -= L21 L21' (lower triangle)
             g.M = n - k0 - 64; g.N = n - k0 - 64; g.K = 64;
             g.lda = n; g.ldb = n; g.ldc = n;
-            g.a = it[q].off3; g.b = it[q].off3;
+            g.a = it[q].off + (k0 + 64) + (int64_t)k0 * n; g.b = g.a;
             g.c = it[q].off + (k0 + 64) + (int64_t)(k0 + 64) * n;
             g.guard = it[q].slot; g.tri = 1;
             tr.push_back(g);
         }
         if (gi.empty()) continue;
-        if (gemm(0, 1, (T)1, (T)0, A, Dinv, W, gi.data(), (int)gi.size(), info)) return -1;
-        if (gemm(0, 1, (T)-1, (T)1, W, W, A, tr.data(), (int)tr.size(), info)) return -1;
-        hipLaunchKernelGGL((k_copy_panel<T>), dim3(64, n_it), dim3(256), 0, g_bk_stream, d_it, n_it, k0, W, A, info);
-        if (BK_LAUNCH_CHECK("k_copy_panel")) return -1;
+        if (gemm(0, 1, (T)1, (T)0, A, Dinv, A, gi.data(), (int)gi.size(), info)) return -1;
+        if (gemm(0, 1, (T)-1, (T)1, A, A, A, tr.data(), (int)tr.size(), info)) return -1;
     }
     return 0;
 }
@@ -334,7 +309,7 @@ __global__ void k_trtri_init(const MItem *__restrict__ items, int nit, const T *
         const int r = (int)(e % n), c = (int)(e / n);
         T v = 0;
         if ((r >> 6) == (c >> 6) && r >= c) v = Dinv[it.off2 + (int64_t)(c >> 6) * 4096 + (r & 63) + 64 * (c & 63)];
-        Linv[it.off + e] = v;
+        Linv[it.off4 + e] = v;
     }
 }
 template <typename T>
@@ -358,16 +333,16 @@ static int trtri_run(const MItem *it, int n_it, const T *L, const T *Dinv, T *Li
                 g.M = s2; g.N = s; g.K = s;
                 g.lda = n; g.ldb = n; g.ldc = n;
                 g.a = it[q].off + (p0 + s) + (int64_t)p0 * n;
-                g.b = it[q].off + p0 + (int64_t)p0 * n;
+                g.b = it[q].off4 + p0 + (int64_t)p0 * n;
                 g.c = it[q].off3 + (p0 + s) + (int64_t)p0 * n;
                 g.guard = it[q].slot; g.tri = 8;
                 g1.push_back(g);
                 // Linv[p0+s.., p0..] = -C^{-1} T, C^{-1} = Linv[p0+s : p0+s+s2, same] (lower)
                 g.M = s2; g.N = s; g.K = s2;
                 g.lda = n; g.ldb = n; g.ldc = n;
-                g.a = it[q].off + (p0 + s) + (int64_t)(p0 + s) * n;
+                g.a = it[q].off4 + (p0 + s) + (int64_t)(p0 + s) * n;
                 g.b = it[q].off3 + (p0 + s) + (int64_t)p0 * n;
-                g.c = it[q].off + (p0 + s) + (int64_t)p0 * n;
+                g.c = it[q].off4 + (p0 + s) + (int64_t)p0 * n;
                 g.guard = it[q].slot; g.tri = 4;
                 g2.push_back(g);
             }
@@ -403,7 +378,17 @@ __global__ __launch_bounds__(256) void k_gemv_t(const GemvItem *__restrict__ ite
     const TA *col = A + it.a + (int64_t)j * it.lda;
     const TX *xv = x + it.x;
     TACC s = 0;
-    for (int i = lane; i < it.M; i += 64) s += (TACC)col[i] * (TACC)xv[i];
+    const int i0 = it.tri == 1 ? j : 0, i1 = it.tri == 2 ? j + 1 : it.M;     // triangular A: only its nonzero rows
+    TACC s1 = 0, s2 = 0, s3 = 0;
+    int i = i0 + lane;
+    for (; i + 192 < i1; i += 256) {
+        s += (TACC)col[i] * (TACC)xv[i];
+        s1 += (TACC)col[i + 64] * (TACC)xv[i + 64];
+        s2 += (TACC)col[i + 128] * (TACC)xv[i + 128];
+        s3 += (TACC)col[i + 192] * (TACC)xv[i + 192];
+    }
+    for (; i < i1; i += 64) s += (TACC)col[i] * (TACC)xv[i];
+    s = (s + s1) + (s2 + s3);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) {

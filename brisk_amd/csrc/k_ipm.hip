@@ -164,7 +164,7 @@ __global__ void k_symmetrize(const MItem *__restrict__ it, int nit, double *__re
     }
 }
 template <typename T>
-__global__ void k_transpose(const MItem *__restrict__ it, int nit, T *__restrict__ H, int64_t delta) {
+__global__ void k_transpose(const MItem *__restrict__ it, int nit, T *__restrict__ H) {
     const int q = blockIdx.y + blockIdx.z * 65535;
     if (q >= nit) return;
     const MItem m = it[q];
@@ -182,7 +182,7 @@ __global__ void k_transpose(const MItem *__restrict__ it, int nit, T *__restrict
         __syncthreads();
         for (int y = ly; y < 32; y += 8) {
             const int r = tc + lx, c = tr + y;                       // dst(r, c) = src(c, r)
-            if (r < n && c < n) H[m.off + delta + r + (int64_t)c * n] = tile[lx][y];
+            if (r < n && c < n) H[m.off4 + r + (int64_t)c * n] = tile[lx][y];
         }
         __syncthreads();
     }
@@ -235,20 +235,20 @@ extern "C" int bk_symmetrize(const MItem *it, int n_it) {
     hipLaunchKernelGGL(k_symmetrize, g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
     return BK_LAUNCH_CHECK("k_symmetrize");
 }
-extern "C" int bk_transpose_f64(const MItem *it, int n_it, int64_t delta) {
+extern "C" int bk_transpose_f64(const MItem *it, int n_it) {
     if (n_it <= 0) return 0;
     dim3 g; mitem_launch_dims(it, n_it, 2, g);
     g.x = g.x > 512 ? 512 : g.x;
     const MItem *d = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
-    hipLaunchKernelGGL((k_transpose<double>), g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap, delta);
+    hipLaunchKernelGGL((k_transpose<double>), g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
     return BK_LAUNCH_CHECK("k_transpose");
 }
-extern "C" int bk_transpose_f32(const MItem *it, int n_it, int64_t delta) {
+extern "C" int bk_transpose_f32(const MItem *it, int n_it) {
     if (n_it <= 0) return 0;
     dim3 g; mitem_launch_dims(it, n_it, 2, g);
     g.x = g.x > 512 ? 512 : g.x;
     const MItem *d = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
-    hipLaunchKernelGGL((k_transpose<float>), g, dim3(256), 0, g_bk_stream, d, n_it, (float *)g_heap, delta);
+    hipLaunchKernelGGL((k_transpose<float>), g, dim3(256), 0, g_bk_stream, d, n_it, (float *)g_heap);
     return BK_LAUNCH_CHECK("k_transpose");
 }
 extern "C" int bk_add_diag(const MItem *it, int n_it, int64_t s_off) {
@@ -461,42 +461,68 @@ __global__ __launch_bounds__(256) void k_lz_start(const LzItem *__restrict__ it,
     const double nrm = 1.0 / sqrt(block_sum(s, sh));
     for (int i = threadIdx.x; i < L.n; i += 256) Q[i] *= nrm;
 }
-__global__ __launch_bounds__(256) void k_lz_step(const LzItem *__restrict__ it, int nit, int j, double *__restrict__ H) {
+// sum over the 4 waves of a workgroup of per-lane values, for np values at once (fixed order)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+#define LZT 1024
+__global__ __launch_bounds__(LZT) void k_lz_step(const LzItem *__restrict__ it, int nit, int j, double *__restrict__ H) {
     const int q = blockIdx.x;
     if (q >= nit) return;
     const LzItem L = it[q];
     if (j >= L.K) return;
-    __shared__ double sh[256];
-    const int n = L.n;
+    constexpr int NW = LZT / 64;
+    __shared__ double sh[NW][32];
+    __shared__ double cf[32];
+    const int n = L.n, t = threadIdx.x, lane = t & 63, wv = t >> 6;
     double *Q = H + L.Q, *w = H + L.w, *ab = H + L.ab;
-    const double *qj = Q + (int64_t)j * n;
-    double s = 0;
-    for (int i = threadIdx.x; i < n; i += 256) s += qj[i] * w[i];
-    const double alpha = block_sum(s, sh);
-    const double bprev = j > 0 ? ab[32 + j - 1] : 0.0;
-    for (int i = threadIdx.x; i < n; i += 256) {
-        double v = w[i] - alpha * qj[i];
-        if (j > 0) v -= bprev * Q[(int64_t)(j - 1) * n + i];
-        w[i] = v;
-    }
-    __syncthreads();
-    for (int pass = 0; pass < 2; pass++)
-        for (int p = 0; p <= j; p++) {
-            const double *qp = Q + (int64_t)p * n;
-            double c = 0;
-            for (int i = threadIdx.x; i < n; i += 256) c += qp[i] * w[i];
-            c = block_sum(c, sh);
-            for (int i = threadIdx.x; i < n; i += 256) w[i] -= c * qp[i];
-            __syncthreads();
+    // classical Gram-Schmidt twice against q_0 .. q_j on w = T q_j: the first pass's coefficient
+    // of q_j is alpha_j; all coefficients of a pass in one sweep (independent loads), summed over
+    // the lanes and then over the waves in a fixed order
+    double alpha = 0;
+    for (int pass = 0; pass < 2; pass++) {
+        double c[32];
+#pragma unroll
+        for (int p = 0; p < 32; p++) c[p] = 0;
+        for (int i = t; i < n; i += LZT) {
+            const double wi = w[i];
+#pragma unroll
+            for (int p = 0; p < 32; p++) if (p <= j) c[p] += Q[(int64_t)p * n + i] * wi;
         }
+#pragma unroll
+        for (int p = 0; p < 32; p++) {
+            if (p <= j) {
+                const double v = wave_sum(c[p]);
+                if (lane == 0) sh[wv][p] = v;
+            }
+        }
+        __syncthreads();
+        if (t <= j) { double v = 0; for (int u = 0; u < NW; u++) v += sh[u][t]; cf[t] = v; }
+        __syncthreads();
+        if (pass == 0) alpha = cf[j];
+        for (int i = t; i < n; i += LZT) {
+            double v = w[i];
+#pragma unroll
+            for (int p = 0; p < 32; p++) if (p <= j) v -= cf[p] * Q[(int64_t)p * n + i];
+            w[i] = v;
+        }
+        __syncthreads();
+    }
     double b2 = 0;
-    for (int i = threadIdx.x; i < n; i += 256) b2 += w[i] * w[i];
-    const double beta = sqrt(block_sum(b2, sh));
-    if (threadIdx.x == 0) { ab[j] = alpha; ab[32 + j] = beta; }
+    for (int i = t; i < n; i += LZT) b2 += w[i] * w[i];
+    b2 = wave_sum(b2);
+    if (lane == 0) sh[wv][0] = b2;
+    __syncthreads();
+    double bb = 0;
+    for (int u = 0; u < NW; u++) bb += sh[u][0];
+    const double beta = sqrt(bb);
+    if (t == 0) { ab[j] = alpha; ab[32 + j] = beta; }
     if (j + 1 < L.K) {
         const double inv = beta > 1e-300 ? 1.0 / beta : 0.0;
         double *qn = Q + (int64_t)(j + 1) * n;
-        for (int i = threadIdx.x; i < n; i += 256) qn[i] = w[i] * inv;
+        for (int i = t; i < n; i += LZT) qn[i] = w[i] * inv;
     }
 }
 extern "C" int bk_lanczos_start(const LzItem *it, int n_it) {
@@ -509,7 +535,7 @@ extern "C" int bk_lanczos_step(const LzItem *it, int n_it, int j) {
     if (n_it <= 0) return 0;
     const LzItem *d = (const LzItem *)bk_stage(it, sizeof(LzItem) * n_it);
     const int tok = bk_prof_begin();
-    hipLaunchKernelGGL(k_lz_step, dim3(n_it), dim3(256), 0, g_bk_stream, d, n_it, j, (double *)g_heap);
+    hipLaunchKernelGGL(k_lz_step, dim3(n_it), dim3(LZT), 0, g_bk_stream, d, n_it, j, (double *)g_heap);
     if (tok >= 0) bk_prof_end(tok, 12, 0, 0);
     return BK_LAUNCH_CHECK("k_lz_step");
 }

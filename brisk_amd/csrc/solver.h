@@ -55,8 +55,8 @@ typedef struct Solve {
     int id;
     /* device heap offsets in doubles (block arena arrays have P.T elements) */
     int64_t hbase, hsize;   /* the solve's heap region (bytes) */
-    int64_t X, Z, Zi, LX, LZ, LXi, LZi, dX, dZ, G, Q, R0, F, W1, W2, C, DinvX, DinvZ, Xb;
-    int64_t lzQ, lzw, lzab;
+    int64_t X, Z, Zi, LX, LZ, LXi, LZi, LXiT, LZiT, dX, dZ, G, Q, R0, F, W1, W2, C, DinvX, DinvZ, Xb;
+    int64_t lzQ, lzw, lzt1, lzt2;
     int64_t M, MF, MLi, MLiT, MDinv, Mwork, Dm;
     int64_t y, dy, rp, rhs, a0, h, qv, ev, wv, bvec, pr, pz, pp, pq, pxb, t1, t2, yb;
     int64_t sc;             /* 64 scalars */

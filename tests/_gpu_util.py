@@ -13,11 +13,11 @@ class GemmItem(ctypes.Structure):
 
 
 class MItem(ctypes.Structure):
-    _fields_ = [("n", ctypes.c_int), ("slot", ctypes.c_int), ("off", I64), ("off2", I64), ("off3", I64)]
+    _fields_ = [("n", ctypes.c_int), ("slot", ctypes.c_int), ("off", I64), ("off2", I64), ("off3", I64), ("off4", I64)]
 
 
 class GemvItem(ctypes.Structure):
-    _fields_ = [("M", ctypes.c_int), ("N", ctypes.c_int), ("lda", ctypes.c_int), ("a", I64), ("x", I64), ("y", I64)]
+    _fields_ = [("M", ctypes.c_int), ("N", ctypes.c_int), ("lda", ctypes.c_int), ("a", I64), ("x", I64), ("y", I64), ("tri", ctypes.c_int)]
 
 
 def lib():

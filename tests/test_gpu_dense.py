@@ -15,9 +15,10 @@ def _spd(n, rng, cond=1e3):
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-def test_gemm_batched_ragged(dt, ta, tb):
+@pytest.mark.parametrize("shapes", [[(70, 45, 33), (1, 1, 1), (64, 64, 64), (130, 17, 200), (5, 129, 7)],
+                                    [(300, 257, 129), (192, 200, 64), (640, 384, 31)]], ids=["small-tiles", "large-tiles"])
+def test_gemm_batched_ragged(dt, ta, tb, shapes):
     rng = np.random.default_rng(1 + ta + 2 * tb)
-    shapes = [(70, 45, 33), (1, 1, 1), (64, 64, 64), (130, 17, 200), (5, 129, 7)]
     A_parts, B_parts, C_parts, rows, refs = [], [], [], [], []
     oa = ob = oc = 0
     for (M, N, K) in shapes:
@@ -48,7 +49,7 @@ def test_gemm_batched_ragged(dt, ta, tb):
 def test_gemm_triangular_flags(dt):
     """Zi = Linv' Linv with the K range trimmed (2|8) and a lower-only update (1)."""
     rng = np.random.default_rng(7)
-    n = 150
+    n = 300
     Li = np.tril(rng.standard_normal((n, n)))
     dA = DBuf(Li.flatten(order="F").astype(dt))
     dC = DBuf(np.zeros(n * n, dt))
@@ -93,7 +94,7 @@ def test_potrf_trtri_batched(dt):
     nb = [((n + 63) // 64) * 4096 for n in sizes]
     doff = np.concatenate([[0], np.cumsum(nb)[:-1]]).astype(np.int64)
     woff = np.array(offs, np.int64)
-    rows = [(n, k, offs[k], int(doff[k]), int(woff[k])) for k, n in enumerate(sizes)]
+    rows = [(n, k, offs[k], int(doff[k]), int(woff[k]), offs[k]) for k, n in enumerate(sizes)]
     it = items_array(MItem, rows)
     dA = DBuf(buf); dD = DBuf(nbytes=int(sum(nb)) * np.dtype(dt).itemsize, dtype=dt)
     dW = DBuf(nbytes=buf.nbytes, dtype=dt)
@@ -121,6 +122,19 @@ def test_potrf_trtri_batched(dt):
         assert np.allclose(np.triu(Lig, 1), 0)
         err = np.abs(Lig @ Lg - np.eye(n)).max()
         assert err < (1e-10 if dt == np.float64 else 5e-3), (n, err)
+
+
+def test_gemv_triangular_ranges():
+    """trans = 1 with tri 1 (A lower: rows >= j) and tri 2 (A upper: rows <= j)."""
+    rng = np.random.default_rng(9)
+    n = 333
+    Lw = np.tril(rng.standard_normal((n, n)))
+    x = rng.standard_normal(n)
+    for tri, A in ((1, Lw), (2, Lw.T.copy())):
+        dA = DBuf(A.flatten(order="F")); dx = DBuf(x); dy = DBuf(np.zeros(n))
+        it = items_array(GemvItem, [(n, n, n, 0, 0, 0, tri)])
+        assert lib().bk_gemv_f64(1, 1.0, 0.0, dA.p, dx.p, dy.p, it, 1) == 0
+        assert np.allclose(dy.get(), A.T @ x, rtol=1e-12, atol=1e-11)
 
 
 @pytest.mark.parametrize("trans", [0, 1])
