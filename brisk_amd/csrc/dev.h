@@ -119,6 +119,13 @@ int bk_add_diag(const MItem *it, int n_it, int64_t s_off);
  * reg read from the heap at reg_off (doubles, per slot); entries below 1e-30 flushed to 0 */
 int bk_equil(const MItem *it, int n_it, int prec, int64_t reg_off);
 
+/* ---- dense route ---- */
+/* M (f64 full, at M_off) = sum of nchunk partial products (lower triangles, f32 or f64 by prec,
+ * chunk c at part_off + c * stride elements), summed in chunk order */
+int bk_splitk_reduce(int prec, int64_t part_off, int nchunk, int64_t stride, int m, int64_t M_off);
+/* f32 copy: floats at dst_off_f32 = doubles at src_off (n elements) */
+int bk_d2f(int64_t n, int64_t src_off, int64_t dst_off_f32);
+
 /* ---- constraint operators (route 0: sparse entries) ---- */
 typedef struct { int k, r, c, pad; double v; } SEnt;   /* upper entry (r <= c) of block k */
 typedef struct { int n; int pad; int64_t off; } BlkRef; /* block k: order and offset in the solve's block arena */

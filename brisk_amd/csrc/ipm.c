@@ -122,6 +122,12 @@ int64_t solve_heap_bytes(const HProb *P) {
     b += 5 * (m * m * 8 + 256) + (((m + 63) / 64) * 4096 * 8 + 256);
     b += 20 * (m * 8 + 256) + NSC * 8 + 256;
     b += (m + 1) * 4 + (int64_t)P->nent * (sizeof(SEnt) + 12) + (int64_t)P->nb * sizeof(BlkRef) + (P->npos + 1) * 12 + 2048;
+    if (P->dense) {
+        const int64_t nn = T;
+        b += nn * m * (8 + 4 + 8 + 8) + 4 * 256;      /* Ad, Af, P, B */
+        b += 32 * m * m * 8 + 256;                     /* split-K partials (at most 32) */
+        b += 2 * nn * 4 + 2 * nn * 8 + 4 * 256;        /* f32 factors, M v scratch */
+    }
     return b;
 }
 
@@ -152,6 +158,18 @@ int solve_setup(Solve *S) {
     ALLOCB(cptr, (m + 1) * 4); ALLOCB(ent, (int64_t)(P->nent + 1) * sizeof(SEnt)); ALLOCB(blk, (int64_t)P->nb * sizeof(BlkRef));
     ALLOCB(pptr, (int64_t)(P->npos + 1) * 4); ALLOCB(ppos, (int64_t)(P->npos + 1) * 8); ALLOCB(pcon, (int64_t)(P->nent + 1) * 4);
     ALLOC(pval, (int64_t)(P->nent + 1) * 8);
+    if (P->dense) {
+        const int64_t nn = T;
+        const int tl = (int)((m + 127) / 128), lower = tl * (tl + 1) / 2;
+        int ns_ = (2048 + lower - 1) / lower;
+        if (ns_ < 1) ns_ = 1;
+        if (ns_ > 32) ns_ = 32;
+        if (ns_ > nn / 256) ns_ = nn / 256 > 0 ? (int)(nn / 256) : 1;
+        S->nsplit = ns_;
+        S->kchunk = (int)((nn + ns_ - 1) / ns_);
+        ALLOC(Ad, nn * m * 8); ALLOC(Af, nn * m * 4); ALLOC(Pm, nn * m * 8); ALLOC(Bm, nn * m * 8);
+        ALLOC(Mpart, (int64_t)S->nsplit * m * m * 8); ALLOC(LXf, nn * 4); ALLOC(LZif, nn * 4); ALLOC(Om1, nn * 8); ALLOC(Om2, nn * 8);
+    }
 #undef ALLOC
 #undef ALLOCB
     if (fail) { solve_release(S); return -1; }
@@ -168,6 +186,10 @@ int solve_setup(Solve *S) {
     rc |= bk_h2d(H + S->pptr, P->pptr, (int64_t)(P->npos + 1) * 4);
     if (P->npos) rc |= bk_h2d(H + S->ppos, P->ppos, (int64_t)P->npos * 8);
     if (P->nent) { rc |= bk_h2d(H + S->pcon, P->pcon, (int64_t)P->nent * 4); rc |= bk_h2d(H + S->pval * 8, P->pval, (int64_t)P->nent * 8); }
+    if (P->dense) {
+        rc |= bk_h2d(H + S->Ad * 8, P->Ad, T * m * 8);
+        rc |= bk_d2f(T * m, S->Ad, 2 * S->Af);
+    }
     if (rc) { solve_release(S); return -1; }
     S->status = -1;
     return 0;
@@ -183,6 +205,8 @@ void solve_release(Solve *S) {
 }
 
 /* ---- batched building blocks ------------------------------------------------------------- */
+#define OFF(f) offsetof(Solve, f)
+static inline int64_t F_(Solve *S, size_t f) { return *(int64_t *)((char *)S + f); }
 static ConItem con_item(Solve *S) {
     ConItem c;
     memset(&c, 0, sizeof(c));
@@ -194,9 +218,18 @@ static ConItem con_item(Solve *S) {
 }
 /* out = alpha A(D) + beta out, D and out given as double offsets per solve */
 static int aop(Solve **S, int ns, const int *act, size_t fD, size_t fout, double alpha, double beta) {
+    /* dense route: out = alpha Ad' vec(D) + beta out (one wave per constraint) */
+    GemvL_clear(&g_vl);
+    for (int q = 0; q < ns; q++) {
+        if ((act && !act[q]) || !S[q]->P.dense) continue;
+        const HProb *P = &S[q]->P;
+        GemvItem g = { (int)P->T, P->m, (int)P->T, S[q]->Ad, F_(S[q], fD), F_(S[q], fout), 0 };
+        GemvL_add(&g_vl, g);
+    }
+    if (g_vl.n) { double *H = heapp(); if (bk_gemv_f64(1, alpha, beta, H, H, H, g_vl.v, g_vl.n)) return -1; }
     ConL_clear(&g_cl);
     for (int q = 0; q < ns; q++) {
-        if (act && !act[q]) continue;
+        if ((act && !act[q]) || S[q]->P.dense) continue;
         ConItem c = con_item(S[q]);
         c.D = *(int64_t *)((char *)S[q] + fD); c.out = *(int64_t *)((char *)S[q] + fout); c.alpha = alpha; c.beta = beta;
         ConL_add(&g_cl, c);
@@ -205,17 +238,24 @@ static int aop(Solve **S, int ns, const int *act, size_t fD, size_t fout, double
 }
 /* D += alpha A*(y) at the touched positions */
 static int aty(Solve **S, int ns, const int *act, size_t fy, size_t fD, double alpha) {
+    /* dense route: vec(D) += alpha Ad y (one thread per entry of D, the constraints in order) */
+    GemvL_clear(&g_vl);
+    for (int q = 0; q < ns; q++) {
+        if ((act && !act[q]) || !S[q]->P.dense) continue;
+        const HProb *P = &S[q]->P;
+        GemvItem g = { (int)P->T, P->m, (int)P->T, S[q]->Ad, F_(S[q], fy), F_(S[q], fD), 0 };
+        GemvL_add(&g_vl, g);
+    }
+    if (g_vl.n) { double *H = heapp(); if (bk_gemv_f64(0, alpha, 1.0, H, H, H, g_vl.v, g_vl.n)) return -1; }
     ConL_clear(&g_cl);
     for (int q = 0; q < ns; q++) {
-        if (act && !act[q]) continue;
+        if ((act && !act[q]) || S[q]->P.dense) continue;
         ConItem c = con_item(S[q]);
         c.y = *(int64_t *)((char *)S[q] + fy); c.D = *(int64_t *)((char *)S[q] + fD); c.alpha = alpha;
         ConL_add(&g_cl, c);
     }
     return bk_aty(g_cl.v, g_cl.n);
 }
-#define OFF(f) offsetof(Solve, f)
-static inline int64_t F_(Solve *S, size_t f) { return *(int64_t *)((char *)S + f); }
 
 /* lincomb over the block arena (len T) or vectors (len m): out = sum c_t in_t */
 static void lin_add(int64_t len, int64_t out, int64_t i0, double c0, int64_t i1, double c1, int64_t i2, double c2, int64_t i3, double c3) {
@@ -485,10 +525,60 @@ static int maxstep2(Solve **S, int ns, const int *act, double *ap, double *ad) {
 }
 
 /* ---- the Schur complement: assemble, factor, solve ------------------------------------------ */
-static int schur_assemble(Solve **S, int ns, const int *act) {
+/* dense route (every constraint dense in one block; the reference's route 2, solver.c:2968-3016,
+ * restated as a Gram product): with X = R R' (R = LX) and Zi = S S' (S = LZi'),
+ * M_ij = tr(A_i X A_j Zi) = <P_i, P_j> with P_i = R' A_i S. Two batched GEMMs form every P_i,
+ * then M = P'P is a split-K product (partials summed in a fixed order). prec 0: f32 MFMA on f32
+ * copies, 1: f64. gram: the start point's A A' (X = Zi = I, P = A). */
+static int schur_dense(Solve *s, int prec, int gram) {
+    const HProb *P = &s->P;
+    const int n = P->bn[0], m = P->m;
+    const int64_t nn = (int64_t)n * n, mul = prec ? 1 : 2;
+    double *H = heapp();
+    int64_t Pb;
+    GemmL_clear(&g_gl);
+    if (gram) Pb = prec ? s->Ad : 2 * s->Af;
+    else {
+        int64_t lx = s->LX, lz = s->LZi;
+        if (!prec) {
+            if (bk_d2f(nn, s->LX, 2 * s->LXf) || bk_d2f(nn, s->LZi, 2 * s->LZif)) return -1;
+            lx = 2 * s->LXf; lz = 2 * s->LZif;
+        }
+        const int64_t A = prec ? s->Ad : 2 * s->Af, B = mul * s->Bm;
+        Pb = mul * s->Pm;
+        for (int i = 0; i < m; i++) { GemmItem g = { n, n, n, n, n, n, lx, A + i * nn, B + i * nn, -1, 2 }; GemmL_add(&g_gl, g); }
+        if (prec ? bk_gemm_f64(1, 0, 1.0, 0.0, H, H, H, g_gl.v, g_gl.n, NULL)
+                 : bk_gemm_f32(1, 0, 1.0f, 0.0f, (float *)H, (float *)H, (float *)H, g_gl.v, g_gl.n, NULL)) return -1;
+        GemmL_clear(&g_gl);
+        for (int i = 0; i < m; i++) { GemmItem g = { n, n, n, n, n, n, B + i * nn, lz, Pb + i * nn, -1, 16 }; GemmL_add(&g_gl, g); }
+        if (prec ? bk_gemm_f64(0, 1, 1.0, 0.0, H, H, H, g_gl.v, g_gl.n, NULL)
+                 : bk_gemm_f32(0, 1, 1.0f, 0.0f, (float *)H, (float *)H, (float *)H, g_gl.v, g_gl.n, NULL)) return -1;
+    }
+    GemmL_clear(&g_gl);
+    const int64_t part = mul * s->Mpart;
+    for (int c = 0; c < s->nsplit; c++) {
+        const int64_t k0 = (int64_t)c * s->kchunk;
+        const int kc = (int)((nn - k0) < s->kchunk ? (nn - k0) : s->kchunk);
+        if (kc <= 0) break;
+        GemmItem g = { m, m, kc, (int)nn, (int)nn, m, Pb + k0, Pb + k0, part + (int64_t)c * m * m, -1, 1 };
+        GemmL_add(&g_gl, g);
+    }
+    if (prec ? bk_gemm_f64(1, 0, 1.0, 0.0, H, H, H, g_gl.v, g_gl.n, NULL)
+             : bk_gemm_f32(1, 0, 1.0f, 0.0f, (float *)H, (float *)H, (float *)H, g_gl.v, g_gl.n, NULL)) return -1;
+    if (bk_splitk_reduce(prec, part, g_gl.n, (int64_t)m * m, m, s->M)) return -1;
+    s->m_exact = prec;
+    return 0;
+}
+static int schur_assemble(Solve **S, int ns, const int *act, int gram) {
     ConL_clear(&g_cl);
     for (int q = 0; q < ns; q++) {
         if (act && !act[q]) continue;
+        if (S[q]->P.dense) {
+            const int prec = gram || S[q]->o.prec == PREC_F64 || S[q]->want_double;
+            if (schur_dense(S[q], prec, gram)) return -1;
+            continue;
+        }
+        S[q]->m_exact = 1;
         ConItem c = con_item(S[q]);
         c.X = S[q]->X; c.Zi = S[q]->Zi; c.M = S[q]->M;
         ConL_add(&g_cl, c);
@@ -626,17 +716,32 @@ static int schur_prec(Solve **S, int ns, const int *act, size_t fr, size_t fz) {
     for (int q = 0; q < ns; q++) if (!act || act[q]) lin_add(S[q]->P.m, F_(S[q], fz), S[q]->Dm, 1.0, S[q]->t1, 0, -1, 0, -1, 0);
     return bk_vmul(g_ll.v, g_ll.n);
 }
-/* y = alpha M x + beta y in FP64 (M stored full) */
+/* y = alpha M x + beta y in FP64: a product with the stored M where it is exact, else the
+ * operator form M x = A(X A*(x) Zi) on the f64 data (reference mvfree_apply, solver.c:4755-4773) */
 static int schur_mv(Solve **S, int ns, const int *act, size_t fx, size_t fy, double alpha, double beta) {
     GemvL_clear(&g_vl);
+    int nop = 0;
     for (int q = 0; q < ns; q++) {
         if (act && !act[q]) continue;
         Solve *s = S[q];
+        if (!s->m_exact) { nop++; continue; }
         GemvItem g = { s->P.m, s->P.m, s->P.m, s->M, F_(s, fx), F_(s, fy), 0 };
         GemvL_add(&g_vl, g);
     }
     double *H = heapp();
-    return bk_gemv_f64(1, alpha, beta, H, H, H, g_vl.v, g_vl.n);
+    if (g_vl.n && bk_gemv_f64(1, alpha, beta, H, H, H, g_vl.v, g_vl.n)) return -1;
+    if (!nop) return 0;
+    int *op = calloc(ns, sizeof(int));
+    for (int q = 0; q < ns; q++) op[q] = (!act || act[q]) && !S[q]->m_exact;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (op[q]) lin_add(S[q]->P.T, S[q]->Om1, -1, 0, -1, 0, -1, 0, -1, 0);
+    int rc = bk_lincomb(g_ll.v, g_ll.n);
+    rc = rc || aty(S, ns, op, fx, OFF(Om1), 1.0);
+    rc = rc || blk_gemm(S, ns, op, 0, 0, 0, 1.0, 0.0, OFF(X), OFF(Om1), OFF(Om2));
+    rc = rc || blk_gemm(S, ns, op, 0, 0, 0, 1.0, 0.0, OFF(Om2), OFF(Zi), OFF(Om1));
+    rc = rc || aop(S, ns, op, OFF(Om1), fy, alpha, beta);
+    free(op);
+    return rc ? -1 : 0;
 }
 /* batched PCG on M x = rhs (reference schur_solve_, solver.c:5483-5622); relres[q] out */
 static int schur_solve(Solve **S, int ns, const int *act0, size_t frhs, size_t fx, double tol_rel, double *relres) {
@@ -816,7 +921,7 @@ int ipm_start_batch(Solve **S, int ns) {
     /* X = I, Zi = I; rhs = A(C); M = A A' (route at X = Zi = I) */
     if (set_identity(S, ns, NULL, OFF(X)) || set_identity(S, ns, NULL, OFF(Zi))) goto out;
     if (aop(S, ns, NULL, OFF(C), OFF(rhs), 1.0, 0.0)) goto out;
-    if (schur_assemble(S, ns, NULL)) goto out;
+    if (schur_assemble(S, ns, NULL, 1)) goto out;
     if (schur_factor(S, ns, NULL, 1, fail)) goto out;
     for (int q = 0; q < ns; q++) if (fail[q]) { S[q]->status = 4; S[q]->done = 1; }
     for (int q = 0; q < ns; q++) act[q] = !S[q]->done;
@@ -1015,7 +1120,7 @@ int ipm_iterate_batch(Solve **S, int ns) {
         if (!any) { rc = 0; goto out; }
     }
     /* Schur complement and its factor */
-    if (schur_assemble(S, ns, act)) goto out;
+    if (schur_assemble(S, ns, act, 0)) goto out;
     if (schur_factor(S, ns, act, 0, fail)) goto out;
     for (int q = 0; q < ns; q++) if (act[q] && fail[q]) { S[q]->status = 4; S[q]->done = 1; act[q] = 0; }
     /* a0 = A(Zi); h = A(Zi theta R0 X) */
@@ -1327,14 +1432,10 @@ int ipm_finish(Solve *s) {
     s->res_X = malloc(sizeof(double) * (tot + 1));
     s->res_Z = malloc(sizeof(double) * (tot + 1));
     s->res_y = y;
+    /* Z = C - A'y on the data as read: cs (C~ - A~'y~), left in W1 by the DIMACS evaluation */
     double *Zint = malloc(sizeof(double) * (P->T + 1));
-    for (int64_t q = 0; q < P->T; q++) Zint[q] = P->C[q] * P->cs;
-    for (int p = 0; p < P->npos; p++) {
-        double sum = 0;
-        for (int t = P->pptr[p]; t < P->pptr[p + 1]; t++) sum += y[P->pcon[t]] * P->pval[t] / P->d[P->pcon[t]];
-        Zint[P->ppos[2 * p]] -= sum;
-        if (P->ppos[2 * p + 1] >= 0) Zint[P->ppos[2 * p + 1]] -= sum;
-    }
+    if (bk_d2h(Zint, H + s->W1, sizeof(double) * P->T)) { free(X); free(Zint); return -1; }
+    for (int64_t q = 0; q < P->T; q++) Zint[q] *= P->cs;
     int64_t o = 0;
     for (int k = 0; k < P->nblk_in; k++) {
         const int kb = P->in_first[k];

@@ -147,6 +147,35 @@ extern "C" int bk_vmul(const LinItem *it, int n_it) {
     return BK_LAUNCH_CHECK("k_vmul");
 }
 
+// ---- dense route helpers ------------------------------------------------------------------------
+// M (f64, full symmetric) = sum over the split-K partials of their lower triangles, in chunk order
+template <typename T>
+__global__ void k_splitk_reduce(const T *__restrict__ part, int nchunk, int64_t stride, int m, double *__restrict__ M) {
+    const int64_t tot = (int64_t)m * m;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % m), j = (int)(e / m);
+        if (i < j) continue;
+        double s = 0;
+        for (int c = 0; c < nchunk; c++) s += (double)part[c * stride + e];
+        M[e] = s;
+        M[j + (int64_t)i * m] = s;
+    }
+}
+extern "C" int bk_splitk_reduce(int prec, int64_t part_off, int nchunk, int64_t stride, int m, int64_t M_off) {
+    const unsigned g = bk_grid1((int64_t)m * m, 256, 8192);
+    if (prec) hipLaunchKernelGGL((k_splitk_reduce<double>), dim3(g), dim3(256), 0, g_bk_stream, ((const double *)g_heap) + part_off, nchunk, stride, m, ((double *)g_heap) + M_off);
+    else hipLaunchKernelGGL((k_splitk_reduce<float>), dim3(g), dim3(256), 0, g_bk_stream, ((const float *)g_heap) + part_off, nchunk, stride, m, ((double *)g_heap) + M_off);
+    return BK_LAUNCH_CHECK("k_splitk_reduce");
+}
+// f32 copy of an f64 array (offsets: doubles for the source, floats for the destination)
+__global__ void k_d2f(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) b[e] = (float)a[e];
+}
+extern "C" int bk_d2f(int64_t n, int64_t src_off, int64_t dst_off_f32) {
+    hipLaunchKernelGGL(k_d2f, dim3(bk_grid1(n, 256, 8192)), dim3(256), 0, g_bk_stream, ((const double *)g_heap) + src_off, ((float *)g_heap) + dst_off_f32, n);
+    return BK_LAUNCH_CHECK("k_d2f");
+}
+
 // ---- per-matrix helpers ---------------------------------------------------------------------
 __global__ void k_symmetrize(const MItem *__restrict__ it, int nit, double *__restrict__ H) {
     const int q = blockIdx.y + blockIdx.z * 65535;

@@ -36,8 +36,72 @@ static int cmp_pos(const void *a, const void *b) {
 void hprob_free(HProb *P) {
     free(P->bs_in); free(P->in_first); free(P->bn); free(P->boff);
     free(P->b); free(P->b0); free(P->d); free(P->du); free(P->C);
-    free(P->cptr); free(P->ent); free(P->pptr); free(P->ppos); free(P->pcon); free(P->pval);
+    free(P->cptr); free(P->ent); free(P->pptr); free(P->ppos); free(P->pcon); free(P->pval); free(P->Ad);
     memset(P, 0, sizeof(*P));
+}
+
+static void finish_scaling(HProb *P);
+/* dense route: the scaled A_i in full symmetric storage, column i of Ad = vec(A_i) */
+static int hprob_build_dense(HProb *P, int m, int n, int64_t nnz, const int *mat, const int *blk, const int *ii, const int *jj, const double *v) {
+    const int64_t nn = (int64_t)n * n;
+    P->dense = 1;
+    P->Ad = calloc((size_t)nn * m + 1, sizeof(double));
+    if (!P->Ad) return -1;
+    for (int64_t q = 0; q < nnz; q++) {
+        if (v[q] == 0) continue;
+        int i = ii[q] - 1, j = jj[q] - 1;
+        if (i > j) { const int s = i; i = j; j = s; }
+        (void)blk;
+        if (mat[q] == 0) {
+            P->C[i + (int64_t)j * n] += -v[q];
+            if (i != j) P->C[j + (int64_t)i * n] += -v[q];
+            continue;
+        }
+        double *A = P->Ad + (int64_t)(mat[q] - 1) * nn;
+        A[i + (int64_t)j * n] += v[q];
+        if (i != j) A[j + (int64_t)i * n] += v[q];
+    }
+    finish_scaling(P);
+    /* scale the columns by d_i */
+    for (int c = 0; c < m; c++) {
+        double *A = P->Ad + (int64_t)c * nn;
+        const double dc = P->d[c];
+        for (int64_t q = 0; q < nn; q++) A[q] *= dc;
+    }
+    P->cptr = calloc(m + 1, sizeof(int));
+    P->ent = malloc(sizeof(SEnt));
+    P->pptr = calloc(1, sizeof(int)); P->ppos = malloc(sizeof(int) * 2); P->pcon = malloc(sizeof(int)); P->pval = malloc(sizeof(double));
+    return 0;
+}
+
+/* norms on the data as read, row equilibration and scaling of b and C (problem.c:1866-1909);
+ * the dense route's norms of A_i come from Ad */
+static void finish_scaling(HProb *P) {
+    const int m = P->m;
+    double c2 = 0, c1 = 0;
+    for (int64_t q = 0; q < P->T; q++) { c2 += P->C[q] * P->C[q]; c1 += fabs(P->C[q]); }
+    P->normC2 = sqrt(c2); P->normC1 = c1;
+    P->normb1 = P->normb2 = 0;
+    for (int i = 0; i < m; i++) { P->normb1 += fabs(P->b0[i]); P->normb2 += P->b0[i] * P->b0[i]; }
+    P->normb2 = sqrt(P->normb2);
+    const int64_t nn = P->T;
+    for (int i = 0; i < m; i++) {
+        double s2 = 0;
+        const double *A = P->Ad + (int64_t)i * nn;
+        for (int64_t q = 0; q < nn; q++) s2 += A[q] * A[q];
+        P->d[i] = s2 > 0 ? 1.0 / sqrt(s2) : 1.0;
+    }
+    double nbv = 0;
+    for (int i = 0; i < m; i++) nbv += P->d[i] * P->b[i] * P->d[i] * P->b[i];
+    nbv = sqrt(nbv);
+    P->bs = nbv > 1 ? nbv : 1.0;
+    for (int i = 0; i < m; i++) { P->b[i] *= P->d[i]; P->du[i] = P->bs / P->d[i]; }
+    P->cs = P->normC2 > 1 ? P->normC2 : 1.0;
+    for (int i = 0; i < m; i++) P->b[i] /= P->bs;
+    for (int64_t q = 0; q < P->T; q++) P->C[q] /= P->cs;
+    double nb2 = 0;
+    for (int i = 0; i < m; i++) nb2 += P->b[i] * P->b[i];
+    P->nbs = sqrt(nb2);
 }
 
 /* returns 0, or 2 for data that is not a valid problem (the reference's exit code for bad data) */
@@ -85,6 +149,11 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
         if (!isfinite(v[q])) return 2;
         if (v[q] != 0 && t > 0) ncon++;
     }
+    /* dense route: one SDP block whose constraints fill at least a quarter of their upper
+     * triangles (the reference's route 2, solver.c:2968-3016, taken for every constraint) */
+    if (nblk == 1 && bs[0] > 1 && m > 0 && (double)ncon >= 0.25 * (double)m * bs[0] * (bs[0] + 1) / 2.0 &&
+        (double)m * bs[0] * bs[0] * 8.0 < 64e9)
+        return hprob_build_dense(P, m, bs[0], nnz, mat, blk, ii, jj, v);
     Trip *tr = malloc(sizeof(Trip) * (ncon + 1));
     int64_t nt = 0;
     for (int64_t q = 0; q < nnz; q++) {

@@ -36,6 +36,10 @@ typedef struct {
     int *pptr, *ppos, *pcon;
     double *pval;
     int diag;               /* every constraint is one diagonal entry: Hadamard Schur form */
+    /* dense route: one SDP block with dense constraints; Ad (host) holds the scaled A_i in full
+     * symmetric storage, column i = vec(A_i) (n*n x m); the entry and position lists are empty */
+    int dense;
+    double *Ad;
 } HProb;
 
 enum { PREC_F64 = 0, PREC_MIXED = 1 };
@@ -61,6 +65,9 @@ typedef struct Solve {
     int64_t y, dy, rp, rhs, a0, h, qv, ev, wv, bvec, pr, pz, pp, pq, pxb, t1, t2, yb;
     int64_t sc;             /* 64 scalars */
     int64_t cptr, ent, blk, pptr, ppos, pcon, pval, dinvoff; /* problem data on the device */
+    int64_t Ad, Af, Pm, Bm, Mpart, LXf, LZif, Om1, Om2;  /* dense route: A f64 / f32 (n^2 x m), P, B, split-K partials, f32 factors, M v scratch */
+    int nsplit, kchunk;     /* dense route: split-K of M = P'P */
+    int m_exact;            /* the stored M is exact in f64 (else M v uses the operator form) */
     /* factor state of M */
     int f32;                /* current factor is f32 */
     int want_double;        /* sticky: f32 no longer used */

@@ -255,6 +255,36 @@ static void schur(const OProb *P, const double *X, const double *Zi, double *M) 
         const OBlk *B = &P->B[k];
         const double *Xk = X + B->off, *Zk = Zi + B->off;
         const int n = B->n;
+        /* blocks whose constraints are dense: M_ab += <A_a, X A_b Zi> with the products formed
+         * densely (the reference's route 2, solver.c:2968-3016) - the same sum as below */
+        double ent = 0; int ncon = 0;
+        for (int a = 0; a < m; a++) { const int e = P->A[(size_t)k * m + a].nnz; if (e) { ent += e; ncon++; } }
+        if (!B->lp && ncon > 0 && ent / ncon > 2.0 * n) {
+            const size_t nn = (size_t)n * n;
+            double *Ad = calloc(nn, sizeof(double)), *G = malloc(sizeof(double) * nn), *W = malloc(sizeof(double) * nn);
+            double *Gs = malloc(sizeof(double) * nn * m);
+            for (int b = 0; b < m; b++) {
+                const OEnt *Eb = &P->A[(size_t)k * m + b];
+                if (!Eb->nnz) continue;
+                memset(Ad, 0, sizeof(double) * nn);
+                for (int q = 0; q < Eb->nnz; q++) Ad[Eb->r[q] + (size_t)Eb->c[q] * n] += Eb->v[q];
+                gemm_nn(n, Xk, Ad, W);
+                gemm_nn(n, W, Zk, Gs + (size_t)b * nn);
+            }
+            for (int a = 0; a < m; a++) {
+                const OEnt *Ea = &P->A[(size_t)k * m + a];
+                if (!Ea->nnz) continue;
+                for (int b = 0; b <= a; b++) {
+                    if (!P->A[(size_t)k * m + b].nnz) continue;
+                    const double *Gb = Gs + (size_t)b * nn;
+                    double s = 0;
+                    for (int q = 0; q < Ea->nnz; q++) s += Ea->v[q] * Gb[Ea->c[q] + (size_t)Ea->r[q] * n];
+                    M[a + (size_t)b * m] += s;
+                }
+            }
+            free(Ad); free(G); free(W); free(Gs);
+            continue;
+        }
         for (int a = 0; a < m; a++) {
             const OEnt *Ea = &P->A[(size_t)k * m + a];
             if (!Ea->nnz) continue;

@@ -15,7 +15,9 @@ CASES = [
     ("maxcut_cycle", 7),
     ("maxcut_cycle", 8),
     ("multi", 4, 10, 20, 60, 0),
-    ("dense", 12, 30, 0),
+    ("dense", 12, 30, 0),           # dense route: P-SYRK assembly, operator-form residual
+    ("dense", 40, 120, 1),
+    ("multi", 6, 30, 40, 300, 2),   # several blocks + an LP block
 ]
 
 
