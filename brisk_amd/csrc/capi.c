@@ -247,7 +247,7 @@ API int brisk_solve_batch(int n, const BriskProblem *probs, int nopt, const char
     return solve_many(n, probs, nopt, opts, res);
 }
 
-int brisk_read_sdpa(const char *fname, BriskProblem *p, void **owned);   /* sdpa.c */
+
 API int brisk_solve_file(const char *fname, int nopt, const char *const *opts, BriskResult *res) {
     if (!res) return -1;
     result_clear(res);

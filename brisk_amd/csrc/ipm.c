@@ -1317,8 +1317,8 @@ out:
     return rc;
 }
 
-/* ---- DIMACS errors of the returned point on the data as read (postsolve.c:3037-3112), FP64 on
- * the device. With X = bs X~, y_i = cs d_i y~_i and Z = C - A'y = cs (C~ - A~'y~):
+/* ---- DIMACS errors of the returned point on the data as read (the measure of postsolve,
+ * postsolve.c:2744-2897), FP64 on the device. With X = bs X~, y_i = cs d_i y~_i and Z = C - A'y = cs (C~ - A~'y~):
  *   A(X) - b = du o (A~(X~) - b~), <C,X> = bs cs <C~,X~>, b'y = bs cs b~'y~, <X,Z> = bs cs <X~, Z~>.
  * lambda_min(X) and lambda_min(Z) are tested by Cholesky: a factor that exists proves the block
  * positive definite (error 0); otherwise the Lanczos bound gives the violation. */
@@ -1352,9 +1352,6 @@ int ipm_dimacs(Solve **S, int ns) {
         dot_add(s->P.T, s->C, s->X, SC(q, 0));
         dot_add(s->P.T, s->X, s->W1, SC(q, 1));
         dot_add(s->P.m, s->bvec, s->y, SC(q, 2));
-        /* largest block Frobenius norm of Z: per block */
-        for (int k = 0; k < s->P.nb && k < NSC - 8; k++)
-            if (s->P.bn[k] > 1) dot_add((int64_t)s->P.bn[k] * s->P.bn[k], s->W1 + s->P.boff[k], s->W1 + s->P.boff[k], SC(q, 8 + k));
     }
     if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
     for (int q = 0; q < ns; q++) {
@@ -1367,12 +1364,10 @@ int ipm_dimacs(Solve **S, int ns) {
         if (bk_d2h(hrp, H + s->rp, sizeof(double) * P->m)) goto out;
         double r2 = 0;
         for (int i = 0; i < P->m; i++) { const double v = hrp[i] * P->du[i]; r2 += v * v; }
-        double zres = 0;
-        for (int k = 0; k < P->nb && k < NSC - 8; k++) if (P->bn[k] > 1) zres = fmax(zres, 2.220446049250313e-16 * P->cs * sqrt(HS(q, 8 + k)));
         const double den = 1 + fabs(s->res_pobj) + fabs(s->res_dobj);
         s->res_err[0] = sqrt(r2) / (1 + P->normb1);
         s->res_err[1] = 0; s->res_err[2] = 0;
-        s->res_err[3] = zres > 1e4 * 2.220446049250313e-16 * (1 + P->normC1) ? zres / (1 + P->normC1) : 0;
+        s->res_err[3] = 0;
         s->res_err[4] = (s->res_pobj - s->res_dobj) / den;
         s->res_err[5] = xz / den;
     }
@@ -1391,7 +1386,7 @@ int ipm_dimacs(Solve **S, int ns) {
         for (int q = 0; q < ns; q++) { need[q] = !okz[q]; any |= need[q]; }
         if (any) {
             if (lammin_all(S, ns, need, OFF(W1), lam)) { free(need); goto out; }
-            for (int q = 0; q < ns; q++) if (need[q]) S[q]->res_err[3] = fmax(S[q]->res_err[3], fmax(0.0, -S[q]->P.cs * lam[q]) / (1 + S[q]->P.normC1));
+            for (int q = 0; q < ns; q++) if (need[q]) S[q]->res_err[3] = fmax(0.0, -S[q]->P.cs * lam[q]) / (1 + S[q]->P.normC1);
         }
         free(need);
     }

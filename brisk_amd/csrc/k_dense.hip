@@ -14,7 +14,7 @@
 #include <stdlib.h>
 #include <math.h>
 #include <vector>
-#include "dev.h"
+#include "brisk_dev.h"
 #include "rt_internal.h"
 #include "chol_diag.h"
 

@@ -14,7 +14,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <vector>
-#include "dev.h"
+#include "brisk_dev.h"
 #include "rt_internal.h"
 
 static char *g_heap = nullptr;

@@ -4,7 +4,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <time.h>
-#include "dev.h"
+#include "brisk_dev.h"
 #include "rt_internal.h"
 
 static int g_init = 0;

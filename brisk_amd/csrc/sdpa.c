@@ -34,7 +34,7 @@ static int next_num(Rd *r, double *out, int header) {
 }
 
 /* owned: one allocation holding every array of p (free it after the solve) */
-int brisk_read_sdpa(const char *fname, BriskProblem *p, void **owned) {
+__attribute__((visibility("default"))) int brisk_read_sdpa(const char *fname, BriskProblem *p, void **owned) {
     FILE *f = fopen(fname, "rb");
     if (!f) return -1;
     fseek(f, 0, SEEK_END);
@@ -74,6 +74,8 @@ int brisk_read_sdpa(const char *fname, BriskProblem *p, void **owned) {
     int *mat = (int *)(v + lines), *blk = mat + lines, *ii = blk + lines, *jj = ii + lines;
     for (int k = 0; k < nblk; k++) { if (next_num(&r, &x, 1)) goto bad; bs[k] = (int)x; }
     for (int k = 0; k < m; k++) { if (next_num(&r, &x, 1)) goto bad; c[k] = x; }
+    /* the separators that close the vector c ("}" after its last entry) */
+    while (r.pos < r.len && (isspace((unsigned char)r.buf[r.pos]) || strchr(",(){}=", r.buf[r.pos]))) r.pos++;
     int64_t nnz = 0;
     for (;;) {
         double e[5];
@@ -92,3 +94,5 @@ bad:
     free(r.buf); free(mem);
     return -1;
 }
+
+__attribute__((visibility("default"))) void brisk_sdpa_free(void *owned) { free(owned); }

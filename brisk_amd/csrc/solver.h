@@ -8,7 +8,7 @@
 #define BRISK_SOLVER_H
 #include <stdint.h>
 #include <stddef.h>
-#include "dev.h"
+#include "brisk_dev.h"
 
 #define BK_LZ_K 30          /* Lanczos steps (reference Params.lanczos_k, solver.c:119) */
 

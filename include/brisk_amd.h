@@ -81,6 +81,12 @@ typedef struct {
     const int *mat, *blk, *i, *j;
     const double *v;
 } BriskProblem;
+/* replaces the reader of problem.c:426 (read_sdpa) that brisk_solve_file uses: the numbers of
+ * an SDPA sparse file into p, whose arrays live in one allocation returned in *owned (release
+ * it with brisk_sdpa_free). Returns 0, or -1 when the file cannot be read or parsed. */
+int brisk_read_sdpa(const char *fname, BriskProblem *p, void **owned);
+void brisk_sdpa_free(void *owned);
+
 /* Solve n problems together: every iteration is one set of batched launches over all of them
  * (their results equal their single solves'). res[k] receives problem k; returns 0 when every
  * problem was read and solved (statuses in the results), else the first error code. */

@@ -1,4 +1,4 @@
-/* dev.h - the thin C ABI between the host solver (C) and the HIP side (rt.hip, k_dense.hip,
+/* brisk_dev.h - the thin C ABI between the host solver (C) and the HIP side (rt.hip, k_dense.hip,
  * k_ipm.hip). Plain pointers and sizes only. Every kernel wrapper is asynchronous on the
  * library's one stream; the host reads scalars back with bk_d2h (which synchronises).
  *

@@ -450,7 +450,27 @@ static double lammin_all(const OProb *P, const double *V, double *work) {
     return lam;
 }
 
-/* ---- DIMACS errors on the data as read (postsolve.c:3037-3112) ---- */
+/* lambda_min for the DIMACS cone errors as the reference's ps_lammin_tol (postsolve.c:668-690):
+ * 0 when the block has a Cholesky factor, -delta when only V + delta I has one, else the
+ * eigenvalue itself; LP blocks by their smallest entry. */
+static double lammin_tol_all(const OProb *P, const double *V, double *work, double delta) {
+    double lam = 0;
+    for (int k = 0; k < P->nb; k++) {
+        const OBlk *B = &P->B[k];
+        const int n = B->n;
+        if (B->lp) { for (int i = 0; i < n; i++) lam = fmin(lam, V[B->off + i]); continue; }
+        memcpy(work, V + B->off, sizeof(double) * (size_t)n * n);
+        if (chol(n, work) == 0) continue;
+        memcpy(work, V + B->off, sizeof(double) * (size_t)n * n);
+        for (int i = 0; i < n; i++) work[i + (size_t)i * n] += delta;
+        if (chol(n, work) == 0) { lam = fmin(lam, -delta); continue; }
+        memcpy(work, V + B->off, sizeof(double) * (size_t)n * n);
+        lam = fmin(lam, oracle_lammin(n, work));
+    }
+    return lam;
+}
+
+/* ---- DIMACS errors on the data as read (the measure of postsolve, postsolve.c:2744-2897) ---- */
 void oracle_dimacs_packed(const OProb *P, const double *Xo, const double *yo, double *err, double *pobj, double *dobj);
 static void dimacs(const OProb *P, const double *Xo, const double *yo, double *err, double *po, double *dob, double *work) {
     const int m = P->m;
@@ -460,21 +480,16 @@ static void dimacs(const OProb *P, const double *Xo, const double *yo, double *e
     ATy(P, ys, Zo);
     for (size_t q = 0; q < P->tot; q++) Zo[q] = P->C[q] * P->cs - Zo[q];
     Aop(P, Xo, rp, 1.0);
-    double pobj = 0, dobj = 0, xz = 0, r2 = 0, zres = 0;
+    double pobj = 0, dobj = 0, xz = 0, r2 = 0;
     for (size_t q = 0; q < P->tot; q++) { pobj += P->C[q] * P->cs * Xo[q]; xz += Xo[q] * Zo[q]; }
     for (int i = 0; i < m; i++) { dobj += P->b0[i] * yo[i]; const double v = rp[i] / P->d[i] - P->b0[i]; r2 += v * v; }
-    for (int k = 0; k < P->nb; k++) {
-        const OBlk *B = &P->B[k];
-        if (B->lp) continue;
-        double f2 = 0; for (size_t q = 0; q < (size_t)B->n * B->n; q++) f2 += Zo[B->off + q] * Zo[B->off + q];
-        zres = fmax(zres, 2.220446049250313e-16 * sqrt(f2));
-    }
-    const double lamx = lammin_all(P, Xo, work), lamz = lammin_all(P, Zo, work);
+    const double lamx = lammin_tol_all(P, Xo, work, 1e-12 * (1 + P->normb1));
+    const double lamz = lammin_tol_all(P, Zo, work, 1e-12 * (1 + P->normC1));
     const double den = 1 + fabs(pobj) + fabs(dobj);
     err[0] = sqrt(r2) / (1 + P->normb1);
     err[1] = fmax(0.0, -fmin(lamx, 0.0)) / (1 + P->normb1);
     err[2] = 0;
-    err[3] = fmax(fmax(0.0, -fmin(lamz, 0.0)), zres > 1e4 * 2.220446049250313e-16 * (1 + P->normC1) ? zres : 0) / (1 + P->normC1);
+    err[3] = fmax(0.0, -fmin(lamz, 0.0)) / (1 + P->normC1);
     err[4] = (pobj - dobj) / den;
     err[5] = xz / den;
     *po = pobj; *dob = dobj;

@@ -145,3 +145,85 @@ def oracle_dimacs(inst, X, y):
     lib.oracle_dimacs(ctypes.byref(d), X.ctypes.data_as(PD), y.ctypes.data_as(PD), err.ctypes.data_as(PD),
                       ctypes.byref(po), ctypes.byref(do))
     return err, po.value, do.value
+
+
+def read_sdpa(path):
+    """The numbers of an SDPA sparse file (.dat-s): comment lines starting with '"' or '*',
+    then m, nblk, the block sizes (separators ",(){}" allowed), the m entries of c and the
+    lines "mat blk i j value" (the format as the reference's read_sdpa, problem.c:426)."""
+    toks = []
+    with open(path) as f:
+        lines = f.read().splitlines()
+    k = 0
+    while k < len(lines) and lines[k].lstrip()[:1] in ('"', '*'):
+        k += 1
+    body = " ".join(lines[k:])
+    for ch in ",(){}=":
+        body = body.replace(ch, " ")
+    toks = body.split()
+
+    def num(t):
+        try:
+            return float(t)
+        except ValueError:
+            return None
+    head, q = [], 0
+    while len(head) < 2 or len(head) < 2 + int(head[1]) + int(head[0]):
+        x = num(toks[q])
+        q += 1
+        if x is not None:          # words in the header ("mDIM") are skipped
+            head.append(x)
+    m, nblk = int(head[0]), int(head[1])
+    bs = [int(t) for t in head[2:2 + nblk]]
+    c = head[2 + nblk:2 + nblk + m]
+    rest = toks[q:]
+    rest = rest[:len(rest) // 5 * 5]
+    e = np.array(rest, dtype=np.float64).reshape(-1, 5)
+    return dict(m=m, nblk=nblk, bs=np.array(bs, np.int32), c=np.array(c), mat=e[:, 0].astype(np.int32),
+                blk=e[:, 1].astype(np.int32), i=e[:, 2].astype(np.int32), j=e[:, 3].astype(np.int32),
+                v=np.ascontiguousarray(e[:, 4]))
+
+
+class ArrInstance:
+    """An instance held as numpy arrays (read from a file or built in a test); the same
+    interface as Instance."""
+
+    def __init__(self, d):
+        self.d = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in d.items()}
+        a = self.d
+        a["bs"] = np.ascontiguousarray(a["bs"], np.int32)
+        a["c"] = np.ascontiguousarray(a["c"], np.float64)
+        for k in ("mat", "blk", "i", "j"):
+            a[k] = np.ascontiguousarray(a[k], np.int32)
+        a["v"] = np.ascontiguousarray(a["v"], np.float64)
+        self.s = Inst(a["m"], a["nblk"], a["bs"].ctypes.data_as(PI), a["c"].ctypes.data_as(PD), len(a["v"]),
+                      a["mat"].ctypes.data_as(PI), a["blk"].ctypes.data_as(PI), a["i"].ctypes.data_as(PI),
+                      a["j"].ctypes.data_as(PI), a["v"].ctypes.data_as(PD))
+
+    @property
+    def m(self):
+        return self.d["m"]
+
+    @property
+    def bs(self):
+        return list(self.d["bs"])
+
+    def arrays(self):
+        return dict(self.d)
+
+    def data_args(self):
+        s = self.s
+        return (s.m, s.nblk, s.bs, s.c, s.nnz, s.mat, s.blk, s.i, s.j, s.v)
+
+
+def golden(name="small"):
+    import json
+    with open(os.path.join(ROOT, "tests", "golden", name + ".json")) as f:
+        return json.load(f)
+
+
+def golden_instance(rec):
+    """The instance a golden record was computed on."""
+    if "file" in rec:
+        return ArrInstance(read_sdpa(os.path.join(ROOT, "tests", "golden", rec["file"])))
+    return Instance(rec["generator"], *rec["args"])

@@ -1,0 +1,87 @@
+"""Batched solves and the serving loop on the GPU: every result is the same alone and in the
+mix (same status and iteration count, objectives within 1e-10 relative); requests that arrive
+while others are in flight are admitted as slots free up; limits end a solve with the
+reference's statuses."""
+import numpy as np
+import pytest
+import brisk_amd
+from _ctypes_util import Instance, golden, golden_instance
+
+pytestmark = pytest.mark.gpu
+
+MIX = [("theta", 30, 0.3, 3), ("maxcut", 100, 0.1, 0), ("multi", 4, 10, 20, 60, 0), ("theta", 50, 0.2, 0),
+       ("dense", 12, 30, 0), ("maxcut_cycle", 7), ("multi", 6, 30, 40, 300, 2), ("theta", 80, 0.1, 5)]
+
+
+def tup(inst):
+    a = inst.arrays()
+    return (a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"])
+
+
+@pytest.fixture(scope="module")
+def alone():
+    out = {}
+    for case in MIX:
+        out[case] = brisk_amd.solve_sdpa(*tup(Instance(*case)), options=["-prec", "double"])
+        assert out[case].status == 0
+    return out
+
+
+def same(a, b):
+    assert a.status == b.status and a.iterations == b.iterations, (a.status, b.status, a.iterations, b.iterations)
+    assert abs(a.pobj - b.pobj) <= 1e-10 * (1 + abs(b.pobj))
+    assert abs(a.dobj - b.dobj) <= 1e-10 * (1 + abs(b.dobj))
+    assert np.allclose(a.dimacs, b.dimacs, rtol=1e-4, atol=1e-13)
+
+
+def test_batch_equals_alone(alone):
+    print("batch of", len(MIX), flush=True)
+    rs = brisk_amd.solve_batch([tup(Instance(*c)) for c in MIX], options=["-prec", "double"])
+    for case, r in zip(MIX, rs):
+        print(case, r.status_str, r.iterations, r.pobj, flush=True)
+        same(r, alone[case])
+
+
+def test_serving_equals_alone(alone):
+    """Continuous batching: three slots, requests arriving in waves while others run."""
+    sv = brisk_amd.Server(["-prec", "double", "-slots", "3", "-heapgb", "2"])
+    ids = {}
+    pending = list(MIX)
+    steps, left, max_inflight = 0, 1, 0
+    while pending or left > 0:
+        if pending and steps % 3 == 0:          # two new requests every third step
+            for c in pending[:2]:
+                ids[c] = sv.submit(*tup(Instance(*c)))
+            pending = pending[2:]
+        left = sv.step()
+        max_inflight = max(max_inflight, sv.inflight())
+        steps += 1
+        assert steps < 2000
+    assert 1 < max_inflight <= 3
+    for case, rid in ids.items():
+        got = sv.take(rid)
+        assert got is not None
+        r, lat = got
+        print(case, r.status_str, r.iterations, r.pobj, "latency %.3f s" % lat, flush=True)
+        assert lat > 0
+        same(r, alone[case])
+    sv.close()
+
+
+def test_iteration_limit():
+    r = brisk_amd.solve_sdpa(*tup(Instance("theta", 50, 0.2, 0)), options=["-maxit", "3"])
+    assert r.status_str == "ITERATION LIMIT" and r.exit_code == 13 and r.iterations == 3
+
+
+def test_time_limit():
+    r = brisk_amd.solve_sdpa(*tup(Instance("maxcut", 200, 0.05, 1)), options=["-timelimit", "1e-6"])
+    assert r.status_str == "TIME LIMIT" and r.exit_code == 15
+
+
+def test_mixed_precision_counts():
+    """A mixed-precision solve factors in FP32 while mu is large and in FP64 at the end."""
+    rec = golden("small")["maxcut_200_0.05_s1"]
+    r = brisk_amd.solve_sdpa(*tup(golden_instance(rec)), options=["-prec", "mixed"])
+    assert r.status == 0
+    f32, f64, pcg = r.counts
+    assert f32 > 0 and f64 > 0 and f32 + f64 >= r.iterations

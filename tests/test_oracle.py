@@ -1,0 +1,237 @@
+"""The oracle pinned on the CPU: against the reference's own outputs (golden fixtures written
+by the reference library compiled from its sources), against closed-form optima, and against
+brute-force restatements of its building blocks (Schur complement, smallest eigenvalue, DIMACS
+errors) from their definitions."""
+import math
+import numpy as np
+import pytest
+
+from _ctypes_util import (Instance, ArrInstance, golden, golden_instance, oracle_solve, oracle_dimacs,
+                          oracle_lib, oracle_data, PD, D)
+import ctypes
+
+GOLD = golden()
+TOL_OBJ = 1e-7       # objectives: relative to 1 + |value|
+TOL_DIMACS = 1e-8
+
+
+def rel(a, b):
+    return abs(a - b) / (1.0 + abs(b))
+
+
+def objective_ok(ours, rec, key):
+    """Within 1e-7 of the reference's FP64 solve, or at least as close to the optimum (the
+    reference's double-double solve) as the reference's FP64 solve is."""
+    ref = rec[key]
+    if rel(ours, ref) <= TOL_OBJ:
+        return True
+    dd = rec["dd"][key]
+    return rel(ours, dd) <= max(rel(ref, dd), 1e-9)
+
+
+@pytest.mark.parametrize("name", sorted(GOLD))
+def test_oracle_vs_reference(name):
+    rec = GOLD[name]
+    inst = golden_instance(rec)
+    o = oracle_solve(inst)
+    assert rec["status"] == 0 and o["status"] == 0, (rec["status"], o["status"])
+    assert objective_ok(o["pobj"], rec, "pobj"), (o["pobj"], rec["pobj"], rec["dd"]["pobj"])
+    assert objective_ok(o["dobj"], rec, "dobj"), (o["dobj"], rec["dobj"], rec["dd"]["dobj"])
+    assert max(abs(e) for e in o["err"]) <= TOL_DIMACS, o["err"]
+    # the errors the oracle reports are those of the point it returns
+    err, po, do = oracle_dimacs(inst, o["X"], o["y"])
+    assert np.allclose(err, o["err"], rtol=1e-6, atol=1e-15)
+    assert po == pytest.approx(o["pobj"], rel=1e-13) and do == pytest.approx(o["dobj"], rel=1e-13)
+
+
+@pytest.mark.parametrize("name", sorted(k for k in GOLD if "X" in GOLD[k]))
+def test_oracle_dimacs_on_reference_point(name):
+    """The DIMACS errors and objectives the oracle computes at the reference's own returned
+    point equal those the reference reported for it."""
+    rec = GOLD[name]
+    inst = golden_instance(rec)
+    err, po, do = oracle_dimacs(inst, np.array(rec["X"]), np.array(rec["y"]))
+    assert po == pytest.approx(rec["pobj"], rel=1e-12, abs=1e-13)
+    assert do == pytest.approx(rec["dobj"], rel=1e-12, abs=1e-13)
+    ref = np.array(rec["dimacs"])
+    # the cone errors (2 and 4) are eigenvalues resolved to about eps |X| (eigen solvers differ)
+    tol = 1e-13 + 1e-6 * np.abs(ref) + np.array([0, 1e-11, 0, 1e-11, 0, 0])
+    assert np.all(np.abs(err - ref) <= tol), (err, ref)
+
+
+# ---- closed forms -----------------------------------------------------------------------------
+
+def cycle_edges(n):
+    return [(i, (i + 1) % n) for i in range(n)]
+
+
+def petersen():
+    outer = [(i, (i + 1) % 5) for i in range(5)]
+    spokes = [(i, i + 5) for i in range(5)]
+    inner = [(5 + i, 5 + (i + 2) % 5) for i in range(5)]
+    return outer + spokes + inner
+
+
+THETA = [
+    ("C5", 5, cycle_edges(5), math.sqrt(5.0)),
+    ("C7", 7, cycle_edges(7), 7 * math.cos(math.pi / 7) / (1 + math.cos(math.pi / 7))),
+    ("C9", 9, cycle_edges(9), 9 * math.cos(math.pi / 9) / (1 + math.cos(math.pi / 9))),
+    ("C41", 41, cycle_edges(41), 41 * math.cos(math.pi / 41) / (1 + math.cos(math.pi / 41))),
+    ("C6", 6, cycle_edges(6), 3.0),
+    ("K6", 6, [(i, j) for i in range(6) for j in range(i + 1, 6)], 1.0),
+    ("empty5", 5, [], 5.0),
+    ("Petersen", 10, petersen(), 4.0),
+]
+
+
+@pytest.mark.parametrize("case", THETA, ids=[c[0] for c in THETA])
+def test_oracle_theta_closed_form(case):
+    """Lovasz theta: odd cycle n cos(pi/n)/(1+cos(pi/n)), even cycle n/2, complete graph 1,
+    empty graph n, Petersen graph 4; the solve's primal value is -theta."""
+    _, n, edges, theta = case
+    inst = Instance("theta_edges", n, edges)
+    assert inst.m == 1 + len(edges)
+    o = oracle_solve(inst)
+    assert o["status"] == 0
+    assert o["pobj"] == pytest.approx(-theta, rel=1e-7)
+    assert o["dobj"] == pytest.approx(-theta, rel=1e-7)
+
+
+@pytest.mark.parametrize("n", [3, 5, 7, 8, 12, 15])
+def test_oracle_maxcut_cycle_closed_form(n):
+    """Max-cut relaxation of the unit-weight cycle: n for even n, n (1 + cos(pi/n)) / 2 for odd n
+    (the Goemans-Williamson bound of C_n); the solve minimises -<L/4, X>."""
+    o = oracle_solve(Instance("maxcut_cycle", n))
+    want = n if n % 2 == 0 else n * (1 + math.cos(math.pi / n)) / 2
+    assert o["status"] == 0
+    assert o["pobj"] == pytest.approx(-want, rel=1e-7)
+
+
+def test_oracle_example1_value():
+    """The SDPA manual's example 1: optimal value 41.9 (BRISK's sign convention)."""
+    rec = GOLD["file_example1"]
+    o = oracle_solve(golden_instance(rec))
+    assert o["status"] == 0 and o["dobj"] == pytest.approx(41.9, abs=1e-6)
+
+
+# SDPLIB's published optimal values (7 digits), in BRISK's sign convention
+SDPLIB = {"file_truss1": 8.999996, "file_control1": -17.78463, "file_theta1": -23.0, "file_theta3": -42.16698,
+          "file_arch0": -0.5665172, "file_mcp250-1": -317.2643}
+
+
+@pytest.mark.parametrize("name", sorted(SDPLIB))
+def test_oracle_sdplib_values(name):
+    o = oracle_solve(golden_instance(GOLD[name]))
+    assert o["status"] == 0 and abs(o["pobj"] - SDPLIB[name]) <= 6e-7 * (1 + abs(SDPLIB[name]))
+
+
+# ---- building blocks from their definitions ---------------------------------------------------
+
+def dense_constraints(inst):
+    """A_i (i = 1..m) and C = -F0 as lists of dense blocks (LP blocks as diagonal matrices),
+    each A_i scaled by 1/||A_i||_F (row equilibration)."""
+    a = inst.arrays()
+    m, bs = a["m"], [abs(int(b)) for b in a["bs"]]
+    A = [[np.zeros((n, n)) for n in bs] for _ in range(m + 1)]
+    for t, k, i, j, v in zip(a["mat"], a["blk"], a["i"], a["j"], a["v"]):
+        B = A[t][k - 1]
+        B[i - 1, j - 1] += v
+        if i != j:
+            B[j - 1, i - 1] += v
+    C = [-B for B in A[0]]
+    out = []
+    for t in range(1, m + 1):
+        nrm = math.sqrt(sum(float(np.sum(B * B)) for B in A[t]))
+        out.append([B / nrm if nrm > 0 else B for B in A[t]])
+    return out, C, bs, [int(b) < 0 for b in a["bs"]]
+
+
+def random_spd(n, rng, lp):
+    if lp:
+        return np.diag(rng.uniform(0.5, 2.0, n))
+    G = rng.standard_normal((n, n))
+    return G @ G.T / n + 0.5 * np.eye(n)
+
+
+def pack(blocks, lp):
+    return np.concatenate([np.diag(B) if l else B.reshape(-1, order="F") for B, l in zip(blocks, lp)])
+
+
+SCHUR_CASES = [("multi", (2, 6, 5, 12, 4)), ("multi", (3, 5, 0, 9, 1)), ("dense", (6, 10, 2)),
+               ("theta", (9, 0.4, 1)), ("maxcut", (10, 0.3, 2))]
+
+
+@pytest.mark.parametrize("case", SCHUR_CASES, ids=["%s%r" % c for c in SCHUR_CASES])
+def test_oracle_schur_brute_force(case):
+    """M_ab = <A_a, X A_b Zi> summed over blocks, from dense matrices (the HKM Schur complement),
+    against the oracle's assembly (route-0 expansion, Hadamard form or dense formula)."""
+    inst = Instance(case[0], *case[1])
+    A, C, bs, lp = dense_constraints(inst)
+    rng = np.random.default_rng(7)
+    X = [random_spd(n, rng, l) for n, l in zip(bs, lp)]
+    Zi = [random_spd(n, rng, l) for n, l in zip(bs, lp)]
+    m = inst.m
+    M = np.zeros((m, m))
+    for a in range(m):
+        for b in range(m):
+            M[a, b] = sum(float(np.sum(A[a][k] * (X[k] @ A[b][k] @ Zi[k]))) for k in range(len(bs)))
+    Mo = np.zeros(m * m)
+    xp, zp = pack(X, lp), pack(Zi, lp)
+    rc = oracle_lib().oracle_schur(ctypes.byref(oracle_data(inst)), xp.ctypes.data_as(PD), zp.ctypes.data_as(PD),
+                                   Mo.ctypes.data_as(PD))
+    assert rc == 0
+    Mo = Mo.reshape(m, m, order="F")
+    assert np.allclose(Mo, M, rtol=1e-11, atol=1e-12 * np.abs(M).max())
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 16, 40])
+def test_oracle_lammin(n):
+    """The smallest eigenvalue (Householder tridiagonalisation + bisection) is a lower bound
+    within 1e-10 of numpy's."""
+    rng = np.random.default_rng(n)
+    for trial in range(4):
+        G = rng.standard_normal((n, n))
+        S = (G + G.T) / 2 + (trial - 1.5) * np.eye(n)
+        lo = np.linalg.eigvalsh(S)[0]
+        B = np.asfortranarray(S).reshape(-1, order="F").copy()
+        got = oracle_lib().oracle_lammin(n, B.ctypes.data_as(PD))
+        sc = np.abs(S).max()
+        assert got <= lo + 1e-14 * sc and got >= lo - 1e-10 * (1 + sc)
+
+
+@pytest.mark.parametrize("name", ["file_example2", "file_truss1", "multi_4_10_20_60_s0", "theta_30_0.3_s3"])
+def test_oracle_dimacs_definitions(name):
+    """DIMACS errors at an arbitrary point from their definitions (Mittelmann's benchmark):
+    err1 = ||A(X) - b|| / (1 + ||b||_1), err2 = max(0, -lmin(X)) / (1 + ||b||_1),
+    err3 = ||C - A'y - Z|| / (1 + ||C||_1) with Z = C - A'y, err4 = max(0, -lmin(Z)) / (1 + ||C||_1),
+    err5 = (<C,X> - b'y) / (1 + |<C,X>| + |b'y|), err6 = <X,Z> / (1 + |<C,X>| + |b'y|);
+    ||C||_1 sums |entries| of every block, lmin over all blocks."""
+    rec = GOLD[name]
+    inst = golden_instance(rec)
+    a = inst.arrays()
+    m, bs = a["m"], [abs(int(b)) for b in a["bs"]]
+    lp = [int(b) < 0 for b in a["bs"]]
+    Araw = [[np.zeros((n, n)) for n in bs] for _ in range(m + 1)]
+    for t, k, i, j, v in zip(a["mat"], a["blk"], a["i"], a["j"], a["v"]):
+        Araw[t][k - 1][i - 1, j - 1] += v
+        if i != j:
+            Araw[t][k - 1][j - 1, i - 1] += v
+    C = [-B for B in Araw[0]]
+    b = np.array(a["c"])
+    rng = np.random.default_rng(3)
+    X = [random_spd(n, rng, l) - 0.3 * np.eye(n) for n, l in zip(bs, lp)]
+    y = rng.standard_normal(m)
+    Z = [C[k] - sum(y[i] * Araw[i + 1][k] for i in range(m)) for k in range(len(bs))]
+    AX = np.array([sum(float(np.sum(Araw[i + 1][k] * X[k])) for k in range(len(bs))) for i in range(m)])
+    nb1 = np.abs(b).sum()
+    nc1 = sum(float(np.abs(np.diag(B)).sum() if l else np.abs(B).sum()) for B, l in zip(C, lp))
+    lx = min(float(np.linalg.eigvalsh(B)[0]) for B in X)
+    lz = min(float(np.linalg.eigvalsh(B)[0]) for B in Z)
+    po = sum(float(np.sum(C[k] * X[k])) for k in range(len(bs)))
+    do = float(b @ y)
+    xz = sum(float(np.sum(X[k] * Z[k])) for k in range(len(bs)))
+    want = [np.linalg.norm(AX - b) / (1 + nb1), max(0.0, -lx) / (1 + nb1), 0.0, max(0.0, -lz) / (1 + nc1),
+            (po - do) / (1 + abs(po) + abs(do)), xz / (1 + abs(po) + abs(do))]
+    err, gpo, gdo = oracle_dimacs(inst, pack(X, lp), y)
+    assert gpo == pytest.approx(po, rel=1e-12) and gdo == pytest.approx(do, rel=1e-12)
+    assert np.allclose(err, want, rtol=1e-8, atol=1e-14), (err, want)
