@@ -10,7 +10,9 @@ its own sources into oracle/_ref/, where present) solves the same instance in a 
 the host cores as the reported baseline. One JSON line on stdout.
 
 Other workloads (--workload): c1 (Lovasz theta of G(50, 0.2)), c3 (dense n=400, m=2000),
-c4 (64 blocks of 60 + LP 500, m=5000), c5 (the serving stream: solves/s, p50/p99 latency).
+c4 (64 blocks of 60 + LP 500, m=5000), c5 (the serving stream: solves/s with every request
+queued at once, then p50/p99 latency under Poisson arrivals at --load times that rate; the
+reference's CPU baseline on every 16th request, one thread per solve in a pool of processes).
 """
 import argparse
 import ctypes
@@ -39,6 +41,23 @@ WORKLOADS = {
     "c3": dict(kind="dense", args=(400, 2000, 0), desc="dense random SDP, one block n=400, m=2000, seed 0, feasible by X0/Z0/y0"),
     "c4": dict(kind="multi", args=(64, 60, 500, 5000, 0), desc="64 PSD blocks of 60 + LP 500, m=5000, 1-3 blocks per row, seed 0"),
 }
+
+
+# the serving stream: theta / max-cut requests of uneven sizes with Poisson arrivals
+C5 = dict(count=4096, seed=0, nmin=20, nmax=200, pmin=0.05, pmax=0.3,
+          desc="serving stream: 4096 theta/max-cut SDPs (50/50), n ~ U{20..200}, p ~ U[0.05,0.3], Poisson arrivals, seed 0")
+
+
+def c5_stream(count=C5["count"], seed=C5["seed"]):
+    """Request k: (kind, n, p, instance seed) and a unit-rate exponential inter-arrival gap
+    (scaled by 1/lambda at run time)."""
+    rng = np.random.default_rng(seed)
+    theta = rng.random(count) < 0.5
+    n = rng.integers(C5["nmin"], C5["nmax"] + 1, count)
+    p = rng.uniform(C5["pmin"], C5["pmax"], count)
+    gaps = rng.exponential(1.0, count)
+    reqs = [("theta" if theta[k] else "maxcut", int(n[k]), float(p[k]), 1000 + k) for k in range(count)]
+    return reqs, gaps
 
 
 def instance(name):
@@ -95,6 +114,155 @@ print(json.dumps(dict(rc=rc, status=L.brisk_result_status(r), pobj=L.brisk_resul
     return dict(error=(out.stderr or out.stdout)[-400:])
 
 
+def cpu_serving_sample(stride, workers):
+    """The reference library on every stride-th request of the stream, one thread per solve
+    (the library is not reentrant), a pool of worker processes; prints one JSON line."""
+    import multiprocessing as mp
+    reqs, _ = c5_stream()
+    sample = reqs[::stride]
+    t0 = time.time()
+    with mp.get_context("fork").Pool(workers, initializer=_ref_worker_init) as pool:
+        out = list(pool.imap_unordered(_ref_worker_solve, sample, chunksize=1))
+    wall = time.time() - t0
+    print(json.dumps(dict(count=len(sample), wall=wall, solves_per_s=len(sample) / wall, workers=workers,
+                          optimal=sum(1 for o in out if o[0] == 0), cpu_s=sum(o[1] for o in out))))
+
+
+_REF = None
+
+
+def _ref_worker_init():
+    global _REF
+    L = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libbrisk.so"))
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    pd, pi, ppc = ctypes.POINTER(D), ctypes.POINTER(I), ctypes.POINTER(ctypes.c_char_p)
+    L.brisk_result_new.restype = P
+    L.brisk_result_delete.argtypes = [P]
+    L.brisk_solve_data.argtypes = [I, I, pi, pd, ctypes.c_int64, pi, pi, pi, pi, pd, I, ppc, P]
+    L.brisk_result_status.argtypes = [P]
+    L.brisk_set_output.argtypes = [I, P]
+    L.brisk_set_output(1, None)
+    _REF = L
+
+
+def _ref_worker_solve(req):
+    from _ctypes_util import Instance
+    kind, n, p, seed = req
+    inst = Instance(kind, n, p, seed)
+    opts = ["-threads", "1", "-q"]
+    o = (ctypes.c_char_p * len(opts))(*[x.encode() for x in opts])
+    r = _REF.brisk_result_new()
+    t = time.time()
+    _REF.brisk_solve_data(*inst.data_args(), len(opts), o, r)
+    t = time.time() - t
+    st = _REF.brisk_result_status(r)
+    _REF.brisk_result_delete(r)
+    return st, t
+
+
+def run_stream(insts, arrivals, opts):
+    """Drive the serving loop: submit each request at its arrival time, step, collect; returns
+    (makespan s, latencies s (finish - arrival), statuses, iterations)."""
+    import brisk_amd
+    sv = brisk_amd.Server(opts)
+    N = len(insts)
+    rid_of = {}
+    lat = np.zeros(N)
+    status = np.full(N, -9)
+    iters = np.zeros(N, dtype=int)
+    nxt, fin, left = 0, 0, 0
+    t0 = time.perf_counter()
+    while fin < N:
+        now = time.perf_counter() - t0
+        while nxt < N and arrivals[nxt] <= now:
+            rid_of[sv.submit(*insts[nxt])] = nxt
+            nxt += 1
+            left += 1
+        if left == 0 and nxt < N:
+            time.sleep(max(0.0, min(arrivals[nxt] - now, 1e-3)))
+            continue
+        left = sv.step()
+        for rid in sv.done():
+            res, _ = sv.take(rid)
+            k = rid_of[rid]
+            lat[k] = time.perf_counter() - t0 - arrivals[k]
+            status[k] = res.status
+            iters[k] = res.iterations
+            fin += 1
+    span = time.perf_counter() - t0
+    sv.close()
+    return span, lat, status, iters
+
+
+def serving(args):
+    """The serving workload: solves/s with every request queued at t = 0 (saturation), then
+    p50/p99 latency under Poisson arrivals at a fixed fraction of that rate."""
+    from _ctypes_util import Instance
+    count = args.requests or C5["count"]
+    reqs, gaps = c5_stream()
+    reqs, gaps = reqs[:count], gaps[:count]
+    insts = []
+    for kind, n, p, seed in reqs:
+        a = Instance(kind, n, p, seed).arrays()
+        insts.append((a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"]))
+    ms = [int(x[1].shape[0]) for x in insts]
+    print("bench: %d requests, m from %d to %d (mean %.0f)" % (count, min(ms), max(ms), np.mean(ms)), file=sys.stderr, flush=True)
+    opts = ["-prec", "mixed", "-slots", str(args.slots)]
+    for w in range(args.warmup):
+        span, _, st, _ = run_stream(insts[:64], np.zeros(64), opts)
+        print("bench: warmup %d: 64 requests in %.3fs" % (w, span), file=sys.stderr, flush=True)
+    runs = []
+    for k in range(args.steps):
+        span, lat, st, it = run_stream(insts, np.zeros(count), opts)
+        runs.append(span)
+        print("bench: saturation run %d: %d requests in %.3fs = %.1f solves/s; optimal %d, p50 %.3fs p99 %.3fs, mean iterations %.1f"
+              % (k, count, span, count / span, int(np.sum(st == 0)), np.percentile(lat, 50), np.percentile(lat, 99), it.mean()),
+              file=sys.stderr, flush=True)
+    span = float(np.median(runs))
+    rate = count / span
+    lam = args.load * rate
+    arr = np.cumsum(gaps) / lam
+    pspan, plat, pst, pit = run_stream(insts, arr, opts)
+    print("bench: Poisson arrivals at %.1f/s (%.2f x the saturation rate): %d requests in %.3fs, optimal %d, p50 %.3fs, p99 %.3fs"
+          % (lam, args.load, count, pspan, int(np.sum(pst == 0)), np.percentile(plat, 50), np.percentile(plat, 99)),
+          file=sys.stderr, flush=True)
+    cpu = None
+    if not args.skip_cpu and os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libbrisk.so")):
+        workers = args.cpu_threads or min(16, os.cpu_count() or 1)
+        env = dict(os.environ, BRISK_NOBUSY="1", OMP_NUM_THREADS="1")
+        try:
+            out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-serving-sample", str(args.cpu_stride),
+                                  "--cpu-threads", str(workers)], capture_output=True, text=True, timeout=900, env=env)
+            line = [x for x in out.stdout.splitlines() if x.startswith("{")]
+            ref = json.loads(line[-1]) if line else dict(error=out.stderr[-400:])
+        except subprocess.TimeoutExpired:
+            ref = dict(error="timeout")
+        if "error" not in ref:
+            cpu = dict(value=round(ref["solves_per_s"], 3), unit="solves/s", cores=workers, kind="reference",
+                       sample="every %dth request of the stream (%d solves, %d optimal) by the reference library, one thread per "
+                              "solve in %d worker processes, %.1f s wall, %s" % (args.cpu_stride, ref["count"], ref["optimal"], workers,
+                                                                                 ref["wall"], cpu_model()))
+        else:
+            cpu = dict(value=None, unit="solves/s", cores=workers, kind="reference", sample="not run: %s" % ref["error"])
+    import brisk_amd
+    out = {
+        "metric": "time-to-1e-8 DIMACS (FP64 vs mixed, 1 GPU); batched solves/s + p50/p99 latency",
+        "value": round(rate, 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * span, 2), "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic (seeded generators, %s)" % C5["desc"],
+        "config": {"workload": C5["desc"], "requests": count, "precision": "mixed", "tol": 1e-8, "slots": args.slots,
+                   "m_range": [min(ms), max(ms)]},
+        "optimal": int(np.sum(st == 0)), "latency_saturation_s": {"p50": round(float(np.percentile(lat, 50)), 4),
+                                                                 "p99": round(float(np.percentile(lat, 99)), 4)},
+        "poisson": {"rate_per_s": round(lam, 2), "load": args.load, "span_s": round(pspan, 3), "optimal": int(np.sum(pst == 0)),
+                    "p50_s": round(float(np.percentile(plat, 50)), 4), "p99_s": round(float(np.percentile(plat, 99)), 4)},
+        "cpu_baseline": cpu,
+        "gpu_over_cpu": None if not cpu or not cpu.get("value") else round(rate / cpu["value"], 2),
+        "device": brisk_amd.load_library().bk_device_name().decode(),
+    }
+    print(json.dumps(out))
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -110,15 +278,26 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c5"])
+    ap.add_argument("--requests", type=int, default=0, help="c5: requests of the stream to serve (default 4096)")
+    ap.add_argument("--slots", type=int, default=128, help="c5: most solves in flight")
+    ap.add_argument("--load", type=float, default=0.8, help="c5: Poisson arrival rate as a fraction of the saturation rate")
+    ap.add_argument("--cpu-stride", type=int, default=16, help="c5: the CPU baseline solves every k-th request")
+    ap.add_argument("--cpu-serving-sample", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
+    if args.cpu_serving_sample:
+        cpu_serving_sample(args.cpu_serving_sample, args.cpu_threads or min(16, os.cpu_count() or 1))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (every GPU box here has one); --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c5":
+        serving(args)
+        return
     import torch
     import brisk_amd
     from _ctypes_util import Instance  # noqa: F401

@@ -68,6 +68,7 @@ def load_library(path=None):
     L.brisk_serve_step.argtypes, L.brisk_serve_step.restype = [P], I
     L.brisk_serve_take.argtypes, L.brisk_serve_take.restype = [P, I, P, PD], I
     L.brisk_serve_inflight.argtypes, L.brisk_serve_inflight.restype = [P], I
+    L.brisk_serve_done.argtypes, L.brisk_serve_done.restype = [P, PI, I], I
     L.brisk_serve_delete.argtypes, L.brisk_serve_delete.restype = [P], None
     L.bk_prof_enable.argtypes, L.bk_prof_enable.restype = [I], None
     L.bk_prof_reset.argtypes, L.bk_prof_reset.restype = [], None
@@ -296,6 +297,12 @@ class Server:
 
     def inflight(self):
         return self.L.brisk_serve_inflight(self.h)
+
+    def done(self, cap=4096):
+        """Ids of the requests finished since the last call, in finishing order."""
+        buf = (I * cap)()
+        k = self.L.brisk_serve_done(self.h, buf, cap)
+        return list(buf[:k])
 
     def close(self):
         if self.h:

@@ -35,6 +35,7 @@ struct BriskServe {
     Solve **fly;                      /* solves in flight */
     int *fly_id;
     int nfly;
+    int *fin, nfin, capfin;           /* finished ids not yet reported by brisk_serve_done */
 };
 
 API BriskServe *brisk_serve_new(int nopt, const char *const *opts) {
@@ -82,6 +83,11 @@ API int brisk_serve_submit(BriskServe *sv, const BriskProblem *p) {
     return id;
 }
 
+static void fin_push(BriskServe *sv, int id) {
+    if (sv->nfin == sv->capfin) { sv->capfin = sv->capfin ? 2 * sv->capfin : 256; sv->fin = realloc(sv->fin, sizeof(int) * sv->capfin); }
+    sv->fin[sv->nfin++] = id;
+}
+
 static void finish_done(BriskServe *sv) {
     int nd = 0;
     for (int q = 0; q < sv->nfly; q++) nd += sv->fly[q]->done;
@@ -99,6 +105,7 @@ static void finish_done(BriskServe *sv) {
         brisk_fill_result(r->res, D[q], 0.0, t0 - r->t_admit, (t1 - t0) / nd, 0.0);
         r->t_done = bk_now();
         r->state = 2;
+        fin_push(sv, D[q]->id);
         solve_release(D[q]);
         hprob_free(&D[q]->P);
         free(D[q]->res_X); free(D[q]->res_Z); free(D[q]->res_y);
@@ -124,7 +131,16 @@ API int brisk_serve_step(BriskServe *sv) {
         Solve *s = sv->req[id].s;
         const size_t need = (size_t)solve_heap_bytes(&s->P) + ((size_t)1 << 20);
         if (heap_largest_free() < need) {
-            if (sv->nfly + nn == 0 && brisk_heap_ensure(need)) { sv->req[id].state = 2; sv->qhead++; s->status = 4; continue; }
+            if (sv->nfly + nn == 0 && brisk_heap_ensure(need)) {
+                /* larger than the device memory: finished unsolved (status -1, exit code -1) */
+                Req *r = &sv->req[id];
+                r->res = brisk_result_new();
+                r->t_done = bk_now();
+                r->state = 2;
+                sv->qhead++;
+                fin_push(sv, id);
+                continue;
+            }
             if (heap_largest_free() < need) break;
         }
         if (solve_setup(s)) break;
@@ -165,6 +181,17 @@ API int brisk_serve_take(BriskServe *sv, int id, BriskResult *res, double *laten
 }
 API int brisk_serve_inflight(const BriskServe *sv) { return sv ? sv->nfly : 0; }
 
+/* ids of the requests finished since the last call, in finishing order (at most cap; the rest
+ * stay for the next call) */
+API int brisk_serve_done(BriskServe *sv, int *ids, int cap) {
+    if (!sv || !ids || cap <= 0) return 0;
+    const int k = sv->nfin < cap ? sv->nfin : cap;
+    memcpy(ids, sv->fin, sizeof(int) * k);
+    memmove(sv->fin, sv->fin + k, sizeof(int) * (sv->nfin - k));
+    sv->nfin -= k;
+    return k;
+}
+
 API void brisk_serve_delete(BriskServe *sv) {
     if (!sv) return;
     for (int q = 0; q < sv->nfly; q++) { solve_release(sv->fly[q]); }
@@ -174,6 +201,6 @@ API void brisk_serve_delete(BriskServe *sv) {
         if (r->res) brisk_result_delete(r->res);
     }
     for (int k = 0; k < sv->nopt; k++) free(sv->opts[k]);
-    free(sv->opts); free(sv->req); free(sv->queue); free(sv->fly); free(sv->fly_id);
+    free(sv->opts); free(sv->req); free(sv->queue); free(sv->fly); free(sv->fly_id); free(sv->fin);
     free(sv);
 }

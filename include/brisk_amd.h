@@ -106,6 +106,8 @@ int brisk_serve_submit(BriskServe *sv, const BriskProblem *p);
 int brisk_serve_step(BriskServe *sv);
 int brisk_serve_take(BriskServe *sv, int id, BriskResult *res, double *latency);
 int brisk_serve_inflight(const BriskServe *sv);
+/* ids of the requests finished since the last call (at most cap), in finishing order */
+int brisk_serve_done(BriskServe *sv, int *ids, int cap);
 void brisk_serve_delete(BriskServe *sv);
 
 #ifdef __cplusplus

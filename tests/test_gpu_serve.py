@@ -58,6 +58,8 @@ def test_serving_equals_alone(alone):
         steps += 1
         assert steps < 2000
     assert 1 < max_inflight <= 3
+    finished = sv.done()
+    assert sorted(finished) == sorted(ids.values())
     for case, rid in ids.items():
         got = sv.take(rid)
         assert got is not None
