@@ -153,7 +153,7 @@ int solve_setup(Solve *S) {
     ALLOC(M, m * m * 8); ALLOC(MF, m * m * 8); ALLOC(MLi, m * m * 8); ALLOC(MLiT, m * m * 8); ALLOC(Mwork, m * m * 8);
     ALLOC(MDinv, ((m + 63) / 64) * 4096 * 8); ALLOC(Dm, m * 8);
     ALLOC(y, m * 8); ALLOC(dy, m * 8); ALLOC(rp, m * 8); ALLOC(rhs, m * 8); ALLOC(a0, m * 8); ALLOC(h, m * 8); ALLOC(qv, m * 8);
-    ALLOC(ev, m * 8); ALLOC(wv, m * 8); ALLOC(bvec, m * 8); ALLOC(pr, m * 8); ALLOC(pz, m * 8); ALLOC(pp, m * 8); ALLOC(pq, m * 8);
+    ALLOC(ev, m * 8); ALLOC(wv, m * 8); ALLOC(bvec, m * 8); ALLOC(duv, m * 8); ALLOC(pr, m * 8); ALLOC(pz, m * 8); ALLOC(pp, m * 8); ALLOC(pq, m * 8);
     ALLOC(pxb, m * 8); ALLOC(t1, m * 8); ALLOC(t2, m * 8); ALLOC(yb, m * 8);
     ALLOCB(cptr, (m + 1) * 4); ALLOCB(ent, (int64_t)(P->nent + 1) * sizeof(SEnt)); ALLOCB(blk, (int64_t)P->nb * sizeof(BlkRef));
     ALLOCB(pptr, (int64_t)(P->npos + 1) * 4); ALLOCB(ppos, (int64_t)(P->npos + 1) * 8); ALLOCB(pcon, (int64_t)(P->nent + 1) * 4);
@@ -175,19 +175,20 @@ int solve_setup(Solve *S) {
     if (fail) { solve_release(S); return -1; }
     char *H = heapp();
     int rc = 0;
-    rc |= bk_h2d(H + S->C * 8, P->C, T * 8);
-    rc |= bk_h2d(H + S->bvec * 8, P->b, m * 8);
-    rc |= bk_h2d(H + S->cptr, P->cptr, (m + 1) * 4);
-    if (P->nent) rc |= bk_h2d(H + S->ent, P->ent, (int64_t)P->nent * sizeof(SEnt));
+    rc |= bk_h2d_async(H + S->C * 8, P->C, T * 8);
+    rc |= bk_h2d_async(H + S->bvec * 8, P->b, m * 8);
+    rc |= bk_h2d_async(H + S->duv * 8, P->du, m * 8);
+    rc |= bk_h2d_async(H + S->cptr, P->cptr, (m + 1) * 4);
+    if (P->nent) rc |= bk_h2d_async(H + S->ent, P->ent, (int64_t)P->nent * sizeof(SEnt));
     BlkRef *br = malloc(sizeof(BlkRef) * P->nb);
     for (int k = 0; k < P->nb; k++) { br[k].n = P->bn[k]; br[k].pad = 0; br[k].off = P->boff[k]; }
-    rc |= bk_h2d(H + S->blk, br, (int64_t)P->nb * sizeof(BlkRef));
+    rc |= bk_h2d_async(H + S->blk, br, (int64_t)P->nb * sizeof(BlkRef));
     free(br);
-    rc |= bk_h2d(H + S->pptr, P->pptr, (int64_t)(P->npos + 1) * 4);
-    if (P->npos) rc |= bk_h2d(H + S->ppos, P->ppos, (int64_t)P->npos * 8);
-    if (P->nent) { rc |= bk_h2d(H + S->pcon, P->pcon, (int64_t)P->nent * 4); rc |= bk_h2d(H + S->pval * 8, P->pval, (int64_t)P->nent * 8); }
+    rc |= bk_h2d_async(H + S->pptr, P->pptr, (int64_t)(P->npos + 1) * 4);
+    if (P->npos) rc |= bk_h2d_async(H + S->ppos, P->ppos, (int64_t)P->npos * 8);
+    if (P->nent) { rc |= bk_h2d_async(H + S->pcon, P->pcon, (int64_t)P->nent * 4); rc |= bk_h2d_async(H + S->pval * 8, P->pval, (int64_t)P->nent * 8); }
     if (P->dense) {
-        rc |= bk_h2d(H + S->Ad * 8, P->Ad, T * m * 8);
+        rc |= bk_h2d_async(H + S->Ad * 8, P->Ad, T * m * 8);
         rc |= bk_d2f(T * m, S->Ad, 2 * S->Af);
     }
     if (rc) { solve_release(S); return -1; }
@@ -1011,14 +1012,10 @@ int ipm_iterate_batch(Solve **S, int ns) {
     double *rel = calloc(ns, sizeof(double)), *ap = calloc(ns, sizeof(double)), *ad = calloc(ns, sizeof(double));
     double *smu = calloc(ns, sizeof(double)), *gam = calloc(ns, sizeof(double)), *apr = calloc(ns, sizeof(double)), *adr = calloc(ns, sizeof(double));
     double *ne = calloc(ns, sizeof(double)), *nrp = calloc(ns, sizeof(double)), *stepped = calloc(ns, sizeof(double));
-    double *hrp = NULL, *hy = NULL;
-    int mmax = 1;
-    for (int q = 0; q < ns; q++) { act[q] = !S[q]->done; if (S[q]->P.m > mmax) mmax = S[q]->P.m; }
-    hrp = malloc(sizeof(double) * mmax); hy = malloc(sizeof(double) * mmax);
+    for (int q = 0; q < ns; q++) act[q] = !S[q]->done;
     int nsl = 1;
     for (int q = 0; q < ns; q++) nsl += S[q]->P.nb;
     if (batch_scratch(ns, nsl + ns)) goto out;
-    double *H = heapp();
 
     /* Zi = LZi' LZi (K range trimmed to k >= max(i, j)), symmetrised */
     if (blk_gemm(S, ns, act, 1, 0, 2 | 8, 1.0, 0.0, OFF(LZi), OFF(LZi), OFF(Zi))) goto out;
@@ -1037,7 +1034,12 @@ int ipm_iterate_batch(Solve **S, int ns) {
         lin_add(S[q]->P.T, S[q]->W1, S[q]->F, 1.0, S[q]->C, 1.0, -1, 0, -1, 0);                 /* A'y + Z */
         lin_add(S[q]->P.T, S[q]->W2, S[q]->F, 1.0, S[q]->R0, S[q]->theta, -1, 0, -1, 0);        /* F + theta R0 */
     }
+    /* t1 = du o rp (the residual on the data as read), t2 = b - rp = A(X) */
+    for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->t2, S[q]->bvec, 1.0, S[q]->rp, -1.0, -1, 0, -1, 0);
     if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->t1, S[q]->rp, 1.0, S[q]->duv, 0, -1, 0, -1, 0);
+    if (bk_vmul(g_ll.v, g_ll.n)) goto out;
     DotL_clear(&g_dl);
     for (int q = 0; q < ns; q++) if (act[q]) {
         Solve *s = S[q];
@@ -1047,22 +1049,17 @@ int ipm_iterate_batch(Solve **S, int ns) {
         dot_add(T, s->X, s->Z, SC(q, 2));
         dot_add(T, s->F, s->F, SC(q, 3));
         dot_add(T, s->W2, s->W2, SC(q, 4));
+        dot_add(s->P.m, s->bvec, s->y, SC(q, 5));
+        dot_add(s->P.m, s->t1, s->t1, SC(q, 6));
+        dot_add(s->P.m, s->t2, s->t2, SC(q, 7));
     }
     if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
     for (int q = 0; q < ns; q++) {
         if (!act[q]) continue;
         Solve *s = S[q];
         const HProb *P = &s->P;
-        const int m = P->m;
-        if (bk_d2h(hrp, H + s->rp, sizeof(double) * m) || bk_d2h(hy, H + s->y, sizeof(double) * m)) goto out;
         const double ncr2 = HS(q, 0), pobj = HS(q, 1), xz = HS(q, 2), rdn2 = HS(q, 3), dev2 = HS(q, 4);
-        double dobj = 0, rpo = 0, nAX = 0;
-        for (int i = 0; i < m; i++) {
-            dobj += P->b[i] * hy[i];
-            const double v = hrp[i] * P->du[i]; rpo += v * v;
-            const double w = P->b[i] - hrp[i]; nAX += w * w;
-        }
-        rpo = sqrt(rpo); nAX = sqrt(nAX);
+        const double dobj = HS(q, 5), rpo = sqrt(HS(q, 6)), nAX = sqrt(HS(q, 7));
         const double mu = xz / P->Ntot;
         const double rdo = P->cs * sqrt(rdn2), sc = P->bs * P->cs;
         const double po = sc * pobj, dob = sc * dobj;
@@ -1087,7 +1084,7 @@ int ipm_iterate_batch(Solve **S, int ns) {
                 s->best_score = score; s->it_best = it;
                 LinL_clear(&g_ll);
                 lin_add(P->T, s->Xb, s->X, 1.0, -1, 0, -1, 0, -1, 0);
-                lin_add(m, s->yb, s->y, 1.0, -1, 0, -1, 0, -1, 0);
+                lin_add(P->m, s->yb, s->y, 1.0, -1, 0, -1, 0, -1, 0);
                 if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
             }
             if (score < s->o.tol) st = 0;
@@ -1313,7 +1310,7 @@ int ipm_iterate_batch(Solve **S, int ns) {
     rc = 0;
 out:
     free(act); free(fail); free(okp); free(okd); free(hasH); free(ract); free(tact); free(rel); free(ap); free(ad); free(smu); free(gam);
-    free(apr); free(adr); free(ne); free(nrp); free(stepped); free(hrp); free(hy);
+    free(apr); free(adr); free(ne); free(nrp); free(stepped);
     return rc;
 }
 

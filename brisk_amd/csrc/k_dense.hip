@@ -17,6 +17,7 @@
 #include "brisk_dev.h"
 #include "rt_internal.h"
 #include "chol_diag.h"
+#include "flat.h"
 
 #define GBM 64
 #define GBN 64
@@ -42,20 +43,20 @@ template <> struct MF<float> {
 // consecutive rows of column-major C (coalesced stores). The next K slab is fetched into
 // registers while the current one is multiplied from LDS.
 template <typename T, int TA, int TB, int BM, int BN>
-__global__ __launch_bounds__(256) void k_gemm(const GemmItem *__restrict__ items, int nitems, const T *__restrict__ A,
-                                              const T *__restrict__ B, T *__restrict__ C, T alpha, T beta,
-                                              const int *__restrict__ info) {
+__global__ __launch_bounds__(256) void k_gemm(const GemmItem *__restrict__ items, const int *__restrict__ pre, int nitems,
+                                              const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, T alpha,
+                                              T beta, const int *__restrict__ info) {
     typedef typename MF<T>::v4 v4;
     constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
     constexpr int LA = BM * GBK / 256, LB = BN * GBK / 256;
-    const int item = blockIdx.y + blockIdx.z * 65535;
-    if (item >= nitems) return;
+    const int item = flat_item(pre, nitems, blockIdx.x);
     const GemmItem it = items[item];
     if (it.guard >= 0 && info && info[it.guard] != 0) return;
-    const int tilesM = (it.M + BM - 1) / BM, tilesN = (it.N + BN - 1) / BN;
-    const int t = blockIdx.x;
-    if (t >= tilesM * tilesN) return;
-    const int tm = t % tilesM, tn = t / tilesM;
+    const int tilesM = (it.M + BM - 1) / BM;
+    const int t = blockIdx.x - pre[item];
+    int tm, tn;
+    if ((it.tri & 1) && it.M == it.N) tri_index(t, tm, tn);   // lower-only square C: its lower tiles only
+    else { tm = t % tilesM; tn = t / tilesM; }
     const int r0 = tm * BM, c0 = tn * BN;
     if ((it.tri & 1) && c0 > r0 + BM - 1) return;
     int kb = 0, ke = it.K;
@@ -154,39 +155,80 @@ static double gemm_flops(const GemmItem &g) {
     if (g.tri & 2) return 2.0 * M * N * K - M * N * (M - 1);                 // op(A) upper
     return 2.0 * M * N * K;
 }
+// A launch plan: the items and, for each tile size, the prefix array of their workgroups, laid
+// out in one host buffer so that a whole sequence of GEMMs (the steps of a factorization) is
+// staged to the device in a single copy.
+struct GemmPlan {
+    size_t items, pre[2];          // byte offsets in the buffer: items, prefix (64 x 64), prefix (128 x 128)
+    int n;
+    int64_t nb[2];                 // workgroups of each launch
+    double flops, bytes;
+};
+static size_t buf_align(std::vector<char> &buf) {
+    const size_t o = (buf.size() + 15) & ~(size_t)15;
+    buf.resize(o);
+    return o;
+}
+template <typename T>
+static GemmPlan gemm_plan(std::vector<char> &buf, const GemmItem *items, int nitems) {
+    GemmPlan P;
+    P.n = nitems; P.flops = 0; P.bytes = 0;
+    P.items = buf_align(buf);
+    buf.insert(buf.end(), (const char *)items, (const char *)(items + nitems));
+    for (int big = 0; big < 2; big++) {
+        const int TB = big ? 128 : 64;
+        P.pre[big] = buf_align(buf);
+        buf.resize(P.pre[big] + sizeof(int) * (nitems + 1));
+        int64_t tot = 0;
+        for (int q = 0; q < nitems; q++) {
+            const GemmItem &g = items[q];
+            ((int *)(buf.data() + P.pre[big]))[q] = (int)tot;
+            // 128 x 128 tiles for the items large enough to fill them, 64 x 64 for the rest
+            if ((g.M >= 192 && g.N >= 192) != (big == 1) || g.M <= 0 || g.N <= 0) continue;
+            const int64_t tm = (g.M + TB - 1) / TB, tn = (g.N + TB - 1) / TB;
+            tot += ((g.tri & 1) && g.M == g.N) ? tm * (tm + 1) / 2 : tm * tn;
+        }
+        ((int *)(buf.data() + P.pre[big]))[nitems] = (int)tot;
+        P.nb[big] = tot;
+    }
+    for (int q = 0; q < nitems; q++) {
+        P.flops += gemm_flops(items[q]);
+        P.bytes += (double)sizeof(T) * ((double)items[q].M * items[q].K + (double)items[q].K * items[q].N + 2.0 * items[q].M * items[q].N);
+    }
+    return P;
+}
+template <typename T>
+static int gemm_exec(const char *dbuf, const GemmPlan &P, int ta, int tb, T alpha, T beta, const T *A, const T *B, T *C, const int *info) {
+    if (P.nb[0] + P.nb[1] == 0) return 0;
+    if (P.nb[0] > 0x7fffffff || P.nb[1] > 0x7fffffff) return -1;
+    const GemmItem *d = (const GemmItem *)(dbuf + P.items);
+    const int tok = bk_prof_begin();
+    for (int big = 1; big >= 0; big--) {
+        if (!P.nb[big]) continue;
+        const int *pre = (const int *)(dbuf + P.pre[big]);
+        const dim3 grid((unsigned)P.nb[big]);
+        const int nitems = P.n;
+#define GL(TA_, TB_, BM_) hipLaunchKernelGGL((k_gemm<T, TA_, TB_, BM_, BM_>), grid, dim3(256), 0, g_bk_stream, d, pre, nitems, A, B, C, alpha, beta, info)
+        if (big) {
+            if (!ta && !tb) GL(0, 0, 128); else if (!ta && tb) GL(0, 1, 128); else if (ta && !tb) GL(1, 0, 128); else GL(1, 1, 128);
+        } else {
+            if (!ta && !tb) GL(0, 0, 64); else if (!ta && tb) GL(0, 1, 64); else if (ta && !tb) GL(1, 0, 64); else GL(1, 1, 64);
+        }
+#undef GL
+        if (BK_LAUNCH_CHECK("k_gemm")) return -1;
+    }
+    if (tok >= 0) bk_prof_end(tok, (sizeof(T) == 8 ? 0 : 4) + 2 * ta + tb, P.flops, P.bytes);
+    return 0;
+}
 template <typename T>
 static int gemm_launch(int ta, int tb, T alpha, T beta, const T *A, const T *B, T *C, const GemmItem *items, int nitems, const int *info) {
     if (nitems <= 0) return 0;
-    // 128 x 128 tiles when every item is large enough to fill them, else 64 x 64
-    int big = 1, maxM = 0, maxN = 0;
-    for (int q = 0; q < nitems; q++) {
-        if (items[q].M < 192 || items[q].N < 192) big = 0;
-        maxM = items[q].M > maxM ? items[q].M : maxM; maxN = items[q].N > maxN ? items[q].N : maxN;
-    }
-    const int TBM = big ? 128 : 64, TBN = big ? 128 : 64;
-    int maxt = 0;
-    for (int q = 0; q < nitems; q++) {
-        const int t = ((items[q].M + TBM - 1) / TBM) * ((items[q].N + TBN - 1) / TBN);
-        if (items[q].M > 0 && items[q].N > 0 && t > maxt) maxt = t;
-    }
-    if (maxt == 0) return 0;
-    const GemmItem *d = (const GemmItem *)bk_stage(items, sizeof(GemmItem) * nitems);
+    std::vector<char> buf;
+    const GemmPlan P = gemm_plan<T>(buf, items, nitems);
+    if (P.nb[0] + P.nb[1] == 0) return 0;
+    const char *d = (const char *)bk_stage(buf.data(), buf.size());
     if (!d) return -1;
-    dim3 grid(maxt, nitems < 65535 ? nitems : 65535, (nitems + 65534) / 65535);
-    const int tok = bk_prof_begin();
-#define GL(TA_, TB_, BM_) hipLaunchKernelGGL((k_gemm<T, TA_, TB_, BM_, BM_>), grid, dim3(256), 0, g_bk_stream, d, nitems, A, B, C, alpha, beta, info)
-    if (big) {
-        if (!ta && !tb) GL(0, 0, 128); else if (!ta && tb) GL(0, 1, 128); else if (ta && !tb) GL(1, 0, 128); else GL(1, 1, 128);
-    } else {
-        if (!ta && !tb) GL(0, 0, 64); else if (!ta && tb) GL(0, 1, 64); else if (ta && !tb) GL(1, 0, 64); else GL(1, 1, 64);
-    }
-#undef GL
-    if (tok >= 0) {
-        double fl = 0, by = 0;
-        for (int q = 0; q < nitems; q++) { fl += gemm_flops(items[q]); by += (double)sizeof(T) * ((double)items[q].M * items[q].K + (double)items[q].K * items[q].N + 2.0 * items[q].M * items[q].N); }
-        bk_prof_end(tok, (sizeof(T) == 8 ? 0 : 4) + 2 * ta + tb, fl, by);
-    }
-    return BK_LAUNCH_CHECK("k_gemm");
+    return gemm_exec<T>(d, P, ta, tb, alpha, beta, A, B, C, info);
 }
 extern "C" int bk_gemm_f64(int ta, int tb, double alpha, double beta, const double *A, const double *B, double *C,
                            const GemmItem *items, int nitems, const int *info) {
@@ -222,39 +264,36 @@ __global__ __launch_bounds__(256) void k_potrf_diag(const MItem *__restrict__ it
 
 // the strict upper triangle of every matrix set to zero
 template <typename T>
-__global__ void k_zero_upper(const MItem *__restrict__ items, int nit, T *__restrict__ A) {
-    const int q = blockIdx.y;
-    if (q >= nit) return;
+__global__ void k_zero_upper(const MItem *__restrict__ items, const int *__restrict__ pre, int nit, T *__restrict__ A) {
+    const int q = flat_item(pre, nit, blockIdx.x);
     const MItem it = items[q];
     const int n = it.n;
-    const int64_t tot = (int64_t)n * n;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t tot = (int64_t)n * n, lb = blockIdx.x - pre[q], nb = pre[q + 1] - pre[q];
+    for (int64_t e = lb * blockDim.x + threadIdx.x; e < tot; e += nb * blockDim.x) {
         const int r = (int)(e % n), c = (int)(e / n);
         if (r < c) A[it.off + e] = (T)0;
     }
 }
 
 template <typename T>
-using GemmFn = int (*)(int, int, T, T, const T *, const T *, T *, const GemmItem *, int, const int *);
-
-template <typename T>
-static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, T *pmin, GemmFn<T> gemm) {
+static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, T *pmin) {
     if (n_it <= 0) return 0;
     int nmax = 0;
     for (int q = 0; q < n_it; q++) nmax = it[q].n > nmax ? it[q].n : nmax;
-    const MItem *d_it = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
-    if (!d_it) return -1;
-    {
-        hipLaunchKernelGGL((k_zero_upper<T>), dim3(bk_grid1((int64_t)nmax * nmax, 256, 1024), n_it), dim3(256), 0, g_bk_stream, d_it, n_it, A);
-        if (BK_LAUNCH_CHECK("k_zero_upper")) return -1;
-    }
+    // every step's panel and trailing GEMMs planned at once: one staging copy for the whole factorization
+    std::vector<char> buf;
+    const size_t it_off = buf_align(buf);
+    buf.insert(buf.end(), (const char *)it, (const char *)(it + n_it));
+    std::vector<int64_t> zc(n_it);
+    for (int q = 0; q < n_it; q++) zc[q] = flat_cap(((int64_t)it[q].n * it[q].n + 255) / 256, 1024);
+    const size_t zpre_off = buf_align(buf);
+    buf.resize(zpre_off + sizeof(int) * (n_it + 1));
+    int64_t ztot = 0;
+    for (int q = 0; q < n_it; q++) { ((int *)(buf.data() + zpre_off))[q] = (int)ztot; ztot += zc[q]; }
+    ((int *)(buf.data() + zpre_off))[n_it] = (int)ztot;
+    std::vector<GemmPlan> pp, pt;
     std::vector<GemmItem> gi, tr;
-    for (int k0 = 0; k0 < nmax; k0 += 64) {
-        const int tok = bk_prof_begin();
-        hipLaunchKernelGGL((k_potrf_diag<T>), dim3(n_it), dim3(256), 0, g_bk_stream, d_it, n_it, k0, A, Dinv, info, pmin);
-        if (tok >= 0) { double fl = 0; for (int q = 0; q < n_it; q++) { const double b = it[q].n > k0 ? fmin(64.0, it[q].n - k0) : 0; fl += b * b * b / 3.0 + b * b * b / 3.0; } bk_prof_end(tok, 8, fl, 0); }
-        if (BK_LAUNCH_CHECK("k_potrf_diag")) return -1;
-        if (k0 + 64 >= nmax) break;
+    for (int k0 = 0; k0 + 64 < nmax; k0 += 64) {
         gi.clear(); tr.clear();
         for (int q = 0; q < n_it; q++) {
             const int n = it[q].n;
@@ -278,17 +317,32 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
             g.guard = it[q].slot; g.tri = 1;
             tr.push_back(g);
         }
-        if (gi.empty()) continue;
-        if (gemm(0, 1, (T)1, (T)0, A, Dinv, A, gi.data(), (int)gi.size(), info)) return -1;
-        if (gemm(0, 1, (T)-1, (T)1, A, A, A, tr.data(), (int)tr.size(), info)) return -1;
+        pp.push_back(gemm_plan<T>(buf, gi.data(), (int)gi.size()));
+        pt.push_back(gemm_plan<T>(buf, tr.data(), (int)tr.size()));
+    }
+    const char *d = (const char *)bk_stage(buf.data(), buf.size());
+    if (!d) return -1;
+    const MItem *d_it = (const MItem *)(d + it_off);
+    if (ztot > 0) {
+        hipLaunchKernelGGL((k_zero_upper<T>), dim3((unsigned)ztot), dim3(256), 0, g_bk_stream, d_it, (const int *)(d + zpre_off), n_it, A);
+        if (BK_LAUNCH_CHECK("k_zero_upper")) return -1;
+    }
+    for (int k0 = 0, s = 0; k0 < nmax; k0 += 64, s++) {
+        const int tok = bk_prof_begin();
+        hipLaunchKernelGGL((k_potrf_diag<T>), dim3(n_it), dim3(256), 0, g_bk_stream, d_it, n_it, k0, A, Dinv, info, pmin);
+        if (tok >= 0) { double fl = 0; for (int q = 0; q < n_it; q++) { const double b = it[q].n > k0 ? fmin(64.0, it[q].n - k0) : 0; fl += b * b * b / 3.0 + b * b * b / 3.0; } bk_prof_end(tok, 8, fl, 0); }
+        if (BK_LAUNCH_CHECK("k_potrf_diag")) return -1;
+        if (k0 + 64 >= nmax) break;
+        if (gemm_exec<T>(d, pp[s], 0, 1, (T)1, (T)0, A, Dinv, A, info)) return -1;
+        if (gemm_exec<T>(d, pt[s], 0, 1, (T)-1, (T)1, A, A, A, info)) return -1;
     }
     return 0;
 }
 extern "C" int bk_potrf_f64(const MItem *it, int n_it, double *A, double *Dinv, double *W, int *info, double *pmin) {
-    return potrf_run<double>(it, n_it, A, Dinv, W, info, pmin, bk_gemm_f64);
+    return potrf_run<double>(it, n_it, A, Dinv, W, info, pmin);
 }
 extern "C" int bk_potrf_f32(const MItem *it, int n_it, float *A, float *Dinv, float *W, int *info, float *pmin) {
-    return potrf_run<float>(it, n_it, A, Dinv, W, info, pmin, bk_gemm_f32);
+    return potrf_run<float>(it, n_it, A, Dinv, W, info, pmin);
 }
 
 // ---- triangular inverse by recursive doubling ----------------------------------------------
@@ -297,15 +351,14 @@ extern "C" int bk_potrf_f32(const MItem *it, int n_it, float *A, float *Dinv, fl
 // block of L between them): two batched GEMMs per level, T = B A^{-1} into scratch, then the
 // product with C^{-1}.
 template <typename T>
-__global__ void k_trtri_init(const MItem *__restrict__ items, int nit, const T *__restrict__ Dinv, T *__restrict__ Linv,
-                             const int *__restrict__ info) {
-    const int q = blockIdx.y;
-    if (q >= nit) return;
+__global__ void k_trtri_init(const MItem *__restrict__ items, const int *__restrict__ pre, int nit, const T *__restrict__ Dinv,
+                             T *__restrict__ Linv, const int *__restrict__ info) {
+    const int q = flat_item(pre, nit, blockIdx.x);
     const MItem it = items[q];
     if (info && info[it.slot] != 0) return;
     const int n = it.n;
-    const int64_t tot = (int64_t)n * n;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t tot = (int64_t)n * n, lb = blockIdx.x - pre[q], nb = pre[q + 1] - pre[q];
+    for (int64_t e = lb * blockDim.x + threadIdx.x; e < tot; e += nb * blockDim.x) {
         const int r = (int)(e % n), c = (int)(e / n);
         T v = 0;
         if ((r >> 6) == (c >> 6) && r >= c) v = Dinv[it.off2 + (int64_t)(c >> 6) * 4096 + (r & 63) + 64 * (c & 63)];
@@ -313,14 +366,23 @@ __global__ void k_trtri_init(const MItem *__restrict__ items, int nit, const T *
     }
 }
 template <typename T>
-static int trtri_run(const MItem *it, int n_it, const T *L, const T *Dinv, T *Linv, T *W, const int *info, GemmFn<T> gemm) {
+static int trtri_run(const MItem *it, int n_it, const T *L, const T *Dinv, T *Linv, T *W, const int *info) {
     if (n_it <= 0) return 0;
     int nmax = 0;
     for (int q = 0; q < n_it; q++) nmax = it[q].n > nmax ? it[q].n : nmax;
-    const MItem *d_it = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
-    if (!d_it) return -1;
-    hipLaunchKernelGGL((k_trtri_init<T>), dim3(bk_grid1((int64_t)nmax * nmax, 256, 1024), n_it), dim3(256), 0, g_bk_stream, d_it, n_it, Dinv, Linv, info);
-    if (BK_LAUNCH_CHECK("k_trtri_init")) return -1;
+    // all levels planned at once: one staging copy
+    std::vector<char> buf;
+    const size_t it_off = buf_align(buf);
+    buf.insert(buf.end(), (const char *)it, (const char *)(it + n_it));
+    const size_t zpre_off = buf_align(buf);
+    buf.resize(zpre_off + sizeof(int) * (n_it + 1));
+    int64_t ztot = 0;
+    for (int q = 0; q < n_it; q++) {
+        ((int *)(buf.data() + zpre_off))[q] = (int)ztot;
+        ztot += flat_cap(((int64_t)it[q].n * it[q].n + 255) / 256, 1024);
+    }
+    ((int *)(buf.data() + zpre_off))[n_it] = (int)ztot;
+    std::vector<GemmPlan> p1, p2;
     std::vector<GemmItem> g1, g2;
     for (int s = 64; s < nmax; s *= 2) {
         g1.clear(); g2.clear();
@@ -347,18 +409,26 @@ static int trtri_run(const MItem *it, int n_it, const T *L, const T *Dinv, T *Li
                 g2.push_back(g);
             }
         }
-        if (g1.empty()) continue;
+        p1.push_back(gemm_plan<T>(buf, g1.data(), (int)g1.size()));
+        p2.push_back(gemm_plan<T>(buf, g2.data(), (int)g2.size()));
+    }
+    const char *d = (const char *)bk_stage(buf.data(), buf.size());
+    if (!d) return -1;
+    if (ztot > 0) hipLaunchKernelGGL((k_trtri_init<T>), dim3((unsigned)ztot), dim3(256), 0, g_bk_stream, (const MItem *)(d + it_off),
+                                     (const int *)(d + zpre_off), n_it, Dinv, Linv, info);
+    if (BK_LAUNCH_CHECK("k_trtri_init")) return -1;
+    for (size_t l = 0; l < p1.size(); l++) {
         // first GEMM: A-operand from L, B-operand from Linv; second: both from Linv / W
-        if (gemm(0, 0, (T)1, (T)0, L, Linv, W, g1.data(), (int)g1.size(), info)) return -1;
-        if (gemm(0, 0, (T)-1, (T)0, Linv, W, Linv, g2.data(), (int)g2.size(), info)) return -1;
+        if (gemm_exec<T>(d, p1[l], 0, 0, (T)1, (T)0, L, Linv, W, info)) return -1;
+        if (gemm_exec<T>(d, p2[l], 0, 0, (T)-1, (T)0, Linv, W, Linv, info)) return -1;
     }
     return 0;
 }
 extern "C" int bk_trtri_f64(const MItem *it, int n_it, const double *L, const double *Dinv, double *Linv, double *W, const int *info) {
-    return trtri_run<double>(it, n_it, L, Dinv, Linv, W, info, bk_gemm_f64);
+    return trtri_run<double>(it, n_it, L, Dinv, Linv, W, info);
 }
 extern "C" int bk_trtri_f32(const MItem *it, int n_it, const float *L, const float *Dinv, float *Linv, float *W, const int *info) {
-    return trtri_run<float>(it, n_it, L, Dinv, Linv, W, info, bk_gemm_f32);
+    return trtri_run<float>(it, n_it, L, Dinv, Linv, W, info);
 }
 
 // ---- batched matrix-vector products -------------------------------------------------------
@@ -367,13 +437,13 @@ extern "C" int bk_trtri_f32(const MItem *it, int n_it, const float *L, const flo
 // trans = 0: y_i = alpha sum_j A[i, j] x_j + beta y_i, one thread per row with the columns in
 // order (rows read coalesced across the wave).
 template <typename TA, typename TX, typename TY, typename TACC>
-__global__ __launch_bounds__(256) void k_gemv_t(const GemvItem *__restrict__ items, int nit, const TA *__restrict__ A,
-                                                const TX *__restrict__ x, TY *__restrict__ y, TACC alpha, TACC beta) {
-    const int q = blockIdx.y + blockIdx.z * 65535;
-    if (q >= nit) return;
+__global__ __launch_bounds__(256) void k_gemv_t(const GemvItem *__restrict__ items, const int *__restrict__ pre, int nit,
+                                                const TA *__restrict__ A, const TX *__restrict__ x, TY *__restrict__ y, TACC alpha,
+                                                TACC beta) {
+    const int q = flat_item(pre, nit, blockIdx.x);
     const GemvItem it = items[q];
     const int lane = threadIdx.x & 63;
-    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int j = (blockIdx.x - pre[q]) * 4 + (threadIdx.x >> 6);
     if (j >= it.N) return;
     const TA *col = A + it.a + (int64_t)j * it.lda;
     const TX *xv = x + it.x;
@@ -399,12 +469,12 @@ __global__ __launch_bounds__(256) void k_gemv_t(const GemvItem *__restrict__ ite
     }
 }
 template <typename TA, typename TX, typename TY, typename TACC>
-__global__ __launch_bounds__(256) void k_gemv_n(const GemvItem *__restrict__ items, int nit, const TA *__restrict__ A,
-                                                const TX *__restrict__ x, TY *__restrict__ y, TACC alpha, TACC beta) {
-    const int q = blockIdx.y + blockIdx.z * 65535;
-    if (q >= nit) return;
+__global__ __launch_bounds__(256) void k_gemv_n(const GemvItem *__restrict__ items, const int *__restrict__ pre, int nit,
+                                                const TA *__restrict__ A, const TX *__restrict__ x, TY *__restrict__ y, TACC alpha,
+                                                TACC beta) {
+    const int q = flat_item(pre, nit, blockIdx.x);
     const GemvItem it = items[q];
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i = (blockIdx.x - pre[q]) * 256 + threadIdx.x;
     if (i >= it.M) return;
     const TA *row = A + it.a + i;
     const TX *xv = x + it.x;
@@ -418,15 +488,17 @@ __global__ __launch_bounds__(256) void k_gemv_n(const GemvItem *__restrict__ ite
 template <typename TA, typename TX, typename TY, typename TACC>
 static int gemv_run(int trans, TACC alpha, TACC beta, const TA *A, const TX *x, TY *y, const GemvItem *it, int n_it) {
     if (n_it <= 0) return 0;
-    int mx = 0;
-    for (int q = 0; q < n_it; q++) { const int v = trans ? (it[q].N + 3) / 4 : (it[q].M + 255) / 256; mx = v > mx ? v : mx; }
-    if (mx == 0) return 0;
-    const GemvItem *d = (const GemvItem *)bk_stage(it, sizeof(GemvItem) * n_it);
-    if (!d) return -1;
-    dim3 grid(mx, n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    std::vector<int64_t> cnt(n_it);
+    for (int q = 0; q < n_it; q++) cnt[q] = trans ? (it[q].N + 3) / 4 : (it[q].M + 255) / 256;
+    const GemvItem *d = nullptr;
+    const int *pre = nullptr;
+    const int64_t nb = flat_stage_items(it, sizeof(GemvItem) * n_it, cnt, (const void **)&d, &pre);
+    if (nb < 0) return -1;
+    if (nb == 0) return 0;
+    const dim3 grid((unsigned)nb);
     const int tok = bk_prof_begin();
-    if (trans) hipLaunchKernelGGL((k_gemv_t<TA, TX, TY, TACC>), grid, dim3(256), 0, g_bk_stream, d, n_it, A, x, y, alpha, beta);
-    else hipLaunchKernelGGL((k_gemv_n<TA, TX, TY, TACC>), grid, dim3(256), 0, g_bk_stream, d, n_it, A, x, y, alpha, beta);
+    if (trans) hipLaunchKernelGGL((k_gemv_t<TA, TX, TY, TACC>), grid, dim3(256), 0, g_bk_stream, d, pre, n_it, A, x, y, alpha, beta);
+    else hipLaunchKernelGGL((k_gemv_n<TA, TX, TY, TACC>), grid, dim3(256), 0, g_bk_stream, d, pre, n_it, A, x, y, alpha, beta);
     if (tok >= 0) {
         double fl = 0, by = 0;
         for (int q = 0; q < n_it; q++) { fl += 2.0 * it[q].M * it[q].N; by += (double)sizeof(TA) * it[q].M * it[q].N + sizeof(TX) * (double)(trans ? it[q].M : it[q].N) + sizeof(TY) * (double)(trans ? it[q].N : it[q].M) * (beta != (TACC)0 ? 2 : 1); }

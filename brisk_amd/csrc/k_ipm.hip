@@ -16,6 +16,7 @@
 #include <vector>
 #include "brisk_dev.h"
 #include "rt_internal.h"
+#include "flat.h"
 
 static char *g_heap = nullptr;
 static size_t g_heap_size = 0;
@@ -36,14 +37,12 @@ extern "C" size_t bk_heap_size(void) { return g_heap_size; }
 
 // ---- dot products ---------------------------------------------------------------------------
 #define DCH 4096
-__global__ __launch_bounds__(256) void k_dot_chunks(const DotItem *__restrict__ it, const int64_t *__restrict__ cbase, int nit,
-                                                    const double *__restrict__ H, double *__restrict__ part) {
-    const int q = blockIdx.y + blockIdx.z * 65535;
-    if (q >= nit) return;
+__global__ __launch_bounds__(256) void k_dot_chunks(const DotItem *__restrict__ it, const int64_t *__restrict__ cbase,
+                                                    const int *__restrict__ pre, int nit, const double *__restrict__ H,
+                                                    double *__restrict__ part) {
+    const int q = flat_item(pre, nit, blockIdx.x);
     const DotItem d = it[q];
-    const int64_t c = blockIdx.x;
-    const int64_t nch = (d.len + DCH - 1) / DCH;
-    if (c >= nch) return;
+    const int64_t c = blockIdx.x - pre[q];
     __shared__ double sh[256];
     const int t = threadIdx.x;
     double s = 0;
@@ -73,13 +72,13 @@ __global__ void k_dot_final(const DotItem *__restrict__ it, const int64_t *__res
 }
 extern "C" int bk_dots(const DotItem *it, int n_it) {
     if (n_it <= 0) return 0;
-    std::vector<int64_t> cb(n_it);
-    int64_t tot = 0, mx = 1;
+    std::vector<int64_t> cb(n_it), cnt(n_it);
+    int64_t tot = 0;
     for (int q = 0; q < n_it; q++) {
         cb[q] = tot;
         const int64_t nch = (it[q].len + DCH - 1) / DCH;
         tot += nch;
-        if (nch > mx) mx = nch;
+        cnt[q] = nch;
     }
     if ((size_t)tot > g_partial_cap) {
         bk_sync();
@@ -87,12 +86,18 @@ extern "C" int bk_dots(const DotItem *it, int n_it) {
         g_partial_cap = (size_t)tot * 2 + 4096;
         if (bk_check(hipMalloc((void **)&g_partial, g_partial_cap * sizeof(double)), "hipMalloc partials")) return -1;
     }
-    const DotItem *d = (const DotItem *)bk_stage(it, sizeof(DotItem) * n_it);
-    const int64_t *dcb = (const int64_t *)bk_stage(cb.data(), sizeof(int64_t) * n_it);
-    if (!d || !dcb) return -1;
-    dim3 grid((unsigned)mx, n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    // items, chunk bases and the prefix array in one staged copy
+    std::vector<char> buf(sizeof(DotItem) * n_it + sizeof(int64_t) * n_it);
+    memcpy(buf.data(), it, sizeof(DotItem) * n_it);
+    memcpy(buf.data() + sizeof(DotItem) * n_it, cb.data(), sizeof(int64_t) * n_it);
+    const void *dv = nullptr;
+    const int *pre = nullptr;
+    const int64_t nb = flat_stage_items(buf.data(), buf.size(), cnt, &dv, &pre);
+    if (nb < 0) return -1;
+    const DotItem *d = (const DotItem *)dv;
+    const int64_t *dcb = (const int64_t *)((const char *)dv + sizeof(DotItem) * n_it);
     const int tok = bk_prof_begin();
-    hipLaunchKernelGGL(k_dot_chunks, grid, dim3(256), 0, g_bk_stream, d, dcb, n_it, (const double *)g_heap, g_partial);
+    if (nb > 0) hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, dcb, pre, n_it, (const double *)g_heap, g_partial);
     if (tok >= 0) { double by = 0; for (int q = 0; q < n_it; q++) by += 16.0 * it[q].len; bk_prof_end(tok, 13, 0, by); }
     if (BK_LAUNCH_CHECK("k_dot_chunks")) return -1;
     hipLaunchKernelGGL(k_dot_final, dim3((n_it + 255) / 256), dim3(256), 0, g_bk_stream, d, dcb, n_it, (double *)g_heap, g_partial);
@@ -100,11 +105,12 @@ extern "C" int bk_dots(const DotItem *it, int n_it) {
 }
 
 // ---- linear combinations ----------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_lincomb(const LinItem *__restrict__ it, int nit, double *__restrict__ H) {
-    const int q = blockIdx.y + blockIdx.z * 65535;
-    if (q >= nit) return;
+__global__ __launch_bounds__(256) void k_lincomb(const LinItem *__restrict__ it, const int *__restrict__ pre, int nit,
+                                                 double *__restrict__ H) {
+    const int q = flat_item(pre, nit, blockIdx.x);
     const LinItem L = it[q];
-    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < L.len; e += (int64_t)gridDim.x * 256) {
+    const int64_t lb = blockIdx.x - pre[q], nb = pre[q + 1] - pre[q];
+    for (int64_t e = lb * 256 + threadIdx.x; e < L.len; e += nb * 256) {
         double v = 0;
 #pragma unroll
         for (int t = 0; t < 4; t++)
@@ -112,25 +118,31 @@ __global__ __launch_bounds__(256) void k_lincomb(const LinItem *__restrict__ it,
         H[L.out + e] = v;
     }
 }
+static int64_t lin_stage(const LinItem *it, int n_it, const LinItem **d, const int **pre) {
+    std::vector<int64_t> cnt(n_it);
+    for (int q = 0; q < n_it; q++) cnt[q] = flat_cap((it[q].len + 255) / 256, 2048);
+    return flat_stage_items(it, sizeof(LinItem) * n_it, cnt, (const void **)d, pre);
+}
 extern "C" int bk_lincomb(const LinItem *it, int n_it) {
     if (n_it <= 0) return 0;
-    int64_t mx = 1;
-    for (int q = 0; q < n_it; q++) mx = it[q].len > mx ? it[q].len : mx;
-    const LinItem *d = (const LinItem *)bk_stage(it, sizeof(LinItem) * n_it);
-    if (!d) return -1;
-    dim3 grid(bk_grid1(mx, 256, 2048), n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    const LinItem *d = nullptr;
+    const int *pre = nullptr;
+    const int64_t nb = lin_stage(it, n_it, &d, &pre);
+    if (nb < 0) return -1;
+    if (nb == 0) return 0;
     const int tok = bk_prof_begin();
-    hipLaunchKernelGGL(k_lincomb, grid, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    hipLaunchKernelGGL(k_lincomb, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, (double *)g_heap);
     if (tok >= 0) { double by = 0; for (int q = 0; q < n_it; q++) { int t = 1; for (int u = 0; u < 4; u++) t += it[q].in[u] >= 0; by += 8.0 * t * it[q].len; } bk_prof_end(tok, 14, 0, by); }
     return BK_LAUNCH_CHECK("k_lincomb");
 }
 
 // out[q] = c0 in0[q] in1[q] (+ c1 in2[q] when in[2] >= 0)
-__global__ __launch_bounds__(256) void k_vmul(const LinItem *__restrict__ it, int nit, double *__restrict__ H) {
-    const int q = blockIdx.y + blockIdx.z * 65535;
-    if (q >= nit) return;
+__global__ __launch_bounds__(256) void k_vmul(const LinItem *__restrict__ it, const int *__restrict__ pre, int nit,
+                                              double *__restrict__ H) {
+    const int q = flat_item(pre, nit, blockIdx.x);
     const LinItem L = it[q];
-    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < L.len; e += (int64_t)gridDim.x * 256) {
+    const int64_t lb = blockIdx.x - pre[q], nb = pre[q + 1] - pre[q];
+    for (int64_t e = lb * 256 + threadIdx.x; e < L.len; e += nb * 256) {
         double v = L.c[0] * H[L.in[0] + e] * H[L.in[1] + e];
         if (L.in[2] >= 0) v += L.c[1] * H[L.in[2] + e];
         H[L.out + e] = v;
@@ -138,12 +150,12 @@ __global__ __launch_bounds__(256) void k_vmul(const LinItem *__restrict__ it, in
 }
 extern "C" int bk_vmul(const LinItem *it, int n_it) {
     if (n_it <= 0) return 0;
-    int64_t mx = 1;
-    for (int q = 0; q < n_it; q++) mx = it[q].len > mx ? it[q].len : mx;
-    const LinItem *d = (const LinItem *)bk_stage(it, sizeof(LinItem) * n_it);
-    if (!d) return -1;
-    dim3 grid(bk_grid1(mx, 256, 2048), n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
-    hipLaunchKernelGGL(k_vmul, grid, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    const LinItem *d = nullptr;
+    const int *pre = nullptr;
+    const int64_t nb = lin_stage(it, n_it, &d, &pre);
+    if (nb < 0) return -1;
+    if (nb == 0) return 0;
+    hipLaunchKernelGGL(k_vmul, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, (double *)g_heap);
     return BK_LAUNCH_CHECK("k_vmul");
 }
 
@@ -177,13 +189,15 @@ extern "C" int bk_d2f(int64_t n, int64_t src_off, int64_t dst_off_f32) {
 }
 
 // ---- per-matrix helpers ---------------------------------------------------------------------
-__global__ void k_symmetrize(const MItem *__restrict__ it, int nit, double *__restrict__ H) {
-    const int q = blockIdx.y + blockIdx.z * 65535;
-    if (q >= nit) return;
-    const MItem m = it[q];
+#define FLAT_MITEM                                                          \
+    const int q = flat_item(pre, nit, blockIdx.x);                          \
+    const MItem m = it[q];                                                  \
+    const int64_t lb = blockIdx.x - pre[q], nb = pre[q + 1] - pre[q];
+__global__ void k_symmetrize(const MItem *__restrict__ it, const int *__restrict__ pre, int nit, double *__restrict__ H) {
+    FLAT_MITEM
     const int n = m.n;
     double *G = H + m.off;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)n * n; e += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t e = lb * blockDim.x + threadIdx.x; e < (int64_t)n * n; e += nb * blockDim.x) {
         const int r = (int)(e % n), c = (int)(e / n);
         if (r > c) {
             const double v = 0.5 * (G[r + (int64_t)c * n] + G[c + (int64_t)r * n]);
@@ -193,15 +207,13 @@ __global__ void k_symmetrize(const MItem *__restrict__ it, int nit, double *__re
     }
 }
 template <typename T>
-__global__ void k_transpose(const MItem *__restrict__ it, int nit, T *__restrict__ H) {
-    const int q = blockIdx.y + blockIdx.z * 65535;
-    if (q >= nit) return;
-    const MItem m = it[q];
+__global__ void k_transpose(const MItem *__restrict__ it, const int *__restrict__ pre, int nit, T *__restrict__ H) {
+    FLAT_MITEM
     const int n = m.n;
     // 32 x 32 tiles through LDS: coalesced reads and writes
     __shared__ T tile[32][33];
     const int tiles = (n + 31) / 32;
-    for (int64_t tt = blockIdx.x; tt < (int64_t)tiles * tiles; tt += gridDim.x) {
+    for (int64_t tt = lb; tt < (int64_t)tiles * tiles; tt += nb) {
         const int tr = (int)(tt % tiles) * 32, tc = (int)(tt / tiles) * 32;
         const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;     // 256 threads: 32 x 8
         for (int y = ly; y < 32; y += 8) {
@@ -216,31 +228,26 @@ __global__ void k_transpose(const MItem *__restrict__ it, int nit, T *__restrict
         __syncthreads();
     }
 }
-__global__ void k_add_diag(const MItem *__restrict__ it, int nit, double *__restrict__ H, int64_t s_off) {
-    const int q = blockIdx.y + blockIdx.z * 65535;
-    if (q >= nit) return;
-    const MItem m = it[q];
+__global__ void k_add_diag(const MItem *__restrict__ it, const int *__restrict__ pre, int nit, double *__restrict__ H, int64_t s_off) {
+    FLAT_MITEM
     const double s = H[s_off + m.slot];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.n; i += gridDim.x * blockDim.x) H[m.off + i + (int64_t)i * m.n] += s;
+    for (int64_t i = lb * blockDim.x + threadIdx.x; i < m.n; i += nb * blockDim.x) H[m.off + i + i * m.n] += s;
 }
-__global__ void k_equil_d(const MItem *__restrict__ it, int nit, double *__restrict__ H) {
-    const int q = blockIdx.y + blockIdx.z * 65535;
-    if (q >= nit) return;
-    const MItem m = it[q];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < m.n; i += gridDim.x * blockDim.x) {
-        const double d = H[m.off + i + (int64_t)i * m.n];
+__global__ void k_equil_d(const MItem *__restrict__ it, const int *__restrict__ pre, int nit, double *__restrict__ H) {
+    FLAT_MITEM
+    for (int64_t i = lb * blockDim.x + threadIdx.x; i < m.n; i += nb * blockDim.x) {
+        const double d = H[m.off + i + i * m.n];
         H[m.off2 + i] = d > 0 ? 1.0 / sqrt(d) : 1.0;
     }
 }
 template <typename T>
-__global__ void k_equil_f(const MItem *__restrict__ it, int nit, const double *__restrict__ H, T *__restrict__ F, int64_t reg_off) {
-    const int q = blockIdx.y + blockIdx.z * 65535;
-    if (q >= nit) return;
-    const MItem m = it[q];
+__global__ void k_equil_f(const MItem *__restrict__ it, const int *__restrict__ pre, int nit, const double *__restrict__ H,
+                          T *__restrict__ F, int64_t reg_off) {
+    FLAT_MITEM
     const int n = m.n;
     const double reg = H[reg_off + m.slot];
     const double *D = H + m.off2;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)n * n; e += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t e = lb * blockDim.x + threadIdx.x; e < (int64_t)n * n; e += nb * blockDim.x) {
         const int r = (int)(e % n), c = (int)(e / n);
         T v = 0;
         if (r >= c) {
@@ -251,51 +258,53 @@ __global__ void k_equil_f(const MItem *__restrict__ it, int nit, const double *_
         F[m.off3 + e] = v;
     }
 }
-static int mitem_launch_dims(const MItem *it, int n_it, int64_t per, dim3 &grid) {
-    int64_t mx = 1;
-    for (int q = 0; q < n_it; q++) { const int64_t v = per == 1 ? it[q].n : (int64_t)it[q].n * it[q].n; mx = v > mx ? v : mx; }
-    grid = dim3(bk_grid1(mx, 256, 1024), n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
-    return 0;
+// workgroups per matrix item: kind 1 one per 256 of the order, 2 one per 256 entries (at most
+// 1024), 3 one per 32 x 32 tile (at most 512)
+static int64_t mitem_stage(const MItem *it, int n_it, int kind, const MItem **d, const int **pre) {
+    std::vector<int64_t> cnt(n_it);
+    for (int q = 0; q < n_it; q++) {
+        const int64_t n = it[q].n, t = (n + 31) / 32;
+        cnt[q] = kind == 1 ? flat_cap((n + 255) / 256, 1024) : kind == 2 ? flat_cap((n * n + 255) / 256, 1024) : flat_cap(t * t, 512);
+    }
+    return flat_stage_items(it, sizeof(MItem) * n_it, cnt, (const void **)d, pre);
 }
+#define MITEM_LAUNCH(kind_)                                                    \
+    if (n_it <= 0) return 0;                                                   \
+    const MItem *d = nullptr;                                                  \
+    const int *pre = nullptr;                                                  \
+    const int64_t nb = mitem_stage(it, n_it, kind_, &d, &pre);                \
+    if (nb < 0) return -1;                                                     \
+    if (nb == 0) return 0;
 extern "C" int bk_symmetrize(const MItem *it, int n_it) {
-    if (n_it <= 0) return 0;
-    dim3 g; mitem_launch_dims(it, n_it, 2, g);
-    const MItem *d = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
-    hipLaunchKernelGGL(k_symmetrize, g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    MITEM_LAUNCH(2)
+    hipLaunchKernelGGL(k_symmetrize, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, (double *)g_heap);
     return BK_LAUNCH_CHECK("k_symmetrize");
 }
 extern "C" int bk_transpose_f64(const MItem *it, int n_it) {
-    if (n_it <= 0) return 0;
-    dim3 g; mitem_launch_dims(it, n_it, 2, g);
-    g.x = g.x > 512 ? 512 : g.x;
-    const MItem *d = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
-    hipLaunchKernelGGL((k_transpose<double>), g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    MITEM_LAUNCH(3)
+    hipLaunchKernelGGL((k_transpose<double>), dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, (double *)g_heap);
     return BK_LAUNCH_CHECK("k_transpose");
 }
 extern "C" int bk_transpose_f32(const MItem *it, int n_it) {
-    if (n_it <= 0) return 0;
-    dim3 g; mitem_launch_dims(it, n_it, 2, g);
-    g.x = g.x > 512 ? 512 : g.x;
-    const MItem *d = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
-    hipLaunchKernelGGL((k_transpose<float>), g, dim3(256), 0, g_bk_stream, d, n_it, (float *)g_heap);
+    MITEM_LAUNCH(3)
+    hipLaunchKernelGGL((k_transpose<float>), dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, (float *)g_heap);
     return BK_LAUNCH_CHECK("k_transpose");
 }
 extern "C" int bk_add_diag(const MItem *it, int n_it, int64_t s_off) {
-    if (n_it <= 0) return 0;
-    dim3 g; mitem_launch_dims(it, n_it, 1, g);
-    const MItem *d = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
-    hipLaunchKernelGGL(k_add_diag, g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap, s_off);
+    MITEM_LAUNCH(1)
+    hipLaunchKernelGGL(k_add_diag, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, (double *)g_heap, s_off);
     return BK_LAUNCH_CHECK("k_add_diag");
 }
 extern "C" int bk_equil(const MItem *it, int n_it, int prec, int64_t reg_off) {
-    if (n_it <= 0) return 0;
-    dim3 g1; mitem_launch_dims(it, n_it, 1, g1);
-    dim3 g2; mitem_launch_dims(it, n_it, 2, g2);
-    const MItem *d = (const MItem *)bk_stage(it, sizeof(MItem) * n_it);
-    hipLaunchKernelGGL(k_equil_d, g1, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    MITEM_LAUNCH(1)
+    hipLaunchKernelGGL(k_equil_d, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, (double *)g_heap);
     if (BK_LAUNCH_CHECK("k_equil_d")) return -1;
-    if (prec == 0) hipLaunchKernelGGL((k_equil_f<float>), g2, dim3(256), 0, g_bk_stream, d, n_it, (const double *)g_heap, (float *)g_heap, reg_off);
-    else hipLaunchKernelGGL((k_equil_f<double>), g2, dim3(256), 0, g_bk_stream, d, n_it, (const double *)g_heap, (double *)g_heap, reg_off);
+    const MItem *d2 = nullptr;
+    const int *pre2 = nullptr;
+    const int64_t nb2 = mitem_stage(it, n_it, 2, &d2, &pre2);
+    if (nb2 < 0) return -1;
+    if (prec == 0) hipLaunchKernelGGL((k_equil_f<float>), dim3((unsigned)nb2), dim3(256), 0, g_bk_stream, d2, pre2, n_it, (const double *)g_heap, (float *)g_heap, reg_off);
+    else hipLaunchKernelGGL((k_equil_f<double>), dim3((unsigned)nb2), dim3(256), 0, g_bk_stream, d2, pre2, n_it, (const double *)g_heap, (double *)g_heap, reg_off);
     return BK_LAUNCH_CHECK("k_equil_f");
 }
 
@@ -303,11 +312,10 @@ extern "C" int bk_equil(const MItem *it, int n_it, int prec, int64_t reg_off) {
 __device__ __forceinline__ const int *HI(const double *H, int64_t off) { return (const int *)(((const char *)H) + off); }
 
 // A(D): one wave per constraint, lanes stride its entries, butterfly in a fixed pattern
-__global__ __launch_bounds__(256) void k_aop(const ConItem *__restrict__ it, int nit, double *__restrict__ H) {
-    const int q = blockIdx.y + blockIdx.z * 65535;
-    if (q >= nit) return;
+__global__ __launch_bounds__(256) void k_aop(const ConItem *__restrict__ it, const int *__restrict__ pre, int nit, double *__restrict__ H) {
+    const int q = flat_item(pre, nit, blockIdx.x);
     const ConItem C = it[q];
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int i = (blockIdx.x - pre[q]) * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (i >= C.m) return;
     const int *cptr = HI(H, C.cptr);
     const SEnt *ent = (const SEnt *)(((const char *)H) + C.ent);
@@ -330,22 +338,24 @@ __global__ __launch_bounds__(256) void k_aop(const ConItem *__restrict__ it, int
 }
 extern "C" int bk_aop(const ConItem *it, int n_it) {
     if (n_it <= 0) return 0;
-    int mx = 1;
-    for (int q = 0; q < n_it; q++) mx = it[q].m > mx ? it[q].m : mx;
-    const ConItem *d = (const ConItem *)bk_stage(it, sizeof(ConItem) * n_it);
-    dim3 g((mx + 3) / 4, n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
+    std::vector<int64_t> cnt(n_it);
+    for (int q = 0; q < n_it; q++) cnt[q] = (it[q].m + 3) / 4;
+    const ConItem *d = nullptr;
+    const int *pre = nullptr;
+    const int64_t nb = flat_stage_items(it, sizeof(ConItem) * n_it, cnt, (const void **)&d, &pre);
+    if (nb < 0) return -1;
+    if (nb == 0) return 0;
     const int tok = bk_prof_begin();
-    hipLaunchKernelGGL(k_aop, g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    hipLaunchKernelGGL(k_aop, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, (double *)g_heap);
     if (tok >= 0) bk_prof_end(tok, 11, 0, 0);
     return BK_LAUNCH_CHECK("k_aop");
 }
 
 // A*(y) added at the positions the constraints touch: D[pos] += alpha sum_i y_i v_i
-__global__ __launch_bounds__(256) void k_aty(const ConItem *__restrict__ it, int nit, double *__restrict__ H) {
-    const int q = blockIdx.y + blockIdx.z * 65535;
-    if (q >= nit) return;
+__global__ __launch_bounds__(256) void k_aty(const ConItem *__restrict__ it, const int *__restrict__ pre, int nit, double *__restrict__ H) {
+    const int q = flat_item(pre, nit, blockIdx.x);
     const ConItem C = it[q];
-    const int p = blockIdx.x * 256 + threadIdx.x;
+    const int p = (blockIdx.x - pre[q]) * 256 + threadIdx.x;
     if (p >= C.npos) return;
     const int *pptr = HI(H, C.pptr), *ppos = HI(H, C.ppos), *pcon = HI(H, C.pcon);
     const double *pval = H + C.pval, *y = H + C.y;
@@ -358,23 +368,26 @@ __global__ __launch_bounds__(256) void k_aty(const ConItem *__restrict__ it, int
 }
 extern "C" int bk_aty(const ConItem *it, int n_it) {
     if (n_it <= 0) return 0;
-    int mx = 1;
-    for (int q = 0; q < n_it; q++) mx = it[q].npos > mx ? it[q].npos : mx;
-    const ConItem *d = (const ConItem *)bk_stage(it, sizeof(ConItem) * n_it);
-    dim3 g((mx + 255) / 256, n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
-    hipLaunchKernelGGL(k_aty, g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    std::vector<int64_t> cnt(n_it);
+    for (int q = 0; q < n_it; q++) cnt[q] = (it[q].npos + 255) / 256;
+    const ConItem *d = nullptr;
+    const int *pre = nullptr;
+    const int64_t nb = flat_stage_items(it, sizeof(ConItem) * n_it, cnt, (const void **)&d, &pre);
+    if (nb < 0) return -1;
+    if (nb == 0) return 0;
+    hipLaunchKernelGGL(k_aty, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, (double *)g_heap);
     return BK_LAUNCH_CHECK("k_aty");
 }
 
 // Schur complement, Hadamard form: every constraint a single diagonal entry v_a at (r_a, r_a) of
-// block k_a: M_ab = v_a v_b X[r_a, r_b] Zi[r_a, r_b] when k_a == k_b (else 0). One column b per
-// blockIdx.y, rows a across the workgroup: X and Zi columns streamed, M written coalesced.
-__global__ __launch_bounds__(256) void k_schur_diag(const ConItem *__restrict__ it, int nit, double *__restrict__ H) {
-    const int q = blockIdx.z;
-    if (q >= nit) return;
+// block k_a: M_ab = v_a v_b X[r_a, r_b] Zi[r_a, r_b] when k_a == k_b (else 0). A workgroup covers
+// 256 rows a of one column b: X and Zi columns streamed, M written coalesced.
+__global__ __launch_bounds__(256) void k_schur_diag(const ConItem *__restrict__ it, const int *__restrict__ pre, int nit,
+                                                    double *__restrict__ H) {
+    const int q = flat_item(pre, nit, blockIdx.x);
     const ConItem C = it[q];
-    if (!C.diag) return;
-    const int b = blockIdx.y, a = blockIdx.x * 256 + threadIdx.x;
+    const int cb = (C.m + 255) / 256, lb = blockIdx.x - pre[q];
+    const int b = lb / cb, a = (lb % cb) * 256 + threadIdx.x;
     if (b >= C.m || a >= C.m) return;
     const SEnt *ent = (const SEnt *)(((const char *)H) + C.ent);
     const BlkRef *blk = (const BlkRef *)(((const char *)H) + C.blk);
@@ -387,71 +400,92 @@ __global__ __launch_bounds__(256) void k_schur_diag(const ConItem *__restrict__ 
     }
     H[C.M + a + (int64_t)b * C.m] = v;
 }
-// Schur complement, route 0: one thread per pair a >= b; both constraints' entries are grouped
-// by block in ascending order, so the common blocks are found by a merge
-__global__ __launch_bounds__(256) void k_schur_r0(const ConItem *__restrict__ it, int nit, double *__restrict__ H) {
-    const int q = blockIdx.y + blockIdx.z * 65535;
-    if (q >= nit) return;
+// Schur complement, route 0: M_ab = sum over the entries p of A_a and q of A_b in a common block
+// of v_p v_q X[c_p, r_q] Zi[c_q, r_p] over the symmetric expansions of both (reference
+// solver.c:2626-2660). A workgroup computes a 16 x 16 tile of pairs of the lower triangle of tiles
+// (16 x 16 threads, rows a along the fast thread index); the constraints' entries are grouped by
+// block in ascending order, so the common blocks are found by a merge. The tile is stored at
+// (a, b) and, through LDS, transposed at (b, a): both stores are 16-row coalesced runs.
+__global__ __launch_bounds__(256) void k_schur_r0(const ConItem *__restrict__ it, const int *__restrict__ pre, int nit,
+                                                  double *__restrict__ H) {
+    const int q = flat_item(pre, nit, blockIdx.x);
     const ConItem C = it[q];
-    if (C.diag) return;
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t npair = (int64_t)C.m * (C.m + 1) / 2;
-    if (t >= npair) return;
-    int64_t a = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-    while (a * (a + 1) / 2 > t) a--;
-    while ((a + 1) * (a + 2) / 2 <= t) a++;
-    const int64_t b = t - a * (a + 1) / 2;
-    const int *cptr = HI(H, C.cptr);
-    const SEnt *ent = (const SEnt *)(((const char *)H) + C.ent);
-    const BlkRef *blk = (const BlkRef *)(((const char *)H) + C.blk);
-    const double *X = H + C.X, *Zi = H + C.Zi;
-    int pa = cptr[a], ea = cptr[a + 1], pb = cptr[b], eb = cptr[b + 1];
+    int ta, tb;
+    tri_index(blockIdx.x - pre[q], ta, tb);
+    const int i = threadIdx.x & 15, j = threadIdx.x >> 4;
+    const int a = ta * 16 + i, b = tb * 16 + j;
+    __shared__ double tile[16][17];
     double s = 0;
-    while (pa < ea && pb < eb) {
-        const int ka = ent[pa].k, kb = ent[pb].k;
-        if (ka < kb) { pa++; continue; }
-        if (kb < ka) { pb++; continue; }
-        int qa = pa; while (qa < ea && ent[qa].k == ka) qa++;
-        int qb = pb; while (qb < eb && ent[qb].k == ka) qb++;
-        const BlkRef B = blk[ka];
-        const int n = B.n;
-        const double *Xk = X + B.off, *Zk = Zi + B.off;
-        for (int u = pa; u < qa; u++) {
-            const SEnt P = ent[u];
-            for (int w = pb; w < qb; w++) {
-                const SEnt Q = ent[w];
-                // expansions (r, c) and, off the diagonal, (c, r) of both entries
-                double acc = Xk[P.c + (int64_t)Q.r * n] * Zk[Q.c + (int64_t)P.r * n];
-                if (Q.r != Q.c) acc += Xk[P.c + (int64_t)Q.c * n] * Zk[Q.r + (int64_t)P.r * n];
-                if (P.r != P.c) {
-                    acc += Xk[P.r + (int64_t)Q.r * n] * Zk[Q.c + (int64_t)P.c * n];
-                    if (Q.r != Q.c) acc += Xk[P.r + (int64_t)Q.c * n] * Zk[Q.r + (int64_t)P.c * n];
+    if (a < C.m && b < C.m) {
+        const int *cptr = HI(H, C.cptr);
+        const SEnt *ent = (const SEnt *)(((const char *)H) + C.ent);
+        const BlkRef *blk = (const BlkRef *)(((const char *)H) + C.blk);
+        const double *X = H + C.X, *Zi = H + C.Zi;
+        int pa = cptr[a], ea = cptr[a + 1], pb = cptr[b], eb = cptr[b + 1];
+        while (pa < ea && pb < eb) {
+            const int ka = ent[pa].k, kb = ent[pb].k;
+            if (ka < kb) { pa++; continue; }
+            if (kb < ka) { pb++; continue; }
+            int qa = pa; while (qa < ea && ent[qa].k == ka) qa++;
+            int qb = pb; while (qb < eb && ent[qb].k == ka) qb++;
+            const BlkRef B = blk[ka];
+            const int n = B.n;
+            const double *Xk = X + B.off, *Zk = Zi + B.off;
+            for (int u = pa; u < qa; u++) {
+                const SEnt P = ent[u];
+                for (int w = pb; w < qb; w++) {
+                    const SEnt Q = ent[w];
+                    // expansions (r, c) and, off the diagonal, (c, r) of both entries
+                    double acc = Xk[P.c + (int64_t)Q.r * n] * Zk[Q.c + (int64_t)P.r * n];
+                    if (Q.r != Q.c) acc += Xk[P.c + (int64_t)Q.c * n] * Zk[Q.r + (int64_t)P.r * n];
+                    if (P.r != P.c) {
+                        acc += Xk[P.r + (int64_t)Q.r * n] * Zk[Q.c + (int64_t)P.c * n];
+                        if (Q.r != Q.c) acc += Xk[P.r + (int64_t)Q.c * n] * Zk[Q.r + (int64_t)P.c * n];
+                    }
+                    s += P.v * Q.v * acc;
                 }
-                s += P.v * Q.v * acc;
             }
+            pa = qa; pb = qb;
         }
-        pa = qa; pb = qb;
+        H[C.M + a + (int64_t)b * C.m] = s;
     }
-    H[C.M + a + b * C.m] = s;
-    H[C.M + b + a * C.m] = s;
+    if (ta == tb) return;            // a diagonal tile holds both triangles already (uniform per workgroup)
+    tile[i][j] = s;
+    __syncthreads();
+    const int r = tb * 16 + i, c = ta * 16 + j;      // M[r, c] = M[c, r] = s(a = c, b = r) = tile[j][i]
+    if (r < C.m && c < C.m) H[C.M + r + (int64_t)c * C.m] = tile[j][i];
 }
 extern "C" int bk_schur(const ConItem *it, int n_it) {
     if (n_it <= 0) return 0;
-    int mxd = 0; int64_t mxp = 0; int anyd = 0, anyr = 0;
+    std::vector<int64_t> cd(n_it), cr(n_it);
+    int anyd = 0, anyr = 0;
     for (int q = 0; q < n_it; q++) {
-        if (it[q].diag) { anyd = 1; mxd = it[q].m > mxd ? it[q].m : mxd; }
-        else { anyr = 1; const int64_t np = (int64_t)it[q].m * (it[q].m + 1) / 2; mxp = np > mxp ? np : mxp; }
+        const int64_t m = it[q].m, t = (m + 15) / 16;
+        cd[q] = it[q].diag ? ((m + 255) / 256) * m : 0;
+        cr[q] = it[q].diag ? 0 : t * (t + 1) / 2;
+        anyd |= it[q].diag != 0; anyr |= it[q].diag == 0;
     }
-    const ConItem *d = (const ConItem *)bk_stage(it, sizeof(ConItem) * n_it);
+    // items and both prefix arrays in one staged copy
+    const size_t po = (sizeof(ConItem) * n_it + 15) & ~(size_t)15, pl = sizeof(int) * (n_it + 1);
+    std::vector<char> buf(po + 2 * pl);
+    memcpy(buf.data(), it, sizeof(ConItem) * n_it);
+    int64_t tot[2] = { 0, 0 };
+    for (int w = 0; w < 2; w++) {
+        int *pre = (int *)(buf.data() + po + w * pl);
+        for (int q = 0; q < n_it; q++) { pre[q] = (int)tot[w]; tot[w] += w ? cr[q] : cd[q]; }
+        pre[n_it] = (int)tot[w];
+        if (tot[w] > 0x7fffffff) return -1;
+    }
+    const char *dv = (const char *)bk_stage(buf.data(), buf.size());
+    if (!dv) return -1;
+    const ConItem *d = (const ConItem *)dv;
     const int tok = bk_prof_begin();
-    if (anyd) {
-        if (n_it > 65535 || mxd > 65535) return -1;
-        hipLaunchKernelGGL(k_schur_diag, dim3((mxd + 255) / 256, mxd, n_it), dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    if (anyd && tot[0] > 0) {
+        hipLaunchKernelGGL(k_schur_diag, dim3((unsigned)tot[0]), dim3(256), 0, g_bk_stream, d, (const int *)(dv + po), n_it, (double *)g_heap);
         if (BK_LAUNCH_CHECK("k_schur_diag")) return -1;
     }
-    if (anyr) {
-        dim3 g((unsigned)((mxp + 255) / 256), n_it < 65535 ? n_it : 65535, (n_it + 65534) / 65535);
-        hipLaunchKernelGGL(k_schur_r0, g, dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    if (anyr && tot[1] > 0) {
+        hipLaunchKernelGGL(k_schur_r0, dim3((unsigned)tot[1]), dim3(256), 0, g_bk_stream, d, (const int *)(dv + po + pl), n_it, (double *)g_heap);
         if (BK_LAUNCH_CHECK("k_schur_r0")) return -1;
     }
     if (tok >= 0) {

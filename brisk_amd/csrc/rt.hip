@@ -101,6 +101,19 @@ extern "C" void *bk_stage(const void *host, size_t bytes) {
     return d;
 }
 
+extern "C" int bk_h2d_async(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return 0;
+    // small uploads go through the pinned ring (no wait); large ones are copied synchronously
+    if (bytes > g_ring_cap / 4) return bk_h2d(dst, src, bytes);
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (g_ring_pos + need > g_ring_cap) bk_sync();
+    char *h = g_ring_h + g_ring_pos;
+    memcpy(h, src, bytes);
+    if (bk_check(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, g_bk_stream), "hipMemcpy H2D async")) return -1;
+    g_ring_pos += need;
+    return 0;
+}
+
 extern "C" double bk_now(void) {
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);

@@ -21,6 +21,7 @@ int bk_failed(void);                             /* a HIP call has failed since 
 void *bk_malloc(size_t bytes);                   /* device memory (NULL on failure) */
 void bk_free(void *p);
 int bk_h2d(void *dst, const void *src, size_t bytes);        /* synchronous copy */
+int bk_h2d_async(void *dst, const void *src, size_t bytes);  /* copy ordered on the stream; src may be reused at once */
 int bk_d2h(void *dst, const void *src, size_t bytes);        /* synchronous (drains the stream) */
 int bk_d2d(void *dst, const void *src, size_t bytes);        /* async device copy */
 int bk_zero(void *dst, size_t bytes);                         /* async fill kernel (graph safe) */
