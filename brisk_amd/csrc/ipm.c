@@ -122,6 +122,7 @@ int64_t solve_heap_bytes(const HProb *P) {
     b += 5 * (m * m * 8 + 256) + (((m + 63) / 64) * 4096 * 8 + 256);
     b += 20 * (m * 8 + 256) + NSC * 8 + 256;
     b += (m + 1) * 4 + (int64_t)P->nent * (sizeof(SEnt) + 12) + (int64_t)P->nb * sizeof(BlkRef) + (P->npos + 1) * 12 + 2048;
+    b += (m + 1) * 4 + (int64_t)(P->ngrp + 1) * sizeof(GrpRef) + 2 * (P->gp_tot * 8 + 256) + 512;
     if (P->dense) {
         const int64_t nn = T;
         b += nn * m * (8 + 4 + 8 + 8) + 4 * 256;      /* Ad, Af, P, B */
@@ -158,6 +159,8 @@ int solve_setup(Solve *S) {
     ALLOCB(cptr, (m + 1) * 4); ALLOCB(ent, (int64_t)(P->nent + 1) * sizeof(SEnt)); ALLOCB(blk, (int64_t)P->nb * sizeof(BlkRef));
     ALLOCB(pptr, (int64_t)(P->npos + 1) * 4); ALLOCB(ppos, (int64_t)(P->npos + 1) * 8); ALLOCB(pcon, (int64_t)(P->nent + 1) * 4);
     ALLOC(pval, (int64_t)(P->nent + 1) * 8);
+    ALLOCB(gptr, (m + 1) * 4); ALLOCB(grp, (int64_t)(P->ngrp + 1) * sizeof(GrpRef));
+    if (P->gp_tot > 0) { ALLOC(Pg, P->gp_tot * 8); ALLOC(Tg, P->gp_tot * 8); }
     if (P->dense) {
         const int64_t nn = T;
         const int tl = (int)((m + 127) / 128), lower = tl * (tl + 1) / 2;
@@ -187,6 +190,14 @@ int solve_setup(Solve *S) {
     rc |= bk_h2d_async(H + S->pptr, P->pptr, (int64_t)(P->npos + 1) * 4);
     if (P->npos) rc |= bk_h2d_async(H + S->ppos, P->ppos, (int64_t)P->npos * 8);
     if (P->nent) { rc |= bk_h2d_async(H + S->pcon, P->pcon, (int64_t)P->nent * 4); rc |= bk_h2d_async(H + S->pval * 8, P->pval, (int64_t)P->nent * 8); }
+    rc |= bk_h2d_async(H + S->gptr, P->gptr, (m + 1) * 4);
+    if (P->ngrp) {
+        /* heavy groups: P offsets made absolute (doubles from the heap base) */
+        GrpRef *gr = malloc(sizeof(GrpRef) * P->ngrp);
+        for (int g = 0; g < P->ngrp; g++) { gr[g] = P->grp[g]; if (gr[g].poff >= 0) gr[g].poff += S->Pg; }
+        rc |= bk_h2d_async(H + S->grp, gr, (int64_t)P->ngrp * sizeof(GrpRef));
+        free(gr);
+    }
     if (P->dense) {
         rc |= bk_h2d_async(H + S->Ad * 8, P->Ad, T * m * 8);
         rc |= bk_d2f(T * m, S->Ad, 2 * S->Af);
@@ -214,6 +225,7 @@ static ConItem con_item(Solve *S) {
     c.m = S->P.m; c.nblk = S->P.nb;
     c.cptr = S->cptr; c.ent = S->ent; c.blk = S->blk;
     c.pptr = S->pptr; c.ppos = S->ppos; c.pcon = S->pcon; c.pval = S->pval;
+    c.gptr = S->gptr; c.grp = S->grp;
     c.npos = S->P.npos; c.diag = S->P.diag;
     return c;
 }
@@ -570,7 +582,44 @@ static int schur_dense(Solve *s, int prec, int gram) {
     s->m_exact = prec;
     return 0;
 }
+/* the products of the heavy groups at the current X, Zi: T = A Zi_k, then P = X_k T (MFMA) */
+static int heavy_products(Solve **S, int ns, const int *act) {
+    int ngi = 0;
+    for (int q = 0; q < ns; q++) if ((!act || act[q]) && !S[q]->P.dense && S[q]->P.gp_tot > 0) ngi = 1;
+    if (!ngi) return 0;
+    GrpItem *gi = NULL;
+    GemmItem *gm = NULL;
+    int n = 0, cap = 0;
+    for (int q = 0; q < ns; q++) {
+        if ((act && !act[q]) || S[q]->P.dense || S[q]->P.gp_tot <= 0) continue;
+        Solve *s = S[q];
+        const HProb *P = &s->P;
+        for (int g = 0; g < P->ngrp; g++) {
+            const GrpRef *G = &P->grp[g];
+            if (G->poff < 0) continue;
+            if (n == cap) { cap = cap ? 2 * cap : 256; gi = realloc(gi, sizeof(GrpItem) * cap); gm = realloc(gm, sizeof(GemmItem) * cap); }
+            const int nk = P->bn[G->k];
+            const int64_t boff = P->boff[G->k];
+            GrpItem it = { nk, G->cnt, s->ent + (int64_t)G->beg * (int64_t)sizeof(SEnt), s->Zi + boff, s->Tg + G->poff };
+            gi[n] = it;
+            GemmItem m;
+            memset(&m, 0, sizeof(m));
+            m.M = nk; m.N = nk; m.K = nk; m.lda = nk; m.ldb = nk; m.ldc = nk;
+            m.a = s->X + boff; m.b = s->Tg + G->poff; m.c = s->Pg + G->poff;
+            m.guard = -1; m.tri = 0;
+            gm[n] = m;
+            n++;
+        }
+    }
+    double *H = heapp();
+    int rc = bk_grp_t(gi, n);
+    if (!rc) rc = bk_gemm_f64(0, 0, 1.0, 0.0, H, H, H, gm, n, NULL);
+    free(gi); free(gm);
+    return rc;
+}
+
 static int schur_assemble(Solve **S, int ns, const int *act, int gram) {
+    if (heavy_products(S, ns, act)) return -1;
     ConL_clear(&g_cl);
     for (int q = 0; q < ns; q++) {
         if (act && !act[q]) continue;

@@ -403,57 +403,153 @@ __global__ __launch_bounds__(256) void k_schur_diag(const ConItem *__restrict__ 
 // Schur complement, route 0: M_ab = sum over the entries p of A_a and q of A_b in a common block
 // of v_p v_q X[c_p, r_q] Zi[c_q, r_p] over the symmetric expansions of both (reference
 // solver.c:2626-2660). A workgroup computes a 16 x 16 tile of pairs of the lower triangle of tiles
-// (16 x 16 threads, rows a along the fast thread index); the constraints' entries are grouped by
-// block in ascending order, so the common blocks are found by a merge. The tile is stored at
-// (a, b) and, through LDS, transposed at (b, a): both stores are 16-row coalesced runs.
-__global__ __launch_bounds__(256) void k_schur_r0(const ConItem *__restrict__ it, const int *__restrict__ pre, int nit,
-                                                  double *__restrict__ H) {
-    const int q = flat_item(pre, nit, blockIdx.x);
-    const ConItem C = it[q];
-    int ta, tb;
-    tri_index(blockIdx.x - pre[q], ta, tb);
-    const int i = threadIdx.x & 15, j = threadIdx.x >> 4;
-    const int a = ta * 16 + i, b = tb * 16 + j;
-    __shared__ double tile[16][17];
+// (16 x 16 threads, rows a along the fast thread index). The tile's 32 constraints are read once
+// into LDS (entry range, first entry, its block); a pair of single-entry constraints (a Lovasz
+// theta edge, for instance) is then four products from X and Zi with no further index loads,
+// other pairs merge their entries (grouped by block in ascending order) over the common blocks.
+// The tile is stored at (a, b) and, through LDS, transposed at (b, a): both stores are 16-row
+// coalesced runs.
+__device__ __forceinline__ double pair_term(const SEnt &P, const SEnt &Q, const double *Xk, const double *Zk, int n) {
+    // expansions (r, c) and, off the diagonal, (c, r) of both entries
+    double acc = Xk[P.c + (int64_t)Q.r * n] * Zk[Q.c + (int64_t)P.r * n];
+    if (Q.r != Q.c) acc += Xk[P.c + (int64_t)Q.c * n] * Zk[Q.r + (int64_t)P.r * n];
+    if (P.r != P.c) {
+        acc += Xk[P.r + (int64_t)Q.r * n] * Zk[Q.c + (int64_t)P.c * n];
+        if (Q.r != Q.c) acc += Xk[P.r + (int64_t)Q.c * n] * Zk[Q.r + (int64_t)P.c * n];
+    }
+    return P.v * Q.v * acc;
+}
+// <A_l, P>: the entries of a (light) group against a heavy group's product P = X A_h Zi
+__device__ __forceinline__ double group_dot(const SEnt *__restrict__ e, int cnt, const double *__restrict__ Pm, int n) {
     double s = 0;
-    if (a < C.m && b < C.m) {
-        const int *cptr = HI(H, C.cptr);
+    for (int w = 0; w < cnt; w++) {
+        const SEnt E = e[w];
+        double x = Pm[E.r + (int64_t)E.c * n];
+        if (E.r != E.c) x += Pm[E.c + (int64_t)E.r * n];
+        s += E.v * x;
+    }
+    return s;
+}
+__global__ __launch_bounds__(256) void k_schur_r0(const ConItem *__restrict__ it, const int *__restrict__ pre, int nit,
+                                                  int64_t ntiles, double *__restrict__ H) {
+    __shared__ double tile[16][17];
+    __shared__ int gbeg[32], gcnt[32];
+    __shared__ GrpRef g0[32];
+    __shared__ SEnt first[32];
+    __shared__ BlkRef fblk[32];
+    const int i = threadIdx.x & 15, j = threadIdx.x >> 4;
+    // a persistent grid: each workgroup walks the tiles of every item with a stride
+    for (int64_t vb = blockIdx.x; vb < ntiles; vb += gridDim.x) {
+        const int q = flat_item(pre, nit, (int)vb);
+        const ConItem C = it[q];
+        int ta, tb;
+        tri_index(vb - pre[q], ta, tb);
+        const int a = ta * 16 + i, b = tb * 16 + j;
+        const int *gptr = HI(H, C.gptr);
+        const GrpRef *grp = (const GrpRef *)(((const char *)H) + C.grp);
         const SEnt *ent = (const SEnt *)(((const char *)H) + C.ent);
         const BlkRef *blk = (const BlkRef *)(((const char *)H) + C.blk);
+        __syncthreads();                 // the previous tile's LDS reads are done
+        if (threadIdx.x < 32) {
+            const int c = threadIdx.x < 16 ? ta * 16 + threadIdx.x : tb * 16 + threadIdx.x - 16;
+            int b0 = 0, nc = 0;
+            if (c < C.m) { b0 = gptr[c]; nc = gptr[c + 1] - b0; }
+            gbeg[threadIdx.x] = b0; gcnt[threadIdx.x] = nc;
+            if (nc > 0) {
+                const GrpRef G = grp[b0];
+                g0[threadIdx.x] = G;
+                const SEnt e = ent[G.beg];
+                first[threadIdx.x] = e; fblk[threadIdx.x] = blk[e.k];
+            }
+        }
+        __syncthreads();
         const double *X = H + C.X, *Zi = H + C.Zi;
-        int pa = cptr[a], ea = cptr[a + 1], pb = cptr[b], eb = cptr[b + 1];
-        while (pa < ea && pb < eb) {
-            const int ka = ent[pa].k, kb = ent[pb].k;
-            if (ka < kb) { pa++; continue; }
-            if (kb < ka) { pb++; continue; }
-            int qa = pa; while (qa < ea && ent[qa].k == ka) qa++;
-            int qb = pb; while (qb < eb && ent[qb].k == ka) qb++;
-            const BlkRef B = blk[ka];
-            const int n = B.n;
-            const double *Xk = X + B.off, *Zk = Zi + B.off;
-            for (int u = pa; u < qa; u++) {
-                const SEnt P = ent[u];
-                for (int w = pb; w < qb; w++) {
-                    const SEnt Q = ent[w];
-                    // expansions (r, c) and, off the diagonal, (c, r) of both entries
-                    double acc = Xk[P.c + (int64_t)Q.r * n] * Zk[Q.c + (int64_t)P.r * n];
-                    if (Q.r != Q.c) acc += Xk[P.c + (int64_t)Q.c * n] * Zk[Q.r + (int64_t)P.r * n];
-                    if (P.r != P.c) {
-                        acc += Xk[P.r + (int64_t)Q.r * n] * Zk[Q.c + (int64_t)P.c * n];
-                        if (Q.r != Q.c) acc += Xk[P.r + (int64_t)Q.c * n] * Zk[Q.r + (int64_t)P.c * n];
+        double s = 0;
+        if (a < C.m && b < C.m) {
+            const int na = gcnt[i], nb = gcnt[16 + j];
+            if (na == 1 && nb == 1 && g0[i].cnt == 1 && g0[16 + j].cnt == 1 && g0[i].poff < 0 && g0[16 + j].poff < 0) {
+                // two single-entry constraints: four products
+                const SEnt P = first[i], Q = first[16 + j];
+                if (P.k == Q.k) {
+                    const BlkRef B = fblk[i];
+                    s = pair_term(P, Q, X + B.off, Zi + B.off, B.n);
+                }
+            } else {
+                // merge the two group lists (ascending block) over the common blocks
+                int pa = gbeg[i], ea = pa + na, pb = gbeg[16 + j], eb = pb + nb;
+                while (pa < ea && pb < eb) {
+                    const GrpRef Ga = grp[pa], Gb = grp[pb];
+                    if (Ga.k < Gb.k) { pa++; continue; }
+                    if (Gb.k < Ga.k) { pb++; continue; }
+                    const BlkRef B = blk[Ga.k];
+                    if (Ga.poff >= 0 && (Gb.poff < 0 || Gb.cnt <= Ga.cnt)) s += group_dot(ent + Gb.beg, Gb.cnt, H + Ga.poff, B.n);
+                    else if (Gb.poff >= 0) s += group_dot(ent + Ga.beg, Ga.cnt, H + Gb.poff, B.n);
+                    else {
+                        const double *Xk = X + B.off, *Zk = Zi + B.off;
+                        for (int u = 0; u < Ga.cnt; u++) {
+                            const SEnt P = ent[Ga.beg + u];
+                            for (int w = 0; w < Gb.cnt; w++) s += pair_term(P, ent[Gb.beg + w], Xk, Zk, B.n);
+                        }
                     }
-                    s += P.v * Q.v * acc;
+                    pa++; pb++;
                 }
             }
-            pa = qa; pb = qb;
+            H[C.M + a + (int64_t)b * C.m] = s;
         }
-        H[C.M + a + (int64_t)b * C.m] = s;
+        if (ta != tb) {                  // a diagonal tile holds both triangles already (uniform per workgroup)
+            tile[i][j] = s;
+            __syncthreads();
+            const int r = tb * 16 + i, c = ta * 16 + j;      // M[r, c] = M[c, r] = s(a = c, b = r) = tile[j][i]
+            if (r < C.m && c < C.m) H[C.M + r + (int64_t)c * C.m] = tile[j][i];
+        }
     }
-    if (ta == tb) return;            // a diagonal tile holds both triangles already (uniform per workgroup)
-    tile[i][j] = s;
-    __syncthreads();
-    const int r = tb * 16 + i, c = ta * 16 + j;      // M[r, c] = M[c, r] = s(a = c, b = r) = tile[j][i]
-    if (r < C.m && c < C.m) H[C.M + r + (int64_t)c * C.m] = tile[j][i];
+}
+
+// T = A Zi_k for the heavy groups: T[r, j] = sum over the expanded entries (r, c, v) of A of
+// v Zi[c, j]; workgroups stride the n^2 outputs (rows fastest), the entries staged in LDS
+__global__ __launch_bounds__(256) void k_grp_t(const GrpItem *__restrict__ it, const int *__restrict__ pre, int nit,
+                                               double *__restrict__ H) {
+    const int q = flat_item(pre, nit, blockIdx.x);
+    const GrpItem G = it[q];
+    const int64_t lb = blockIdx.x - pre[q], nbk = pre[q + 1] - pre[q];
+    const int n = G.n;
+    const SEnt *ent = (const SEnt *)(((const char *)H) + G.ent);
+    const double *Zi = H + G.zi;
+    double *T = H + G.t;
+    __shared__ SEnt se[256];
+    for (int64_t o0 = lb * 256; o0 < (int64_t)n * n; o0 += nbk * 256) {
+        const int64_t o = o0 + threadIdx.x;
+        const int r = (int)(o % n), jc = (int)(o / n);
+        double acc = 0;
+        for (int c0 = 0; c0 < G.cnt; c0 += 256) {
+            const int nc = min(256, G.cnt - c0);
+            __syncthreads();
+            if (threadIdx.x < nc) se[threadIdx.x] = ent[c0 + threadIdx.x];
+            __syncthreads();
+            if (o < (int64_t)n * n) {
+                for (int w = 0; w < nc; w++) {
+                    const SEnt E = se[w];
+                    if (E.r == r) acc += E.v * Zi[E.c + (int64_t)jc * n];
+                    if (E.c == r && E.r != E.c) acc += E.v * Zi[E.r + (int64_t)jc * n];
+                }
+            }
+        }
+        if (o < (int64_t)n * n) T[o] = acc;
+    }
+}
+extern "C" int bk_grp_t(const GrpItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    std::vector<int64_t> cnt(n_it);
+    for (int q = 0; q < n_it; q++) cnt[q] = flat_cap(((int64_t)it[q].n * it[q].n + 255) / 256, 64);
+    const GrpItem *d = nullptr;
+    const int *pre = nullptr;
+    const int64_t nb = flat_stage_items(it, sizeof(GrpItem) * n_it, cnt, (const void **)&d, &pre);
+    if (nb < 0) return -1;
+    if (nb == 0) return 0;
+    const int tok = bk_prof_begin();
+    hipLaunchKernelGGL(k_grp_t, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, (double *)g_heap);
+    if (tok >= 0) bk_prof_end(tok, 10, 0, 0);
+    return BK_LAUNCH_CHECK("k_grp_t");
 }
 extern "C" int bk_schur(const ConItem *it, int n_it) {
     if (n_it <= 0) return 0;
@@ -485,7 +581,8 @@ extern "C" int bk_schur(const ConItem *it, int n_it) {
         if (BK_LAUNCH_CHECK("k_schur_diag")) return -1;
     }
     if (anyr && tot[1] > 0) {
-        hipLaunchKernelGGL(k_schur_r0, dim3((unsigned)tot[1]), dim3(256), 0, g_bk_stream, d, (const int *)(dv + po + pl), n_it, (double *)g_heap);
+        const unsigned g = (unsigned)(tot[1] < 8192 ? tot[1] : 8192);
+        hipLaunchKernelGGL(k_schur_r0, dim3(g), dim3(256), 0, g_bk_stream, d, (const int *)(dv + po + pl), n_it, tot[1], (double *)g_heap);
         if (BK_LAUNCH_CHECK("k_schur_r0")) return -1;
     }
     if (tok >= 0) {

@@ -36,7 +36,7 @@ static int cmp_pos(const void *a, const void *b) {
 void hprob_free(HProb *P) {
     free(P->bs_in); free(P->in_first); free(P->bn); free(P->boff);
     free(P->b); free(P->b0); free(P->d); free(P->du); free(P->C);
-    free(P->cptr); free(P->ent); free(P->pptr); free(P->ppos); free(P->pcon); free(P->pval); free(P->Ad);
+    free(P->cptr); free(P->ent); free(P->gptr); free(P->grp); free(P->pptr); free(P->ppos); free(P->pcon); free(P->pval); free(P->Ad);
     memset(P, 0, sizeof(*P));
 }
 
@@ -70,6 +70,7 @@ static int hprob_build_dense(HProb *P, int m, int n, int64_t nnz, const int *mat
     }
     P->cptr = calloc(m + 1, sizeof(int));
     P->ent = malloc(sizeof(SEnt));
+    P->gptr = calloc(m + 1, sizeof(int)); P->grp = malloc(sizeof(GrpRef)); P->ngrp = 0; P->gp_tot = 0;
     P->pptr = calloc(1, sizeof(int)); P->ppos = malloc(sizeof(int) * 2); P->pcon = malloc(sizeof(int)); P->pval = malloc(sizeof(double));
     return 0;
 }
@@ -105,6 +106,58 @@ static void finish_scaling(HProb *P) {
 }
 
 /* returns 0, or 2 for data that is not a valid problem (the reference's exit code for bad data) */
+/* groups of entries per (constraint, block) and the heavy ones: a group of 3 or more entries
+ * of an SDP block keeps its product P = X A Zi (one MFMA GEMM per group), so that a pair of
+ * constraints costs 2 |A_light| loads instead of 4 |A_a| |A_b|. The heaviest groups are chosen
+ * first, within a budget of device memory for their P and T arrays (the larger of 256 MB and
+ * 16 m^2 bytes). */
+static const GrpRef *g_sort_grp;      /* the groups being ranked (set around the qsort) */
+static int cmp_grp_cnt(const void *a, const void *b) {
+    const GrpRef *G = g_sort_grp;
+    const int x = *(const int *)a, y = *(const int *)b;
+    if (G[x].cnt != G[y].cnt) return G[y].cnt - G[x].cnt;
+    return x - y;
+}
+static int build_groups(HProb *P) {
+    const int m = P->m;
+    int ng = 0;
+    for (int a = 0; a < m; a++)
+        for (int e = P->cptr[a]; e < P->cptr[a + 1]; e++)
+            if (e == P->cptr[a] || P->ent[e].k != P->ent[e - 1].k) ng++;
+    P->ngrp = ng;
+    P->gptr = calloc(m + 1, sizeof(int));
+    P->grp = malloc(sizeof(GrpRef) * (ng + 1));
+    if (!P->gptr || !P->grp) return -1;
+    int g = 0;
+    for (int a = 0; a < m; a++) {
+        P->gptr[a] = g;
+        for (int e = P->cptr[a]; e < P->cptr[a + 1]; e++) {
+            if (e == P->cptr[a] || P->ent[e].k != P->ent[e - 1].k) {
+                GrpRef r = { P->ent[e].k, e, 0, 0, -1 };
+                P->grp[g++] = r;
+            }
+            P->grp[g - 1].cnt++;
+        }
+    }
+    P->gptr[m] = g;
+    P->gp_tot = 0;
+    int *cand = malloc(sizeof(int) * (ng + 1)), nc = 0;
+    for (int q = 0; q < ng; q++) if (P->grp[q].cnt >= 3 && P->bn[P->grp[q].k] > 1) cand[nc++] = q;
+    g_sort_grp = P->grp;
+    qsort(cand, nc, sizeof(int), cmp_grp_cnt);
+    g_sort_grp = NULL;
+    const int64_t budget = (int64_t)fmax(256.0 * (1 << 20), 16.0 * (double)m * m);
+    for (int t = 0; t < nc; t++) {
+        GrpRef *G = &P->grp[cand[t]];
+        const int64_t n = P->bn[G->k];
+        if ((P->gp_tot + n * n) * 16 > budget) continue;
+        G->poff = P->gp_tot;
+        P->gp_tot += n * n;
+    }
+    free(cand);
+    return 0;
+}
+
 int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64_t nnz, const int *mat, const int *blk,
                 const int *ii, const int *jj, const double *v) {
     memset(P, 0, sizeof(*P));
@@ -213,6 +266,7 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
         P->ent[q].k = tr[q].k; P->ent[q].r = tr[q].r; P->ent[q].c = tr[q].c; P->ent[q].pad = 0; P->ent[q].v = tr[q].v;
     }
     for (int i = 0; i < m; i++) P->cptr[i + 1] += P->cptr[i];
+    if (build_groups(P)) { free(tr); return 2; }
     P->diag = m > 0;
     for (int i = 0; i < m && P->diag; i++) {
         if (P->cptr[i + 1] - P->cptr[i] != 1) P->diag = 0;

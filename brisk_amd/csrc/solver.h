@@ -31,6 +31,12 @@ typedef struct {
     int *cptr;
     SEnt *ent;
     int nent;
+    /* groups: the runs of one constraint's entries in one block (gptr: m+1); heavy groups keep
+     * their product X A Zi (gp_tot doubles in all, poff within that arena) */
+    int ngrp;
+    int *gptr;
+    GrpRef *grp;
+    int64_t gp_tot;
     /* positions touched by constraints, each with its (constraint, value) list */
     int npos;
     int *pptr, *ppos, *pcon;
@@ -65,6 +71,7 @@ typedef struct Solve {
     int64_t y, dy, rp, rhs, a0, h, qv, ev, wv, bvec, duv, pr, pz, pp, pq, pxb, t1, t2, yb;
     int64_t sc;             /* 64 scalars */
     int64_t cptr, ent, blk, pptr, ppos, pcon, pval, dinvoff; /* problem data on the device */
+    int64_t gptr, grp, Pg, Tg;      /* constraint groups (bytes), heavy-group products P and T (doubles) */
     int64_t Ad, Af, Pm, Bm, Mpart, LXf, LZif, Om1, Om2;  /* dense route: A f64 / f32 (n^2 x m), P, B, split-K partials, f32 factors, M v scratch */
     int nsplit, kchunk;     /* dense route: split-K of M = P'P */
     int m_exact;            /* the stored M is exact in f64 (else M v uses the operator form) */

@@ -130,12 +130,17 @@ int bk_d2f(int64_t n, int64_t src_off, int64_t dst_off_f32);
 /* ---- constraint operators (route 0: sparse entries) ---- */
 typedef struct { int k, r, c, pad; double v; } SEnt;   /* upper entry (r <= c) of block k */
 typedef struct { int n; int pad; int64_t off; } BlkRef; /* block k: order and offset in the solve's block arena */
+/* the entries of one constraint in one block (a run of its SEnt list): block k, first entry beg,
+ * count cnt; poff >= 0 (doubles) where the product P = X_k A Zi_k of a heavy group is kept */
+typedef struct { int k, beg, cnt, pad; int64_t poff; } GrpRef;
 /* one item per solve: m constraints; CSR cptr (m+1 ints) into SEnt entries (ent); block table
  * (blk); cptr, ent, blk, pptr, ppos, pcon are BYTE offsets from the heap base, the rest doubles;
  * aop: out[i] = beta out[i] + alpha <A_i, D> (D read as sym(D): both triangles);
  * aty: D[pos] += alpha sum_i y_i A_i over the positions of the position CSR (pptr, pcon, pval);
  * schur: M (m x m full) = sum over common blocks of sum_{p,q} v_p v_q X[c_p, r_q] Zi[c_q, r_p]
- * (full symmetric expansion of each upper entry) */
+ * (full symmetric expansion of each upper entry); where one of the two constraints is heavy in
+ * the block (its product P = X A Zi stored, see bk_grp_t), <A_light, P_heavy> instead;
+ * gptr (m+1 ints) and grp (GrpRef) are the constraints' groups (BYTE offsets) */
 typedef struct {
     int m, nblk;
     int64_t cptr, ent, blk;          /* constraint CSR (int), SEnt, BlkRef */
@@ -144,10 +149,15 @@ typedef struct {
     int64_t dpos, dval;              /* diag: per constraint its position in the arena (int64) and value */
     int64_t D, out, y, X, Zi, M;     /* operand offsets (doubles) for this call */
     double alpha, beta;
+    int64_t gptr, grp;               /* constraint groups (BYTE offsets) */
 } ConItem;
 int bk_aop(const ConItem *it, int n_it);
 int bk_aty(const ConItem *it, int n_it);
 int bk_schur(const ConItem *it, int n_it);
+/* T = A Zi_k for heavy groups: the n x n product of a group's sparse symmetric entries (SEnt
+ * list at byte offset ent, cnt entries) with the block of Zi at zi; stored at t (doubles) */
+typedef struct { int n, cnt; int64_t ent, zi, t; } GrpItem;
+int bk_grp_t(const GrpItem *it, int n_it);
 
 /* ---- Lanczos on explicit symmetric T (per block) ---- */
 /* step j for every item: given w = T q_j (at w), two passes of classical Gram-Schmidt against
