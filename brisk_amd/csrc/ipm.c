@@ -313,7 +313,7 @@ static int blk_sym(Solve **S, int ns, const int *act, size_t f) {
 /* Cholesky of every block of the active (solve, side) pairs in one batched factorisation:
  * L = V (copied), factored in place; side s uses the fields fV[s], fL[s], fD[s] and the scratch
  * fW[s]; act and ok are indexed side * ns + q (ok = 1 when every block is positive definite) */
-static int chol_sides(Solve **S, int ns, int nside, const int *act, const size_t *fV, const size_t *fL, const size_t *fD,
+static int chol_sides_(Solve **S, int ns, int nside, const int *act, const size_t *fV, const size_t *fL, const size_t *fD,
                       const size_t *fW, int *ok) {
     LinL_clear(&g_ll);
     for (int sd = 0; sd < nside; sd++)
@@ -346,6 +346,13 @@ static int chol_sides(Solve **S, int ns, int nside, const int *act, const size_t
     }
     free(first);
     return 0;
+}
+static int chol_sides(Solve **S, int ns, int nside, const int *act, const size_t *fV, const size_t *fL, const size_t *fD,
+                      const size_t *fW, int *ok) {
+    const double t_ = bk_prof_on() ? bk_now() : 0;
+    const int r_ = chol_sides_(S, ns, nside, act, fV, fL, fD, fW, ok);
+    if (t_ > 0) bk_prof_host(49, bk_now() - t_);
+    return r_;
 }
 static int blk_chol(Solve **S, int ns, const int *act, size_t fV, size_t fL, size_t fDinv, int *ok) {
     const size_t a[1] = { fV }, b[1] = { fL }, c[1] = { fDinv }, w[1] = { OFF(W2) };
@@ -428,7 +435,7 @@ static void tri_min(int k, const double *a, const double *b, double *theta, doub
  * coefficients come back and a block whose extreme Ritz value is resolved to lz_tol stops
  * (reference solver.c:6083-6225). out[side * ns + q] = min over the blocks of solve q of the
  * bound theta - res of lambda_min(T) (want_step == 0) or the step 1/max(0, -lambda) (1). */
-static int lanczos_min(Solve **S, int ns, const int *act, int nside, const size_t *fLi, const size_t *fLiT, const size_t *fV,
+static int lanczos_min_(Solve **S, int ns, const int *act, int nside, const size_t *fLi, const size_t *fLiT, const size_t *fV,
                        int want_step, double lz_tol, double *out) {
     LzL_clear(&g_zl);
     int kmax = 0, nbt = 0;
@@ -457,7 +464,31 @@ static int lanczos_min(Solve **S, int ns, const int *act, int nside, const size_
     int rc = -1;
     double *H = heapp();
     LzItem *zz = malloc(sizeof(LzItem) * (nz + 1));
-    if (bk_lanczos_start(g_zl.v, nz)) goto out;
+    LzfItem *fz = malloc(sizeof(LzfItem) * (nz + 1));
+    int *fused = calloc(nz + 1, sizeof(int));
+    int nf = 0, nrest = 0;
+    /* blocks of order <= 256: the whole bound in one fused launch; the rest step below */
+    kmax = 0;
+    for (int c = 0; c < nz; c++) {
+        const LzItem *z = &g_zl.v[c];
+        if (z->n <= 256) {
+            Solve *s = S[kq[c]];
+            const int sd = ks[c];
+            const int64_t bo = s->P.boff[kb[c]];
+            LzfItem f;
+            memset(&f, 0, sizeof(f));
+            f.n = z->n; f.K = z->K; f.mode = fLi[sd] ? 1 : 0;
+            f.Li = fLi[sd] ? F_(s, fLi[sd]) + bo : 0; f.LiT = fLiT[sd] ? F_(s, fLiT[sd]) + bo : 0; f.V = F_(s, fV[sd]) + bo;
+            f.out = z->ab; f.tol = lz_tol;
+            fz[nf++] = f;
+            fused[c] = 1; live[c] = 0;
+        } else {
+            zz[nrest++] = *z;
+            if (z->K > kmax) kmax = z->K;
+        }
+    }
+    if (nf && bk_lanczos_fused(fz, nf)) goto out;
+    if (nrest && bk_lanczos_start(zz, nrest)) goto out;
     for (int j = 0; j < kmax; j++) {
         /* w = T q_j for the live blocks */
         const int nstage = fLi[0] ? 3 : 1;
@@ -514,6 +545,10 @@ static int lanczos_min(Solve **S, int ns, const int *act, int nside, const size_
         }
         if (!any) break;
     }
+    if (nf) {
+        if (bk_d2h(g_hab, H + g_bab / 8, sizeof(double) * 64 * nz)) goto out;
+        for (int c = 0; c < nz; c++) if (fused[c]) lam[c] = g_hab[64 * c];
+    }
     for (int side = 0; side < nside; side++)
         for (int q = 0; q < ns; q++) if (!act || act[q]) out[side * ns + q] = want_step ? 1e30 : INFINITY;
     for (int c = 0; c < nz; c++) {
@@ -523,18 +558,31 @@ static int lanczos_min(Solve **S, int ns, const int *act, int nside, const size_
     }
     rc = 0;
 out:
-    free(kq); free(kb); free(ks); free(live); free(lam); free(zz);
+    free(kq); free(kb); free(ks); free(live); free(lam); free(zz); free(fz); free(fused);
     return rc;
+}
+static int lanczos_min(Solve **S, int ns, const int *act, int nside, const size_t *fLi, const size_t *fLiT, const size_t *fV,
+                       int want_step, double lz_tol, double *out) {
+    const double t_ = bk_prof_on() ? bk_now() : 0;
+    const int r_ = lanczos_min_(S, ns, act, nside, fLi, fLiT, fV, want_step, lz_tol, out);
+    if (t_ > 0) bk_prof_host(50, bk_now() - t_);
+    return r_;
 }
 /* alpha_max per solve for X + ap dX >= 0 and Z + ad dZ >= 0 given the inverse Cholesky factors
  * (reference maxstep, solver.c:6300-6388): both sides in the same launches */
-static int maxstep2(Solve **S, int ns, const int *act, double *ap, double *ad) {
+static int maxstep2_(Solve **S, int ns, const int *act, double *ap, double *ad) {
     const size_t fLi[2] = { OFF(LXi), OFF(LZi) }, fLiT[2] = { OFF(LXiT), OFF(LZiT) }, fV[2] = { OFF(dX), OFF(dZ) };
     double *o = malloc(sizeof(double) * 2 * ns);
     const int rc = lanczos_min(S, ns, act, 2, fLi, fLiT, fV, 1, 0.02, o);
     for (int q = 0; q < ns; q++) { ap[q] = o[q]; ad[q] = o[ns + q]; }
     free(o);
     return rc;
+}
+static int maxstep2(Solve **S, int ns, const int *act, double *ap, double *ad) {
+    const double t_ = bk_prof_on() ? bk_now() : 0;
+    const int r_ = maxstep2_(S, ns, act, ap, ad);
+    if (t_ > 0) bk_prof_host(43, bk_now() - t_);
+    return r_;
 }
 
 /* ---- the Schur complement: assemble, factor, solve ------------------------------------------ */
@@ -618,7 +666,7 @@ static int heavy_products(Solve **S, int ns, const int *act) {
     return rc;
 }
 
-static int schur_assemble(Solve **S, int ns, const int *act, int gram) {
+static int schur_assemble_(Solve **S, int ns, const int *act, int gram) {
     if (heavy_products(S, ns, act)) return -1;
     ConL_clear(&g_cl);
     for (int q = 0; q < ns; q++) {
@@ -635,11 +683,17 @@ static int schur_assemble(Solve **S, int ns, const int *act, int gram) {
     }
     return bk_schur(g_cl.v, g_cl.n);
 }
+static int schur_assemble(Solve **S, int ns, const int *act, int gram) {
+    const double t_ = bk_prof_on() ? bk_now() : 0;
+    const int r_ = schur_assemble_(S, ns, act, gram);
+    if (t_ > 0) bk_prof_host(40, bk_now() - t_);
+    return r_;
+}
 
 /* factor M of the active solves (reference solver.c:5266-5420): f32 when allowed (accepted if
  * the smallest squared pivot is >= 1e-6), else f64 with the shift ladder and pivot test.
  * fail[q] = 1 when no factorisation was found. */
-static int schur_factor(Solve **S, int ns, const int *act, int force64, int *fail) {
+static int schur_factor_(Solve **S, int ns, const int *act, int force64, int *fail) {
     int *todo = calloc(ns, sizeof(int)), *f64 = calloc(ns, sizeof(int));
     double *reg = calloc(ns, sizeof(double));
     int nslot = ns;
@@ -740,6 +794,12 @@ err:
     free(todo); free(f64); free(reg);
     return -1;
 }
+static int schur_factor(Solve **S, int ns, const int *act, int force64, int *fail) {
+    const double t_ = bk_prof_on() ? bk_now() : 0;
+    const int r_ = schur_factor_(S, ns, act, force64, fail);
+    if (t_ > 0) bk_prof_host(41, bk_now() - t_);
+    return r_;
+}
 
 /* z = M^{-1} r through the factor: z = D Li' Li D r */
 static int schur_prec(Solve **S, int ns, const int *act, size_t fr, size_t fz) {
@@ -794,7 +854,7 @@ static int schur_mv(Solve **S, int ns, const int *act, size_t fx, size_t fy, dou
     return rc ? -1 : 0;
 }
 /* batched PCG on M x = rhs (reference schur_solve_, solver.c:5483-5622); relres[q] out */
-static int schur_solve(Solve **S, int ns, const int *act0, size_t frhs, size_t fx, double tol_rel, double *relres) {
+static int schur_solve_(Solve **S, int ns, const int *act0, size_t frhs, size_t fx, double tol_rel, double *relres) {
     int *act = malloc(sizeof(int) * ns), *used = calloc(ns, sizeof(int)), *stall = calloc(ns, sizeof(int)), *maxit = malloc(sizeof(int) * ns);
     double *nb = malloc(sizeof(double) * ns), *tol = malloc(sizeof(double) * ns), *best = malloc(sizeof(double) * ns), *rz = malloc(sizeof(double) * ns);
     double *sgain = malloc(sizeof(double) * ns);
@@ -931,6 +991,12 @@ out:
     free(act); free(used); free(stall); free(maxit); free(nb); free(tol); free(best); free(rz); free(sgain); free(smax);
     return rc;
 }
+static int schur_solve(Solve **S, int ns, const int *act0, size_t frhs, size_t fx, double tol_rel, double *relres) {
+    const double t_ = bk_prof_on() ? bk_now() : 0;
+    const int r_ = schur_solve_(S, ns, act0, frhs, fx, tol_rel, relres);
+    if (t_ > 0) bk_prof_host(42, bk_now() - t_);
+    return r_;
+}
 
 /* ---- start point (reference solver.c:13940-14210) ----------------------------------------- */
 static int set_identity(Solve **S, int ns, const int *act, size_t f) {
@@ -952,7 +1018,14 @@ static int lammin_all(Solve **S, int ns, const int *act, size_t fV, double *lam)
     return lanczos_min(S, ns, act, 1, z, z, v, 0, 0.02, lam);
 }
 
+static int ipm_start_batch_(Solve **S, int ns);
 int ipm_start_batch(Solve **S, int ns) {
+    const double t_ = bk_prof_on() ? bk_now() : 0;
+    const int r_ = ipm_start_batch_(S, ns);
+    if (t_ > 0) bk_prof_host(46, bk_now() - t_);
+    return r_;
+}
+static int ipm_start_batch_(Solve **S, int ns) {
     int rc = -1;
     int *fail = calloc(ns, sizeof(int)), *ok = calloc(ns, sizeof(int)), *act = calloc(ns, sizeof(int));
     double *rel = calloc(ns, sizeof(double)), *lx = calloc(ns, sizeof(double)), *lz = calloc(ns, sizeof(double));
@@ -1054,7 +1127,14 @@ out:
 }
 
 /* ---- one iteration (reference solver.c:14394-15189) --------------------------------------- */
+static int ipm_iterate_batch_(Solve **S, int ns);
 int ipm_iterate_batch(Solve **S, int ns) {
+    const double t_ = bk_prof_on() ? bk_now() : 0;
+    const int r_ = ipm_iterate_batch_(S, ns);
+    if (t_ > 0) bk_prof_host(45, bk_now() - t_);
+    return r_;
+}
+static int ipm_iterate_batch_(Solve **S, int ns) {
     int rc = -1;
     int *act = calloc(ns, sizeof(int)), *fail = calloc(ns, sizeof(int)), *okp = calloc(ns, sizeof(int)), *okd = calloc(ns, sizeof(int));
     int *hasH = calloc(ns, sizeof(int)), *ract = calloc(ns, sizeof(int)), *tact = calloc(ns, sizeof(int));
@@ -1505,7 +1585,14 @@ int ipm_run_batch(Solve **S, int ns) {
     }
     return ipm_end_batch(S, ns);
 }
+static int ipm_end_batch_(Solve **S, int ns);
 int ipm_end_batch(Solve **S, int ns) {
+    const double t_ = bk_prof_on() ? bk_now() : 0;
+    const int r_ = ipm_end_batch_(S, ns);
+    if (t_ > 0) bk_prof_host(47, bk_now() - t_);
+    return r_;
+}
+static int ipm_end_batch_(Solve **S, int ns) {
     if (ipm_dimacs(S, ns)) return -1;
     for (int q = 0; q < ns; q++) if (ipm_finish(S[q])) return -1;
     return 0;

@@ -18,6 +18,8 @@
 #include "rt_internal.h"
 #include "flat.h"
 
+#define BK_LZK 30                     // Lanczos steps at most (the reference's lanczos_k)
+
 static char *g_heap = nullptr;
 static size_t g_heap_size = 0;
 static double *g_partial = nullptr;        // dot-product chunk partials
@@ -684,6 +686,187 @@ __global__ __launch_bounds__(LZT) void k_lz_step(const LzItem *__restrict__ it, 
         double *qn = Q + (int64_t)(j + 1) * n;
         for (int i = t; i < n; i += LZT) qn[i] = w[i] * inv;
     }
+}
+// ---- fused Lanczos for blocks of order <= 256: one workgroup per block, one row per thread -----
+#define LZF 256
+// sum over the workgroup of np per-thread values (p <= j), fixed order: lanes, then the 4 waves
+__device__ __forceinline__ void lzf_sums(double (&c)[BK_LZK], int j, double (*red)[BK_LZK], double *cf) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+#pragma unroll
+    for (int p = 0; p < BK_LZK; p++) {
+        if (p <= j) {
+            const double v = wave_sum(c[p]);
+            if (lane == 0) red[wv][p] = v;
+        }
+    }
+    __syncthreads();
+    if (t <= j) cf[t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+    __syncthreads();
+}
+// number of eigenvalues below x of the m x m tridiagonal (a, b) (Sturm sequence)
+__device__ __forceinline__ int sturm_count(const double *a, const double *b, int m, double x) {
+    int cnt = 0;
+    double q = a[0] - x;
+    if (q < 0) cnt++;
+    for (int k = 1; k < m; k++) {
+        if (fabs(q) < 1e-300) q = q < 0 ? -1e-300 : 1e-300;
+        q = a[k] - x - b[k - 1] * b[k - 1] / q;
+        if (q < 0) cnt++;
+    }
+    return cnt;
+}
+__global__ __launch_bounds__(LZF) void k_lz_fused(const LzfItem *__restrict__ it, int nit, double *__restrict__ H) {
+    const int qi = blockIdx.x;
+    if (qi >= nit) return;
+    const LzfItem L = it[qi];
+    const int n = L.n, K = L.K, t = threadIdx.x, lane = t & 63;
+    __shared__ double xs[LZF];
+    __shared__ double red[4][BK_LZK];
+    __shared__ double cf[BK_LZK];
+    __shared__ double al[BK_LZK], be[BK_LZK];
+    __shared__ double lam_sh;
+    __shared__ int stop_sh;
+    const double *Li = H + L.Li, *LiT = H + L.LiT, *V = H + L.V;
+    double Q[BK_LZK];
+#pragma unroll
+    for (int p = 0; p < BK_LZK; p++) Q[p] = 0;
+    // q_0: the reference's start vector, normalised
+    double q0 = t < n ? 1.0 + 0.5 * sin(1.7 * t + 0.3) : 0.0;
+    {
+        double c[BK_LZK];
+        c[0] = q0 * q0;
+        lzf_sums(c, 0, red, cf);
+        q0 *= 1.0 / sqrt(cf[0]);
+    }
+    Q[0] = q0;
+    double lam = 0;
+    for (int j = 0; j < K; j++) {
+        // w = T q_j: the current basis vector through LDS, one output row per thread
+        double qj = 0;
+#pragma unroll
+        for (int p = 0; p < BK_LZK; p++) if (p == j) qj = Q[p];
+        __syncthreads();
+        xs[t] = qj;
+        __syncthreads();
+        double w = 0;
+        if (L.mode == 0) {
+            if (t < n) for (int c = 0; c < n; c++) w += V[t + (int64_t)c * n] * xs[c];
+        } else {
+            double t1 = 0;                                   // Li' q (upper: columns c >= row)
+            if (t < n) for (int c = t; c < n; c++) t1 += LiT[t + (int64_t)c * n] * xs[c];
+            __syncthreads();
+            xs[t] = t1;
+            __syncthreads();
+            double t2 = 0;                                   // V t1
+            if (t < n) for (int c = 0; c < n; c++) t2 += V[t + (int64_t)c * n] * xs[c];
+            __syncthreads();
+            xs[t] = t2;
+            __syncthreads();
+            if (t < n) for (int c = 0; c <= t; c++) w += Li[t + (int64_t)c * n] * xs[c];   // Li t2 (lower)
+        }
+        // classical Gram-Schmidt twice; the first pass's coefficient of q_j is alpha_j
+        double alpha = 0;
+        for (int pass = 0; pass < 2; pass++) {
+            double c[BK_LZK];
+#pragma unroll
+            for (int p = 0; p < BK_LZK; p++) c[p] = (p <= j) ? Q[p] * w : 0.0;
+            lzf_sums(c, j, red, cf);
+            if (pass == 0) alpha = cf[j];
+#pragma unroll
+            for (int p = 0; p < BK_LZK; p++) if (p <= j) w -= cf[p] * Q[p];
+        }
+        double beta;
+        {
+            double c[BK_LZK];
+            c[0] = w * w;
+            lzf_sums(c, 0, red, cf);
+            beta = sqrt(cf[0]);
+        }
+        if (t == 0) { al[j] = alpha; be[j] = beta; }
+        if (j + 1 < K) {
+            const double inv = beta > 1e-300 ? 1.0 / beta : 0.0;
+#pragma unroll
+            for (int p = 0; p < BK_LZK; p++) if (p == j + 1) Q[p] = w * inv;
+        }
+        const int kk = j + 1;
+        if (!(kk == K || (kk >= 6 && kk % 3 == 0))) continue;
+        __syncthreads();
+        // the smallest Ritz value of the leading m x m tridiagonal: bisection with 64 Sturm counts
+        // per round (wave 0), then two inverse iterations below it for the last eigenvector entry
+        if (t < 64) {
+            int m = kk, brk = 0;
+            double scale = 0;
+            for (int u = 0; u < kk; u++) {
+                if (fabs(al[u]) > scale) scale = fabs(al[u]);
+                if (!(be[u] > 1e-12 * (scale + 1e-300))) { m = u + 1; brk = 1; break; }
+            }
+            int bad = 0;
+            for (int u = 0; u < m; u++) if (!isfinite(al[u]) || !isfinite(be[u])) bad = 1;
+            double out = -1e300;
+            int stop = 1;
+            if (!bad) {
+                double lo = INFINITY, hi = -INFINITY;
+                for (int u = 0; u < m; u++) {
+                    const double r = (u > 0 ? fabs(be[u - 1]) : 0.0) + (u + 1 < m ? fabs(be[u]) : 0.0);
+                    lo = fmin(lo, al[u] - r); hi = fmax(hi, al[u] + r);
+                }
+                for (int round = 0; round < 16; round++) {
+                    const double w_ = hi - lo;
+                    if (!(w_ > 2e-16 * fmax(fabs(lo), fabs(hi))) || w_ <= 1e-300) break;
+                    const double x = lo + w_ * (double)(lane + 1) / 65.0;
+                    const int c = sturm_count(al, be, m, x);
+                    const unsigned long long bal = __ballot(c >= 1);
+                    const int f = bal ? __ffsll((long long)bal) - 1 : 63;
+                    const double nlo = f > 0 ? lo + w_ * (double)f / 65.0 : lo;
+                    hi = lo + w_ * (double)(f + 1) / 65.0;
+                    lo = nlo;
+                }
+                const double th = 0.5 * (lo + hi);
+                double res = 0;
+                if (!brk && m < n) {
+                    // (T - sigma I) x = r, sigma just below lambda_min (an SPD tridiagonal solve)
+                    const double sig = lo - (hi - lo) - 1e-15 * (scale + 1e-300);
+                    double x[BK_LZK], dd[BK_LZK];
+#pragma unroll
+                    for (int u = 0; u < BK_LZK; u++) x[u] = 1.0;
+                    for (int iter = 0; iter < 2; iter++) {
+                        dd[0] = al[0] - sig;
+                        for (int u = 1; u < m; u++) {
+                            if (fabs(dd[u - 1]) < 1e-300) dd[u - 1] = 1e-300;
+                            const double l = be[u - 1] / dd[u - 1];
+                            dd[u] = al[u] - sig - l * be[u - 1];
+                            x[u] -= l * x[u - 1];
+                        }
+                        if (fabs(dd[m - 1]) < 1e-300) dd[m - 1] = 1e-300;
+                        x[m - 1] /= dd[m - 1];
+                        for (int u = m - 2; u >= 0; u--) x[u] = (x[u] - be[u] * x[u + 1]) / dd[u];
+                        double nx = 0;
+                        for (int u = 0; u < m; u++) nx += x[u] * x[u];
+                        nx = 1.0 / sqrt(nx);
+                        for (int u = 0; u < m; u++) x[u] *= nx;
+                    }
+                    res = fabs(be[m - 1] * x[m - 1]);
+                }
+                out = th - res;
+                stop = brk || kk >= K || res <= L.tol * fabs(th) + 1e-14 * scale;
+            }
+            if (t == 0) { lam_sh = out; stop_sh = stop; }
+        }
+        __syncthreads();
+        lam = lam_sh;
+        if (stop_sh) break;
+    }
+    if (t == 0) H[L.out] = lam;
+}
+extern "C" int bk_lanczos_fused(const LzfItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    for (int q = 0; q < n_it; q++) if (it[q].n > LZF || it[q].K > BK_LZK || it[q].K < 1) return -1;
+    const LzfItem *d = (const LzfItem *)bk_stage(it, sizeof(LzfItem) * n_it);
+    if (!d) return -1;
+    const int tok = bk_prof_begin();
+    hipLaunchKernelGGL(k_lz_fused, dim3(n_it), dim3(LZF), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    if (tok >= 0) bk_prof_end(tok, 12, 0, 0);
+    return BK_LAUNCH_CHECK("k_lz_fused");
 }
 extern "C" int bk_lanczos_start(const LzItem *it, int n_it) {
     if (n_it <= 0) return 0;

@@ -50,8 +50,14 @@ extern "C" void *bk_malloc(size_t bytes) {
 }
 extern "C" void bk_free(void *p) { if (p) (void)hipFree(p); }
 
+static int g_prof_on_host = 0;
+static inline int g_prof_host_on(void) { return g_prof_on_host; }
+// host-side counters (read as profile classes 62: waits on the stream, 63: staged copies)
+static double g_sync_ms = 0, g_sync_n = 0, g_stage_n = 0, g_stage_bytes = 0;
 extern "C" int bk_sync(void) {
+    const double t0 = g_prof_host_on() ? bk_now() : 0;
     int r = bk_check(hipStreamSynchronize(g_bk_stream), "hipStreamSynchronize");
+    if (t0 > 0) { g_sync_ms += 1e3 * (bk_now() - t0); g_sync_n += 1; }
     g_ring_pos = 0;
     return r;
 }
@@ -97,6 +103,7 @@ extern "C" void *bk_stage(const void *host, size_t bytes) {
     char *h = g_ring_h + g_ring_pos, *d = g_ring_d + g_ring_pos;
     memcpy(h, host, bytes);
     if (bk_check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, g_bk_stream), "stage")) return nullptr;
+    g_stage_n += 1; g_stage_bytes += (double)bytes;
     g_ring_pos += need;
     return d;
 }
@@ -165,9 +172,11 @@ extern "C" void bk_prof_enable(int on) {
         g_prof_evinit = 1;
     }
     g_prof_on = on;
+    g_prof_on_host = on;
 }
 extern "C" void bk_prof_reset(void) {
     prof_flush();
+    g_sync_ms = g_sync_n = g_stage_n = g_stage_bytes = 0;
     memset(g_acc_ms, 0, sizeof(g_acc_ms)); memset(g_acc_n, 0, sizeof(g_acc_n));
     memset(g_acc_flops, 0, sizeof(g_acc_flops)); memset(g_acc_bytes, 0, sizeof(g_acc_bytes));
 }
@@ -183,8 +192,15 @@ extern "C" void bk_prof_end(int tok, int cls, double flops, double bytes) {
     (void)hipEventRecord(g_prof_ev[2 * tok + 1], g_bk_stream);
     g_prof_cls[tok] = cls; g_prof_flops[tok] = flops; g_prof_bytes[tok] = bytes;
 }
+extern "C" int bk_prof_on(void) { return g_prof_on; }
+extern "C" void bk_prof_host(int cls, double seconds) {
+    if (cls < 0 || cls >= PROF_CLS) return;
+    g_acc_ms[cls] += 1e3 * seconds; g_acc_n[cls] += 1;
+}
 extern "C" void bk_prof_read(int cls, double *out4) {
     prof_flush();
+    if (cls == 62) { out4[0] = g_sync_ms; out4[1] = g_sync_n; out4[2] = 0; out4[3] = 0; return; }
+    if (cls == 63) { out4[0] = 0; out4[1] = g_stage_n; out4[2] = 0; out4[3] = g_stage_bytes; return; }
     if (cls < 0 || cls >= PROF_CLS) { out4[0] = out4[1] = out4[2] = out4[3] = 0; return; }
     out4[0] = g_acc_ms[cls]; out4[1] = g_acc_n[cls]; out4[2] = g_acc_flops[cls]; out4[3] = g_acc_bytes[cls];
 }

@@ -124,6 +124,7 @@ static void finish_done(BriskServe *sv) {
 API int brisk_serve_step(BriskServe *sv) {
     if (!sv) return -1;
     /* admission: in arrival order while a slot and the solve's projected memory are free */
+    const double t_adm = bk_prof_on() ? bk_now() : 0;
     Solve **nw = malloc(sizeof(Solve *) * sv->slots);
     int nn = 0;
     while (sv->qhead < sv->qtail && sv->nfly + nn < sv->slots) {
@@ -149,12 +150,15 @@ API int brisk_serve_step(BriskServe *sv) {
         sv->req[id].t_admit = bk_now();
         nw[nn++] = s;
     }
+    if (t_adm > 0) bk_prof_host(48, bk_now() - t_adm);
     if (nn) {
         if (ipm_start_batch(nw, nn)) { free(nw); return -1; }
         for (int q = 0; q < nn; q++) { sv->fly[sv->nfly] = nw[q]; sv->fly_id[sv->nfly] = nw[q]->id; sv->nfly++; }
     }
     free(nw);
+    const double t_fin = bk_prof_on() ? bk_now() : 0;
     finish_done(sv);
+    if (t_fin > 0) bk_prof_host(51, bk_now() - t_fin);
     if (sv->nfly) {
         if (ipm_iterate_batch(sv->fly, sv->nfly)) return -1;
         finish_done(sv);

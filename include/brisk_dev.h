@@ -42,6 +42,8 @@ void bk_prof_reset(void);
 int bk_prof_begin(void);
 void bk_prof_end(int tok, int cls, double flops, double bytes);
 void bk_prof_read(int cls, double *out4);
+int bk_prof_on(void);                            /* profiling enabled */
+void bk_prof_host(int cls, double seconds);      /* add a host-side interval to class cls (40-59) */
 /* classes: 0-3 k_gemm<double, ta, tb> (2 ta + tb), 4-7 k_gemm<float, ta, tb>, 8 k_potrf_diag,
  * 9 k_gemv, 10 Schur assembly, 11 constraint operators, 12 Lanczos steps, 13 dot products,
  * 14 linear combinations */
@@ -168,6 +170,13 @@ typedef struct { int n, K; int64_t Q, w, ab; } LzItem;
 int bk_lanczos_step(const LzItem *it, int n_it, int j);
 /* q_0 = the fixed start vector of the reference (1 + 0.5 sin(1.7 i + 0.3), normalised) */
 int bk_lanczos_start(const LzItem *it, int n_it);
+/* the whole Lanczos bound in one workgroup per block (order n <= 256): T = V (mode 0) or
+ * T = Li V Li' (mode 1, Li lower and LiT = Li' stored); K steps with CGS2 reorthogonalisation,
+ * every 3 steps from the 6th (and at K) the smallest Ritz value theta of the tridiagonal and its
+ * residual res = |beta z_last|; stops when res <= tol |theta| (+1e-14 scale), at a breakdown or
+ * at K. Writes theta - res to out (doubles), -1e300 when the coefficients are not finite. */
+typedef struct { int n, K, mode, pad; int64_t Li, LiT, V, out; double tol; } LzfItem;
+int bk_lanczos_fused(const LzfItem *it, int n_it);
 
 #ifdef __cplusplus
 }
