@@ -54,8 +54,11 @@ static int batch_scratch(int ns, int nslots) {
         if (g_bsc >= 0) heap_free(g_bsc);
         g_bsc_cap = ns + 16;
         g_bsc = heap_alloc(sizeof(double) * NSC * g_bsc_cap);
-        g_hsc = realloc(g_hsc, sizeof(double) * NSC * g_bsc_cap);
-        if (g_bsc < 0) return -1;
+        bk_sync();
+        bk_host_free(g_hsc);
+        g_hsc = bk_host_alloc(sizeof(double) * NSC * g_bsc_cap);
+        bk_dot_mirror(g_bsc / 8, (int64_t)NSC * g_bsc_cap, g_hsc);
+        if (g_bsc < 0 || !g_hsc) return -1;
     }
     if (nslots > g_bint_cap) {
         if (g_bint >= 0) heap_free(g_bint);
@@ -63,9 +66,10 @@ static int batch_scratch(int ns, int nslots) {
         g_bint_cap = nslots + 64;
         g_bint = heap_alloc(sizeof(int) * g_bint_cap);
         g_bdbl = heap_alloc(sizeof(double) * g_bint_cap);
-        g_hint = realloc(g_hint, sizeof(int) * g_bint_cap);
-        g_hdbl = realloc(g_hdbl, sizeof(double) * g_bint_cap);
-        if (g_bint < 0 || g_bdbl < 0) return -1;
+        bk_host_free(g_hint); bk_host_free(g_hdbl);
+        g_hint = bk_host_alloc(sizeof(int) * g_bint_cap);
+        g_hdbl = bk_host_alloc(sizeof(double) * g_bint_cap);
+        if (g_bint < 0 || g_bdbl < 0 || !g_hint || !g_hdbl) return -1;
     }
     return 0;
 }
@@ -76,6 +80,8 @@ void ipm_release_scratch(void) {
     if (g_bdbl >= 0) heap_free(g_bdbl);
     if (g_bab >= 0) heap_free(g_bab);
     g_bsc = g_bint = g_bdbl = g_bab = -1;
+    bk_dot_mirror(0, 0, NULL);
+    bk_host_free(g_hsc); g_hsc = NULL;
     g_bsc_cap = g_bint_cap = g_bab_cap = 0;
 }
 /* Lanczos coefficients of every block of a batch: 64 doubles per block */
@@ -84,7 +90,8 @@ static int batch_ab(int nblocks) {
     if (g_bab >= 0) heap_free(g_bab);
     g_bab_cap = nblocks + 256;
     g_bab = heap_alloc(sizeof(double) * 64 * g_bab_cap);
-    g_hab = realloc(g_hab, sizeof(double) * 64 * g_bab_cap);
+    bk_host_free(g_hab);
+    g_hab = bk_host_alloc(sizeof(double) * 64 * g_bab_cap);
     return g_bab < 0 ? -1 : 0;
 }
 static inline int64_t SC(int q, int k) { return g_bsc / 8 + (int64_t)q * NSC + k; }  /* double offset of scalar k of batch slot q */
@@ -92,7 +99,8 @@ static inline double HS(int q, int k) { return g_hsc[(size_t)q * NSC + k]; }
 static void *heapp(void) { return bk_heap(); }
 static int *dint(void) { return (int *)((char *)bk_heap() + g_bint); }
 static double *ddbl(void) { return (double *)((char *)bk_heap() + g_bdbl); }
-static int read_scalars(int ns) { return bk_d2h(g_hsc, (char *)bk_heap() + g_bsc, sizeof(double) * NSC * ns); }
+/* the scalars of the batch: the dot products write them to the pinned host mirror directly */
+static int read_scalars(int ns) { (void)ns; return bk_sync(); }
 
 /* ---- solve setup ----------------------------------------------------------------------------- */
 typedef struct { int64_t dinv_tot, lz_tot; int64_t *dboff, *lzoff; } BlkAux;
@@ -120,7 +128,7 @@ int64_t solve_heap_bytes(const HProb *P) {
     b += 2 * (dinv * 8 + 256);
     b += 2 * (BK_LZ_K + 3) * lz * 8 + 1024;
     b += 5 * (m * m * 8 + 256) + (((m + 63) / 64) * 4096 * 8 + 256);
-    b += 20 * (m * 8 + 256) + NSC * 8 + 256;
+    b += 22 * (m * 8 + 256) + NSC * 8 + 256;
     b += (m + 1) * 4 + (int64_t)P->nent * (sizeof(SEnt) + 12) + (int64_t)P->nb * sizeof(BlkRef) + (P->npos + 1) * 12 + 2048;
     b += (m + 1) * 4 + (int64_t)(P->ngrp + 1) * sizeof(GrpRef) + 2 * (P->gp_tot * 8 + 256) + 512;
     if (P->dense) {
@@ -155,7 +163,7 @@ int solve_setup(Solve *S) {
     ALLOC(MDinv, ((m + 63) / 64) * 4096 * 8); ALLOC(Dm, m * 8);
     ALLOC(y, m * 8); ALLOC(dy, m * 8); ALLOC(rp, m * 8); ALLOC(rhs, m * 8); ALLOC(a0, m * 8); ALLOC(h, m * 8); ALLOC(qv, m * 8);
     ALLOC(ev, m * 8); ALLOC(wv, m * 8); ALLOC(bvec, m * 8); ALLOC(duv, m * 8); ALLOC(pr, m * 8); ALLOC(pz, m * 8); ALLOC(pp, m * 8); ALLOC(pq, m * 8);
-    ALLOC(pxb, m * 8); ALLOC(t1, m * 8); ALLOC(t2, m * 8); ALLOC(yb, m * 8);
+    ALLOC(pxb, m * 8); ALLOC(ps, m * 8); ALLOC(t1, m * 8); ALLOC(t2, m * 8); ALLOC(yb, m * 8);
     ALLOCB(cptr, (m + 1) * 4); ALLOCB(ent, (int64_t)(P->nent + 1) * sizeof(SEnt)); ALLOCB(blk, (int64_t)P->nb * sizeof(BlkRef));
     ALLOCB(pptr, (int64_t)(P->npos + 1) * 4); ALLOCB(ppos, (int64_t)(P->npos + 1) * 8); ALLOCB(pcon, (int64_t)(P->nent + 1) * 4);
     ALLOC(pval, (int64_t)(P->nent + 1) * 8);
@@ -898,40 +906,69 @@ static int schur_solve_(Solve **S, int ns, const int *act0, size_t frhs, size_t 
         for (int q = 0; q < ns; q++) any |= act[q];
         int *was = malloc(sizeof(int) * ns);
         memcpy(was, act, sizeof(int) * ns);
+        /* Chronopoulos-Gear form of preconditioned CG: one reduction per step (r.u, w.u, r.r
+         * with u = prec(r), w = M u) instead of three; p and s = M p follow by recurrence */
+        double *al = calloc(ns, sizeof(double)), *be = calloc(ns, sizeof(double)), *ga = calloc(ns, sizeof(double));
         if (any) {
-            /* xb = x; z = prec(r); p = z; rz = r.z */
+            /* xb = x; u = prec(r); w = M u; gamma = r.u, delta = w.u */
             LinL_clear(&g_ll);
             for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->pxb, F_(S[q], fx), 1.0, -1, 0, -1, 0, -1, 0);
-            if (bk_lincomb(g_ll.v, g_ll.n)) { free(was); goto out; }
-            if (schur_prec(S, ns, act, OFF(pr), OFF(pz))) { free(was); goto out; }
-            LinL_clear(&g_ll); DotL_clear(&g_dl);
-            for (int q = 0; q < ns; q++) if (act[q]) { lin_add(S[q]->P.m, S[q]->pp, S[q]->pz, 1.0, -1, 0, -1, 0, -1, 0); dot_add(S[q]->P.m, S[q]->pr, S[q]->pz, SC(q, 0)); }
-            if (bk_lincomb(g_ll.v, g_ll.n) || bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) { free(was); goto out; }
-            for (int q = 0; q < ns; q++) if (act[q]) rz[q] = HS(q, 0);
+            if (bk_lincomb(g_ll.v, g_ll.n)) { free(was); free(al); free(be); free(ga); goto out; }
+            if (schur_prec(S, ns, act, OFF(pr), OFF(pz)) || schur_mv(S, ns, act, OFF(pz), OFF(pq), 1.0, 0.0)) { free(was); free(al); free(be); free(ga); goto out; }
+            DotL_clear(&g_dl);
+            for (int q = 0; q < ns; q++) if (act[q]) { dot_add(S[q]->P.m, S[q]->pr, S[q]->pz, SC(q, 0)); dot_add(S[q]->P.m, S[q]->pq, S[q]->pz, SC(q, 1)); }
+            if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) { free(was); free(al); free(be); free(ga); goto out; }
+            any = 0;
+            for (int q = 0; q < ns; q++) {
+                if (!act[q]) continue;
+                const double g = HS(q, 0), d = HS(q, 1);
+                if (!(d > 0)) { act[q] = 0; continue; }
+                ga[q] = g; al[q] = g / d; be[q] = 0;
+                any = 1;
+            }
         }
         for (int k = 0; any; k++) {
-            /* q = M p; pq = p.q */
-            if (schur_mv(S, ns, act, OFF(pp), OFF(pq), 1.0, 0.0)) { free(was); goto out; }
-            DotL_clear(&g_dl);
-            for (int q = 0; q < ns; q++) if (act[q]) dot_add(S[q]->P.m, S[q]->pp, S[q]->pq, SC(q, 0));
-            if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) { free(was); goto out; }
+            /* x += a (u + b p), r -= a (w + b s) (old p, s); then p = u + b p, s = w + b s */
             LinL_clear(&g_ll);
             for (int q = 0; q < ns; q++) {
                 if (!act[q]) continue;
-                const double pq = HS(q, 0);
-                if (!(pq > 0)) { act[q] = 0; continue; }
-                const double a = rz[q] / pq;
-                lin_add(S[q]->P.m, F_(S[q], fx), F_(S[q], fx), 1.0, S[q]->pp, a, -1, 0, -1, 0);
-                lin_add(S[q]->P.m, S[q]->pr, S[q]->pr, 1.0, S[q]->pq, -a, -1, 0, -1, 0);
+                Solve *s = S[q];
+                const double a = al[q], ab = al[q] * be[q];
+                if (k == 0) {
+                    lin_add(s->P.m, F_(s, fx), F_(s, fx), 1.0, s->pz, a, -1, 0, -1, 0);
+                    lin_add(s->P.m, s->pr, s->pr, 1.0, s->pq, -a, -1, 0, -1, 0);
+                } else {
+                    lin_add(s->P.m, F_(s, fx), F_(s, fx), 1.0, s->pz, a, s->pp, ab, -1, 0);
+                    lin_add(s->P.m, s->pr, s->pr, 1.0, s->pq, -a, s->ps, -ab, -1, 0);
+                }
                 used[q]++;
-                S[q]->pcg_total++;
+                s->pcg_total++;
             }
-            if (bk_lincomb(g_ll.v, g_ll.n)) { free(was); goto out; }
-            DotL_clear(&g_dl);
-            for (int q = 0; q < ns; q++) if (act[q]) dot_add(S[q]->P.m, S[q]->pr, S[q]->pr, SC(q, 0));
-            if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) { free(was); goto out; }
+            if (bk_lincomb(g_ll.v, g_ll.n)) { free(was); free(al); free(be); free(ga); goto out; }
             LinL_clear(&g_ll);
-            int *cont = calloc(ns, sizeof(int));
+            for (int q = 0; q < ns; q++) {
+                if (!act[q]) continue;
+                Solve *s = S[q];
+                if (k == 0) {
+                    lin_add(s->P.m, s->pp, s->pz, 1.0, -1, 0, -1, 0, -1, 0);
+                    lin_add(s->P.m, s->ps, s->pq, 1.0, -1, 0, -1, 0, -1, 0);
+                } else {
+                    lin_add(s->P.m, s->pp, s->pz, 1.0, s->pp, be[q], -1, 0, -1, 0);
+                    lin_add(s->P.m, s->ps, s->pq, 1.0, s->ps, be[q], -1, 0, -1, 0);
+                }
+            }
+            if (bk_lincomb(g_ll.v, g_ll.n)) { free(was); free(al); free(be); free(ga); goto out; }
+            /* u = prec(r), w = M u, and the one reduction: r.r, r.u, w.u */
+            if (schur_prec(S, ns, act, OFF(pr), OFF(pz)) || schur_mv(S, ns, act, OFF(pz), OFF(pq), 1.0, 0.0)) { free(was); free(al); free(be); free(ga); goto out; }
+            DotL_clear(&g_dl);
+            for (int q = 0; q < ns; q++) if (act[q]) {
+                dot_add(S[q]->P.m, S[q]->pr, S[q]->pr, SC(q, 0));
+                dot_add(S[q]->P.m, S[q]->pr, S[q]->pz, SC(q, 1));
+                dot_add(S[q]->P.m, S[q]->pq, S[q]->pz, SC(q, 2));
+            }
+            if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) { free(was); free(al); free(be); free(ga); goto out; }
+            LinL_clear(&g_ll);
+            any = 0;
             for (int q = 0; q < ns; q++) {
                 if (!act[q]) continue;
                 const double nr = sqrt(HS(q, 0));
@@ -940,28 +977,16 @@ static int schur_solve_(Solve **S, int ns, const int *act0, size_t frhs, size_t 
                     best[q] = nr;
                     lin_add(S[q]->P.m, S[q]->pxb, F_(S[q], fx), 1.0, -1, 0, -1, 0, -1, 0);
                 } else stall[q]++;
-                if (nr <= tol[q] || stall[q] >= smax[q] || used[q] >= maxit[q]) act[q] = 0;
-                else cont[q] = 1;
+                if (nr <= tol[q] || stall[q] >= smax[q] || used[q] >= maxit[q]) { act[q] = 0; continue; }
+                const double gn = HS(q, 1), d = HS(q, 2);
+                const double b2 = gn / ga[q], den = d - b2 * gn / al[q];
+                if (!(den > 0) || !isfinite(b2)) { act[q] = 0; continue; }
+                be[q] = b2; al[q] = gn / den; ga[q] = gn;
+                any = 1;
             }
-            if (bk_lincomb(g_ll.v, g_ll.n)) { free(cont); free(was); goto out; }
-            any = 0;
-            for (int q = 0; q < ns; q++) any |= cont[q];
-            if (any) {
-                if (schur_prec(S, ns, cont, OFF(pr), OFF(pz))) { free(cont); free(was); goto out; }
-                DotL_clear(&g_dl);
-                for (int q = 0; q < ns; q++) if (cont[q]) dot_add(S[q]->P.m, S[q]->pr, S[q]->pz, SC(q, 0));
-                if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) { free(cont); free(was); goto out; }
-                LinL_clear(&g_ll);
-                for (int q = 0; q < ns; q++) {
-                    if (!cont[q]) continue;
-                    const double rz2 = HS(q, 0), beta = rz2 / rz[q];
-                    rz[q] = rz2;
-                    lin_add(S[q]->P.m, S[q]->pp, S[q]->pz, 1.0, S[q]->pp, beta, -1, 0, -1, 0);
-                }
-                if (bk_lincomb(g_ll.v, g_ll.n)) { free(cont); free(was); goto out; }
-            }
-            free(cont);
+            if (bk_lincomb(g_ll.v, g_ll.n)) { free(was); free(al); free(be); free(ga); goto out; }
         }
+        free(al); free(be); free(ga);
         /* x = xb; true residual */
         any = 0;
         for (int q = 0; q < ns; q++) any |= was[q];

@@ -275,12 +275,18 @@ __global__ void k_zero_upper(const MItem *__restrict__ items, const int *__restr
     }
 }
 
+// Blocked right-looking Cholesky in panels of 256 columns, each factored left-looking in four
+// 64-column steps: the 64 columns are first brought up to date with the panel's earlier steps
+// (a GEMM with K <= 192), then the diagonal block is factored and inverted in LDS and the rows
+// below are multiplied by its inverse; the trailing matrix takes the whole panel's update as one
+// lower-triangle GEMM with K = 256 (a quarter of the launches and of the trailing traffic of
+// 64-wide steps).
 template <typename T>
 static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, T *pmin) {
     if (n_it <= 0) return 0;
     int nmax = 0;
     for (int q = 0; q < n_it; q++) nmax = it[q].n > nmax ? it[q].n : nmax;
-    // every step's panel and trailing GEMMs planned at once: one staging copy for the whole factorization
+    // every step's GEMMs planned at once: one staging copy for the whole factorization
     std::vector<char> buf;
     const size_t it_off = buf_align(buf);
     buf.insert(buf.end(), (const char *)it, (const char *)(it + n_it));
@@ -291,34 +297,58 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
     int64_t ztot = 0;
     for (int q = 0; q < n_it; q++) { ((int *)(buf.data() + zpre_off))[q] = (int)ztot; ztot += zc[q]; }
     ((int *)(buf.data() + zpre_off))[n_it] = (int)ztot;
-    std::vector<GemmPlan> pp, pt;
-    std::vector<GemmItem> gi, tr;
-    for (int k0 = 0; k0 + 64 < nmax; k0 += 64) {
-        gi.clear(); tr.clear();
-        for (int q = 0; q < n_it; q++) {
-            const int n = it[q].n;
-            if (k0 + 64 >= n) continue;
-            GemmItem g;
-            // panel, in place: A[k0+64:, k0:k0+64] := A[k0+64:, k0:k0+64] Dinv_k' (Dinv_k' upper
-            // triangular). The 64 x 64 tiles span the panel's full width, so every workgroup reads
-            // its rows completely before it writes them.
-            g.M = n - k0 - 64; g.N = 64; g.K = 64;
-            g.lda = n; g.ldb = 64; g.ldc = n;
-            g.a = it[q].off + (k0 + 64) + (int64_t)k0 * n;
-            g.b = it[q].off2 + (int64_t)(k0 / 64) * 64 * 64;
-            g.c = g.a;
-            g.guard = it[q].slot; g.tri = 16;
-            gi.push_back(g);
-            // trailing: A22 -= L21 L21' (lower triangle)
-            g.M = n - k0 - 64; g.N = n - k0 - 64; g.K = 64;
-            g.lda = n; g.ldb = n; g.ldc = n;
-            g.a = it[q].off + (k0 + 64) + (int64_t)k0 * n; g.b = g.a;
-            g.c = it[q].off + (k0 + 64) + (int64_t)(k0 + 64) * n;
-            g.guard = it[q].slot; g.tri = 1;
-            tr.push_back(g);
+    struct Step { int k0; GemmPlan upd, panel, trail; int has_trail; };
+    std::vector<Step> st;
+    std::vector<GemmItem> gu, gp, gt;
+    for (int K0 = 0; K0 < nmax; K0 += 256) {
+        for (int sub = 0; sub < 4; sub++) {
+            const int k0 = K0 + 64 * sub;
+            if (k0 >= nmax) break;
+            gu.clear(); gp.clear(); gt.clear();
+            for (int q = 0; q < n_it; q++) {
+                const int n = it[q].n;
+                if (k0 >= n) continue;
+                GemmItem g;
+                if (sub > 0) {
+                    // columns k0 .. k0+64 (rows k0 ..) -= L[k0.., K0..k0] L[k0..k0+64, K0..k0]'
+                    g.M = n - k0; g.N = min(64, n - k0); g.K = k0 - K0;
+                    g.lda = n; g.ldb = n; g.ldc = n;
+                    g.a = it[q].off + k0 + (int64_t)K0 * n; g.b = g.a;
+                    g.c = it[q].off + k0 + (int64_t)k0 * n;
+                    g.guard = it[q].slot; g.tri = 0;
+                    gu.push_back(g);
+                }
+                if (k0 + 64 < n) {
+                    // panel, in place: A[k0+64:, k0:k0+64] := A[k0+64:, k0:k0+64] Dinv_k' (Dinv_k'
+                    // upper triangular). The 64 x 64 tiles span the panel's full width, so every
+                    // workgroup reads its rows completely before it writes them.
+                    g.M = n - k0 - 64; g.N = 64; g.K = 64;
+                    g.lda = n; g.ldb = 64; g.ldc = n;
+                    g.a = it[q].off + (k0 + 64) + (int64_t)k0 * n;
+                    g.b = it[q].off2 + (int64_t)(k0 / 64) * 64 * 64;
+                    g.c = g.a;
+                    g.guard = it[q].slot; g.tri = 16;
+                    gp.push_back(g);
+                }
+                const int e = K0 + 256;
+                if (sub == 3 && e < n) {
+                    // trailing: A[e:, e:] -= L[e:, K0:e] L[e:, K0:e]' (lower triangle)
+                    g.M = n - e; g.N = n - e; g.K = 256;
+                    g.lda = n; g.ldb = n; g.ldc = n;
+                    g.a = it[q].off + e + (int64_t)K0 * n; g.b = g.a;
+                    g.c = it[q].off + e + (int64_t)e * n;
+                    g.guard = it[q].slot; g.tri = 1;
+                    gt.push_back(g);
+                }
+            }
+            Step S;
+            S.k0 = k0;
+            S.upd = gemm_plan<T>(buf, gu.data(), (int)gu.size());
+            S.panel = gemm_plan<T>(buf, gp.data(), (int)gp.size());
+            S.trail = gemm_plan<T>(buf, gt.data(), (int)gt.size());
+            S.has_trail = !gt.empty();
+            st.push_back(S);
         }
-        pp.push_back(gemm_plan<T>(buf, gi.data(), (int)gi.size()));
-        pt.push_back(gemm_plan<T>(buf, tr.data(), (int)tr.size()));
     }
     const char *d = (const char *)bk_stage(buf.data(), buf.size());
     if (!d) return -1;
@@ -327,14 +357,15 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
         hipLaunchKernelGGL((k_zero_upper<T>), dim3((unsigned)ztot), dim3(256), 0, g_bk_stream, d_it, (const int *)(d + zpre_off), n_it, A);
         if (BK_LAUNCH_CHECK("k_zero_upper")) return -1;
     }
-    for (int k0 = 0, s = 0; k0 < nmax; k0 += 64, s++) {
+    for (size_t u = 0; u < st.size(); u++) {
+        const int k0 = st[u].k0;
+        if (gemm_exec<T>(d, st[u].upd, 0, 1, (T)-1, (T)1, A, A, A, info)) return -1;
         const int tok = bk_prof_begin();
         hipLaunchKernelGGL((k_potrf_diag<T>), dim3(n_it), dim3(256), 0, g_bk_stream, d_it, n_it, k0, A, Dinv, info, pmin);
         if (tok >= 0) { double fl = 0; for (int q = 0; q < n_it; q++) { const double b = it[q].n > k0 ? fmin(64.0, it[q].n - k0) : 0; fl += b * b * b / 3.0 + b * b * b / 3.0; } bk_prof_end(tok, 8, fl, 0); }
         if (BK_LAUNCH_CHECK("k_potrf_diag")) return -1;
-        if (k0 + 64 >= nmax) break;
-        if (gemm_exec<T>(d, pp[s], 0, 1, (T)1, (T)0, A, Dinv, A, info)) return -1;
-        if (gemm_exec<T>(d, pt[s], 0, 1, (T)-1, (T)1, A, A, A, info)) return -1;
+        if (gemm_exec<T>(d, st[u].panel, 0, 1, (T)1, (T)0, A, Dinv, A, info)) return -1;
+        if (st[u].has_trail && gemm_exec<T>(d, st[u].trail, 0, 1, (T)-1, (T)1, A, A, A, info)) return -1;
     }
     return 0;
 }

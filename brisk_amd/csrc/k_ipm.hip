@@ -62,16 +62,21 @@ __global__ __launch_bounds__(256) void k_dot_chunks(const DotItem *__restrict__ 
     }
     if (t == 0) part[cbase[q] + c] = sh[0];
 }
+// results in [moff, moff + mlen) go straight to the host mirror (pinned, device-visible)
 __global__ void k_dot_final(const DotItem *__restrict__ it, const int64_t *__restrict__ cbase, int nit, double *__restrict__ H,
-                            const double *__restrict__ part) {
+                            const double *__restrict__ part, int64_t moff, int64_t mlen, double *__restrict__ mirror) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nit) return;
     const DotItem d = it[q];
     const int64_t nch = (d.len + DCH - 1) / DCH;
     double s = 0;
     for (int64_t c = 0; c < nch; c++) s += part[cbase[q] + c];
-    H[d.out] = s;
+    if (mirror && d.out >= moff && d.out < moff + mlen) mirror[d.out - moff] = s;
+    else H[d.out] = s;
 }
+static int64_t g_mirror_off = 0, g_mirror_len = 0;
+static double *g_mirror = nullptr;
+extern "C" void bk_dot_mirror(int64_t off, int64_t len, double *host) { g_mirror_off = off; g_mirror_len = len; g_mirror = host; }
 extern "C" int bk_dots(const DotItem *it, int n_it) {
     if (n_it <= 0) return 0;
     std::vector<int64_t> cb(n_it), cnt(n_it);
@@ -102,7 +107,8 @@ extern "C" int bk_dots(const DotItem *it, int n_it) {
     if (nb > 0) hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, dcb, pre, n_it, (const double *)g_heap, g_partial);
     if (tok >= 0) { double by = 0; for (int q = 0; q < n_it; q++) by += 16.0 * it[q].len; bk_prof_end(tok, 13, 0, by); }
     if (BK_LAUNCH_CHECK("k_dot_chunks")) return -1;
-    hipLaunchKernelGGL(k_dot_final, dim3((n_it + 255) / 256), dim3(256), 0, g_bk_stream, d, dcb, n_it, (double *)g_heap, g_partial);
+    hipLaunchKernelGGL(k_dot_final, dim3((n_it + 255) / 256), dim3(256), 0, g_bk_stream, d, dcb, n_it, (double *)g_heap, g_partial,
+                       g_mirror_off, g_mirror_len, g_mirror);
     return BK_LAUNCH_CHECK("k_dot_final");
 }
 

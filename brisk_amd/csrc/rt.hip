@@ -11,10 +11,21 @@ static int g_init = 0;
 static int g_failed = 0;
 static char g_err[256] = "";
 hipStream_t g_bk_stream = nullptr;
+static hipEvent_t g_sync_ev;
 
 // staging ring: host pinned + device, recycled at every synchronisation
 static char *g_ring_h = nullptr, *g_ring_d = nullptr;
 static size_t g_ring_cap = 0, g_ring_pos = 0;
+#define CACHE_CAP ((size_t)32 << 20)
+#define CACHE_CHUNK ((size_t)64 << 10)
+#define CACHE_TAB 16384
+typedef struct { uint64_t hash; uint32_t bytes, off; uint64_t wrap; } CacheEnt;
+static char *g_cache_h = nullptr, *g_cache_d = nullptr;
+static size_t g_cache_pos = 0;
+static uint64_t g_cache_wrap = 1, g_epoch = 1;
+static CacheEnt g_cache_tab[CACHE_TAB];
+static uint64_t g_chunk_use[CACHE_CAP / CACHE_CHUNK];
+static double g_cache_hits = 0, g_cache_miss = 0;
 
 int bk_check(hipError_t e, const char *what) {
     if (e == hipSuccess) return 0;
@@ -29,12 +40,19 @@ extern "C" int bk_init(void) {
     int n = 0;
     if (bk_check(hipGetDeviceCount(&n), "hipGetDeviceCount")) return -1;
     if (n < 1) { snprintf(g_err, sizeof(g_err), "no GPU"); g_failed = 1; return -1; }
+    // the host waits on the stream many times per iteration: spin rather than sleep (a
+    // request that fails because a context already exists is ignored)
+    (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
+    (void)hipGetLastError();
     if (bk_check(hipSetDevice(0), "hipSetDevice")) return -1;
     if (bk_check(hipStreamCreateWithFlags(&g_bk_stream, hipStreamNonBlocking), "hipStreamCreate")) return -1;
+    if (bk_check(hipEventCreateWithFlags(&g_sync_ev, hipEventDisableTiming), "hipEventCreate")) return -1;
     g_ring_cap = (size_t)64 << 20;
     if (bk_check(hipHostMalloc((void **)&g_ring_h, g_ring_cap, hipHostMallocDefault), "hipHostMalloc ring")) return -1;
     if (bk_check(hipMalloc((void **)&g_ring_d, g_ring_cap), "hipMalloc ring")) return -1;
     g_ring_pos = 0;
+    if (bk_check(hipHostMalloc((void **)&g_cache_h, CACHE_CAP, hipHostMallocDefault), "hipHostMalloc cache")) return -1;
+    if (bk_check(hipMalloc((void **)&g_cache_d, CACHE_CAP), "hipMalloc cache")) return -1;
     g_init = 1;
     return 0;
 }
@@ -49,6 +67,13 @@ extern "C" void *bk_malloc(size_t bytes) {
     return p;
 }
 extern "C" void bk_free(void *p) { if (p) (void)hipFree(p); }
+extern "C" void *bk_host_alloc(size_t bytes) {
+    void *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    if (bk_check(hipHostMalloc(&p, bytes, hipHostMallocDefault), "hipHostMalloc")) return nullptr;
+    return p;
+}
+extern "C" void bk_host_free(void *p) { if (p) (void)hipHostFree(p); }
 
 static int g_prof_on_host = 0;
 static inline int g_prof_host_on(void) { return g_prof_on_host; }
@@ -56,9 +81,17 @@ static inline int g_prof_host_on(void) { return g_prof_on_host; }
 static double g_sync_ms = 0, g_sync_n = 0, g_stage_n = 0, g_stage_bytes = 0;
 extern "C" int bk_sync(void) {
     const double t0 = g_prof_host_on() ? bk_now() : 0;
-    int r = bk_check(hipStreamSynchronize(g_bk_stream), "hipStreamSynchronize");
+    // an event after the queued work, polled without sleeping: the host resumes within
+    // microseconds of the GPU instead of after a wake-up
+    int r = bk_check(hipEventRecord(g_sync_ev, g_bk_stream), "hipEventRecord");
+    if (!r) {
+        hipError_t e;
+        while ((e = hipEventQuery(g_sync_ev)) == hipErrorNotReady) { }
+        r = bk_check(e, "hipEventQuery");
+    }
     if (t0 > 0) { g_sync_ms += 1e3 * (bk_now() - t0); g_sync_n += 1; }
     g_ring_pos = 0;
+    g_epoch++;
     return r;
 }
 extern "C" int bk_h2d(void *dst, const void *src, size_t bytes) {
@@ -95,7 +128,72 @@ extern "C" int bk_zero(void *dst, size_t bytes) {
     return bk_check(hipGetLastError(), "k_zero");
 }
 
+// ---- staged item arrays: a content-addressed cache in device memory -------------------------
+// Most item arrays repeat exactly from one launch to the next (the same solves, the same
+// operands); a repeated array is found by its hash and reused without a copy. The pool is a ring
+// of 64 KB chunks; a chunk used since the last synchronisation is never overwritten (the ring
+// synchronises first), so a queued kernel never sees its items change.
+static uint64_t stage_hash(const void *p, size_t n) {
+    const unsigned char *b = (const unsigned char *)p;
+    uint64_t h = 0x9e3779b97f4a7c15ull ^ (uint64_t)n;
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        memcpy(&w, b + i, 8);
+        h = (h ^ w) * 0xff51afd7ed558ccdull;
+        h ^= h >> 32;
+    }
+    for (; i < n; i++) h = (h ^ b[i]) * 0x100000001b3ull;
+    return h ^ (h >> 29);
+}
+// an entry is live while the ring has not passed over its bytes since it was written
+static int cache_live(const CacheEnt &e) {
+    if (!e.bytes) return 0;
+    if (e.wrap == g_cache_wrap) return 1;
+    return e.wrap + 1 == g_cache_wrap && e.off >= g_cache_pos;
+}
+static void *stage_cached(const void *host, size_t bytes) {
+    const uint64_t h = stage_hash(host, bytes);
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    size_t slot = h % CACHE_TAB;
+    for (int probe = 0; probe < 8; probe++, slot = (slot + 1) % CACHE_TAB) {
+        CacheEnt &e = g_cache_tab[slot];
+        if (!cache_live(e)) continue;
+        if (e.hash == h && e.bytes == bytes && !memcmp(g_cache_h + e.off, host, bytes)) {
+            // data of the previous lap, ahead of the ring: the ring must not pass over it before a synchronisation
+            if (e.wrap != g_cache_wrap)
+                for (size_t c = e.off / CACHE_CHUNK; c <= (e.off + need - 1) / CACHE_CHUNK; c++) g_chunk_use[c] = g_epoch;
+            g_cache_hits += 1;
+            return g_cache_d + e.off;
+        }
+    }
+    // miss: the next bytes of the ring (synchronise first if they were used in this epoch)
+    // a new lap starts after a synchronisation (the last lap's arrays may be in use); within a
+    // lap, data of the previous lap read in this epoch is not overwritten before one
+    if (g_cache_pos + need > CACHE_CAP) { bk_sync(); g_cache_pos = 0; g_cache_wrap++; }
+    for (size_t c = g_cache_pos / CACHE_CHUNK; c <= (g_cache_pos + need - 1) / CACHE_CHUNK; c++)
+        if (g_chunk_use[c] == g_epoch) { bk_sync(); break; }
+    const size_t off = g_cache_pos;
+    char *hp = g_cache_h + off, *dp = g_cache_d + off;
+    memcpy(hp, host, bytes);
+    if (bk_check(hipMemcpyAsync(dp, hp, bytes, hipMemcpyHostToDevice, g_bk_stream), "stage")) return nullptr;
+    g_cache_pos += need;
+    // insert (replacing a dead entry in the probe window, else the first)
+    slot = h % CACHE_TAB;
+    size_t put = slot;
+    for (int probe = 0; probe < 8; probe++) {
+        const size_t s2 = (slot + probe) % CACHE_TAB;
+        if (!cache_live(g_cache_tab[s2])) { put = s2; break; }
+    }
+    CacheEnt ne = { h, (uint32_t)bytes, (uint32_t)off, g_cache_wrap };
+    g_cache_tab[put] = ne;
+    g_cache_miss += 1;
+    g_stage_n += 1; g_stage_bytes += (double)bytes;
+    return dp;
+}
+
 extern "C" void *bk_stage(const void *host, size_t bytes) {
+    if (g_cache_d && bytes > 0 && bytes <= CACHE_CHUNK) return stage_cached(host, bytes);
     size_t need = (bytes + 255) & ~(size_t)255;
     if (need == 0) need = 256;
     if (need > g_ring_cap) { snprintf(g_err, sizeof(g_err), "staging request of %zu bytes too large", bytes); g_failed = 1; return nullptr; }
@@ -176,7 +274,7 @@ extern "C" void bk_prof_enable(int on) {
 }
 extern "C" void bk_prof_reset(void) {
     prof_flush();
-    g_sync_ms = g_sync_n = g_stage_n = g_stage_bytes = 0;
+    g_sync_ms = g_sync_n = g_stage_n = g_stage_bytes = 0; g_cache_hits = g_cache_miss = 0;
     memset(g_acc_ms, 0, sizeof(g_acc_ms)); memset(g_acc_n, 0, sizeof(g_acc_n));
     memset(g_acc_flops, 0, sizeof(g_acc_flops)); memset(g_acc_bytes, 0, sizeof(g_acc_bytes));
 }
@@ -200,7 +298,7 @@ extern "C" void bk_prof_host(int cls, double seconds) {
 extern "C" void bk_prof_read(int cls, double *out4) {
     prof_flush();
     if (cls == 62) { out4[0] = g_sync_ms; out4[1] = g_sync_n; out4[2] = 0; out4[3] = 0; return; }
-    if (cls == 63) { out4[0] = 0; out4[1] = g_stage_n; out4[2] = 0; out4[3] = g_stage_bytes; return; }
+    if (cls == 63) { out4[0] = 0; out4[1] = g_stage_n; out4[2] = g_cache_hits; out4[3] = g_stage_bytes; return; }
     if (cls < 0 || cls >= PROF_CLS) { out4[0] = out4[1] = out4[2] = out4[3] = 0; return; }
     out4[0] = g_acc_ms[cls]; out4[1] = g_acc_n[cls]; out4[2] = g_acc_flops[cls]; out4[3] = g_acc_bytes[cls];
 }

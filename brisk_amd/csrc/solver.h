@@ -68,7 +68,7 @@ typedef struct Solve {
     int64_t X, Z, Zi, LX, LZ, LXi, LZi, LXiT, LZiT, dX, dZ, G, Q, R0, F, W1, W2, C, DinvX, DinvZ, Xb;
     int64_t lzQ, lzw, lzt1, lzt2;
     int64_t M, MF, MLi, MLiT, MDinv, Mwork, Dm;
-    int64_t y, dy, rp, rhs, a0, h, qv, ev, wv, bvec, duv, pr, pz, pp, pq, pxb, t1, t2, yb;
+    int64_t y, dy, rp, rhs, a0, h, qv, ev, wv, bvec, duv, pr, pz, pp, pq, ps, pxb, t1, t2, yb;
     int64_t sc;             /* 64 scalars */
     int64_t cptr, ent, blk, pptr, ppos, pcon, pval, dinvoff; /* problem data on the device */
     int64_t gptr, grp, Pg, Tg;      /* constraint groups (bytes), heavy-group products P and T (doubles) */

@@ -20,6 +20,8 @@ const char *bk_error(void);                      /* last HIP error text ("" if n
 int bk_failed(void);                             /* a HIP call has failed since bk_init */
 void *bk_malloc(size_t bytes);                   /* device memory (NULL on failure) */
 void bk_free(void *p);
+void *bk_host_alloc(size_t bytes);               /* pinned host memory (direct DMA copies) */
+void bk_host_free(void *p);
 int bk_h2d(void *dst, const void *src, size_t bytes);        /* synchronous copy */
 int bk_h2d_async(void *dst, const void *src, size_t bytes);  /* copy ordered on the stream; src may be reused at once */
 int bk_d2h(void *dst, const void *src, size_t bytes);        /* synchronous (drains the stream) */
@@ -103,6 +105,9 @@ size_t bk_heap_size(void);
  * the data and len only. All offsets are elements of double from the heap base. */
 typedef struct { int64_t len, xo, yo, out; } DotItem;
 int bk_dots(const DotItem *it, int n_it);
+/* dot results whose output offset lies in [off, off + len) (doubles) are written to host[out - off]
+ * (pinned, device-visible memory) instead of the heap: read them after bk_sync, no copy */
+void bk_dot_mirror(int64_t off, int64_t len, double *host);
 
 /* out[q] = c0 in0[q] + c1 in1[q] + c2 in2[q] + c3 in3[q] for q < len (an offset < 0 means the
  * term is absent; out may alias any input) */
