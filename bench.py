@@ -60,6 +60,26 @@ def c5_stream(count=C5["count"], seed=C5["seed"]):
     return reqs, gaps
 
 
+# measured HBM bytes per dispatch by kernel class (PMC FETCH_SIZE + WRITE_SIZE, copy-calibrated),
+# kept with the rocprofv3 summaries; keyed by workload
+TRAFFIC_FILES = {"c2": "maxcut2000_mixed_hbm_traffic_pmc.md"}
+
+
+def measured_traffic(workload, kernel):
+    fn = TRAFFIC_FILES.get(workload)
+    if not fn:
+        return None
+    try:
+        for line in open(os.path.join(ROOT, "profiles", fn)):
+            if line.startswith("traffic_bytes_per_launch "):
+                k, v = line[len("traffic_bytes_per_launch "):].rsplit("=", 1)
+                if k.strip() == kernel:
+                    return float(v)
+    except OSError:
+        return None
+    return None
+
+
 def instance(name):
     from _ctypes_util import Instance
     w = WORKLOADS[name]
@@ -238,10 +258,14 @@ def serving(args):
         except subprocess.TimeoutExpired:
             ref = dict(error="timeout")
         if "error" not in ref:
-            cpu = dict(value=round(ref["solves_per_s"], 3), unit="solves/s", cores=workers, kind="reference",
+            # the rate of perfectly packed workers (solves x workers / CPU seconds): an upper bound of
+            # the pool's throughput on the whole stream, above the sample's own wall-clock rate
+            packed = ref["count"] * workers / max(ref["cpu_s"], 1e-9)
+            cpu = dict(value=round(packed, 3), unit="solves/s", cores=workers, kind="reference",
                        sample="every %dth request of the stream (%d solves, %d optimal) by the reference library, one thread per "
-                              "solve in %d worker processes, %.1f s wall, %s" % (args.cpu_stride, ref["count"], ref["optimal"], workers,
-                                                                                 ref["wall"], cpu_model()))
+                              "solve in %d worker processes: %.1f CPU s, %.1f s wall (%.2f solves/s measured; value = solves x "
+                              "workers / CPU s), %s" % (args.cpu_stride, ref["count"], ref["optimal"], workers, ref["cpu_s"],
+                                                       ref["wall"], ref["solves_per_s"], cpu_model()))
         else:
             cpu = dict(value=None, unit="solves/s", cores=workers, kind="reference", sample="not run: %s" % ref["error"])
     import brisk_amd
@@ -351,11 +375,12 @@ def main():
         prec = "f64" if dom < 4 or dom in (8,) else "f32"
         ach = d["flops"] / max(d["launches"], 1) / avg_s / 1e12
         roof = dict(bound="mfma", achieved=round(ach, 3), peak=PEAK[prec], unit="TFLOP/s", frac=round(ach / PEAK[prec], 4),
-                    traffic=None, kernel=KCLASS[dom], launches=int(d["launches"]), avg_us=round(avg_s * 1e6, 2),
+                    traffic=measured_traffic(name, KCLASS[dom]), kernel=KCLASS[dom], launches=int(d["launches"]), avg_us=round(avg_s * 1e6, 2),
                     flops_per_launch=d["flops"] / max(d["launches"], 1), share_of_kernel_time=round(d["ms"] / max(tot, 1e-9), 3))
     else:
         ach = d["bytes"] / max(d["launches"], 1) / avg_s / 1e9
-        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK, unit="GB/s", frac=round(ach / HBM_PEAK, 4), traffic=None,
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK, unit="GB/s", frac=round(ach / HBM_PEAK, 4),
+                    traffic=measured_traffic(name, KCLASS[dom]),
                     kernel=KCLASS[dom], launches=int(d["launches"]), avg_us=round(avg_s * 1e6, 2),
                     bytes_per_launch=d["bytes"] / max(d["launches"], 1), share_of_kernel_time=round(d["ms"] / max(tot, 1e-9), 3))
     cpu = None
