@@ -497,7 +497,9 @@ static void dimacs(const OProb *P, const double *Xo, const double *yo, double *e
 }
 
 /* ------------------------------------------------------------------------------------- */
+static int oracle_solve_hsd(const OracleData *Dd, const OracleOptions *opt, OracleResult *res);
 int oracle_solve(const OracleData *Dd, const OracleOptions *opt, OracleResult *res) {
+    if (opt && opt->method == 1) return oracle_solve_hsd(Dd, opt, res);
     memset(res, 0, sizeof(*res));
     res->status = -1;
     OProb P;
@@ -765,6 +767,297 @@ int oracle_schur(const OracleData *Dd, const double *X, const double *Zi, double
     int rc = prob_build(Dd, &P);
     if (rc) return rc;
     schur(&P, X, Zi, M);
+    prob_free(&P);
+    return 0;
+}
+
+/* ---- the homogeneous self-dual embedding (reference hsd_run, solver.c:9803-12211) ----------
+ * Variables X, y, Z, tau, kappa of the scaled problem with residuals
+ *   rp = tau b - A(X),  Rd = tau C - A'(y) - Z,  rg = kappa - b'y + <C, X>,
+ *   mu = (<X, Z> + tau kappa) / (nu + 1)                                   (solver.c:10357)
+ * and the Newton system, every residual reduced by eta = 1 - sigma:
+ *   A(dX) - b dtau = eta rp;  A'(dy) + dZ - C dtau = eta Rd;  b'dy - <C, dX> - dkappa = eta rg;
+ *   dX = sigma mu Zi - X - sym(Zi dZ X) (+ corrector);  kappa dtau + tau dkappa = sigma mu - tau kappa
+ * which gives dy = u + dtau v with M u = w1 and M v = w2 = b + A(X C Zi) (two right-hand sides of
+ * the same Schur complement M), dtau from the gap row, dkappa = (sigma mu - tau kappa - kappa
+ * dtau) / tau (solver.c:11127), one step length for all variables, and tau normalised back to 1
+ * when it grows past 10 (solver.c:12054-12069). Mehrotra predictor-corrector with the standard
+ * loop's sigma and gamma rules; the solution is (X, y, Z) / tau. */
+static int oracle_solve_hsd(const OracleData *Dd, const OracleOptions *opt, OracleResult *res) {
+    memset(res, 0, sizeof(*res));
+    res->status = -1;
+    OProb P;
+    int rc = prob_build(Dd, &P);
+    if (rc) return rc;
+    const int m = P.m;
+    const size_t T = P.tot;
+    const double tol = opt && opt->tol > 0 ? opt->tol : 1e-8;
+    const int maxit = opt && opt->maxit > 0 ? opt->maxit : 100;
+    int maxn = 1;
+    for (int k = 0; k < P.nb; k++) if (P.B[k].n > maxn) maxn = P.B[k].n;
+    double *X = calloc(T + 1, sizeof(double)), *Z = calloc(T + 1, sizeof(double)), *Zi = calloc(T + 1, sizeof(double));
+    double *LX = calloc(T + 1, sizeof(double)), *LZ = calloc(T + 1, sizeof(double)), *dX = calloc(T + 1, sizeof(double));
+    double *dZ = calloc(T + 1, sizeof(double)), *F = calloc(T + 1, sizeof(double)), *G = calloc(T + 1, sizeof(double));
+    double *Q = calloc(T + 1, sizeof(double)), *Rd = calloc(T + 1, sizeof(double)), *Xb = calloc(T + 1, sizeof(double));
+    double *CX = calloc(T + 1, sizeof(double)), *dXa = calloc(T + 1, sizeof(double)), *dZa = calloc(T + 1, sizeof(double));
+    double *W = malloc(sizeof(double) * (size_t)maxn * maxn + 8), *work = malloc(sizeof(double) * (size_t)maxn * maxn + 8);
+    double *y = calloc(m + 1, sizeof(double)), *dy = calloc(m + 1, sizeof(double)), *rp = calloc(m + 1, sizeof(double));
+    double *w1 = calloc(m + 1, sizeof(double)), *w2 = calloc(m + 1, sizeof(double)), *u = calloc(m + 1, sizeof(double));
+    double *v = calloc(m + 1, sizeof(double)), *gv = calloc(m + 1, sizeof(double)), *a0 = calloc(m + 1, sizeof(double));
+    double *t1 = calloc(m + 1, sizeof(double)), *yb = calloc(m + 1, sizeof(double)), *aq = calloc(m + 1, sizeof(double));
+    double *ev0 = calloc(m + 1, sizeof(double)), *ev1 = calloc(m + 1, sizeof(double));
+    double *dX0 = calloc(T + 1, sizeof(double)), *dX1 = calloc(T + 1, sizeof(double)), *dZ0 = calloc(T + 1, sizeof(double)), *dZ1 = calloc(T + 1, sizeof(double));
+    OSchur S = { m, calloc((size_t)m * m + 1, sizeof(double)), calloc((size_t)m * m + 1, sizeof(double)), calloc(m + 1, sizeof(double)), 0, 0, 0 };
+    int status = 3, it = 0, it_best = 0, nfail = 0, small = 0;
+    double best_score = 1e300;
+    /* start: the standard loop's point (least-norm X0, least-squares y0, shifts into the cone),
+     * tau = 1 and kappa = <X0, Z0> / nu so that tau kappa starts at mu */
+    add_identity(&P, X, 1.0);
+    memcpy(Zi, X, sizeof(double) * T);
+    Aop(&P, P.C, w1, 1.0);
+    schur(&P, X, Zi, S.M);
+    if (sc_factor(&S, &nfail)) { status = 4; goto done; }
+    sc_solve(&S, P.b, dy, 1e-8);
+    sc_solve(&S, w1, y, 1e-8);
+    memset(X, 0, sizeof(double) * T);
+    ATy(&P, dy, X);
+    ATy(&P, y, F);
+    for (size_t q = 0; q < T; q++) Z[q] = P.C[q] - F[q];
+    {
+        const double ax = -lammin_all(&P, X, work), az = -lammin_all(&P, Z, work);
+        add_identity(&P, X, ax >= -1e-8 ? 1.0 + ax : 0.0);
+        add_identity(&P, Z, fmax(0.0, 1.0 + az));
+    }
+    for (int k = 0; k < P.nb; k++) {
+        const OBlk *B = &P.B[k];
+        const size_t len = B->lp ? (size_t)B->n : (size_t)B->n * B->n;
+        if (B->lp) { memcpy(LX + B->off, X + B->off, sizeof(double) * len); memcpy(LZ + B->off, Z + B->off, sizeof(double) * len); continue; }
+        for (double extra = 1e-8; ; extra *= 10) {
+            memcpy(LX + B->off, X + B->off, sizeof(double) * len);
+            if (chol(B->n, LX + B->off) == 0) break;
+            for (int i = 0; i < B->n; i++) X[B->off + i + (size_t)i * B->n] += extra;
+        }
+        for (double extra = 1e-8; ; extra *= 10) {
+            memcpy(LZ + B->off, Z + B->off, sizeof(double) * len);
+            if (chol(B->n, LZ + B->off) == 0) break;
+            for (int i = 0; i < B->n; i++) Z[B->off + i + (size_t)i * B->n] += extra;
+        }
+    }
+    double tau = 1.0, kap = fmax(dot(T, X, Z) / P.Ntot, 1e-12);
+    const double nu1 = P.Ntot + 1.0;
+    int anysdp = 0;
+    for (int k = 0; k < P.nb; k++) if (!P.B[k].lp) anysdp = 1;
+    memcpy(yb, y, sizeof(double) * m); memcpy(Xb, X, sizeof(double) * T);
+
+    for (it = 0; it <= maxit; it++) {
+        for (int k = 0; k < P.nb; k++) {
+            const OBlk *B = &P.B[k];
+            if (B->lp) { for (int i = 0; i < B->n; i++) Zi[B->off + i] = 1.0 / Z[B->off + i]; }
+            else inv_from_chol(B->n, LZ + B->off, Zi + B->off);
+        }
+        /* residuals */
+        for (int i = 0; i < m; i++) rp[i] = tau * P.b[i];
+        Aop(&P, X, rp, -1.0);
+        ATy(&P, y, F);
+        double xz = 0, cx = 0, rdn2 = 0;
+        for (size_t q = 0; q < T; q++) {
+            Rd[q] = tau * P.C[q] - F[q] - Z[q];
+            xz += X[q] * Z[q]; cx += P.C[q] * X[q]; rdn2 += Rd[q] * Rd[q];
+        }
+        const double by = dot(m, P.b, y);
+        const double rg = kap - by + cx;
+        const double mu = (xz + tau * kap) / nu1;
+        /* the normalised iterate's measures (as the standard loop's, solver.c:14426-14482) */
+        double rpo = 0;
+        for (int i = 0; i < m; i++) { const double t = rp[i] / tau * P.du[i]; rpo += t * t; }
+        rpo = sqrt(rpo);
+        const double sc = P.bs * P.cs, po = sc * cx / tau, dob = sc * by / tau;
+        const double pinf = rpo / (1 + P.normb2), dinf = P.cs * sqrt(rdn2) / tau / (1 + P.normC2);
+        const double oden = 1 + fabs(po) + fabs(dob);
+        const double relgap = fabs(po - dob) / oden, relcomp = sc * xz / (tau * tau) / oden;
+        double score = fmax(fmax(relgap, relcomp), fmax(pinf, dinf));
+        if (!isfinite(score)) score = INFINITY;
+        if (it < ORACLE_TRACE) { res->trace_pobj[it] = po; res->trace_dobj[it] = dob; res->trace_mu[it] = sc * mu; res->trace_score[it] = score; }
+        res->iters = it;
+        if (!isfinite(po) || !isfinite(dob)) { status = 4; break; }
+        if (score < best_score) {
+            best_score = score; it_best = it;
+            for (int i = 0; i < m; i++) yb[i] = y[i] / tau;
+            for (size_t q = 0; q < T; q++) Xb[q] = X[q] / tau;
+        }
+        if (score < tol) { status = 0; break; }
+        if (it == maxit) { status = 3; break; }
+        if (small >= 3) { status = 4; break; }
+        if (it - it_best >= 5 && best_score < 1e-6) { status = 4; break; }
+        if (it - it_best >= 12) { status = 4; break; }
+        /* the Schur complement at (X, Zi); v = M^-1 (b + A(X C Zi)) serves both directions */
+        schur(&P, X, Zi, S.M);
+        if (sc_factor(&S, &nfail)) { status = 4; break; }
+        prod3(&P, Zi, P.C, X, CX, W);                      /* Zi C X */
+        memset(gv, 0, sizeof(double) * m);
+        Aop(&P, CX, gv, 1.0);                              /* A(X C Zi) */
+        for (int i = 0; i < m; i++) w2[i] = P.b[i] + gv[i];
+        sc_solve(&S, w2, v, 0);
+        const double c2 = dot(T, P.C, CX);                 /* <C, X C Zi> */
+        prod3(&P, Zi, Rd, X, G, W);                        /* Zi Rd X */
+        const double c1 = dot(T, P.C, G);
+        memset(t1, 0, sizeof(double) * m);
+        Aop(&P, G, t1, 1.0);                               /* A(X Rd Zi) */
+        memset(a0, 0, sizeof(double) * m);
+        Aop(&P, Zi, a0, 1.0);
+        const double cz = dot(T, P.C, Zi);
+        /* the direction at (sigma, eta = 1 - sigma, second-order term K): dy = u + dtau v, dtau from
+         * the gap row, dkappa from the tau-kappa row; pass 0 is the affine direction (sigma = 0,
+         * no K); the corrector family is affine in sigma (solver.c:11356-11375), evaluated at
+         * sigma = 0 and 1 with K from the affine direction, and sigma is chosen among the Mehrotra
+         * value sigma_m, 0.2 sigma_m and 3 sigma_m by the largest predicted reduction alpha (1 - sigma)
+         * of mu (solver.c:11195-11228, 11476-11484) */
+        double sigma_m = 0, dta = 0, dka = 0, dtv[2] = { 0, 0 }, dkv[2] = { 0, 0 };
+        for (int pass = 0; pass < 3; pass++) {
+            const double sg = pass == 2 ? 1.0 : 0.0, eta = 1.0 - sg, smu = sg * mu;
+            for (int i = 0; i < m; i++) w1[i] = eta * rp[i] - smu * a0[i] + (P.b[i] * tau - rp[i]) + eta * t1[i];
+            double cK = 0, ctk = 0;
+            if (pass >= 1) {
+                if (pass == 1) {
+                    prod3(&P, Zi, dZa, dXa, Q, W);         /* K = -sym(Zi dZa dXa), from the affine direction */
+                    sym_all(&P, Q);
+                    memset(gv, 0, sizeof(double) * m);
+                    Aop(&P, Q, gv, 1.0);
+                    memcpy(aq, gv, sizeof(double) * m);    /* A(Q) */
+                }
+                for (int i = 0; i < m; i++) w1[i] += aq[i];
+                cK = -dot(T, P.C, Q);
+                ctk = -dta * dka;
+            }
+            sc_solve(&S, w1, u, 0);
+            memset(gv, 0, sizeof(double) * m);
+            Aop(&P, CX, gv, 1.0);
+            const double c0 = smu * cz - cx;
+            const double den = dot(m, P.b, v) + c2 - dot(m, gv, v) + kap / tau;
+            const double num = eta * rg - dot(m, P.b, u) + c0 - eta * c1 + dot(m, gv, u) + cK + (smu - tau * kap + ctk) / tau;
+            const double dtau = num / den;
+            const double dkap = (smu - tau * kap + ctk - kap * dtau) / tau;
+            double *dyp = pass == 0 ? dy : (pass == 1 ? ev0 : ev1);
+            for (int i = 0; i < m; i++) dyp[i] = u[i] + dtau * v[i];
+            ATy(&P, dyp, F);
+            double *dZp = pass == 0 ? dZ : (pass == 1 ? dZ0 : dZ1), *dXp = pass == 0 ? dX : (pass == 1 ? dX0 : dX1);
+            for (size_t q = 0; q < T; q++) dZp[q] = eta * Rd[q] - F[q] + P.C[q] * dtau;
+            prod3(&P, Zi, dZp, X, G, W);
+            for (int k = 0; k < P.nb; k++) {
+                const OBlk *B = &P.B[k];
+                const size_t len = B->lp ? (size_t)B->n : (size_t)B->n * B->n;
+                double *dXk = dXp + B->off;
+                const double *Gk = G + B->off, *Zik = Zi + B->off, *Xk = X + B->off, *Qk = Q + B->off;
+                for (size_t q = 0; q < len; q++) dXk[q] = -Gk[q];
+                if (!B->lp) symmetrize(B->n, dXk);
+                for (size_t q = 0; q < len; q++) dXk[q] += smu * Zik[q] - Xk[q] + (pass >= 1 ? -Qk[q] : 0.0);
+            }
+            if (pass == 0) {
+                double amax = fmin(maxstep(&P, X, LX, dX, work), maxstep(&P, Z, LZ, dZ, work));
+                if (dtau < 0) amax = fmin(amax, -tau / dtau);
+                if (dkap < 0) amax = fmin(amax, -kap / dkap);
+                const double aa = fmin(1.0, amax);
+                const double xdz = dot(T, X, dZ), dxz = dot(T, dX, Z), dxdz = dot(T, dX, dZ);
+                const double mua = (xz + aa * (dxz + xdz) + aa * aa * dxdz + (tau + aa * dtau) * (kap + aa * dkap)) / nu1;
+                const double expon = relgap > 1e-6 ? fmax(1.0, 3.0 * aa * aa) : 3.0;
+                sigma_m = fmin(1.0, pow(fmax(mua / mu, 0.0), expon));
+                memcpy(dXa, dX, sizeof(double) * T); memcpy(dZa, dZ, sizeof(double) * T);
+                dta = dtau; dka = dkap;
+                (void)anysdp;
+            } else { dtv[pass - 1] = dtau; dkv[pass - 1] = dkap; }
+        }
+        /* the sigma search on d0 + sigma (d1 - d0) */
+        double best_m = -1, best_s = sigma_m, best_a = 0;
+        {
+            const double cand[3] = { sigma_m, fmin(0.95, fmax(1e-4, 0.2 * sigma_m)), fmin(0.95, fmax(1e-3, 3.0 * sigma_m)) };
+            for (int c = 0; c < 3; c++) {
+                const double sg = cand[c];
+                for (size_t q = 0; q < T; q++) { dX[q] = dX0[q] + sg * (dX1[q] - dX0[q]); dZ[q] = dZ0[q] + sg * (dZ1[q] - dZ0[q]); }
+                const double dtau = dtv[0] + sg * (dtv[1] - dtv[0]), dkap = dkv[0] + sg * (dkv[1] - dkv[0]);
+                double am = fmin(maxstep(&P, X, LX, dX, work), maxstep(&P, Z, LZ, dZ, work));
+                if (dtau < 0) am = fmin(am, -tau / dtau);
+                if (dkap < 0) am = fmin(am, -kap / dkap);
+                const double gam = 0.9 + (0.99 - 0.9) * fmin(1.0, am);
+                const double al = fmin(1.0, gam * am), merit = al * (1.0 - sg);
+                if (merit > best_m) { best_m = merit; best_s = sg; best_a = al; }
+            }
+        }
+        {
+            const double sg = best_s;
+            for (size_t q = 0; q < T; q++) { dX[q] = dX0[q] + sg * (dX1[q] - dX0[q]); dZ[q] = dZ0[q] + sg * (dZ1[q] - dZ0[q]); }
+            for (int i = 0; i < m; i++) dy[i] = ev0[i] + sg * (ev1[i] - ev0[i]);
+            const double dtau = dtv[0] + sg * (dtv[1] - dtv[0]), dkap = dkv[0] + sg * (dkv[1] - dkv[0]);
+            double a = best_a;
+            /* X and Z positive definite at the common step (x0.8 on failure, solver.c:6393-6446) */
+            double ax = a, az = a;
+            if (!try_step(&P, X, LX, dX, &ax)) { status = 4; goto fin; }
+            if (ax < a) a = ax;
+            if (!try_step(&P, Z, LZ, dZ, &az)) { status = 4; goto fin; }
+            if (az < a) {
+                const double back = az - a;
+                for (size_t q = 0; q < T; q++) X[q] += back * dX[q];
+                double one = 1.0;
+                memset(Q, 0, sizeof(double) * T);
+                if (!try_step(&P, X, LX, Q, &one)) { status = 4; goto fin; }
+                a = az;
+            } else if (a < az) {
+                const double back = a - az;
+                for (size_t q = 0; q < T; q++) Z[q] += back * dZ[q];
+                double one = 1.0;
+                memset(Q, 0, sizeof(double) * T);
+                if (!try_step(&P, Z, LZ, Q, &one)) { status = 4; goto fin; }
+            }
+            for (int i = 0; i < m; i++) y[i] += a * dy[i];
+            tau += a * dtau;
+            kap += a * dkap;
+            res->trace_ap[it] = a; res->trace_ad[it] = sg;
+            small = a < 1e-6 ? small + 1 : 0;
+            if (!(tau > 0) || !(kap > 0)) { status = 4; goto fin; }
+        }
+        /* normalise tau to 1 when it has grown (solver.c:12054-12069) */
+        if (tau > 10.0) {
+            const double f = 1.0 / tau, sq = sqrt(f);
+            for (size_t q = 0; q < T; q++) { X[q] *= f; Z[q] *= f; }
+            for (int k = 0; k < P.nb; k++) {
+                const OBlk *B = &P.B[k];
+                const size_t len = B->lp ? (size_t)B->n : (size_t)B->n * B->n;
+                for (size_t q = 0; q < len; q++) { LX[B->off + q] *= B->lp ? f : sq; LZ[B->off + q] *= B->lp ? f : sq; }
+            }
+            for (int i = 0; i < m; i++) y[i] *= f;
+            kap *= f;
+            tau = 1.0;
+        }
+    }
+fin:
+    if (status == 0) {
+        for (int i = 0; i < m; i++) y[i] /= tau;
+        for (size_t q = 0; q < T; q++) X[q] /= tau;
+    } else { memcpy(y, yb, sizeof(double) * m); memcpy(X, Xb, sizeof(double) * T); }
+done:
+    res->m = m; res->nblk = P.nb;
+    res->y = malloc(sizeof(double) * (m + 1));
+    res->X = malloc(sizeof(double) * (T + 1));
+    for (int i = 0; i < m; i++) res->y[i] = P.cs * P.d[i] * y[i];
+    for (size_t q = 0; q < T; q++) res->X[q] = P.bs * X[q];
+    res->xlen = T;
+    {
+        double pobj, dobj;
+        dimacs(&P, res->X, res->y, res->err, &pobj, &dobj, work);
+        res->pobj = pobj; res->dobj = dobj;
+        double acc = fmax(fmax(res->err[0], res->err[1]), fmax(res->err[3], fmax(fabs(res->err[4]), fabs(res->err[5]))));
+        if (!isfinite(acc)) acc = INFINITY;
+        if (status == 0 || status == 3 || status == 4 || status == 5) {
+            if (acc <= tol) status = 0;
+            else if (acc <= 1e-6) status = 5;
+            else if (status == 0) status = 4;
+        }
+        res->acc = acc;
+    }
+    res->status = status;
+    free(X); free(Z); free(Zi); free(LX); free(LZ); free(dX); free(dZ); free(F); free(G); free(Q); free(Rd); free(Xb);
+    free(CX); free(dXa); free(dZa); free(W); free(work); free(y); free(dy); free(rp); free(w1); free(w2); free(u); free(v);
+    free(gv); free(a0); free(t1); free(yb); free(aq); free(ev0); free(ev1); free(dX0); free(dX1); free(dZ0); free(dZ1);
+    free(S.M); free(S.L); free(S.D);
     prob_free(&P);
     return 0;
 }

@@ -14,7 +14,7 @@ typedef struct {                /* the numbers of an SDPA sparse file (capi.c:17
     const int *mat, *blk, *i, *j;
     const double *v;
 } OracleData;
-typedef struct { double tol; int maxit; } OracleOptions;
+typedef struct { double tol; int maxit; int method; } OracleOptions;   /* method 0: infeasible start, 1: self-dual embedding */
 typedef struct {
     int status, iters, m, nblk;
     double pobj, dobj, acc;

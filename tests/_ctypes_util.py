@@ -22,7 +22,7 @@ TR = 128
 
 
 class OracleOptions(ctypes.Structure):
-    _fields_ = [("tol", D), ("maxit", I)]
+    _fields_ = [("tol", D), ("maxit", I), ("method", I)]
 
 
 class OracleResult(ctypes.Structure):
@@ -120,11 +120,11 @@ def oracle_data(inst):
     return OracleData(s.m, s.nblk, s.bs, s.c, s.nnz, s.mat, s.blk, s.i, s.j, s.v)
 
 
-def oracle_solve(inst, tol=1e-8, maxit=100):
+def oracle_solve(inst, tol=1e-8, maxit=100, method=0):
     lib = oracle_lib()
     d = oracle_data(inst)
     r = OracleResult()
-    rc = lib.oracle_solve(ctypes.byref(d), ctypes.byref(OracleOptions(tol, maxit)), ctypes.byref(r))
+    rc = lib.oracle_solve(ctypes.byref(d), ctypes.byref(OracleOptions(tol, maxit, method)), ctypes.byref(r))
     if rc != 0:
         raise ValueError("oracle_solve: bad data (%d)" % rc)
     out = dict(status=r.status, iters=r.iters, pobj=r.pobj, dobj=r.dobj, err=list(r.err), acc=r.acc,

@@ -235,3 +235,16 @@ def test_oracle_dimacs_definitions(name):
     err, gpo, gdo = oracle_dimacs(inst, pack(X, lp), y)
     assert gpo == pytest.approx(po, rel=1e-12) and gdo == pytest.approx(do, rel=1e-12)
     assert np.allclose(err, want, rtol=1e-8, atol=1e-14), (err, want)
+
+
+@pytest.mark.parametrize("name", sorted(k for k in GOLD if k != "file_arch0"))
+def test_oracle_hsd_vs_reference(name):
+    """The self-dual embedding variant (method 1) reaches the reference's optimum too: the
+    normalised iterate (X, y) / tau, DIMACS errors <= 1e-8, objectives as the standard method's."""
+    rec = GOLD[name]
+    inst = golden_instance(rec)
+    o = oracle_solve(inst, method=1)
+    assert o["status"] == 0, o["status"]
+    assert objective_ok(o["pobj"], rec, "pobj"), (o["pobj"], rec["pobj"], rec["dd"]["pobj"])
+    assert objective_ok(o["dobj"], rec, "dobj"), (o["dobj"], rec["dobj"], rec["dd"]["dobj"])
+    assert max(abs(e) for e in o["err"]) <= TOL_DIMACS, o["err"]
