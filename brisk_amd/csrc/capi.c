@@ -104,7 +104,7 @@ API int brisk_result_trace(const BriskResult *r, double *po, double *dob, double
 
 /* ---- options ---- */
 int brisk_parse_options(int nopt, const char *const *opts, SolveOpts *o, int m, int *slots, double *heapgb) {
-    o->tol = 1e-8; o->maxit = 100; o->verbose = 0; o->mixed_until = 1e-6; o->time_limit = 0;
+    o->tol = 1e-8; o->maxit = 100; o->verbose = 0; o->mixed_until = 1e-6; o->time_limit = 0; o->hsd = 0;
     o->prec = m >= 1000 ? PREC_MIXED : PREC_F64;
     for (int k = 0; k < nopt; k++) {
         const char *a = opts[k];
@@ -122,7 +122,8 @@ int brisk_parse_options(int nopt, const char *const *opts, SolveOpts *o, int m, 
         else if (!strcmp(a, "-mixeduntil")) { NEED(); o->mixed_until = atof(nx); }
         else if (!strcmp(a, "-slots")) { NEED(); if (slots) *slots = atoi(nx); }
         else if (!strcmp(a, "-heapgb")) { NEED(); if (heapgb) *heapgb = atof(nx); }
-        else if (!strcmp(a, "-threads") || !strcmp(a, "-bm") || !strcmp(a, "-hsd") || !strcmp(a, "-gpu")) { NEED(); }
+        else if (!strcmp(a, "-hsd")) { NEED(); o->hsd = atoi(nx) > 0; }
+        else if (!strcmp(a, "-threads") || !strcmp(a, "-bm") || !strcmp(a, "-gpu")) { NEED(); }
         else if (!strcmp(a, "-v")) o->verbose = 1;
         else if (!strcmp(a, "-q")) o->verbose = 0;
         else { say(1, "brisk: unknown option %s\n", a); return 1; }
@@ -194,6 +195,7 @@ static int solve_many(int n, const BriskProblem *pr, int nopt, const char *const
         const int e = hprob_build(&S[q].P, p->m, p->nblk, p->bs, p->c, p->nnz, p->mat, p->blk, p->i, p->j, p->v);
         built = q + 1;
         if (e) { rc = e; goto out; }
+        S[q].P.want_hsd = S[q].o.hsd;
         need += (size_t)solve_heap_bytes(&S[q].P);
         SP[q] = &S[q];
         S[q].id = q;

@@ -131,6 +131,7 @@ int64_t solve_heap_bytes(const HProb *P) {
     b += 22 * (m * 8 + 256) + NSC * 8 + 256;
     b += (m + 1) * 4 + (int64_t)P->nent * (sizeof(SEnt) + 12) + (int64_t)P->nb * sizeof(BlkRef) + (P->npos + 1) * 12 + 2048;
     b += (m + 1) * 4 + (int64_t)(P->ngrp + 1) * sizeof(GrpRef) + 2 * (P->gp_tot * 8 + 256) + 512;
+    if (P->want_hsd) b += 6 * (T * 8 + 256) + 5 * (m * 8 + 256);
     if (P->dense) {
         const int64_t nn = T;
         b += nn * m * (8 + 4 + 8 + 8) + 4 * 256;      /* Ad, Af, P, B */
@@ -169,6 +170,12 @@ int solve_setup(Solve *S) {
     ALLOC(pval, (int64_t)(P->nent + 1) * 8);
     ALLOCB(gptr, (m + 1) * 4); ALLOCB(grp, (int64_t)(P->ngrp + 1) * sizeof(GrpRef));
     if (P->gp_tot > 0) { ALLOC(Pg, P->gp_tot * 8); ALLOC(Tg, P->gp_tot * 8); }
+    S->hsd = S->o.hsd;
+    if (S->hsd) {
+        ALLOC(Rd, T * 8); ALLOC(CX, T * 8); ALLOC(dX0, T * 8); ALLOC(dZ0, T * 8); ALLOC(dX1, T * 8); ALLOC(dZ1, T * 8);
+        ALLOC(hv, m * 8); ALLOC(hg, m * 8); ALLOC(ht, m * 8); ALLOC(hy0, m * 8); ALLOC(hy1, m * 8);
+    }
+    S->tau = 1.0; S->kap = 0.0;
     if (P->dense) {
         const int64_t nn = T;
         const int tl = (int)((m + 127) / 128), lower = tl * (tl + 1) / 2;
@@ -365,6 +372,19 @@ static int chol_sides(Solve **S, int ns, int nside, const int *act, const size_t
 static int blk_chol(Solve **S, int ns, const int *act, size_t fV, size_t fL, size_t fDinv, int *ok) {
     const size_t a[1] = { fV }, b[1] = { fL }, c[1] = { fDinv }, w[1] = { OFF(W2) };
     return chol_sides(S, ns, 1, act, a, b, c, w, ok);
+}
+/* Cholesky of V + h_q I (h_q in g_hdbl[q]) for the active solves, built in dX: V is left as it is */
+static int shifted_chol(Solve **S, int ns, const int *act, size_t fV, size_t fL, size_t fDinv, int *ok) {
+    LinL_clear(&g_ll);
+    MItL_clear(&g_ml);
+    for (int q = 0; q < ns; q++) {
+        if (!act[q]) continue;
+        lin_add(S[q]->P.T, S[q]->dX, F_(S[q], fV), 1.0, -1, 0, -1, 0, -1, 0);
+        for (int k = 0; k < S[q]->P.nb; k++) { MItem it = { S[q]->P.bn[k], q, S[q]->dX + S[q]->P.boff[k], 0, 0, 0 }; MItL_add(&g_ml, it); }
+    }
+    if (!g_ml.n) { for (int q = 0; q < ns; q++) ok[q] = 1; return 0; }
+    if (bk_lincomb(g_ll.v, g_ll.n) || bk_h2d(ddbl(), g_hdbl, sizeof(double) * ns) || bk_add_diag(g_ml.v, g_ml.n, g_bdbl / 8)) return -1;
+    return blk_chol(S, ns, act, OFF(dX), fL, fDinv, ok);
 }
 /* Li = L^{-1} and LiT = Li' per block for the active (solve, side) pairs (after a successful
  * factorisation), all in one batched triangular inversion and one transpose */
@@ -1153,12 +1173,21 @@ out:
 
 /* ---- one iteration (reference solver.c:14394-15189) --------------------------------------- */
 static int ipm_iterate_batch_(Solve **S, int ns);
+static int ipm_iterate_hsd_(Solve **S, int ns);
 int ipm_iterate_batch(Solve **S, int ns) {
     const double t_ = bk_prof_on() ? bk_now() : 0;
-    const int r_ = ipm_iterate_batch_(S, ns);
+    /* the infeasible-start solves and the self-dual ones advance in two batches */
+    Solve **a = malloc(sizeof(Solve *) * (ns + 1)), **b = malloc(sizeof(Solve *) * (ns + 1));
+    int na = 0, nb = 0;
+    for (int q = 0; q < ns; q++) { if (S[q]->hsd) b[nb++] = S[q]; else a[na++] = S[q]; }
+    int r_ = 0;
+    if (na) r_ = ipm_iterate_batch_(a, na);
+    if (!r_ && nb) r_ = ipm_iterate_hsd_(b, nb);
+    free(a); free(b);
     if (t_ > 0) bk_prof_host(45, bk_now() - t_);
     return r_;
 }
+int ipm_iterate_hsd(Solve **S, int ns) { return ipm_iterate_hsd_(S, ns); }
 static int ipm_iterate_batch_(Solve **S, int ns) {
     int rc = -1;
     int *act = calloc(ns, sizeof(int)), *fail = calloc(ns, sizeof(int)), *okp = calloc(ns, sizeof(int)), *okd = calloc(ns, sizeof(int));
@@ -1526,20 +1555,52 @@ int ipm_dimacs(Solve **S, int ns) {
      * and Z on the data as read (W1) */
     if (blk_chol(S, ns, act, OFF(X), OFF(LX), OFF(DinvX), okx)) goto out;
     if (blk_chol(S, ns, act, OFF(W1), OFF(LZ), OFF(DinvZ), okz)) goto out;
-    {
-        int *need = calloc(ns, sizeof(int)), any = 0;
-        for (int q = 0; q < ns; q++) if (!okx[q]) { need[q] = 1; any = 1; }
-        if (any) {
-            if (lammin_all(S, ns, need, OFF(X), lam)) { free(need); goto out; }
-            for (int q = 0; q < ns; q++) if (need[q]) S[q]->res_err[1] = fmax(0.0, -S[q]->P.bs * lam[q]) / (1 + S[q]->P.normb1);
+    /* a side without a factor: lambda_min >= -h when V + h I factors. h = delta = 1e-12 (1 + |b|_1)
+     * for X, 1e-12 (1 + |C|_1) for Z, then decades of delta up to 1e9 delta and three bisections
+     * in the exponent, so the bound is within a factor 1.33 (ps_lammin_tol, postsolve.c:668-726);
+     * beyond 1e9 delta (an error past 1e-3, a verdict only) the Lanczos estimate */
+    for (int side = 0; side < 2; side++) {
+        int *ok = side ? okz : okx, *need = calloc(ns, sizeof(int)), *ok2 = calloc(ns, sizeof(int)), any = 0;
+        double *dl = calloc(ns, sizeof(double)), *lo = calloc(ns, sizeof(double)), *hi = calloc(ns, sizeof(double));
+        const size_t fV = side ? OFF(W1) : OFF(X), fL = side ? OFF(LZ) : OFF(LX), fD = side ? OFF(DinvZ) : OFF(DinvX);
+        for (int q = 0; q < ns; q++) if (!ok[q]) {
+            const HProb *P = &S[q]->P;
+            dl[q] = side ? 1e-12 * (1 + P->normC1) / P->cs : 1e-12 * (1 + P->normb1) / P->bs;   /* scaled data */
+            lo[q] = 0; hi[q] = -1; need[q] = 1; any = 1;
         }
-        any = 0;
-        for (int q = 0; q < ns; q++) { need[q] = !okz[q]; any |= need[q]; }
-        if (any) {
-            if (lammin_all(S, ns, need, OFF(W1), lam)) { free(need); goto out; }
-            for (int q = 0; q < ns; q++) if (need[q]) S[q]->res_err[3] = fmax(0.0, -S[q]->P.cs * lam[q]) / (1 + S[q]->P.normC1);
+        /* the ladder: h = 10^k delta, k = 0..9, for the solves not yet bounded */
+        for (int k = 0; any && k <= 9; k++) {
+            for (int q = 0; q < ns; q++) g_hdbl[q] = need[q] ? dl[q] * pow(10.0, k) : 0;
+            if (shifted_chol(S, ns, need, fV, fL, fD, ok2)) { free(need); free(ok2); free(dl); free(lo); free(hi); goto out; }
+            any = 0;
+            for (int q = 0; q < ns; q++) if (need[q]) {
+                if (ok2[q]) { need[q] = 0; hi[q] = g_hdbl[q]; lo[q] = k ? hi[q] / 10 : 0; }
+                else any = 1;
+            }
         }
-        free(need);
+        /* three bisections in the exponent for the solves bounded past delta */
+        int *bis = calloc(ns, sizeof(int)), nb = 0;
+        for (int q = 0; q < ns; q++) if (hi[q] > 0 && lo[q] > 0) { bis[q] = 1; nb++; }
+        for (int b = 0; nb && b < 3; b++) {
+            for (int q = 0; q < ns; q++) g_hdbl[q] = bis[q] ? sqrt(lo[q] * hi[q]) : 0;
+            if (shifted_chol(S, ns, bis, fV, fL, fD, ok2)) { free(bis); free(need); free(ok2); free(dl); free(lo); free(hi); goto out; }
+            for (int q = 0; q < ns; q++) if (bis[q]) { if (ok2[q]) hi[q] = g_hdbl[q]; else lo[q] = g_hdbl[q]; }
+        }
+        free(bis);
+        for (int q = 0; q < ns; q++) if (!ok[q] && hi[q] > 0) {
+            const HProb *P = &S[q]->P;
+            if (side) S[q]->res_err[3] = P->cs * hi[q] / (1 + P->normC1);
+            else S[q]->res_err[1] = P->bs * hi[q] / (1 + P->normb1);
+        }
+        if (any) {
+            if (lammin_all(S, ns, need, fV, lam)) { free(need); free(ok2); free(dl); free(lo); free(hi); goto out; }
+            for (int q = 0; q < ns; q++) if (need[q]) {
+                const HProb *P = &S[q]->P;
+                const double e = fmax(0.0, -(side ? P->cs : P->bs) * lam[q]) / (1 + (side ? P->normC1 : P->normb1));
+                S[q]->res_err[side ? 3 : 1] = fmax(e, (side ? P->cs : P->bs) * dl[q] * 1e9 / (1 + (side ? P->normC1 : P->normb1)));
+            }
+        }
+        free(need); free(ok2); free(dl); free(lo); free(hi);
     }
     /* status from the measured errors (main.c:2349-2359, brisk.h:187-189) */
     for (int q = 0; q < ns; q++) {
@@ -1598,6 +1659,355 @@ int ipm_finish(Solve *s) {
     free(X); free(Zint);
     s->t_solve = bk_now() - s->t_start;
     return 0;
+}
+
+/* ---- the homogeneous self-dual embedding (reference hsd_run, solver.c:9803-12211) ------------
+ * Variables X, y, Z, tau, kappa with rp = tau b - A(X), Rd = tau C - A*(y) - Z, rg = kappa - b'y +
+ * <C, X>, mu = (<X, Z> + tau kappa) / (nu + 1) (solver.c:10357). Each iteration: the Schur factor at
+ * (X, Zi); v = M^-1 (b + A(X C Zi)) for the tau column; the affine direction (sigma = 0) and its
+ * step; the corrector family at sigma = 0 and 1 with the second-order term of the affine direction
+ * (affine in sigma, solver.c:11356-11375); sigma chosen among sigma_m, 0.2 sigma_m and 3 sigma_m by
+ * the predicted mu reduction alpha (1 - sigma) (solver.c:11195-11228); one step length for all
+ * variables; dkappa = (sigma mu - tau kappa - kappa dtau) / tau (solver.c:11127); tau normalised
+ * to 1 when it passes 10 (solver.c:12054-12069). The solution is (X, y) / tau. */
+static int ipm_iterate_hsd_(Solve **S, int ns) {
+    int rc = -1;
+    int *act = calloc(ns, sizeof(int)), *fail = calloc(ns, sizeof(int)), *okp = calloc(ns, sizeof(int)), *okd = calloc(ns, sizeof(int));
+    int *nrm = calloc(ns, sizeof(int));
+    double *V = calloc(32 * (size_t)ns, sizeof(double));       /* per-solve scalars, 32 each */
+    double *rel = calloc(ns, sizeof(double)), *apx = calloc(ns, sizeof(double)), *apz = calloc(ns, sizeof(double));
+#define HV(q, k) V[32 * (size_t)(q) + (k)]
+    enum { MU, XZ, CXv, BY, RG, C1, C2, CZ, BV, GV, DEN, SIGM, DTA, DKA, DT0, DK0, DT1, DK1, CK, BESTS, BESTA, BESTM, CTK };
+    for (int q = 0; q < ns; q++) act[q] = !S[q]->done;
+    int nsl = 1;
+    for (int q = 0; q < ns; q++) nsl += S[q]->P.nb;
+    if (batch_scratch(ns, nsl + ns)) goto out;
+    /* Zi = LZi' LZi */
+    if (blk_gemm(S, ns, act, 1, 0, 2 | 8, 1.0, 0.0, OFF(LZi), OFF(LZi), OFF(Zi))) goto out;
+    if (blk_sym(S, ns, act, OFF(Zi))) goto out;
+    /* residuals: rp = tau b - A(X), Rd = tau C - Z - A*(y) */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        Solve *s = S[q];
+        lin_add(s->P.m, s->rp, s->bvec, s->tau, -1, 0, -1, 0, -1, 0);
+        lin_add(s->P.T, s->Rd, s->C, s->tau, s->Z, -1.0, -1, 0, -1, 0);
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (aop(S, ns, act, OFF(X), OFF(rp), -1.0, 1.0)) goto out;
+    if (aty(S, ns, act, OFF(y), OFF(Rd), -1.0)) goto out;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->t2, S[q]->bvec, S[q]->tau, S[q]->rp, -1.0, -1, 0, -1, 0);
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->t1, S[q]->rp, 1.0, S[q]->duv, 0, -1, 0, -1, 0);
+    if (bk_vmul(g_ll.v, g_ll.n)) goto out;
+    DotL_clear(&g_dl);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        Solve *s = S[q];
+        dot_add(s->P.T, s->X, s->Z, SC(q, 0));
+        dot_add(s->P.T, s->C, s->X, SC(q, 1));
+        dot_add(s->P.T, s->Rd, s->Rd, SC(q, 2));
+        dot_add(s->P.m, s->bvec, s->y, SC(q, 3));
+        dot_add(s->P.m, s->t1, s->t1, SC(q, 4));
+        dot_add(s->P.m, s->t2, s->t2, SC(q, 5));
+    }
+    if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) {
+        if (!act[q]) continue;
+        Solve *s = S[q];
+        const HProb *P = &s->P;
+        const double xz = HS(q, 0), cx = HS(q, 1), rdn2 = HS(q, 2), by = HS(q, 3), rpo = sqrt(HS(q, 4)), nAX = sqrt(HS(q, 5));
+        if (s->kap <= 0) s->kap = fmax(xz / P->Ntot, 1e-12);           /* start: tau kappa = mu */
+        const double tau = s->tau, kap = s->kap;
+        HV(q, XZ) = xz; HV(q, CXv) = cx; HV(q, BY) = by;
+        HV(q, MU) = (xz + tau * kap) / (P->Ntot + 1.0);
+        HV(q, RG) = kap - by + cx;
+        const double sc = P->bs * P->cs, po = sc * cx / tau, dob = sc * by / tau;
+        s->pinf = rpo / tau / (1 + P->normb2);
+        s->dinf = P->cs * sqrt(rdn2) / tau / (1 + P->normC2);
+        const double oden = 1 + fabs(po) + fabs(dob);
+        s->relgap = fabs(po - dob) / oden;
+        s->relcomp = sc * xz / (tau * tau) / oden;
+        s->pobj = po; s->dobj = dob; s->mu = xz / P->Ntot / (tau * tau);
+        double score = fmax(fmax(s->relgap, s->relcomp), fmax(s->pinf, s->dinf));
+        if (!isfinite(score)) score = INFINITY;
+        s->score = score;
+        const int it = s->it;
+        if (it < 128) { s->tr_pobj[it] = po; s->tr_dobj[it] = dob; s->tr_mu[it] = sc * s->mu; }
+        if (s->o.verbose)
+            solve_log("%4d %+.10e %+.10e %.1e %.1e %.1e %.1e %.3f tau %.2e kap %.2e %s %.3fs\n", it, po, dob, s->pinf, s->dinf, s->relgap,
+                      sc * s->mu, s->alp, tau, kap, s->f32 ? "f32" : "f64", bk_now() - s->t_start);
+        int st = -1;
+        if (!isfinite(po) || !isfinite(dob)) st = 4;
+        else {
+            if (score < s->best_score) {
+                s->best_score = score; s->it_best = it;
+                lin_add(P->T, s->Xb, s->X, 1.0 / tau, -1, 0, -1, 0, -1, 0);
+                lin_add(P->m, s->yb, s->y, 1.0 / tau, -1, 0, -1, 0, -1, 0);
+            }
+            if (score < s->o.tol) st = 0;
+            else if (it >= s->o.maxit) st = 3;
+            else if (s->stall >= 3) st = 4;
+            else if (it - s->it_best >= 5 && s->best_score < 1e-6) st = 4;
+            else if (it - s->it_best >= 12) st = 4;
+            else if (s->o.time_limit > 0 && bk_now() - s->t_start > s->o.time_limit) st = 6;
+            (void)nAX;
+        }
+        if (st == 0) nrm[q] = 1;  /* the normalised iterate is the solution (scaled below) */
+        if (st >= 0) { s->status = st; s->done = 1; act[q] = 0; continue; }
+        if (s->o.prec == PREC_MIXED && fmax(s->relgap, fmax(s->pinf, s->dinf)) <= s->o.mixed_until) s->want_double = 1;
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (nrm[q]) {
+        Solve *s = S[q];
+        lin_add(s->P.T, s->X, s->X, 1.0 / s->tau, -1, 0, -1, 0, -1, 0);
+        lin_add(s->P.m, s->y, s->y, 1.0 / s->tau, -1, 0, -1, 0, -1, 0);
+        nrm[q] = 0;
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    {
+        int any = 0;
+        for (int q = 0; q < ns; q++) any |= act[q];
+        if (!any) { rc = 0; goto out; }
+    }
+    if (schur_assemble(S, ns, act, 0)) goto out;
+    if (schur_factor(S, ns, act, 0, fail)) goto out;
+    for (int q = 0; q < ns; q++) if (act[q] && fail[q]) { S[q]->status = 4; S[q]->done = 1; act[q] = 0; }
+    /* CX = Zi C X, G = Zi Rd X; hg = A(CX), ht = A(G), a0 = A(Zi); v = M^-1 (b + hg) */
+    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(C), OFF(W1))) goto out;
+    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(CX))) goto out;
+    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(Rd), OFF(W1))) goto out;
+    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G))) goto out;
+    if (aop(S, ns, act, OFF(CX), OFF(hg), 1.0, 0.0) || aop(S, ns, act, OFF(G), OFF(ht), 1.0, 0.0) || aop(S, ns, act, OFF(Zi), OFF(a0), 1.0, 0.0)) goto out;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->rhs, S[q]->bvec, 1.0, S[q]->hg, 1.0, -1, 0, -1, 0);
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (schur_solve(S, ns, act, OFF(rhs), OFF(hv), 0.0, rel)) goto out;
+    DotL_clear(&g_dl);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        Solve *s = S[q];
+        dot_add(s->P.T, s->C, s->CX, SC(q, 0));
+        dot_add(s->P.T, s->C, s->G, SC(q, 1));
+        dot_add(s->P.T, s->C, s->Zi, SC(q, 2));
+        dot_add(s->P.m, s->bvec, s->hv, SC(q, 3));
+        dot_add(s->P.m, s->hg, s->hv, SC(q, 4));
+    }
+    if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        HV(q, C2) = HS(q, 0); HV(q, C1) = HS(q, 1); HV(q, CZ) = HS(q, 2); HV(q, BV) = HS(q, 3); HV(q, GV) = HS(q, 4);
+        HV(q, DEN) = HV(q, BV) + HV(q, C2) - HV(q, GV) + S[q]->kap / S[q]->tau;
+    }
+    /* three directions: 0 affine (sigma 0, no K) into dy/dX/dZ; 1 sigma = 0 with K into hy0/dX0/dZ0;
+     * 2 sigma = 1 with K into hy1/dX1/dZ1 */
+    for (int pass = 0; pass < 3; pass++) {
+        const size_t fy = pass == 0 ? OFF(dy) : pass == 1 ? OFF(hy0) : OFF(hy1);
+        const size_t fdX = pass == 0 ? OFF(dX) : pass == 1 ? OFF(dX0) : OFF(dX1);
+        const size_t fdZ = pass == 0 ? OFF(dZ) : pass == 1 ? OFF(dZ0) : OFF(dZ1);
+        /* w1 = eta rp - smu a0 + (tau b - rp) + eta A(Zi Rd X) [+ A(Q)] */
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) if (act[q]) {
+            Solve *s = S[q];
+            if (pass == 0) lin_add(s->P.m, s->rhs, s->bvec, s->tau, s->ht, 1.0, -1, 0, -1, 0);
+            else if (pass == 1) lin_add(s->P.m, s->rhs, s->bvec, s->tau, s->ht, 1.0, s->qv, 1.0, -1, 0);
+            else lin_add(s->P.m, s->rhs, s->bvec, s->tau, s->rp, -1.0, s->a0, -HV(q, MU), s->qv, 1.0);
+        }
+        if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+        if (schur_solve(S, ns, act, OFF(rhs), fy, pass == 0 ? 1e-7 : 0.0, rel)) goto out;
+        DotL_clear(&g_dl);
+        for (int q = 0; q < ns; q++) if (act[q]) {
+            Solve *s = S[q];
+            dot_add(s->P.m, s->bvec, F_(s, fy), SC(q, 0));
+            dot_add(s->P.m, s->hg, F_(s, fy), SC(q, 1));
+            if (pass == 1) dot_add(s->P.T, s->C, s->Q, SC(q, 2));
+        }
+        if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) {
+            if (!act[q]) continue;
+            Solve *s = S[q];
+            const double tau = s->tau, kap = s->kap, mu = HV(q, MU);
+            const double sg = pass == 2 ? 1.0 : 0.0, eta = 1.0 - sg, smu = sg * mu;
+            if (pass == 1) { HV(q, CK) = -HS(q, 2); HV(q, CTK) = -HV(q, DTA) * HV(q, DKA); }
+            const double cK = pass ? HV(q, CK) : 0.0, ctk = pass ? HV(q, CTK) : 0.0;
+            const double c0 = smu * HV(q, CZ) - HV(q, CXv);
+            const double num = eta * HV(q, RG) - HS(q, 0) + c0 - eta * HV(q, C1) + HS(q, 1) + cK + (smu - tau * kap + ctk) / tau;
+            const double dtau = num / HV(q, DEN), dkap = (smu - tau * kap + ctk - kap * dtau) / tau;
+            if (pass == 0) { HV(q, DTA) = dtau; HV(q, DKA) = dkap; }
+            else if (pass == 1) { HV(q, DT0) = dtau; HV(q, DK0) = dkap; }
+            else { HV(q, DT1) = dtau; HV(q, DK1) = dkap; }
+            /* dy = u + dtau v; dZ = eta Rd + dtau C (then - A*(dy)) */
+            lin_add(s->P.m, F_(s, fy), F_(s, fy), 1.0, s->hv, dtau, -1, 0, -1, 0);
+            if (pass < 2) lin_add(s->P.T, F_(s, fdZ), s->Rd, eta, s->C, dtau, -1, 0, -1, 0);
+            else lin_add(s->P.T, F_(s, fdZ), s->C, dtau, -1, 0, -1, 0, -1, 0);
+        }
+        if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+        if (aty(S, ns, act, fy, fdZ, -1.0)) goto out;
+        /* dX = smu Zi - X - sym(Zi dZ X) [- Q] */
+        if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(Zi), fdZ, OFF(W1))) goto out;
+        if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G))) goto out;
+        if (blk_sym(S, ns, act, OFF(G))) goto out;
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) if (act[q]) {
+            Solve *s = S[q];
+            if (pass == 0) lin_add(s->P.T, F_(s, fdX), s->G, -1.0, s->X, -1.0, -1, 0, -1, 0);
+            else if (pass == 1) lin_add(s->P.T, F_(s, fdX), s->G, -1.0, s->X, -1.0, s->Q, -1.0, -1, 0);
+            else lin_add(s->P.T, F_(s, fdX), s->G, -1.0, s->X, -1.0, s->Q, -1.0, s->Zi, HV(q, MU));
+        }
+        if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+        if (pass == 0) {
+            /* the affine step, mu after it, sigma_m; then Q = sym(Zi dZa dXa) (W1 = Zi dZa) */
+            if (maxstep2(S, ns, act, apx, apz)) goto out;
+            DotL_clear(&g_dl);
+            for (int q = 0; q < ns; q++) if (act[q]) {
+                Solve *s = S[q];
+                dot_add(s->P.T, s->X, s->dZ, SC(q, 0));
+                dot_add(s->P.T, s->dX, s->Z, SC(q, 1));
+                dot_add(s->P.T, s->dX, s->dZ, SC(q, 2));
+            }
+            if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+            for (int q = 0; q < ns; q++) if (act[q]) {
+                Solve *s = S[q];
+                const double tau = s->tau, kap = s->kap, dtau = HV(q, DTA), dkap = HV(q, DKA);
+                double am = fmin(apx[q], apz[q]);
+                if (dtau < 0) am = fmin(am, -tau / dtau);
+                if (dkap < 0) am = fmin(am, -kap / dkap);
+                const double aa = fmin(1.0, am);
+                const double mua = (HV(q, XZ) + aa * (HS(q, 0) + HS(q, 1)) + aa * aa * HS(q, 2) + (tau + aa * dtau) * (kap + aa * dkap)) / (s->P.Ntot + 1.0);
+                const double expon = s->relgap > 1e-6 ? fmax(1.0, 3.0 * aa * aa) : 3.0;
+                HV(q, SIGM) = fmin(1.0, pow(fmax(mua / HV(q, MU), 0.0), expon));
+            }
+            if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(dX), OFF(Q))) goto out;
+            if (blk_sym(S, ns, act, OFF(Q))) goto out;
+            if (aop(S, ns, act, OFF(Q), OFF(qv), 1.0, 0.0)) goto out;
+        }
+    }
+    /* the sigma search on d0 + sigma (d1 - d0) */
+    for (int q = 0; q < ns; q++) if (act[q]) { HV(q, BESTM) = -1; HV(q, BESTS) = HV(q, SIGM); HV(q, BESTA) = 0; }
+    for (int c = 0; c < 3; c++) {
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) if (act[q]) {
+            Solve *s = S[q];
+            const double sm = HV(q, SIGM);
+            const double sg = c == 0 ? sm : c == 1 ? fmin(0.95, fmax(1e-4, 0.2 * sm)) : fmin(0.95, fmax(1e-3, 3.0 * sm));
+            HV(q, 24 + c) = sg;
+            lin_add(s->P.T, s->dX, s->dX0, 1.0 - sg, s->dX1, sg, -1, 0, -1, 0);
+            lin_add(s->P.T, s->dZ, s->dZ0, 1.0 - sg, s->dZ1, sg, -1, 0, -1, 0);
+        }
+        if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+        if (maxstep2(S, ns, act, apx, apz)) goto out;
+        for (int q = 0; q < ns; q++) if (act[q]) {
+            Solve *s = S[q];
+            const double sg = HV(q, 24 + c);
+            const double dtau = HV(q, DT0) + sg * (HV(q, DT1) - HV(q, DT0)), dkap = HV(q, DK0) + sg * (HV(q, DK1) - HV(q, DK0));
+            double am = fmin(apx[q], apz[q]);
+            if (dtau < 0) am = fmin(am, -s->tau / dtau);
+            if (dkap < 0) am = fmin(am, -s->kap / dkap);
+            const double gam = 0.9 + (0.99 - 0.9) * fmin(1.0, am);
+            const double al = fmin(1.0, gam * am), merit = al * (1.0 - sg);
+            if (merit > HV(q, BESTM)) { HV(q, BESTM) = merit; HV(q, BESTS) = sg; HV(q, BESTA) = al; }
+        }
+    }
+    /* the chosen direction; X and Z positive definite at one common step (x0.8 on failure) */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        Solve *s = S[q];
+        const double sg = HV(q, BESTS);
+        lin_add(s->P.T, s->dX, s->dX0, 1.0 - sg, s->dX1, sg, -1, 0, -1, 0);
+        lin_add(s->P.T, s->dZ, s->dZ0, 1.0 - sg, s->dZ1, sg, -1, 0, -1, 0);
+        lin_add(s->P.m, s->dy, s->hy0, 1.0 - sg, s->hy1, sg, -1, 0, -1, 0);
+        apx[q] = apz[q] = HV(q, BESTA);
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    {
+        int *ta = calloc(2 * ns, sizeof(int)), *okv = calloc(2 * ns, sizeof(int)), *ok = calloc(2 * ns, sizeof(int));
+        double *st = calloc(2 * ns, sizeof(double));
+        int e = 0;
+        for (int t = 0; t < 2 * ns; t++) ta[t] = act[t % ns];
+        for (int pass = 0; pass < 2 && !e; pass++) {
+            /* pass 0: each side its own retries; pass 1: the longer side back to the common step */
+            if (pass == 1) {
+                int any = 0;
+                for (int t = 0; t < 2 * ns; t++) {
+                    const int q = t % ns;
+                    ta[t] = 0;
+                    if (!act[q] || !okv[q] || !okv[ns + q]) continue;
+                    const double a = fmin(apx[q], apz[q]);
+                    double *my = t < ns ? &apx[q] : &apz[q];
+                    if (*my > a) { *my = a; ta[t] = 1; any = 1; }
+                }
+                if (!any) break;
+            }
+            for (int tries = 0; tries < 40; tries++) {
+                int any = 0;
+                for (int t = 0; t < 2 * ns; t++) any |= ta[t];
+                if (!any) break;
+                LinL_clear(&g_ll);
+                for (int t = 0; t < 2 * ns; t++) {
+                    if (!ta[t]) continue;
+                    const int sd = t / ns, q = t % ns;
+                    double *a = sd ? &apz[q] : &apx[q];
+                    const double inc = *a - st[t];
+                    st[t] = *a;
+                    lin_add(S[q]->P.T, F_(S[q], F_V[sd]), F_(S[q], F_V[sd]), 1.0, F_(S[q], F_dV[sd]), inc, -1, 0, -1, 0);
+                }
+                if (bk_lincomb(g_ll.v, g_ll.n)) { e = 1; break; }
+                if (chol_sides(S, ns, 2, ta, F_V, F_L, F_D, F_W, ok)) { e = 1; break; }
+                for (int t = 0; t < 2 * ns; t++) {
+                    if (!ta[t]) continue;
+                    if (ok[t]) { ta[t] = 0; okv[t] = 1; }
+                    else { okv[t] = 0; if (t / ns) apz[t % ns] *= 0.8; else apx[t % ns] *= 0.8; }
+                }
+            }
+        }
+        for (int q = 0; q < ns; q++) { okp[q] = okv[q]; okd[q] = okv[ns + q]; }
+        if (!e) {
+            /* update y, tau, kappa; normalise tau to 1 past 10 (X and Z are refactored) */
+            LinL_clear(&g_ll);
+            int anyn = 0;
+            for (int q = 0; q < ns; q++) {
+                if (!act[q]) continue;
+                Solve *s = S[q];
+                if (!okp[q] || !okd[q]) { s->status = 4; s->done = 1; continue; }
+                const double a = fmin(apx[q], apz[q]), sg = HV(q, BESTS);
+                const double dtau = HV(q, DT0) + sg * (HV(q, DT1) - HV(q, DT0)), dkap = HV(q, DK0) + sg * (HV(q, DK1) - HV(q, DK0));
+                lin_add(s->P.m, s->y, s->y, 1.0, s->dy, a, -1, 0, -1, 0);
+                s->tau += a * dtau; s->kap += a * dkap;
+                s->alp = s->ald = a;
+                s->stall = a < 1e-6 ? s->stall + 1 : 0;
+                if (!(s->tau > 0) || !(s->kap > 0)) { s->status = 4; s->done = 1; okp[q] = 0; continue; }
+                if (s->tau > 10.0) {
+                    const double f = 1.0 / s->tau;
+                    lin_add(s->P.T, s->X, s->X, f, -1, 0, -1, 0, -1, 0);
+                    lin_add(s->P.T, s->Z, s->Z, f, -1, 0, -1, 0, -1, 0);
+                    nrm[q] = 1; anyn = 1;
+                }
+            }
+            if (bk_lincomb(g_ll.v, g_ll.n)) e = 1;
+            if (!e && anyn) {
+                LinL_clear(&g_ll);
+                for (int q = 0; q < ns; q++) if (nrm[q]) {
+                    Solve *s = S[q];
+                    lin_add(s->P.m, s->y, s->y, 1.0 / s->tau, -1, 0, -1, 0, -1, 0);
+                    s->kap /= s->tau; s->tau = 1.0;
+                }
+                if (bk_lincomb(g_ll.v, g_ll.n)) e = 1;
+                for (int t = 0; t < 2 * ns; t++) ta[t] = nrm[t % ns];
+                if (!e && chol_sides(S, ns, 2, ta, F_V, F_L, F_D, F_W, ok)) e = 1;
+            }
+            for (int t = 0; t < 2 * ns; t++) okv[t] = act[t % ns] && okp[t % ns] && okd[t % ns] && !S[t % ns]->done;
+            if (!e) e = inv_sides(S, ns, 2, okv, F_L, F_D, F_Li, F_LiT, F_W);
+        }
+        free(ta); free(okv); free(ok); free(st);
+        if (e) goto out;
+    }
+    for (int q = 0; q < ns; q++) if (act[q] && !S[q]->done) S[q]->it++;
+    rc = 0;
+out:
+#undef HV
+    free(act); free(fail); free(okp); free(okd); free(nrm); free(V); free(rel); free(apx); free(apz);
+    return rc;
 }
 
 int ipm_run_batch(Solve **S, int ns) {

@@ -46,6 +46,7 @@ typedef struct {
      * symmetric storage, column i = vec(A_i) (n*n x m); the entry and position lists are empty */
     int dense;
     double *Ad;
+    int want_hsd;           /* the solve will run the self-dual embedding (its arrays are counted) */
 } HProb;
 
 enum { PREC_F64 = 0, PREC_MIXED = 1 };
@@ -57,6 +58,7 @@ typedef struct {
     int verbose;
     double mixed_until;     /* mixed: f32 factor while max(relgap, pinf, dinf) exceeds this (1e-6) */
     double time_limit;      /* s, <= 0 none */
+    int hsd;                /* 1: self-dual embedding, 0: infeasible start */
 } SolveOpts;
 
 typedef struct Solve {
@@ -72,6 +74,7 @@ typedef struct Solve {
     int64_t sc;             /* 64 scalars */
     int64_t cptr, ent, blk, pptr, ppos, pcon, pval, dinvoff; /* problem data on the device */
     int64_t gptr, grp, Pg, Tg;      /* constraint groups (bytes), heavy-group products P and T (doubles) */
+    int64_t Rd, CX, dX0, dZ0, dX1, dZ1, hv, hg, ht, hy0, hy1;   /* self-dual embedding: residual, Zi C X, the corrector family, vectors */
     int64_t Ad, Af, Pm, Bm, Mpart, LXf, LZif, Om1, Om2;  /* dense route: A f64 / f32 (n^2 x m), P, B, split-K partials, f32 factors, M v scratch */
     int nsplit, kchunk;     /* dense route: split-K of M = P'P */
     int m_exact;            /* the stored M is exact in f64 (else M v uses the operator form) */
@@ -83,6 +86,8 @@ typedef struct Solve {
     /* iteration state */
     int it, status, done, stall, it_best;
     double theta, r0norm, best_score, alp, ald;
+    int hsd;                /* 1: the homogeneous self-dual embedding (tau, kappa) */
+    double tau, kap;
     double pobj, dobj, pinf, dinf, relgap, relcomp, mu, score;
     double err_it[7];
     double t_start, t_solve;
@@ -113,6 +118,7 @@ int ipm_run_batch(Solve **S, int ns);       /* runs every solve to its end */
 /* stepwise interface for the serving loop: start, one iteration for a batch, finish */
 int ipm_start_batch(Solve **S, int ns);
 int ipm_iterate_batch(Solve **S, int ns);   /* one iteration; sets done on solves that end */
+int ipm_iterate_hsd(Solve **S, int ns);     /* the same for the self-dual embedding */
 int ipm_dimacs(Solve **S, int ns);          /* DIMACS errors and status of the returned points (device) */
 int ipm_end_batch(Solve **S, int ns);       /* DIMACS on the device, status, results to the host */
 int ipm_finish(Solve *S);                   /* results of one solve to the host (after ipm_end_batch's DIMACS) */

@@ -72,3 +72,17 @@ def test_full_size_vs_reference(name):
     a = golden_instance(rec).arrays()
     r = brisk_amd.solve_sdpa(a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"])
     check(r, rec)
+
+
+HSD_CASES = sorted(k for k in SMALL if k != "file_arch0")
+
+
+@pytest.mark.parametrize("prec", ["double", "mixed"])
+@pytest.mark.parametrize("name", HSD_CASES)
+def test_hsd_vs_reference(name, prec):
+    """The self-dual embedding (-hsd 1) against the reference's outputs."""
+    rec = SMALL[name]
+    print("case", name, prec, "hsd", flush=True)
+    a = golden_instance(rec).arrays()
+    r = brisk_amd.solve_sdpa(a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"], options=["-prec", prec, "-hsd", "1"])
+    check(r, rec)
