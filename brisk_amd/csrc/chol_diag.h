@@ -1,151 +1,231 @@
 // chol_diag.h - factor and invert one 64 x 64 diagonal block in a workgroup of 256 threads
-// (included by k_dense.hip). Four pivots per barrier: every thread publishes one value of the
-// block's four columns, then eliminates the 4 x 4 diagonal piece redundantly and applies the
-// rank-4 update to the 16 entries it owns (row t % 64, columns t / 64 + 4u), in registers. The
-// inverse of the factor is built by recursive doubling: for block size s every pair of s x s
-// diagonal blocks A, C with the block B between them gets X[C, A] = -X[C, C] B X[A, A].
+// (included by k_dense.hip).
+//
+// Layout: thread (lane r, wave w) holds row r of the block at the 16 columns c = 4u + w in
+// registers. The factor runs in 16 groups of four columns with one barrier each: wave t
+// publishes column j0 + t to LDS, every thread factors the 4 x 4 diagonal piece itself
+// (reciprocal square roots by v_rsq plus Newton steps, no divisions), forms the four values
+// l(r, j0..j0+3) of its own row and writes them to its wave's private copy in LDS, from which
+// the wave reads the rows it needs for the rank-4 update of its own columns (an in-order
+// exchange within the wave: no second barrier).
+//
+// The inverse X = L^{-1} is built by recursive doubling from the 4 x 4 diagonal blocks: for
+// block size S every pair of S x S diagonal blocks A, C with the block B = L[C, A] between them
+// gets X[C, A] = -X[C, C] (B X[A, A]). Sizes 4 and 8 run on the vector units with fully
+// unrolled inner products, sizes 16 and 32 on MFMA with one 16 x 16 tile per wave.
 #pragma once
-// one doubling level: T = L[C, A] X[A, A], then X[C, A] = -X[C, C] T, for the 64 / (2S) pairs
+
+template <typename T> struct DiagMF;
+// out(g, r): the row of the 16 x 16 MFMA result held by register r of a lane in lane group
+// g = lane / 16 (the column is lane % 16)
+template <> struct DiagMF<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int out(int g, int r) { return g + 4 * r; }
+    // 1 / sqrt(x): the hardware estimate and two Newton steps y <- y (3/2 - x y^2 / 2)
+    static __device__ __forceinline__ double rsq(double x) {
+        double y = __builtin_amdgcn_rsq(x);
+        const double h = 0.5 * x;
+#pragma unroll
+        for (int s = 0; s < 2; s++) { const double hy = h * y; y = y * __builtin_fma(-hy, y, 1.5); }
+        return y;
+    }
+};
+template <> struct DiagMF<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int out(int g, int r) { return 4 * g + r; }
+    static __device__ __forceinline__ float rsq(float x) {
+        const float y = __builtin_amdgcn_rsqf(x), hy = 0.5f * x * y;
+        return y * __builtin_fmaf(-hy, y, 1.5f);
+    }
+};
+
+// LDS written by this wave is visible to the wave's later reads (LDS executes one wave's
+// instructions in order; the fences keep the compiler from reordering across this point)
+__device__ __forceinline__ void diag_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// one doubling level for S <= 8 (32 S elements per phase, one per thread); the upper parts of L
+// and X are zero in LDS, so the inner products run over the whole block without bounds
 template <typename T, int S>
-__device__ __forceinline__ void diag_inv_level(T (*Ls)[65], T (*Xs)[65], T (*Ts)[65], int tid) {
+__device__ __forceinline__ void diag_inv_simt(T (*Ls)[65], T (*Xs)[65], T (*Ts)[65], int tid) {
     constexpr int TOT = 32 * S;
-    for (int e = tid; e < TOT; e += 256) {
-        const int pr = e / (S * S), w = e % (S * S), r = w % S, c = w / S;
-        const int p = pr * 2 * S, row = p + S + r, cl = p + c;
-        T a0 = 0, a1 = 0;
-        int k = c;
-        for (; k + 1 < S; k += 2) { a0 += Ls[p + k][row] * Xs[cl][p + k]; a1 += Ls[p + k + 1][row] * Xs[cl][p + k + 1]; }
-        if (k < S) a0 += Ls[p + k][row] * Xs[cl][p + k];
-        Ts[cl][row] = a0 + a1;
+    const int pr = tid / (S * S), e = tid % (S * S), r = e % S, c = e / S;
+    const int p = pr * 2 * S, row = p + S + r, cl = p + c;
+    if (tid < TOT) {
+        T a = 0;
+#pragma unroll
+        for (int k = 0; k < S; k++) a += Ls[p + k][row] * Xs[cl][p + k];
+        Ts[cl][row] = a;
     }
     __syncthreads();
-    for (int e = tid; e < TOT; e += 256) {
-        const int pr = e / (S * S), w = e % (S * S), r = w % S, c = w / S;
-        const int p = pr * 2 * S, row = p + S + r, cl = p + c;
-        T a0 = 0, a1 = 0;
-        int k = 0;
-        for (; k + 1 <= r; k += 2) { a0 += Xs[p + S + k][row] * Ts[cl][p + S + k]; a1 += Xs[p + S + k + 1][row] * Ts[cl][p + S + k + 1]; }
-        if (k <= r) a0 += Xs[p + S + k][row] * Ts[cl][p + S + k];
-        Xs[cl][row] = -(a0 + a1);
+    if (tid < TOT) {
+        T a = 0;
+#pragma unroll
+        for (int k = 0; k < S; k++) a += Xs[p + S + k][row] * Ts[cl][p + S + k];
+        Xs[cl][row] = -a;
     }
     __syncthreads();
 }
+// one doubling level for S = 16, 32 on MFMA: wave q computes one 16 x 16 tile (pairs x tiles =
+// 2 x 1 for S = 16, 1 x 4 for S = 32). Operands: a = left(row, k), b = right(k, col), so
+// mma(b, a) accumulates the tile with its row index in lane % 16.
+template <typename T, int S>
+__device__ __forceinline__ void diag_inv_mfma(T (*Ls)[65], T (*Xs)[65], T (*Ts)[65], int tid) {
+    typedef typename DiagMF<T>::v4 v4;
+    constexpr int NT = S / 16, JOBS = (32 / S) * NT * NT;
+    const int lane = tid & 63, wave = tid >> 6, l16 = lane & 15, g4 = lane >> 4;
+    const int pr = wave / (NT * NT), tt = wave % (NT * NT), ti = tt % NT, tj = tt / NT;
+    const int p = pr * 2 * S, row0 = p + S + 16 * ti, col0 = p + 16 * tj;
+    if (wave < JOBS) {
+        v4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int kk = 0; kk < S; kk += 4) {
+            const int k = p + kk + g4;
+            acc = DiagMF<T>::mma(Xs[col0 + l16][k], Ls[k][row0 + l16], acc);      // (L[C, A] X[A, A])
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) Ts[col0 + DiagMF<T>::out(g4, r)][row0 + l16] = acc[r];
+    }
+    __syncthreads();
+    if (wave < JOBS) {
+        v4 acc = {0, 0, 0, 0};
+#pragma unroll
+        for (int kk = 0; kk < S; kk += 4) {
+            const int k = p + S + kk + g4;
+            acc = DiagMF<T>::mma(Ts[col0 + l16][k], Xs[k][row0 + l16], acc);      // X[C, C] T
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) Xs[col0 + DiagMF<T>::out(g4, r)][row0 + l16] = -acc[r];
+    }
+    __syncthreads();
+}
+
+// Factor the nb x nb block at Ab (leading dimension n, lower triangle read) in place as L L'
+// (strict upper part zeroed) and write L^{-1} (64 x 64, zero outside the nb x nb lower part) to
+// Di. Rows and columns beyond nb are factored as the identity. Returns 0, or the 1-based column
+// whose pivot is not positive; pm_out is the smallest squared pivot.
 template <typename T>
 __device__ int diag_factor_inverse(T *Ab, int n, int nb, T *Di, T &pm_out) {
-    __shared__ T cbuf[2][4][64];
-    __shared__ T Ls[64][65];     // Ls[c][r] = L(r, c)
-    __shared__ T Xs[64][65];     // Xs[c][r] = X(r, c), X = L^{-1}
+    __shared__ T Ls[64][65];       // Ls[c][r] = L(r, c)
+    __shared__ T Xs[64][65];       // Xs[c][r] = X(r, c), X = L^{-1}
     __shared__ T Ts[64][65];
-    __shared__ int sfail;
-    const int tid = threadIdx.x, ri = tid & 63, cb = tid >> 6;
+    __shared__ T colb[2][4][64];   // the group's four columns (double-buffered by group parity)
+    __shared__ T lb[4][64][4];     // per wave: l(r, j0..j0+3) of every row r
+    __shared__ T rdg[64];          // 1 / L(j, j)
+    const int tid = threadIdx.x, r = tid & 63, w = tid >> 6;
     T v[16];
 #pragma unroll
     for (int u = 0; u < 16; u++) {
-        const int c = cb + 4 * u;
-        v[u] = (ri < nb && c < nb && ri >= c) ? Ab[ri + (int64_t)c * n] : (T)0;
+        const int c = 4 * u + w;
+        v[u] = (r < nb && c < nb) ? (r >= c ? Ab[r + (int64_t)c * n] : (T)0) : (r == c ? (T)1 : (T)0);
     }
-    if (tid == 0) sfail = 0;
     T pm = (T)1e30;
-    T dsave[16];              // pivots d_j of this thread's columns (for the final scaling)
+    int fail = 0;
 #pragma unroll
-    for (int u = 0; u < 16; u++) dsave[u] = (T)1;
-    const int nblk4 = (nb + 3) / 4;
-#pragma unroll 1
-    for (int J = 0; J < 16; J++) {
-        if (J >= nblk4) break;
-        const int j0 = 4 * J, buf = J & 1;
-        // publish columns j0 .. j0+3 (column j0 + cb is this thread's v[J]); rows >= nb read 0
-        T vj = (T)0;
-#pragma unroll
-        for (int u = 0; u < 16; u++) if (u == J) vj = v[u];
-        cbuf[buf][cb][ri] = vj;
+    for (int g = 0; g < 16; g++) {
+        const int j0 = 4 * g, bf = g & 1;
+        colb[bf][w][r] = v[g];
         __syncthreads();
-        // the 4 x 4 diagonal piece, eliminated locally (redundantly in every thread)
-        T D[4][4];
-#pragma unroll
-        for (int t = 0; t < 4; t++)
-#pragma unroll
-            for (int s = 0; s < 4; s++) D[t][s] = (j0 + s < nb) ? cbuf[buf][t][j0 + s] : (T)0;   // D[t][s] = c_t[j0+s]
-        // values of the 4 columns at this thread's row and at its 16 columns
-        T cr[4], cl[4][16];
+        T D[4][4], cr[4];
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            cr[t] = cbuf[buf][t][ri];
+            cr[t] = colb[bf][t][r];
 #pragma unroll
-            for (int u = 0; u < 16; u++) cl[t][u] = cbuf[buf][t][cb + 4 * u];
+            for (int s = t; s < 4; s++) D[s][t] = colb[bf][t][j0 + s];
         }
-        T d[4], id[4];
-        int fl = 0;
+        // the 4 x 4 diagonal piece (the same arithmetic in every thread, so every thread takes
+        // the same branch on a failed pivot)
+        T L4[4][4], y[4];
 #pragma unroll
         for (int t = 0; t < 4; t++) {
-            const bool live = j0 + t < nb;
-            d[t] = live ? D[t][t] : (T)1;
-            if (live && !(d[t] > (T)0) && !fl) fl = j0 + t + 1;
-            id[t] = (T)1 / d[t];
-            // eliminate column t from the later columns of the block (at rows j0+s, ri and the 16 columns)
+            T p = D[t][t];
 #pragma unroll
-            for (int w = t + 1; w < 4; w++) {
-                const T g = D[t][w] * id[t];          // c_t[j0+w] / d_t
+            for (int q = 0; q < t; q++) p -= L4[t][q] * L4[t][q];
+            if (!(p > (T)0) && !fail) fail = j0 + t + 1;
+            if (j0 + t < nb) pm = p < pm ? p : pm;
+            y[t] = DiagMF<T>::rsq(p);
+            L4[t][t] = p * y[t];
 #pragma unroll
-                for (int s = 0; s < 4; s++) D[w][s] -= g * D[t][s];
-                cr[w] -= g * cr[t];
+            for (int s = t + 1; s < 4; s++) {
+                T a = D[s][t];
 #pragma unroll
-                for (int u = 0; u < 16; u++) cl[w][u] -= g * cl[t][u];
+                for (int q = 0; q < t; q++) a -= L4[s][q] * L4[t][q];
+                L4[s][t] = a * y[t];
             }
         }
-        if (fl) { if (tid == 0) sfail = fl; break; }
+        if (fail) break;
+        if (w == 0 && r < 4) rdg[j0 + r] = r == 0 ? y[0] : r == 1 ? y[1] : r == 2 ? y[2] : y[3];
+        // this row's l(r, j0..j0+3); the piece's own rows take L4, rows above the columns 0
+        T l[4];
 #pragma unroll
-        for (int t = 0; t < 4; t++) if (j0 + t < nb) pm = d[t] < pm ? d[t] : pm;
-        // this thread's block column (j0 + cb) as it stands at its pivot, and its pivot
+        for (int t = 0; t < 4; t++) {
+            T a = cr[t];
 #pragma unroll
-        for (int u = 0; u < 16; u++) if (u == J) {
-            T cv = cr[0];
-#pragma unroll
-            for (int t = 1; t < 4; t++) if (t == cb) cv = cr[t];
-            v[u] = (ri >= j0 + cb) ? cv : (T)0;
-            T dv = d[0];
-#pragma unroll
-            for (int t = 1; t < 4; t++) if (t == cb) dv = d[t];
-            dsave[u] = dv;
+            for (int q = 0; q < t; q++) a -= l[q] * L4[t][q];
+            l[t] = a * y[t];
         }
-        // trailing update of the columns beyond the block: A(ri, l) -= sum_t c_t[ri] c_t[l] / d_t
 #pragma unroll
-        for (int u = 0; u < 16; u++) {
-            const int l = cb + 4 * u;
-            if (u > J && l <= ri) {
-                T acc = v[u];
+        for (int s = 0; s < 4; s++)
+            if (r == j0 + s) {
 #pragma unroll
-                for (int t = 0; t < 4; t++) acc -= cr[t] * id[t] * cl[t][u];
-                v[u] = acc;
+                for (int t = 0; t < 4; t++) l[t] = t <= s ? L4[s][t] : (T)0;
             }
+#pragma unroll
+        for (int t = 0; t < 4; t++) l[t] = r >= j0 + t ? l[t] : (T)0;
+#pragma unroll
+        for (int t = 0; t < 4; t++) lb[w][r][t] = l[t];
+        diag_wave_sync();
+        v[g] = w == 0 ? l[0] : w == 1 ? l[1] : w == 2 ? l[2] : l[3];
+        // rank-4 update of this thread's later columns k: a(r, k) -= sum_t l(r, t) l(k, t)
+#pragma unroll
+        for (int h = g + 1; h < 16; h++) {
+            const int k = 4 * h + w;
+            T a = v[h];
+#pragma unroll
+            for (int t = 0; t < 4; t++) a -= l[t] * lb[w][k][t];
+            v[h] = a;
         }
     }
     __syncthreads();
-    if (sfail) return sfail;
-    // L(r, c) = a(r, c) / sqrt(d_c)
+    if (fail) return fail;
 #pragma unroll
     for (int u = 0; u < 16; u++) {
-        const int c = cb + 4 * u;
-        const T x = (ri >= c && ri < nb && c < nb) ? v[u] / sqrt(dsave[u]) : (T)0;
+        const int c = 4 * u + w;
+        const T x = r >= c ? v[u] : (T)0;
         v[u] = x;
-        Ls[c][ri] = x;
-        Xs[c][ri] = (T)0;
+        Ls[c][r] = x;
+        Xs[c][r] = (T)0;
     }
     __syncthreads();
-    if (tid < 64) Xs[tid][tid] = tid < nb ? (T)1 / Ls[tid][tid] : (T)1;
+    // the 4 x 4 diagonal blocks of X: thread (b, t) forms column t of block b by substitution
+    if (tid < 64) {
+        const int p = 4 * (tid >> 2), t = tid & 3;
+        T x[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            T a = 0;
+#pragma unroll
+            for (int q = 0; q < s; q++) a += Ls[p + q][p + s] * x[q];
+            x[s] = s < t ? (T)0 : s == t ? rdg[p + s] : -rdg[p + s] * a;
+        }
+#pragma unroll
+        for (int s = 0; s < 4; s++) Xs[p + t][p + s] = x[s];
+    }
     __syncthreads();
-    // inverse by recursive doubling (block sizes 1, 2, .., 32 as compile-time levels)
-    diag_inv_level<T, 1>(Ls, Xs, Ts, tid);
-    diag_inv_level<T, 2>(Ls, Xs, Ts, tid);
-    diag_inv_level<T, 4>(Ls, Xs, Ts, tid);
-    diag_inv_level<T, 8>(Ls, Xs, Ts, tid);
-    diag_inv_level<T, 16>(Ls, Xs, Ts, tid);
-    diag_inv_level<T, 32>(Ls, Xs, Ts, tid);
+    diag_inv_simt<T, 4>(Ls, Xs, Ts, tid);
+    diag_inv_simt<T, 8>(Ls, Xs, Ts, tid);
+    diag_inv_mfma<T, 16>(Ls, Xs, Ts, tid);
+    diag_inv_mfma<T, 32>(Ls, Xs, Ts, tid);
 #pragma unroll
     for (int u = 0; u < 16; u++) {
-        const int c = cb + 4 * u;
-        if (ri < nb && c < nb) Ab[ri + (int64_t)c * n] = v[u];
-        Di[ri + 64 * c] = (ri < nb && c < nb && ri >= c) ? Xs[c][ri] : (T)0;
+        const int c = 4 * u + w;
+        if (r < nb && c < nb) Ab[r + (int64_t)c * n] = v[u];
+        Di[r + 64 * c] = (r < nb && c < nb && r >= c) ? Xs[c][r] : (T)0;
     }
     pm_out = pm;
     return 0;
