@@ -40,15 +40,22 @@ template <> struct MF<float> {
 // C = alpha op(A) op(B) + beta C for one BM x BN tile of one item. Four waves in a 2 x 2 grid,
 // each owning a (BM/2) x (BN/2) quadrant of 16 x 16 MFMA tiles. The MFMA is fed with the B tile
 // as its A operand and the A tile as its B operand, so each output register row is a run of 16
-// consecutive rows of column-major C (coalesced stores). The next K slab is fetched into
-// registers while the current one is multiplied from LDS.
+// consecutive rows of column-major C (coalesced stores). LDS holds two K slabs: slab s + 1 is
+// fetched into registers before slab s is multiplied and written to the other buffer after it,
+// so each slab costs one barrier. Every thread's loads keep one coordinate fixed and advance the
+// other by a constant stride; a load outside the item reads the item's first element and is
+// zeroed when it is written to LDS (no branches around the loads, which stay in flight across
+// the products). Rows of a slab are padded to BM + 1 (BN + 1) elements: 16 consecutive k rows
+// written by consecutive threads then fall in distinct banks.
 template <typename T, int TA, int TB, int BM, int BN>
 __global__ __launch_bounds__(256) void k_gemm(const GemmItem *__restrict__ items, const int *__restrict__ pre, int nitems,
                                               const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, T alpha,
                                               T beta, const int *__restrict__ info) {
     typedef typename MF<T>::v4 v4;
+    constexpr int BK = GBK;
     constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
-    constexpr int LA = BM * GBK / 256, LB = BN * GBK / 256;
+    constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
+    static_assert(256 % BM == 0 && 256 % BN == 0 && 256 % BK == 0 && LA <= 32 && LB <= 32, "tile shape");
     const int item = flat_item(pre, nitems, blockIdx.x);
     const GemmItem it = items[item];
     if (it.guard >= 0 && info && info[it.guard] != 0) return;
@@ -64,9 +71,9 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmItem *__restrict__ items
     if (it.tri & 4) ke = min(ke, r0 + BM);       // op(A) lower triangular: A(i,k) = 0 for k > i
     if (it.tri & 8) kb = max(kb, c0);            // op(B) lower triangular: B(k,j) = 0 for k < j
     if (it.tri & 16) ke = min(ke, c0 + BN);      // op(B) upper triangular: B(k,j) = 0 for k > j
-    kb = (kb / GBK) * GBK;
-    __shared__ T As[GBK][BM + 4];
-    __shared__ T Bs[GBK][BN + 4];
+    kb = (kb / BK) * BK;
+    __shared__ T As[2][BK][BM + 1];
+    __shared__ T Bs[2][BK][BN + 1];
     const T *a = A + it.a, *b = B + it.b;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, wm = wave & 1, wn = wave >> 1;
     const int M = it.M, N = it.N, lda = it.lda, ldb = it.ldb;
@@ -77,52 +84,84 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmItem *__restrict__ items
         for (int j = 0; j < FN; j++)
 #pragma unroll
             for (int r = 0; r < 4; r++) acc[i][j][r] = 0;
+    // load geometry of element e = tid + 256 r: op(A)(i, k) and op(B)(k, j), with the coordinate
+    // that advances with r stepping SA (SB) per r
+    constexpr int SA = TA == 0 ? 256 / BM : 256 / BK, SB = TB == 0 ? 256 / BK : 256 / BN;
+    const int ia = TA == 0 ? tid % BM : tid / BK, ka = TA == 0 ? tid / BM : tid % BK;
+    const int jb = TB == 0 ? tid / BK : tid % BN, kq = TB == 0 ? tid % BK : tid / BN;
+    const int64_t stA = (int64_t)SA * lda, stB = (int64_t)SB * ldb;
+    uint32_t va = 0, vb = 0;                     // elements whose row (column) lies inside the item
+#pragma unroll
+    for (int r = 0; r < LA; r++) { const int i = TA == 0 ? ia : ia + SA * r; va |= (r0 + i < M ? 1u : 0u) << r; }
+#pragma unroll
+    for (int r = 0; r < LB; r++) { const int j = TB == 0 ? jb + SB * r : jb; vb |= (c0 + j < N ? 1u : 0u) << r; }
     T ra[LA], rb[LB];
+    uint32_t ma = 0, mb = 0;                     // loads of the slab in flight that are inside the item
     auto fetch = [&](int k0) {
+        const bool full = k0 + BK <= ke;
+        const T *pa = TA == 0 ? a + (r0 + ia) + (int64_t)(k0 + ka) * lda : a + (k0 + ka) + (int64_t)(r0 + ia) * lda;
+        const T *pb = TB == 0 ? b + (k0 + kq) + (int64_t)(c0 + jb) * ldb : b + (c0 + jb) + (int64_t)(k0 + kq) * ldb;
+        ma = 0; mb = 0;
 #pragma unroll
         for (int r = 0; r < LA; r++) {
-            const int e = tid + 256 * r;
-            int i, k;
-            if (TA == 0) { i = e % BM; k = e / BM; } else { k = e % GBK; i = e / GBK; }
-            const int gi = r0 + i, gk = k0 + k;
-            ra[r] = (gi < M && gk < ke) ? (TA == 0 ? a[gi + (int64_t)gk * lda] : a[gk + (int64_t)gi * lda]) : (T)0;
+            const int gk = TA == 0 ? k0 + ka + SA * r : k0 + ka;
+            const bool ok = ((va >> r) & 1) && (full || gk < ke);
+            ma |= (ok ? 1u : 0u) << r;
+            ra[r] = *(ok ? pa + r * stA : a);
         }
 #pragma unroll
         for (int r = 0; r < LB; r++) {
-            const int e = tid + 256 * r;
-            int j, k;
-            if (TB == 0) { k = e % GBK; j = e / GBK; } else { j = e % BN; k = e / BN; }
-            const int gj = c0 + j, gk = k0 + k;
-            rb[r] = (gj < N && gk < ke) ? (TB == 0 ? b[gk + (int64_t)gj * ldb] : b[gj + (int64_t)gk * ldb]) : (T)0;
+            const int gk = TB == 0 ? k0 + kq : k0 + kq + SB * r;
+            const bool ok = ((vb >> r) & 1) && (full || gk < ke);
+            mb |= (ok ? 1u : 0u) << r;
+            rb[r] = *(ok ? pb + r * stB : b);
         }
     };
-    if (kb < ke) fetch(kb);
-    for (int k0 = kb; k0 < ke; k0 += GBK) {
+    auto store = [&](int buf) {
 #pragma unroll
         for (int r = 0; r < LA; r++) {
             const int e = tid + 256 * r;
-            if (TA == 0) As[e / BM][e % BM] = ra[r]; else As[e % GBK][e / GBK] = ra[r];
+            const T x = ((ma >> r) & 1) ? ra[r] : (T)0;
+            if (TA == 0) As[buf][e / BM][e % BM] = x; else As[buf][e % BK][e / BK] = x;
         }
 #pragma unroll
         for (int r = 0; r < LB; r++) {
             const int e = tid + 256 * r;
-            if (TB == 0) Bs[e % GBK][e / GBK] = rb[r]; else Bs[e / BN][e % BN] = rb[r];
+            const T x = ((mb >> r) & 1) ? rb[r] : (T)0;
+            if (TB == 0) Bs[buf][e % BK][e / BK] = x; else Bs[buf][e / BN][e % BN] = x;
         }
-        __syncthreads();
-        if (k0 + GBK < ke) fetch(k0 + GBK);
+    };
+    const int nk = kb < ke ? (ke - kb + BK - 1) / BK : 0;
+    if (nk > 0) { fetch(kb); store(0); }
+    __syncthreads();
+    for (int s = 0; s < nk; s++) {
+        const int buf = s & 1;
+        if (s + 1 < nk) fetch(kb + (s + 1) * BK);
+        {
+            // the operands of step kk + 1 are read from LDS while step kk's products issue
+            T av[2][FM], bv[2][FN];
+            const int l16 = lane & 15, g4 = lane >> 4;
 #pragma unroll
-        for (int kk = 0; kk < GBK; kk += 4) {
-            T av[FM], bv[FN];
-            const int kr = kk + (lane >> 4), l16 = lane & 15;
+            for (int q = 0; q < FM; q++) av[0][q] = As[buf][g4][wm * WM + q * 16 + l16];
 #pragma unroll
-            for (int s = 0; s < FM; s++) av[s] = As[kr][wm * WM + s * 16 + l16];
+            for (int q = 0; q < FN; q++) bv[0][q] = Bs[buf][g4][wn * WN + q * 16 + l16];
 #pragma unroll
-            for (int s = 0; s < FN; s++) bv[s] = Bs[kr][wn * WN + s * 16 + l16];
+            for (int kk = 0; kk < BK / 4; kk++) {
+                const int cu = kk & 1, nx = cu ^ 1;
+                if (kk + 1 < BK / 4) {
+                    const int kr = 4 * (kk + 1) + g4;
 #pragma unroll
-            for (int si = 0; si < FM; si++)
+                    for (int q = 0; q < FM; q++) av[nx][q] = As[buf][kr][wm * WM + q * 16 + l16];
 #pragma unroll
-                for (int sj = 0; sj < FN; sj++) acc[si][sj] = MF<T>::mma(bv[sj], av[si], acc[si][sj]);
+                    for (int q = 0; q < FN; q++) bv[nx][q] = Bs[buf][kr][wn * WN + q * 16 + l16];
+                }
+#pragma unroll
+                for (int si = 0; si < FM; si++)
+#pragma unroll
+                    for (int sj = 0; sj < FN; sj++) acc[si][sj] = MF<T>::mma(bv[cu][sj], av[cu][si], acc[si][sj]);
+            }
         }
+        if (s + 1 < nk) store(buf ^ 1);
         __syncthreads();
     }
     T *c = C + it.c;
