@@ -214,6 +214,17 @@ static GemmPlan gemm_plan(std::vector<char> &buf, const GemmItem *items, int nit
     P.n = nitems; P.flops = 0; P.bytes = 0;
     P.items = buf_align(buf);
     buf.insert(buf.end(), (const char *)items, (const char *)(items + nitems));
+    // 128 x 128 tiles for the items large enough to fill them, provided the launch then still has
+    // a tile for every CU (fewer 128-wide tiles run K-serial on a fraction of the chip: 64 x 64
+    // tiles are faster there); 64 x 64 for the rest
+    int64_t nbig = 0;
+    for (int q = 0; q < nitems; q++) {
+        const GemmItem &g = items[q];
+        if (g.M < 192 || g.N < 192) continue;
+        const int64_t tm = (g.M + 127) / 128, tn = (g.N + 127) / 128;
+        nbig += ((g.tri & 1) && g.M == g.N) ? tm * (tm + 1) / 2 : tm * tn;
+    }
+    const bool use_big = nbig >= 256;
     for (int big = 0; big < 2; big++) {
         const int TB = big ? 128 : 64;
         P.pre[big] = buf_align(buf);
@@ -222,8 +233,7 @@ static GemmPlan gemm_plan(std::vector<char> &buf, const GemmItem *items, int nit
         for (int q = 0; q < nitems; q++) {
             const GemmItem &g = items[q];
             ((int *)(buf.data() + P.pre[big]))[q] = (int)tot;
-            // 128 x 128 tiles for the items large enough to fill them, 64 x 64 for the rest
-            if ((g.M >= 192 && g.N >= 192) != (big == 1) || g.M <= 0 || g.N <= 0) continue;
+            if ((use_big && g.M >= 192 && g.N >= 192) != (big == 1) || g.M <= 0 || g.N <= 0) continue;
             const int64_t tm = (g.M + TB - 1) / TB, tn = (g.N + TB - 1) / TB;
             tot += ((g.tri & 1) && g.M == g.N) ? tm * (tm + 1) / 2 : tm * tn;
         }
