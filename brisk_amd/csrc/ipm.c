@@ -119,6 +119,12 @@ static SAux *aux_of(Solve *S) {
     return NULL;
 }
 
+/* Mwork: the factorization's n x n scratch, and between factorizations the partial sums of the
+ * symmetric products with M (T x T slots of 64, T = ceil(m / 64)) */
+static int64_t mwork_elems(int64_t m) {
+    const int64_t t = (m + 63) / 64, a = m * m, b = t * t * 64;
+    return a > b ? a : b;
+}
 int64_t solve_heap_bytes(const HProb *P) {
     const int64_t T = P->T, m = P->m;
     int64_t dinv = 0, lz = 0;
@@ -127,7 +133,7 @@ int64_t solve_heap_bytes(const HProb *P) {
     b += 19 * (T * 8 + 256);                              /* block arena arrays */
     b += 2 * (dinv * 8 + 256);
     b += 2 * (BK_LZ_K + 3) * lz * 8 + 1024;
-    b += 5 * (m * m * 8 + 256) + (((m + 63) / 64) * 4096 * 8 + 256);
+    b += 4 * (m * m * 8 + 256) + (mwork_elems(m) * 8 + 256) + (((m + 63) / 64) * 4096 * 8 + 256);
     b += 22 * (m * 8 + 256) + NSC * 8 + 256;
     b += (m + 1) * 4 + (int64_t)P->nent * (sizeof(SEnt) + 12) + (int64_t)P->nb * sizeof(BlkRef) + (P->npos + 1) * 12 + 2048;
     b += (m + 1) * 4 + (int64_t)(P->ngrp + 1) * sizeof(GrpRef) + 2 * (P->gp_tot * 8 + 256) + 512;
@@ -160,7 +166,7 @@ int solve_setup(Solve *S) {
     ALLOC(W2, T * 8); ALLOC(C, T * 8); ALLOC(Xb, T * 8);
     ALLOC(DinvX, dinv * 8); ALLOC(DinvZ, dinv * 8);
     ALLOC(lzQ, 2 * (int64_t)BK_LZ_K * lz * 8); ALLOC(lzw, 2 * lz * 8); ALLOC(lzt1, 2 * lz * 8); ALLOC(lzt2, 2 * lz * 8);
-    ALLOC(M, m * m * 8); ALLOC(MF, m * m * 8); ALLOC(MLi, m * m * 8); ALLOC(MLiT, m * m * 8); ALLOC(Mwork, m * m * 8);
+    ALLOC(M, m * m * 8); ALLOC(MF, m * m * 8); ALLOC(MLi, m * m * 8); ALLOC(MLiT, m * m * 8); ALLOC(Mwork, mwork_elems(m) * 8);
     ALLOC(MDinv, ((m + 63) / 64) * 4096 * 8); ALLOC(Dm, m * 8);
     ALLOC(y, m * 8); ALLOC(dy, m * 8); ALLOC(rp, m * 8); ALLOC(rhs, m * 8); ALLOC(a0, m * 8); ALLOC(h, m * 8); ALLOC(qv, m * 8);
     ALLOC(ev, m * 8); ALLOC(wv, m * 8); ALLOC(bvec, m * 8); ALLOC(duv, m * 8); ALLOC(pr, m * 8); ALLOC(pz, m * 8); ALLOC(pp, m * 8); ALLOC(pq, m * 8);
@@ -857,17 +863,21 @@ static int schur_prec(Solve **S, int ns, const int *act, size_t fr, size_t fz) {
 /* y = alpha M x + beta y in FP64: a product with the stored M where it is exact, else the
  * operator form M x = A(X A*(x) Zi) on the f64 data (reference mvfree_apply, solver.c:4755-4773) */
 static int schur_mv(Solve **S, int ns, const int *act, size_t fx, size_t fy, double alpha, double beta) {
-    GemvL_clear(&g_vl);
-    int nop = 0;
+    /* stored M: the symmetric product from its lower triangle, with the partial sums in Mwork
+     * (free between factorizations) */
+    SymvItem *sv = malloc(sizeof(SymvItem) * (ns + 1));
+    int nsv = 0, nop = 0;
     for (int q = 0; q < ns; q++) {
         if (act && !act[q]) continue;
         Solve *s = S[q];
         if (!s->m_exact) { nop++; continue; }
-        GemvItem g = { s->P.m, s->P.m, s->P.m, s->M, F_(s, fx), F_(s, fy), 0 };
-        GemvL_add(&g_vl, g);
+        SymvItem g = { s->P.m, s->P.m, s->M, F_(s, fx), F_(s, fy), s->Mwork };
+        sv[nsv++] = g;
     }
     double *H = heapp();
-    if (g_vl.n && bk_gemv_f64(1, alpha, beta, H, H, H, g_vl.v, g_vl.n)) return -1;
+    const int rs = nsv ? bk_symv_f64(alpha, beta, H, H, H, H, sv, nsv) : 0;
+    free(sv);
+    if (rs) return -1;
     if (!nop) return 0;
     int *op = calloc(ns, sizeof(int));
     for (int q = 0; q < ns; q++) op[q] = (!act || act[q]) && !S[q]->m_exact;
@@ -948,6 +958,11 @@ static int schur_solve_(Solve **S, int ns, const int *act0, size_t frhs, size_t 
             }
         }
         for (int k = 0; any; k++) {
+            if (bk_prof_on()) {        /* profile counters: PCG rounds (class 52), active solves summed over rounds (53) */
+                int na = 0;
+                for (int q = 0; q < ns; q++) na += act[q];
+                bk_prof_host(52, 0.0); bk_prof_host(53, 1e-3 * na);
+            }
             /* x += a (u + b p), r -= a (w + b s) (old p, s); then p = u + b p, s = w + b s */
             LinL_clear(&g_ll);
             for (int q = 0; q < ns; q++) {

@@ -595,3 +595,93 @@ extern "C" int bk_gemv_f32(int trans, float alpha, float beta, const float *A, c
 extern "C" int bk_gemv_mixed(int trans, double alpha, double beta, const float *A, const double *x, double *y, const GemvItem *it, int n_it) {
     return gemv_run<float, double, double, double>(trans, alpha, beta, A, x, y, it, n_it);
 }
+
+// ---- symmetric matrix-vector product from the lower triangle ----------------------------------
+// One workgroup per lower 64 x 64 tile (I, J), I >= J, enumerated row-major over the lower
+// triangle. The tile is staged in LDS (coalesced column reads); threads 0-63 form the row part
+// r(i) = sum_c A(64I + i, 64J + c) x(64J + c), threads 64-127 the column part
+// c(j) = sum_r A(64I + r, 64J + j) x(64I + r) (I > J only). A diagonal tile reads its lower
+// triangle only and mirrors it.
+__global__ __launch_bounds__(256) void k_symv_tiles(const SymvItem *__restrict__ items, const int *__restrict__ pre, int nit,
+                                                    const double *__restrict__ A, const double *__restrict__ X, double *__restrict__ P) {
+    const int q = flat_item(pre, nit, blockIdx.x);
+    const SymvItem it = items[q];
+    int I, J;
+    tri_index(blockIdx.x - pre[q], I, J);
+    const int n = it.n, nT = (n + 63) / 64, tid = threadIdx.x;
+    __shared__ double Ts[64][65];    // Ts[c][r] = A(64 I + r, 64 J + c)
+    __shared__ double xs[2][64];     // x of the column block J, of the row block I
+    const double *a = A + it.a;
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int e = tid + 256 * u, r = e & 63, c = e >> 6, gi = 64 * I + r, gj = 64 * J + c;
+        Ts[c][r] = (gi < n && gj < n) ? a[gi + (int64_t)gj * it.ld] : 0.0;
+    }
+    if (tid < 64) xs[0][tid] = 64 * J + tid < n ? X[it.x + 64 * J + tid] : 0.0;
+    else if (tid < 128) xs[1][tid - 64] = 64 * I + tid - 64 < n ? X[it.x + 64 * I + tid - 64] : 0.0;
+    __syncthreads();
+    double *p = P + it.p;
+    if (tid < 64) {
+        const int i = tid;
+        double s0 = 0, s1 = 0;
+        if (I > J) {
+#pragma unroll 16
+            for (int c = 0; c < 64; c += 2) { s0 += Ts[c][i] * xs[0][c]; s1 += Ts[c + 1][i] * xs[0][c + 1]; }
+        } else {
+#pragma unroll 16
+            for (int c = 0; c < 64; c += 2) {
+                s0 += (c <= i ? Ts[c][i] : Ts[i][c]) * xs[0][c];
+                s1 += (c + 1 <= i ? Ts[c + 1][i] : Ts[i][c + 1]) * xs[0][c + 1];
+            }
+        }
+        p[((int64_t)I * nT + J) * 64 + i] = s0 + s1;
+    } else if (tid < 128 && I > J) {
+        const int j = tid - 64;
+        double s0 = 0, s1 = 0;
+#pragma unroll 16
+        for (int r = 0; r < 64; r += 2) { s0 += Ts[j][r] * xs[1][r]; s1 += Ts[j][r + 1] * xs[1][r + 1]; }
+        p[((int64_t)J * nT + I) * 64 + j] = s0 + s1;
+    }
+}
+// y(64 a + i) = alpha sum_b P(a, b)(i) + beta y(64 a + i): one 64-thread workgroup per row block,
+// the slots summed in the order b = 0 .. T - 1
+__global__ __launch_bounds__(64) void k_symv_reduce(const SymvItem *__restrict__ items, const int *__restrict__ pre, int nit,
+                                                    const double *__restrict__ P, double *__restrict__ Y, double alpha, double beta) {
+    const int q = flat_item(pre, nit, blockIdx.x);
+    const SymvItem it = items[q];
+    const int a = blockIdx.x - pre[q], n = it.n, nT = (n + 63) / 64, i = threadIdx.x, row = 64 * a + i;
+    if (row >= n) return;
+    const double *p = P + it.p + (int64_t)a * nT * 64 + i;
+    double s = 0;
+    for (int b = 0; b < nT; b++) s += p[(int64_t)b * 64];
+    double *y = Y + it.y + row;
+    *y = beta != 0.0 ? alpha * s + beta * *y : alpha * s;
+}
+extern "C" int bk_symv_f64(double alpha, double beta, const double *A, const double *x, double *y, double *p, const SymvItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    std::vector<int64_t> ct(n_it), cr(n_it);
+    double by = 0;
+    for (int q = 0; q < n_it; q++) {
+        const int64_t nT = (it[q].n + 63) / 64;
+        ct[q] = it[q].n > 0 ? nT * (nT + 1) / 2 : 0;
+        cr[q] = it[q].n > 0 ? nT : 0;
+        by += 8.0 * ((double)it[q].n * (it[q].n + 1) / 2 + 3.0 * it[q].n);
+    }
+    const SymvItem *d;
+    const int *pt;
+    const int64_t nt = flat_stage_items(it, sizeof(SymvItem) * n_it, ct, (const void **)&d, &pt);
+    if (nt < 0) return -1;
+    if (nt == 0) return 0;
+    const int *pr;
+    const int64_t nr = flat_stage(cr, &pr);
+    if (nr <= 0) return -1;
+    const int tok = bk_prof_begin();
+    hipLaunchKernelGGL(k_symv_tiles, dim3((unsigned)nt), dim3(256), 0, g_bk_stream, d, pt, n_it, A, x, p);
+    hipLaunchKernelGGL(k_symv_reduce, dim3((unsigned)nr), dim3(64), 0, g_bk_stream, d, pr, n_it, (const double *)p, y, alpha, beta);
+    if (tok >= 0) {
+        double fl = 0;
+        for (int q = 0; q < n_it; q++) fl += 2.0 * it[q].n * it[q].n;
+        bk_prof_end(tok, 15, fl, by);
+    }
+    return BK_LAUNCH_CHECK("symv");
+}

@@ -15,6 +15,11 @@ static hipEvent_t g_sync_ev;
 
 // staging ring: host pinned + device, recycled at every synchronisation
 static char *g_ring_h = nullptr, *g_ring_d = nullptr;
+// transfers up to KCOPY_MAX bytes are kernels on the library stream that read or write pinned
+// host memory directly (k_pull, k_push): a DMA transfer between kernels costs the compute queue
+// far more idle time than the few microseconds of such a kernel
+#define KCOPY_MAX ((size_t)1 << 20)
+static char *g_bounce = nullptr;      // pinned: destination of small device-to-host reads
 static size_t g_ring_cap = 0, g_ring_pos = 0;
 #define CACHE_CAP ((size_t)32 << 20)
 #define CACHE_CHUNK ((size_t)64 << 10)
@@ -53,6 +58,7 @@ extern "C" int bk_init(void) {
     g_ring_pos = 0;
     if (bk_check(hipHostMalloc((void **)&g_cache_h, CACHE_CAP, hipHostMallocDefault), "hipHostMalloc cache")) return -1;
     if (bk_check(hipMalloc((void **)&g_cache_d, CACHE_CAP), "hipMalloc cache")) return -1;
+    if (bk_check(hipHostMalloc((void **)&g_bounce, KCOPY_MAX, hipHostMallocDefault), "hipHostMalloc bounce")) return -1;
     g_init = 1;
     return 0;
 }
@@ -94,13 +100,46 @@ extern "C" int bk_sync(void) {
     g_epoch++;
     return r;
 }
+// byte copy by a kernel: 16- or 8-byte words when both ends are aligned to them, then the tail
+__global__ void k_kcopy(unsigned char *__restrict__ dst, const unsigned char *__restrict__ src, size_t bytes) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, nt = (size_t)gridDim.x * blockDim.x;
+    const uintptr_t al = (uintptr_t)dst | (uintptr_t)src;
+    size_t head = 0;
+    if ((al & 15) == 0) {
+        const size_t n16 = bytes / 16;
+        for (size_t q = t; q < n16; q += nt) ((uint4 *)dst)[q] = ((const uint4 *)src)[q];
+        head = n16 * 16;
+    } else if ((al & 7) == 0) {
+        const size_t n8 = bytes / 8;
+        for (size_t q = t; q < n8; q += nt) ((unsigned long long *)dst)[q] = ((const unsigned long long *)src)[q];
+        head = n8 * 8;
+    }
+    for (size_t q = head + t; q < bytes; q += nt) dst[q] = src[q];
+}
+static int kcopy(void *dst, const void *src, size_t bytes, const char *what) {
+    const size_t g = (bytes + 4095) / 4096;
+    hipLaunchKernelGGL(k_kcopy, dim3((unsigned)(g < 64 ? (g ? g : 1) : 64)), dim3(256), 0, g_bk_stream, (unsigned char *)dst,
+                       (const unsigned char *)src, bytes);
+    return bk_check(hipGetLastError(), what);
+}
+// host (pinned) -> device on the stream
+static int pull(void *dst, const void *pinned, size_t bytes, const char *what) {
+    if (bytes <= KCOPY_MAX) return kcopy(dst, pinned, bytes, what);
+    return bk_check(hipMemcpyAsync(dst, pinned, bytes, hipMemcpyHostToDevice, g_bk_stream), what);
+}
 extern "C" int bk_h2d(void *dst, const void *src, size_t bytes) {
     if (!bytes) return 0;
+    if (bytes <= KCOPY_MAX) return bk_h2d_async(dst, src, bytes) || bk_sync() ? -1 : 0;
     if (bk_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_bk_stream), "hipMemcpy H2D")) return -1;
     return bk_sync();
 }
 extern "C" int bk_d2h(void *dst, const void *src, size_t bytes) {
     if (!bytes) return bk_sync();
+    if (bytes <= KCOPY_MAX) {
+        if (kcopy(g_bounce, src, bytes, "d2h") || bk_sync()) return -1;
+        memcpy(dst, g_bounce, bytes);
+        return 0;
+    }
     if (bk_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_bk_stream), "hipMemcpy D2H")) return -1;
     return bk_sync();
 }
@@ -176,7 +215,7 @@ static void *stage_cached(const void *host, size_t bytes) {
     const size_t off = g_cache_pos;
     char *hp = g_cache_h + off, *dp = g_cache_d + off;
     memcpy(hp, host, bytes);
-    if (bk_check(hipMemcpyAsync(dp, hp, bytes, hipMemcpyHostToDevice, g_bk_stream), "stage")) return nullptr;
+    if (pull(dp, hp, bytes, "stage")) return nullptr;
     g_cache_pos += need;
     // insert (replacing a dead entry in the probe window, else the first)
     slot = h % CACHE_TAB;
@@ -200,7 +239,7 @@ extern "C" void *bk_stage(const void *host, size_t bytes) {
     if (g_ring_pos + need > g_ring_cap) bk_sync();            // every staged copy has completed: recycle
     char *h = g_ring_h + g_ring_pos, *d = g_ring_d + g_ring_pos;
     memcpy(h, host, bytes);
-    if (bk_check(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, g_bk_stream), "stage")) return nullptr;
+    if (pull(d, h, bytes, "stage")) return nullptr;
     g_stage_n += 1; g_stage_bytes += (double)bytes;
     g_ring_pos += need;
     return d;
@@ -214,7 +253,7 @@ extern "C" int bk_h2d_async(void *dst, const void *src, size_t bytes) {
     if (g_ring_pos + need > g_ring_cap) bk_sync();
     char *h = g_ring_h + g_ring_pos;
     memcpy(h, src, bytes);
-    if (bk_check(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, g_bk_stream), "hipMemcpy H2D async")) return -1;
+    if (pull(dst, h, bytes, "hipMemcpy H2D async")) return -1;
     g_ring_pos += need;
     return 0;
 }

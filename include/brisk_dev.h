@@ -48,7 +48,7 @@ int bk_prof_on(void);                            /* profiling enabled */
 void bk_prof_host(int cls, double seconds);      /* add a host-side interval to class cls (40-59) */
 /* classes: 0-3 k_gemm<double, ta, tb> (2 ta + tb), 4-7 k_gemm<float, ta, tb>, 8 k_potrf_diag,
  * 9 k_gemv, 10 Schur assembly, 11 constraint operators, 12 Lanczos steps, 13 dot products,
- * 14 linear combinations */
+ * 14 linear combinations, 15 k_symv (tiles and reduction) */
 
 /* ---- batched GEMM: C_it = alpha op(A_it) op(B_it) + beta C_it ---- */
 typedef struct {
@@ -93,6 +93,13 @@ int bk_gemv_f64(int trans, double alpha, double beta, const double *A, const dou
 int bk_gemv_f32(int trans, float alpha, float beta, const float *A, const float *x, float *y, const GemvItem *it, int n_it);
 /* mixed: y (f64) = alpha op(A) x + beta y with A in f32, x in f64 (products and sums in f64) */
 int bk_gemv_mixed(int trans, double alpha, double beta, const float *A, const double *x, double *y, const GemvItem *it, int n_it);
+/* symmetric: y = alpha A x + beta y for symmetric n x n A of which only the lower triangle is
+ * read (half the traffic of bk_gemv_f64). Each 64 x 64 lower tile (I, J) writes its row part
+ * A_IJ x_J and, for I > J, its column part A_IJ' x_I to the partial slots (I, J) and (J, I) of
+ * p (T x T slots of 64 doubles, T = ceil(n / 64)); each row block then sums its T slots in a
+ * fixed order, so the result depends on the item's own data only. Offsets in doubles. */
+typedef struct { int n, ld; int64_t a, x, y, p; } SymvItem;
+int bk_symv_f64(double alpha, double beta, const double *A, const double *x, double *y, double *p, const SymvItem *it, int n_it);
 
 /* ---- the device heap: one allocation; every array is an offset from its base ---- */
 int bk_heap_init(size_t bytes);                  /* (re)allocate; 0 on success */

@@ -20,6 +20,10 @@ class GemvItem(ctypes.Structure):
     _fields_ = [("M", ctypes.c_int), ("N", ctypes.c_int), ("lda", ctypes.c_int), ("a", I64), ("x", I64), ("y", I64), ("tri", ctypes.c_int)]
 
 
+class SymvItem(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("ld", ctypes.c_int), ("a", I64), ("x", I64), ("y", I64), ("p", I64)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -48,6 +52,7 @@ def lib():
         for nm in ("bk_gemv_f64", "bk_gemv_f32", "bk_gemv_mixed"):
             sc = ctypes.c_float if nm == "bk_gemv_f32" else ctypes.c_double
             getattr(L, nm).argtypes = [ctypes.c_int, sc, sc, P, P, P, P, ctypes.c_int]
+        L.bk_symv_f64.argtypes = [ctypes.c_double, ctypes.c_double, P, P, P, P, P, ctypes.c_int]
         if L.bk_init() != 0:
             raise RuntimeError("bk_init failed: %s" % L.bk_error())
         _lib = L

@@ -2,7 +2,7 @@
 triangular inverse, matrix-vector products) against numpy/LAPACK on the same inputs."""
 import numpy as np
 import pytest
-from _gpu_util import lib, DBuf, GemmItem, MItem, GemvItem, items_array
+from _gpu_util import lib, DBuf, GemmItem, MItem, GemvItem, SymvItem, items_array
 
 pytestmark = pytest.mark.gpu
 
@@ -155,3 +155,29 @@ def test_gemv_batched(trans):
     out = dy.get()
     for (o, ref) in refs:
         assert np.allclose(out[o:o + ref.size], ref, rtol=1e-12, atol=1e-12)
+
+
+def test_symv_lower_batched():
+    """Symmetric products from the lower triangle only (the upper holds garbage), ragged batch,
+    ld > n, against numpy; each item's partial slots are its own (ceil(n/64)^2 * 64 doubles)."""
+    rng = np.random.default_rng(11)
+    sizes = [1, 7, 63, 64, 65, 130, 300]
+    As, xs, ys, rows, refs = [], [], [], [], []
+    oa = ox = oy = op = 0
+    for n in sizes:
+        ld = n + 3
+        S = rng.standard_normal((n, n)); S = S + S.T
+        A = np.full((ld, n), 1e30)
+        A[:n, :] = np.where(np.tril(np.ones((n, n), bool)), S, 1e30)
+        x = rng.standard_normal(n); y = rng.standard_normal(n)
+        refs.append((oy, 0.7 * S @ x - 0.3 * y))
+        t = (n + 63) // 64
+        rows.append((n, ld, oa, ox, oy, op))
+        As.append(A.flatten(order="F")); xs.append(x); ys.append(y)
+        oa += A.size; ox += n; oy += n; op += max(n * n, t * t * 64)
+    dA = DBuf(np.concatenate(As)); dx = DBuf(np.concatenate(xs)); dy = DBuf(np.concatenate(ys)); dp = DBuf(np.zeros(op))
+    it = items_array(SymvItem, rows)
+    assert lib().bk_symv_f64(0.7, -0.3, dA.p, dx.p, dy.p, dp.p, it, len(rows)) == 0
+    out = dy.get()
+    for (o, ref) in refs:
+        assert np.allclose(out[o:o + ref.size], ref, rtol=1e-12, atol=1e-11)
