@@ -635,64 +635,6 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
-#define LZT 1024
-__global__ __launch_bounds__(LZT) void k_lz_step(const LzItem *__restrict__ it, int nit, int j, double *__restrict__ H) {
-    const int q = blockIdx.x;
-    if (q >= nit) return;
-    const LzItem L = it[q];
-    if (j >= L.K) return;
-    constexpr int NW = LZT / 64;
-    __shared__ double sh[NW][32];
-    __shared__ double cf[32];
-    const int n = L.n, t = threadIdx.x, lane = t & 63, wv = t >> 6;
-    double *Q = H + L.Q, *w = H + L.w, *ab = H + L.ab;
-    // classical Gram-Schmidt twice against q_0 .. q_j on w = T q_j: the first pass's coefficient
-    // of q_j is alpha_j; all coefficients of a pass in one sweep (independent loads), summed over
-    // the lanes and then over the waves in a fixed order
-    double alpha = 0;
-    for (int pass = 0; pass < 2; pass++) {
-        double c[32];
-#pragma unroll
-        for (int p = 0; p < 32; p++) c[p] = 0;
-        for (int i = t; i < n; i += LZT) {
-            const double wi = w[i];
-#pragma unroll
-            for (int p = 0; p < 32; p++) if (p <= j) c[p] += Q[(int64_t)p * n + i] * wi;
-        }
-#pragma unroll
-        for (int p = 0; p < 32; p++) {
-            if (p <= j) {
-                const double v = wave_sum(c[p]);
-                if (lane == 0) sh[wv][p] = v;
-            }
-        }
-        __syncthreads();
-        if (t <= j) { double v = 0; for (int u = 0; u < NW; u++) v += sh[u][t]; cf[t] = v; }
-        __syncthreads();
-        if (pass == 0) alpha = cf[j];
-        for (int i = t; i < n; i += LZT) {
-            double v = w[i];
-#pragma unroll
-            for (int p = 0; p < 32; p++) if (p <= j) v -= cf[p] * Q[(int64_t)p * n + i];
-            w[i] = v;
-        }
-        __syncthreads();
-    }
-    double b2 = 0;
-    for (int i = t; i < n; i += LZT) b2 += w[i] * w[i];
-    b2 = wave_sum(b2);
-    if (lane == 0) sh[wv][0] = b2;
-    __syncthreads();
-    double bb = 0;
-    for (int u = 0; u < NW; u++) bb += sh[u][0];
-    const double beta = sqrt(bb);
-    if (t == 0) { ab[j] = alpha; ab[32 + j] = beta; }
-    if (j + 1 < L.K) {
-        const double inv = beta > 1e-300 ? 1.0 / beta : 0.0;
-        double *qn = Q + (int64_t)(j + 1) * n;
-        for (int i = t; i < n; i += LZT) qn[i] = w[i] * inv;
-    }
-}
 // ---- fused Lanczos for blocks of order <= 256: one workgroup per block, one row per thread -----
 #define LZF 256
 // sum over the workgroup of np per-thread values (p <= j), fixed order: lanes, then the 4 waves
@@ -880,11 +822,128 @@ extern "C" int bk_lanczos_start(const LzItem *it, int n_it) {
     hipLaunchKernelGGL(k_lz_start, dim3(n_it), dim3(256), 0, g_bk_stream, d, n_it, (double *)g_heap);
     return BK_LAUNCH_CHECK("k_lz_start");
 }
+// ---- Lanczos step for blocks of order > 256 over many workgroups ----------------------------
+// Workgroup b of item q owns rows [256 c, 256 c + 256), c = b - pre[q], one row per thread.
+// Partial sums go to slots of LZS doubles per workgroup (coefficients of q_0..q_j at 0..31, the
+// squared norm at 32); every workgroup of the item adds the item's slots in workgroup order, so
+// the coefficients are the same in all of them and independent of the batch.
+#define LZS 33
+// per-workgroup sums of np values (p <= j) over the 4 waves, fixed order, into red[p] (LDS)
+__device__ __forceinline__ void lzm_wg_sums(double (&c)[32], int j, double (*red)[32], double *out) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+#pragma unroll
+    for (int p = 0; p < 32; p++) {
+        if (p <= j) {
+            const double v = wave_sum(c[p]);
+            if (lane == 0) red[wv][p] = v;
+        }
+    }
+    __syncthreads();
+    if (t <= j) out[t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
+    __syncthreads();
+}
+// slots of pass 0: c_p = sum over the rows of q_p(i) w(i)
+__global__ __launch_bounds__(256) void k_lzm_part(const LzItem *__restrict__ items, const int *__restrict__ pre, int nit, int j,
+                                                  double *__restrict__ H, double *__restrict__ part) {
+    const int q = flat_item(pre, nit, blockIdx.x);
+    const LzItem L = items[q];
+    if (j >= L.K) return;
+    __shared__ double red[4][32];
+    __shared__ double cf[32];
+    const int t = threadIdx.x, i = 256 * (blockIdx.x - pre[q]) + t;
+    const double *Q = H + L.Q, *w = H + L.w;
+    double c[32];
+    const double wi = i < L.n ? w[i] : 0.0;
+#pragma unroll
+    for (int p = 0; p < 32; p++) c[p] = (p <= j && i < L.n) ? Q[(int64_t)p * L.n + i] * wi : 0.0;
+    lzm_wg_sums(c, j, red, cf);
+    if (t <= j) part[(int64_t)blockIdx.x * LZS + t] = cf[t];
+}
+// one Gram-Schmidt pass: w -= sum_p cf_p q_p with cf the item's summed slots of `in`; pass 0
+// records alpha_j = cf_j and writes the next pass's slots, pass 1 the squared-norm slot
+__global__ __launch_bounds__(256) void k_lzm_apply(const LzItem *__restrict__ items, const int *__restrict__ pre, int nit, int j, int pass,
+                                                   double *__restrict__ H, const double *__restrict__ in, double *__restrict__ outp) {
+    const int q = flat_item(pre, nit, blockIdx.x);
+    const LzItem L = items[q];
+    if (j >= L.K) return;
+    __shared__ double red[4][32];
+    __shared__ double cf[32], cs[32];
+    const int t = threadIdx.x, b0 = pre[q], nb = pre[q + 1] - b0, i = 256 * (blockIdx.x - b0) + t;
+    if (t <= j) {
+        double v = 0;
+        for (int u = 0; u < nb; u++) v += in[(int64_t)(b0 + u) * LZS + t];
+        cf[t] = v;
+    }
+    __syncthreads();
+    double *Q = H + L.Q, *w = H + L.w;
+    if (pass == 0 && blockIdx.x == b0 && t == 0) H[L.ab + j] = cf[j];
+    double v = 0;
+    if (i < L.n) {
+        v = w[i];
+#pragma unroll
+        for (int p = 0; p < 32; p++) if (p <= j) v -= cf[p] * Q[(int64_t)p * L.n + i];
+        w[i] = v;
+    }
+    if (pass == 0) {
+        double c[32];
+#pragma unroll
+        for (int p = 0; p < 32; p++) c[p] = (p <= j && i < L.n) ? Q[(int64_t)p * L.n + i] * v : 0.0;
+        lzm_wg_sums(c, j, red, cs);
+        if (t <= j) outp[(int64_t)blockIdx.x * LZS + t] = cs[t];
+    } else {
+        double c[32];
+        c[0] = v * v;
+        lzm_wg_sums(c, 0, red, cs);
+        if (t == 0) outp[(int64_t)blockIdx.x * LZS + 32] = cs[0];
+    }
+}
+// beta_j = |w| from the item's norm slots; q_{j+1} = w / beta_j
+__global__ __launch_bounds__(256) void k_lzm_fin(const LzItem *__restrict__ items, const int *__restrict__ pre, int nit, int j,
+                                                 double *__restrict__ H, const double *__restrict__ in) {
+    const int q = flat_item(pre, nit, blockIdx.x);
+    const LzItem L = items[q];
+    if (j >= L.K) return;
+    __shared__ double bsh;
+    const int t = threadIdx.x, b0 = pre[q], nb = pre[q + 1] - b0, i = 256 * (blockIdx.x - b0) + t;
+    if (t == 0) {
+        double v = 0;
+        for (int u = 0; u < nb; u++) v += in[(int64_t)(b0 + u) * LZS + 32];
+        bsh = sqrt(v);
+        if (blockIdx.x == b0) H[L.ab + 32 + j] = bsh;
+    }
+    __syncthreads();
+    const double beta = bsh;
+    if (j + 1 < L.K && i < L.n) {
+        const double inv = beta > 1e-300 ? 1.0 / beta : 0.0;
+        H[L.Q + (int64_t)(j + 1) * L.n + i] = H[L.w + i] * inv;
+    }
+}
+static double *g_lzm_part = nullptr;
+static int64_t g_lzm_cap = 0;
 extern "C" int bk_lanczos_step(const LzItem *it, int n_it, int j) {
     if (n_it <= 0) return 0;
-    const LzItem *d = (const LzItem *)bk_stage(it, sizeof(LzItem) * n_it);
+    std::vector<int64_t> cnt(n_it);
+    for (int q = 0; q < n_it; q++) {
+        if (it[q].K > 32) return -1;
+        cnt[q] = it[q].n > 0 ? (it[q].n + 255) / 256 : 0;
+    }
+    const LzItem *d;
+    const int *pre;
+    const int64_t nb = flat_stage_items(it, sizeof(LzItem) * n_it, cnt, (const void **)&d, &pre);
+    if (nb < 0) return -1;
+    if (nb == 0) return 0;
+    if (2 * nb * LZS > g_lzm_cap) {          // two slot sets: the passes alternate between them
+        if (bk_sync()) return -1;
+        if (g_lzm_part) (void)hipFree(g_lzm_part);
+        g_lzm_cap = 2 * nb * LZS + 4096;
+        if (bk_check(hipMalloc((void **)&g_lzm_part, sizeof(double) * g_lzm_cap), "hipMalloc lanczos slots")) { g_lzm_cap = 0; g_lzm_part = nullptr; return -1; }
+    }
+    double *pa = g_lzm_part, *pb = g_lzm_part + nb * LZS, *Hd = (double *)g_heap;
     const int tok = bk_prof_begin();
-    hipLaunchKernelGGL(k_lz_step, dim3(n_it), dim3(LZT), 0, g_bk_stream, d, n_it, j, (double *)g_heap);
+    hipLaunchKernelGGL(k_lzm_part, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, j, Hd, pa);
+    hipLaunchKernelGGL(k_lzm_apply, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, j, 0, Hd, (const double *)pa, pb);
+    hipLaunchKernelGGL(k_lzm_apply, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, j, 1, Hd, (const double *)pb, pa);
+    hipLaunchKernelGGL(k_lzm_fin, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, j, Hd, (const double *)pa);
     if (tok >= 0) bk_prof_end(tok, 12, 0, 0);
     return BK_LAUNCH_CHECK("k_lz_step");
 }
