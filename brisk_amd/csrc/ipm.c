@@ -728,10 +728,9 @@ static int schur_assemble(Solve **S, int ns, const int *act, int gram) {
  * the smallest squared pivot is >= 1e-6), else f64 with the shift ladder and pivot test.
  * fail[q] = 1 when no factorisation was found. */
 static int schur_factor_(Solve **S, int ns, const int *act, int force64, int *fail) {
-    int *todo = calloc(ns, sizeof(int)), *f64 = calloc(ns, sizeof(int));
+    int *todo = calloc(ns, sizeof(int)), *f64 = calloc(ns, sizeof(int)), *ran = calloc(ns, sizeof(int));
     double *reg = calloc(ns, sizeof(double));
-    int nslot = ns;
-    if (batch_scratch(ns, nslot + 1)) { free(todo); free(f64); free(reg); return -1; }
+    if (batch_scratch(ns, 3 * ns + 1)) { free(todo); free(f64); free(ran); free(reg); return -1; }
     double *H = heapp();
     for (int q = 0; q < ns; q++) {
         fail[q] = 0;
@@ -742,24 +741,37 @@ static int schur_factor_(Solve **S, int ns, const int *act, int force64, int *fa
         const double floor_ = fmax(1e-15, s->P.m * 2.220446049250313e-16);
         reg[q] = f64[q] && s->reg_hint ? fmax(floor_, 0.01 * s->reg_ok) : 0;
     }
+    /* each attempt factors the f32 solves on the library stream and the f64 ones on the side
+     * stream at the same time (two independent latency-bound chains). Shifts at ddbl[q], smallest
+     * squared pivots of the f64 factors at ddbl[ns + q], of the f32 factors at the floats from
+     * ddbl[2 ns], factor info at dint[q]. */
+    double *pm64 = ddbl() + ns;
+    float *pm32 = (float *)(ddbl() + 2 * ns);
+    double tA = bk_prof_on() ? bk_now() : 0;
     for (int attempt = 0; attempt < 12; attempt++) {
-        int any = 0;
-        for (int q = 0; q < ns; q++) any |= todo[q];
-        if (!any) break;
-        /* equilibrate into MF (f32 view of the same bytes for the f32 factor) */
+        int cnt[2] = { 0, 0 };
+        for (int q = 0; q < ns; q++) {
+            ran[q] = todo[q] ? f64[q] : -1;
+            g_hdbl[q] = todo[q] ? reg[q] : 0;
+            if (todo[q]) cnt[f64[q]]++;
+        }
+        if (!cnt[0] && !cnt[1]) break;
+        if (bk_h2d(ddbl(), g_hdbl, sizeof(double) * ns)) goto err;
+        bk_zero(dint(), sizeof(int) * ns);
+        if (cnt[0] && cnt[1] && bk_fork()) goto err;
         for (int prec = 0; prec < 2; prec++) {
+            if (!cnt[prec]) continue;
+            bk_side(prec == 1);
+            /* equilibrate into MF (f32 view of the same bytes for the f32 factor) */
             MItL_clear(&g_ml);
             for (int q = 0; q < ns; q++) {
-                if (!todo[q] || f64[q] != prec) continue;
+                if (ran[q] != prec) continue;
                 Solve *s = S[q];
-                g_hdbl[q] = reg[q];
                 const int64_t f = prec ? s->MF : 2 * s->MF;      /* element offset in the factor's type */
                 MItem it = { s->P.m, q, s->M, s->Dm, f };
                 MItL_add(&g_ml, it);
             }
-            if (!g_ml.n) continue;
-            if (bk_h2d(ddbl(), g_hdbl, sizeof(double) * ns)) goto err;
-            if (bk_equil(g_ml.v, g_ml.n, prec, g_bdbl / 8)) goto err;
+            if (bk_equil(g_ml.v, g_ml.n, prec, g_bdbl / 8)) { bk_join(); goto err; }
             /* factor in place: items now point to MF in the factor's element type */
             MItem *fi = malloc(sizeof(MItem) * g_ml.n);
             for (int t = 0; t < g_ml.n; t++) {
@@ -768,64 +780,75 @@ static int schur_factor_(Solve **S, int ns, const int *act, int force64, int *fa
                 const int64_t mul = prec ? 1 : 2;
                 fi[t].n = s->P.m; fi[t].slot = q; fi[t].off = mul * s->MF; fi[t].off2 = mul * s->MDinv; fi[t].off3 = mul * s->Mwork;
             }
-            bk_zero(dint(), sizeof(int) * ns);
-            int rc = prec ? bk_potrf_f64(fi, g_ml.n, H, H, H, dint(), ddbl())
-                          : bk_potrf_f32(fi, g_ml.n, (float *)H, (float *)H, (float *)H, dint(), (float *)ddbl());
+            int rc = prec ? bk_potrf_f64(fi, g_ml.n, H, H, H, dint(), pm64)
+                          : bk_potrf_f32(fi, g_ml.n, (float *)H, (float *)H, (float *)H, dint(), pm32);
             free(fi);
-            if (rc) goto err;
-            if (bk_d2h(g_hint, dint(), sizeof(int) * ns)) goto err;
-            if (bk_d2h(g_hdbl, ddbl(), sizeof(double) * ns)) goto err;
-            for (int t = 0; t < g_ml.n; t++) {
-                const int q = g_ml.v[t].slot;
-                Solve *s = S[q];
-                const double pmin = prec ? g_hdbl[q] : (double)((float *)g_hdbl)[q];
-                const double floor_ = fmax(1e-15, s->P.m * 2.220446049250313e-16);
-                if (!prec) {
-                    if (g_hint[q] == 0 && pmin >= 1e-6) { todo[q] = 0; s->f32 = 1; s->reg = 0; s->nfactor_f32++; }
-                    else { f64[q] = 1; s->want_double = 1; reg[q] = s->reg_hint ? fmax(floor_, 0.01 * s->reg_ok) : 0; }
+            if (rc) { bk_join(); goto err; }
+        }
+        if (bk_join()) goto err;
+        if (bk_d2h(g_hint, dint(), sizeof(int) * ns)) goto err;
+        if (bk_d2h(g_hdbl + ns, pm64, sizeof(double) * 2 * ns)) goto err;
+        const float *hp32 = (const float *)(g_hdbl + 2 * ns);
+        for (int q = 0; q < ns; q++) {
+            if (ran[q] < 0) continue;
+            const int prec = ran[q];
+            Solve *s = S[q];
+            const double pmin = prec ? g_hdbl[ns + q] : (double)hp32[q];
+            const double floor_ = fmax(1e-15, s->P.m * 2.220446049250313e-16);
+            if (!prec) {
+                /* an f32 factor that fails (or has a pivot below 1e-6) is redone in f64 */
+                if (g_hint[q] == 0 && pmin >= 1e-6) { todo[q] = 0; s->f32 = 1; s->reg = 0; s->nfactor_f32++; }
+                else { f64[q] = 1; s->want_double = 1; reg[q] = s->reg_hint ? fmax(floor_, 0.01 * s->reg_ok) : 0; }
+                continue;
+            }
+            if (g_hint[q] == 0) {
+                if (pmin >= 1e-14 || reg[q] >= floor_) {
+                    todo[q] = 0; s->f32 = 0; s->reg = reg[q]; s->reg_hint = reg[q] > 0; s->reg_ok = reg[q]; s->nfactor_f64++;
                     continue;
                 }
-                if (g_hint[q] == 0) {
-                    if (pmin >= 1e-14 || reg[q] >= floor_) {
-                        todo[q] = 0; s->f32 = 0; s->reg = reg[q]; s->reg_hint = reg[q] > 0; s->reg_ok = reg[q]; s->nfactor_f64++;
-                        continue;
-                    }
-                    reg[q] = fmax(floor_, reg[q] * 100);
-                } else reg[q] = reg[q] == 0 ? floor_ : reg[q] * 100;
-                if (attempt >= 10) { todo[q] = 0; fail[q] = 1; }
-            }
+                reg[q] = fmax(floor_, reg[q] * 100);
+            } else reg[q] = reg[q] == 0 ? floor_ : reg[q] * 100;
+            if (attempt >= 10) { todo[q] = 0; fail[q] = 1; }
         }
     }
     for (int q = 0; q < ns; q++) if (todo[q]) fail[q] = 1;
-    /* explicit inverse of the factor and its transpose: every later solve is two products */
-    for (int prec = 0; prec < 2; prec++) {
-        int c = 0;
-        MItL_clear(&g_ml);
-        for (int q = 0; q < ns; q++) {
-            if ((act && !act[q]) || fail[q] || S[q]->f32 == prec) continue;
-            Solve *s = S[q];
-            const int64_t mul = prec ? 1 : 2;
-            MItem it = { s->P.m, q, mul * s->MF, mul * s->MDinv, mul * s->Mwork, mul * s->MLi };
-            MItL_add(&g_ml, it);
-            c++;
-        }
-        if (!c) continue;
+    if (tA > 0) { bk_sync(); bk_prof_host(54, bk_now() - tA); tA = bk_now(); }
+    /* explicit inverse of the factor and its transpose (every later solve is two products),
+     * again the f32 factors on the library stream and the f64 ones on the side stream */
+    {
+        int cnt[2] = { 0, 0 };
+        for (int q = 0; q < ns; q++) if (!(act && !act[q]) && !fail[q]) cnt[S[q]->f32 ? 0 : 1]++;
         bk_zero(dint(), sizeof(int) * ns);
-        int rc = prec ? bk_trtri_f64(g_ml.v, g_ml.n, H, H, H, H, dint())
-                      : bk_trtri_f32(g_ml.v, g_ml.n, (float *)H, (float *)H, (float *)H, (float *)H, dint());
-        if (rc) goto err;
-        for (int t = 0; t < g_ml.n; t++) {
-            const Solve *s = S[g_ml.v[t].slot];
-            const int64_t mul = prec ? 1 : 2;
-            g_ml.v[t].off = mul * s->MLi; g_ml.v[t].off4 = mul * s->MLiT;
+        if (cnt[0] && cnt[1] && bk_fork()) goto err;
+        for (int prec = 0; prec < 2; prec++) {
+            if (!cnt[prec]) continue;
+            bk_side(prec == 1);
+            MItL_clear(&g_ml);
+            for (int q = 0; q < ns; q++) {
+                if ((act && !act[q]) || fail[q] || S[q]->f32 == prec) continue;
+                Solve *s = S[q];
+                const int64_t mul = prec ? 1 : 2;
+                MItem it = { s->P.m, q, mul * s->MF, mul * s->MDinv, mul * s->Mwork, mul * s->MLi };
+                MItL_add(&g_ml, it);
+            }
+            int rc = prec ? bk_trtri_f64(g_ml.v, g_ml.n, H, H, H, H, dint())
+                          : bk_trtri_f32(g_ml.v, g_ml.n, (float *)H, (float *)H, (float *)H, (float *)H, dint());
+            if (rc) { bk_join(); goto err; }
+            for (int t = 0; t < g_ml.n; t++) {
+                const Solve *s = S[g_ml.v[t].slot];
+                const int64_t mul = prec ? 1 : 2;
+                g_ml.v[t].off = mul * s->MLi; g_ml.v[t].off4 = mul * s->MLiT;
+            }
+            rc = prec ? bk_transpose_f64(g_ml.v, g_ml.n) : bk_transpose_f32(g_ml.v, g_ml.n);
+            if (rc) { bk_join(); goto err; }
         }
-        rc = prec ? bk_transpose_f64(g_ml.v, g_ml.n) : bk_transpose_f32(g_ml.v, g_ml.n);
-        if (rc) goto err;
+        if (bk_join()) goto err;
     }
-    free(todo); free(f64); free(reg);
+    if (tA > 0) { bk_sync(); bk_prof_host(55, bk_now() - tA); }
+    free(todo); free(f64); free(ran); free(reg);
     return 0;
 err:
-    free(todo); free(f64); free(reg);
+    free(todo); free(f64); free(ran); free(reg);
     return -1;
 }
 static int schur_factor(Solve **S, int ns, const int *act, int force64, int *fail) {

@@ -10,7 +10,10 @@
 static int g_init = 0;
 static int g_failed = 0;
 static char g_err[256] = "";
-hipStream_t g_bk_stream = nullptr;
+hipStream_t g_bk_stream = nullptr;   // the stream launches go to (the library stream unless forked)
+static hipStream_t g_main_stream = nullptr, g_side_stream = nullptr;
+static hipEvent_t g_fork_ev = nullptr, g_join_ev = nullptr, g_side_ev = nullptr;
+static int g_forked = 0;
 static hipEvent_t g_sync_ev;
 
 // staging ring: host pinned + device, recycled at every synchronisation
@@ -50,7 +53,8 @@ extern "C" int bk_init(void) {
     (void)hipSetDeviceFlags(hipDeviceScheduleSpin);
     (void)hipGetLastError();
     if (bk_check(hipSetDevice(0), "hipSetDevice")) return -1;
-    if (bk_check(hipStreamCreateWithFlags(&g_bk_stream, hipStreamNonBlocking), "hipStreamCreate")) return -1;
+    if (bk_check(hipStreamCreateWithFlags(&g_main_stream, hipStreamNonBlocking), "hipStreamCreate")) return -1;
+    g_bk_stream = g_main_stream;
     if (bk_check(hipEventCreateWithFlags(&g_sync_ev, hipEventDisableTiming), "hipEventCreate")) return -1;
     g_ring_cap = (size_t)64 << 20;
     if (bk_check(hipHostMalloc((void **)&g_ring_h, g_ring_cap, hipHostMallocDefault), "hipHostMalloc ring")) return -1;
@@ -85,15 +89,44 @@ static int g_prof_on_host = 0;
 static inline int g_prof_host_on(void) { return g_prof_on_host; }
 // host-side counters (read as profile classes 62: waits on the stream, 63: staged copies)
 static double g_sync_ms = 0, g_sync_n = 0, g_stage_n = 0, g_stage_bytes = 0;
+// A second stream for independent work of a batch (the f32 and the f64 factorizations):
+// bk_fork makes it wait for the work queued so far, bk_side directs launches to it, bk_join
+// makes the library stream wait for it. Both streams have the default priority.
+extern "C" int bk_fork(void) {
+    if (g_forked) return 0;
+    if (!g_side_stream) {
+        if (bk_check(hipStreamCreateWithFlags(&g_side_stream, hipStreamNonBlocking), "hipStreamCreate side")) return -1;
+        if (bk_check(hipEventCreateWithFlags(&g_fork_ev, hipEventDisableTiming), "hipEventCreate") ||
+            bk_check(hipEventCreateWithFlags(&g_join_ev, hipEventDisableTiming), "hipEventCreate") ||
+            bk_check(hipEventCreateWithFlags(&g_side_ev, hipEventDisableTiming), "hipEventCreate")) return -1;
+    }
+    if (bk_check(hipEventRecord(g_fork_ev, g_main_stream), "hipEventRecord fork") ||
+        bk_check(hipStreamWaitEvent(g_side_stream, g_fork_ev, 0), "hipStreamWaitEvent fork")) return -1;
+    g_forked = 1;
+    return 0;
+}
+extern "C" void bk_side(int on) { g_bk_stream = (on && g_forked) ? g_side_stream : g_main_stream; }
+extern "C" int bk_join(void) {
+    g_bk_stream = g_main_stream;
+    if (!g_forked) return 0;
+    g_forked = 0;
+    if (bk_check(hipEventRecord(g_join_ev, g_side_stream), "hipEventRecord join")) return -1;
+    return bk_check(hipStreamWaitEvent(g_main_stream, g_join_ev, 0), "hipStreamWaitEvent join");
+}
 extern "C" int bk_sync(void) {
     const double t0 = g_prof_host_on() ? bk_now() : 0;
     // an event after the queued work, polled without sleeping: the host resumes within
-    // microseconds of the GPU instead of after a wake-up
-    int r = bk_check(hipEventRecord(g_sync_ev, g_bk_stream), "hipEventRecord");
+    // microseconds of the GPU instead of after a wake-up (both streams while forked)
+    int r = bk_check(hipEventRecord(g_sync_ev, g_main_stream), "hipEventRecord");
+    if (!r && g_forked) r = bk_check(hipEventRecord(g_side_ev, g_side_stream), "hipEventRecord side");
     if (!r) {
         hipError_t e;
         while ((e = hipEventQuery(g_sync_ev)) == hipErrorNotReady) { }
         r = bk_check(e, "hipEventQuery");
+        if (!r && g_forked) {
+            while ((e = hipEventQuery(g_side_ev)) == hipErrorNotReady) { }
+            r = bk_check(e, "hipEventQuery side");
+        }
     }
     if (t0 > 0) { g_sync_ms += 1e3 * (bk_now() - t0); g_sync_n += 1; }
     g_ring_pos = 0;
@@ -293,7 +326,8 @@ static double g_acc_ms[PROF_CLS], g_acc_n[PROF_CLS], g_acc_flops[PROF_CLS], g_ac
 static int g_prof_evinit = 0;
 static void prof_flush(void) {
     if (!g_prof_n) return;
-    (void)hipStreamSynchronize(g_bk_stream);
+    (void)hipStreamSynchronize(g_main_stream);
+    if (g_side_stream) (void)hipStreamSynchronize(g_side_stream);
     for (int i = 0; i < g_prof_n; i++) {
         float ms = 0;
         if (hipEventElapsedTime(&ms, g_prof_ev[2 * i], g_prof_ev[2 * i + 1]) == hipSuccess) {

@@ -28,6 +28,12 @@ int bk_d2h(void *dst, const void *src, size_t bytes);        /* synchronous (dra
 int bk_d2d(void *dst, const void *src, size_t bytes);        /* async device copy */
 int bk_zero(void *dst, size_t bytes);                         /* async fill kernel (graph safe) */
 int bk_sync(void);
+/* a second stream for independent parts of a batch: bk_fork (it waits for the work queued so
+ * far), bk_side(1) / bk_side(0) direct launches to it / back, bk_join (the library stream waits
+ * for it). bk_sync waits for both while forked. */
+int bk_fork(void);
+void bk_side(int on);
+int bk_join(void);
 void *bk_stage(const void *host, size_t bytes);  /* device copy of host data, valid until next sync */
 double bk_now(void);                             /* host wall clock, s */
 size_t bk_mem_free(void);                        /* free device memory, bytes */
