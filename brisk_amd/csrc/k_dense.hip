@@ -597,50 +597,71 @@ extern "C" int bk_gemv_mixed(int trans, double alpha, double beta, const float *
 }
 
 // ---- symmetric matrix-vector product from the lower triangle ----------------------------------
-// One workgroup per lower 64 x 64 tile (I, J), I >= J, enumerated row-major over the lower
-// triangle. The tile is staged in LDS (coalesced column reads); threads 0-63 form the row part
-// r(i) = sum_c A(64I + i, 64J + c) x(64J + c), threads 64-127 the column part
-// c(j) = sum_r A(64I + r, 64J + j) x(64I + r) (I > J only). A diagonal tile reads its lower
+// The lower 64 x 64 tiles (I, J), I >= J, of an item are enumerated row-major over the lower
+// triangle and dealt to workgroups in chunks of SYMV_CH consecutive tiles. A workgroup stages a
+// tile in LDS (coalesced column reads; the next tile's loads are in flight meanwhile) and all 256
+// threads form quarter sums: thread (i, h) the row part over columns 16h..16h+15 of row i and the
+// column part over rows 16h..16h+15 of column i. The quarters are added in order and the tile
+// writes r(i) = sum_c A(64I + i, 64J + c) x(64J + c) to slot (I, J) and, for I > J,
+// c(j) = sum_r A(64I + r, 64J + j) x(64I + r) to slot (J, I). A diagonal tile reads its lower
 // triangle only and mirrors it.
+#define SYMV_CH 4
 __global__ __launch_bounds__(256) void k_symv_tiles(const SymvItem *__restrict__ items, const int *__restrict__ pre, int nit,
                                                     const double *__restrict__ A, const double *__restrict__ X, double *__restrict__ P) {
     const int q = flat_item(pre, nit, blockIdx.x);
     const SymvItem it = items[q];
-    int I, J;
-    tri_index(blockIdx.x - pre[q], I, J);
-    const int n = it.n, nT = (n + 63) / 64, tid = threadIdx.x;
+    const int n = it.n, nT = (n + 63) / 64, tid = threadIdx.x, li = tid & 63, h = tid >> 6;
+    const int64_t ntile = (int64_t)nT * (nT + 1) / 2, t0 = (int64_t)(blockIdx.x - pre[q]) * SYMV_CH;
+    const int64_t t1 = t0 + SYMV_CH < ntile ? t0 + SYMV_CH : ntile;
     __shared__ double Ts[64][65];    // Ts[c][r] = A(64 I + r, 64 J + c)
     __shared__ double xs[2][64];     // x of the column block J, of the row block I
+    __shared__ double red[2][4][64]; // quarter sums of the row and the column parts
     const double *a = A + it.a;
-#pragma unroll
-    for (int u = 0; u < 16; u++) {
-        const int e = tid + 256 * u, r = e & 63, c = e >> 6, gi = 64 * I + r, gj = 64 * J + c;
-        Ts[c][r] = (gi < n && gj < n) ? a[gi + (int64_t)gj * it.ld] : 0.0;
-    }
-    if (tid < 64) xs[0][tid] = 64 * J + tid < n ? X[it.x + 64 * J + tid] : 0.0;
-    else if (tid < 128) xs[1][tid - 64] = 64 * I + tid - 64 < n ? X[it.x + 64 * I + tid - 64] : 0.0;
-    __syncthreads();
     double *p = P + it.p;
-    if (tid < 64) {
-        const int i = tid;
-        double s0 = 0, s1 = 0;
+    double ra[16];
+    auto load = [&](int64_t t) {
+        int I, J;
+        tri_index(t, I, J);
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int e = tid + 256 * u, r = e & 63, c = e >> 6, gi = 64 * I + r, gj = 64 * J + c;
+            ra[u] = (gi < n && gj < n) ? a[gi + (int64_t)gj * it.ld] : 0.0;
+        }
+    };
+    if (t0 < t1) load(t0);
+    for (int64_t t = t0; t < t1; t++) {
+        int I, J;
+        tri_index(t, I, J);
+#pragma unroll
+        for (int u = 0; u < 16; u++) { const int e = tid + 256 * u; Ts[e >> 6][e & 63] = ra[u]; }
+        if (tid < 64) xs[0][tid] = 64 * J + tid < n ? X[it.x + 64 * J + tid] : 0.0;
+        else if (tid < 128) xs[1][tid - 64] = 64 * I + tid - 64 < n ? X[it.x + 64 * I + tid - 64] : 0.0;
+        __syncthreads();
+        if (t + 1 < t1) load(t + 1);           // the next tile's loads stay in flight during the sums
+        double sr = 0, sc = 0;
         if (I > J) {
-#pragma unroll 16
-            for (int c = 0; c < 64; c += 2) { s0 += Ts[c][i] * xs[0][c]; s1 += Ts[c + 1][i] * xs[0][c + 1]; }
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const int c = 16 * h + k;
+                sr += Ts[c][li] * xs[0][c];
+                sc += Ts[li][c] * xs[1][c];
+            }
         } else {
-#pragma unroll 16
-            for (int c = 0; c < 64; c += 2) {
-                s0 += (c <= i ? Ts[c][i] : Ts[i][c]) * xs[0][c];
-                s1 += (c + 1 <= i ? Ts[c + 1][i] : Ts[i][c + 1]) * xs[0][c + 1];
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const int c = 16 * h + k;
+                sr += (c <= li ? Ts[c][li] : Ts[li][c]) * xs[0][c];
             }
         }
-        p[((int64_t)I * nT + J) * 64 + i] = s0 + s1;
-    } else if (tid < 128 && I > J) {
-        const int j = tid - 64;
-        double s0 = 0, s1 = 0;
-#pragma unroll 16
-        for (int r = 0; r < 64; r += 2) { s0 += Ts[j][r] * xs[1][r]; s1 += Ts[j][r + 1] * xs[1][r + 1]; }
-        p[((int64_t)J * nT + I) * 64 + j] = s0 + s1;
+        red[0][h][li] = sr;
+        red[1][h][li] = sc;
+        __syncthreads();
+        if (tid < 64) p[((int64_t)I * nT + J) * 64 + tid] = (red[0][0][tid] + red[0][1][tid]) + (red[0][2][tid] + red[0][3][tid]);
+        else if (tid < 128 && I > J) {
+            const int j = tid - 64;
+            p[((int64_t)J * nT + I) * 64 + j] = (red[1][0][j] + red[1][1][j]) + (red[1][2][j] + red[1][3][j]);
+        }
+        __syncthreads();
     }
 }
 // y(64 a + i) = alpha sum_b P(a, b)(i) + beta y(64 a + i): one 64-thread workgroup per row block,
@@ -663,7 +684,7 @@ extern "C" int bk_symv_f64(double alpha, double beta, const double *A, const dou
     double by = 0;
     for (int q = 0; q < n_it; q++) {
         const int64_t nT = (it[q].n + 63) / 64;
-        ct[q] = it[q].n > 0 ? nT * (nT + 1) / 2 : 0;
+        ct[q] = it[q].n > 0 ? (nT * (nT + 1) / 2 + SYMV_CH - 1) / SYMV_CH : 0;
         cr[q] = it[q].n > 0 ? nT : 0;
         by += 8.0 * ((double)it[q].n * (it[q].n + 1) / 2 + 3.0 * it[q].n);
     }
