@@ -371,7 +371,7 @@ def main():
                 p["flops"] / max(p["ms"], 1e-9) / 1e9, p["bytes"] / max(p["ms"], 1e-9) / 1e6), file=sys.stderr, flush=True)
     d = prof[dom]
     avg_s = d["ms"] / max(d["launches"], 1) / 1e3
-    if d["flops"] > 0:
+    if dom <= 8 and d["flops"] > 0:           # the MFMA classes: GEMMs and the diagonal-block factor
         prec = "f64" if dom < 4 or dom in (8,) else "f32"
         ach = d["flops"] / max(d["launches"], 1) / avg_s / 1e12
         roof = dict(bound="mfma", achieved=round(ach, 3), peak=PEAK[prec], unit="TFLOP/s", frac=round(ach / PEAK[prec], 4),
