@@ -316,7 +316,7 @@ def main():
         cpu_serving_sample(args.cpu_serving_sample, args.cpu_threads or min(16, os.cpu_count() or 1))
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (every GPU box here has one); --gpus %d is not supported" % args.gpus,
+        print("bench.py: the solve runs on one GPU; --gpus %d is not supported (the multi-GPU sharding is a design only)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "c5":
