@@ -33,7 +33,7 @@ HBM_PEAK = 8000.0                        # GB/s
 KCLASS = {0: "k_gemm<double, 0, 0>", 1: "k_gemm<double, 0, 1>", 2: "k_gemm<double, 1, 0>", 3: "k_gemm<double, 1, 1>",
           4: "k_gemm<float, 0, 0>", 5: "k_gemm<float, 0, 1>", 6: "k_gemm<float, 1, 0>", 7: "k_gemm<float, 1, 1>",
           8: "k_potrf_diag", 9: "k_gemv", 10: "Schur assembly", 11: "k_aop", 12: "k_lz_step", 13: "k_dot_chunks",
-          14: "k_lincomb", 15: "k_symv"}
+          14: "k_lincomb", 15: "k_symv", 16: "k_trsv"}
 
 WORKLOADS = {
     "c1": dict(kind="theta", args=(50, 0.2, 0), desc="Lovasz theta SDP of G(50, 0.2), seed 0 (C=-J, tr X=1, X_ij=0 on edges)"),

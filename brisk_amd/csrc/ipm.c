@@ -727,6 +727,11 @@ static int schur_assemble(Solve **S, int ns, const int *act, int gram) {
 /* factor M of the active solves (reference solver.c:5266-5420): f32 when allowed (accepted if
  * the smallest squared pivot is >= 1e-6), else f64 with the shift ladder and pivot test.
  * fail[q] = 1 when no factorisation was found. */
+static TrsvItem trsv_item(const Solve *s, int prec) {
+    const int64_t mul = prec ? 1 : 2;              /* element offsets in the factor's type */
+    TrsvItem t = { s->P.m, 0, 0, mul * s->MF, mul * s->MLi, mul * s->MLiT, s->t1, s->t2 };
+    return t;
+}
 static int schur_factor_(Solve **S, int ns, const int *act, int force64, int *fail) {
     int *todo = calloc(ns, sizeof(int)), *f64 = calloc(ns, sizeof(int)), *ran = calloc(ns, sizeof(int));
     double *reg = calloc(ns, sizeof(double));
@@ -813,34 +818,76 @@ static int schur_factor_(Solve **S, int ns, const int *act, int force64, int *fa
     }
     for (int q = 0; q < ns; q++) if (todo[q]) fail[q] = 1;
     if (tA > 0) { bk_sync(); bk_prof_host(54, bk_now() - tA); tA = bk_now(); }
-    /* explicit inverse of the factor and its transpose (every later solve is two products),
-     * again the f32 factors on the library stream and the f64 ones on the side stream */
+    /* every later solve with the factor is either two products with the explicit inverse and its
+     * transpose, or two blocked triangular solves with the factor itself (the inverses of its
+     * 256 x 256 diagonal blocks only): whichever the cost model below finds cheaper for the
+     * precision group. Again the f32 factors on the library stream and the f64 ones on the side
+     * stream. */
     {
-        int cnt[2] = { 0, 0 };
-        for (int q = 0; q < ns; q++) if (!(act && !act[q]) && !fail[q]) cnt[S[q]->f32 ? 0 : 1]++;
+        int cnt[2] = { 0, 0 }, use_trsv[2] = { 0, 0 };
+        double fl[2] = { 0, 0 };
+        int nbk[2] = { 0, 0 };
+        for (int q = 0; q < ns; q++) {
+            if ((act && !act[q]) || fail[q]) continue;
+            const int prec = S[q]->f32 ? 0 : 1;
+            const double m = S[q]->P.m;
+            cnt[prec]++;
+            fl[prec] += m * m * m / 3.0;
+            const int b = (S[q]->P.m + 255) / 256;
+            if (b > nbk[prec]) nbk[prec] = b;
+        }
+        for (int prec = 0; prec < 2; prec++) {
+            /* the inverse: its flops at the rates measured for it (f32 3e13, f64 1.5e13 flop/s) and
+             * a fixed 0.3 ms of launches; the solves: two launches of about 15 us per block column
+             * (measured over serving batches), 6 preconditioner applications per f32 factor (the
+             * PCG's usual count), 3 per f64. The solves only where the inverse costs more than
+             * 5 ms: a single solve of order 6000 keeps the inverse, a serving batch whose factor
+             * flops add up takes the solves. */
+            const double t_inv = fl[prec] / (prec ? 1.5e13 : 3.0e13) + 3e-4;
+            const double t_sv = (prec ? 3 : 6) * 2.0 * nbk[prec] * 15e-6;
+            use_trsv[prec] = t_inv > 5e-3 && t_sv < 0.5 * t_inv;
+        }
+        /* a solve's -schurapply option overrides the model for it */
+        for (int q = 0; q < ns; q++)
+            if (!(act && !act[q]) && !fail[q]) S[q]->trsv = S[q]->o.apply < 0 ? use_trsv[S[q]->f32 ? 0 : 1] : S[q]->o.apply;
         bk_zero(dint(), sizeof(int) * ns);
         if (cnt[0] && cnt[1] && bk_fork()) goto err;
         for (int prec = 0; prec < 2; prec++) {
             if (!cnt[prec]) continue;
             bk_side(prec == 1);
-            MItL_clear(&g_ml);
-            for (int q = 0; q < ns; q++) {
-                if ((act && !act[q]) || fail[q] || S[q]->f32 == prec) continue;
-                Solve *s = S[q];
-                const int64_t mul = prec ? 1 : 2;
-                MItem it = { s->P.m, q, mul * s->MF, mul * s->MDinv, mul * s->Mwork, mul * s->MLi };
-                MItL_add(&g_ml, it);
-            }
-            int rc = prec ? bk_trtri_f64(g_ml.v, g_ml.n, H, H, H, H, dint())
+            for (int mode = 0; mode < 2; mode++) {
+                MItL_clear(&g_ml);
+                for (int q = 0; q < ns; q++) {
+                    if ((act && !act[q]) || fail[q] || S[q]->f32 == prec || S[q]->trsv != mode) continue;
+                    Solve *s = S[q];
+                    const int64_t mul = prec ? 1 : 2;
+                    MItem it = { s->P.m, q, mul * s->MF, mul * s->MDinv, mul * s->Mwork, mul * s->MLi };
+                    MItL_add(&g_ml, it);
+                }
+                if (!g_ml.n) continue;
+                int rc;
+                if (mode) {
+                    rc = prec ? bk_trtri_diag_f64(g_ml.v, g_ml.n, H, H, H, H, dint())
+                              : bk_trtri_diag_f32(g_ml.v, g_ml.n, (float *)H, (float *)H, (float *)H, (float *)H, dint());
+                    if (rc) { bk_join(); goto err; }
+                    TrsvItem *tv = malloc(sizeof(TrsvItem) * (g_ml.n + 1));
+                    for (int t = 0; t < g_ml.n; t++) tv[t] = trsv_item(S[g_ml.v[t].slot], prec);
+                    rc = prec ? bk_trsv_prep_f64(tv, g_ml.n, H) : bk_trsv_prep_f32(tv, g_ml.n, (float *)H);
+                    free(tv);
+                    if (rc) { bk_join(); goto err; }
+                    continue;
+                }
+                rc = prec ? bk_trtri_f64(g_ml.v, g_ml.n, H, H, H, H, dint())
                           : bk_trtri_f32(g_ml.v, g_ml.n, (float *)H, (float *)H, (float *)H, (float *)H, dint());
-            if (rc) { bk_join(); goto err; }
-            for (int t = 0; t < g_ml.n; t++) {
-                const Solve *s = S[g_ml.v[t].slot];
-                const int64_t mul = prec ? 1 : 2;
-                g_ml.v[t].off = mul * s->MLi; g_ml.v[t].off4 = mul * s->MLiT;
+                if (rc) { bk_join(); goto err; }
+                for (int t = 0; t < g_ml.n; t++) {
+                    const Solve *s = S[g_ml.v[t].slot];
+                    const int64_t mul = prec ? 1 : 2;
+                    g_ml.v[t].off = mul * s->MLi; g_ml.v[t].off4 = mul * s->MLiT;
+                }
+                rc = prec ? bk_transpose_f64(g_ml.v, g_ml.n) : bk_transpose_f32(g_ml.v, g_ml.n);
+                if (rc) { bk_join(); goto err; }
             }
-            rc = prec ? bk_transpose_f64(g_ml.v, g_ml.n) : bk_transpose_f32(g_ml.v, g_ml.n);
-            if (rc) { bk_join(); goto err; }
         }
         if (bk_join()) goto err;
     }
@@ -858,17 +905,29 @@ static int schur_factor(Solve **S, int ns, const int *act, int force64, int *fai
     return r_;
 }
 
-/* z = M^{-1} r through the factor: z = D Li' Li D r */
+/* z = M^{-1} r through the factor: z = D Li' Li D r (two products with the inverse, or the two
+ * blocked triangular solves with L) */
 static int schur_prec(Solve **S, int ns, const int *act, size_t fr, size_t fz) {
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) if (!act || act[q]) lin_add(S[q]->P.m, S[q]->t1, S[q]->Dm, 1.0, F_(S[q], fr), 0, -1, 0, -1, 0);
     if (bk_vmul(g_ll.v, g_ll.n)) return -1;
     double *H = heapp();
+    for (int prec = 0; prec < 2; prec++) {
+        int nt = 0;
+        for (int q = 0; q < ns; q++) if ((!act || act[q]) && S[q]->f32 != prec && S[q]->trsv) nt++;
+        if (!nt) continue;
+        TrsvItem *tv = malloc(sizeof(TrsvItem) * nt);
+        nt = 0;
+        for (int q = 0; q < ns; q++) if ((!act || act[q]) && S[q]->f32 != prec && S[q]->trsv) tv[nt++] = trsv_item(S[q], prec);
+        const int rc = prec ? bk_trsv_f64(tv, nt, H, H) : bk_trsv_f32(tv, nt, (const float *)H, H);
+        free(tv);
+        if (rc) return -1;
+    }
     for (int pass = 0; pass < 2; pass++) {
         for (int prec = 0; prec < 2; prec++) {
             GemvL_clear(&g_vl);
             for (int q = 0; q < ns; q++) {
-                if ((act && !act[q]) || S[q]->f32 == prec) continue;
+                if ((act && !act[q]) || S[q]->f32 == prec || S[q]->trsv) continue;
                 Solve *s = S[q];
                 const int64_t mul = prec ? 1 : 2;
                 /* pass 0: t2 = Li t1 (columns of Li' = rows of Li); pass 1: t1 = Li' t2 */

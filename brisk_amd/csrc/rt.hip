@@ -35,6 +35,11 @@ static CacheEnt g_cache_tab[CACHE_TAB];
 static uint64_t g_chunk_use[CACHE_CAP / CACHE_CHUNK];
 static double g_cache_hits = 0, g_cache_miss = 0;
 
+int bk_fail(const char *what) {
+    snprintf(g_err, sizeof(g_err), "%s", what);
+    fprintf(stderr, "brisk gpu: %s\n", g_err);
+    return -1;
+}
 int bk_check(hipError_t e, const char *what) {
     if (e == hipSuccess) return 0;
     g_failed = 1;

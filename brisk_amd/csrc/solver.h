@@ -59,6 +59,7 @@ typedef struct {
     double mixed_until;     /* mixed: f32 factor while max(relgap, pinf, dinf) exceeds this (1e-6) */
     double time_limit;      /* s, <= 0 none */
     int hsd;                /* 1: self-dual embedding, 0: infeasible start */
+    int apply;              /* solves with the Schur factor: -1 cost model, 0 explicit inverse, 1 block triangular solves */
 } SolveOpts;
 
 typedef struct Solve {
@@ -80,6 +81,7 @@ typedef struct Solve {
     int m_exact;            /* the stored M is exact in f64 (else M v uses the operator form) */
     /* factor state of M */
     int f32;                /* current factor is f32 */
+    int trsv;               /* current factor is applied by block triangular solves (no explicit inverse) */
     int want_double;        /* sticky: f32 no longer used */
     double reg, reg_ok;
     int reg_hint;

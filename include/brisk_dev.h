@@ -54,7 +54,7 @@ int bk_prof_on(void);                            /* profiling enabled */
 void bk_prof_host(int cls, double seconds);      /* add a host-side interval to class cls (40-59) */
 /* classes: 0-3 k_gemm<double, ta, tb> (2 ta + tb), 4-7 k_gemm<float, ta, tb>, 8 k_potrf_diag,
  * 9 k_gemv, 10 Schur assembly, 11 constraint operators, 12 Lanczos steps, 13 dot products,
- * 14 linear combinations, 15 k_symv (tiles and reduction) */
+ * 14 linear combinations, 15 k_symv (tiles and reduction), 16 k_trsv (both directions) */
 
 /* ---- batched GEMM: C_it = alpha op(A_it) op(B_it) + beta C_it ---- */
 typedef struct {
@@ -92,6 +92,23 @@ int bk_potrf_f32(const MItem *it, int n_it, float *A, float *Dinv, float *W, int
  * inverses (Dinv + off2); W + off3 scratch (n*n). Items with info[slot] != 0 are skipped. */
 int bk_trtri_f64(const MItem *it, int n_it, const double *L, const double *Dinv, double *Linv, double *W, const int *info);
 int bk_trtri_f32(const MItem *it, int n_it, const float *L, const float *Dinv, float *Linv, float *W, const int *info);
+/* the same, limited to the 256 x 256 diagonal blocks: Linv + off4 holds the inverse of every
+ * diagonal block of L (rows and columns 256 b .. 256 b + 255) and zeros elsewhere */
+int bk_trtri_diag_f64(const MItem *it, int n_it, const double *L, const double *Dinv, double *Linv, double *W, const int *info);
+int bk_trtri_diag_f32(const MItem *it, int n_it, const float *L, const float *Dinv, float *Linv, float *W, const int *info);
+/* ---- blocked triangular solves with a factor: L v1 = v0, then L' v0 = v1 ----
+ * (the preconditioner application z = L^{-T} L^{-1} r of the PCG without an explicit inverse,
+ * reference schur_solve_ solver.c:5483-5622 applies its factor by the same two solves).
+ * L (order m, leading dimension m) at offset L; Di at offset Di holds the inverses of its 256 x
+ * 256 diagonal blocks (bk_trtri_diag); DiT at offset DiT receives their transposes from
+ * bk_trsv_prep, once per factor. v0, v1: f64 vectors (offsets in doubles); v0 is the right-hand
+ * side and receives the solution, v1 is overwritten. Matrix offsets in the factor's type; pre
+ * is set by the library. One launch per 256-wide block column and direction. */
+typedef struct { int m, c; int64_t pre, L, Di, DiT, v0, v1; } TrsvItem;
+int bk_trsv_prep_f64(const TrsvItem *it, int n_it, double *A);
+int bk_trsv_prep_f32(const TrsvItem *it, int n_it, float *A);
+int bk_trsv_f64(const TrsvItem *it, int n_it, const double *A, double *V);
+int bk_trsv_f32(const TrsvItem *it, int n_it, const float *A, double *V);
 
 /* ---- batched matrix-vector: y_it = alpha op(A_it) x_it + beta y_it ---- */
 typedef struct { int M, N, lda; int64_t a, x, y; int tri; } GemvItem;   /* tri (trans = 1 only): 1 A lower (rows >= j), 2 A upper (rows <= j) */

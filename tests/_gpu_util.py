@@ -24,6 +24,10 @@ class SymvItem(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("ld", ctypes.c_int), ("a", I64), ("x", I64), ("y", I64), ("p", I64)]
 
 
+class TrsvItem(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int), ("c", ctypes.c_int), ("pre", I64), ("L", I64), ("Di", I64), ("DiT", I64), ("v0", I64), ("v1", I64)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -49,6 +53,12 @@ def lib():
             getattr(L, nm).argtypes = [P, ctypes.c_int, P, P, P, P, P]
         for nm in ("bk_trtri_f64", "bk_trtri_f32"):
             getattr(L, nm).argtypes = [P, ctypes.c_int, P, P, P, P, P]
+        for nm in ("bk_trtri_diag_f64", "bk_trtri_diag_f32"):
+            getattr(L, nm).argtypes = [P, ctypes.c_int, P, P, P, P, P]
+        for nm in ("bk_trsv_prep_f64", "bk_trsv_prep_f32"):
+            getattr(L, nm).argtypes = [P, ctypes.c_int, P]
+        for nm in ("bk_trsv_f64", "bk_trsv_f32"):
+            getattr(L, nm).argtypes = [P, ctypes.c_int, P, P]
         for nm in ("bk_gemv_f64", "bk_gemv_f32", "bk_gemv_mixed"):
             sc = ctypes.c_float if nm == "bk_gemv_f32" else ctypes.c_double
             getattr(L, nm).argtypes = [ctypes.c_int, sc, sc, P, P, P, P, ctypes.c_int]

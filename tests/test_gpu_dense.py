@@ -2,7 +2,7 @@
 triangular inverse, matrix-vector products) against numpy/LAPACK on the same inputs."""
 import numpy as np
 import pytest
-from _gpu_util import lib, DBuf, GemmItem, MItem, GemvItem, SymvItem, items_array
+from _gpu_util import lib, DBuf, GemmItem, MItem, GemvItem, SymvItem, TrsvItem, items_array
 
 pytestmark = pytest.mark.gpu
 
@@ -122,6 +122,105 @@ def test_potrf_trtri_batched(dt):
         assert np.allclose(np.triu(Lig, 1), 0)
         err = np.abs(Lig @ Lg - np.eye(n)).max()
         assert err < (1e-10 if dt == np.float64 else 5e-3), (n, err)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_potrf_graph_replay(dt):
+    """The same factorization (plan and operands) four times: the first call runs eagerly, the
+    second captures its launch sequence into a graph and the later ones replay it. Unrelated
+    eager work (a GEMM) and a synchronization run between the calls; every call's factor, info
+    and pivot minima equal the first call's bit for bit."""
+    rng = np.random.default_rng(11)
+    sizes = [40, 64, 129, 300, 513]
+    mats = [_spd(n, rng, 1e2) for n in sizes]
+    buf, offs = _pack(mats, dt)
+    nb = [((n + 63) // 64) * 4096 for n in sizes]
+    doff = np.concatenate([[0], np.cumsum(nb)[:-1]]).astype(np.int64)
+    rows = [(n, k, offs[k], int(doff[k]), offs[k], offs[k]) for k, n in enumerate(sizes)]
+    it = items_array(MItem, rows)
+    dA = DBuf(buf); dD = DBuf(nbytes=int(sum(nb)) * np.dtype(dt).itemsize, dtype=dt)
+    dW = DBuf(nbytes=buf.nbytes, dtype=dt)
+    zi = np.zeros(len(sizes), np.int32); zp = np.zeros(len(sizes), dt)
+    dinfo = DBuf(zi); dpmin = DBuf(zp)
+    L = lib()
+    potrf = L.bk_potrf_f64 if dt == np.float64 else L.bk_potrf_f32
+    gemm = L.bk_gemm_f64 if dt == np.float64 else L.bk_gemm_f32
+    n = 200
+    X = rng.standard_normal((n, n)).astype(dt)
+    dX = DBuf(X.flatten(order="F")); dY = DBuf(nbytes=X.nbytes, dtype=dt)
+    git = items_array(GemmItem, [(n, n, n, n, n, n, 0, 0, 0, -1, 0)])
+    ref = None
+    for call in range(4):
+        L.bk_h2d(dA.p, buf.ctypes.data, buf.nbytes)
+        L.bk_h2d(dinfo.p, zi.ctypes.data, zi.nbytes); L.bk_h2d(dpmin.p, zp.ctypes.data, zp.nbytes)
+        assert potrf(it, len(rows), dA.p, dD.p, dW.p, dinfo.p, dpmin.p) == 0, call
+        assert gemm(0, 1, 1.0, 0.0, dX.p, dX.p, dY.p, git, 1, None) == 0
+        got = (dA.get(), dD.get(), dinfo.get(), dpmin.get())
+        y = dY.get().reshape((n, n), order="F")
+        assert np.allclose(y, X.astype(np.float64) @ X.T.astype(np.float64), rtol=1e-4, atol=1e-3 * n)
+        if ref is None:
+            ref = got
+            assert list(got[2]) == [0] * len(sizes)
+            for k, m in enumerate(sizes):
+                Lg = got[0][offs[k]:offs[k] + m * m].reshape((m, m), order="F").astype(np.float64)
+                assert np.allclose(Lg, np.linalg.cholesky(mats[k]), rtol=2e-4, atol=2e-4 * np.abs(Lg).max()), m
+        else:
+            for a, b in zip(got, ref):
+                assert np.array_equal(a, b), call
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_trsv_batched(dt):
+    """Factor, inverted 256 x 256 diagonal blocks, then the two blocked triangular solves for a
+    ragged batch (orders around the 64 and 256 block edges). Checked against the same block
+    algorithm in numpy on the device's own L and diagonal inverses (1e-12: it pins every index),
+    and against a direct solve with L L' (the f32 factor's diagonal inverses carry f32 error)."""
+    rng = np.random.default_rng(13)
+    sizes = [1000, 1, 63, 64, 65, 255, 256, 257, 600, 5]
+    mats = [_spd(n, rng, 1e3) for n in sizes]
+    buf, offs = _pack(mats, dt)
+    tot = buf.size
+    big = np.concatenate([buf, np.zeros(2 * tot, dt)])          # L | diagonal inverses | transposes
+    nb = [((n + 63) // 64) * 4096 for n in sizes]
+    doff = np.concatenate([[0], np.cumsum(nb)[:-1]]).astype(np.int64)
+    rows = [(n, k, offs[k], int(doff[k]), offs[k], tot + offs[k]) for k, n in enumerate(sizes)]
+    it = items_array(MItem, rows)
+    isz = np.dtype(dt).itemsize
+    dA = DBuf(big); dD = DBuf(nbytes=int(sum(nb)) * isz, dtype=dt); dW = DBuf(nbytes=tot * isz, dtype=dt)
+    dinfo = DBuf(np.zeros(len(sizes), np.int32)); dpmin = DBuf(np.zeros(len(sizes), dt))
+    L = lib()
+    f64 = dt == np.float64
+    assert (L.bk_potrf_f64 if f64 else L.bk_potrf_f32)(it, len(rows), dA.p, dD.p, dW.p, dinfo.p, dpmin.p) == 0
+    assert list(dinfo.get()) == [0] * len(sizes)
+    assert (L.bk_trtri_diag_f64 if f64 else L.bk_trtri_diag_f32)(it, len(rows), dA.p, dD.p, dA.p, dW.p, dinfo.p) == 0
+    voff = np.concatenate([[0], np.cumsum(sizes)[:-1]]).astype(np.int64)
+    totm = int(sum(sizes))
+    r = rng.standard_normal(totm)
+    dV = DBuf(np.concatenate([r, np.zeros(totm)]))
+    tv = items_array(TrsvItem, [(n, 0, 0, offs[k], tot + offs[k], 2 * tot + offs[k], int(voff[k]), totm + int(voff[k]))
+                                for k, n in enumerate(sizes)])
+    assert (L.bk_trsv_prep_f64 if f64 else L.bk_trsv_prep_f32)(tv, len(sizes), dA.p) == 0
+    assert (L.bk_trsv_f64 if f64 else L.bk_trsv_f32)(tv, len(sizes), dA.p, dV.p) == 0
+    z = dV.get()[:totm]
+    g = dA.get().astype(np.float64)
+    for k, n in enumerate(sizes):
+        Lk = g[offs[k]:offs[k] + n * n].reshape((n, n), order="F")
+        Dk = g[tot + offs[k]:tot + offs[k] + n * n].reshape((n, n), order="F")
+        rk = r[voff[k]:voff[k] + n]
+        y = rk.copy(); out = np.zeros(n)
+        for kb in range(0, n, 256):                                 # L out = y, block by block
+            e = min(n, kb + 256)
+            out[kb:e] = Dk[kb:e, kb:e] @ y[kb:e]
+            y[e:] -= Lk[e:, kb:e] @ out[kb:e]
+        zz = np.zeros(n)
+        for kb in reversed(range(0, n, 256)):                       # L' zz = out
+            e = min(n, kb + 256)
+            zz[kb:e] = Dk[kb:e, kb:e].T @ out[kb:e]
+            out[:kb] -= Lk[kb:e, :kb].T @ zz[kb:e]
+        zk = z[voff[k]:voff[k] + n]
+        assert np.abs(zk - zz).max() <= 1e-12 * np.abs(zz).max(), n
+        ref = np.linalg.solve(Lk @ Lk.T, rk)
+        assert np.abs(zk - ref).max() <= (1e-9 if f64 else 2e-3) * np.abs(ref).max(), n
 
 
 def test_gemv_triangular_ranges():

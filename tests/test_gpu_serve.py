@@ -42,6 +42,24 @@ def test_batch_equals_alone(alone):
         same(r, alone[case])
 
 
+@pytest.mark.parametrize("prec", ["double", "mixed"])
+def test_schur_apply_modes_agree(prec):
+    """The factor applied by its explicit inverse (alone) and by the blocked triangular solves (in
+    a batch): the same statuses, iteration counts and objectives to the batch tolerance."""
+    cases = MIX[:5]
+    rs = brisk_amd.solve_batch([tup(Instance(*c)) for c in cases], options=["-prec", prec, "-schurapply", "solves"])
+    for case, r in zip(cases, rs):
+        a = brisk_amd.solve_sdpa(*tup(Instance(*case)), options=["-prec", prec, "-schurapply", "inverse"])
+        print(case, prec, r.status_str, r.iterations, r.pobj, a.iterations, a.pobj, flush=True)
+        if prec == "double":    # the same bar as alone against batched, DIMACS errors (near 1e-9) to 1e-2
+            assert r.status == a.status and r.iterations == a.iterations
+            assert abs(r.pobj - a.pobj) <= 1e-10 * (1 + abs(a.pobj)) and abs(r.dobj - a.dobj) <= 1e-10 * (1 + abs(a.dobj))
+            assert np.allclose(r.dimacs, a.dimacs, rtol=1e-2, atol=1e-13)
+        else:                   # the f32 preconditioners differ: the PCG may take other rounds
+            assert r.status == a.status == 0
+            assert abs(r.pobj - a.pobj) <= 1e-7 * (1 + abs(a.pobj)) and abs(r.dobj - a.dobj) <= 1e-7 * (1 + abs(a.dobj))
+
+
 def test_serving_equals_alone(alone):
     """Continuous batching: three slots, requests arriving in waves while others run."""
     sv = brisk_amd.Server(["-prec", "double", "-slots", "3", "-heapgb", "2"])
