@@ -227,7 +227,7 @@ def serving(args):
         insts.append((a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"]))
     ms = [int(x[1].shape[0]) for x in insts]
     print("bench: %d requests, m from %d to %d (mean %.0f)" % (count, min(ms), max(ms), np.mean(ms)), file=sys.stderr, flush=True)
-    opts = ["-prec", "mixed", "-slots", str(args.slots)]
+    opts = ["-prec", "mixed", "-slots", str(args.slots), "-schurapply", args.schur_apply]
     for w in range(args.warmup):
         span, _, st, _ = run_stream(insts[:64], np.zeros(64), opts)
         print("bench: warmup %d: 64 requests in %.3fs" % (w, span), file=sys.stderr, flush=True)
@@ -308,6 +308,8 @@ def main():
     ap.add_argument("--load", type=float, default=0.8, help="c5: Poisson arrival rate as a fraction of the saturation rate")
     ap.add_argument("--cpu-stride", type=int, default=16, help="c5: the CPU baseline solves every k-th request")
     ap.add_argument("--cpu-serving-sample", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--schur-apply", default="inverse", choices=["auto", "inverse", "solves"],
+                    help="c5: how the Schur factor is applied in the PCG (-schurapply)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)

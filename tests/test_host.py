@@ -84,7 +84,7 @@ def call_data(L, d, *o):
 
 
 @pytest.mark.parametrize("o", [("-nosuch",), ("-prec",), ("-prec", "quad"), ("-tol", "0"), ("-tol", "-1"),
-                               ("-maxit", "-3"), ("-threads",)])
+                               ("-maxit", "-3"), ("-threads",), ("-schurapply",), ("-schurapply", "lu")])
 def test_bad_options(lib, o):
     """An unknown option or a bad value is exit code 1 (capi.c:175-192), before any GPU work."""
     setup_sig(lib)
