@@ -825,8 +825,23 @@ extern "C" int bk_trsv_f32(const TrsvItem *it, int n_it, const float *A, double 
 // ---- batched matrix-vector products -------------------------------------------------------
 // trans = 1: y_j = alpha sum_i A[i, j] x_i + beta y_j, one wave per output (column of A read
 // contiguously), fixed-order lane partials + butterfly: deterministic.
-// trans = 0: y_i = alpha sum_j A[i, j] x_j + beta y_i, one thread per row with the columns in
-// order (rows read coalesced across the wave).
+// trans = 0: y_i = alpha sum_j A[i, j] x_j + beta y_i, one thread per row (workgroups of one
+// wave), GVN_U partial sums over the columns (rows read coalesced across the wave).
+// loads in flight per lane (gemv_t: columns of a few thousand rows are common, and 8 serve them
+// best) or per thread (gemv_n), and their partial sums' fixed order (pairwise)
+#define GVT_U 8
+#define GVN_U 16
+template <typename T, int U>
+__device__ __forceinline__ T gv_sum(const T (&p)[U]) {
+    T s[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) s[u] = p[u];
+#pragma unroll
+    for (int h = U / 2; h > 0; h /= 2)
+#pragma unroll
+        for (int u = 0; u < h; u++) s[u] = s[u] + s[u + h];
+    return s[0];
+}
 template <typename TA, typename TX, typename TY, typename TACC>
 __global__ __launch_bounds__(256) void k_gemv_t(const GemvItem *__restrict__ items, const int *__restrict__ pre, int nit,
                                                 const TA *__restrict__ A, const TX *__restrict__ x, TY *__restrict__ y, TACC alpha,
@@ -838,18 +853,26 @@ __global__ __launch_bounds__(256) void k_gemv_t(const GemvItem *__restrict__ ite
     if (j >= it.N) return;
     const TA *col = A + it.a + (int64_t)j * it.lda;
     const TX *xv = x + it.x;
-    TACC s = 0;
     const int i0 = it.tri == 1 ? j : 0, i1 = it.tri == 2 ? j + 1 : it.M;     // triangular A: only its nonzero rows
-    TACC s1 = 0, s2 = 0, s3 = 0;
+    // GVT_U loads of the column in flight per lane (independent partial sums, added in a fixed
+    // order): a long column streams at HBM speed instead of at the latency of a few loads
+    TACC p[GVT_U];
+#pragma unroll
+    for (int u = 0; u < GVT_U; u++) p[u] = 0;
     int i = i0 + lane;
-    for (; i + 192 < i1; i += 256) {
-        s += (TACC)col[i] * (TACC)xv[i];
-        s1 += (TACC)col[i + 64] * (TACC)xv[i + 64];
-        s2 += (TACC)col[i + 128] * (TACC)xv[i + 128];
-        s3 += (TACC)col[i + 192] * (TACC)xv[i + 192];
+    for (; i + 64 * (GVT_U - 1) < i1; i += 64 * GVT_U) {
+#pragma unroll
+        for (int u = 0; u < GVT_U; u++) p[u] += (TACC)col[i + 64 * u] * (TACC)xv[i + 64 * u];
     }
-    for (; i < i1; i += 64) s += (TACC)col[i] * (TACC)xv[i];
-    s = (s + s1) + (s2 + s3);
+    if (i < i1) {       // the tail (< 64 GVT_U rows) as one batch, addresses clamped to i1 - 1 and masked
+        TA a[GVT_U];
+        TX b[GVT_U];
+#pragma unroll
+        for (int u = 0; u < GVT_U; u++) { const int ii = min(i + 64 * u, i1 - 1); a[u] = col[ii]; b[u] = xv[ii]; }
+#pragma unroll
+        for (int u = 0; u < GVT_U; u++) p[u] += i + 64 * u < i1 ? (TACC)a[u] * (TACC)b[u] : (TACC)0;
+    }
+    TACC s = gv_sum<TACC, GVT_U>(p);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     if (lane == 0) {
@@ -860,17 +883,30 @@ __global__ __launch_bounds__(256) void k_gemv_t(const GemvItem *__restrict__ ite
     }
 }
 template <typename TA, typename TX, typename TY, typename TACC>
-__global__ __launch_bounds__(256) void k_gemv_n(const GemvItem *__restrict__ items, const int *__restrict__ pre, int nit,
+__global__ __launch_bounds__(64) void k_gemv_n(const GemvItem *__restrict__ items, const int *__restrict__ pre, int nit,
                                                 const TA *__restrict__ A, const TX *__restrict__ x, TY *__restrict__ y, TACC alpha,
                                                 TACC beta) {
     const int q = flat_item(pre, nit, blockIdx.x);
     const GemvItem it = items[q];
-    const int i = (blockIdx.x - pre[q]) * 256 + threadIdx.x;
+    const int i = (blockIdx.x - pre[q]) * 64 + threadIdx.x;
     if (i >= it.M) return;
     const TA *row = A + it.a + i;
     const TX *xv = x + it.x;
-    TACC s = 0;
-    for (int j = 0; j < it.N; j++) s += (TACC)row[(int64_t)j * it.lda] * (TACC)xv[j];
+    // GVN_U columns' loads in flight per thread (independent partial sums over j = GVN_U q + u,
+    // added in a fixed order)
+    TACC p[GVN_U];
+#pragma unroll
+    for (int u = 0; u < GVN_U; u++) p[u] = 0;
+    int j = 0;
+    for (; j + GVN_U - 1 < it.N; j += GVN_U) {
+        TA a[GVN_U];
+#pragma unroll
+        for (int u = 0; u < GVN_U; u++) a[u] = row[(int64_t)(j + u) * it.lda];
+#pragma unroll
+        for (int u = 0; u < GVN_U; u++) p[u] += (TACC)a[u] * (TACC)xv[j + u];
+    }
+    for (; j < it.N; j++) p[0] += (TACC)row[(int64_t)j * it.lda] * (TACC)xv[j];
+    TACC s = gv_sum<TACC, GVN_U>(p);
     TY *yp = y + it.y + i;
     TACC v = alpha * s;
     if (beta != (TACC)0) v += beta * (TACC)*yp;
@@ -880,7 +916,7 @@ template <typename TA, typename TX, typename TY, typename TACC>
 static int gemv_run(int trans, TACC alpha, TACC beta, const TA *A, const TX *x, TY *y, const GemvItem *it, int n_it) {
     if (n_it <= 0) return 0;
     std::vector<int64_t> cnt(n_it);
-    for (int q = 0; q < n_it; q++) cnt[q] = trans ? (it[q].N + 3) / 4 : (it[q].M + 255) / 256;
+    for (int q = 0; q < n_it; q++) cnt[q] = trans ? (it[q].N + 3) / 4 : (it[q].M + 63) / 64;
     const GemvItem *d = nullptr;
     const int *pre = nullptr;
     const int64_t nb = flat_stage_items(it, sizeof(GemvItem) * n_it, cnt, (const void **)&d, &pre);
@@ -889,7 +925,7 @@ static int gemv_run(int trans, TACC alpha, TACC beta, const TA *A, const TX *x, 
     const dim3 grid((unsigned)nb);
     const int tok = bk_prof_begin();
     if (trans) hipLaunchKernelGGL((k_gemv_t<TA, TX, TY, TACC>), grid, dim3(256), 0, g_bk_stream, d, pre, n_it, A, x, y, alpha, beta);
-    else hipLaunchKernelGGL((k_gemv_n<TA, TX, TY, TACC>), grid, dim3(256), 0, g_bk_stream, d, pre, n_it, A, x, y, alpha, beta);
+    else hipLaunchKernelGGL((k_gemv_n<TA, TX, TY, TACC>), grid, dim3(64), 0, g_bk_stream, d, pre, n_it, A, x, y, alpha, beta);
     if (tok >= 0) {
         double fl = 0, by = 0;
         for (int q = 0; q < n_it; q++) { fl += 2.0 * it[q].M * it[q].N; by += (double)sizeof(TA) * it[q].M * it[q].N + sizeof(TX) * (double)(trans ? it[q].M : it[q].N) + sizeof(TY) * (double)(trans ? it[q].N : it[q].M) * (beta != (TACC)0 ? 2 : 1); }
