@@ -308,7 +308,7 @@ def main():
     ap.add_argument("--load", type=float, default=0.8, help="c5: Poisson arrival rate as a fraction of the saturation rate")
     ap.add_argument("--cpu-stride", type=int, default=16, help="c5: the CPU baseline solves every k-th request")
     ap.add_argument("--cpu-serving-sample", type=int, default=0, help=argparse.SUPPRESS)
-    ap.add_argument("--schur-apply", default="inverse", choices=["auto", "inverse", "solves"],
+    ap.add_argument("--schur-apply", default="auto", choices=["auto", "inverse", "solves"],
                     help="c5: how the Schur factor is applied in the PCG (-schurapply)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")

@@ -840,15 +840,15 @@ static int schur_factor_(Solve **S, int ns, const int *act, int force64, int *fa
             /* the inverse: its flops at the rates measured for it (f32 3e13, f64 1.5e13 flop/s) and
              * a fixed 0.3 ms of launches; the solves: two launches of about 15 us per block column
              * (measured over serving batches), 6 preconditioner applications per f32 factor (the
-             * PCG's usual count), 3 per f64. The solves lengthen every PCG round by their launch
-             * chain, which the serving latency feels, so they are taken only where the inverse
-             * costs more than 15 ms and over three times the solves: a saturated serving batch,
-             * never a single solve or a lightly loaded batch. Used with -schurapply auto only
-             * (the default is the inverse: over the serving stream the solves gained a few percent
-             * of throughput at saturation and cost 10-30% of latency under Poisson arrivals). */
+             * PCG's usual count), 3 per f64 (the FP64 factor's PCG ends in a round or two). The
+             * solves lengthen every PCG round by their launch chain, which the latency feels, so
+             * they are taken for an f64 group whose inverse costs over 3 ms and twice the solves,
+             * for an f32 group only past 15 ms and three times the solves. Measured over the
+             * serving stream: 365 -> 384 solves/s at saturation, Poisson p99 0.87 -> 1.02 s (at a
+             * 5% higher rate); f32 groups on the solves too added latency for no throughput. */
             const double t_inv = fl[prec] / (prec ? 1.5e13 : 3.0e13) + 3e-4;
             const double t_sv = (prec ? 3 : 6) * 2.0 * nbk[prec] * 15e-6;
-            use_trsv[prec] = t_inv > 15e-3 && t_sv < t_inv / 3;
+            use_trsv[prec] = prec ? t_inv > 3e-3 && t_sv < t_inv / 2 : t_inv > 15e-3 && t_sv < t_inv / 3;
         }
         /* a solve's -schurapply option (inverse, solves, or auto: the model) */
         for (int q = 0; q < ns; q++)

@@ -59,7 +59,7 @@ typedef struct {
     double mixed_until;     /* mixed: f32 factor while max(relgap, pinf, dinf) exceeds this (1e-6) */
     double time_limit;      /* s, <= 0 none */
     int hsd;                /* 1: self-dual embedding, 0: infeasible start */
-    int apply;              /* solves with the Schur factor: 0 explicit inverse (default), 1 block triangular solves, -1 a cost model per batch */
+    int apply;              /* solves with the Schur factor: -1 a cost model per batch (default), 0 explicit inverse, 1 block triangular solves */
 } SolveOpts;
 
 typedef struct Solve {
