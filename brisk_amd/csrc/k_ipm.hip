@@ -663,6 +663,69 @@ __device__ __forceinline__ int sturm_count(const double *a, const double *b, int
     }
     return cnt;
 }
+// the smallest Ritz value of the leading m x m tridiagonal (al, be) after kk of K steps of a block
+// of order n: bisection with 64 Sturm counts per round (one wave, lane = lane index), then two
+// inverse iterations below it for the last eigenvector entry. out = theta - |beta z_last| (or
+// -1e300 for coefficients that are not finite); stop when converged to tol, at a breakdown or at K.
+__device__ void lz_ritz(const double *al, const double *be, int kk, int K, int n, double tol, int lane, double &out_, int &stop_) {
+    int m = kk, brk = 0;
+    double scale = 0;
+    for (int u = 0; u < kk; u++) {
+        if (fabs(al[u]) > scale) scale = fabs(al[u]);
+        if (!(be[u] > 1e-12 * (scale + 1e-300))) { m = u + 1; brk = 1; break; }
+    }
+    int bad = 0;
+    for (int u = 0; u < m; u++) if (!isfinite(al[u]) || !isfinite(be[u])) bad = 1;
+    double out = -1e300;
+    int stop = 1;
+    if (!bad) {
+        double lo = INFINITY, hi = -INFINITY;
+        for (int u = 0; u < m; u++) {
+            const double r = (u > 0 ? fabs(be[u - 1]) : 0.0) + (u + 1 < m ? fabs(be[u]) : 0.0);
+            lo = fmin(lo, al[u] - r); hi = fmax(hi, al[u] + r);
+        }
+        for (int round = 0; round < 16; round++) {
+            const double w_ = hi - lo;
+            if (!(w_ > 2e-16 * fmax(fabs(lo), fabs(hi))) || w_ <= 1e-300) break;
+            const double x = lo + w_ * (double)(lane + 1) / 65.0;
+            const int c = sturm_count(al, be, m, x);
+            const unsigned long long bal = __ballot(c >= 1);
+            const int f = bal ? __ffsll((long long)bal) - 1 : 63;
+            const double nlo = f > 0 ? lo + w_ * (double)f / 65.0 : lo;
+            hi = lo + w_ * (double)(f + 1) / 65.0;
+            lo = nlo;
+        }
+        const double th = 0.5 * (lo + hi);
+        double res = 0;
+        if (!brk && m < n) {
+            // (T - sigma I) x = r, sigma just below lambda_min (an SPD tridiagonal solve)
+            const double sig = lo - (hi - lo) - 1e-15 * (scale + 1e-300);
+            double x[BK_LZK], dd[BK_LZK];
+#pragma unroll
+            for (int u = 0; u < BK_LZK; u++) x[u] = 1.0;
+            for (int iter = 0; iter < 2; iter++) {
+                dd[0] = al[0] - sig;
+                for (int u = 1; u < m; u++) {
+                    if (fabs(dd[u - 1]) < 1e-300) dd[u - 1] = 1e-300;
+                    const double l = be[u - 1] / dd[u - 1];
+                    dd[u] = al[u] - sig - l * be[u - 1];
+                    x[u] -= l * x[u - 1];
+                }
+                if (fabs(dd[m - 1]) < 1e-300) dd[m - 1] = 1e-300;
+                x[m - 1] /= dd[m - 1];
+                for (int u = m - 2; u >= 0; u--) x[u] = (x[u] - be[u] * x[u + 1]) / dd[u];
+                double nx = 0;
+                for (int u = 0; u < m; u++) nx += x[u] * x[u];
+                nx = 1.0 / sqrt(nx);
+                for (int u = 0; u < m; u++) x[u] *= nx;
+            }
+            res = fabs(be[m - 1] * x[m - 1]);
+        }
+        out = th - res;
+        stop = brk || kk >= K || res <= tol * fabs(th) + 1e-14 * scale;
+    }
+    out_ = out; stop_ = stop;
+}
 __global__ __launch_bounds__(LZF) void k_lz_fused(const LzfItem *__restrict__ it, int nit, double *__restrict__ H) {
     const int qi = blockIdx.x;
     if (qi >= nit) return;
@@ -739,65 +802,11 @@ __global__ __launch_bounds__(LZF) void k_lz_fused(const LzfItem *__restrict__ it
         const int kk = j + 1;
         if (!(kk == K || (kk >= 6 && kk % 3 == 0))) continue;
         __syncthreads();
-        // the smallest Ritz value of the leading m x m tridiagonal: bisection with 64 Sturm counts
-        // per round (wave 0), then two inverse iterations below it for the last eigenvector entry
+        // the smallest Ritz value of the leading m x m tridiagonal (wave 0)
         if (t < 64) {
-            int m = kk, brk = 0;
-            double scale = 0;
-            for (int u = 0; u < kk; u++) {
-                if (fabs(al[u]) > scale) scale = fabs(al[u]);
-                if (!(be[u] > 1e-12 * (scale + 1e-300))) { m = u + 1; brk = 1; break; }
-            }
-            int bad = 0;
-            for (int u = 0; u < m; u++) if (!isfinite(al[u]) || !isfinite(be[u])) bad = 1;
-            double out = -1e300;
-            int stop = 1;
-            if (!bad) {
-                double lo = INFINITY, hi = -INFINITY;
-                for (int u = 0; u < m; u++) {
-                    const double r = (u > 0 ? fabs(be[u - 1]) : 0.0) + (u + 1 < m ? fabs(be[u]) : 0.0);
-                    lo = fmin(lo, al[u] - r); hi = fmax(hi, al[u] + r);
-                }
-                for (int round = 0; round < 16; round++) {
-                    const double w_ = hi - lo;
-                    if (!(w_ > 2e-16 * fmax(fabs(lo), fabs(hi))) || w_ <= 1e-300) break;
-                    const double x = lo + w_ * (double)(lane + 1) / 65.0;
-                    const int c = sturm_count(al, be, m, x);
-                    const unsigned long long bal = __ballot(c >= 1);
-                    const int f = bal ? __ffsll((long long)bal) - 1 : 63;
-                    const double nlo = f > 0 ? lo + w_ * (double)f / 65.0 : lo;
-                    hi = lo + w_ * (double)(f + 1) / 65.0;
-                    lo = nlo;
-                }
-                const double th = 0.5 * (lo + hi);
-                double res = 0;
-                if (!brk && m < n) {
-                    // (T - sigma I) x = r, sigma just below lambda_min (an SPD tridiagonal solve)
-                    const double sig = lo - (hi - lo) - 1e-15 * (scale + 1e-300);
-                    double x[BK_LZK], dd[BK_LZK];
-#pragma unroll
-                    for (int u = 0; u < BK_LZK; u++) x[u] = 1.0;
-                    for (int iter = 0; iter < 2; iter++) {
-                        dd[0] = al[0] - sig;
-                        for (int u = 1; u < m; u++) {
-                            if (fabs(dd[u - 1]) < 1e-300) dd[u - 1] = 1e-300;
-                            const double l = be[u - 1] / dd[u - 1];
-                            dd[u] = al[u] - sig - l * be[u - 1];
-                            x[u] -= l * x[u - 1];
-                        }
-                        if (fabs(dd[m - 1]) < 1e-300) dd[m - 1] = 1e-300;
-                        x[m - 1] /= dd[m - 1];
-                        for (int u = m - 2; u >= 0; u--) x[u] = (x[u] - be[u] * x[u + 1]) / dd[u];
-                        double nx = 0;
-                        for (int u = 0; u < m; u++) nx += x[u] * x[u];
-                        nx = 1.0 / sqrt(nx);
-                        for (int u = 0; u < m; u++) x[u] *= nx;
-                    }
-                    res = fabs(be[m - 1] * x[m - 1]);
-                }
-                out = th - res;
-                stop = brk || kk >= K || res <= L.tol * fabs(th) + 1e-14 * scale;
-            }
+            double out;
+            int stop;
+            lz_ritz(al, be, kk, K, n, L.tol, lane, out, stop);
             if (t == 0) { lam_sh = out; stop_sh = stop; }
         }
         __syncthreads();
