@@ -49,10 +49,18 @@ __global__ __launch_bounds__(256) void k_dot_chunks(const DotItem *__restrict__ 
     const int t = threadIdx.x;
     double s = 0;
     const double *x = H + d.xo, *y = H + d.yo;
+    if ((c + 1) * DCH <= d.len) {               // a full chunk: no guards, the loads overlap
+#pragma unroll
+        for (int u = 0; u < DCH / 256; u++) {
+            const int64_t e = c * DCH + t + 256 * u;
+            s += x[e] * y[e];
+        }
+    } else {
 #pragma unroll 4
-    for (int u = 0; u < DCH / 256; u++) {
-        const int64_t e = c * DCH + t + 256 * u;
-        if (e < d.len) s += x[e] * y[e];
+        for (int u = 0; u < DCH / 256; u++) {
+            const int64_t e = c * DCH + t + 256 * u;
+            if (e < d.len) s += x[e] * y[e];
+        }
     }
     sh[t] = s;
     __syncthreads();
@@ -63,14 +71,20 @@ __global__ __launch_bounds__(256) void k_dot_chunks(const DotItem *__restrict__ 
     if (t == 0) part[cbase[q] + c] = sh[0];
 }
 // results in [moff, moff + mlen) go straight to the host mirror (pinned, device-visible)
-__global__ void k_dot_final(const DotItem *__restrict__ it, const int64_t *__restrict__ cbase, int nit, double *__restrict__ H,
-                            const double *__restrict__ part, int64_t moff, int64_t mlen, double *__restrict__ mirror) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+// one wave per item: lane l adds the chunks l, l + 64, ... in order, then a butterfly over the
+// lanes (a fixed order that depends on the chunk count only)
+__global__ __launch_bounds__(256) void k_dot_final(const DotItem *__restrict__ it, const int64_t *__restrict__ cbase, int nit,
+                                                   double *__restrict__ H, const double *__restrict__ part, int64_t moff, int64_t mlen,
+                                                   double *__restrict__ mirror) {
+    const int q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (q >= nit) return;
     const DotItem d = it[q];
     const int64_t nch = (d.len + DCH - 1) / DCH;
     double s = 0;
-    for (int64_t c = 0; c < nch; c++) s += part[cbase[q] + c];
+    for (int64_t c = lane; c < nch; c += 64) s += part[cbase[q] + c];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane) return;
     if (mirror && d.out >= moff && d.out < moff + mlen) mirror[d.out - moff] = s;
     else H[d.out] = s;
 }
@@ -107,7 +121,7 @@ extern "C" int bk_dots(const DotItem *it, int n_it) {
     if (nb > 0) hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, dcb, pre, n_it, (const double *)g_heap, g_partial);
     if (tok >= 0) { double by = 0; for (int q = 0; q < n_it; q++) by += 16.0 * it[q].len; bk_prof_end(tok, 13, 0, by); }
     if (BK_LAUNCH_CHECK("k_dot_chunks")) return -1;
-    hipLaunchKernelGGL(k_dot_final, dim3((n_it + 255) / 256), dim3(256), 0, g_bk_stream, d, dcb, n_it, (double *)g_heap, g_partial,
+    hipLaunchKernelGGL(k_dot_final, dim3((n_it + 3) / 4), dim3(256), 0, g_bk_stream, d, dcb, n_it, (double *)g_heap, g_partial,
                        g_mirror_off, g_mirror_len, g_mirror);
     return BK_LAUNCH_CHECK("k_dot_final");
 }

@@ -131,8 +131,9 @@ size_t bk_heap_size(void);
 
 /* ---- segmented deterministic reductions ---- */
 /* out[s] = sum_q x[xo + q] * y[yo + q], q < len, for each item; chunked partial sums (4096
- * elements per chunk, fixed-order tree in a chunk) summed in chunk order: the result depends on
- * the data and len only. All offsets are elements of double from the heap base. */
+ * elements per chunk, fixed-order tree in a chunk), the chunks summed by 64 lanes (lane l the
+ * chunks l, l + 64, ... in order) and a butterfly: the result depends on the data and len only.
+ * All offsets are elements of double from the heap base. */
 typedef struct { int64_t len, xo, yo, out; } DotItem;
 int bk_dots(const DotItem *it, int n_it);
 /* dot results whose output offset lies in [off, off + len) (doubles) are written to host[out - off]
