@@ -215,17 +215,34 @@ extern "C" int bk_d2f(int64_t n, int64_t src_off, int64_t dst_off_f32) {
     const int q = flat_item(pre, nit, blockIdx.x);                          \
     const MItem m = it[q];                                                  \
     const int64_t lb = blockIdx.x - pre[q], nb = pre[q + 1] - pre[q];
+// G = (G + G')/2 over pairs of 32 x 32 tiles (I, J), I >= J, both staged in LDS so that every
+// read and write is coalesced; each entry pair gets 0.5 (g_rc + g_cr)
 __global__ void k_symmetrize(const MItem *__restrict__ it, const int *__restrict__ pre, int nit, double *__restrict__ H) {
     FLAT_MITEM
-    const int n = m.n;
+    const int n = m.n, tiles = (n + 31) / 32;
     double *G = H + m.off;
-    for (int64_t e = lb * blockDim.x + threadIdx.x; e < (int64_t)n * n; e += nb * blockDim.x) {
-        const int r = (int)(e % n), c = (int)(e / n);
-        if (r > c) {
-            const double v = 0.5 * (G[r + (int64_t)c * n] + G[c + (int64_t)r * n]);
-            G[r + (int64_t)c * n] = v;
-            G[c + (int64_t)r * n] = v;
+    __shared__ double ta[32][33], tb[32][33];                 // ta[c][r] = G(32 I + r, 32 J + c), tb the mirror tile
+    const int lx = threadIdx.x & 31, ly = threadIdx.x >> 5;   // 256 threads: 32 x 8
+    for (int64_t tt = lb; tt < (int64_t)tiles * (tiles + 1) / 2; tt += nb) {
+        int I = (int)((sqrt(8.0 * (double)tt + 1.0) - 1.0) * 0.5);
+        while ((int64_t)I * (I + 1) / 2 > tt) I--;
+        while ((int64_t)(I + 1) * (I + 2) / 2 <= tt) I++;
+        const int J = (int)(tt - (int64_t)I * (I + 1) / 2);
+        const int r0 = 32 * I, c0 = 32 * J;
+        for (int y = ly; y < 32; y += 8) {
+            if (r0 + lx < n && c0 + y < n) ta[y][lx] = G[(r0 + lx) + (int64_t)(c0 + y) * n];    // G(r0 + lx, c0 + y)
+            if (c0 + lx < n && r0 + y < n) tb[y][lx] = G[(c0 + lx) + (int64_t)(r0 + y) * n];    // G(c0 + lx, r0 + y)
         }
+        __syncthreads();
+        for (int y = ly; y < 32; y += 8) {
+            // G(r0 + lx, c0 + y) and its mirror G(c0 + y, r0 + lx) = tb[lx][y]
+            if (r0 + lx < n && c0 + y < n && (I > J || lx > y))
+                G[(r0 + lx) + (int64_t)(c0 + y) * n] = 0.5 * (ta[y][lx] + tb[lx][y]);
+            // G(c0 + lx, r0 + y) and its mirror G(r0 + y, c0 + lx) = ta[lx][y]
+            if (c0 + lx < n && r0 + y < n && (I > J || y > lx))
+                G[(c0 + lx) + (int64_t)(r0 + y) * n] = 0.5 * (tb[y][lx] + ta[lx][y]);
+        }
+        __syncthreads();
     }
 }
 template <typename T>
@@ -286,7 +303,7 @@ static int64_t mitem_stage(const MItem *it, int n_it, int kind, const MItem **d,
     std::vector<int64_t> cnt(n_it);
     for (int q = 0; q < n_it; q++) {
         const int64_t n = it[q].n, t = (n + 31) / 32;
-        cnt[q] = kind == 1 ? flat_cap((n + 255) / 256, 1024) : kind == 2 ? flat_cap((n * n + 255) / 256, 1024) : flat_cap(t * t, 512);
+        cnt[q] = kind == 1 ? flat_cap((n + 255) / 256, 1024) : kind == 2 ? flat_cap(t * (t + 1) / 2, 512) : flat_cap(t * t, 512);
     }
     return flat_stage_items(it, sizeof(MItem) * n_it, cnt, (const void **)d, pre);
 }
