@@ -39,8 +39,10 @@ template <> struct MF<float> {
     static __device__ __forceinline__ int out(int g, int r) { return 4 * g + r; }
 };
 
-// C = alpha op(A) op(B) + beta C for one BM x BN tile of one item. Four waves in a 2 x 2 grid,
-// each owning a (BM/2) x (BN/2) quadrant of 16 x 16 MFMA tiles. The MFMA is fed with the B tile
+// C = alpha op(A) op(B) + beta C for one BM x BN tile of one item. Four waves in a 2 x 2 grid
+// (64 x 64 tiles) or eight in a 2 x 4 grid (128 x 128 tiles: two waves per SIMD, so that one's
+// LDS reads and barrier waits overlap the other's products), each owning a (BM/2) x (BN/2) or
+// (BM/2) x (BN/4) part of 16 x 16 MFMA tiles. The MFMA is fed with the B tile
 // as its A operand and the A tile as its B operand, so each output register row is a run of 16
 // consecutive rows of column-major C (coalesced stores). LDS holds two K slabs: slab s + 1 is
 // fetched into registers before slab s is multiplied and written to the other buffer after it,
@@ -49,15 +51,16 @@ template <> struct MF<float> {
 // zeroed when it is written to LDS (no branches around the loads, which stay in flight across
 // the products). Rows of a slab are padded to BM + 1 (BN + 1) elements: 16 consecutive k rows
 // written by consecutive threads then fall in distinct banks.
+#define GEMM_NT(BM_) ((BM_) == 128 ? 512 : 256)    // threads of a k_gemm workgroup
 template <typename T, int TA, int TB, int BM, int BN>
-__global__ __launch_bounds__(256) void k_gemm(const GemmItem *__restrict__ items, const int *__restrict__ pre, int nitems,
+__global__ __launch_bounds__(GEMM_NT(BM)) void k_gemm(const GemmItem *__restrict__ items, const int *__restrict__ pre, int nitems,
                                               const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, T alpha,
                                               T beta, const int *__restrict__ info) {
     typedef typename MF<T>::v4 v4;
-    constexpr int BK = GBK;
-    constexpr int WM = BM / 2, WN = BN / 2, FM = WM / 16, FN = WN / 16;
-    constexpr int LA = BM * BK / 256, LB = BN * BK / 256;
-    static_assert(256 % BM == 0 && 256 % BN == 0 && 256 % BK == 0 && LA <= 32 && LB <= 32, "tile shape");
+    constexpr int BK = GBK, NT = GEMM_NT(BM);
+    constexpr int WM = BM / 2, WN = BN / (NT / 128), FM = WM / 16, FN = WN / 16;
+    constexpr int LA = BM * BK / NT, LB = BN * BK / NT;
+    static_assert(NT % BM == 0 && NT % BN == 0 && NT % BK == 0 && LA <= 32 && LB <= 32 && FN >= 1, "tile shape");
     const int item = flat_item(pre, nitems, blockIdx.x);
     const GemmItem it = items[item];
     if (it.guard >= 0 && info && info[it.guard] != 0) return;
@@ -86,9 +89,9 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmItem *__restrict__ items
         for (int j = 0; j < FN; j++)
 #pragma unroll
             for (int r = 0; r < 4; r++) acc[i][j][r] = 0;
-    // load geometry of element e = tid + 256 r: op(A)(i, k) and op(B)(k, j), with the coordinate
+    // load geometry of element e = tid + NT r: op(A)(i, k) and op(B)(k, j), with the coordinate
     // that advances with r stepping SA (SB) per r
-    constexpr int SA = TA == 0 ? 256 / BM : 256 / BK, SB = TB == 0 ? 256 / BK : 256 / BN;
+    constexpr int SA = TA == 0 ? NT / BM : NT / BK, SB = TB == 0 ? NT / BK : NT / BN;
     const int ia = TA == 0 ? tid % BM : tid / BK, ka = TA == 0 ? tid / BM : tid % BK;
     const int jb = TB == 0 ? tid / BK : tid % BN, kq = TB == 0 ? tid % BK : tid / BN;
     const int64_t stA = (int64_t)SA * lda, stB = (int64_t)SB * ldb;
@@ -122,13 +125,13 @@ __global__ __launch_bounds__(256) void k_gemm(const GemmItem *__restrict__ items
     auto store = [&](int buf) {
 #pragma unroll
         for (int r = 0; r < LA; r++) {
-            const int e = tid + 256 * r;
+            const int e = tid + NT * r;
             const T x = ((ma >> r) & 1) ? ra[r] : (T)0;
             if (TA == 0) As[buf][e / BM][e % BM] = x; else As[buf][e % BK][e / BK] = x;
         }
 #pragma unroll
         for (int r = 0; r < LB; r++) {
-            const int e = tid + 256 * r;
+            const int e = tid + NT * r;
             const T x = ((mb >> r) & 1) ? rb[r] : (T)0;
             if (TB == 0) Bs[buf][e % BK][e / BK] = x; else Bs[buf][e / BN][e % BN] = x;
         }
@@ -259,7 +262,7 @@ static int gemm_exec(const char *dbuf, const GemmPlan &P, int ta, int tb, T alph
         const int *pre = (const int *)(dbuf + P.pre[big]);
         const dim3 grid((unsigned)P.nb[big]);
         const int nitems = P.n;
-#define GL(TA_, TB_, BM_) hipLaunchKernelGGL((k_gemm<T, TA_, TB_, BM_, BM_>), grid, dim3(256), 0, g_bk_stream, d, pre, nitems, A, B, C, alpha, beta, info)
+#define GL(TA_, TB_, BM_) hipLaunchKernelGGL((k_gemm<T, TA_, TB_, BM_, BM_>), grid, dim3(GEMM_NT(BM_)), 0, g_bk_stream, d, pre, nitems, A, B, C, alpha, beta, info)
         if (big) {
             if (!ta && !tb) GL(0, 0, 128); else if (!ta && tb) GL(0, 1, 128); else if (ta && !tb) GL(1, 0, 128); else GL(1, 1, 128);
         } else {
