@@ -39,10 +39,9 @@ template <> struct MF<float> {
     static __device__ __forceinline__ int out(int g, int r) { return 4 * g + r; }
 };
 
-// C = alpha op(A) op(B) + beta C for one BM x BN tile of one item. Four waves in a 2 x 2 grid
-// (64 x 64 tiles) or eight in a 2 x 4 grid (128 x 128 tiles: two waves per SIMD, so that one's
-// LDS reads and barrier waits overlap the other's products), each owning a (BM/2) x (BN/2) or
-// (BM/2) x (BN/4) part of 16 x 16 MFMA tiles. The MFMA is fed with the B tile
+// C = alpha op(A) op(B) + beta C for one BM x BN tile of one item. Eight waves in a 2 x 4 grid
+// (two waves per SIMD, so that one's LDS reads and barrier waits overlap the other's products),
+// each owning a (BM/2) x (BN/4) part of 16 x 16 MFMA tiles. The MFMA is fed with the B tile
 // as its A operand and the A tile as its B operand, so each output register row is a run of 16
 // consecutive rows of column-major C (coalesced stores). LDS holds two K slabs: slab s + 1 is
 // fetched into registers before slab s is multiplied and written to the other buffer after it,
@@ -51,7 +50,7 @@ template <> struct MF<float> {
 // zeroed when it is written to LDS (no branches around the loads, which stay in flight across
 // the products). Rows of a slab are padded to BM + 1 (BN + 1) elements: 16 consecutive k rows
 // written by consecutive threads then fall in distinct banks.
-#define GEMM_NT(BM_) ((BM_) == 128 ? 512 : 256)    // threads of a k_gemm workgroup
+#define GEMM_NT(BM_) 512                           // threads of a k_gemm workgroup (either tile size)
 template <typename T, int TA, int TB, int BM, int BN>
 __global__ __launch_bounds__(GEMM_NT(BM)) void k_gemm(const GemmItem *__restrict__ items, const int *__restrict__ pre, int nitems,
                                               const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, T alpha,
