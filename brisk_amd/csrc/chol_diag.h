@@ -1,9 +1,10 @@
-// chol_diag.h - factor and invert one 64 x 64 diagonal block in a workgroup of 256 threads
+// chol_diag.h - factor and invert one 64 x 64 diagonal block in a workgroup of 512 threads
 // (included by k_dense.hip).
 //
-// Layout: thread (lane r, wave w) holds row r of the block at the 16 columns c = 4u + w in
-// registers. The factor runs in 16 groups of four columns with one barrier each: wave t
-// publishes column j0 + t to LDS, every thread factors the 4 x 4 diagonal piece itself
+// Layout: thread (lane r, wave w), eight waves (two per SIMD, whose dependency chains
+// interleave), holds row r of the block at the 8 columns c = 8u + w in registers. The factor runs
+// in 16 groups of four columns with one barrier each: the four waves holding the group's columns
+// publish them to LDS, every thread factors the 4 x 4 diagonal piece itself
 // (reciprocal square roots by v_rsq plus Newton steps, no divisions), forms the four values
 // l(r, j0..j0+3) of its own row and writes them to its wave's private copy in LDS, from which
 // the wave reads the rows it needs for the rank-4 update of its own columns (an in-order
@@ -115,21 +116,24 @@ __device__ int diag_factor_inverse(T *Ab, int n, int nb, T *Di, T &pm_out) {
     __shared__ T Xs[64][65];       // Xs[c][r] = X(r, c), X = L^{-1}
     __shared__ T Ts[64][65];
     __shared__ T colb[2][4][64];   // the group's four columns (double-buffered by group parity)
-    __shared__ T lb[4][64][4];     // per wave: l(r, j0..j0+3) of every row r
+    __shared__ T lb[8][64][4];     // per wave: l(r, j0..j0+3) of every row r
     __shared__ T rdg[64];          // 1 / L(j, j)
     const int tid = threadIdx.x, r = tid & 63, w = tid >> 6;
-    T v[16];
+    // thread (r, w) holds row r at the 8 columns c = 8 u + w; group g's columns 4 g + t are held
+    // by the waves 4 (g & 1) + t, at u = g / 2
+    T v[8];
 #pragma unroll
-    for (int u = 0; u < 16; u++) {
-        const int c = 4 * u + w;
+    for (int u = 0; u < 8; u++) {
+        const int c = 8 * u + w;
         v[u] = (r < nb && c < nb) ? (r >= c ? Ab[r + (int64_t)c * n] : (T)0) : (r == c ? (T)1 : (T)0);
     }
     T pm = (T)1e30;
     int fail = 0;
+    const int half = w >> 2, wq = w & 3;
 #pragma unroll
     for (int g = 0; g < 16; g++) {
-        const int j0 = 4 * g, bf = g & 1;
-        colb[bf][w][r] = v[g];
+        const int j0 = 4 * g, bf = g & 1, hu = g >> 1;
+        if (half == (g & 1)) colb[bf][wq][r] = v[hu];
         __syncthreads();
         T D[4][4], cr[4];
 #pragma unroll
@@ -180,11 +184,20 @@ __device__ int diag_factor_inverse(T *Ab, int n, int nb, T *Di, T &pm_out) {
 #pragma unroll
         for (int t = 0; t < 4; t++) lb[w][r][t] = l[t];
         diag_wave_sync();
-        v[g] = w == 0 ? l[0] : w == 1 ? l[1] : w == 2 ? l[2] : l[3];
+        // the group's columns are final; the other half's column at u = g / 2 is a later one when
+        // g is even (column 4 g + 4 + wq), an earlier one when g is odd
+        if (half == (g & 1)) v[hu] = wq == 0 ? l[0] : wq == 1 ? l[1] : wq == 2 ? l[2] : l[3];
+        else if (!(g & 1)) {
+            const int k = 8 * hu + w;
+            T a = v[hu];
+#pragma unroll
+            for (int t = 0; t < 4; t++) a -= l[t] * lb[w][k][t];
+            v[hu] = a;
+        }
         // rank-4 update of this thread's later columns k: a(r, k) -= sum_t l(r, t) l(k, t)
 #pragma unroll
-        for (int h = g + 1; h < 16; h++) {
-            const int k = 4 * h + w;
+        for (int h = hu + 1; h < 8; h++) {
+            const int k = 8 * h + w;
             T a = v[h];
 #pragma unroll
             for (int t = 0; t < 4; t++) a -= l[t] * lb[w][k][t];
@@ -194,8 +207,8 @@ __device__ int diag_factor_inverse(T *Ab, int n, int nb, T *Di, T &pm_out) {
     __syncthreads();
     if (fail) return fail;
 #pragma unroll
-    for (int u = 0; u < 16; u++) {
-        const int c = 4 * u + w;
+    for (int u = 0; u < 8; u++) {
+        const int c = 8 * u + w;
         const T x = r >= c ? v[u] : (T)0;
         v[u] = x;
         Ls[c][r] = x;
@@ -222,8 +235,8 @@ __device__ int diag_factor_inverse(T *Ab, int n, int nb, T *Di, T &pm_out) {
     diag_inv_mfma<T, 16>(Ls, Xs, Ts, tid);
     diag_inv_mfma<T, 32>(Ls, Xs, Ts, tid);
 #pragma unroll
-    for (int u = 0; u < 16; u++) {
-        const int c = 4 * u + w;
+    for (int u = 0; u < 8; u++) {
+        const int c = 8 * u + w;
         if (r < nb && c < nb) Ab[r + (int64_t)c * n] = v[u];
         Di[r + 64 * c] = (r < nb && c < nb && r >= c) ? Xs[c][r] : (T)0;
     }

@@ -297,7 +297,7 @@ extern "C" int bk_gemm_f32(int ta, int tb, float alpha, float beta, const float 
 // unblocked), records the first failing column and the smallest squared pivot, writes L_kk back
 // (strict upper part of the block zeroed) and its inverse to Dinv.
 template <typename T>
-__global__ __launch_bounds__(256) void k_potrf_diag(const MItem *__restrict__ items, int nit, int k0, T *__restrict__ A,
+__global__ __launch_bounds__(512) void k_potrf_diag(const MItem *__restrict__ items, int nit, int k0, T *__restrict__ A,
                                                     T *__restrict__ Dinv, int *__restrict__ info, T *__restrict__ pmin) {
     const int q = blockIdx.x;
     if (q >= nit) return;
@@ -476,7 +476,7 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
             const int k0 = st[u].k0;
             if (gemm_exec<T>(d, st[u].upd, 0, 1, (T)-1, (T)1, A, A, A, info)) return -1;
             const int tok = bk_prof_begin();
-            hipLaunchKernelGGL((k_potrf_diag<T>), dim3(n_it), dim3(256), 0, g_bk_stream, d_it, n_it, k0, A, Dinv, info, pmin);
+            hipLaunchKernelGGL((k_potrf_diag<T>), dim3(n_it), dim3(512), 0, g_bk_stream, d_it, n_it, k0, A, Dinv, info, pmin);
             if (tok >= 0) { double fl = 0; for (int q = 0; q < n_it; q++) { const double b = it[q].n > k0 ? fmin(64.0, it[q].n - k0) : 0; fl += b * b * b / 3.0 + b * b * b / 3.0; } bk_prof_end(tok, 8, fl, 0); }
             if (BK_LAUNCH_CHECK("k_potrf_diag")) return -1;
             if (gemm_exec<T>(d, st[u].panel, 0, 1, (T)1, (T)0, A, Dinv, A, info)) return -1;
