@@ -673,7 +673,7 @@ __device__ __forceinline__ void lzf_sums(double (&c)[BK_LZK], int j, double (*re
     const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
 #pragma unroll
     for (int p = 0; p < BK_LZK; p++) {
-        if (p <= j) {
+        if (p <= j && wv < 4) {                     // the row threads (the first 4 waves)
             const double v = wave_sum(c[p]);
             if (lane == 0) red[wv][p] = v;
         }
@@ -757,12 +757,37 @@ __device__ void lz_ritz(const double *al, const double *be, int kk, int K, int n
     }
     out_ = out; stop_ = stop;
 }
-__global__ __launch_bounds__(LZF) void k_lz_fused(const LzfItem *__restrict__ it, int nit, double *__restrict__ H) {
+// a product row . xs over the row's columns c in [row, n) (kind 1), [0, n) (0) or [0, row] (2):
+// thread (row, part) forms the part's share of the columns, the row's thread adds the LZP parts
+// in order (every thread returns; only the first LZF hold rows)
+#define LZP 4
+__device__ __forceinline__ double lzf_mv(const double *__restrict__ M, int n, int kind, const double *xs, double (*pr)[LZF]) {
+    const int t = threadIdx.x, row = t % LZF, part = t / LZF;
+    const int lo = kind == 1 ? row : 0, hi = kind == 2 ? row + 1 : n, q = (n + LZP - 1) / LZP;
+    const int c0 = max(lo, part * q), c1 = min(hi, part * q + q);
+    double s0 = 0, s1 = 0;
+    if (row < n) {
+        int c = c0;
+        for (; c + 1 < c1; c += 2) { s0 += M[row + (int64_t)c * n] * xs[c]; s1 += M[row + (int64_t)(c + 1) * n] * xs[c + 1]; }
+        if (c < c1) s0 += M[row + (int64_t)c * n] * xs[c];
+    }
+    pr[part][row] = s0 + s1;
+    __syncthreads();
+    double v = 0;
+    if (t < LZF) {
+#pragma unroll
+        for (int u = 0; u < LZP; u++) v += pr[u][t];
+    }
+    __syncthreads();
+    return v;
+}
+__global__ __launch_bounds__(LZF * LZP) void k_lz_fused(const LzfItem *__restrict__ it, int nit, double *__restrict__ H) {
     const int qi = blockIdx.x;
     if (qi >= nit) return;
     const LzfItem L = it[qi];
     const int n = L.n, K = L.K, t = threadIdx.x, lane = t & 63;
     __shared__ double xs[LZF];
+    __shared__ double pr[LZP][LZF];
     __shared__ double red[4][BK_LZK];
     __shared__ double cf[BK_LZK];
     __shared__ double al[BK_LZK], be[BK_LZK];
@@ -788,23 +813,18 @@ __global__ __launch_bounds__(LZF) void k_lz_fused(const LzfItem *__restrict__ it
 #pragma unroll
         for (int p = 0; p < BK_LZK; p++) if (p == j) qj = Q[p];
         __syncthreads();
-        xs[t] = qj;
+        if (t < LZF) xs[t] = qj;
         __syncthreads();
-        double w = 0;
-        if (L.mode == 0) {
-            if (t < n) for (int c = 0; c < n; c++) w += V[t + (int64_t)c * n] * xs[c];
-        } else {
-            double t1 = 0;                                   // Li' q (upper: columns c >= row)
-            if (t < n) for (int c = t; c < n; c++) t1 += LiT[t + (int64_t)c * n] * xs[c];
+        double w;
+        if (L.mode == 0) w = lzf_mv(V, n, 0, xs, pr);
+        else {
+            const double t1 = lzf_mv(LiT, n, 1, xs, pr);    // Li' q (upper: columns c >= row)
+            if (t < LZF) xs[t] = t1;
             __syncthreads();
-            xs[t] = t1;
+            const double t2 = lzf_mv(V, n, 0, xs, pr);      // V t1
+            if (t < LZF) xs[t] = t2;
             __syncthreads();
-            double t2 = 0;                                   // V t1
-            if (t < n) for (int c = 0; c < n; c++) t2 += V[t + (int64_t)c * n] * xs[c];
-            __syncthreads();
-            xs[t] = t2;
-            __syncthreads();
-            if (t < n) for (int c = 0; c <= t; c++) w += Li[t + (int64_t)c * n] * xs[c];   // Li t2 (lower)
+            w = lzf_mv(Li, n, 2, xs, pr);                   // Li t2 (lower)
         }
         // classical Gram-Schmidt twice; the first pass's coefficient of q_j is alpha_j
         double alpha = 0;
@@ -852,7 +872,7 @@ extern "C" int bk_lanczos_fused(const LzfItem *it, int n_it) {
     const LzfItem *d = (const LzfItem *)bk_stage(it, sizeof(LzfItem) * n_it);
     if (!d) return -1;
     const int tok = bk_prof_begin();
-    hipLaunchKernelGGL(k_lz_fused, dim3(n_it), dim3(LZF), 0, g_bk_stream, d, n_it, (double *)g_heap);
+    hipLaunchKernelGGL(k_lz_fused, dim3(n_it), dim3(LZF * LZP), 0, g_bk_stream, d, n_it, (double *)g_heap);
     if (tok >= 0) bk_prof_end(tok, 12, 0, 0);
     return BK_LAUNCH_CHECK("k_lz_fused");
 }
