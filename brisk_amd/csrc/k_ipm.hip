@@ -910,12 +910,15 @@ __global__ __launch_bounds__(256) void k_lzm_part(const LzItem *__restrict__ ite
     if (j >= L.K) return;
     __shared__ double red[4][32];
     __shared__ double cf[32];
-    const int t = threadIdx.x, i = 256 * (blockIdx.x - pre[q]) + t;
+    const int t = threadIdx.x, i = 256 * (blockIdx.x - pre[q]) + t, ic = min(i, L.n - 1);
     const double *Q = H + L.Q, *w = H + L.w;
     double c[32];
     const double wi = i < L.n ? w[i] : 0.0;
+    // every basis entry loaded (index clamped into the basis, value masked): 32 loads in flight
 #pragma unroll
-    for (int p = 0; p < 32; p++) c[p] = (p <= j && i < L.n) ? Q[(int64_t)p * L.n + i] * wi : 0.0;
+    for (int p = 0; p < 32; p++) c[p] = Q[(int64_t)min(p, j) * L.n + ic];
+#pragma unroll
+    for (int p = 0; p < 32; p++) c[p] = (p <= j && i < L.n) ? c[p] * wi : 0.0;
     lzm_wg_sums(c, j, red, cf);
     if (t <= j) part[(int64_t)blockIdx.x * LZS + t] = cf[t];
 }
@@ -937,17 +940,25 @@ __global__ __launch_bounds__(256) void k_lzm_apply(const LzItem *__restrict__ it
     __syncthreads();
     double *Q = H + L.Q, *w = H + L.w;
     if (pass == 0 && blockIdx.x == b0 && t == 0) H[L.ab + j] = cf[j];
+    // this row's basis entries q_0..q_j, all loads in flight (index clamped, value masked), used by
+    // the update and by the next pass's partial sums
+    const int ic = min(i, L.n - 1);
+    double qv[32];
+#pragma unroll
+    for (int p = 0; p < 32; p++) qv[p] = Q[(int64_t)min(p, j) * L.n + ic];
+#pragma unroll
+    for (int p = 0; p < 32; p++) qv[p] = p <= j ? qv[p] : 0.0;
     double v = 0;
     if (i < L.n) {
         v = w[i];
 #pragma unroll
-        for (int p = 0; p < 32; p++) if (p <= j) v -= cf[p] * Q[(int64_t)p * L.n + i];
+        for (int p = 0; p < 32; p++) if (p <= j) v -= cf[p] * qv[p];
         w[i] = v;
     }
     if (pass == 0) {
         double c[32];
 #pragma unroll
-        for (int p = 0; p < 32; p++) c[p] = (p <= j && i < L.n) ? Q[(int64_t)p * L.n + i] * v : 0.0;
+        for (int p = 0; p < 32; p++) c[p] = (p <= j && i < L.n) ? qv[p] * v : 0.0;
         lzm_wg_sums(c, j, red, cs);
         if (t <= j) outp[(int64_t)blockIdx.x * LZS + t] = cs[t];
     } else {
