@@ -170,6 +170,21 @@ __global__ __launch_bounds__(GEMM_NT(BM)) void k_gemm(const GemmItem *__restrict
     }
     T *c = C + it.c;
     const int ldc = it.ldc;
+    // with beta != 0 every C entry of the thread is read first, all loads in flight (an entry
+    // outside the item reads its nearest entry inside, and is not stored)
+    T cv[FM][FN][4];
+    if (beta != (T)0) {
+#pragma unroll
+        for (int si = 0; si < FM; si++)
+#pragma unroll
+            for (int sj = 0; sj < FN; sj++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int gi = min(r0 + wm * WM + si * 16 + (lane & 15), M - 1);
+                    const int gj = min(c0 + wn * WN + sj * 16 + MF<T>::out(lane >> 4, r), N - 1);
+                    cv[si][sj][r] = c[gi + (int64_t)gj * ldc];
+                }
+    }
 #pragma unroll
     for (int si = 0; si < FM; si++)
 #pragma unroll
@@ -179,10 +194,9 @@ __global__ __launch_bounds__(GEMM_NT(BM)) void k_gemm(const GemmItem *__restrict
                 const int gi = r0 + wm * WM + si * 16 + (lane & 15);
                 const int gj = c0 + wn * WN + sj * 16 + MF<T>::out(lane >> 4, r);
                 if (gi < M && gj < N && (!(it.tri & 1) || gi >= gj)) {
-                    T *cp = c + gi + (int64_t)gj * ldc;
                     T v = alpha * acc[si][sj][r];
-                    if (beta != (T)0) v += beta * *cp;
-                    *cp = v;
+                    if (beta != (T)0) v += beta * cv[si][sj][r];
+                    c[gi + (int64_t)gj * ldc] = v;
                 }
             }
 }
