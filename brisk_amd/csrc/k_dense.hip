@@ -1036,8 +1036,16 @@ __global__ __launch_bounds__(64) void k_symv_reduce(const SymvItem *__restrict__
     const int a = blockIdx.x - pre[q], n = it.n, nT = (n + 63) / 64, i = threadIdx.x, row = 64 * a + i;
     if (row >= n) return;
     const double *p = P + it.p + (int64_t)a * nT * 64 + i;
+    // the slots in order, loaded 8 at a time (index clamped, value masked) so that the loads of a
+    // batch are in flight together; the sum's order is unchanged
     double s = 0;
-    for (int b = 0; b < nT; b++) s += p[(int64_t)b * 64];
+    for (int b0 = 0; b0 < nT; b0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = p[(int64_t)min(b0 + u, nT - 1) * 64];
+#pragma unroll
+        for (int u = 0; u < 8; u++) if (b0 + u < nT) s += v[u];
+    }
     double *y = Y + it.y + row;
     *y = beta != 0.0 ? alpha * s + beta * *y : alpha * s;
 }
