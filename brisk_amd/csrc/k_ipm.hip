@@ -902,6 +902,19 @@ __device__ __forceinline__ void lzm_wg_sums(double (&c)[32], int j, double (*red
     if (t <= j) out[t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
     __syncthreads();
 }
+// sum of a column of workgroup slots in workgroup order, 8 loads at a time (index clamped,
+// value masked): the loads of a batch are in flight together, the order of the sum unchanged
+__device__ __forceinline__ double lzm_slot_sum(const double *__restrict__ in, int b0, int nb, int col) {
+    double v = 0;
+    for (int u0 = 0; u0 < nb; u0 += 8) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = in[(int64_t)(b0 + min(u0 + u, nb - 1)) * LZS + col];
+#pragma unroll
+        for (int u = 0; u < 8; u++) if (u0 + u < nb) v += x[u];
+    }
+    return v;
+}
 // slots of pass 0: c_p = sum over the rows of q_p(i) w(i)
 __global__ __launch_bounds__(256) void k_lzm_part(const LzItem *__restrict__ items, const int *__restrict__ pre, int nit, int j,
                                                   double *__restrict__ H, double *__restrict__ part) {
@@ -932,11 +945,7 @@ __global__ __launch_bounds__(256) void k_lzm_apply(const LzItem *__restrict__ it
     __shared__ double red[4][32];
     __shared__ double cf[32], cs[32];
     const int t = threadIdx.x, b0 = pre[q], nb = pre[q + 1] - b0, i = 256 * (blockIdx.x - b0) + t;
-    if (t <= j) {
-        double v = 0;
-        for (int u = 0; u < nb; u++) v += in[(int64_t)(b0 + u) * LZS + t];
-        cf[t] = v;
-    }
+    if (t <= j) cf[t] = lzm_slot_sum(in, b0, nb, t);
     __syncthreads();
     double *Q = H + L.Q, *w = H + L.w;
     if (pass == 0 && blockIdx.x == b0 && t == 0) H[L.ab + j] = cf[j];
@@ -977,8 +986,7 @@ __global__ __launch_bounds__(256) void k_lzm_fin(const LzItem *__restrict__ item
     __shared__ double bsh;
     const int t = threadIdx.x, b0 = pre[q], nb = pre[q + 1] - b0, i = 256 * (blockIdx.x - b0) + t;
     if (t == 0) {
-        double v = 0;
-        for (int u = 0; u < nb; u++) v += in[(int64_t)(b0 + u) * LZS + 32];
+        const double v = lzm_slot_sum(in, b0, nb, 32);
         bsh = sqrt(v);
         if (blockIdx.x == b0) H[L.ab + 32 + j] = bsh;
     }
