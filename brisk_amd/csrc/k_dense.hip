@@ -45,18 +45,20 @@ template <> struct MF<float> {
 // as its A operand and the A tile as its B operand, so each output register row is a run of 16
 // consecutive rows of column-major C (coalesced stores). LDS holds two K slabs: slab s + 1 is
 // fetched into registers before slab s is multiplied and written to the other buffer after it,
-// so each slab costs one barrier. Every thread's loads keep one coordinate fixed and advance the
-// other by a constant stride; a load outside the item reads the item's first element and is
-// zeroed when it is written to LDS (no branches around the loads, which stay in flight across
-// the products). Rows of a slab are padded to BM + 1 (BN + 1) elements: 16 consecutive k rows
-// written by consecutive threads then fall in distinct banks.
+// so each slab costs one barrier. A slab is 16 deep (32 for the f32 128 x 128 tile). Each
+// thread's load addresses are fixed for the launch, a row (column) outside the item clamped to
+// the item's last one and the value zeroed when it is written to LDS; a slab adds one offset
+// common to the workgroup, so the loads cost one address add each, carry no branches and stay
+// in flight across the products (per-slab index and mask arithmetic had cost 5% of f64 and 13%
+// of f32 throughput at n = 2000). Rows of a slab are padded to BM + 1 (BN + 1) elements: 16
+// consecutive k rows written by consecutive threads then fall in distinct banks.
 #define GEMM_NT(BM_) 512                           // threads of a k_gemm workgroup (either tile size)
 template <typename T, int TA, int TB, int BM, int BN>
 __global__ __launch_bounds__(GEMM_NT(BM)) void k_gemm(const GemmItem *__restrict__ items, const int *__restrict__ pre, int nitems,
                                               const T *__restrict__ A, const T *__restrict__ B, T *__restrict__ C, T alpha,
                                               T beta, const int *__restrict__ info) {
     typedef typename MF<T>::v4 v4;
-    constexpr int BK = GBK, NT = GEMM_NT(BM);
+    constexpr int BK = BM == 128 && sizeof(T) == 4 ? 2 * GBK : GBK, NT = GEMM_NT(BM);
     constexpr int WM = BM / 2, WN = BN / (NT / 128), FM = WM / 16, FN = WN / 16;
     constexpr int LA = BM * BK / NT, LB = BN * BK / NT;
     static_assert(NT % BM == 0 && NT % BN == 0 && NT % BK == 0 && LA <= 32 && LB <= 32 && FN >= 1, "tile shape");
@@ -93,32 +95,48 @@ __global__ __launch_bounds__(GEMM_NT(BM)) void k_gemm(const GemmItem *__restrict
     constexpr int SA = TA == 0 ? NT / BM : NT / BK, SB = TB == 0 ? NT / BK : NT / BN;
     const int ia = TA == 0 ? tid % BM : tid / BK, ka = TA == 0 ? tid / BM : tid % BK;
     const int jb = TB == 0 ? tid / BK : tid % BN, kq = TB == 0 ? tid % BK : tid / BN;
-    const int64_t stA = (int64_t)SA * lda, stB = (int64_t)SB * ldb;
     uint32_t va = 0, vb = 0;                     // elements whose row (column) lies inside the item
 #pragma unroll
     for (int r = 0; r < LA; r++) { const int i = TA == 0 ? ia : ia + SA * r; va |= (r0 + i < M ? 1u : 0u) << r; }
 #pragma unroll
     for (int r = 0; r < LB; r++) { const int j = TB == 0 ? jb + SB * r : jb; vb |= (c0 + j < N ? 1u : 0u) << r; }
+    // each load's address at k = 0, its row (column) clamped into the item (the value is zeroed
+    // when it is written to LDS); a slab adds one offset common to the workgroup
+    const T *pa[LA], *pb[LB];
+#pragma unroll
+    for (int r = 0; r < LA; r++)
+        pa[r] = TA == 0 ? a + min(r0 + ia, M - 1) + (int64_t)(ka + SA * r) * lda
+                        : a + ka + (int64_t)min(r0 + ia + SA * r, M - 1) * lda;
+#pragma unroll
+    for (int r = 0; r < LB; r++)
+        pb[r] = TB == 0 ? b + kq + (int64_t)min(c0 + jb + SB * r, N - 1) * ldb
+                        : b + min(c0 + jb, N - 1) + (int64_t)(kq + SB * r) * ldb;
     T ra[LA], rb[LB];
     uint32_t ma = 0, mb = 0;                     // loads of the slab in flight that are inside the item
     auto fetch = [&](int k0) {
-        const bool full = k0 + BK <= ke;
-        const T *pa = TA == 0 ? a + (r0 + ia) + (int64_t)(k0 + ka) * lda : a + (k0 + ka) + (int64_t)(r0 + ia) * lda;
-        const T *pb = TB == 0 ? b + (k0 + kq) + (int64_t)(c0 + jb) * ldb : b + (c0 + jb) + (int64_t)(k0 + kq) * ldb;
-        ma = 0; mb = 0;
+        const int64_t oa = TA == 0 ? (int64_t)k0 * lda : k0, ob = TB == 0 ? k0 : (int64_t)k0 * ldb;
+        if (k0 + BK <= ke) {
 #pragma unroll
-        for (int r = 0; r < LA; r++) {
-            const int gk = TA == 0 ? k0 + ka + SA * r : k0 + ka;
-            const bool ok = ((va >> r) & 1) && (full || gk < ke);
-            ma |= (ok ? 1u : 0u) << r;
-            ra[r] = *(ok ? pa + r * stA : a);
-        }
+            for (int r = 0; r < LA; r++) ra[r] = pa[r][oa];
 #pragma unroll
-        for (int r = 0; r < LB; r++) {
-            const int gk = TB == 0 ? k0 + kq : k0 + kq + SB * r;
-            const bool ok = ((vb >> r) & 1) && (full || gk < ke);
-            mb |= (ok ? 1u : 0u) << r;
-            rb[r] = *(ok ? pb + r * stB : b);
+            for (int r = 0; r < LB; r++) rb[r] = pb[r][ob];
+            ma = va; mb = vb;
+        } else {
+            // the last, partial slab: a k beyond the range reads the item's first element instead
+            ma = 0; mb = 0;
+#pragma unroll
+            for (int r = 0; r < LA; r++) {
+                const bool ok = (TA == 0 ? k0 + ka + SA * r : k0 + ka) < ke;
+                ma |= (ok ? 1u : 0u) << r;
+                ra[r] = *(ok ? pa[r] + oa : a);
+            }
+#pragma unroll
+            for (int r = 0; r < LB; r++) {
+                const bool ok = (TB == 0 ? k0 + kq : k0 + kq + SB * r) < ke;
+                mb |= (ok ? 1u : 0u) << r;
+                rb[r] = *(ok ? pb[r] + ob : b);
+            }
+            ma &= va; mb &= vb;
         }
     };
     auto store = [&](int buf) {

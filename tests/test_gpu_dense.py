@@ -16,7 +16,9 @@ def _spd(n, rng, cond=1e3):
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 @pytest.mark.parametrize("shapes", [[(70, 45, 33), (1, 1, 1), (64, 64, 64), (130, 17, 200), (5, 129, 7)],
-                                    [(300, 257, 129), (192, 200, 64), (640, 384, 31)]], ids=["small-tiles", "large-tiles"])
+                                    [(300, 257, 129), (192, 200, 64), (640, 384, 31)],
+                                    [(2011, 1999, 77), (200, 300, 50), (150, 400, 21)]],
+                         ids=["small-tiles", "large-tiles", "128-tiles"])
 def test_gemm_batched_ragged(dt, ta, tb, shapes):
     rng = np.random.default_rng(1 + ta + 2 * tb)
     A_parts, B_parts, C_parts, rows, refs = [], [], [], [], []
@@ -46,10 +48,11 @@ def test_gemm_batched_ragged(dt, ta, tb, shapes):
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
-def test_gemm_triangular_flags(dt):
-    """Zi = Linv' Linv with the K range trimmed (2|8) and a lower-only update (1)."""
+@pytest.mark.parametrize("n", [300, 2100])
+def test_gemm_triangular_flags(dt, n):
+    """Zi = Linv' Linv with the K range trimmed (2|8) and a lower-only update (1); n = 2100 runs
+    the trimmed product on 128 x 128 tiles."""
     rng = np.random.default_rng(7)
-    n = 300
     Li = np.tril(rng.standard_normal((n, n)))
     dA = DBuf(Li.flatten(order="F").astype(dt))
     dC = DBuf(np.zeros(n * n, dt))
