@@ -69,7 +69,17 @@ __global__ __launch_bounds__(GEMM_NT(BM)) void k_gemm(const GemmItem *__restrict
     const int t = blockIdx.x - pre[item];
     int tm, tn;
     if ((it.tri & 1) && it.M == it.N) tri_index(t, tm, tn);   // lower-only square C: its lower tiles only
-    else { tm = t % tilesM; tn = t / tilesM; }
+    else if (BM == 128) {
+        // XCD-aware order: workgroups are dealt to the eight XCDs round-robin, so tile t runs on
+        // XCD t % 8 (for an item whose first tile is dealt to XCD 0). The eighth of the item's tiles
+        // on one XCD is made one run of the order below, which walks groups of 4 tile rows column
+        // by column: a 16 x 16 item gives each XCD a 4 x 8 block of tiles, whose A and B strips it
+        // reads through its own L2 (2 x 16 strips before). Both steps are bijections.
+        const int nt = tilesM * ((it.N + BN - 1) / BN);
+        const int u = nt % 8 == 0 ? (t % 8) * (nt / 8) + t / 8 : t;
+        const int gw = 4 * ((it.N + BN - 1) / BN), fm = (u / gw) * 4, gs = min(tilesM - fm, 4);
+        tm = fm + (u % gw) % gs; tn = (u % gw) / gs;
+    } else { tm = t % tilesM; tn = t / tilesM; }
     const int r0 = tm * BM, c0 = tn * BN;
     if ((it.tri & 1) && c0 > r0 + BM - 1) return;
     int kb = 0, ke = it.K;
