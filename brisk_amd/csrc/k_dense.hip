@@ -677,21 +677,24 @@ __device__ __forceinline__ double trsv_dot(const T *__restrict__ p, int64_t stri
 }
 // loads in flight per batch: 32 of f32, 16 of f64 (within the 128 registers of a 1024-thread group)
 #define TRSV_NB (sizeof(T) == 8 ? 16 : 32)
+#ifndef TRSV_WG1
+#define TRSV_WG1 512
+#endif
 // 1024 threads: the block's 256 x 256 product as thread (row, quarter) over 64 columns each, the
 // update as 16 groups of 16 columns (forward) or 16 waves of 4 columns (backward); partial sums
 // added in a fixed order
 template <typename T>
-__global__ __launch_bounds__(1024) void k_trsv_fwd(const TrsvItem *__restrict__ items, int nact, int k, const T *__restrict__ A,
-                                                   double *__restrict__ V) {
-    // the item: the last i with pre_i - 4 k i <= blockIdx (its chunks 4k .. c_i - 1 follow)
+__global__ __launch_bounds__(1024) void k_trsv_fwd(const TrsvItem *__restrict__ items, const int *__restrict__ wpre, int nact, int k,
+                                                   int G, const T *__restrict__ A, double *__restrict__ V) {
+    // the item: the last i with wpre[i] <= blockIdx; its workgroups take G chunks each, from 4 k
     const int g = blockIdx.x;
     int lo = 0, hi = nact - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (items[mid].pre - 4LL * k * mid <= g) lo = mid; else hi = mid - 1;
+        if (wpre[mid] <= g) lo = mid; else hi = mid - 1;
     }
     const TrsvItem it = items[lo];
-    const int chunk = 4 * k + (int)(g - (it.pre - 4LL * k * lo));
+    const int first = 4 * k + G * (g - wpre[lo]);
     const int m = it.m, kb = 256 * k, nbk = min(256, m - kb), t = threadIdx.x;
     __shared__ double bs[256 + 32], ys[256 + 32], part[16][256];
     double *b = V + it.v0;
@@ -713,34 +716,38 @@ __global__ __launch_bounds__(1024) void k_trsv_fwd(const TrsvItem *__restrict__ 
     if (t < 256) ys[t] = t < nbk ? (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]) : 0.0;
     else if (t < 288) ys[t] = 0.0;
     __syncthreads();
-    const int r0 = 64 * chunk;
-    if (chunk < 4 * k + 4) {
-        if (t < 64 && r0 + t < m) V[it.v1 + r0 + t] = ys[r0 + t - kb];
-        return;
-    }
-    // b[rows] -= L[rows, kb .. kb + nbk) y: thread (row, group h) over the columns 16 h .. 16 h + 15
-    const int lane = t & 63, h = t >> 6, row = r0 + lane, c0 = 16 * h, c1 = min(c0 + 16, nbk);
-    double s = 0;
-    if (row < m && c0 < c1) {
-        const T *l = A + it.L + row + (int64_t)kb * m;
-        s = trsv_dot<T, 16>(l, m, c0, c0, c1, ys);
-    }
-    part[h][lane] = s;
-    __syncthreads();
-    if (t < 64 && row < m) {
-        double a = 0;
+    for (int chunk = first; chunk < first + G && chunk < it.c; chunk++) {
+        const int r0 = 64 * chunk;
+        if (chunk < 4 * k + 4) {
+            if (t < 64 && r0 + t < m) V[it.v1 + r0 + t] = ys[r0 + t - kb];
+            continue;
+        }
+        // b[rows] -= L[rows, kb .. kb + nbk) y: thread (row, group h) over the columns 16 h .. 16 h + 15
+        const int lane = t & 63, h = t >> 6, row = r0 + lane, c0 = 16 * h, c1 = min(c0 + 16, nbk);
+        double s = 0;
+        if (row < m && c0 < c1) {
+            const T *l = A + it.L + row + (int64_t)kb * m;
+            s = trsv_dot<T, 16>(l, m, c0, c0, c1, ys);
+        }
+        part[h][lane] = s;
+        __syncthreads();
+        if (t < 64 && row < m) {
+            double a = 0;
 #pragma unroll
-        for (int u = 0; u < 16; u++) a += part[u][t];
-        b[row] -= a;
+            for (int u = 0; u < 16; u++) a += part[u][t];
+            b[row] -= a;
+        }
+        __syncthreads();
     }
 }
 template <typename T>
-__global__ __launch_bounds__(1024) void k_trsv_bwd(const TrsvItem *__restrict__ items, int nact, int k, const T *__restrict__ A,
-                                                   double *__restrict__ V) {
-    const int per = 4 * k + 4, q = blockIdx.x / per, chunk = blockIdx.x % per;
+__global__ __launch_bounds__(1024) void k_trsv_bwd(const TrsvItem *__restrict__ items, int nact, int k, int G,
+                                                   const T *__restrict__ A, double *__restrict__ V) {
+    // per item ceil((4 k + 4) / G) workgroups of G column chunks each
+    const int per = (4 * k + 4 + G - 1) / G, q = blockIdx.x / per, first = G * (blockIdx.x % per);
     if (q >= nact) return;
     const TrsvItem it = items[q];
-    if (chunk >= it.c) return;
+    if (first >= it.c) return;
     const int m = it.m, kb = 256 * k, nbk = min(256, m - kb), t = threadIdx.x, lane = t & 63, w = t >> 6;
     __shared__ double ys[256 + 32], zs[256 + 32], part[4][256];
     double *y = V + it.v1;
@@ -762,29 +769,31 @@ __global__ __launch_bounds__(1024) void k_trsv_bwd(const TrsvItem *__restrict__ 
     if (t < 256) zs[t] = t < nbk ? (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]) : 0.0;
     else if (t < 288) zs[t] = 0.0;
     __syncthreads();
-    const int r0 = 64 * chunk;
-    if (chunk >= 4 * k) {
-        if (t < 64 && r0 + t < m) V[it.v0 + r0 + t] = zs[r0 + t - kb];
-        return;
-    }
-    // y[j] -= sum_r L[kb + r, j] z_r for the columns j = r0 + 4 w .. r0 + 4 w + 3 (j < kb): the
-    // wave reads its 4 columns (4 rows per lane each, all 16 loads issued together), then a
-    // butterfly per column
-    T v[4][4];
+    for (int chunk = first; chunk < first + G && chunk < it.c && chunk < 4 * k + 4; chunk++) {
+        const int r0 = 64 * chunk;
+        if (chunk >= 4 * k) {
+            if (t < 64 && r0 + t < m) V[it.v0 + r0 + t] = zs[r0 + t - kb];
+            continue;
+        }
+        // y[j] -= sum_r L[kb + r, j] z_r for the columns j = r0 + 4 w .. r0 + 4 w + 3 (j < kb): the
+        // wave reads its 4 columns (4 rows per lane each, all 16 loads issued together), then a
+        // butterfly per column
+        T v[4][4];
 #pragma unroll
-    for (int jj = 0; jj < 4; jj++) {
-        const T *col = A + it.L + kb + (int64_t)(r0 + 4 * w + jj) * m;
+        for (int jj = 0; jj < 4; jj++) {
+            const T *col = A + it.L + kb + (int64_t)(r0 + 4 * w + jj) * m;
 #pragma unroll
-        for (int u = 0; u < 4; u++) v[jj][u] = col[min(lane + 64 * u, nbk - 1)];      // masked below
-    }
+            for (int u = 0; u < 4; u++) v[jj][u] = col[min(lane + 64 * u, nbk - 1)];      // masked below
+        }
 #pragma unroll
-    for (int jj = 0; jj < 4; jj++) {
-        double s = 0;
+        for (int jj = 0; jj < 4; jj++) {
+            double s = 0;
 #pragma unroll
-        for (int u = 0; u < 4; u++) s += (lane + 64 * u < nbk ? (double)v[jj][u] : 0.0) * zs[lane + 64 * u];
+            for (int u = 0; u < 4; u++) s += (lane + 64 * u < nbk ? (double)v[jj][u] : 0.0) * zs[lane + 64 * u];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0) y[r0 + 4 * w + jj] -= s;
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+            if (lane == 0) y[r0 + 4 * w + jj] -= s;
+        }
     }
 }
 // DiT = Di' on the diagonal blocks: one 64 x 64 tile of the lower part of a block per workgroup
@@ -838,26 +847,49 @@ static int trsv_run(const TrsvItem *it, int n_it, const T *A, double *V) {
     if (n_it <= 0) return 0;
     std::vector<TrsvItem> v;
     if (trsv_sorted<T>(it, n_it, v)) return bk_fail("trsv: an item of order <= 0");
-    const TrsvItem *d = (const TrsvItem *)bk_stage(v.data(), sizeof(TrsvItem) * n_it);
-    if (!d) return -1;
     const int nbk = (v[0].m + 255) / 256;
     std::vector<int> act(nbk);                          // items with a block k: a prefix
     for (int k = 0, a = n_it; k < nbk; k++) {
         while (a > 0 && v[a - 1].m <= 256 * k) a--;
         act[k] = a;
     }
-    const int tok = bk_prof_begin();
+    // Chunks per workgroup: every workgroup of a launch first forms the block's solution from the
+    // 256 x 256 inverted diagonal block, so a launch of many workgroups re-reads that block many
+    // times. Above TRSV_WG1 one-chunk workgroups, each workgroup takes 4 chunks. The forward
+    // launches read their workgroup prefix per item from the staged buffer after the items.
+    std::vector<char> buf((const char *)v.data(), (const char *)(v.data() + n_it));
+    std::vector<int> gf(nbk), gb(nbk);
+    std::vector<size_t> wo(nbk);
+    std::vector<int64_t> gridf(nbk);
     for (int k = 0; k < nbk; k++) {
         const int a = act[k];
-        const int64_t grid = (a < n_it ? v[a].pre : v[n_it - 1].pre + v[n_it - 1].c) - 4LL * k * a;
-        if (grid <= 0 || grid > 0x7fffffff) return bk_fail("trsv: bad forward grid");
-        hipLaunchKernelGGL((k_trsv_fwd<T>), dim3((unsigned)grid), dim3(1024), 0, g_bk_stream, d, a, k, A, V);
+        int64_t tot = 0;
+        for (int i = 0; i < a; i++) tot += v[i].c - 4LL * k;
+        gf[k] = tot > TRSV_WG1 ? 4 : 1;
+        gb[k] = (int64_t)a * (4 * k + 4) > TRSV_WG1 ? 4 : 1;
+        wo[k] = buf_align(buf);
+        buf.resize(wo[k] + sizeof(int) * (a + 1));
+        int64_t acc = 0;
+        for (int i = 0; i <= a; i++) {
+            ((int *)(buf.data() + wo[k]))[i] = (int)acc;
+            if (i < a) acc += (v[i].c - 4LL * k + gf[k] - 1) / gf[k];
+        }
+        gridf[k] = acc;
+        if (acc <= 0 || acc > 0x7fffffff) return bk_fail("trsv: bad forward grid");
+    }
+    const char *dd = (const char *)bk_stage(buf.data(), buf.size());
+    if (!dd) return -1;
+    const TrsvItem *d = (const TrsvItem *)dd;
+    const int tok = bk_prof_begin();
+    for (int k = 0; k < nbk; k++) {
+        hipLaunchKernelGGL((k_trsv_fwd<T>), dim3((unsigned)gridf[k]), dim3(1024), 0, g_bk_stream, d, (const int *)(dd + wo[k]),
+                           act[k], k, gf[k], A, V);
         if (BK_LAUNCH_CHECK("k_trsv_fwd")) return -1;
     }
     for (int k = nbk - 1; k >= 0; k--) {
-        const int64_t grid = (int64_t)act[k] * (4 * k + 4);
+        const int64_t grid = (int64_t)act[k] * ((4 * k + 4 + gb[k] - 1) / gb[k]);
         if (grid <= 0 || grid > 0x7fffffff) return bk_fail("trsv: bad backward grid");
-        hipLaunchKernelGGL((k_trsv_bwd<T>), dim3((unsigned)grid), dim3(1024), 0, g_bk_stream, d, act[k], k, A, V);
+        hipLaunchKernelGGL((k_trsv_bwd<T>), dim3((unsigned)grid), dim3(1024), 0, g_bk_stream, d, act[k], k, gb[k], A, V);
         if (BK_LAUNCH_CHECK("k_trsv_bwd")) return -1;
     }
     if (tok >= 0) {

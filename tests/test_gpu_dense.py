@@ -173,14 +173,23 @@ def test_potrf_graph_replay(dt):
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
-def test_trsv_batched(dt):
+@pytest.mark.parametrize("sizes", [[1000, 1, 63, 64, 65, 255, 256, 257, 600, 5], [2100] * 16 + [300, 70]],
+                         ids=["ragged", "wide"])
+def test_trsv_batched(dt, sizes):
     """Factor, inverted 256 x 256 diagonal blocks, then the two blocked triangular solves for a
     ragged batch (orders around the 64 and 256 block edges). Checked against the same block
     algorithm in numpy on the device's own L and diagonal inverses (1e-12: it pins every index),
-    and against a direct solve with L L' (the f32 factor's diagonal inverses carry f32 error)."""
+    and against a direct solve with L L' (the f32 factor's diagonal inverses carry f32 error).
+    The wide batch has more than 512 chunks in its first forward and last backward launches, whose
+    workgroups then take four chunks each."""
     rng = np.random.default_rng(13)
-    sizes = [1000, 1, 63, 64, 65, 255, 256, 257, 600, 5]
-    mats = [_spd(n, rng, 1e3) for n in sizes]
+    if len(sizes) > 12:
+        mats = []
+        for n in sizes:
+            B = rng.standard_normal((n, n))
+            mats.append(B @ B.T / n + np.eye(n))
+    else:
+        mats = [_spd(n, rng, 1e3) for n in sizes]
     buf, offs = _pack(mats, dt)
     tot = buf.size
     big = np.concatenate([buf, np.zeros(2 * tot, dt)])          # L | diagonal inverses | transposes
