@@ -680,6 +680,9 @@ __device__ __forceinline__ double trsv_dot(const T *__restrict__ p, int64_t stri
 #ifndef TRSV_WG1
 #define TRSV_WG1 512
 #endif
+#ifndef TRSV_G
+#define TRSV_G 4
+#endif
 // 1024 threads: the block's 256 x 256 product as thread (row, quarter) over 64 columns each, the
 // update as 16 groups of 16 columns (forward) or 16 waves of 4 columns (backward); partial sums
 // added in a fixed order
@@ -865,8 +868,8 @@ static int trsv_run(const TrsvItem *it, int n_it, const T *A, double *V) {
         const int a = act[k];
         int64_t tot = 0;
         for (int i = 0; i < a; i++) tot += v[i].c - 4LL * k;
-        gf[k] = tot > TRSV_WG1 ? 4 : 1;
-        gb[k] = (int64_t)a * (4 * k + 4) > TRSV_WG1 ? 4 : 1;
+        gf[k] = tot > TRSV_WG1 ? TRSV_G : 1;
+        gb[k] = (int64_t)a * (4 * k + 4) > TRSV_WG1 ? TRSV_G : 1;
         wo[k] = buf_align(buf);
         buf.resize(wo[k] + sizeof(int) * (a + 1));
         int64_t acc = 0;
