@@ -45,6 +45,11 @@ API int brisk_interrupted(void) { return g_stop != 0; }
 API void brisk_clear_interrupt(void) { g_stop = 0; }
 API void brisk_set_interrupt_owner(int caller) { g_stop_owner = caller != 0; }
 int brisk_stop_requested(void) { return g_stop != 0; }
+/* read at every call, so that a test can set and clear BRISK_FAULT within one process */
+int brisk_fault(const char *name) {
+    const char *f = getenv("BRISK_FAULT");
+    return f && strstr(f, name) != NULL;
+}
 
 static void say(int is_err, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 #include <stdarg.h>
@@ -63,6 +68,14 @@ static void result_clear(BriskResult *r) {
     free(r->bs); free(r->y); free(r->Xp); free(r->Zp); free(r->X); free(r->Z);
     memset(r, 0, sizeof(*r));
     r->status = -1;
+}
+/* a result that was not solved: status -1, exit code -1 (the reference's code for an aborted
+ * call, capi.c:171-174), and its cause */
+void brisk_result_unsolved(BriskResult *r, const char *cause) {
+    if (!r) return;
+    result_clear(r);
+    r->exit_code = -1;
+    snprintf(r->cause, sizeof r->cause, "%s", cause ? cause : "");
 }
 void brisk_result_move(BriskResult *dst, BriskResult *src) {
     result_clear(dst);
@@ -181,7 +194,7 @@ void brisk_fill_result(BriskResult *r, Solve *s, double t_setup, double t_iter, 
         o += P->bs_in[k] > 0 ? (int64_t)P->bs_in[k] * P->bs_in[k] : -P->bs_in[k];
     }
     if (r->status != 0)
-        snprintf(r->cause, sizeof r->cause, "DIMACS error %.1e after %d iterations", fmax(fmax(r->err[0], r->err[3]), fmax(fabs(r->err[4]), fabs(r->err[5]))), r->iters);
+        snprintf(r->cause, sizeof r->cause, "DIMACS error %.1e after %d iterations", acc_err(r->err), r->iters);
 }
 
 static int solve_many(int n, const BriskProblem *pr, int nopt, const char *const *opts, BriskResult **res) {

@@ -149,6 +149,7 @@ int64_t solve_heap_bytes(const HProb *P) {
 
 int solve_setup(Solve *S) {
     HProb *P = &S->P;
+    if (brisk_fault("setup")) return -1;
     if (g_naux == g_capaux) { g_capaux = g_capaux ? 2 * g_capaux : 16; g_aux = realloc(g_aux, sizeof(SAux) * g_capaux); }
     SAux *A = &g_aux[g_naux++];
     memset(A, 0, sizeof(*A));
@@ -824,35 +825,27 @@ static int schur_factor_(Solve **S, int ns, const int *act, int force64, int *fa
      * precision group. Again the f32 factors on the library stream and the f64 ones on the side
      * stream. */
     {
-        int cnt[2] = { 0, 0 }, use_trsv[2] = { 0, 0 };
-        double fl[2] = { 0, 0 };
-        int nbk[2] = { 0, 0 };
+        int cnt[2] = { 0, 0 };
         for (int q = 0; q < ns; q++) {
             if ((act && !act[q]) || fail[q]) continue;
-            const int prec = S[q]->f32 ? 0 : 1;
-            const double m = S[q]->P.m;
+            Solve *s = S[q];
+            const int prec = s->f32 ? 0 : 1;
             cnt[prec]++;
-            fl[prec] += m * m * m / 3.0;
-            const int b = (S[q]->P.m + 255) / 256;
-            if (b > nbk[prec]) nbk[prec] = b;
+            /* a solve's -schurapply option (inverse, solves, or auto: the cost model below, on the
+             * solve's own order and precision, so that its arithmetic does not depend on the
+             * batch). The inverse: m^3 / 3 flops at the rates measured for it (f32 3e13, f64
+             * 1.5e13 flop/s) plus 0.3 ms of launches; the solves: two launches of about 15 us per
+             * 256-wide block column (measured over serving batches) for each of the 6 (f32) or 3
+             * (f64) preconditioner applications of a PCG. The solves lengthen every PCG round by
+             * their launch chain, so they are taken for an f64 factor whose inverse costs over
+             * 3 ms and twice the solves (m above about 5000), for an f32 factor only past 15 ms
+             * and three times the solves. */
+            const double m = s->P.m;
+            const double t_inv = m * m * m / 3.0 / (prec ? 1.5e13 : 3.0e13) + 3e-4;
+            const double t_sv = (prec ? 3 : 6) * 2.0 * ((s->P.m + 255) / 256) * 15e-6;
+            const int model = prec ? t_inv > 3e-3 && t_sv < t_inv / 2 : t_inv > 15e-3 && t_sv < t_inv / 3;
+            s->trsv = s->o.apply < 0 ? model : s->o.apply;
         }
-        for (int prec = 0; prec < 2; prec++) {
-            /* the inverse: its flops at the rates measured for it (f32 3e13, f64 1.5e13 flop/s) and
-             * a fixed 0.3 ms of launches; the solves: two launches of about 15 us per block column
-             * (measured over serving batches), 6 preconditioner applications per f32 factor (the
-             * PCG's usual count), 3 per f64 (the FP64 factor's PCG ends in a round or two). The
-             * solves lengthen every PCG round by their launch chain, which the latency feels, so
-             * they are taken for an f64 group whose inverse costs over 3 ms and twice the solves,
-             * for an f32 group only past 15 ms and three times the solves. Measured over the
-             * serving stream: 365 -> 384 solves/s at saturation, Poisson p99 0.87 -> 1.02 s (at a
-             * 5% higher rate); f32 groups on the solves too added latency for no throughput. */
-            const double t_inv = fl[prec] / (prec ? 1.5e13 : 3.0e13) + 3e-4;
-            const double t_sv = (prec ? 3 : 6) * 2.0 * nbk[prec] * 15e-6;
-            use_trsv[prec] = prec ? t_inv > 3e-3 && t_sv < t_inv / 2 : t_inv > 15e-3 && t_sv < t_inv / 3;
-        }
-        /* a solve's -schurapply option (inverse, solves, or auto: the model) */
-        for (int q = 0; q < ns; q++)
-            if (!(act && !act[q]) && !fail[q]) S[q]->trsv = S[q]->o.apply < 0 ? use_trsv[S[q]->f32 ? 0 : 1] : S[q]->o.apply;
         bk_zero(dint(), sizeof(int) * ns);
         if (cnt[0] && cnt[1] && bk_fork()) goto err;
         for (int prec = 0; prec < 2; prec++) {
@@ -1352,7 +1345,7 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
         s->relgap = fabs(po - dob) / oden;
         s->relcomp = sc * xz / oden;
         s->pobj = po; s->dobj = dob; s->mu = mu;
-        double score = fmax(fmax(s->relgap, s->relcomp), fmax(s->pinf, s->dinf));
+        double score = iterate_score(s->relgap, s->relcomp, s->pinf, s->dinf);
         if (!isfinite(score)) score = INFINITY;
         s->score = score;
         const int it = s->it;
@@ -1656,9 +1649,12 @@ int ipm_dimacs(Solve **S, int ns) {
     if (blk_chol(S, ns, act, OFF(X), OFF(LX), OFF(DinvX), okx)) goto out;
     if (blk_chol(S, ns, act, OFF(W1), OFF(LZ), OFF(DinvZ), okz)) goto out;
     /* a side without a factor: lambda_min >= -h when V + h I factors. h = delta = 1e-12 (1 + |b|_1)
-     * for X, 1e-12 (1 + |C|_1) for Z, then decades of delta up to 1e9 delta and three bisections
-     * in the exponent, so the bound is within a factor 1.33 (ps_lammin_tol, postsolve.c:668-726);
-     * beyond 1e9 delta (an error past 1e-3, a verdict only) the Lanczos estimate */
+     * for X, 1e-12 (1 + |C|_1) for Z, then decades of delta up to 1e9 delta, three bisections in
+     * the exponent (a factor 1.33, as ps_lammin_tol's ladder, postsolve.c:668-726) and bisections
+     * of the interval until it is within 1e-4 of h: the reported -lambda_min is the smallest
+     * shift that factored, within 1e-4 relative of the eigenvalue the reference computes for
+     * blocks of order <= 500 (postsolve.c:685); beyond 1e9 delta (an error past 1e-3, a verdict
+     * only) the Lanczos estimate */
     for (int side = 0; side < 2; side++) {
         int *ok = side ? okz : okx, *need = calloc(ns, sizeof(int)), *ok2 = calloc(ns, sizeof(int)), any = 0;
         double *dl = calloc(ns, sizeof(double)), *lo = calloc(ns, sizeof(double)), *hi = calloc(ns, sizeof(double));
@@ -1678,13 +1674,20 @@ int ipm_dimacs(Solve **S, int ns) {
                 else any = 1;
             }
         }
-        /* three bisections in the exponent for the solves bounded past delta */
+        /* three bisections in the exponent, then of the interval, for the solves bounded past
+         * delta (at most 16 more factorizations) */
         int *bis = calloc(ns, sizeof(int)), nb = 0;
         for (int q = 0; q < ns; q++) if (hi[q] > 0 && lo[q] > 0) { bis[q] = 1; nb++; }
-        for (int b = 0; nb && b < 3; b++) {
-            for (int q = 0; q < ns; q++) g_hdbl[q] = bis[q] ? sqrt(lo[q] * hi[q]) : 0;
+        for (int b = 0; nb && b < 19; b++) {
+            for (int q = 0; q < ns; q++) g_hdbl[q] = bis[q] ? (b < 3 ? sqrt(lo[q] * hi[q]) : 0.5 * (lo[q] + hi[q])) : 0;
             if (shifted_chol(S, ns, bis, fV, fL, fD, ok2)) { free(bis); free(need); free(ok2); free(dl); free(lo); free(hi); goto out; }
-            for (int q = 0; q < ns; q++) if (bis[q]) { if (ok2[q]) hi[q] = g_hdbl[q]; else lo[q] = g_hdbl[q]; }
+            nb = 0;
+            for (int q = 0; q < ns; q++) {
+                if (!bis[q]) continue;
+                if (ok2[q]) hi[q] = g_hdbl[q]; else lo[q] = g_hdbl[q];
+                if (b >= 2 && hi[q] - lo[q] <= 1e-4 * hi[q]) bis[q] = 0;
+                else nb++;
+            }
         }
         free(bis);
         for (int q = 0; q < ns; q++) if (!ok[q] && hi[q] > 0) {
@@ -1696,25 +1699,14 @@ int ipm_dimacs(Solve **S, int ns) {
             if (lammin_all(S, ns, need, fV, lam)) { free(need); free(ok2); free(dl); free(lo); free(hi); goto out; }
             for (int q = 0; q < ns; q++) if (need[q]) {
                 const HProb *P = &S[q]->P;
-                const double e = fmax(0.0, -(side ? P->cs : P->bs) * lam[q]) / (1 + (side ? P->normC1 : P->normb1));
-                S[q]->res_err[side ? 3 : 1] = fmax(e, (side ? P->cs : P->bs) * dl[q] * 1e9 / (1 + (side ? P->normC1 : P->normb1)));
+                const double e = nmax(0.0, -(side ? P->cs : P->bs) * lam[q]) / (1 + (side ? P->normC1 : P->normb1));
+                S[q]->res_err[side ? 3 : 1] = nmax(e, (side ? P->cs : P->bs) * dl[q] * 1e9 / (1 + (side ? P->normC1 : P->normb1)));
             }
         }
         free(need); free(ok2); free(dl); free(lo); free(hi);
     }
     /* status from the measured errors (main.c:2349-2359, brisk.h:187-189) */
-    for (int q = 0; q < ns; q++) {
-        Solve *s = S[q];
-        double acc = fmax(fmax(s->res_err[0], s->res_err[1]), fmax(s->res_err[3], fmax(fabs(s->res_err[4]), fabs(s->res_err[5]))));
-        if (!isfinite(acc)) acc = INFINITY;
-        int st = s->status;
-        if (st == 0 || st == 3 || st == 4 || st == 5 || st == 6) {
-            if (acc <= s->o.tol) st = 0;
-            else if (acc <= 1e-6) st = 5;
-            else if (st == 0 || st == 5) st = 4;
-        }
-        s->status = st;
-    }
+    for (int q = 0; q < ns; q++) S[q]->status = status_from_errors(S[q]->status, S[q]->res_err, S[q]->o.tol);
     rc = 0;
 out:
     free(act); free(okx); free(okz); free(lam); free(hrp);
@@ -1830,7 +1822,7 @@ static int ipm_iterate_hsd_(Solve **S, int ns) {
         s->relgap = fabs(po - dob) / oden;
         s->relcomp = sc * xz / (tau * tau) / oden;
         s->pobj = po; s->dobj = dob; s->mu = xz / P->Ntot / (tau * tau);
-        double score = fmax(fmax(s->relgap, s->relcomp), fmax(s->pinf, s->dinf));
+        double score = iterate_score(s->relgap, s->relcomp, s->pinf, s->dinf);
         if (!isfinite(score)) score = INFINITY;
         s->score = score;
         const int it = s->it;
@@ -2131,4 +2123,53 @@ static int ipm_end_batch_(Solve **S, int ns) {
     if (ipm_dimacs(S, ns)) return -1;
     for (int q = 0; q < ns; q++) if (ipm_finish(S[q])) return -1;
     return 0;
+}
+
+/* ---- kernel-level probes: one kernel family of the iteration on caller data, through the same
+ * setup, layout and launches as the iteration (entry points of the parity tests, probe.c) ---- */
+/* Zi = LZi' LZi from Z, and LX from X, as at the start of an iteration; then the Schur
+ * complement at (X, Zi) in the precision asked for (prec 1 FP64; 0 the mixed path's FP32 dense
+ * route), into s->M */
+int ipm_probe_schur(Solve *s, int prec) {
+    Solve *S[1] = { s };
+    int ok[2] = { 0, 0 }, a2[2] = { 1, 1 };
+    s->o.prec = prec ? PREC_F64 : PREC_MIXED;
+    s->want_double = prec != 0;
+    if (batch_scratch(1, s->P.nb + 2)) return -1;
+    if (chol_sides(S, 1, 2, NULL, F_V, F_L, F_D, F_W, ok) || !ok[0] || !ok[1]) return -1;
+    if (inv_sides(S, 1, 2, a2, F_L, F_D, F_Li, F_LiT, F_W)) return -1;
+    if (blk_gemm(S, 1, NULL, 1, 0, 2 | 8, 1.0, 0.0, OFF(LZi), OFF(LZi), OFF(Zi)) || blk_sym(S, 1, NULL, OFF(Zi))) return -1;
+    return schur_assemble(S, 1, NULL, 0) || bk_sync() ? -1 : 0;
+}
+/* s->rp = A(D) with D in s->W1 */
+int ipm_probe_aop(Solve *s) {
+    Solve *S[1] = { s };
+    return aop(S, 1, NULL, OFF(W1), OFF(rp), 1.0, 0.0) || bk_sync() ? -1 : 0;
+}
+/* s->W1 = A*(y) with y in s->y */
+int ipm_probe_aty(Solve *s) {
+    Solve *S[1] = { s };
+    LinL_clear(&g_ll);
+    lin_add(s->P.T, s->W1, -1, 0, -1, 0, -1, 0, -1, 0);
+    if (bk_lincomb(g_ll.v, g_ll.n)) return -1;
+    return aty(S, 1, NULL, OFF(y), OFF(W1), 1.0) || bk_sync() ? -1 : 0;
+}
+/* the Lanczos bound of the step-length test, smallest over the solve's blocks: mode 0
+ * lambda_min(X) (the start point's test), mode 1 lambda_min(LX^-1 dX LX^-T) with LX the
+ * Cholesky factor of X (the test of maxstep); *out = theta - res */
+int ipm_probe_lammin(Solve *s, int mode, double *out) {
+    Solve *S[1] = { s };
+    if (batch_scratch(1, s->P.nb + 2)) return -1;
+    if (mode == 0) return lammin_all(S, 1, NULL, OFF(X), out);
+    int ok = 0, a1 = 1;
+    if (blk_chol(S, 1, NULL, OFF(X), OFF(LX), OFF(DinvX), &ok) || !ok) return -1;
+    if (inv_sides(S, 1, 1, &a1, F_L, F_D, F_Li, F_LiT, F_W)) return -1;
+    const size_t fLi[1] = { OFF(LXi) }, fLiT[1] = { OFF(LXiT) }, fV[1] = { OFF(dX) };
+    return lanczos_min(S, 1, NULL, 1, fLi, fLiT, fV, 0, 0.02, out);
+}
+/* the DIMACS errors of (X, y) in s->X, s->y (scaled), as for a returned point */
+int ipm_probe_dimacs(Solve *s) {
+    Solve *S[1] = { s };
+    s->status = 0;
+    return ipm_dimacs(S, 1);
 }

@@ -457,61 +457,57 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
     struct Step { int k0; GemmPlan upd, panel, trail; int has_trail; };
     std::vector<Step> st;
     std::vector<GemmItem> gu, gp, gt;
-    // panel width: 128 for a batch whose factorization is latency-bound (sum of n^3 below 5e10,
-    // e.g. two n = 2000 blocks: 1.37 -> 1.30 ms in FP64), 256 for larger batches, whose trailing
-    // updates run at higher throughput with K = 256 (four n = 5000 blocks in FP32: 4.87 -> 4.60 ms)
-    double sn3 = 0;
-    for (int q = 0; q < n_it; q++) sn3 += (double)it[q].n * it[q].n * it[q].n;
-    const int PW = sn3 < 5e10 ? 128 : 256;
-    for (int K0 = 0; K0 < nmax; K0 += PW) {
-        for (int sub = 0; sub < PW / 64; sub++) {
-            const int k0 = K0 + 64 * sub;
-            if (k0 >= nmax) break;
-            gu.clear(); gp.clear(); gt.clear();
-            for (int q = 0; q < n_it; q++) {
-                const int n = it[q].n;
-                if (k0 >= n) continue;
-                GemmItem g;
-                if (sub > 0) {
-                    // columns k0 .. k0+64 (rows k0 ..) -= L[k0.., K0..k0] L[k0..k0+64, K0..k0]'
-                    g.M = n - k0; g.N = min(64, n - k0); g.K = k0 - K0;
-                    g.lda = n; g.ldb = n; g.ldc = n;
-                    g.a = it[q].off + k0 + (int64_t)K0 * n; g.b = g.a;
-                    g.c = it[q].off + k0 + (int64_t)k0 * n;
-                    g.guard = it[q].slot; g.tri = 0;
-                    gu.push_back(g);
-                }
-                if (k0 + 64 < n) {
-                    // panel, in place: A[k0+64:, k0:k0+64] := A[k0+64:, k0:k0+64] Dinv_k' (Dinv_k'
-                    // upper triangular). The 64 x 64 tiles span the panel's full width, so every
-                    // workgroup reads its rows completely before it writes them.
-                    g.M = n - k0 - 64; g.N = 64; g.K = 64;
-                    g.lda = n; g.ldb = 64; g.ldc = n;
-                    g.a = it[q].off + (k0 + 64) + (int64_t)k0 * n;
-                    g.b = it[q].off2 + (int64_t)(k0 / 64) * 64 * 64;
-                    g.c = g.a;
-                    g.guard = it[q].slot; g.tri = 16;
-                    gp.push_back(g);
-                }
-                const int e = K0 + PW;
-                if (sub == PW / 64 - 1 && e < n) {
-                    // trailing: A[e:, e:] -= L[e:, K0:e] L[e:, K0:e]' (lower triangle)
-                    g.M = n - e; g.N = n - e; g.K = PW;
-                    g.lda = n; g.ldb = n; g.ldc = n;
-                    g.a = it[q].off + e + (int64_t)K0 * n; g.b = g.a;
-                    g.c = it[q].off + e + (int64_t)e * n;
-                    g.guard = it[q].slot; g.tri = 1;
-                    gt.push_back(g);
-                }
+    // panel width, chosen per matrix from its own order so that a matrix is factored the same way
+    // in any batch: 128 columns below n^3 = 5e10 (n < 3684; latency-bound, e.g. two n = 2000
+    // blocks in FP64: 1.37 -> 1.30 ms), 256 above, where the trailing updates run at higher
+    // throughput with K = 256 (four n = 5000 blocks in FP32: 4.87 -> 4.60 ms)
+    auto panel_width = [](int n) { return (double)n * n * n < 5e10 ? 128 : 256; };
+    for (int k0 = 0; k0 < nmax; k0 += 64) {
+        gu.clear(); gp.clear(); gt.clear();
+        for (int q = 0; q < n_it; q++) {
+            const int n = it[q].n;
+            if (k0 >= n) continue;
+            const int PW = panel_width(n), K0 = k0 - k0 % PW, sub = (k0 - K0) / 64;
+            GemmItem g;
+            if (sub > 0) {
+                // columns k0 .. k0+64 (rows k0 ..) -= L[k0.., K0..k0] L[k0..k0+64, K0..k0]'
+                g.M = n - k0; g.N = min(64, n - k0); g.K = k0 - K0;
+                g.lda = n; g.ldb = n; g.ldc = n;
+                g.a = it[q].off + k0 + (int64_t)K0 * n; g.b = g.a;
+                g.c = it[q].off + k0 + (int64_t)k0 * n;
+                g.guard = it[q].slot; g.tri = 0;
+                gu.push_back(g);
             }
-            Step S;
-            S.k0 = k0;
-            S.upd = gemm_plan<T>(buf, gu.data(), (int)gu.size());
-            S.panel = gemm_plan<T>(buf, gp.data(), (int)gp.size());
-            S.trail = gemm_plan<T>(buf, gt.data(), (int)gt.size());
-            S.has_trail = !gt.empty();
-            st.push_back(S);
+            if (k0 + 64 < n) {
+                // panel, in place: A[k0+64:, k0:k0+64] := A[k0+64:, k0:k0+64] Dinv_k' (Dinv_k'
+                // upper triangular). The 64 x 64 tiles span the panel's full width, so every
+                // workgroup reads its rows completely before it writes them.
+                g.M = n - k0 - 64; g.N = 64; g.K = 64;
+                g.lda = n; g.ldb = 64; g.ldc = n;
+                g.a = it[q].off + (k0 + 64) + (int64_t)k0 * n;
+                g.b = it[q].off2 + (int64_t)(k0 / 64) * 64 * 64;
+                g.c = g.a;
+                g.guard = it[q].slot; g.tri = 16;
+                gp.push_back(g);
+            }
+            const int e = K0 + PW;
+            if (sub == PW / 64 - 1 && e < n) {
+                // trailing: A[e:, e:] -= L[e:, K0:e] L[e:, K0:e]' (lower triangle)
+                g.M = n - e; g.N = n - e; g.K = PW;
+                g.lda = n; g.ldb = n; g.ldc = n;
+                g.a = it[q].off + e + (int64_t)K0 * n; g.b = g.a;
+                g.c = it[q].off + e + (int64_t)e * n;
+                g.guard = it[q].slot; g.tri = 1;
+                gt.push_back(g);
+            }
         }
+        Step S;
+        S.k0 = k0;
+        S.upd = gemm_plan<T>(buf, gu.data(), (int)gu.size());
+        S.panel = gemm_plan<T>(buf, gp.data(), (int)gp.size());
+        S.trail = gemm_plan<T>(buf, gt.data(), (int)gt.size());
+        S.has_trail = !gt.empty();
+        st.push_back(S);
     }
     // the launch sequence, reading every plan from the device copy d
     auto launch = [&](const char *d) -> int {

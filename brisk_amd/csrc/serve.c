@@ -17,6 +17,7 @@
 int brisk_parse_options(int nopt, const char *const *opts, SolveOpts *o, int m, int *slots, double *heapgb);
 int brisk_heap_ensure(size_t need);
 void brisk_fill_result(BriskResult *r, Solve *s, double t_setup, double t_iter, double t_dimacs, double t_down);
+void brisk_result_unsolved(BriskResult *r, const char *cause);
 
 typedef struct {
     int state;                  /* 0 waiting, 1 in flight, 2 finished, 3 taken */
@@ -122,6 +123,22 @@ static void finish_done(BriskServe *sv) {
     free(D);
 }
 
+/* a request that cannot be solved ends unsolved (status -1, exit code -1), its cause given */
+static void finish_unsolved(BriskServe *sv, int id, const char *cause) {
+    Req *r = &sv->req[id];
+    if (r->s) {
+        hprob_free(&r->s->P);
+        free(r->s->res_X); free(r->s->res_Z); free(r->s->res_y);
+        free(r->s);
+        r->s = NULL;
+    }
+    r->res = brisk_result_new();
+    brisk_result_unsolved(r->res, cause);
+    r->t_done = bk_now();
+    r->state = 2;
+    fin_push(sv, id);
+}
+
 API int brisk_serve_step(BriskServe *sv) {
     if (!sv) return -1;
     /* admission: in arrival order while a slot and the solve's projected memory are free */
@@ -134,18 +151,21 @@ API int brisk_serve_step(BriskServe *sv) {
         const size_t need = (size_t)solve_heap_bytes(&s->P) + ((size_t)1 << 20);
         if (heap_largest_free() < need) {
             if (sv->nfly + nn == 0 && brisk_heap_ensure(need)) {
-                /* larger than the device memory: finished unsolved (status -1, exit code -1) */
-                Req *r = &sv->req[id];
-                r->res = brisk_result_new();
-                r->t_done = bk_now();
-                r->state = 2;
+                /* larger than the device memory */
                 sv->qhead++;
-                fin_push(sv, id);
+                finish_unsolved(sv, id, "the solve needs more device memory than the GPU has");
                 continue;
             }
             if (heap_largest_free() < need) break;
         }
-        if (solve_setup(s)) break;
+        if (solve_setup(s)) {
+            /* with solves in flight the request waits for them to free memory; with none, no
+             * later step can do better: it ends unsolved */
+            if (sv->nfly + nn > 0) break;
+            sv->qhead++;
+            finish_unsolved(sv, id, "device setup of the solve failed");
+            continue;
+        }
         sv->qhead++;
         sv->req[id].state = 1;
         sv->req[id].t_admit = bk_now();
@@ -153,7 +173,12 @@ API int brisk_serve_step(BriskServe *sv) {
     }
     if (t_adm > 0) bk_prof_host(48, bk_now() - t_adm);
     if (nn) {
-        if (ipm_start_batch(nw, nn)) { free(nw); return -1; }
+        if (ipm_start_batch(nw, nn)) {
+            /* a GPU failure: the newcomers end unsolved, their device memory released */
+            for (int q = 0; q < nn; q++) { solve_release(nw[q]); finish_unsolved(sv, nw[q]->id, "the start point failed on the GPU"); }
+            free(nw);
+            return -1;
+        }
         for (int q = 0; q < nn; q++) { sv->fly[sv->nfly] = nw[q]; sv->fly_id[sv->nfly] = nw[q]->id; sv->nfly++; }
     }
     free(nw);

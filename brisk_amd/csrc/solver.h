@@ -6,11 +6,43 @@
  * (step lengths, sigma, theta, convergence state) on the host. */
 #ifndef BRISK_SOLVER_H
 #define BRISK_SOLVER_H
+#include <math.h>
 #include <stdint.h>
 #include <stddef.h>
 #include "brisk_dev.h"
 
 #define BK_LZ_K 30          /* Lanczos steps (reference Params.lanczos_k, solver.c:119) */
+
+/* max and min in which a NaN operand wins as +infinity (max) or -infinity (min): a NaN error or
+ * score never passes a test (reference brisk_nmax / brisk_nmin, brisk.h:182-184) */
+static inline double nmax(double a, double b) { return (a != a || b != b) ? INFINITY : (b > a ? b : a); }
+static inline double nmin(double a, double b) { return (a != a || b != b) ? -INFINITY : (a < b ? a : b); }
+/* the error a result is accepted by: primal equations, X's cone, Z's cone, gap, complementarity
+ * (err[0], err[1], err[3], |err[4]|, |err[5]|, 0-based; reference brisk_acc_err, brisk.h:187-189) */
+static inline double acc_err(const double *e) {
+    return nmax(nmax(e[0], e[1]), nmax(e[3], nmax(fabs(e[4]), fabs(e[5]))));
+}
+/* the status of a returned point from its measured DIMACS errors (reference main.c:2349-2359):
+ * a run that ended optimal, at a limit or in difficulties is OPTIMAL when acc <= tol, REDUCED
+ * ACCURACY (5) when acc <= 1e-6, NUMERICAL DIFFICULTIES (4) when it had claimed optimal or
+ * reduced accuracy; infeasibility verdicts (1, 2) stand */
+static inline int status_from_errors(int st, const double *err, double tol) {
+    const double acc = acc_err(err);
+    if (st == 0 || st == 3 || st == 4 || st == 5 || st == 6) {
+        if (acc <= tol) return 0;
+        if (acc <= 1e-6) return 5;
+        if (st == 0 || st == 5) return 4;
+    }
+    return st;
+}
+/* the termination score of an iterate: the largest of relative gap, complementarity and the
+ * two infeasibilities (reference res_score, solver.c:8089-8103) */
+static inline double iterate_score(double relgap, double relcomp, double pinf, double dinf) {
+    return nmax(nmax(relgap, relcomp), nmax(pinf, dinf));
+}
+/* fault injection for the tests of failure paths (the reference's brisk_fault, brisk.h:169-173):
+ * BRISK_FAULT=<names> makes each named check fail ("setup": a solve's device setup) */
+int brisk_fault(const char *name);
 
 typedef struct {
     /* the problem as read */
@@ -59,7 +91,7 @@ typedef struct {
     double mixed_until;     /* mixed: f32 factor while max(relgap, pinf, dinf) exceeds this (1e-6) */
     double time_limit;      /* s, <= 0 none */
     int hsd;                /* 1: self-dual embedding, 0: infeasible start */
-    int apply;              /* solves with the Schur factor: -1 a cost model per batch (default), 0 explicit inverse, 1 block triangular solves */
+    int apply;              /* solves with the Schur factor: -1 a cost model on the solve's own order and precision (default), 0 explicit inverse, 1 block triangular solves */
 } SolveOpts;
 
 typedef struct Solve {

@@ -82,10 +82,11 @@ typedef struct {
     int64_t off4;            /* trtri: the inverse; transpose: the destination */
 } MItem;
 
-/* Cholesky of a set of matrices, blocked right-looking (panels of 128 or 256 columns factored in
- * 64-column steps), in place on A (its lower triangle is read; on return A holds L with the
- * strict upper triangle zeroed). Diagonal-block inverses go to Dinv + off2 (64*64 per 64-column
- * step); W + off3 is scratch (n*64 used).
+/* Cholesky of a set of matrices, blocked right-looking (panels of 128 columns, or 256 for a matrix
+ * of order 3684 or more, chosen per matrix; each factored in 64-column steps), in place on A (its
+ * lower triangle is read; on return A holds L with the strict upper triangle zeroed).
+ * Diagonal-block inverses go to Dinv + off2 (64*64 per 64-column step); W + off3 is scratch
+ * (n*64 used).
  * info[slot] = first failing column + 1 (0: factored); pmin[slot] = smallest squared pivot. */
 int bk_potrf_f64(const MItem *it, int n_it, double *A, double *Dinv, double *W, int *info, double *pmin);
 int bk_potrf_f32(const MItem *it, int n_it, float *A, float *Dinv, float *W, int *info, float *pmin);

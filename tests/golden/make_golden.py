@@ -7,11 +7,14 @@ Runs only in a container that holds the reference tree (the fixtures travel, the
 not).  Usage:
     python tests/golden/make_golden.py small          # -> tests/golden/small.json (+ sdpa/*.dat-s)
     python tests/golden/make_golden.py large c2 c4    # -> tests/golden/large.json (entries merged)
-Each record: rc, status, pobj, dobj, iterations, DIMACS errors, y (for m <= 400), X (blocks
-concatenated, SDP column-major, when it has at most 4000 numbers), the options
+    python tests/golden/make_golden.py stream         # -> tests/golden/stream.json (serving requests)
+Each record: rc, status, pobj, dobj, iterations, DIMACS errors, y (for m <= 2000), X (blocks
+concatenated, SDP column-major, when it has at most 10000 numbers), the options
 and the seconds the reference took on this container's host; under "dd" the same problem solved
 by the reference in double-double arithmetic ("-prec dd", DIMACS errors near 1e-25), which
-gives the optimal value to far more digits than an FP64 solve at 1e-8.
+gives the optimal value to far more digits than an FP64 solve at 1e-8, and its y and X: their
+distance to the FP64 solve's point shows how far apart two 1e-8 solutions of the problem can lie
+(large where the optimum is not unique).
 """
 import ctypes, json, os, shutil, sys, time
 
@@ -45,6 +48,11 @@ LARGE = {
     "c3": ("dense_400_2000_s0", "dense", (400, 2000, 0), []),
 }
 
+# requests of the serving stream (theta / max-cut, n ~ U{20..200}, p ~ U[0.05, 0.3]; the stream's
+# own seeded draws, request k built with instance seed 1000 + k): the first 26, which hold two
+# theta requests with m > 1500 (m = 1529, 1707), and request 31 (theta, m = 4985)
+STREAM_IDS = list(range(26)) + [31]
+
 _lib = None
 
 
@@ -75,7 +83,7 @@ def lib():
     return _lib
 
 
-def record(call, opts):
+def record(call, opts, ymax=2000, xmax=10000):
     L = lib()
     r = L.brisk_result_new()
     o = (ctypes.c_char_p * len(opts))(*[x.encode() for x in opts])
@@ -89,12 +97,12 @@ def record(call, opts):
     out = dict(rc=rc, status=L.brisk_result_status(r), exit_code=L.brisk_result_exit_code(r),
                pobj=L.brisk_result_pobj(r), dobj=L.brisk_result_dobj(r), iterations=L.brisk_result_iterations(r),
                dimacs=list(e), m=m, options=opts, seconds=round(t, 3))
-    if yp and 0 < m <= 400:
+    if yp and 0 < m <= ymax:
         out["y"] = [yp[k] for k in range(m)]
     nb = L.brisk_result_nblk(r)
     bs = [L.brisk_result_blocksize(r, k) for k in range(nb)]
     xlen = sum(b * b if b > 0 else -b for b in bs)
-    if L.brisk_result_have_x(r) and 0 < xlen <= 4000:
+    if L.brisk_result_have_x(r) and 0 < xlen <= xmax:
         X = []
         for k, b in enumerate(bs):
             xp = L.brisk_result_X(r, k)
@@ -105,14 +113,14 @@ def record(call, opts):
     return out
 
 
-def gen_record(kind, args, extra, dd=True):
+def gen_record(kind, args, extra, dd=True, ymax=2000):
     inst = Instance(kind, *args)
     L = lib()
     opts = ["-threads", "8"] + list(extra)
-    rec = record(lambda n, o, r: L.brisk_solve_data(*inst.data_args(), n, o, r), opts)
+    rec = record(lambda n, o, r: L.brisk_solve_data(*inst.data_args(), n, o, r), opts, ymax)
     if dd:
         hp = record(lambda n, o, r: L.brisk_solve_data(*inst.data_args(), n, o, r), opts + ["-prec", "dd"])
-        rec["dd"] = {k: hp[k] for k in ("status", "pobj", "dobj", "dimacs", "seconds")}
+        rec["dd"] = {k: hp[k] for k in ("status", "pobj", "dobj", "dimacs", "seconds", "y", "X") if k in hp}
     rec.update(generator=kind, args=list(args))
     return rec
 
@@ -122,7 +130,7 @@ def file_record(name):
     L = lib()
     rec = record(lambda n, o, r: L.brisk_solve_file(path.encode(), n, o, r), ["-threads", "8"])
     hp = record(lambda n, o, r: L.brisk_solve_file(path.encode(), n, o, r), ["-threads", "8", "-prec", "dd"])
-    rec["dd"] = {k: hp[k] for k in ("status", "pobj", "dobj", "dimacs", "seconds")}
+    rec["dd"] = {k: hp[k] for k in ("status", "pobj", "dobj", "dimacs", "seconds", "y", "X") if k in hp}
     rec.update(file="sdpa/%s.dat-s" % name)
     return rec
 
@@ -141,12 +149,25 @@ def main():
             out["file_" + name] = file_record(name)
             print(name, out["file_" + name]["status"], out["file_" + name]["pobj"], flush=True)
         dst = os.path.join(HERE, "small.json")
+    elif which == "stream":
+        sys.path.insert(0, ROOT)
+        import bench
+        reqs, _ = bench.c5_stream()
+        out = {}
+        for k in STREAM_IDS:
+            kind, n, p, seed = reqs[k]
+            name = "req%04d_%s_%d" % (k, kind, n)
+            m = Instance(kind, n, p, seed).m
+            out[name] = gen_record(kind, (n, p, seed), [], dd=m <= 2000, ymax=2000)
+            out[name]["request"] = k
+            print(name, out[name]["m"], out[name]["status"], out[name]["pobj"], out[name]["seconds"], flush=True)
+        dst = os.path.join(HERE, "stream.json")
     else:
         dst = os.path.join(HERE, "large.json")
         out = json.load(open(dst)) if os.path.exists(dst) else {}
         for key in sys.argv[2:]:
             name, kind, args, extra = LARGE[key]
-            out[name] = gen_record(kind, args, extra, dd=False)
+            out[name] = gen_record(kind, args, extra, dd=False, ymax=5000)
             out[name]["workload"] = key
             print(name, out[name]["status"], out[name]["pobj"], out[name]["seconds"], flush=True)
     with open(dst, "w") as f:

@@ -1,41 +1,21 @@
 """The GPU solve against the reference's own outputs (golden fixtures written by the reference
 library on the same instances): same status, objectives within 1e-7 relative, DIMACS errors
-<= 1e-8; the SDPA example files through brisk_solve_file; the full-size workloads."""
+<= 1e-8, and the returned y and X within the iterate-distance bar (tests/_parity.py); the SDPA
+example files through brisk_solve_file; the full-size workloads in both precisions."""
 import os
-import numpy as np
 import pytest
 import brisk_amd
 from _ctypes_util import ROOT, golden, golden_instance
+from _parity import check_vs_reference
 
 pytestmark = pytest.mark.gpu
 
 SMALL = golden("small")
 LARGE = golden("large") if os.path.exists(os.path.join(ROOT, "tests", "golden", "large.json")) else {}
-TOL_OBJ, TOL_DIMACS = 1e-7, 1e-8
 
 
-def rel(a, b):
-    return abs(a - b) / (1.0 + abs(b))
-
-
-def objective_ok(ours, rec, key):
-    """Within 1e-7 of the reference's FP64 solve, or (where that solve is itself further than
-    that from the optimum its double-double solve gives) at least as close to the optimum."""
-    if rel(ours, rec[key]) <= TOL_OBJ:
-        return True
-    if "dd" not in rec:
-        return False
-    dd = rec["dd"][key]
-    return rel(ours, dd) <= max(rel(rec[key], dd), 1e-9)
-
-
-def check(r, rec):
-    print("gpu", r.status_str, r.iterations, repr(r.pobj), repr(r.dobj), r.dimacs, r.counts, flush=True)
-    print("ref", rec["status"], rec["iterations"], repr(rec["pobj"]), repr(rec["dobj"]), rec["dimacs"], flush=True)
-    assert r.status == rec["status"] == 0, r.status_str
-    assert objective_ok(r.pobj, rec, "pobj")
-    assert objective_ok(r.dobj, rec, "dobj")
-    assert np.max(np.abs(r.dimacs)) <= TOL_DIMACS
+def check(r, rec, label=""):
+    return check_vs_reference(r, rec, label)
 
 
 GEN = sorted(k for k in SMALL if "generator" in SMALL[k])
@@ -59,19 +39,19 @@ def test_file_vs_reference(name, prec):
     print("case", name, prec, flush=True)
     r = brisk_amd.solve_file(os.path.join(ROOT, "tests", "golden", rec["file"]), options=["-prec", prec])
     check(r, rec)
-    # y agrees with the reference's where the dual optimum is unique (checked loosely)
     assert r.blocksizes == [int(b) for b in golden_instance(rec).bs]
 
 
+@pytest.mark.parametrize("prec", ["mixed", "double"])
 @pytest.mark.parametrize("name", sorted(LARGE))
-def test_full_size_vs_reference(name):
+def test_full_size_vs_reference(name, prec):
     """The full-size workloads (max-cut n = 2000, 64 blocks + LP with m = 5000, dense
-    n = 400 with m = 2000) in the default precision for their size (mixed for m >= 1000)."""
+    n = 400 with m = 2000) in mixed precision (the default for m >= 1000) and in FP64."""
     rec = LARGE[name]
-    print("case", name, flush=True)
+    print("case", name, prec, flush=True)
     a = golden_instance(rec).arrays()
-    r = brisk_amd.solve_sdpa(a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"])
-    check(r, rec)
+    r = brisk_amd.solve_sdpa(a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"], options=["-prec", prec])
+    check(r, rec, prec)
 
 
 @pytest.mark.parametrize("prec", ["double", "mixed"])

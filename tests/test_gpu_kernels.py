@@ -1,0 +1,189 @@
+"""Kernel-level parity of the hot path on the GPU, through the probe entry points of
+libbrisk_amd.so (include/brisk_probe.h), on random points of real problems: the Schur
+complement assembly of every route against the oracle's (pinned by brute force in
+test_oracle.py), the constraint operators A(D) and A*(y) against dense numpy, the Lanczos bound of
+the step-length test against eigenvalues, and the DIMACS errors (including the cone errors of
+non-PSD points) against the oracle's, whose cone errors are eigenvalues."""
+import ctypes
+import numpy as np
+import pytest
+import brisk_amd
+from _ctypes_util import Instance, ArrInstance, oracle_lib, oracle_data, oracle_dimacs, oracle_solve, PD
+from _dense import dense_constraints, random_spd, random_sym, pack, unpack
+
+pytestmark = pytest.mark.gpu
+
+
+class _BP(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int), ("nblk", ctypes.c_int), ("bs", ctypes.c_void_p), ("c", ctypes.c_void_p),
+                ("nnz", ctypes.c_int64), ("mat", ctypes.c_void_p), ("blk", ctypes.c_void_p), ("i", ctypes.c_void_p),
+                ("j", ctypes.c_void_p), ("v", ctypes.c_void_p)]
+
+
+def _lib():
+    L = brisk_amd.load_library()
+    P = ctypes.c_void_p
+    L.brisk_probe_schur.argtypes = [P, PD, PD, ctypes.c_int, PD]
+    L.brisk_probe_aop.argtypes = [P, PD, PD]
+    L.brisk_probe_aty.argtypes = [P, PD, PD]
+    L.brisk_probe_lammin.argtypes = [P, PD, PD, ctypes.c_int, PD]
+    L.brisk_probe_dimacs.argtypes = [P, PD, PD, PD, PD]
+    return L
+
+
+def bp(inst):
+    s = inst.s
+    return _BP(s.m, s.nblk, ctypes.cast(s.bs, ctypes.c_void_p), ctypes.cast(s.c, ctypes.c_void_p), s.nnz,
+               ctypes.cast(s.mat, ctypes.c_void_p), ctypes.cast(s.blk, ctypes.c_void_p), ctypes.cast(s.i, ctypes.c_void_p),
+               ctypes.cast(s.j, ctypes.c_void_p), ctypes.cast(s.v, ctypes.c_void_p))
+
+
+def arr(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return x, x.ctypes.data_as(PD)
+
+
+# every assembly route: single-entry gathers with a heavy identity row (theta), the Hadamard form
+# (max-cut), blocks + LP with heavy groups (multi), the dense route (dense)
+SCHUR_CASES = [("theta", (30, 0.3, 3)), ("theta", (60, 0.15, 1)), ("maxcut", (80, 0.1, 0)),
+               ("multi", (4, 10, 20, 60, 0)), ("multi", (6, 30, 40, 300, 2)), ("multi", (3, 12, 0, 40, 5)),
+               ("dense", (12, 30, 0)), ("dense", (40, 120, 1))]
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+@pytest.mark.parametrize("case", SCHUR_CASES, ids=["%s%r" % c for c in SCHUR_CASES])
+def test_schur_vs_oracle(case, prec):
+    """M_ij = <A_i, X A_j Z^-1> from the GPU's assembly route against the oracle's assembly at
+    the same random SPD X, Z (Z^-1 by numpy): FP64 to 1e-11 of max |M|; the dense route's FP32
+    products (prec 0, the mixed path) to 2e-5."""
+    if prec == 0 and case[0] != "dense":
+        pytest.skip("the FP32 assembly is the dense route's only")
+    inst = Instance(case[0], *case[1])
+    print("case", case, "prec", prec, "m", inst.m, flush=True)
+    A, C, bs, lp, norms = dense_constraints(inst)
+    rng = np.random.default_rng(11)
+    X = [random_spd(n, rng, l) for n, l in zip(bs, lp)]
+    Z = [random_spd(n, rng, l) for n, l in zip(bs, lp)]
+    Zi = [np.linalg.inv(B) for B in Z]
+    m = inst.m
+    xp, xpp = arr(pack(X, lp))
+    zp, zpp = arr(pack(Z, lp))
+    zip_, zipp = arr(pack(Zi, lp))
+    Mo = np.zeros(m * m)
+    assert oracle_lib().oracle_schur(ctypes.byref(oracle_data(inst)), xpp, zipp, Mo.ctypes.data_as(PD)) == 0
+    Mo = Mo.reshape(m, m, order="F") * np.outer(norms, norms)     # the oracle's M is of the equilibrated A_i
+    Mg = np.zeros(m * m)
+    rc = _lib().brisk_probe_schur(ctypes.byref(bp(inst)), xpp, zpp, prec, Mg.ctypes.data_as(PD))
+    assert rc == 0
+    Mg = Mg.reshape(m, m, order="F")
+    err = np.abs(Mg - Mo).max() / np.abs(Mo).max()
+    print("max |M_gpu - M_oracle| / max |M| = %.2e" % err, flush=True)
+    assert err <= (1e-11 if prec else 2e-5)
+    assert np.allclose(Mg, Mg.T, rtol=0, atol=(1e-12 if prec else 1e-5) * np.abs(Mo).max())
+
+
+OP_CASES = SCHUR_CASES + [("maxcut_cycle", (7,))]
+
+
+@pytest.mark.parametrize("case", OP_CASES, ids=["%s%r" % c for c in OP_CASES])
+def test_operators_vs_dense(case):
+    """A(D)_i = <A_i, D> and A*(y) = sum y_i A_i at random symmetric D and y, against dense
+    numpy on the problem as given (1e-13 relative)."""
+    inst = Instance(case[0], *case[1])
+    A, C, bs, lp, _ = dense_constraints(inst, scaled=False)
+    rng = np.random.default_rng(5)
+    D = [random_sym(n, rng, l) for n, l in zip(bs, lp)]
+    y = rng.standard_normal(inst.m)
+    want_aop = np.array([sum(float(np.sum(Ai[k] * D[k])) for k in range(len(bs))) for Ai in A])
+    want_aty = [sum(y[i] * A[i][k] for i in range(inst.m)) for k in range(len(bs))]
+    L = _lib()
+    dp, dpp = arr(pack(D, lp))
+    got = np.zeros(inst.m)
+    assert L.brisk_probe_aop(ctypes.byref(bp(inst)), dpp, got.ctypes.data_as(PD)) == 0
+    sc = np.abs(want_aop).max() + 1e-300
+    print("A(D): max err %.2e of %.2e" % (np.abs(got - want_aop).max(), sc), flush=True)
+    assert np.abs(got - want_aop).max() <= 1e-13 * sc * max(1, max(bs))
+    yp, ypp = arr(y)
+    out = np.zeros(len(dp))
+    assert L.brisk_probe_aty(ctypes.byref(bp(inst)), ypp, out.ctypes.data_as(PD)) == 0
+    got_b = unpack(out, bs, lp)
+    sc = max(np.abs(B).max() for B in want_aty) + 1e-300
+    for g, w in zip(got_b, want_aty):
+        assert np.abs(g - w).max() <= 1e-13 * sc * max(1, inst.m ** 0.5)
+
+
+def one_block(n):
+    """A problem with one SDP block of order n and one constraint (tr X = 1): the step-length
+    probe takes its block structure from it."""
+    return ArrInstance(dict(m=1, nblk=1, bs=np.array([n]), c=np.array([1.0]), mat=np.ones(n, np.int32),
+                            blk=np.ones(n, np.int32), i=np.arange(1, n + 1, dtype=np.int32),
+                            j=np.arange(1, n + 1, dtype=np.int32), v=np.ones(n)))
+
+
+@pytest.mark.parametrize("n", [5, 17, 30, 64, 200, 256, 300, 700])
+def test_lanczos_bound_vs_eigenvalues(n):
+    """The Lanczos bound theta - res of the step-length test (blocks of order <= 256: the fused
+    one-workgroup kernel; above: the per-step launches) against numpy's eigenvalues: never above
+    lambda_min (beyond rounding), and within the stopping rule's Ritz accuracy 0.02 |theta| of
+    it (reference solver.c:6041, 6221). Mode 0 tests lambda_min(X), mode 1 lambda_min(L^-1 dX L^-T)."""
+    rng = np.random.default_rng(n)
+    inst = one_block(n)
+    L = _lib()
+    for trial in range(3):
+        G = rng.standard_normal((n, n))
+        X = G @ G.T / n + (0.05 + trial) * np.eye(n)
+        dX = random_sym(n, rng, False) / np.sqrt(n)
+        for mode in (0, 1):
+            if mode == 0:
+                lam = np.linalg.eigvalsh(X)[0]
+            else:
+                Lc = np.linalg.cholesky(X)
+                Li = np.linalg.inv(Lc)
+                lam = np.linalg.eigvalsh(Li @ dX @ Li.T)[0]
+            xp, xpp = arr(X.reshape(-1, order="F"))
+            dp, dpp = arr(dX.reshape(-1, order="F"))
+            out = ctypes.c_double()
+            assert L.brisk_probe_lammin(ctypes.byref(bp(inst)), xpp, dpp, mode, ctypes.byref(out)) == 0
+            got = out.value
+            print("n %d trial %d mode %d: bound %.12g lambda_min %.12g" % (n, trial, mode, got, lam), flush=True)
+            assert got <= lam + 1e-10 * (1 + abs(lam))
+            assert got >= lam - 0.021 * abs(lam) - 1e-10
+
+
+DIMACS_CASES = [("theta", (30, 0.3, 3)), ("multi", (4, 10, 20, 60, 0)), ("maxcut", (60, 0.1, 2)), ("dense", (12, 30, 0))]
+
+
+@pytest.mark.parametrize("shift", [0.0, 1e-9, 1e-7, 1e-5])
+@pytest.mark.parametrize("case", DIMACS_CASES, ids=["%s%r" % c for c in DIMACS_CASES])
+def test_dimacs_vs_oracle(case, shift):
+    """The six DIMACS errors of a point (X, y) as the GPU measures a returned point, against the
+    oracle's (eigenvalues for the cone errors): the oracle's optimal point with X shifted by
+    -shift I and y moved by shift times a random vector, so that X and Z = C - A'y leave the
+    cone by about shift (inside it, or on its boundary, for shift 0). Errors 1, 3, 5 and 6 and the
+    objectives to 1e-10 relative; the cone errors, from the factorization ladder, no smaller than
+    the eigenvalue's and within 1.1e-4 relative above it."""
+    inst = Instance(case[0], *case[1])
+    A, C, bs, lp, _ = dense_constraints(inst, scaled=False)
+    o = oracle_solve(inst)
+    assert o["status"] == 0
+    X = unpack(o["X"], bs, lp)
+    X = [B - shift * np.eye(B.shape[0]) for B in X]
+    rng = np.random.default_rng(3)
+    y = o["y"] + shift * rng.standard_normal(inst.m)
+    xv = pack(X, lp)
+    want, po, do = oracle_dimacs(inst, xv, y)
+    xp, xpp = arr(xv)
+    yp, ypp = arr(y)
+    err = np.zeros(6)
+    obj = np.zeros(2)
+    assert _lib().brisk_probe_dimacs(ctypes.byref(bp(inst)), xpp, ypp, err.ctypes.data_as(PD), obj.ctypes.data_as(PD)) == 0
+    print("gpu", err, obj, flush=True)
+    print("oracle", want, po, do, flush=True)
+    assert obj[0] == pytest.approx(po, rel=1e-10, abs=1e-13) and obj[1] == pytest.approx(do, rel=1e-10, abs=1e-13)
+    for e in (0, 2, 4, 5):
+        assert err[e] == pytest.approx(want[e], rel=1e-8, abs=1e-14)
+    for e in (1, 3):
+        # below the ladder's first rung (1e-12 of the norms) a cone error is resolution, not a value
+        floor = 2e-12
+        assert err[e] >= want[e] * (1 - 1e-9) - floor
+        assert err[e] <= want[e] * (1 + 1.1e-4) + floor

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 import brisk_amd
 from _ctypes_util import Instance, golden, golden_instance
+from _parity import assert_identical, check_vs_reference
 
 pytestmark = pytest.mark.gpu
 
@@ -105,3 +106,67 @@ def test_mixed_precision_counts():
     assert r.status == 0
     f32, f64, pcg = r.counts
     assert f32 > 0 and f64 > 0 and f32 + f64 >= r.iterations
+
+
+STREAM = golden("stream")
+
+
+def test_serving_mixed_stream_equals_alone():
+    """The serving stream's requests in mixed precision (its stated precision): 27 requests of the
+    stream (requests 0-25 and 31: theta and max-cut with m from 32 to 4985, three theta requests
+    with m > 1500, so that f32 factors, their refinement and the 256-column panels of the largest
+    are used), arriving three every second step into 8 slots and a 1.5 GB heap, on which the
+    largest must wait for memory. Every served result equals the same request solved alone, bit
+    for bit (status, iterations, objectives, DIMACS errors, y, X, Z), and matches the reference
+    library's solve of that request (tests/golden/stream.json)."""
+    names = sorted(STREAM, key=lambda k: STREAM[k]["request"])
+    tups = {k: tup(golden_instance(STREAM[k])) for k in names}
+    opts = ["-prec", "mixed"]
+    solo = {}
+    for k in names:
+        print("alone", k, "m =", STREAM[k]["m"], flush=True)
+        solo[k] = brisk_amd.solve_sdpa(*tups[k], options=opts)
+    sv = brisk_amd.Server(opts + ["-slots", "8", "-heapgb", "1.5"])
+    ids, pending = {}, list(names)
+    steps, left, max_inflight, waited = 0, 1, 0, False
+    while pending or left > 0:
+        if pending and steps % 2 == 0:
+            for k in pending[:3]:
+                ids[k] = sv.submit(*tups[k])
+            pending = pending[3:]
+        left = sv.step()
+        fly = sv.inflight()
+        max_inflight = max(max_inflight, fly)
+        waited |= left - fly > 0 and fly < 8          # a request waits with a slot free: memory
+        steps += 1
+        assert steps < 5000
+    print("steps", steps, "max in flight", max_inflight, "waited on memory", waited, flush=True)
+    assert max_inflight > 3 and waited
+    assert sorted(sv.done()) == sorted(ids.values())
+    for k in names:
+        r, lat = sv.take(ids[k])
+        print(k, r.status_str, r.iterations, r.counts, repr(r.pobj), "latency %.3f s" % lat, flush=True)
+        assert_identical(r, solo[k])
+        check_vs_reference(r, STREAM[k], k)
+    sv.close()
+
+
+def test_serving_setup_failure_ends_unsolved(monkeypatch):
+    """A request whose device setup fails (forced from the host with BRISK_FAULT=setup, the
+    reference's fault-injection hook) with nothing in flight ends unsolved with its cause, and the
+    loop drains instead of reporting it as waiting forever; without the fault the next request
+    is solved."""
+    sv = brisk_amd.Server(["-prec", "double", "-slots", "2", "-heapgb", "0.5"])
+    monkeypatch.setenv("BRISK_FAULT", "setup")
+    bad = sv.submit(*tup(Instance("theta", 30, 0.3, 3)))
+    assert sv.step() == 0
+    r, _ = sv.take(bad)
+    assert r.status == -1 and r.exit_code == -1 and "setup" in r.cause
+    monkeypatch.delenv("BRISK_FAULT")
+    good = sv.submit(*tup(Instance("theta", 30, 0.3, 3)))
+    for _ in range(200):
+        if sv.step() == 0:
+            break
+    r, _ = sv.take(good)
+    assert r.status == 0
+    sv.close()

@@ -29,10 +29,10 @@ def lib():
     return L
 
 
-@pytest.mark.parametrize("header,so", [("brisk_amd.h", LIB), ("brisk_dev.h", LIB), ("brisk_gen.h", GEN)])
+@pytest.mark.parametrize("header,so", [("brisk_amd.h", LIB), ("brisk_dev.h", LIB), ("brisk_probe.h", LIB), ("brisk_gen.h", GEN)])
 def test_exports(header, so):
     names = declared(header)
-    assert len(names) > 8
+    assert len(names) >= 5
     L = ctypes.CDLL(so)
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
@@ -213,3 +213,27 @@ def test_binding_requires_library(tmp_path):
     import brisk_amd
     with pytest.raises(RuntimeError):
         brisk_amd.load_library(str(tmp_path / "nolib.so"))
+
+
+def test_acceptance_rules_nan(tmp_path):
+    """The acceptance error, status rule and termination score of the host solver (solver.h) on
+    a C driver: a NaN error never passes (reference brisk_nmax, brisk.h:182-189; status from the
+    measured errors, main.c:2349-2359)."""
+    import subprocess
+    exe = str(tmp_path / "host_rules")
+    subprocess.run(["gcc", "-O1", "-std=gnu11", "-I" + os.path.join(ROOT, "brisk_amd", "csrc"), "-I" + INC,
+                    os.path.join(ROOT, "tests", "host_rules.c"), "-o", exe, "-lm"], check=True)
+    out = dict(line.split() for line in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.splitlines())
+    inf = float("inf")
+    assert float(out["nmax_nan_first"]) == inf and float(out["nmax_nan_second"]) == inf
+    assert float(out["nmax_plain"]) == 3.0
+    assert float(out["nmin_nan"]) == -inf and float(out["nmin_plain"]) == 2.0
+    assert float(out["acc_plain"]) == 3e-9 and int(out["status_plain"]) == 0
+    assert float(out["acc_nan_cone"]) == inf
+    # an optimal claim with a NaN cone or complementarity error ends in numerical difficulties
+    assert int(out["status_nan_cone"]) == 4 and int(out["status_nan_compl"]) == 4
+    assert int(out["status_nan_err3"]) == 0
+    assert int(out["status_reduced"]) == 5 and int(out["status_limit_reduced"]) == 5
+    assert int(out["status_numeric"]) == 4 and int(out["status_limit_kept"]) == 3
+    assert int(out["status_infeasible_kept"]) == 1
+    assert float(out["score_nan"]) == inf and float(out["score_plain"]) == 3e-9

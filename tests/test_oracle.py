@@ -6,6 +6,7 @@ import math
 import numpy as np
 import pytest
 
+from _dense import dense_constraints, random_spd, pack  # noqa: F401
 from _ctypes_util import (Instance, ArrInstance, golden, golden_instance, oracle_solve, oracle_dimacs,
                           oracle_lib, oracle_data, PD, D)
 import ctypes
@@ -127,36 +128,6 @@ def test_oracle_sdplib_values(name):
 
 # ---- building blocks from their definitions ---------------------------------------------------
 
-def dense_constraints(inst):
-    """A_i (i = 1..m) and C = -F0 as lists of dense blocks (LP blocks as diagonal matrices),
-    each A_i scaled by 1/||A_i||_F (row equilibration)."""
-    a = inst.arrays()
-    m, bs = a["m"], [abs(int(b)) for b in a["bs"]]
-    A = [[np.zeros((n, n)) for n in bs] for _ in range(m + 1)]
-    for t, k, i, j, v in zip(a["mat"], a["blk"], a["i"], a["j"], a["v"]):
-        B = A[t][k - 1]
-        B[i - 1, j - 1] += v
-        if i != j:
-            B[j - 1, i - 1] += v
-    C = [-B for B in A[0]]
-    out = []
-    for t in range(1, m + 1):
-        nrm = math.sqrt(sum(float(np.sum(B * B)) for B in A[t]))
-        out.append([B / nrm if nrm > 0 else B for B in A[t]])
-    return out, C, bs, [int(b) < 0 for b in a["bs"]]
-
-
-def random_spd(n, rng, lp):
-    if lp:
-        return np.diag(rng.uniform(0.5, 2.0, n))
-    G = rng.standard_normal((n, n))
-    return G @ G.T / n + 0.5 * np.eye(n)
-
-
-def pack(blocks, lp):
-    return np.concatenate([np.diag(B) if l else B.reshape(-1, order="F") for B, l in zip(blocks, lp)])
-
-
 SCHUR_CASES = [("multi", (2, 6, 5, 12, 4)), ("multi", (3, 5, 0, 9, 1)), ("dense", (6, 10, 2)),
                ("theta", (9, 0.4, 1)), ("maxcut", (10, 0.3, 2))]
 
@@ -166,7 +137,7 @@ def test_oracle_schur_brute_force(case):
     """M_ab = <A_a, X A_b Zi> summed over blocks, from dense matrices (the HKM Schur complement),
     against the oracle's assembly (route-0 expansion, Hadamard form or dense formula)."""
     inst = Instance(case[0], *case[1])
-    A, C, bs, lp = dense_constraints(inst)
+    A, C, bs, lp, _ = dense_constraints(inst)
     rng = np.random.default_rng(7)
     X = [random_spd(n, rng, l) for n, l in zip(bs, lp)]
     Zi = [random_spd(n, rng, l) for n, l in zip(bs, lp)]
