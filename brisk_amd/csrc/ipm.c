@@ -514,7 +514,9 @@ static int lanczos_min_(Solve **S, int ns, const int *act, int nside, const size
             memset(&f, 0, sizeof(f));
             f.n = z->n; f.K = z->K; f.mode = fLi[sd] ? 1 : 0;
             f.Li = fLi[sd] ? F_(s, fLi[sd]) + bo : 0; f.LiT = fLiT[sd] ? F_(s, fLiT[sd]) + bo : 0; f.V = F_(s, fV[sd]) + bo;
-            f.out = z->ab; f.tol = lz_tol;
+            /* blocks of order <= 32 run every step (K = n up to 30): the reference takes the
+             * exact eigenvalue there (small_lammin, solver.c:6085-6089), and n steps give it */
+            f.out = z->ab; f.tol = z->n <= 32 ? 0.0 : lz_tol;
             fz[nf++] = f;
             fused[c] = 1; live[c] = 0;
         } else {

@@ -721,7 +721,12 @@ __device__ void lz_ritz(const double *al, const double *be, int kk, int K, int n
             const double x = lo + w_ * (double)(lane + 1) / 65.0;
             const int c = sturm_count(al, be, m, x);
             const unsigned long long bal = __ballot(c >= 1);
-            const int f = bal ? __ffsll((long long)bal) - 1 : 63;
+            if (!bal) {
+                // lambda_min lies above the 64th point: in [lo + 64 w / 65, hi]
+                lo = lo + w_ * 64.0 / 65.0;
+                continue;
+            }
+            const int f = __ffsll((long long)bal) - 1;
             const double nlo = f > 0 ? lo + w_ * (double)f / 65.0 : lo;
             hi = lo + w_ * (double)(f + 1) / 65.0;
             lo = nlo;

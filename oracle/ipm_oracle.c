@@ -16,7 +16,9 @@
  *   predictor          solver.c:14787-14818, sigma/gamma solver.c:14829-14872
  *   corrector          solver.c:14997-15055, refinement solver.c:15058-15105
  *   step lengths       solver.c:5947-6027 (tridiagonalisation + Sturm bisection, used here for
- *                      every block size: the exact lambda_min the Lanczos bound approximates)
+ *                      every block size: the exact lambda_min the Lanczos bound approximates);
+ *                      the Lanczos bound itself, solver.c:6029-6225 (oracle_lanczos,
+ *                      oracle_step_lambda), for the checks of the GPU's bound
  *   catastrophe guard  solver.c:15143-15160, try_step solver.c:6393-6446, update 15185-15188
  *   DIMACS errors      postsolve.c:3037-3112, status rule main.c:2349-2359, brisk.h:187-189
  *
@@ -377,6 +379,137 @@ static double sc_solve(const OSchur *S, const double *rhs, double *x, double tol
     const double rel = sqrt(dot(m, r, r)) / nb;
     free(r);
     return rel;
+}
+
+/* ---- the Lanczos bound of the step-length test (reference sdp_maxstep_lam, solver.c:6083-6225,
+ * and ritz_bound, solver.c:6029-6038), restated for the kernel-level checks of the GPU's bound;
+ * the oracle's own iteration uses the exact lambda_min (above) ---- */
+/* the smallest eigenvalue theta of the k x k tridiagonal (a, b) by Sturm bisection, and the last
+ * entry of its unit eigenvector by inverse iteration (dense LU with partial pivoting, k <= 64) */
+static void tri_smallest(int k, const double *a, const double *b, double *theta, double *zlast) {
+    double lo = INFINITY, hi = -INFINITY, sc = 0;
+    for (int i = 0; i < k; i++) {
+        const double rad = (i > 0 ? fabs(b[i - 1]) : 0) + (i + 1 < k ? fabs(b[i]) : 0);
+        lo = fmin(lo, a[i] - rad); hi = fmax(hi, a[i] + rad); sc = fmax(sc, fabs(a[i]) + rad);
+    }
+    for (int it = 0; it < 300 && hi - lo > 1e-15 * sc; it++) {
+        const double x = 0.5 * (lo + hi);
+        int cnt = 0;
+        double q = a[0] - x;
+        if (q < 0) cnt++;
+        for (int i = 1; i < k; i++) {
+            if (q == 0) q = 1e-300;
+            q = a[i] - x - b[i - 1] * b[i - 1] / q;
+            if (q < 0) cnt++;
+        }
+        if (cnt >= 1) hi = x; else lo = x;
+    }
+    *theta = 0.5 * (lo + hi);
+    double M[64 * 64], z[64];
+    for (int i = 0; i < k; i++) z[i] = 1.0;
+    for (int rep = 0; rep < 3; rep++) {
+        const double sh = *theta - 1e-12 * (sc + 1e-300);
+        memset(M, 0, sizeof(double) * 64 * 64);
+        for (int i = 0; i < k; i++) {
+            M[i + 64 * i] = a[i] - sh;
+            if (i + 1 < k) M[i + 64 * (i + 1)] = M[(i + 1) + 64 * i] = b[i];
+        }
+        for (int c = 0; c < k; c++) {            /* LU with partial pivoting, applied to z */
+            int pr = c;
+            for (int r = c + 1; r < k; r++) if (fabs(M[r + 64 * c]) > fabs(M[pr + 64 * c])) pr = r;
+            if (pr != c) {
+                for (int j = 0; j < k; j++) { const double t = M[c + 64 * j]; M[c + 64 * j] = M[pr + 64 * j]; M[pr + 64 * j] = t; }
+                const double t = z[c]; z[c] = z[pr]; z[pr] = t;
+            }
+            if (M[c + 64 * c] == 0) M[c + 64 * c] = 1e-300;
+            for (int r = c + 1; r < k; r++) {
+                const double f = M[r + 64 * c] / M[c + 64 * c];
+                for (int j = c; j < k; j++) M[r + 64 * j] -= f * M[c + 64 * j];
+                z[r] -= f * z[c];
+            }
+        }
+        for (int r = k - 1; r >= 0; r--) {
+            double t = z[r];
+            for (int j = r + 1; j < k; j++) t -= M[r + 64 * j] * z[j];
+            z[r] = t / M[r + 64 * r];
+        }
+        double nz = 0;
+        for (int i = 0; i < k; i++) nz += z[i] * z[i];
+        nz = 1.0 / sqrt(nz);
+        for (int i = 0; i < k; i++) z[i] *= nz;
+    }
+    *zlast = z[k - 1];
+}
+/* Lanczos on T = L^{-1} D L^{-T} (L lower, n x n; T = D when L is NULL) from the reference's
+ * fixed start 1 + 0.5 sin(1.7 i + 0.3): at most K steps (K <= 64); each w = T q_j, alpha_j = q_j'w,
+ * two Gram-Schmidt passes over q_0..q_j, beta_j = |w|; a breakdown when beta_j <= 1e-12 max|alpha|
+ * (beta_j = 0); from the 6th step, every 3rd, the Ritz test res <= tol |theta| + 1e-14 max|alpha|
+ * stops it. Returns theta - res for the final tridiagonal (res = |beta_last z_last|), or -1e300
+ * for coefficients that are not finite; info3 = { steps, theta, res } when given. */
+double oracle_lanczos(int n, const double *L, const double *D, int K, double tol, double *info3) {
+    if (K > n) K = n;
+    if (K > 64) K = 64;
+    double *Q = malloc(sizeof(double) * (size_t)n * (K + 1)), *w = malloc(sizeof(double) * n), *v = malloc(sizeof(double) * n);
+    double al[64], be[64], theta = 0, zl = 0, res = 0, scale = 0;
+    double nrm = 0;
+    for (int i = 0; i < n; i++) { Q[i] = 1.0 + 0.5 * sin(1.7 * i + 0.3); nrm += Q[i] * Q[i]; }
+    nrm = 1.0 / sqrt(nrm);
+    for (int i = 0; i < n; i++) Q[i] *= nrm;
+    int kk = 0;
+    for (int j = 0; j < K; j++) {
+        const double *qj = Q + (size_t)j * n;
+        memcpy(v, qj, sizeof(double) * n);
+        if (L) ltsolve(n, L, v);
+        for (int i = 0; i < n; i++) { double t = 0; for (int c = 0; c < n; c++) t += D[i + (size_t)c * n] * v[c]; w[i] = t; }
+        if (L) lsolve(n, L, w);
+        const double a = dot(n, qj, w);
+        al[j] = a;
+        for (int pass = 0; pass < 2; pass++)
+            for (int i = 0; i <= j; i++) {
+                const double *qi = Q + (size_t)i * n;
+                const double c = dot(n, qi, w);
+                for (int r = 0; r < n; r++) w[r] -= c * qi[r];
+            }
+        const double b = sqrt(dot(n, w, w));
+        be[j] = b;
+        kk = j + 1;
+        if (fabs(a) > scale) scale = fabs(a);
+        if (b <= 1e-12 * (scale + 1e-300)) { be[j] = 0; break; }
+        if (kk >= 6 && kk % 3 == 0) {
+            tri_smallest(kk, al, be, &theta, &zl);
+            res = fabs(be[kk - 1] * zl);
+            if (res <= tol * fabs(theta) + 1e-14 * scale) break;
+        }
+        if (j + 1 < K) { double *qn = Q + (size_t)(j + 1) * n; for (int i = 0; i < n; i++) qn[i] = w[i] / b; }
+    }
+    int bad = 0;
+    for (int t = 0; t < kk; t++) if (!isfinite(al[t]) || !isfinite(be[t])) bad = 1;
+    double out = -1e300;
+    if (!bad) {
+        tri_smallest(kk, al, be, &theta, &zl);
+        res = fabs(be[kk - 1] * zl);
+        out = theta - res;
+    }
+    if (info3) { info3[0] = kk; info3[1] = theta; info3[2] = res; }
+    free(Q); free(w); free(v);
+    return out;
+}
+/* the reference's bound of lambda_min(L^{-1} D L^{-T}) for one block (sdp_maxstep_lam,
+ * solver.c:6083-6090): the exact value for blocks of order <= 32 (small_lammin), else the
+ * 30-step Lanczos bound with Ritz accuracy 0.02 */
+double oracle_step_lambda(int n, const double *L, const double *D) {
+    if (n > 32) return oracle_lanczos(n, L, D, 30, 0.02, NULL);
+    double *T = malloc(sizeof(double) * (size_t)n * n);
+    memcpy(T, D, sizeof(double) * (size_t)n * n);
+    if (L) {
+        for (int j = 0; j < n; j++) lsolve(n, L, T + (size_t)j * n);
+        for (int j = 0; j < n; j++) for (int i = j + 1; i < n; i++) { const double t = T[i + (size_t)j * n]; T[i + (size_t)j * n] = T[j + (size_t)i * n]; T[j + (size_t)i * n] = t; }
+        for (int j = 0; j < n; j++) lsolve(n, L, T + (size_t)j * n);
+        symmetrize(n, T);
+    }
+    const double lam = oracle_lammin(n, T);
+    free(T);
+    return lam;
 }
 
 /* ---- step lengths (solver.c:6300-6388 with the exact lambda_min) ---- */

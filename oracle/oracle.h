@@ -29,4 +29,6 @@ void oracle_result_free(OracleResult *r);
 int oracle_dimacs(const OracleData *d, const double *X, const double *y, double *err, double *pobj, double *dobj);
 int oracle_schur(const OracleData *d, const double *X, const double *Zi, double *M);
 double oracle_lammin(int n, double *B);
+double oracle_lanczos(int n, const double *L, const double *D, int K, double tol, double *info3);
+double oracle_step_lambda(int n, const double *L, const double *D);
 #endif

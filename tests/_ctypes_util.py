@@ -112,6 +112,10 @@ def oracle_lib():
         _orc.oracle_schur.argtypes = [pO, PD, PD, PD]
         _orc.oracle_lammin.argtypes = [I, PD]
         _orc.oracle_lammin.restype = D
+        _orc.oracle_lanczos.argtypes = [I, PD, PD, I, D, PD]
+        _orc.oracle_lanczos.restype = D
+        _orc.oracle_step_lambda.argtypes = [I, PD, PD]
+        _orc.oracle_step_lambda.restype = D
     return _orc
 
 
@@ -227,3 +231,23 @@ def golden_instance(rec):
     if "file" in rec:
         return ArrInstance(read_sdpa(os.path.join(ROOT, "tests", "golden", rec["file"])))
     return Instance(rec["generator"], *rec["args"])
+
+
+def oracle_lanczos(D_, L=None, K=30, tol=0.02):
+    """The oracle's restatement of the reference's Lanczos bound on L^{-1} D L^{-T} (or D):
+    (bound, steps, theta, res)."""
+    n = D_.shape[0]
+    Df = np.asfortranarray(D_, dtype=np.float64)
+    Lf = None if L is None else np.asfortranarray(L, dtype=np.float64)
+    info = np.zeros(3)
+    b = oracle_lib().oracle_lanczos(n, None if Lf is None else Lf.ctypes.data_as(PD), Df.ctypes.data_as(PD), K, tol,
+                                    info.ctypes.data_as(PD))
+    return b, int(info[0]), info[1], info[2]
+
+
+def oracle_step_lambda(D_, L=None):
+    """The reference's step-length bound of one block: exact for order <= 32, else the Lanczos bound."""
+    n = D_.shape[0]
+    Df = np.asfortranarray(D_, dtype=np.float64)
+    Lf = None if L is None else np.asfortranarray(L, dtype=np.float64)
+    return oracle_lib().oracle_step_lambda(n, None if Lf is None else Lf.ctypes.data_as(PD), Df.ctypes.data_as(PD))

@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from _dense import dense_constraints, random_spd, pack  # noqa: F401
+from _ctypes_util import oracle_lanczos, oracle_step_lambda  # noqa: E402
 from _ctypes_util import (Instance, ArrInstance, golden, golden_instance, oracle_solve, oracle_dimacs,
                           oracle_lib, oracle_data, PD, D)
 import ctypes
@@ -219,3 +220,79 @@ def test_oracle_hsd_vs_reference(name):
     assert objective_ok(o["pobj"], rec, "pobj"), (o["pobj"], rec["pobj"], rec["dd"]["pobj"])
     assert objective_ok(o["dobj"], rec, "dobj"), (o["dobj"], rec["dobj"], rec["dd"]["dobj"])
     assert max(abs(e) for e in o["err"]) <= TOL_DIMACS, o["err"]
+
+
+# ---- the Lanczos bound of the step-length test (reference solver.c:6029-6225) ------------------
+
+def _krylov_ritz(T, k):
+    """Rayleigh-Ritz on the k-dimensional Krylov space of T from the reference's start vector,
+    by a QR of the Krylov matrix (independent of the Lanczos recurrence): the smallest Ritz value
+    and the residual norm of its Ritz vector."""
+    n = T.shape[0]
+    q = 1.0 + 0.5 * np.sin(1.7 * np.arange(n) + 0.3)
+    q /= np.linalg.norm(q)
+    Kr = [q]
+    for _ in range(k - 1):
+        v = T @ Kr[-1]
+        Kr.append(v / np.linalg.norm(v))
+    Q, _ = np.linalg.qr(np.array(Kr).T)
+    w, V = np.linalg.eigh(Q.T @ T @ Q)
+    y = Q @ V[:, 0]
+    return w[0], np.linalg.norm(T @ y - w[0] * y)
+
+
+@pytest.mark.parametrize("with_l", [False, True])
+@pytest.mark.parametrize("n", [40, 90])
+def test_oracle_lanczos_rayleigh_ritz(n, with_l):
+    """After k = 6 steps (no early stop) the restated Lanczos gives the smallest Ritz value of T on
+    the Krylov space span{q, Tq, .., T^5 q} and |beta z_last| = the residual of its Ritz vector;
+    with L, T = L^{-1} D L^{-T}."""
+    rng = np.random.default_rng(n + with_l)
+    G = rng.standard_normal((n, n))
+    D_ = (G + G.T) / 2
+    L = None
+    T = D_
+    if with_l:
+        H = rng.standard_normal((n, n))
+        L = np.linalg.cholesky(H @ H.T / n + np.eye(n))
+        Li = np.linalg.inv(L)
+        T = Li @ D_ @ Li.T
+    b, k, theta, res = oracle_lanczos(D_, L, K=6, tol=0.0)
+    th_rr, res_rr = _krylov_ritz(T, 6)
+    sc = np.abs(T).max()
+    assert k == 6
+    assert abs(theta - th_rr) <= 1e-9 * sc
+    assert abs(res - res_rr) <= 1e-7 * sc
+    assert b == pytest.approx(theta - res, rel=1e-15, abs=1e-300)
+
+
+@pytest.mark.parametrize("n", [3, 12, 30])
+def test_oracle_lanczos_full_space_exact(n):
+    """K = n steps without early stop span the whole space: the bound is lambda_min."""
+    rng = np.random.default_rng(100 + n)
+    G = rng.standard_normal((n, n))
+    D_ = (G + G.T) / 2
+    b, k, theta, res = oracle_lanczos(D_, None, K=n, tol=0.0)
+    lam = np.linalg.eigvalsh(D_)[0]
+    assert abs(b - lam) <= 1e-9 * (1 + abs(lam))
+
+
+@pytest.mark.parametrize("n", [33, 64, 200])
+def test_oracle_lanczos_conservative(n):
+    """With the reference's 30 steps and accuracy 0.02, on shifted Wishart matrices (a clustered
+    lower spectrum, the step test's usual case) the bound theta - res stays below lambda_min,
+    and theta (a Ritz value) is never below it; for order <= 32 the reference's bound is the
+    exact eigenvalue (small_lammin)."""
+    rng = np.random.default_rng(200 + n)
+    for trial in range(4):
+        G = rng.standard_normal((n, n))
+        D_ = G @ G.T / n + (trial - 1.5) * np.eye(n)
+        lam = np.linalg.eigvalsh(D_)[0]
+        b, k, theta, res = oracle_lanczos(D_, None)
+        sc = np.abs(D_).max()
+        assert b <= lam + 1e-12 * sc and theta >= lam - 1e-12 * sc
+        assert oracle_step_lambda(D_) == b
+    for m in (5, 32):
+        G = rng.standard_normal((m, m))
+        D_ = (G + G.T) / 2
+        assert abs(oracle_step_lambda(D_) - np.linalg.eigvalsh(D_)[0]) <= 1e-9 * (1 + np.abs(D_).max())
