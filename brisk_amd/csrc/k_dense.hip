@@ -1034,12 +1034,10 @@ extern "C" int bk_gemv_mixed(int trans, double alpha, double beta, const float *
 // c(j) = sum_r A(64I + r, 64J + j) x(64I + r) to slot (J, I). A diagonal tile reads its lower
 // triangle only and mirrors it.
 #define SYMV_CH 4
-__global__ __launch_bounds__(256) void k_symv_tiles(const SymvItem *__restrict__ items, const int *__restrict__ pre, int nit,
-                                                    const double *__restrict__ A, const double *__restrict__ X, double *__restrict__ P) {
-    const int q = flat_item(pre, nit, blockIdx.x);
-    const SymvItem it = items[q];
+__device__ __forceinline__ void symv_tiles(const SymvItem it, int64_t wg, const double *__restrict__ A, const double *__restrict__ X,
+                                           double *__restrict__ P) {
     const int n = it.n, nT = (n + 63) / 64, tid = threadIdx.x, li = tid & 63, h = tid >> 6;
-    const int64_t ntile = (int64_t)nT * (nT + 1) / 2, t0 = (int64_t)(blockIdx.x - pre[q]) * SYMV_CH;
+    const int64_t ntile = (int64_t)nT * (nT + 1) / 2, t0 = wg * SYMV_CH;
     const int64_t t1 = t0 + SYMV_CH < ntile ? t0 + SYMV_CH : ntile;
     __shared__ double Ts[64][65];    // Ts[c][r] = A(64 I + r, 64 J + c)
     __shared__ double xs[2][64];     // x of the column block J, of the row block I
@@ -1092,13 +1090,16 @@ __global__ __launch_bounds__(256) void k_symv_tiles(const SymvItem *__restrict__
         __syncthreads();
     }
 }
+__global__ __launch_bounds__(256) void k_symv_tiles(const SymvItem *__restrict__ items, const int *__restrict__ pre, int nit,
+                                                    const double *__restrict__ A, const double *__restrict__ X, double *__restrict__ P) {
+    const int q = flat_item(pre, nit, blockIdx.x);
+    symv_tiles(items[q], blockIdx.x - pre[q], A, X, P);
+}
 // y(64 a + i) = alpha sum_b P(a, b)(i) + beta y(64 a + i): one 64-thread workgroup per row block,
 // the slots summed in the order b = 0 .. T - 1
-__global__ __launch_bounds__(64) void k_symv_reduce(const SymvItem *__restrict__ items, const int *__restrict__ pre, int nit,
-                                                    const double *__restrict__ P, double *__restrict__ Y, double alpha, double beta) {
-    const int q = flat_item(pre, nit, blockIdx.x);
-    const SymvItem it = items[q];
-    const int a = blockIdx.x - pre[q], n = it.n, nT = (n + 63) / 64, i = threadIdx.x, row = 64 * a + i;
+__device__ __forceinline__ void symv_reduce(const SymvItem it, int a, const double *__restrict__ P, double *__restrict__ Y, double alpha,
+                                            double beta) {
+    const int n = it.n, nT = (n + 63) / 64, i = threadIdx.x, row = 64 * a + i;
     if (row >= n) return;
     const double *p = P + it.p + (int64_t)a * nT * 64 + i;
     // the slots in order, loaded 8 at a time (index clamped, value masked) so that the loads of a
@@ -1113,6 +1114,11 @@ __global__ __launch_bounds__(64) void k_symv_reduce(const SymvItem *__restrict__
     }
     double *y = Y + it.y + row;
     *y = beta != 0.0 ? alpha * s + beta * *y : alpha * s;
+}
+__global__ __launch_bounds__(64) void k_symv_reduce(const SymvItem *__restrict__ items, const int *__restrict__ pre, int nit,
+                                                    const double *__restrict__ P, double *__restrict__ Y, double alpha, double beta) {
+    const int q = flat_item(pre, nit, blockIdx.x);
+    symv_reduce(items[q], blockIdx.x - pre[q], P, Y, alpha, beta);
 }
 extern "C" int bk_symv_f64(double alpha, double beta, const double *A, const double *x, double *y, double *p, const SymvItem *it, int n_it) {
     if (n_it <= 0) return 0;
@@ -1141,4 +1147,77 @@ extern "C" int bk_symv_f64(double alpha, double beta, const double *A, const dou
         bk_prof_end(tok, 15, fl, by);
     }
     return BK_LAUNCH_CHECK("symv");
+}
+
+// ---- the symmetric product without device allocation (bk_symv_f64_noalloc) --------------------
+// For callers that cannot allocate device memory (code under graph capture, tight serving loops):
+// the same two kernels as bk_symv_f64, with the partial-sum slots in the file-scope array
+// `scratch` instead of a caller workspace, and the item descriptors and workgroup prefixes
+// passed by value as kernel arguments instead of staged device copies, so that a call makes no
+// allocation and no copy. Every GPU the library runs on holds its own copy of the array until
+// the process ends; the slots a call uses are all written by its first kernel before its second
+// reads them, so nothing needs clearing, but calls that overlap on one GPU race on the array.
+#define SYMV_NA_SCRATCH (4 << 20)                  // doubles: 32 MiB of partial-sum slots
+#define SYMV_NA_ITEMS 16                           // items per call (their descriptors are kernel arguments)
+static __device__ __align__(256) double scratch[SYMV_NA_SCRATCH];
+struct SymvBatch {
+    SymvItem it[SYMV_NA_ITEMS];
+    int pre[SYMV_NA_ITEMS + 1];                    // workgroups of item q: [pre[q], pre[q + 1])
+    int nit;
+};
+__global__ __launch_bounds__(256) void k_symv_tiles_na(const SymvBatch b, const double *__restrict__ A, const double *__restrict__ X,
+                                                       double *__restrict__ P) {
+    int q = 0;
+    while (q + 1 < b.nit && b.pre[q + 1] <= (int)blockIdx.x) q++;
+    symv_tiles(b.it[q], blockIdx.x - b.pre[q], A, X, P);
+}
+__global__ __launch_bounds__(64) void k_symv_reduce_na(const SymvBatch b, const double *__restrict__ P, double *__restrict__ Y, double alpha,
+                                                       double beta) {
+    int q = 0;
+    while (q + 1 < b.nit && b.pre[q + 1] <= (int)blockIdx.x) q++;
+    symv_reduce(b.it[q], blockIdx.x - b.pre[q], P, Y, alpha, beta);
+}
+// the device address of `scratch` for each device the library has run on (the array's name in
+// host code is a host-side stand-in, never a device address)
+static double *g_symv_scratch[64];
+extern "C" int bk_symv_scratch_init(void) {
+    int dev = 0;
+    if (bk_check(hipGetDevice(&dev), "hipGetDevice") || dev < 0 || dev >= 64) return -1;
+    if (g_symv_scratch[dev]) return 0;
+    void *ptr = nullptr;
+    if (bk_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress scratch") || !ptr) return -1;
+    g_symv_scratch[dev] = (double *)ptr;
+    return 0;
+}
+extern "C" int bk_symv_f64_noalloc(double alpha, double beta, const double *A, const double *x, double *y, const SymvItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    if (!it || n_it > SYMV_NA_ITEMS) return bk_fail("bk_symv_f64_noalloc: at most 16 items per call");
+    int dev = 0;
+    if (bk_check(hipGetDevice(&dev), "hipGetDevice") || dev < 0 || dev >= 64) return -1;
+    if (!g_symv_scratch[dev] && bk_symv_scratch_init()) return -1;
+    SymvBatch bt, br;
+    memset(&bt, 0, sizeof(bt));
+    int64_t slots = 0, nt = 0, nr = 0;
+    for (int q = 0; q < n_it; q++) {
+        const int n = it[q].n;
+        if (n < 0) return bk_fail("bk_symv_f64_noalloc: an item of negative order");
+        const int64_t T = (n + 63) / 64;
+        bt.it[q] = it[q];
+        bt.it[q].p = slots;                          // the item's slots within scratch
+        slots += T * T * 64;
+        if (slots > SYMV_NA_SCRATCH)
+            return bk_fail("bk_symv_f64_noalloc: the items need more than the 32 MiB of partial-sum slots (one item of order <= 16384)");
+        bt.pre[q] = (int)nt;
+        nt += n > 0 ? (T * (T + 1) / 2 + SYMV_CH - 1) / SYMV_CH : 0;
+    }
+    bt.pre[n_it] = (int)nt;
+    bt.nit = n_it;
+    br = bt;
+    for (int q = 0; q < n_it; q++) { br.pre[q] = (int)nr; nr += it[q].n > 0 ? (it[q].n + 63) / 64 : 0; }
+    br.pre[n_it] = (int)nr;
+    if (nt == 0) return 0;
+    double *P = g_symv_scratch[dev];
+    hipLaunchKernelGGL(k_symv_tiles_na, dim3((unsigned)nt), dim3(256), 0, g_bk_stream, bt, A, x, P);
+    hipLaunchKernelGGL(k_symv_reduce_na, dim3((unsigned)nr), dim3(64), 0, g_bk_stream, br, (const double *)P, y, alpha, beta);
+    return BK_LAUNCH_CHECK("symv noalloc");
 }

@@ -68,6 +68,8 @@ extern "C" int bk_init(void) {
     if (bk_check(hipHostMalloc((void **)&g_cache_h, CACHE_CAP, hipHostMallocDefault), "hipHostMalloc cache")) return -1;
     if (bk_check(hipMalloc((void **)&g_cache_d, CACHE_CAP), "hipMalloc cache")) return -1;
     if (bk_check(hipHostMalloc((void **)&g_bounce, KCOPY_MAX, hipHostMallocDefault), "hipHostMalloc bounce")) return -1;
+    // the device address of the no-allocation symmetric product's scratch array (k_dense.hip)
+    if (bk_symv_scratch_init()) return -1;
     g_init = 1;
     return 0;
 }

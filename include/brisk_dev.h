@@ -125,6 +125,17 @@ int bk_gemv_mixed(int trans, double alpha, double beta, const float *A, const do
  * fixed order, so the result depends on the item's own data only. Offsets in doubles. */
 typedef struct { int n, ld; int64_t a, x, y, p; } SymvItem;
 int bk_symv_f64(double alpha, double beta, const double *A, const double *x, double *y, double *p, const SymvItem *it, int n_it);
+/* The same product with no device allocation and no workspace argument. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The partial-sum slots live in a
+ * file-scope device array of 32 MiB, one per GPU, held until the process ends; the items' p
+ * fields are ignored. At most 16 items per call and sum over the items of ceil(n/64)^2 * 64
+ * slots of 4 Mi doubles (a single item of order up to 16384): larger calls return -1 with the
+ * reason in bk_error() and launch nothing. Bit-identical to bk_symv_f64 (the same kernels'
+ * arithmetic). Calls that overlap on one GPU (other streams, other threads) race on the array. */
+int bk_symv_f64_noalloc(double alpha, double beta, const double *A, const double *x, double *y, const SymvItem *it, int n_it);
+/* looks up the device address of that array for the current device (bk_init calls it) */
+int bk_symv_scratch_init(void);
 
 /* ---- the device heap: one allocation; every array is an offset from its base ---- */
 int bk_heap_init(size_t bytes);                  /* (re)allocate; 0 on success */
