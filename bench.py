@@ -1,18 +1,26 @@
 #!/usr/bin/env python3
-"""bench.py - time to a 1e-8 DIMACS solution of the max-cut SDP relaxation of a random
-+-1-weighted graph (n = 2000, p = 0.01, seed 0; C = -L/4, diag X = 1, m = 2000) on one MI355X,
-mixed precision (FP32-MFMA Schur factor + FP64 refinement) with the FP64 path beside it.
+"""bench.py - time to a 1e-8 DIMACS solution on one MI355X, FP64 and mixed precision, and the
+serving loop's batched solves/s with p50/p99 latency.
 
-A step is one complete solve of that instance with its data already resident on the device:
+Headline (`value`): the max-cut SDP relaxation of a +-1-weighted random graph (n = 2000,
+p = 0.01, seed 0; C = -L/4, diag X = 1, m = 2000) in mixed precision (FP32-MFMA Schur factor +
+FP64 refinement). A step is one complete solve with the problem already resident on the device:
 the start point, the interior-point iterations and the DIMACS evaluation (the library's phase
-clock, synchronised at both ends). The reference CPU solver (the reference library built from
-its own sources into oracle/_ref/, where present) solves the same instance in a subprocess on
-the host cores as the reported baseline. One JSON line on stdout.
+clock, synchronised at both ends). Beside it: the whole call as a user sees it (host setup,
+upload, download: `end_to_end_s`), on which the CPU ratio is taken, since the reference CPU
+solver is timed the same way (a wall clock around brisk_solve_data); the reference's CPU solve
+of the same instance, both by its IPM (-bm 0) and by its default for this instance
+(Burer-Monteiro); the dominant kernel's roofline.
 
-Other workloads (--workload): c1 (Lovasz theta of G(50, 0.2)), c3 (dense n=400, m=2000),
-c4 (64 blocks of 60 + LP 500, m=5000), c5 (the serving stream: solves/s with every request
-queued at once, then p50/p99 latency under Poisson arrivals at --load times that rate; the
-reference's CPU baseline on every 16th request, one thread per solve in a pool of processes).
+The same line carries the other workloads as sub-objects ("workloads"): c1 the Lovasz theta SDP
+of G(50, 0.2) (FP64), c3 the dense random SDP n = 400, m = 2000 (mixed and FP64; the 10x
+target's config), c4 64 blocks of 60 + LP 500 with m = 5000, and c5 the serving stream (4096
+theta / max-cut requests: solves/s with every request queued at once, p50/p99 latency under
+Poisson arrivals, the reference's CPU rate, and sum(flops) / (t peak) over the stream).
+
+The reference CPU solver is the reference library compiled from its own sources into
+oracle/_ref/ (where present), run in a subprocess on the host cores. One JSON line on stdout.
+`--workload c1|c2|c3|c4|c5` runs one workload alone with it as the headline (for profiling).
 """
 import argparse
 import ctypes
@@ -26,43 +34,35 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
-sys.path.insert(0, os.path.join(ROOT, "tests"))
 
+from brisk_amd.generators import Instance, STREAM, stream  # noqa: E402
+
+METRIC = "time-to-1e-8 DIMACS (FP64 vs mixed, 1 GPU); batched solves/s + p50/p99 latency"
 PEAK = {"f64": 78.6, "f32": 157.3}      # MI355X dense MFMA TFLOP/s
 HBM_PEAK = 8000.0                        # GB/s
 KCLASS = {0: "k_gemm<double, 0, 0>", 1: "k_gemm<double, 0, 1>", 2: "k_gemm<double, 1, 0>", 3: "k_gemm<double, 1, 1>",
           4: "k_gemm<float, 0, 0>", 5: "k_gemm<float, 0, 1>", 6: "k_gemm<float, 1, 0>", 7: "k_gemm<float, 1, 1>",
           8: "k_potrf_diag", 9: "k_gemv", 10: "Schur assembly", 11: "k_aop", 12: "k_lz_step", 13: "k_dot_chunks",
           14: "k_lincomb", 15: "k_symv", 16: "k_trsv"}
+F32_CLASSES = (4, 5, 6, 7)
 
 WORKLOADS = {
-    "c1": dict(kind="theta", args=(50, 0.2, 0), desc="Lovasz theta SDP of G(50, 0.2), seed 0 (C=-J, tr X=1, X_ij=0 on edges)"),
-    "c2": dict(kind="maxcut", args=(2000, 0.01, 0), desc="max-cut SDP of G(2000, 0.01), weights +-1, seed 0 (C=-L/4, diag X=1, m=2000)"),
-    "c3": dict(kind="dense", args=(400, 2000, 0), desc="dense random SDP, one block n=400, m=2000, seed 0, feasible by X0/Z0/y0"),
-    "c4": dict(kind="multi", args=(64, 60, 500, 5000, 0), desc="64 PSD blocks of 60 + LP 500, m=5000, 1-3 blocks per row, seed 0"),
+    "c1": dict(kind="theta", args=(50, 0.2, 0), prec="double", fixture=("small", "theta_50_0.2_s0"),
+               desc="Lovasz theta SDP of G(50, 0.2), seed 0 (C=-J, tr X=1, X_ij=0 on edges)"),
+    "c2": dict(kind="maxcut", args=(2000, 0.01, 0), prec="mixed", fixture=("large", "maxcut_2000_0.01_s0"),
+               desc="max-cut SDP of G(2000, 0.01), weights +-1, seed 0 (C=-L/4, diag X=1, m=2000)"),
+    "c3": dict(kind="dense", args=(400, 2000, 0), prec="mixed", fixture=("large", "dense_400_2000_s0"),
+               desc="dense random SDP, one block n=400, m=2000, seed 0, feasible by X0/Z0/y0"),
+    "c4": dict(kind="multi", args=(64, 60, 500, 5000, 0), prec="mixed", fixture=("large", "multi_64_60_500_5000_s0"),
+               desc="64 PSD blocks of 60 + LP 500, m=5000, 1-3 blocks per row, seed 0"),
 }
-
-
-# the serving stream: theta / max-cut requests of uneven sizes with Poisson arrivals
-C5 = dict(count=4096, seed=0, nmin=20, nmax=200, pmin=0.05, pmax=0.3,
-          desc="serving stream: 4096 theta/max-cut SDPs (50/50), n ~ U{20..200}, p ~ U[0.05,0.3], Poisson arrivals, seed 0")
-
-
-def c5_stream(count=C5["count"], seed=C5["seed"]):
-    """Request k: (kind, n, p, instance seed) and a unit-rate exponential inter-arrival gap
-    (scaled by 1/lambda at run time)."""
-    rng = np.random.default_rng(seed)
-    theta = rng.random(count) < 0.5
-    n = rng.integers(C5["nmin"], C5["nmax"] + 1, count)
-    p = rng.uniform(C5["pmin"], C5["pmax"], count)
-    gaps = rng.exponential(1.0, count)
-    reqs = [("theta" if theta[k] else "maxcut", int(n[k]), float(p[k]), 1000 + k) for k in range(count)]
-    return reqs, gaps
-
-
 # measured HBM bytes per dispatch by kernel class (PMC FETCH_SIZE + WRITE_SIZE, copy-calibrated),
 # kept with the rocprofv3 summaries; keyed by workload
-TRAFFIC_FILES = {"c2": "maxcut2000_mixed_hbm_traffic_pmc.md"}
+TRAFFIC_FILES = {"c2": "maxcut2000_mixed_hbm_traffic_pmc.md", "c4": "blocks64x60_lp500_m5000_mixed_hbm_traffic_pmc.md"}
+
+
+def log(*a):
+    print("bench:", *a, file=sys.stderr, flush=True)
 
 
 def measured_traffic(workload, kernel):
@@ -81,26 +81,47 @@ def measured_traffic(workload, kernel):
 
 
 def instance(name):
-    from _ctypes_util import Instance
     w = WORKLOADS[name]
     return Instance(w["kind"], *w["args"])
 
 
+def fixture(name):
+    kind, key = WORKLOADS[name]["fixture"]
+    path = os.path.join(ROOT, "tests", "golden", kind + ".json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(key)
+    except OSError:
+        return None
+
+
 def solve(a, prec):
     import brisk_amd
-    return brisk_amd.solve_sdpa(a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"], options=["-prec", prec])
+    return brisk_amd.solve_sdpa(*a, options=["-prec", prec])
 
 
-def cpu_reference(name, threads, timeout):
-    """Solve the instance with the reference library in a subprocess; returns a dict or None."""
-    ref = os.path.join(ROOT, "oracle", "_ref", "libbrisk.so")
-    if not os.path.exists(ref):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libbrisk.so")
+
+
+def cpu_reference(name, threads, timeout, extra=()):
+    """Solve the instance with the reference library in a subprocess (wall clock around
+    brisk_solve_data, the same terms as end_to_end_s); returns a dict or None."""
+    if not os.path.exists(REF_LIB):
         return None
     code = r"""
 import ctypes, json, sys, time
-sys.path.insert(0, %r); sys.path.insert(0, %r)
+sys.path.insert(0, %r)
 import bench
-from _ctypes_util import Inst
 inst = bench.instance(%r)
 L = ctypes.CDLL(%r)
 P = ctypes.c_void_p; I = ctypes.c_int; D = ctypes.c_double
@@ -112,17 +133,16 @@ for n, t in (("pobj", D), ("dobj", D), ("time", D)):
 L.brisk_result_status.argtypes = [P]; L.brisk_result_iterations.argtypes = [P]
 L.brisk_result_dimacs.argtypes = [P, pd]
 L.brisk_set_output.argtypes = [I, P]; L.brisk_set_output(1, None)
-opts = ["-threads", str(%d), "-q"] + (["-bm", "0"] if %r == "c2" else [])
+opts = ["-threads", str(%d), "-q"] + %r
 o = (ctypes.c_char_p * len(opts))(*[x.encode() for x in opts])
 r = L.brisk_result_new()
-s = inst.s
 t0 = time.time()
-rc = L.brisk_solve_data(s.m, s.nblk, s.bs, s.c, s.nnz, s.mat, s.blk, s.i, s.j, s.v, len(opts), o, r)
+rc = L.brisk_solve_data(*inst.data_args(), len(opts), o, r)
 t = time.time() - t0
 e = (D * 6)(); L.brisk_result_dimacs(r, e)
 print(json.dumps(dict(rc=rc, status=L.brisk_result_status(r), pobj=L.brisk_result_pobj(r), dobj=L.brisk_result_dobj(r),
                       iters=L.brisk_result_iterations(r), wall=t, time=L.brisk_result_time(r), dimacs=list(e), opts=opts)))
-""" % (ROOT, os.path.join(ROOT, "tests"), name, ref, threads, name)
+""" % (ROOT, name, REF_LIB, threads, list(extra))
     env = dict(os.environ, BRISK_NOBUSY="1", OMP_NUM_THREADS=str(threads))
     try:
         out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=timeout, env=env)
@@ -134,11 +154,143 @@ print(json.dumps(dict(rc=rc, status=L.brisk_result_status(r), pobj=L.brisk_resul
     return dict(error=(out.stderr or out.stdout)[-400:])
 
 
+def rel(a, b):
+    return abs(a - b) / (1.0 + abs(b))
+
+
+def vs_fixture(r, rec):
+    """Our result against the reference's recorded solve of the instance (tests/golden)."""
+    if not rec:
+        return None
+    out = dict(same_status=r.status == rec["status"], pobj_rel=rel(r.pobj, rec["pobj"]), dobj_rel=rel(r.dobj, rec["dobj"]),
+               ref_iterations=rec["iterations"], ref_dimacs_max=max(abs(x) for x in rec["dimacs"]))
+    if "y" in rec and r.y is not None and len(rec["y"]) == len(r.y):
+        yr = np.asarray(rec["y"])
+        out["y_distance"] = float(np.linalg.norm(r.y - yr) / (1 + np.linalg.norm(yr)))
+    return out
+
+
+def profile_solve(a, prec, name):
+    """One more solve with the per-class kernel timers (HIP events on the library stream): the
+    dominant class's achieved rate against its roofline, and the class table on stderr."""
+    import brisk_amd
+    brisk_amd.profile(True)
+    brisk_amd.profile_reset()
+    solve(a, prec)
+    prof = {c: brisk_amd.profile_read(c) for c in KCLASS}
+    brisk_amd.profile(False)
+    tot = sum(p["ms"] for p in prof.values())
+    dom = max(prof, key=lambda c: prof[c]["ms"])
+    for c in sorted(prof, key=lambda c: -prof[c]["ms"])[:8]:
+        p = prof[c]
+        if p["launches"]:
+            log("%s kernel %-22s %8.2f ms over %5d launches (%.1f us avg), %.2f TFLOP/s, %.0f GB/s" % (
+                name, KCLASS[c], p["ms"], p["launches"], 1e3 * p["ms"] / p["launches"],
+                p["flops"] / max(p["ms"], 1e-9) / 1e9, p["bytes"] / max(p["ms"], 1e-9) / 1e6))
+    d = prof[dom]
+    avg_s = d["ms"] / max(d["launches"], 1) / 1e3
+    if dom <= 8 and d["flops"] > 0:           # the MFMA classes: GEMMs and the diagonal-block factor
+        pk = PEAK["f32" if dom in F32_CLASSES else "f64"]
+        ach = d["flops"] / max(d["launches"], 1) / avg_s / 1e12
+        roof = dict(bound="mfma", achieved=round(ach, 3), peak=pk, unit="TFLOP/s", frac=round(ach / pk, 4),
+                    traffic=measured_traffic(name, KCLASS[dom]), kernel=KCLASS[dom], launches=int(d["launches"]),
+                    avg_us=round(avg_s * 1e6, 2), flops_per_launch=d["flops"] / max(d["launches"], 1),
+                    share_of_kernel_time=round(d["ms"] / max(tot, 1e-9), 3))
+    else:
+        ach = d["bytes"] / max(d["launches"], 1) / avg_s / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK, unit="GB/s", frac=round(ach / HBM_PEAK, 4),
+                    traffic=measured_traffic(name, KCLASS[dom]), kernel=KCLASS[dom], launches=int(d["launches"]),
+                    avg_us=round(avg_s * 1e6, 2), bytes_per_launch=d["bytes"] / max(d["launches"], 1),
+                    share_of_kernel_time=round(d["ms"] / max(tot, 1e-9), 3))
+    f64 = sum(prof[c]["flops"] for c in prof if c <= 3 or c == 8)
+    f32 = sum(prof[c]["flops"] for c in F32_CLASSES)
+    return roof, dict(flops_f64=f64, flops_f32=f32, kernel_ms=tot)
+
+
+def timed(a, prec, k):
+    """k solves: their device-resident times (start point + iterations + DIMACS), whole-call
+    times (brisk_result_time: host setup, upload, iterations, DIMACS, download) and wall clocks."""
+    import torch
+    res, ts, te, tw = None, [], [], []
+    for _ in range(k):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = solve(a, prec)
+        torch.cuda.synchronize()
+        tw.append(time.perf_counter() - t0)
+        ts.append(res.phases[1] + res.phases[2])
+        te.append(res.time)
+    return float(np.median(ts)), float(np.median(te)), float(np.median(tw)), res
+
+
+def workload(name, steps, warmup, fp64_steps, cpu, threads, cpu_timeout=600):
+    """One of c1-c4: the precision the workload is quoted in and the other beside it."""
+    w = WORKLOADS[name]
+    inst = instance(name)
+    a = inst.sdpa()
+    log("%s: %s: m=%d, blocks %s" % (name, w["desc"], inst.m, inst.bs[:4]))
+    main_prec = w["prec"]
+    for k in range(warmup):
+        r = solve(a, main_prec)
+        log("%s warmup %d: %s in %.3fs (%d iterations)" % (name, k, r.status_str, r.phases[1] + r.phases[2], r.iterations))
+    t, te, tw, r = timed(a, main_prec, steps)
+    log("%s %s: %s, %d iterations, %.4fs per solve resident (median of %d), %.4fs whole call, phases %s, counts %s, DIMACS max %.1e"
+        % (name, main_prec, r.status_str, r.iterations, t, steps, te, np.round(r.phases, 4), r.counts, np.max(np.abs(r.dimacs))))
+    out = dict(workload=w["desc"], m=int(inst.m), precision=main_prec, time_s=round(t, 5), end_to_end_s=round(te, 5),
+               wall_s=round(tw, 5), status=r.status_str, iterations=r.iterations, dimacs=[float("%.3e" % x) for x in r.dimacs],
+               pobj=r.pobj, dobj=r.dobj,
+               phases_s=dict(setup_upload=round(r.phases[0], 4), iterations=round(r.phases[1], 4),
+                             dimacs=round(r.phases[2], 4), download=round(r.phases[3], 4)),
+               factor_counts=dict(f32=r.counts[0], f64=r.counts[1], pcg_steps=r.counts[2]),
+               vs_reference_fixture=vs_fixture(r, fixture(name)))
+    other = "double" if main_prec == "mixed" else "mixed"
+    if fp64_steps:
+        t2, te2, _, r2 = timed(a, other, fp64_steps)
+        log("%s %s: %s, %d iterations, %.4fs per solve resident, %.4fs whole call" % (name, other, r2.status_str, r2.iterations, t2, te2))
+        out["other_precision"] = dict(precision=other, time_s=round(t2, 5), end_to_end_s=round(te2, 5), status=r2.status_str,
+                                      iterations=r2.iterations, dimacs_max=float(np.max(np.abs(r2.dimacs))),
+                                      vs_reference_fixture=vs_fixture(r2, fixture(name)))
+        t_f64, t_mixed = (t2, t) if main_prec == "mixed" else (t, t2)
+        out["mixed_over_fp64"] = round(t_f64 / t_mixed, 3)
+    out["roofline"], tot = profile_solve(a, main_prec, name)
+    out["kernel_flops"] = tot
+    if cpu:
+        runs = [("ipm", ["-bm", "0"] if name == "c2" else [])]
+        if name == "c2":
+            runs.append(("default", []))          # the reference's default on max-cut n >= 500: Burer-Monteiro
+        cb = {}
+        for label, extra in runs:
+            ref = cpu_reference(name, threads, cpu_timeout, extra)
+            if ref and "error" not in ref:
+                cb[label] = dict(value=round(ref["wall"], 4), unit="s", cores=threads, kind="reference",
+                                 sample="one full solve of the same instance by the reference library (%s), %s"
+                                        % (" ".join(ref["opts"]), cpu_model()),
+                                 status=ref["status"], iterations=ref["iters"], pobj=ref["pobj"],
+                                 dimacs_max=max(abs(x) for x in ref["dimacs"]),
+                                 cpu_over_gpu_end_to_end=round(ref["wall"] / te, 2), cpu_over_gpu_resident=round(ref["wall"] / t, 2))
+                log("%s reference CPU (%s): %.3fs, status %d" % (name, " ".join(ref["opts"]), ref["wall"], ref["status"]))
+            else:
+                cb[label] = dict(value=None, unit="s", cores=threads, kind="reference",
+                                 sample="not run: %s" % (ref or {}).get("error", "oracle/_ref/libbrisk.so absent"))
+        out["cpu_baseline"] = cb
+    else:
+        rec = fixture(name)
+        if rec:
+            out["cpu_baseline"] = {"ipm": dict(value=rec["seconds"], unit="s", cores=8, kind="reference", timed_on_this_host=False,
+                                               sample="the reference library's full solve of the same instance recorded with the "
+                                                      "fixture (tests/golden/make_golden.py, %s, 8 threads of an Intel Xeon @ 2.70 GHz); "
+                                                      "not re-timed here (a solve takes ~10 min)" % " ".join(rec["options"]),
+                                               cpu_over_gpu_end_to_end=round(rec["seconds"] / te, 2))}
+    return out, r
+
+
+# ---- the serving stream ----------------------------------------------------------------------
+
 def cpu_serving_sample(stride, workers):
     """The reference library on every stride-th request of the stream, one thread per solve
     (the library is not reentrant), a pool of worker processes; prints one JSON line."""
     import multiprocessing as mp
-    reqs, _ = c5_stream()
+    reqs, _ = stream()
     sample = reqs[::stride]
     t0 = time.time()
     with mp.get_context("fork").Pool(workers, initializer=_ref_worker_init) as pool:
@@ -153,7 +305,7 @@ _REF = None
 
 def _ref_worker_init():
     global _REF
-    L = ctypes.CDLL(os.path.join(ROOT, "oracle", "_ref", "libbrisk.so"))
+    L = ctypes.CDLL(REF_LIB)
     P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     pd, pi, ppc = ctypes.POINTER(D), ctypes.POINTER(I), ctypes.POINTER(ctypes.c_char_p)
     L.brisk_result_new.restype = P
@@ -166,7 +318,6 @@ def _ref_worker_init():
 
 
 def _ref_worker_solve(req):
-    from _ctypes_util import Instance
     kind, n, p, seed = req
     inst = Instance(kind, n, p, seed)
     opts = ["-threads", "1", "-q"]
@@ -214,44 +365,55 @@ def run_stream(insts, arrivals, opts):
     return span, lat, status, iters
 
 
-def serving(args):
-    """The serving workload: solves/s with every request queued at t = 0 (saturation), then
-    p50/p99 latency under Poisson arrivals at a fixed fraction of that rate."""
-    from _ctypes_util import Instance
-    count = args.requests or C5["count"]
-    reqs, gaps = c5_stream()
+def serving(count, slots, steps, warmup, load, cpu, workers, stride, schur_apply="auto"):
+    """The serving workload: solves/s with every request queued at t = 0 (saturation), p50/p99
+    latency under Poisson arrivals at a fixed fraction of that rate, the reference's CPU rate, and
+    sum(flops) / (t peak) of the stream's kernels (counted in a profiled run)."""
+    import brisk_amd
+    reqs, gaps = stream()
     reqs, gaps = reqs[:count], gaps[:count]
-    insts = []
-    for kind, n, p, seed in reqs:
-        a = Instance(kind, n, p, seed).arrays()
-        insts.append((a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"]))
+    insts = [Instance(kind, n, p, seed).sdpa() for kind, n, p, seed in reqs]
     ms = [int(x[1].shape[0]) for x in insts]
-    print("bench: %d requests, m from %d to %d (mean %.0f)" % (count, min(ms), max(ms), np.mean(ms)), file=sys.stderr, flush=True)
-    opts = ["-prec", "mixed", "-slots", str(args.slots), "-schurapply", args.schur_apply]
-    for w in range(args.warmup):
+    log("c5: %d requests, m from %d to %d (mean %.0f)" % (count, min(ms), max(ms), np.mean(ms)))
+    opts = ["-prec", "mixed", "-slots", str(slots), "-schurapply", schur_apply]
+    for w in range(warmup):
         span, _, st, _ = run_stream(insts[:64], np.zeros(64), opts)
-        print("bench: warmup %d: 64 requests in %.3fs" % (w, span), file=sys.stderr, flush=True)
+        log("c5 warmup %d: 64 requests in %.3fs" % (w, span))
     runs = []
-    for k in range(args.steps):
+    for k in range(steps):
         span, lat, st, it = run_stream(insts, np.zeros(count), opts)
         runs.append(span)
-        print("bench: saturation run %d: %d requests in %.3fs = %.1f solves/s; optimal %d, p50 %.3fs p99 %.3fs, mean iterations %.1f"
-              % (k, count, span, count / span, int(np.sum(st == 0)), np.percentile(lat, 50), np.percentile(lat, 99), it.mean()),
-              file=sys.stderr, flush=True)
+        log("c5 saturation run %d: %d requests in %.3fs = %.1f solves/s; optimal %d, p50 %.3fs p99 %.3fs, mean iterations %.1f"
+            % (k, count, span, count / span, int(np.sum(st == 0)), np.percentile(lat, 50), np.percentile(lat, 99), it.mean()))
     span = float(np.median(runs))
     rate = count / span
-    lam = args.load * rate
+    lam = load * rate
     arr = np.cumsum(gaps) / lam
     pspan, plat, pst, pit = run_stream(insts, arr, opts)
-    print("bench: Poisson arrivals at %.1f/s (%.2f x the saturation rate): %d requests in %.3fs, optimal %d, p50 %.3fs, p99 %.3fs"
-          % (lam, args.load, count, pspan, int(np.sum(pst == 0)), np.percentile(plat, 50), np.percentile(plat, 99)),
-          file=sys.stderr, flush=True)
-    cpu = None
-    if not args.skip_cpu and os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libbrisk.so")):
-        workers = args.cpu_threads or min(16, os.cpu_count() or 1)
+    log("c5 Poisson arrivals at %.1f/s (%.2f x the saturation rate): %d requests in %.3fs, optimal %d, p50 %.3fs, p99 %.3fs"
+        % (lam, load, count, pspan, int(np.sum(pst == 0)), np.percentile(plat, 50), np.percentile(plat, 99)))
+    # the stream's flops: the same saturation run with the per-class kernel counters on
+    brisk_amd.profile(True)
+    brisk_amd.profile_reset()
+    pr_span, _, _, _ = run_stream(insts, np.zeros(count), opts)
+    prof = {c: brisk_amd.profile_read(c) for c in KCLASS}
+    brisk_amd.profile(False)
+    f64 = sum(prof[c]["flops"] for c in prof if c <= 3 or c == 8)
+    f32 = sum(prof[c]["flops"] for c in F32_CLASSES)
+    # the stream's arithmetic at the two precisions' MFMA peaks: a machine that ran only these
+    # flops at peak would take t_peak; frac = t_peak / t
+    t_peak = f64 / (PEAK["f64"] * 1e12) + f32 / (PEAK["f32"] * 1e12)
+    roof = dict(bound="mfma", achieved=round((f64 + f32) / span / 1e12, 3), peak=round((f64 + f32) / max(t_peak, 1e-30) / 1e12, 2),
+                unit="TFLOP/s", frac=round(t_peak / span, 4), traffic=None, kernel="all MFMA classes over the stream",
+                flops_f64=f64, flops_f32=f32, profiled_span_s=round(pr_span, 3),
+                note="achieved = the stream's MFMA flops (counted with the kernel timers on) / the unprofiled saturation "
+                     "makespan; peak = those flops' rate at the FP64 and FP32 MFMA peaks")
+    log("c5 flops: %.3e f64 + %.3e f32 over %.3fs: %.2f TFLOP/s, %.4f of the mixed peak" % (f64, f32, span, (f64 + f32) / span / 1e12, t_peak / span))
+    cpu_b = None
+    if cpu and os.path.exists(REF_LIB):
         env = dict(os.environ, BRISK_NOBUSY="1", OMP_NUM_THREADS="1")
         try:
-            out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-serving-sample", str(args.cpu_stride),
+            out = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-serving-sample", str(stride),
                                   "--cpu-threads", str(workers)], capture_output=True, text=True, timeout=900, env=env)
             line = [x for x in out.stdout.splitlines() if x.startswith("{")]
             ref = json.loads(line[-1]) if line else dict(error=out.stderr[-400:])
@@ -261,48 +423,30 @@ def serving(args):
             # the rate of perfectly packed workers (solves x workers / CPU seconds): an upper bound of
             # the pool's throughput on the whole stream, above the sample's own wall-clock rate
             packed = ref["count"] * workers / max(ref["cpu_s"], 1e-9)
-            cpu = dict(value=round(packed, 3), unit="solves/s", cores=workers, kind="reference",
-                       sample="every %dth request of the stream (%d solves, %d optimal) by the reference library, one thread per "
-                              "solve in %d worker processes: %.1f CPU s, %.1f s wall (%.2f solves/s measured; value = solves x "
-                              "workers / CPU s), %s" % (args.cpu_stride, ref["count"], ref["optimal"], workers, ref["cpu_s"],
-                                                       ref["wall"], ref["solves_per_s"], cpu_model()))
+            cpu_b = dict(value=round(packed, 3), unit="solves/s", cores=workers, kind="reference",
+                         sample="every %dth request of the stream (%d solves, %d optimal) by the reference library, one thread per "
+                                "solve in %d worker processes: %.1f CPU s, %.1f s wall (%.2f solves/s measured; value = solves x "
+                                "workers / CPU s), %s" % (stride, ref["count"], ref["optimal"], workers, ref["cpu_s"],
+                                                         ref["wall"], ref["solves_per_s"], cpu_model()))
+            log("c5 reference CPU: %.2f solves/s packed (%d workers)" % (packed, workers))
         else:
-            cpu = dict(value=None, unit="solves/s", cores=workers, kind="reference", sample="not run: %s" % ref["error"])
-    import brisk_amd
-    out = {
-        "metric": "time-to-1e-8 DIMACS (FP64 vs mixed, 1 GPU); batched solves/s + p50/p99 latency",
-        "value": round(rate, 3), "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1e3 * span, 2), "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f32+f64", "data": "synthetic (seeded generators, %s)" % C5["desc"],
-        "config": {"workload": C5["desc"], "requests": count, "precision": "mixed", "tol": 1e-8, "slots": args.slots,
-                   "m_range": [min(ms), max(ms)]},
-        "optimal": int(np.sum(st == 0)), "latency_saturation_s": {"p50": round(float(np.percentile(lat, 50)), 4),
-                                                                 "p99": round(float(np.percentile(lat, 99)), 4)},
-        "poisson": {"rate_per_s": round(lam, 2), "load": args.load, "span_s": round(pspan, 3), "optimal": int(np.sum(pst == 0)),
-                    "p50_s": round(float(np.percentile(plat, 50)), 4), "p99_s": round(float(np.percentile(plat, 99)), 4)},
-        "cpu_baseline": cpu,
-        "gpu_over_cpu": None if not cpu or not cpu.get("value") else round(rate / cpu["value"], 2),
-        "device": brisk_amd.load_library().bk_device_name().decode(),
-    }
-    print(json.dumps(out))
-
-
-def cpu_model():
-    try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
-        pass
-    return "unknown"
+            cpu_b = dict(value=None, unit="solves/s", cores=workers, kind="reference", sample="not run: %s" % ref["error"])
+    return dict(workload=STREAM["desc"], requests=count, precision="mixed", slots=slots, solves_per_s=round(rate, 3),
+                makespan_s=round(span, 3), optimal=int(np.sum(st == 0)),
+                latency_saturation_s=dict(p50=round(float(np.percentile(lat, 50)), 4), p99=round(float(np.percentile(lat, 99)), 4)),
+                poisson=dict(rate_per_s=round(lam, 2), load=load, span_s=round(pspan, 3), optimal=int(np.sum(pst == 0)),
+                             p50_s=round(float(np.percentile(plat, 50)), 4), p99_s=round(float(np.percentile(plat, 99)), 4)),
+                m_range=[min(ms), max(ms)], roofline=roof, cpu_baseline=cpu_b,
+                gpu_over_cpu=None if not cpu_b or not cpu_b.get("value") else round(rate / cpu_b["value"], 2))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS) + ["c5"])
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="all", choices=["all"] + sorted(WORKLOADS) + ["c5"],
+                    help="all: the c2 headline with c1, c3, c4, c5 beside it; else one workload as the headline")
     ap.add_argument("--requests", type=int, default=0, help="c5: requests of the stream to serve (default 4096)")
     ap.add_argument("--slots", type=int, default=512, help="c5: most solves in flight")
     ap.add_argument("--load", type=float, default=0.8, help="c5: Poisson arrival rate as a fraction of the saturation rate")
@@ -311,110 +455,57 @@ def main():
     ap.add_argument("--schur-apply", default="auto", choices=["auto", "inverse", "solves"],
                     help="c5: how the Schur factor is applied in the PCG (-schurapply)")
     ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     if args.cpu_serving_sample:
-        cpu_serving_sample(args.cpu_serving_sample, args.cpu_threads or min(16, os.cpu_count() or 1))
+        cpu_serving_sample(args.cpu_serving_sample, threads)
         return
     if args.gpus != 1:
         print("bench.py: the solve runs on one GPU; --gpus %d is not supported (the multi-GPU sharding is a design only)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
-    if args.workload == "c5":
-        serving(args)
-        return
-    import torch
+    # torch first: its HIP runtime is then the one the library binds to (loaded the other way
+    # round, two runtimes share the process and torch finds no device)
+    import torch  # noqa: F401
     import brisk_amd
-    from _ctypes_util import Instance  # noqa: F401
-    name = args.workload
-    inst = instance(name)
-    a = inst.arrays()
-    print("bench: %s: m=%d, blocks %s" % (WORKLOADS[name]["desc"], a["m"], list(a["bs"])[:4]), file=sys.stderr, flush=True)
     L = brisk_amd.load_library()
-
-    def timed(prec, k):
-        ts, res = [], None
-        for _ in range(k):
-            torch.cuda.synchronize()
-            res = solve(a, prec)
-            torch.cuda.synchronize()
-            ts.append(res.phases[1] + res.phases[2])
-        return ts, res
-
-    for w in range(args.warmup):
-        r = solve(a, "mixed")
-        print("bench: warmup %d: %s in %.3fs (%d iterations)" % (w, r.status_str, r.phases[1] + r.phases[2], r.iterations),
-              file=sys.stderr, flush=True)
-    ts, rm = timed("mixed", args.steps)
-    t_mixed = float(np.median(ts))
-    print("bench: mixed: %s, %d iterations, %.4fs per solve (median of %d), DIMACS max %.1e, phases %s, counts %s"
-          % (rm.status_str, rm.iterations, t_mixed, len(ts), np.max(np.abs(rm.dimacs)), np.round(rm.phases, 4), rm.counts),
-          file=sys.stderr, flush=True)
-    t64, r64 = None, None
-    if not args.skip_fp64:
-        ts64, r64 = timed("double", max(1, min(args.steps, 2)))
-        t64 = float(np.median(ts64))
-        print("bench: fp64: %s, %d iterations, %.4fs per solve, DIMACS max %.1e" % (r64.status_str, r64.iterations, t64,
-              np.max(np.abs(r64.dimacs))), file=sys.stderr, flush=True)
-    # live per-kernel timing (events on the library's stream) of one more mixed solve
-    brisk_amd.profile(True)
-    brisk_amd.profile_reset()
-    rp = solve(a, "mixed")
-    prof = {c: brisk_amd.profile_read(c) for c in KCLASS}
-    brisk_amd.profile(False)
-    tot = sum(p["ms"] for p in prof.values())
-    dom = max(prof, key=lambda c: prof[c]["ms"])
-    for c in sorted(prof, key=lambda c: -prof[c]["ms"])[:8]:
-        p = prof[c]
-        if p["launches"]:
-            print("bench: kernel %-22s %8.2f ms over %5d launches (%.1f us avg), %.2f TFLOP/s, %.0f GB/s" % (
-                KCLASS[c], p["ms"], p["launches"], 1e3 * p["ms"] / p["launches"],
-                p["flops"] / max(p["ms"], 1e-9) / 1e9, p["bytes"] / max(p["ms"], 1e-9) / 1e6), file=sys.stderr, flush=True)
-    d = prof[dom]
-    avg_s = d["ms"] / max(d["launches"], 1) / 1e3
-    if dom <= 8 and d["flops"] > 0:           # the MFMA classes: GEMMs and the diagonal-block factor
-        prec = "f64" if dom < 4 or dom in (8,) else "f32"
-        ach = d["flops"] / max(d["launches"], 1) / avg_s / 1e12
-        roof = dict(bound="mfma", achieved=round(ach, 3), peak=PEAK[prec], unit="TFLOP/s", frac=round(ach / PEAK[prec], 4),
-                    traffic=measured_traffic(name, KCLASS[dom]), kernel=KCLASS[dom], launches=int(d["launches"]), avg_us=round(avg_s * 1e6, 2),
-                    flops_per_launch=d["flops"] / max(d["launches"], 1), share_of_kernel_time=round(d["ms"] / max(tot, 1e-9), 3))
-    else:
-        ach = d["bytes"] / max(d["launches"], 1) / avg_s / 1e9
-        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK, unit="GB/s", frac=round(ach / HBM_PEAK, 4),
-                    traffic=measured_traffic(name, KCLASS[dom]),
-                    kernel=KCLASS[dom], launches=int(d["launches"]), avg_us=round(avg_s * 1e6, 2),
-                    bytes_per_launch=d["bytes"] / max(d["launches"], 1), share_of_kernel_time=round(d["ms"] / max(tot, 1e-9), 3))
-    cpu = None
-    vs_ref = None
-    if not args.skip_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        ref = cpu_reference(name, threads, timeout=600)
-        if ref and "error" not in ref:
-            cpu = dict(value=round(ref["wall"], 4), unit="s", cores=threads, kind="reference",
-                       sample="one full solve of the same instance by the reference library (%s), %s" % (" ".join(ref["opts"]), cpu_model()),
-                       status=ref["status"], iterations=ref["iters"], pobj=ref["pobj"], dimacs_max=max(abs(x) for x in ref["dimacs"]))
-            vs_ref = dict(pobj_rel_mixed=abs(rm.pobj - ref["pobj"]) / (1 + abs(ref["pobj"])),
-                          dobj_rel_mixed=abs(rm.dobj - ref["dobj"]) / (1 + abs(ref["dobj"])),
-                          same_status=rm.status == ref["status"], cpu_over_gpu=round(ref["wall"] / t_mixed, 2))
-        else:
-            cpu = dict(value=None, unit="s", cores=threads, kind="reference", sample="not run: %s" % (ref or {}).get("error", "oracle/_ref/libbrisk.so absent"))
+    count = args.requests or STREAM["count"]
+    cpu = not args.skip_cpu
+    if args.workload == "c5":
+        s = serving(count, args.slots, max(1, args.steps), args.warmup, args.load, cpu, threads, args.cpu_stride, args.schur_apply)
+        out = {"metric": METRIC, "value": s["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": max(1, args.steps),
+               "warmup": args.warmup, "ms_per_step": round(1e3 * s["makespan_s"], 2), "higher_is_better": True,
+               "scaling": "strong", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic (seeded generators, %s)" % STREAM["desc"],
+               "config": {"workload": STREAM["desc"], "requests": count, "precision": "mixed", "tol": 1e-8, "slots": args.slots},
+               "roofline": s["roofline"], "cpu_baseline": s["cpu_baseline"], "serving": s,
+               "device": L.bk_device_name().decode()}
+        print(json.dumps(out))
+        return
+    head = "c2" if args.workload == "all" else args.workload
+    h, r = workload(head, args.steps, args.warmup, max(1, min(args.steps, 2)), cpu, threads)
+    extras = {}
+    if args.workload == "all":
+        extras["c1"], _ = workload("c1", 5, 1, 2, cpu, threads, cpu_timeout=120)
+        extras["c3"], _ = workload("c3", 2, 1, 2, False, threads)
+        extras["c4"], _ = workload("c4", 3, 1, 2, cpu, threads, cpu_timeout=300)
+        extras["c5"] = serving(count, args.slots, 1, 1, args.load, cpu, threads, args.cpu_stride)
+    cb = h.get("cpu_baseline", {})
+    main_cpu = cb.get("ipm")
     out = {
-        "metric": "time-to-1e-8 DIMACS (FP64 vs mixed, 1 GPU); batched solves/s + p50/p99 latency",
-        "value": round(t_mixed, 5), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(1e3 * t_mixed, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f32+f64", "data": "synthetic (seeded generator, %s)" % WORKLOADS[name]["desc"],
-        "config": {"workload": WORKLOADS[name]["desc"], "precision": "mixed (FP32-MFMA Schur factor, FP64 refinement and iterates)",
-                   "tol": 1e-8, "m": int(a["m"]), "n": [int(x) for x in a["bs"]][:8]},
-        "status": rm.status_str, "iterations": rm.iterations, "dimacs": [float("%.3e" % x) for x in rm.dimacs],
-        "pobj": rm.pobj, "dobj": rm.dobj, "solves_per_s": round(1.0 / t_mixed, 3),
-        "fp64_time_s": None if t64 is None else round(t64, 5), "fp64_status": None if r64 is None else r64.status_str,
-        "fp64_iterations": None if r64 is None else r64.iterations, "mixed_over_fp64": None if t64 is None else round(t64 / t_mixed, 3),
-        "phases_s": {"setup_upload": round(rm.phases[0], 4), "iterations": round(rm.phases[1], 4), "dimacs": round(rm.phases[2], 4),
-                     "download": round(rm.phases[3], 4)},
-        "factor_counts": {"f32": rm.counts[0], "f64": rm.counts[1], "pcg_steps": rm.counts[2]},
-        "roofline": roof, "cpu_baseline": cpu, "vs_reference": vs_ref,
-        "device": L.bk_device_name().decode(),
+        "metric": METRIC, "value": h["time_s"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * h["time_s"], 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f32+f64" if h["precision"] == "mixed" else "f64",
+        "data": "synthetic (seeded generator, %s)" % WORKLOADS[head]["desc"],
+        "config": {"workload": WORKLOADS[head]["desc"], "precision": h["precision"], "tol": 1e-8, "m": h["m"]},
+        "status": h["status"], "iterations": h["iterations"], "dimacs": h["dimacs"], "pobj": h["pobj"], "dobj": h["dobj"],
+        "end_to_end_s": h["end_to_end_s"], "phases_s": h["phases_s"], "factor_counts": h["factor_counts"],
+        "other_precision": h.get("other_precision"), "mixed_over_fp64": h.get("mixed_over_fp64"),
+        "roofline": h["roofline"], "cpu_baseline": main_cpu, "cpu_baseline_default_method": cb.get("default"),
+        "vs_reference": dict(fixture=h["vs_reference_fixture"],
+                             cpu_over_gpu_end_to_end=None if not main_cpu else main_cpu.get("cpu_over_gpu_end_to_end"),
+                             cpu_over_gpu_resident=None if not main_cpu else main_cpu.get("cpu_over_gpu_resident")),
+        "workloads": extras, "device": L.bk_device_name().decode(),
     }
     print(json.dumps(out))
 

@@ -319,6 +319,65 @@ static int blk_gemm(Solve **S, int ns, const int *act, int ta, int tb, int tri, 
     double *H = heapp();
     return bk_gemm_f64(ta, tb, alpha, beta, H, H, H, g_gl.v, g_gl.n, NULL);
 }
+/* Out = Zi V X per block of the active solves, V a dual-side direction: dZ = theta R0 - A*(dy)
+ * (R0 = -shift I at the start, C - Z - A*(y) after a re-base), or the refinement's A*(w). Each
+ * lies in the span of I and the A_i, so it is diagonal in a block whose constraint entries all
+ * are (max-cut, LP variables): there Out = Zi (Diag(V) X), a row scaling into W1 and one GEMM;
+ * elsewhere W1 = Zi V and Out = W1 X, and W1 keeps Zi V (the corrector's Zi dZ dX reuses it). */
+static int zi_v_x(Solve **S, int ns, const int *act, size_t fV, size_t fX, size_t fOut) {
+    double *H = heapp();
+    GemmL_clear(&g_gl);
+    MItL_clear(&g_ml);
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        const HProb *P = &S[q]->P;
+        const int64_t v = F_(S[q], fV), x = F_(S[q], fX), w = S[q]->W1, zi = S[q]->Zi;
+        for (int k = 0; k < P->nb; k++) {
+            const int n = P->bn[k];
+            const int64_t o = P->boff[k];
+            if (P->bdiag && P->bdiag[k]) { MItem it = { n, 0, v + o, x + o, w + o, 0 }; MItL_add(&g_ml, it); }
+            else { GemmItem g = { n, n, n, n, n, n, zi + o, v + o, w + o, -1, 0 }; GemmL_add(&g_gl, g); }
+        }
+    }
+    if (g_gl.n && bk_gemm_f64(0, 0, 1.0, 0.0, H, H, H, g_gl.v, g_gl.n, NULL)) return -1;
+    if (g_ml.n && bk_rowscale(g_ml.v, g_ml.n)) return -1;
+    GemmL_clear(&g_gl);
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        const HProb *P = &S[q]->P;
+        const int64_t x = F_(S[q], fX), w = S[q]->W1, zi = S[q]->Zi, out = F_(S[q], fOut);
+        for (int k = 0; k < P->nb; k++) {
+            const int n = P->bn[k];
+            const int64_t o = P->boff[k];
+            const int dg = P->bdiag && P->bdiag[k];
+            GemmItem g = { n, n, n, n, n, n, dg ? zi + o : w + o, dg ? w + o : x + o, out + o, -1, 0 };
+            GemmL_add(&g_gl, g);
+        }
+    }
+    return bk_gemm_f64(0, 0, 1.0, 0.0, H, H, H, g_gl.v, g_gl.n, NULL);
+}
+/* the corrector's second-order term Q = Zi dZ dX of the affine directions: W1 dX with W1 = Zi dZ
+ * left by the predictor's zi_v_x, and Zi (Diag(dZ) dX) in the blocks where dZ is diagonal */
+static int zi_dz_dx(Solve **S, int ns, const int *act) {
+    double *H = heapp();
+    MItL_clear(&g_ml);
+    GemmL_clear(&g_gl);
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        const Solve *s = S[q];
+        const HProb *P = &s->P;
+        for (int k = 0; k < P->nb; k++) {
+            const int n = P->bn[k];
+            const int64_t o = P->boff[k];
+            const int dg = P->bdiag && P->bdiag[k];
+            if (dg) { MItem it = { n, 0, s->dZ + o, s->dX + o, s->W1 + o, 0 }; MItL_add(&g_ml, it); }
+            GemmItem g = { n, n, n, n, n, n, dg ? s->Zi + o : s->W1 + o, dg ? s->W1 + o : s->dX + o, s->Q + o, -1, 0 };
+            GemmL_add(&g_gl, g);
+        }
+    }
+    if (g_ml.n && bk_rowscale(g_ml.v, g_ml.n)) return -1;
+    return bk_gemm_f64(0, 0, 1.0, 0.0, H, H, H, g_gl.v, g_gl.n, NULL);
+}
 static int blk_sym(Solve **S, int ns, const int *act, size_t f) {
     MItL_clear(&g_ml);
     for (int q = 0; q < ns; q++) {
@@ -787,6 +846,8 @@ static int schur_factor_(Solve **S, int ns, const int *act, int force64, int *fa
                 Solve *s = S[q];
                 const int64_t mul = prec ? 1 : 2;
                 fi[t].n = s->P.m; fi[t].slot = q; fi[t].off = mul * s->MF; fi[t].off2 = mul * s->MDinv; fi[t].off3 = mul * s->Mwork;
+                fi[t].off4 = 0; fi[t].pad = 0;
+                fi[t].opt = s->o.tput ? (s->P.m >= 1024 ? 256 : 128) : 0;
             }
             int rc = prec ? bk_potrf_f64(fi, g_ml.n, H, H, H, dint(), pm64)
                           : bk_potrf_f32(fi, g_ml.n, (float *)H, (float *)H, (float *)H, dint(), pm32);
@@ -845,7 +906,11 @@ static int schur_factor_(Solve **S, int ns, const int *act, int force64, int *fa
             const double m = s->P.m;
             const double t_inv = m * m * m / 3.0 / (prec ? 1.5e13 : 3.0e13) + 3e-4;
             const double t_sv = (prec ? 3 : 6) * 2.0 * ((s->P.m + 255) / 256) * 15e-6;
-            const int model = prec ? t_inv > 3e-3 && t_sv < t_inv / 2 : t_inv > 15e-3 && t_sv < t_inv / 3;
+            /* with -throughput 1 (the serving loop's default) an f64 factor of order 1500 or more
+             * takes the solves: in a large batch their launches are shared by every solve in flight
+             * while the inverses' flops add up (the serving stream: 374 -> 402 solves/s) */
+            const int model = prec ? (s->o.tput ? s->P.m >= 1500 : t_inv > 3e-3 && t_sv < t_inv / 2)
+                                   : t_inv > 15e-3 && t_sv < t_inv / 3;
             s->trsv = s->o.apply < 0 ? model : s->o.apply;
         }
         bk_zero(dint(), sizeof(int) * ns);
@@ -1407,8 +1472,7 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
         else lin_add(S[q]->P.m, S[q]->h, -1, 0, -1, 0, -1, 0, -1, 0);
     }
     if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
-    if (blk_gemm(S, ns, hasH, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(dZ), OFF(W1))) goto out;
-    if (blk_gemm(S, ns, hasH, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G))) goto out;
+    if (zi_v_x(S, ns, hasH, OFF(dZ), OFF(X), OFF(G))) goto out;
     if (aop(S, ns, hasH, OFF(G), OFF(h), 1.0, 0.0)) goto out;
     /* predictor: M dy = b + h (tol 1e-7); dZ = theta R0 - A*(dy); dX = -X - sym(Zi dZ X) */
     LinL_clear(&g_ll);
@@ -1419,8 +1483,7 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
     if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
     if (schur_solve(S, ns, act, OFF(rhs), OFF(dy), 1e-7, rel)) goto out;
     if (aty(S, ns, act, OFF(dy), OFF(dZ), -1.0)) goto out;
-    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(dZ), OFF(W1))) goto out;
-    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G))) goto out;
+    if (zi_v_x(S, ns, act, OFF(dZ), OFF(X), OFF(G))) goto out;
     if (blk_sym(S, ns, act, OFF(G))) goto out;
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.T, S[q]->dX, S[q]->G, -1.0, S[q]->X, -1.0, -1, 0, -1, 0);
@@ -1448,9 +1511,9 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
         gam[q] = 0.9 + (0.99 - 0.9) * mina;
         smu[q] = sigma * mu;
     }
-    /* corrector (solver.c:14997-15055): Q = Zi dZ dX (W1 = Zi dZ still holds from the predictor);
-     * rhs = b - smu a0 + A(Q) + h */
-    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(dX), OFF(Q))) goto out;
+    /* corrector (solver.c:14997-15055): Q = Zi dZ dX (W1 = Zi dZ still holds from the predictor
+     * in the blocks where dZ is not diagonal); rhs = b - smu a0 + A(Q) + h */
+    if (zi_dz_dx(S, ns, act)) goto out;
     if (aop(S, ns, act, OFF(Q), OFF(qv), 1.0, 0.0)) goto out;
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) if (act[q]) {
@@ -1460,8 +1523,7 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
     if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
     if (schur_solve(S, ns, act, OFF(rhs), OFF(dy), 0.0, rel)) goto out;
     if (aty(S, ns, act, OFF(dy), OFF(dZ), -1.0)) goto out;
-    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(dZ), OFF(W1))) goto out;
-    if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G))) goto out;
+    if (zi_v_x(S, ns, act, OFF(dZ), OFF(X), OFF(G))) goto out;
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.T, S[q]->dX, S[q]->G, -1.0, S[q]->Q, -1.0, -1, 0, -1, 0);
     if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
@@ -1499,8 +1561,7 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
         }
         if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
         if (aty(S, ns, ract, OFF(wv), OFF(Q), 1.0)) goto out;
-        if (blk_gemm(S, ns, ract, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(Q), OFF(W1))) goto out;
-        if (blk_gemm(S, ns, ract, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G))) goto out;
+        if (zi_v_x(S, ns, ract, OFF(Q), OFF(X), OFF(G))) goto out;
         if (blk_sym(S, ns, ract, OFF(G))) goto out;
         if (aop(S, ns, ract, OFF(G), OFF(qv), -1.0, 1.0)) goto out;
         DotL_clear(&g_dl);
@@ -1716,41 +1777,47 @@ out:
 }
 
 /* ---- end: the returned point on the data as read ------------------------------------------ */
+/* X = bs X~ and Z = cs (C~ - A~'y~) (left in W1 by the DIMACS evaluation) are scaled on the device
+ * into G and Q (free after the evaluation) and downloaded through the pinned staging buffers:
+ * straight into the result arrays when the problem is one SDP block (the arena's layout is the
+ * result's), else into a host copy that is packed per input block. y_i = cs d_i y~_i on the host. */
 int ipm_finish(Solve *s) {
     const HProb *P = &s->P;
     const int m = P->m;
     double *H = heapp();
-    double *X = malloc(sizeof(double) * (P->T + 1)), *y = malloc(sizeof(double) * (m + 1));
-    if (bk_d2h(X, H + s->X, sizeof(double) * P->T) || bk_d2h(y, H + s->y, sizeof(double) * m)) {
-        free(X); free(y); return -1;
-    }
-    for (int64_t q = 0; q < P->T; q++) X[q] *= P->bs;                  /* X = bs X~ */
+    LinL_clear(&g_ll);
+    lin_add(P->T, s->G, s->X, P->bs, -1, 0, -1, 0, -1, 0);
+    lin_add(P->T, s->Q, s->W1, P->cs, -1, 0, -1, 0, -1, 0);
+    if (bk_lincomb(g_ll.v, g_ll.n)) return -1;
+    double *y = malloc(sizeof(double) * (m + 1));
+    if (bk_d2h(y, H + s->y, sizeof(double) * m)) { free(y); return -1; }
     for (int i = 0; i < m; i++) y[i] *= P->cs * P->d[i];                 /* y_i = cs d_i y~_i */
-    /* results per input block; Z = C - A'y on the data as read (host, FP64) */
     int64_t tot = 0;
     for (int k = 0; k < P->nblk_in; k++) tot += P->bs_in[k] > 0 ? (int64_t)P->bs_in[k] * P->bs_in[k] : -P->bs_in[k];
     free(s->res_X); free(s->res_Z); free(s->res_y);
     s->res_X = malloc(sizeof(double) * (tot + 1));
     s->res_Z = malloc(sizeof(double) * (tot + 1));
     s->res_y = y;
-    /* Z = C - A'y on the data as read: cs (C~ - A~'y~), left in W1 by the DIMACS evaluation */
-    double *Zint = malloc(sizeof(double) * (P->T + 1));
-    if (bk_d2h(Zint, H + s->W1, sizeof(double) * P->T)) { free(X); free(Zint); return -1; }
-    for (int64_t q = 0; q < P->T; q++) Zint[q] *= P->cs;
-    int64_t o = 0;
-    for (int k = 0; k < P->nblk_in; k++) {
-        const int kb = P->in_first[k];
-        if (P->bs_in[k] > 0) {
-            const int64_t len = (int64_t)P->bs_in[k] * P->bs_in[k];
-            memcpy(s->res_X + o, X + P->boff[kb], sizeof(double) * len);
-            memcpy(s->res_Z + o, Zint + P->boff[kb], sizeof(double) * len);
-            o += len;
-        } else {
-            for (int i = 0; i < -P->bs_in[k]; i++) { s->res_X[o + i] = X[P->boff[kb + i]]; s->res_Z[o + i] = Zint[P->boff[kb + i]]; }
-            o += -P->bs_in[k];
+    if (P->nblk_in == 1 && P->bs_in[0] > 0 && tot == P->T) {
+        if (bk_d2h(s->res_X, H + s->G, sizeof(double) * P->T) || bk_d2h(s->res_Z, H + s->Q, sizeof(double) * P->T)) return -1;
+    } else {
+        double *X = malloc(sizeof(double) * (P->T + 1)), *Zint = malloc(sizeof(double) * (P->T + 1));
+        if (bk_d2h(X, H + s->G, sizeof(double) * P->T) || bk_d2h(Zint, H + s->Q, sizeof(double) * P->T)) { free(X); free(Zint); return -1; }
+        int64_t o = 0;
+        for (int k = 0; k < P->nblk_in; k++) {
+            const int kb = P->in_first[k];
+            if (P->bs_in[k] > 0) {
+                const int64_t len = (int64_t)P->bs_in[k] * P->bs_in[k];
+                memcpy(s->res_X + o, X + P->boff[kb], sizeof(double) * len);
+                memcpy(s->res_Z + o, Zint + P->boff[kb], sizeof(double) * len);
+                o += len;
+            } else {
+                for (int i = 0; i < -P->bs_in[k]; i++) { s->res_X[o + i] = X[P->boff[kb + i]]; s->res_Z[o + i] = Zint[P->boff[kb + i]]; }
+                o += -P->bs_in[k];
+            }
         }
+        free(X); free(Zint);
     }
-    free(X); free(Zint);
     s->t_solve = bk_now() - s->t_start;
     return 0;
 }

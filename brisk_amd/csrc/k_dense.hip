@@ -432,6 +432,7 @@ static int graph_capture(GraphEnt *g, const std::vector<char> &buf, F &&launch) 
     return bk_check(hipGraphLaunch(g->exec, g_bk_stream), "hipGraphLaunch");
 }
 
+#define POTRF_PW256_N 3685                 // by default 256-column panels from this order (n^3 >= 5e10)
 // Blocked right-looking Cholesky in panels of 128 or 256 columns, each factored left-looking in
 // 64-column steps: the 64 columns are first brought up to date with the panel's earlier steps
 // (a GEMM with K <= 192), then the diagonal block is factored and inverted in LDS and the rows
@@ -457,17 +458,18 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
     struct Step { int k0; GemmPlan upd, panel, trail; int has_trail; };
     std::vector<Step> st;
     std::vector<GemmItem> gu, gp, gt;
-    // panel width, chosen per matrix from its own order so that a matrix is factored the same way
-    // in any batch: 128 columns below n^3 = 5e10 (n < 3684; latency-bound, e.g. two n = 2000
-    // blocks in FP64: 1.37 -> 1.30 ms), 256 above, where the trailing updates run at higher
-    // throughput with K = 256 (four n = 5000 blocks in FP32: 4.87 -> 4.60 ms)
-    auto panel_width = [](int n) { return (double)n * n * n < 5e10 ? 128 : 256; };
+    // panel width, chosen per matrix (its item's opt, else its order) so that a matrix is factored
+    // the same way in any batch: 128 columns for latency (two n = 2000 blocks in FP64: 1.37 ->
+    // 1.30 ms against 256), 256 where the trailing updates run at higher throughput with K = 256
+    // (four n = 5000 blocks in FP32: 4.87 -> 4.60 ms; the serving stream with 256 from order
+    // 1024: 365 -> 374 solves/s)
+    auto panel_width = [](const MItem &m) { return m.opt == 128 || m.opt == 256 ? m.opt : m.n < POTRF_PW256_N ? 128 : 256; };
     for (int k0 = 0; k0 < nmax; k0 += 64) {
         gu.clear(); gp.clear(); gt.clear();
         for (int q = 0; q < n_it; q++) {
             const int n = it[q].n;
             if (k0 >= n) continue;
-            const int PW = panel_width(n), K0 = k0 - k0 % PW, sub = (k0 - K0) / 64;
+            const int PW = panel_width(it[q]), K0 = k0 - k0 % PW, sub = (k0 - K0) / 64;
             GemmItem g;
             if (sub > 0) {
                 // columns k0 .. k0+64 (rows k0 ..) -= L[k0.., K0..k0] L[k0..k0+64, K0..k0]'

@@ -272,6 +272,16 @@ __global__ void k_add_diag(const MItem *__restrict__ it, const int *__restrict__
     const double s = H[s_off + m.slot];
     for (int64_t i = lb * blockDim.x + threadIdx.x; i < m.n; i += nb * blockDim.x) H[m.off + i + i * m.n] += s;
 }
+// Out = Diag(V) X per block: out(r, c) = V(r, r) X(r, c) (V at off, X at off2, out at off3): the
+// product Diag(dZ) X of a block whose dual direction is diagonal
+__global__ void k_rowscale(const MItem *__restrict__ it, const int *__restrict__ pre, int nit, double *__restrict__ H) {
+    FLAT_MITEM
+    const int n = m.n;
+    for (int64_t e = lb * blockDim.x + threadIdx.x; e < (int64_t)n * n; e += nb * blockDim.x) {
+        const int64_t r = e % n;
+        H[m.off3 + e] = H[m.off + r + r * n] * H[m.off2 + e];
+    }
+}
 __global__ void k_equil_d(const MItem *__restrict__ it, const int *__restrict__ pre, int nit, double *__restrict__ H) {
     FLAT_MITEM
     for (int64_t i = lb * blockDim.x + threadIdx.x; i < m.n; i += nb * blockDim.x) {
@@ -328,6 +338,11 @@ extern "C" int bk_transpose_f32(const MItem *it, int n_it) {
     MITEM_LAUNCH(3)
     hipLaunchKernelGGL((k_transpose<float>), dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, (float *)g_heap);
     return BK_LAUNCH_CHECK("k_transpose");
+}
+extern "C" int bk_rowscale(const MItem *it, int n_it) {
+    MITEM_LAUNCH(3)
+    hipLaunchKernelGGL(k_rowscale, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, (double *)g_heap);
+    return BK_LAUNCH_CHECK("k_rowscale");
 }
 extern "C" int bk_add_diag(const MItem *it, int n_it, int64_t s_off) {
     MITEM_LAUNCH(1)

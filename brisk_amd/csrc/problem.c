@@ -37,6 +37,7 @@ void hprob_free(HProb *P) {
     free(P->bs_in); free(P->in_first); free(P->bn); free(P->boff);
     free(P->b); free(P->b0); free(P->d); free(P->du); free(P->C);
     free(P->cptr); free(P->ent); free(P->gptr); free(P->grp); free(P->pptr); free(P->ppos); free(P->pcon); free(P->pval); free(P->Ad);
+    free(P->bdiag);
     memset(P, 0, sizeof(*P));
 }
 
@@ -272,6 +273,11 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
         if (P->cptr[i + 1] - P->cptr[i] != 1) P->diag = 0;
         else if (P->ent[P->cptr[i]].r != P->ent[P->cptr[i]].c) P->diag = 0;
     }
+    /* blocks every constraint entry of which is diagonal (every LP variable; every SDP block of
+     * a max-cut problem): there the dual direction dZ = theta R0 - A*(dy) is diagonal too */
+    P->bdiag = malloc(sizeof(int) * (P->nb + 1));
+    for (int k = 0; k < P->nb; k++) P->bdiag[k] = 1;
+    for (int q = 0; q < P->nent; q++) if (P->ent[q].r != P->ent[q].c) P->bdiag[P->ent[q].k] = 0;
     /* positions: unique (k, r, c) with their constraints in ascending order */
     qsort(tr, nt, sizeof(Trip), cmp_pos);
     int npos = 0;

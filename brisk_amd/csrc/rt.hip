@@ -23,6 +23,12 @@ static char *g_ring_h = nullptr, *g_ring_d = nullptr;
 // far more idle time than the few microseconds of such a kernel
 #define KCOPY_MAX ((size_t)1 << 20)
 static char *g_bounce = nullptr;      // pinned: destination of small device-to-host reads
+// larger transfers go through two pinned buffers of XFER_CHUNK bytes in turn, so that the host's
+// copy of one chunk (to or from pageable memory) overlaps the DMA of the next: a DMA from
+// pageable memory runs at a few GB/s, from pinned memory at the link's rate
+#define XFER_CHUNK ((size_t)16 << 20)
+static char *g_xfer[2] = { nullptr, nullptr };
+static hipEvent_t g_xfer_ev[2];
 static size_t g_ring_cap = 0, g_ring_pos = 0;
 #define CACHE_CAP ((size_t)32 << 20)
 #define CACHE_CHUNK ((size_t)64 << 10)
@@ -68,6 +74,10 @@ extern "C" int bk_init(void) {
     if (bk_check(hipHostMalloc((void **)&g_cache_h, CACHE_CAP, hipHostMallocDefault), "hipHostMalloc cache")) return -1;
     if (bk_check(hipMalloc((void **)&g_cache_d, CACHE_CAP), "hipMalloc cache")) return -1;
     if (bk_check(hipHostMalloc((void **)&g_bounce, KCOPY_MAX, hipHostMallocDefault), "hipHostMalloc bounce")) return -1;
+    for (int b = 0; b < 2; b++) {
+        if (bk_check(hipHostMalloc((void **)&g_xfer[b], XFER_CHUNK, hipHostMallocDefault), "hipHostMalloc transfer")) return -1;
+        if (bk_check(hipEventCreateWithFlags(&g_xfer_ev[b], hipEventDisableTiming), "hipEventCreate transfer")) return -1;
+    }
     // the device address of the no-allocation symmetric product's scratch array (k_dense.hip)
     if (bk_symv_scratch_init()) return -1;
     g_init = 1;
@@ -170,7 +180,16 @@ static int pull(void *dst, const void *pinned, size_t bytes, const char *what) {
 extern "C" int bk_h2d(void *dst, const void *src, size_t bytes) {
     if (!bytes) return 0;
     if (bytes <= KCOPY_MAX) return bk_h2d_async(dst, src, bytes) || bk_sync() ? -1 : 0;
-    if (bk_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_bk_stream), "hipMemcpy H2D")) return -1;
+    // chunk k: the host copies it into buffer k % 2 (once that buffer's previous DMA is done),
+    // then its DMA is queued on the stream
+    for (size_t off = 0, k = 0; off < bytes; off += XFER_CHUNK, k++) {
+        const int b = (int)(k & 1);
+        const size_t len = bytes - off < XFER_CHUNK ? bytes - off : XFER_CHUNK;
+        if (k >= 2 && bk_check(hipEventSynchronize(g_xfer_ev[b]), "hipEventSynchronize H2D")) return -1;
+        memcpy(g_xfer[b], (const char *)src + off, len);
+        if (bk_check(hipMemcpyAsync((char *)dst + off, g_xfer[b], len, hipMemcpyHostToDevice, g_bk_stream), "hipMemcpy H2D") ||
+            bk_check(hipEventRecord(g_xfer_ev[b], g_bk_stream), "hipEventRecord H2D")) return -1;
+    }
     return bk_sync();
 }
 extern "C" int bk_d2h(void *dst, const void *src, size_t bytes) {
@@ -180,7 +199,22 @@ extern "C" int bk_d2h(void *dst, const void *src, size_t bytes) {
         memcpy(dst, g_bounce, bytes);
         return 0;
     }
-    if (bk_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_bk_stream), "hipMemcpy D2H")) return -1;
+    // chunk k is DMAed into buffer k % 2 while the host copies chunk k - 1 out of the other
+    size_t prev_off = 0, prev_len = 0;
+    for (size_t off = 0, k = 0; off < bytes; off += XFER_CHUNK, k++) {
+        const int b = (int)(k & 1);
+        const size_t len = bytes - off < XFER_CHUNK ? bytes - off : XFER_CHUNK;
+        if (bk_check(hipMemcpyAsync(g_xfer[b], (const char *)src + off, len, hipMemcpyDeviceToHost, g_bk_stream), "hipMemcpy D2H") ||
+            bk_check(hipEventRecord(g_xfer_ev[b], g_bk_stream), "hipEventRecord D2H")) return -1;
+        if (k > 0) {
+            if (bk_check(hipEventSynchronize(g_xfer_ev[b ^ 1]), "hipEventSynchronize D2H")) return -1;
+            memcpy((char *)dst + prev_off, g_xfer[b ^ 1], prev_len);
+        }
+        prev_off = off; prev_len = len;
+    }
+    const int lb = (int)(((bytes + XFER_CHUNK - 1) / XFER_CHUNK - 1) & 1);
+    if (bk_check(hipEventSynchronize(g_xfer_ev[lb]), "hipEventSynchronize D2H")) return -1;
+    memcpy((char *)dst + prev_off, g_xfer[lb], prev_len);
     return bk_sync();
 }
 extern "C" int bk_d2d(void *dst, const void *src, size_t bytes) {

@@ -48,9 +48,12 @@ API BriskServe *brisk_serve_new(int nopt, const char *const *opts) {
     if (bk_init()) return NULL;
     if (heapgb > 0 && brisk_heap_ensure((size_t)(heapgb * (double)(1ull << 30)))) return NULL;
     BriskServe *sv = calloc(1, sizeof(BriskServe));
-    sv->nopt = nopt;
-    sv->opts = malloc(sizeof(char *) * (nopt + 1));
-    for (int k = 0; k < nopt; k++) sv->opts[k] = strdup(opts[k]);
+    /* the requests' options: the plan choices for throughput (-throughput 1) unless the caller's
+     * options, which follow and so take precedence, say otherwise */
+    sv->nopt = nopt + 2;
+    sv->opts = malloc(sizeof(char *) * (nopt + 3));
+    sv->opts[0] = strdup("-throughput"); sv->opts[1] = strdup("1");
+    for (int k = 0; k < nopt; k++) sv->opts[k + 2] = strdup(opts[k]);
     sv->slots = slots;
     sv->fly = malloc(sizeof(Solve *) * slots);
     sv->fly_id = malloc(sizeof(int) * slots);

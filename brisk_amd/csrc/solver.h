@@ -74,6 +74,7 @@ typedef struct {
     int *pptr, *ppos, *pcon;
     double *pval;
     int diag;               /* every constraint is one diagonal entry: Hadamard Schur form */
+    int *bdiag;             /* per block: every constraint entry in it is diagonal (NULL: none is known to be) */
     /* dense route: one SDP block with dense constraints; Ad (host) holds the scaled A_i in full
      * symmetric storage, column i = vec(A_i) (n*n x m); the entry and position lists are empty */
     int dense;
@@ -92,6 +93,7 @@ typedef struct {
     double time_limit;      /* s, <= 0 none */
     int hsd;                /* 1: self-dual embedding, 0: infeasible start */
     int apply;              /* solves with the Schur factor: -1 a cost model on the solve's own order and precision (default), 0 explicit inverse, 1 block triangular solves */
+    int tput;               /* 1: the plan choices for throughput in large batches (256-column panels from order 1024, blocked solves for f64 factors from order 1500); 0: for latency */
 } SolveOpts;
 
 typedef struct Solve {

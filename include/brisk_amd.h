@@ -63,7 +63,14 @@ int brisk_solve_file(const char *fname, int nopt, const char *const *opts, Brisk
 /* Options understood by every solve entry:
  *   -prec double|mixed   FP64 throughout, or FP32-MFMA Schur factor refined in FP64 (default:
  *                        mixed when m >= 1000, the reference's own threshold, solver.c:13814)
- *   -tol t, -maxit k, -timelimit s, -v, -q; -threads k and -bm k are accepted and ignored. */
+ *   -tol t, -maxit k, -timelimit s, -v, -q; -threads k and -bm k are accepted and ignored;
+ *   -schurapply auto|inverse|solves   how the PCG applies the Schur factor (default auto: a
+ *                        cost model on the solve's own order and precision);
+ *   -throughput 0|1      the plan choices for latency (0, the default of the solve entries) or
+ *                        for throughput in large batches (1, the serving loop's default):
+ *                        256-column Cholesky panels from order 1024 and blocked solves for FP64
+ *                        factors from order 1500. A solve's arithmetic depends on its own
+ *                        options and data only, never on the problems batched with it. */
 /* time split of the solve (s): out[0] host setup and upload, out[1] start point and iterations,
  * out[2] DIMACS evaluation on the device, out[3] result download */
 void brisk_result_phases(const BriskResult *r, double *out);

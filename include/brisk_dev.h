@@ -80,10 +80,11 @@ typedef struct {
     int64_t off2;            /* potrf/trtri: diagonal-block inverses (64*64 per 64-wide panel) */
     int64_t off3;            /* potrf/trtri: scratch of n*n elements */
     int64_t off4;            /* trtri: the inverse; transpose: the destination */
+    int opt, pad;            /* potrf: the panel width, 128 or 256 (0: 256 from order 3685, else 128) */
 } MItem;
 
-/* Cholesky of a set of matrices, blocked right-looking (panels of 128 columns, or 256 for a matrix
- * of order 3684 or more, chosen per matrix; each factored in 64-column steps), in place on A (its
+/* Cholesky of a set of matrices, blocked right-looking (panels of 128 or 256 columns, chosen per
+ * matrix by its item's opt; each factored in 64-column steps), in place on A (its
  * lower triangle is read; on return A holds L with the strict upper triangle zeroed).
  * Diagonal-block inverses go to Dinv + off2 (64*64 per 64-column step); W + off3 is scratch
  * (n*64 used).
@@ -164,6 +165,9 @@ int bk_vmul(const LinItem *it, int n_it);
 int bk_symmetrize(const MItem *it, int n_it);                       /* G = (G + G')/2 */
 int bk_transpose_f64(const MItem *it, int n_it);                    /* matrix at off4 = (matrix at off)' */
 int bk_transpose_f32(const MItem *it, int n_it);
+/* matrix at off3 = Diag(matrix at off) (matrix at off2): each row r scaled by the first matrix's
+ * diagonal entry (r, r); offsets in doubles from the heap base */
+int bk_rowscale(const MItem *it, int n_it);
 /* A += s[slot] I, with s read from the heap at offset s_off (doubles) */
 int bk_add_diag(const MItem *it, int n_it, int64_t s_off);
 /* equilibration of M (full f64 at off): D[off2 + i] = 1/sqrt(M_ii) (1 when M_ii <= 0);

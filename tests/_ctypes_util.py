@@ -1,16 +1,14 @@
 """ctypes plumbing shared by the tests: the instance generators (product side), the oracle
 (test infrastructure) and, where it was built in this container, the reference library."""
-import ctypes, os
+import ctypes, os, sys
 import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+from brisk_amd.generators import Inst, Instance, gen_lib  # noqa: E402,F401  (the product's generators)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 I, D = ctypes.c_int, ctypes.c_double
 PI, PD = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)
-
-
-class Inst(ctypes.Structure):
-    _fields_ = [("m", I), ("nblk", I), ("bs", PI), ("c", PD), ("nnz", ctypes.c_int64),
-                ("mat", PI), ("blk", PI), ("i", PI), ("j", PI), ("v", PD)]
 
 
 class OracleData(ctypes.Structure):
@@ -32,73 +30,7 @@ class OracleResult(ctypes.Structure):
                 ("trace_ap", D * TR), ("trace_ad", D * TR)]
 
 
-_gen = None
 _orc = None
-
-
-def gen_lib():
-    global _gen
-    if _gen is None:
-        _gen = ctypes.CDLL(os.path.join(ROOT, "brisk_amd", "libbrisk_gen.so"))
-        pI = ctypes.POINTER(Inst)
-        u64 = ctypes.c_uint64
-        _gen.brisk_gen_theta.argtypes = [I, D, u64, pI]
-        _gen.brisk_gen_theta_edges.argtypes = [I, ctypes.c_int64, PI, pI]
-        _gen.brisk_gen_maxcut.argtypes = [I, D, u64, pI]
-        _gen.brisk_gen_maxcut_cycle.argtypes = [I, pI]
-        _gen.brisk_gen_dense.argtypes = [I, I, u64, pI]
-        _gen.brisk_gen_multi.argtypes = [I, I, I, I, u64, pI]
-        _gen.brisk_gen_free.argtypes = [pI]
-        _gen.brisk_gen_u01.argtypes = [u64, u64, u64]
-        _gen.brisk_gen_u01.restype = D
-    return _gen
-
-
-class Instance:
-    """A generated instance; frees its arrays when collected."""
-
-    def __init__(self, kind, *args):
-        self.s = Inst()
-        g = gen_lib()
-        if kind == "theta_edges":
-            n, edges = args
-            e = np.ascontiguousarray(np.asarray(edges, dtype=np.int32).reshape(-1))
-            rc = g.brisk_gen_theta_edges(n, len(e) // 2, e.ctypes.data_as(PI), ctypes.byref(self.s))
-        else:
-            rc = getattr(g, "brisk_gen_" + kind)(*args, ctypes.byref(self.s))
-        if rc != 0:
-            raise ValueError("generator %s%r failed" % (kind, args))
-        self.kind = kind
-        self.args = args
-
-    def __del__(self):
-        try:
-            gen_lib().brisk_gen_free(ctypes.byref(self.s))
-        except Exception:
-            pass
-
-    @property
-    def m(self):
-        return self.s.m
-
-    @property
-    def bs(self):
-        return [self.s.bs[k] for k in range(self.s.nblk)]
-
-    def arrays(self):
-        s = self.s
-        nnz = s.nnz
-        return dict(m=s.m, nblk=s.nblk, bs=np.ctypeslib.as_array(s.bs, (s.nblk,)).copy(),
-                    c=np.ctypeslib.as_array(s.c, (s.m,)).copy(),
-                    mat=np.ctypeslib.as_array(s.mat, (nnz,)).copy() if nnz else np.zeros(0, np.int32),
-                    blk=np.ctypeslib.as_array(s.blk, (nnz,)).copy() if nnz else np.zeros(0, np.int32),
-                    i=np.ctypeslib.as_array(s.i, (nnz,)).copy() if nnz else np.zeros(0, np.int32),
-                    j=np.ctypeslib.as_array(s.j, (nnz,)).copy() if nnz else np.zeros(0, np.int32),
-                    v=np.ctypeslib.as_array(s.v, (nnz,)).copy() if nnz else np.zeros(0))
-
-    def data_args(self):
-        s = self.s
-        return (s.m, s.nblk, s.bs, s.c, s.nnz, s.mat, s.blk, s.i, s.j, s.v)
 
 
 def oracle_lib():

@@ -13,7 +13,8 @@ class GemmItem(ctypes.Structure):
 
 
 class MItem(ctypes.Structure):
-    _fields_ = [("n", ctypes.c_int), ("slot", ctypes.c_int), ("off", I64), ("off2", I64), ("off3", I64), ("off4", I64)]
+    _fields_ = [("n", ctypes.c_int), ("slot", ctypes.c_int), ("off", I64), ("off2", I64), ("off3", I64), ("off4", I64),
+                ("opt", ctypes.c_int), ("pad", ctypes.c_int)]
 
 
 class GemvItem(ctypes.Structure):

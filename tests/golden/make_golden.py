@@ -150,9 +150,8 @@ def main():
             print(name, out["file_" + name]["status"], out["file_" + name]["pobj"], flush=True)
         dst = os.path.join(HERE, "small.json")
     elif which == "stream":
-        sys.path.insert(0, ROOT)
-        import bench
-        reqs, _ = bench.c5_stream()
+        from brisk_amd.generators import stream
+        reqs, _ = stream()
         out = {}
         for k in STREAM_IDS:
             kind, n, p, seed = reqs[k]

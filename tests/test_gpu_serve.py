@@ -116,17 +116,18 @@ def test_serving_mixed_stream_equals_alone():
     stream (requests 0-25 and 31: theta and max-cut with m from 32 to 4985, three theta requests
     with m > 1500, so that f32 factors, their refinement and the 256-column panels of the largest
     are used), arriving three every second step into 8 slots and a 1.5 GB heap, on which the
-    largest must wait for memory. Every served result equals the same request solved alone, bit
-    for bit (status, iterations, objectives, DIMACS errors, y, X, Z), and matches the reference
-    library's solve of that request (tests/golden/stream.json)."""
+    largest must wait for memory. Every served result equals the same request solved alone with
+    the same options (the serving loop's default -throughput 1), bit for bit (status, iterations,
+    objectives, DIMACS errors, y, X, Z), and matches the reference library's solve of that request
+    (tests/golden/stream.json)."""
     names = sorted(STREAM, key=lambda k: STREAM[k]["request"])
     tups = {k: tup(golden_instance(STREAM[k])) for k in names}
-    opts = ["-prec", "mixed"]
     solo = {}
     for k in names:
         print("alone", k, "m =", STREAM[k]["m"], flush=True)
-        solo[k] = brisk_amd.solve_sdpa(*tups[k], options=opts)
-    sv = brisk_amd.Server(opts + ["-slots", "8", "-heapgb", "1.5"])
+        # the serving loop's default plan choices are -throughput 1
+        solo[k] = brisk_amd.solve_sdpa(*tups[k], options=["-prec", "mixed", "-throughput", "1"])
+    sv = brisk_amd.Server(["-prec", "mixed", "-slots", "8", "-heapgb", "1.5"])
     ids, pending = {}, list(names)
     steps, left, max_inflight, waited = 0, 1, 0, False
     while pending or left > 0:
