@@ -221,14 +221,18 @@ static int solve_many(int n, const BriskProblem *pr, int nopt, const char *const
         SP[q] = &S[q];
         S[q].id = q;
     }
+    const double tb = bk_now();
     if (bk_init()) { say(1, "brisk: no usable GPU: %s\n", bk_error()); rc = -3; goto out; }
     if (brisk_heap_ensure(need + ((size_t)8 << 20))) { say(1, "brisk: device memory: %zu bytes needed\n", need); rc = -2; goto out; }
+    const double th = bk_now();
     for (int q = 0; q < n; q++) {
         if (solve_setup(&S[q])) { say(1, "brisk: device setup failed: %s\n", bk_error()); rc = -2; goto out; }
         setup = q + 1;
     }
     bk_sync();
     const double t1 = bk_now();
+    if (S[0].o.verbose)
+        say(0, "setup: problem build %.4f s, device and heap %.4f s, upload %.4f s\n", tb - t0, th - tb, t1 - th);
     if (ipm_start_batch(SP, n)) { rc = -4; goto out; }
     for (int guard = 0; guard < 10000; guard++) {
         int any = 0;

@@ -137,6 +137,7 @@ int64_t solve_heap_bytes(const HProb *P) {
     b += 22 * (m * 8 + 256) + NSC * 8 + 256;
     b += (m + 1) * 4 + (int64_t)P->nent * (sizeof(SEnt) + 12) + (int64_t)P->nb * sizeof(BlkRef) + (P->npos + 1) * 12 + 2048;
     b += (m + 1) * 4 + (int64_t)(P->ngrp + 1) * sizeof(GrpRef) + 2 * (P->gp_tot * 8 + 256) + 512;
+    b += P->npair * 8 + 256;
     if (P->want_hsd) b += 6 * (T * 8 + 256) + 5 * (m * 8 + 256);
     if (P->dense) {
         const int64_t nn = T;
@@ -177,6 +178,7 @@ int solve_setup(Solve *S) {
     ALLOC(pval, (int64_t)(P->nent + 1) * 8);
     ALLOCB(gptr, (m + 1) * 4); ALLOCB(grp, (int64_t)(P->ngrp + 1) * sizeof(GrpRef));
     if (P->gp_tot > 0) { ALLOC(Pg, P->gp_tot * 8); ALLOC(Tg, P->gp_tot * 8); }
+    if (P->npair > 0) ALLOCB(pairs, P->npair * 8);
     S->hsd = S->o.hsd;
     if (S->hsd) {
         ALLOC(Rd, T * 8); ALLOC(CX, T * 8); ALLOC(dX0, T * 8); ALLOC(dZ0, T * 8); ALLOC(dX1, T * 8); ALLOC(dZ1, T * 8);
@@ -220,6 +222,11 @@ int solve_setup(Solve *S) {
         rc |= bk_h2d_async(H + S->grp, gr, (int64_t)P->ngrp * sizeof(GrpRef));
         free(gr);
     }
+    if (P->npair > 0) {
+        /* the pairs, and M's zeros outside them (the assembly writes the pairs only) */
+        rc |= bk_h2d_async(H + S->pairs, P->pairs, P->npair * 8);
+        rc |= bk_zero(H + S->M * 8, (size_t)m * m * 8);
+    }
     if (P->dense) {
         rc |= bk_h2d_async(H + S->Ad * 8, P->Ad, T * m * 8);
         rc |= bk_d2f(T * m, S->Ad, 2 * S->Af);
@@ -249,6 +256,7 @@ static ConItem con_item(Solve *S) {
     c.pptr = S->pptr; c.ppos = S->ppos; c.pcon = S->pcon; c.pval = S->pval;
     c.gptr = S->gptr; c.grp = S->grp;
     c.npos = S->P.npos; c.diag = S->P.diag;
+    c.pairs = S->pairs; c.npair = S->P.npair;
     return c;
 }
 /* out = alpha A(D) + beta out, D and out given as double offsets per solve */

@@ -484,6 +484,47 @@ __device__ __forceinline__ double group_dot(const SEnt *__restrict__ e, int cnt,
     }
     return s;
 }
+// M_ab over the common blocks of constraints a and b: their group lists (ascending block) merged;
+// a heavy group's stored product against the other's entries, else the entry pairs
+__device__ __forceinline__ double pair_sum(const int *__restrict__ gptr, const GrpRef *__restrict__ grp, const SEnt *__restrict__ ent,
+                                           const BlkRef *__restrict__ blk, const double *__restrict__ H, const double *X,
+                                           const double *Zi, int a, int b) {
+    double s = 0;
+    int pa = gptr[a], ea = gptr[a + 1], pb = gptr[b], eb = gptr[b + 1];
+    while (pa < ea && pb < eb) {
+        const GrpRef Ga = grp[pa], Gb = grp[pb];
+        if (Ga.k < Gb.k) { pa++; continue; }
+        if (Gb.k < Ga.k) { pb++; continue; }
+        const BlkRef B = blk[Ga.k];
+        if (Ga.poff >= 0 && (Gb.poff < 0 || Gb.cnt <= Ga.cnt)) s += group_dot(ent + Gb.beg, Gb.cnt, H + Ga.poff, B.n);
+        else if (Gb.poff >= 0) s += group_dot(ent + Ga.beg, Ga.cnt, H + Gb.poff, B.n);
+        else {
+            const double *Xk = X + B.off, *Zk = Zi + B.off;
+            for (int u = 0; u < Ga.cnt; u++) {
+                const SEnt P = ent[Ga.beg + u];
+                for (int w = 0; w < Gb.cnt; w++) s += pair_term(P, ent[Gb.beg + w], Xk, Zk, B.n);
+            }
+        }
+        pa++; pb++;
+    }
+    return s;
+}
+// the same pairs, listed: one thread per pair (a, b), a >= b, of constraints that share a block
+// (by column b, rows a ascending), the others left at the zeros written at setup. For problems
+// of many blocks, where few pairs share one (64 blocks of 60: 6%); the sums are pair_sum's.
+__global__ __launch_bounds__(256) void k_schur_pairs(const ConItem *__restrict__ it, const int *__restrict__ pre, int nit,
+                                                     double *__restrict__ H) {
+    const int q = flat_item(pre, nit, blockIdx.x);
+    const ConItem C = it[q];
+    const int64_t idx = (int64_t)(blockIdx.x - pre[q]) * 256 + threadIdx.x;
+    if (idx >= C.npair) return;
+    const int *pr = HI(H, C.pairs);
+    const int a = pr[2 * idx], b = pr[2 * idx + 1];
+    const double s = pair_sum(HI(H, C.gptr), (const GrpRef *)(((const char *)H) + C.grp), (const SEnt *)(((const char *)H) + C.ent),
+                              (const BlkRef *)(((const char *)H) + C.blk), H, H + C.X, H + C.Zi, a, b);
+    H[C.M + a + (int64_t)b * C.m] = s;
+    if (a != b) H[C.M + b + (int64_t)a * C.m] = s;
+}
 __global__ __launch_bounds__(256) void k_schur_r0(const ConItem *__restrict__ it, const int *__restrict__ pre, int nit,
                                                   int64_t ntiles, double *__restrict__ H) {
     __shared__ double tile[16][17];
@@ -530,23 +571,7 @@ __global__ __launch_bounds__(256) void k_schur_r0(const ConItem *__restrict__ it
                 }
             } else {
                 // merge the two group lists (ascending block) over the common blocks
-                int pa = gbeg[i], ea = pa + na, pb = gbeg[16 + j], eb = pb + nb;
-                while (pa < ea && pb < eb) {
-                    const GrpRef Ga = grp[pa], Gb = grp[pb];
-                    if (Ga.k < Gb.k) { pa++; continue; }
-                    if (Gb.k < Ga.k) { pb++; continue; }
-                    const BlkRef B = blk[Ga.k];
-                    if (Ga.poff >= 0 && (Gb.poff < 0 || Gb.cnt <= Ga.cnt)) s += group_dot(ent + Gb.beg, Gb.cnt, H + Ga.poff, B.n);
-                    else if (Gb.poff >= 0) s += group_dot(ent + Ga.beg, Ga.cnt, H + Gb.poff, B.n);
-                    else {
-                        const double *Xk = X + B.off, *Zk = Zi + B.off;
-                        for (int u = 0; u < Ga.cnt; u++) {
-                            const SEnt P = ent[Ga.beg + u];
-                            for (int w = 0; w < Gb.cnt; w++) s += pair_term(P, ent[Gb.beg + w], Xk, Zk, B.n);
-                        }
-                    }
-                    pa++; pb++;
-                }
+                s = pair_sum(gptr, grp, ent, blk, H, X, Zi, a, b);
             }
             H[C.M + a + (int64_t)b * C.m] = s;
         }
@@ -607,22 +632,24 @@ extern "C" int bk_grp_t(const GrpItem *it, int n_it) {
 }
 extern "C" int bk_schur(const ConItem *it, int n_it) {
     if (n_it <= 0) return 0;
-    std::vector<int64_t> cd(n_it), cr(n_it);
-    int anyd = 0, anyr = 0;
+    std::vector<int64_t> cd(n_it), cr(n_it), cp(n_it);
+    int anyd = 0, anyr = 0, anyp = 0;
     for (int q = 0; q < n_it; q++) {
         const int64_t m = it[q].m, t = (m + 15) / 16;
+        const int pl_ = !it[q].diag && it[q].npair > 0;
         cd[q] = it[q].diag ? ((m + 255) / 256) * m : 0;
-        cr[q] = it[q].diag ? 0 : t * (t + 1) / 2;
-        anyd |= it[q].diag != 0; anyr |= it[q].diag == 0;
+        cr[q] = it[q].diag || pl_ ? 0 : t * (t + 1) / 2;
+        cp[q] = pl_ ? (it[q].npair + 255) / 256 : 0;
+        anyd |= it[q].diag != 0; anyr |= cr[q] > 0; anyp |= pl_;
     }
-    // items and both prefix arrays in one staged copy
+    // items and the three prefix arrays in one staged copy
     const size_t po = (sizeof(ConItem) * n_it + 15) & ~(size_t)15, pl = sizeof(int) * (n_it + 1);
-    std::vector<char> buf(po + 2 * pl);
+    std::vector<char> buf(po + 3 * pl);
     memcpy(buf.data(), it, sizeof(ConItem) * n_it);
-    int64_t tot[2] = { 0, 0 };
-    for (int w = 0; w < 2; w++) {
+    int64_t tot[3] = { 0, 0, 0 };
+    for (int w = 0; w < 3; w++) {
         int *pre = (int *)(buf.data() + po + w * pl);
-        for (int q = 0; q < n_it; q++) { pre[q] = (int)tot[w]; tot[w] += w ? cr[q] : cd[q]; }
+        for (int q = 0; q < n_it; q++) { pre[q] = (int)tot[w]; tot[w] += w == 2 ? cp[q] : w ? cr[q] : cd[q]; }
         pre[n_it] = (int)tot[w];
         if (tot[w] > 0x7fffffff) return -1;
     }
@@ -638,6 +665,11 @@ extern "C" int bk_schur(const ConItem *it, int n_it) {
         const unsigned g = (unsigned)(tot[1] < 8192 ? tot[1] : 8192);
         hipLaunchKernelGGL(k_schur_r0, dim3(g), dim3(256), 0, g_bk_stream, d, (const int *)(dv + po + pl), n_it, tot[1], (double *)g_heap);
         if (BK_LAUNCH_CHECK("k_schur_r0")) return -1;
+    }
+    if (anyp && tot[2] > 0) {
+        hipLaunchKernelGGL(k_schur_pairs, dim3((unsigned)tot[2]), dim3(256), 0, g_bk_stream, d, (const int *)(dv + po + 2 * pl), n_it,
+                           (double *)g_heap);
+        if (BK_LAUNCH_CHECK("k_schur_pairs")) return -1;
     }
     if (tok >= 0) {
         double by = 0, fl = 0;

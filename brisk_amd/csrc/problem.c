@@ -12,9 +12,64 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include "par.h"
 #include "solver.h"
 
 typedef struct { int con, k, r, c; double v; } Trip;
+
+/* ---- the setup passes over the entries, in parallel on the host cores ---- */
+typedef struct {
+    int m, nblk, n;
+    const int *bs, *mat, *blk, *ii, *jj;
+    const double *v;
+    double *Ad, *C, *d;
+    const int64_t *cbeg;            /* dense route: entries of matrix t at [cbeg[t], cbeg[t + 1]) */
+    int bad[16];
+    int64_t ncon[16];
+    int unsorted[16];
+} SetupCtx;
+/* the entries valid (indices within their blocks, LP entries diagonal, finite values); the
+ * count of nonzero constraint entries; whether the matrix numbers never decrease */
+static void check_range(void *c_, int64_t lo, int64_t hi, int part) {
+    SetupCtx *c = c_;
+    int bad = 0, uns = 0;
+    int64_t ncon = 0;
+    for (int64_t q = lo; q < hi; q++) {
+        const int t = c->mat[q], k = c->blk[q] - 1;
+        if (t < 0 || t > c->m || k < 0 || k >= c->nblk) { bad = 1; break; }
+        const int n = abs(c->bs[k]);
+        if (c->ii[q] < 1 || c->jj[q] < 1 || c->ii[q] > n || c->jj[q] > n) { bad = 1; break; }
+        if (c->bs[k] < 0 && c->ii[q] != c->jj[q]) { bad = 1; break; }
+        if (!isfinite(c->v[q])) { bad = 1; break; }
+        if (c->v[q] != 0 && t > 0) ncon++;
+        if (q > 0 && c->mat[q] < c->mat[q - 1]) uns = 1;
+    }
+    c->bad[part] = bad; c->ncon[part] = ncon; c->unsorted[part] = uns;
+}
+/* dense route, matrices lo .. hi - 1 (entries grouped by matrix): each column of Ad built from
+ * its entries (full symmetric storage), its norm taken and the column scaled by d = 1/norm */
+static void dense_columns(void *c_, int64_t lo, int64_t hi, int part) {
+    SetupCtx *c = c_;
+    (void)part;
+    const int n = c->n;
+    const int64_t nn = (int64_t)n * n;
+    for (int64_t t = lo; t < hi; t++) {
+        double *A = c->Ad + t * nn;
+        memset(A, 0, sizeof(double) * nn);
+        for (int64_t q = c->cbeg[t + 1]; q < c->cbeg[t + 2]; q++) {
+            if (c->v[q] == 0) continue;
+            int i = c->ii[q] - 1, j = c->jj[q] - 1;
+            if (i > j) { const int s = i; i = j; j = s; }
+            A[i + (int64_t)j * n] += c->v[q];
+            if (i != j) A[j + (int64_t)i * n] += c->v[q];
+        }
+        double s2 = 0;
+        for (int64_t q = 0; q < nn; q++) s2 += A[q] * A[q];
+        const double dt = s2 > 0 ? 1.0 / sqrt(s2) : 1.0;
+        c->d[t] = dt;
+        for (int64_t q = 0; q < nn; q++) A[q] *= dt;
+    }
+}
 
 static int cmp_con(const void *a, const void *b) {
     const Trip *x = a, *y = b;
@@ -37,38 +92,61 @@ void hprob_free(HProb *P) {
     free(P->bs_in); free(P->in_first); free(P->bn); free(P->boff);
     free(P->b); free(P->b0); free(P->d); free(P->du); free(P->C);
     free(P->cptr); free(P->ent); free(P->gptr); free(P->grp); free(P->pptr); free(P->ppos); free(P->pcon); free(P->pval); free(P->Ad);
-    free(P->bdiag);
+    free(P->bdiag); free(P->pairs);
     memset(P, 0, sizeof(*P));
 }
 
-static void finish_scaling(HProb *P);
-/* dense route: the scaled A_i in full symmetric storage, column i of Ad = vec(A_i) */
-static int hprob_build_dense(HProb *P, int m, int n, int64_t nnz, const int *mat, const int *blk, const int *ii, const int *jj, const double *v) {
+/* dense route: the scaled A_i in full symmetric storage, column i of Ad = vec(A_i); with the
+ * entries grouped by matrix (as an SDPA file lists them) the columns are built, measured and
+ * scaled in parallel, each by one thread while it is in cache */
+static void finish_scaling(HProb *P, int have_d);
+static int hprob_build_dense(HProb *P, int m, int n, int64_t nnz, const int *mat, const int *blk, const int *ii, const int *jj,
+                             const double *v, int sorted) {
     const int64_t nn = (int64_t)n * n;
     P->dense = 1;
-    P->Ad = calloc((size_t)nn * m + 1, sizeof(double));
+    P->Ad = malloc(sizeof(double) * ((size_t)nn * m + 1));
     if (!P->Ad) return -1;
-    for (int64_t q = 0; q < nnz; q++) {
-        if (v[q] == 0) continue;
-        int i = ii[q] - 1, j = jj[q] - 1;
-        if (i > j) { const int s = i; i = j; j = s; }
-        (void)blk;
-        if (mat[q] == 0) {
-            P->C[i + (int64_t)j * n] += -v[q];
-            if (i != j) P->C[j + (int64_t)i * n] += -v[q];
-            continue;
+    (void)blk;
+    int have_d = 0;
+    if (sorted) {
+        /* cbeg[t + 1] .. cbeg[t + 2]: the entries of matrix t (cbeg[0] .. cbeg[1]: F0) */
+        int64_t *cbeg = malloc(sizeof(int64_t) * (m + 3));
+        int64_t q = 0;
+        for (int t = 0; t <= m + 1; t++) {
+            while (q < nnz && mat[q] < t) q++;
+            cbeg[t] = q;
         }
-        double *A = P->Ad + (int64_t)(mat[q] - 1) * nn;
-        A[i + (int64_t)j * n] += v[q];
-        if (i != j) A[j + (int64_t)i * n] += v[q];
+        cbeg[m + 2] = nnz;
+        for (int64_t e = cbeg[0]; e < cbeg[1]; e++) {          /* C = -F0 */
+            if (v[e] == 0) continue;
+            int i = ii[e] - 1, j = jj[e] - 1;
+            if (i > j) { const int s = i; i = j; j = s; }
+            P->C[i + (int64_t)j * n] += -v[e];
+            if (i != j) P->C[j + (int64_t)i * n] += -v[e];
+        }
+        SetupCtx c;
+        memset(&c, 0, sizeof(c));
+        c.n = n; c.ii = ii; c.jj = jj; c.v = v; c.Ad = P->Ad; c.d = P->d; c.cbeg = cbeg;
+        par_for(m, 16, dense_columns, &c);
+        free(cbeg);
+        have_d = 1;
+    } else {
+        memset(P->Ad, 0, sizeof(double) * (size_t)nn * m);
+        for (int64_t q = 0; q < nnz; q++) {
+            if (v[q] == 0) continue;
+            int i = ii[q] - 1, j = jj[q] - 1;
+            if (i > j) { const int s = i; i = j; j = s; }
+            if (mat[q] == 0) {
+                P->C[i + (int64_t)j * n] += -v[q];
+                if (i != j) P->C[j + (int64_t)i * n] += -v[q];
+                continue;
+            }
+            double *A = P->Ad + (int64_t)(mat[q] - 1) * nn;
+            A[i + (int64_t)j * n] += v[q];
+            if (i != j) A[j + (int64_t)i * n] += v[q];
+        }
     }
-    finish_scaling(P);
-    /* scale the columns by d_i */
-    for (int c = 0; c < m; c++) {
-        double *A = P->Ad + (int64_t)c * nn;
-        const double dc = P->d[c];
-        for (int64_t q = 0; q < nn; q++) A[q] *= dc;
-    }
+    finish_scaling(P, have_d);
     P->cptr = calloc(m + 1, sizeof(int));
     P->ent = malloc(sizeof(SEnt));
     P->gptr = calloc(m + 1, sizeof(int)); P->grp = malloc(sizeof(GrpRef)); P->ngrp = 0; P->gp_tot = 0;
@@ -77,8 +155,9 @@ static int hprob_build_dense(HProb *P, int m, int n, int64_t nnz, const int *mat
 }
 
 /* norms on the data as read, row equilibration and scaling of b and C (problem.c:1866-1909);
- * the dense route's norms of A_i come from Ad */
-static void finish_scaling(HProb *P) {
+ * the dense route's norms of A_i come from Ad (have_d: d and the scaled columns are there
+ * already; else the columns are measured and scaled here) */
+static void finish_scaling(HProb *P, int have_d) {
     const int m = P->m;
     double c2 = 0, c1 = 0;
     for (int64_t q = 0; q < P->T; q++) { c2 += P->C[q] * P->C[q]; c1 += fabs(P->C[q]); }
@@ -87,12 +166,14 @@ static void finish_scaling(HProb *P) {
     for (int i = 0; i < m; i++) { P->normb1 += fabs(P->b0[i]); P->normb2 += P->b0[i] * P->b0[i]; }
     P->normb2 = sqrt(P->normb2);
     const int64_t nn = P->T;
-    for (int i = 0; i < m; i++) {
-        double s2 = 0;
-        const double *A = P->Ad + (int64_t)i * nn;
-        for (int64_t q = 0; q < nn; q++) s2 += A[q] * A[q];
-        P->d[i] = s2 > 0 ? 1.0 / sqrt(s2) : 1.0;
-    }
+    if (!have_d)
+        for (int i = 0; i < m; i++) {
+            double s2 = 0;
+            double *A = P->Ad + (int64_t)i * nn;
+            for (int64_t q = 0; q < nn; q++) s2 += A[q] * A[q];
+            P->d[i] = s2 > 0 ? 1.0 / sqrt(s2) : 1.0;
+            for (int64_t q = 0; q < nn; q++) A[q] *= P->d[i];
+        }
     double nbv = 0;
     for (int i = 0; i < m; i++) nbv += P->d[i] * P->b[i] * P->d[i] * P->b[i];
     nbv = sqrt(nbv);
@@ -104,6 +185,76 @@ static void finish_scaling(HProb *P) {
     double nb2 = 0;
     for (int i = 0; i < m; i++) nb2 += P->b[i] * P->b[i];
     P->nbs = sqrt(nb2);
+}
+
+/* P->pairs: the pairs (a, b), a >= b, of constraints with a block in common, by column b and rows
+ * a ascending (2 ints each), when they are at most a quarter of the lower triangle; else none
+ * (npair 0) and the assembly visits every pair. Built from the lists of the constraints of each
+ * block (the groups), with a stamp per row against pairs that share several blocks. */
+/* col[0 .. nc) holds runs in descending order, run r at [rb[r], rb[r + 1]), no value twice: into
+ * ascending order through tmp, by merging each reversed run into the result so far */
+static void merge_runs(int *col, int nc, const int *rb, int nrun, int *tmp) {
+    int len = 0;                                 /* col[0 .. len) sorted ascending */
+    for (int r = 0; r < nrun; r++) {
+        const int b0 = rb[r], b1 = rb[r + 1];
+        int i = 0, j = b1 - 1, o = 0;            /* the run read from its end: ascending */
+        while (i < len && j >= b0) tmp[o++] = col[i] < col[j] ? col[i++] : col[j--];
+        while (i < len) tmp[o++] = col[i++];
+        while (j >= b0) tmp[o++] = col[j--];
+        memcpy(col, tmp, sizeof(int) * o);
+        len = o;
+    }
+    (void)nc;
+}
+static int build_pairs(HProb *P) {
+    const int m = P->m, nb = P->nb;
+    P->pairs = NULL; P->npair = 0;
+    if (m < 64) return 0;
+    int *bcnt = calloc(nb + 1, sizeof(int)), *bptr = malloc(sizeof(int) * (nb + 1)), *bcon = malloc(sizeof(int) * (P->ngrp + 1));
+    int *stamp = malloc(sizeof(int) * m), *col = malloc(sizeof(int) * m), *tmp = malloc(sizeof(int) * m);
+    int maxg = 1;
+    for (int a = 0; a < m; a++) if (P->gptr[a + 1] - P->gptr[a] > maxg) maxg = P->gptr[a + 1] - P->gptr[a];
+    int *rb = malloc(sizeof(int) * (maxg + 2));
+    if (!bcnt || !bptr || !bcon || !stamp || !col || !tmp || !rb) {
+        free(bcnt); free(bptr); free(bcon); free(stamp); free(col); free(tmp); free(rb);
+        return -1;
+    }
+    for (int g = 0; g < P->ngrp; g++) bcnt[P->grp[g].k]++;
+    bptr[0] = 0;
+    for (int k = 0; k < nb; k++) bptr[k + 1] = bptr[k] + bcnt[k];
+    for (int k = 0; k < nb; k++) bcnt[k] = bptr[k];
+    for (int a = 0; a < m; a++)
+        for (int g = P->gptr[a]; g < P->gptr[a + 1]; g++) bcon[bcnt[P->grp[g].k]++] = a;   /* ascending a per block */
+    const double cap = 0.25 * (double)m * (m + 1) / 2.0;
+    for (int pass = 0; pass < 2; pass++) {
+        int64_t np = 0;
+        for (int a = 0; a < m; a++) stamp[a] = -1;
+        for (int b = 0; b < m; b++) {
+            int nc = 0, nrun = 0;
+            for (int g = P->gptr[b]; g < P->gptr[b + 1]; g++) {
+                const int k = P->grp[g].k;
+                rb[nrun++] = nc;
+                for (int t = bptr[k + 1] - 1; t >= bptr[k] && bcon[t] >= b; t--) {
+                    const int a = bcon[t];
+                    if (stamp[a] != b) { stamp[a] = b; col[nc++] = a; }
+                }
+            }
+            rb[nrun] = nc;
+            if (pass == 1) {
+                merge_runs(col, nc, rb, nrun, tmp);
+                for (int t = 0; t < nc; t++) { P->pairs[2 * (np + t)] = col[t]; P->pairs[2 * (np + t) + 1] = b; }
+            }
+            np += nc;
+            if (pass == 0 && np > cap) break;
+        }
+        if (pass == 0) {
+            if (np > cap || np == 0) break;
+            P->pairs = malloc(sizeof(int) * 2 * (size_t)np);
+            if (!P->pairs) { free(bcnt); free(bptr); free(bcon); free(stamp); free(col); free(tmp); free(rb); return -1; }
+        } else P->npair = np;
+    }
+    free(bcnt); free(bptr); free(bcon); free(stamp); free(col); free(tmp); free(rb);
+    return 0;
 }
 
 /* returns 0, or 2 for data that is not a valid problem (the reference's exit code for bad data) */
@@ -192,22 +343,26 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
     P->b = malloc(sizeof(double) * (m + 1)); P->b0 = malloc(sizeof(double) * (m + 1));
     P->d = malloc(sizeof(double) * (m + 1)); P->du = malloc(sizeof(double) * (m + 1));
     for (int i = 0; i < m; i++) P->b[i] = P->b0[i] = c[i];
-    /* triplets of the constraints (internal block, upper position) */
+    /* triplets of the constraints (internal block, upper position): checked in parallel */
     int64_t ncon = 0;
-    for (int64_t q = 0; q < nnz; q++) {
-        const int t = mat[q], k = blk[q] - 1;
-        if (t < 0 || t > m || k < 0 || k >= nblk) return 2;
-        const int n = abs(bs[k]);
-        if (ii[q] < 1 || jj[q] < 1 || ii[q] > n || jj[q] > n) return 2;
-        if (bs[k] < 0 && ii[q] != jj[q]) return 2;
-        if (!isfinite(v[q])) return 2;
-        if (v[q] != 0 && t > 0) ncon++;
+    int sorted = 1;
+    {
+        SetupCtx c;
+        memset(&c, 0, sizeof(c));
+        c.m = m; c.nblk = nblk; c.bs = bs; c.mat = mat; c.blk = blk; c.ii = ii; c.jj = jj; c.v = v;
+        par_for(nnz, 1 << 20, check_range, &c);
+        const int P_ = par_parts(nnz, 1 << 20);
+        for (int p = 0; p < P_; p++) {
+            if (c.bad[p]) return 2;
+            ncon += c.ncon[p];
+            if (c.unsorted[p]) sorted = 0;
+        }
     }
     /* dense route: one SDP block whose constraints fill at least a quarter of their upper
      * triangles (the reference's route 2, solver.c:2968-3016, taken for every constraint) */
     if (nblk == 1 && bs[0] > 1 && m > 0 && (double)ncon >= 0.25 * (double)m * bs[0] * (bs[0] + 1) / 2.0 &&
         (double)m * bs[0] * bs[0] * 8.0 < 64e9)
-        return hprob_build_dense(P, m, bs[0], nnz, mat, blk, ii, jj, v);
+        return hprob_build_dense(P, m, bs[0], nnz, mat, blk, ii, jj, v, sorted);
     Trip *tr = malloc(sizeof(Trip) * (ncon + 1));
     int64_t nt = 0;
     for (int64_t q = 0; q < nnz; q++) {
@@ -273,6 +428,9 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
         if (P->cptr[i + 1] - P->cptr[i] != 1) P->diag = 0;
         else if (P->ent[P->cptr[i]].r != P->ent[P->cptr[i]].c) P->diag = 0;
     }
+    /* the pairs of constraints that share a block, when they are few (many blocks): the Schur
+     * assembly then visits only those (the other entries of M are zero) */
+    if (!P->diag && build_pairs(P)) { free(tr); return 2; }
     /* blocks every constraint entry of which is diagonal (every LP variable; every SDP block of
      * a max-cut problem): there the dual direction dZ = theta R0 - A*(dy) is diagonal too */
     P->bdiag = malloc(sizeof(int) * (P->nb + 1));

@@ -1,6 +1,7 @@
 // rt.hip - device runtime of the solver library: one stream, device allocations, a pinned
 // staging ring for the small item arrays the batched launches read, events and errors.
 #include <hip/hip_runtime.h>
+#include <thread>
 #include <stdio.h>
 #include <string.h>
 #include <time.h>
@@ -29,6 +30,22 @@ static char *g_bounce = nullptr;      // pinned: destination of small device-to-
 #define XFER_CHUNK ((size_t)16 << 20)
 static char *g_xfer[2] = { nullptr, nullptr };
 static hipEvent_t g_xfer_ev[2];
+// a host copy of one staging chunk, split over up to 8 threads (one thread copies a few GB/s)
+static void pcopy(void *dst, const void *src, size_t len) {
+    const size_t part = (size_t)2 << 20;
+    int np = (int)(len / part);
+    if (np > 8) np = 8;
+    const unsigned hw = std::thread::hardware_concurrency();
+    if (hw && np > (int)hw) np = (int)hw;
+    if (np < 2) { memcpy(dst, src, len); return; }
+    std::thread th[8];
+    for (int p = 1; p < np; p++) {
+        const size_t lo = len * p / np, hi = len * (p + 1) / np;
+        th[p] = std::thread([=] { memcpy((char *)dst + lo, (const char *)src + lo, hi - lo); });
+    }
+    memcpy(dst, src, len / np);
+    for (int p = 1; p < np; p++) th[p].join();
+}
 static size_t g_ring_cap = 0, g_ring_pos = 0;
 #define CACHE_CAP ((size_t)32 << 20)
 #define CACHE_CHUNK ((size_t)64 << 10)
@@ -186,7 +203,7 @@ extern "C" int bk_h2d(void *dst, const void *src, size_t bytes) {
         const int b = (int)(k & 1);
         const size_t len = bytes - off < XFER_CHUNK ? bytes - off : XFER_CHUNK;
         if (k >= 2 && bk_check(hipEventSynchronize(g_xfer_ev[b]), "hipEventSynchronize H2D")) return -1;
-        memcpy(g_xfer[b], (const char *)src + off, len);
+        pcopy(g_xfer[b], (const char *)src + off, len);
         if (bk_check(hipMemcpyAsync((char *)dst + off, g_xfer[b], len, hipMemcpyHostToDevice, g_bk_stream), "hipMemcpy H2D") ||
             bk_check(hipEventRecord(g_xfer_ev[b], g_bk_stream), "hipEventRecord H2D")) return -1;
     }
@@ -208,13 +225,13 @@ extern "C" int bk_d2h(void *dst, const void *src, size_t bytes) {
             bk_check(hipEventRecord(g_xfer_ev[b], g_bk_stream), "hipEventRecord D2H")) return -1;
         if (k > 0) {
             if (bk_check(hipEventSynchronize(g_xfer_ev[b ^ 1]), "hipEventSynchronize D2H")) return -1;
-            memcpy((char *)dst + prev_off, g_xfer[b ^ 1], prev_len);
+            pcopy((char *)dst + prev_off, g_xfer[b ^ 1], prev_len);
         }
         prev_off = off; prev_len = len;
     }
     const int lb = (int)(((bytes + XFER_CHUNK - 1) / XFER_CHUNK - 1) & 1);
     if (bk_check(hipEventSynchronize(g_xfer_ev[lb]), "hipEventSynchronize D2H")) return -1;
-    memcpy((char *)dst + prev_off, g_xfer[lb], prev_len);
+    pcopy((char *)dst + prev_off, g_xfer[lb], prev_len);
     return bk_sync();
 }
 extern "C" int bk_d2d(void *dst, const void *src, size_t bytes) {

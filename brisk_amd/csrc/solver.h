@@ -75,6 +75,8 @@ typedef struct {
     double *pval;
     int diag;               /* every constraint is one diagonal entry: Hadamard Schur form */
     int *bdiag;             /* per block: every constraint entry in it is diagonal (NULL: none is known to be) */
+    int *pairs;             /* the pairs (a, b), a >= b, of constraints sharing a block, by column (npair of them), or NULL */
+    int64_t npair;
     /* dense route: one SDP block with dense constraints; Ad (host) holds the scaled A_i in full
      * symmetric storage, column i = vec(A_i) (n*n x m); the entry and position lists are empty */
     int dense;
@@ -109,6 +111,7 @@ typedef struct Solve {
     int64_t sc;             /* 64 scalars */
     int64_t cptr, ent, blk, pptr, ppos, pcon, pval, dinvoff; /* problem data on the device */
     int64_t gptr, grp, Pg, Tg;      /* constraint groups (bytes), heavy-group products P and T (doubles) */
+    int64_t pairs;                  /* the constraint pairs sharing a block (bytes) */
     int64_t Rd, CX, dX0, dZ0, dX1, dZ1, hv, hg, ht, hy0, hy1;   /* self-dual embedding: residual, Zi C X, the corrector family, vectors */
     int64_t Ad, Af, Pm, Bm, Mpart, LXf, LZif, Om1, Om2;  /* dense route: A f64 / f32 (n^2 x m), P, B, split-K partials, f32 factors, M v scratch */
     int nsplit, kchunk;     /* dense route: split-K of M = P'P */

@@ -205,6 +205,9 @@ typedef struct {
     int64_t D, out, y, X, Zi, M;     /* operand offsets (doubles) for this call */
     double alpha, beta;
     int64_t gptr, grp;               /* constraint groups (BYTE offsets) */
+    int64_t pairs, npair;            /* schur: npair > 0 lists the pairs (a, b), a >= b, sharing a block (2 ints each,
+                                        BYTE offset; by column b, rows a ascending): only those are assembled, the
+                                        other entries of M keep the zeros they were given once */
 } ConItem;
 int bk_aop(const ConItem *it, int n_it);
 int bk_aty(const ConItem *it, int n_it);

@@ -46,9 +46,10 @@ def arr(x):
 
 # every assembly route: single-entry gathers with a heavy identity row (theta), the Hadamard form
 # (max-cut), blocks + LP with heavy groups (multi), the dense route (dense)
+# (many blocks with m >= 64: only the pairs of constraints sharing a block are assembled)
 SCHUR_CASES = [("theta", (30, 0.3, 3)), ("theta", (60, 0.15, 1)), ("maxcut", (80, 0.1, 0)),
                ("multi", (4, 10, 20, 60, 0)), ("multi", (6, 30, 40, 300, 2)), ("multi", (3, 12, 0, 40, 5)),
-               ("dense", (12, 30, 0)), ("dense", (40, 120, 1))]
+               ("multi", (24, 12, 60, 500, 7)), ("dense", (12, 30, 0)), ("dense", (40, 120, 1))]
 
 
 @pytest.mark.parametrize("prec", [1, 0])
