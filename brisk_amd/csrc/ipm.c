@@ -228,8 +228,10 @@ int solve_setup(Solve *S) {
         rc |= bk_zero(H + S->M * 8, (size_t)m * m * 8);
     }
     if (P->dense) {
+        /* the columns as read, then scaled by d (staged in t1) with their f32 copy made */
         rc |= bk_h2d_async(H + S->Ad * 8, P->Ad, T * m * 8);
-        rc |= bk_d2f(T * m, S->Ad, 2 * S->Af);
+        rc |= bk_h2d_async(H + S->t1 * 8, P->d, m * 8);
+        rc |= bk_colscale_d2f(T, m, S->Ad, 2 * S->Af, S->t1);
     }
     if (rc) { solve_release(S); return -1; }
     S->status = -1;

@@ -209,6 +209,27 @@ extern "C" int bk_d2f(int64_t n, int64_t src_off, int64_t dst_off_f32) {
     hipLaunchKernelGGL(k_d2f, dim3(bk_grid1(n, 256, 8192)), dim3(256), 0, g_bk_stream, ((const double *)g_heap) + src_off, ((float *)g_heap) + dst_off_f32, n);
     return BK_LAUNCH_CHECK("k_d2f");
 }
+// a(r, c) *= d[c] in place and b(r, c) = (float)a(r, c), columns of `rows` elements: the grid's
+// y index walks the columns, x the rows of one
+__global__ void k_colscale_d2f(double *__restrict__ a, float *__restrict__ b, const double *__restrict__ d, int64_t rows, int cols) {
+    for (int c = blockIdx.y; c < cols; c += gridDim.y) {
+        const double s = d[c];
+        double *ac = a + (int64_t)c * rows;
+        float *bc = b + (int64_t)c * rows;
+        for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (int64_t)gridDim.x * blockDim.x) {
+            const double v = ac[r] * s;
+            ac[r] = v;
+            bc[r] = (float)v;
+        }
+    }
+}
+extern "C" int bk_colscale_d2f(int64_t rows, int cols, int64_t off, int64_t dst_off_f32, int64_t d_off) {
+    if (rows <= 0 || cols <= 0) return 0;
+    const dim3 grid(bk_grid1(rows, 256, 64), (unsigned)(cols < 8192 ? cols : 8192));
+    hipLaunchKernelGGL(k_colscale_d2f, grid, dim3(256), 0, g_bk_stream, ((double *)g_heap) + off,
+                       ((float *)g_heap) + dst_off_f32, ((const double *)g_heap) + d_off, rows, cols);
+    return BK_LAUNCH_CHECK("k_colscale_d2f");
+}
 
 // ---- per-matrix helpers ---------------------------------------------------------------------
 #define FLAT_MITEM                                                          \

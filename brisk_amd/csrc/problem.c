@@ -23,17 +23,19 @@ typedef struct {
     const int *bs, *mat, *blk, *ii, *jj;
     const double *v;
     double *Ad, *C, *d;
-    const int64_t *cbeg;            /* dense route: entries of matrix t at [cbeg[t], cbeg[t + 1]) */
+    const int64_t *rb, *re;         /* dense route: the entries of matrix t (0: F0) at [rb[t], re[t]) */
+    int64_t *chg[16];               /* per part: the positions q > 0 where the matrix number changes (at most m + 1) */
     int bad[16];
-    int64_t ncon[16];
-    int unsorted[16];
+    int64_t ncon[16], nchg[16];
 } SetupCtx;
 /* the entries valid (indices within their blocks, LP entries diagonal, finite values); the
- * count of nonzero constraint entries; whether the matrix numbers never decrease */
+ * count of nonzero constraint entries; where the matrix number changes (the first m + 1 places,
+ * nchg counting all of them) */
 static void check_range(void *c_, int64_t lo, int64_t hi, int part) {
     SetupCtx *c = c_;
-    int bad = 0, uns = 0;
-    int64_t ncon = 0;
+    int bad = 0;
+    int64_t ncon = 0, nchg = 0;
+    int64_t *chg = malloc(sizeof(int64_t) * ((size_t)c->m + 2));
     for (int64_t q = lo; q < hi; q++) {
         const int t = c->mat[q], k = c->blk[q] - 1;
         if (t < 0 || t > c->m || k < 0 || k >= c->nblk) { bad = 1; break; }
@@ -42,12 +44,49 @@ static void check_range(void *c_, int64_t lo, int64_t hi, int part) {
         if (c->bs[k] < 0 && c->ii[q] != c->jj[q]) { bad = 1; break; }
         if (!isfinite(c->v[q])) { bad = 1; break; }
         if (c->v[q] != 0 && t > 0) ncon++;
-        if (q > 0 && c->mat[q] < c->mat[q - 1]) uns = 1;
+        if (q > 0 && t != c->mat[q - 1]) {
+            if (chg && nchg <= c->m) chg[nchg] = q;
+            nchg++;
+        }
     }
-    c->bad[part] = bad; c->ncon[part] = ncon; c->unsorted[part] = uns;
+    c->bad[part] = bad; c->ncon[part] = ncon; c->chg[part] = chg;
+    c->nchg[part] = chg ? nchg : (int64_t)c->m + 1;     /* (no list: taken as not grouped) */
+}
+/* the dense route's host columns: the largest buffer seen (at most 32 GiB) is kept when its
+ * problem is freed and handed to the next dense problem that fits in it, so that a repeated
+ * solve of that size writes memory that is already mapped (the first touch of a fresh 2.5 GB
+ * buffer costs more than building the columns). The library is called from one thread. */
+#define AD_KEEP_MAX ((size_t)4 << 30)           /* doubles */
+static double *g_ad_keep;
+static size_t g_ad_keep_n;
+static double *ad_take(size_t n, size_t *cap) {
+    if (g_ad_keep && g_ad_keep_n >= n) {
+        double *p = g_ad_keep;
+        *cap = g_ad_keep_n;
+        g_ad_keep = NULL; g_ad_keep_n = 0;
+        return p;
+    }
+    *cap = n;
+    return malloc(sizeof(double) * n);
+}
+static void ad_give(double *p, size_t cap) {
+    if (!p) return;
+    if (cap <= AD_KEEP_MAX && (!g_ad_keep || cap > g_ad_keep_n)) {
+        free(g_ad_keep);
+        g_ad_keep = p; g_ad_keep_n = cap;
+    } else free(p);
+}
+/* 1 / ||A||_F of an n x n column of Ad, or 1 for a zero column */
+static double column_dinv(const double *A, int64_t nn) {
+    double s2 = 0;
+    for (int64_t q = 0; q < nn; q++) s2 += A[q] * A[q];
+    return s2 > 0 ? 1.0 / sqrt(s2) : 1.0;
 }
 /* dense route, matrices lo .. hi - 1 (entries grouped by matrix): each column of Ad built from
- * its entries (full symmetric storage), its norm taken and the column scaled by d = 1/norm */
+ * its entries (full symmetric storage) and d = 1 / its norm taken. The norm is summed during
+ * the scatter when no position of the column is written twice (a second write finds a nonzero;
+ * the entries' squares then are the column's), else taken from the column afterwards. The
+ * columns stay unscaled here: the device scales them by d as it makes their f32 copy */
 static void dense_columns(void *c_, int64_t lo, int64_t hi, int part) {
     SetupCtx *c = c_;
     (void)part;
@@ -56,18 +95,20 @@ static void dense_columns(void *c_, int64_t lo, int64_t hi, int part) {
     for (int64_t t = lo; t < hi; t++) {
         double *A = c->Ad + t * nn;
         memset(A, 0, sizeof(double) * nn);
-        for (int64_t q = c->cbeg[t + 1]; q < c->cbeg[t + 2]; q++) {
-            if (c->v[q] == 0) continue;
+        double s2 = 0;
+        int twice = 0;
+        for (int64_t q = c->rb[t + 1]; q < c->re[t + 1]; q++) {
+            const double v = c->v[q];
+            if (v == 0) continue;
             int i = c->ii[q] - 1, j = c->jj[q] - 1;
             if (i > j) { const int s = i; i = j; j = s; }
-            A[i + (int64_t)j * n] += c->v[q];
-            if (i != j) A[j + (int64_t)i * n] += c->v[q];
+            double *u = A + i + (int64_t)j * n;
+            twice |= *u != 0;
+            *u += v;
+            if (i != j) { A[j + (int64_t)i * n] += v; s2 += 2 * v * v; }
+            else s2 += v * v;
         }
-        double s2 = 0;
-        for (int64_t q = 0; q < nn; q++) s2 += A[q] * A[q];
-        const double dt = s2 > 0 ? 1.0 / sqrt(s2) : 1.0;
-        c->d[t] = dt;
-        for (int64_t q = 0; q < nn; q++) A[q] *= dt;
+        c->d[t] = twice ? column_dinv(A, nn) : s2 > 0 ? 1.0 / sqrt(s2) : 1.0;
     }
 }
 
@@ -91,33 +132,51 @@ static int cmp_pos(const void *a, const void *b) {
 void hprob_free(HProb *P) {
     free(P->bs_in); free(P->in_first); free(P->bn); free(P->boff);
     free(P->b); free(P->b0); free(P->d); free(P->du); free(P->C);
-    free(P->cptr); free(P->ent); free(P->gptr); free(P->grp); free(P->pptr); free(P->ppos); free(P->pcon); free(P->pval); free(P->Ad);
+    free(P->cptr); free(P->ent); free(P->gptr); free(P->grp); free(P->pptr); free(P->ppos); free(P->pcon); free(P->pval);
+    ad_give(P->Ad, P->ad_cap);
     free(P->bdiag); free(P->pairs);
     memset(P, 0, sizeof(*P));
 }
 
-/* dense route: the scaled A_i in full symmetric storage, column i of Ad = vec(A_i); with the
- * entries grouped by matrix (as an SDPA file lists them) the columns are built, measured and
- * scaled in parallel, each by one thread while it is in cache */
+/* the entries of each matrix t = 0 .. m as one range [rb[t], re[t]) (empty: rb = re = 0) when
+ * every matrix's entries are contiguous, in any order of the matrices (an SDPA file lists them
+ * so); the change positions come from the parts of the validity pass. Returns 0, or 1 when the
+ * entries are not grouped that way */
+static int matrix_ranges(const SetupCtx *c, int64_t nnz, int64_t *rb, int64_t *re) {
+    const int m = c->m, P_ = par_parts(nnz, 1 << 20);
+    int64_t tot = 0;
+    for (int p = 0; p < P_; p++) tot += c->nchg[p];
+    if (tot > m) return 1;
+    memset(rb, 0, sizeof(int64_t) * (m + 1));
+    memset(re, 0, sizeof(int64_t) * (m + 1));
+    int64_t a = 0;                                   /* the current run starts at a */
+    for (int p = 0; p <= P_ && nnz > 0; p++) {
+        const int64_t nc = p < P_ ? c->nchg[p] : 1;
+        for (int64_t r = 0; r < nc; r++) {
+            const int64_t b = p < P_ ? c->chg[p][r] : nnz;
+            const int t = c->mat[a];
+            if (re[t] > 0) return 1;                /* a second run of matrix t */
+            rb[t] = a; re[t] = b;
+            a = b;
+        }
+    }
+    return 0;
+}
+
+/* dense route: the scaled A_i in full symmetric storage, column i of Ad = vec(A_i); with each
+ * matrix's entries together (rb, re: matrix_ranges) the columns are built, measured and scaled
+ * in parallel, each by one thread while it is in cache */
 static void finish_scaling(HProb *P, int have_d);
 static int hprob_build_dense(HProb *P, int m, int n, int64_t nnz, const int *mat, const int *blk, const int *ii, const int *jj,
-                             const double *v, int sorted) {
+                             const double *v, const int64_t *rb, const int64_t *re) {
     const int64_t nn = (int64_t)n * n;
     P->dense = 1;
-    P->Ad = malloc(sizeof(double) * ((size_t)nn * m + 1));
+    P->Ad = ad_take((size_t)nn * m + 1, &P->ad_cap);
     if (!P->Ad) return -1;
     (void)blk;
     int have_d = 0;
-    if (sorted) {
-        /* cbeg[t + 1] .. cbeg[t + 2]: the entries of matrix t (cbeg[0] .. cbeg[1]: F0) */
-        int64_t *cbeg = malloc(sizeof(int64_t) * (m + 3));
-        int64_t q = 0;
-        for (int t = 0; t <= m + 1; t++) {
-            while (q < nnz && mat[q] < t) q++;
-            cbeg[t] = q;
-        }
-        cbeg[m + 2] = nnz;
-        for (int64_t e = cbeg[0]; e < cbeg[1]; e++) {          /* C = -F0 */
+    if (rb) {
+        for (int64_t e = rb[0]; e < re[0]; e++) {          /* C = -F0 */
             if (v[e] == 0) continue;
             int i = ii[e] - 1, j = jj[e] - 1;
             if (i > j) { const int s = i; i = j; j = s; }
@@ -126,9 +185,8 @@ static int hprob_build_dense(HProb *P, int m, int n, int64_t nnz, const int *mat
         }
         SetupCtx c;
         memset(&c, 0, sizeof(c));
-        c.n = n; c.ii = ii; c.jj = jj; c.v = v; c.Ad = P->Ad; c.d = P->d; c.cbeg = cbeg;
+        c.n = n; c.ii = ii; c.jj = jj; c.v = v; c.Ad = P->Ad; c.d = P->d; c.rb = rb; c.re = re;
         par_for(m, 16, dense_columns, &c);
-        free(cbeg);
         have_d = 1;
     } else {
         memset(P->Ad, 0, sizeof(double) * (size_t)nn * m);
@@ -155,8 +213,8 @@ static int hprob_build_dense(HProb *P, int m, int n, int64_t nnz, const int *mat
 }
 
 /* norms on the data as read, row equilibration and scaling of b and C (problem.c:1866-1909);
- * the dense route's norms of A_i come from Ad (have_d: d and the scaled columns are there
- * already; else the columns are measured and scaled here) */
+ * the dense route's d_i = 1 / ||A_i|| from its columns of Ad (have_d: taken already); the
+ * columns themselves are scaled on the device */
 static void finish_scaling(HProb *P, int have_d) {
     const int m = P->m;
     double c2 = 0, c1 = 0;
@@ -165,15 +223,8 @@ static void finish_scaling(HProb *P, int have_d) {
     P->normb1 = P->normb2 = 0;
     for (int i = 0; i < m; i++) { P->normb1 += fabs(P->b0[i]); P->normb2 += P->b0[i] * P->b0[i]; }
     P->normb2 = sqrt(P->normb2);
-    const int64_t nn = P->T;
     if (!have_d)
-        for (int i = 0; i < m; i++) {
-            double s2 = 0;
-            double *A = P->Ad + (int64_t)i * nn;
-            for (int64_t q = 0; q < nn; q++) s2 += A[q] * A[q];
-            P->d[i] = s2 > 0 ? 1.0 / sqrt(s2) : 1.0;
-            for (int64_t q = 0; q < nn; q++) A[q] *= P->d[i];
-        }
+        for (int i = 0; i < m; i++) P->d[i] = column_dinv(P->Ad + (int64_t)i * P->T, P->T);
     double nbv = 0;
     for (int i = 0; i < m; i++) nbv += P->d[i] * P->b[i] * P->d[i] * P->b[i];
     nbv = sqrt(nbv);
@@ -345,24 +396,26 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
     for (int i = 0; i < m; i++) P->b[i] = P->b0[i] = c[i];
     /* triplets of the constraints (internal block, upper position): checked in parallel */
     int64_t ncon = 0;
-    int sorted = 1;
-    {
-        SetupCtx c;
-        memset(&c, 0, sizeof(c));
-        c.m = m; c.nblk = nblk; c.bs = bs; c.mat = mat; c.blk = blk; c.ii = ii; c.jj = jj; c.v = v;
-        par_for(nnz, 1 << 20, check_range, &c);
-        const int P_ = par_parts(nnz, 1 << 20);
-        for (int p = 0; p < P_; p++) {
-            if (c.bad[p]) return 2;
-            ncon += c.ncon[p];
-            if (c.unsorted[p]) sorted = 0;
-        }
-    }
+    SetupCtx sc;
+    memset(&sc, 0, sizeof(sc));
+    sc.m = m; sc.nblk = nblk; sc.bs = bs; sc.mat = mat; sc.blk = blk; sc.ii = ii; sc.jj = jj; sc.v = v;
+    par_for(nnz, 1 << 20, check_range, &sc);
+    const int nparts = nnz > 0 ? par_parts(nnz, 1 << 20) : 0;
+    int bad = 0;
+    for (int p = 0; p < nparts; p++) { bad |= sc.bad[p]; ncon += sc.ncon[p]; }
+    if (bad) { for (int p = 0; p < nparts; p++) free(sc.chg[p]); return 2; }
     /* dense route: one SDP block whose constraints fill at least a quarter of their upper
      * triangles (the reference's route 2, solver.c:2968-3016, taken for every constraint) */
     if (nblk == 1 && bs[0] > 1 && m > 0 && (double)ncon >= 0.25 * (double)m * bs[0] * (bs[0] + 1) / 2.0 &&
-        (double)m * bs[0] * bs[0] * 8.0 < 64e9)
-        return hprob_build_dense(P, m, bs[0], nnz, mat, blk, ii, jj, v, sorted);
+        (double)m * bs[0] * bs[0] * 8.0 < 64e9) {
+        int64_t *rb = malloc(sizeof(int64_t) * (m + 1)), *re = malloc(sizeof(int64_t) * (m + 1));
+        const int grouped = !matrix_ranges(&sc, nnz, rb, re);
+        for (int p = 0; p < nparts; p++) free(sc.chg[p]);
+        const int e = hprob_build_dense(P, m, bs[0], nnz, mat, blk, ii, jj, v, grouped ? rb : NULL, grouped ? re : NULL);
+        free(rb); free(re);
+        return e;
+    }
+    for (int p = 0; p < nparts; p++) free(sc.chg[p]);
     Trip *tr = malloc(sizeof(Trip) * (ncon + 1));
     int64_t nt = 0;
     for (int64_t q = 0; q < nnz; q++) {

@@ -77,10 +77,12 @@ typedef struct {
     int *bdiag;             /* per block: every constraint entry in it is diagonal (NULL: none is known to be) */
     int *pairs;             /* the pairs (a, b), a >= b, of constraints sharing a block, by column (npair of them), or NULL */
     int64_t npair;
-    /* dense route: one SDP block with dense constraints; Ad (host) holds the scaled A_i in full
-     * symmetric storage, column i = vec(A_i) (n*n x m); the entry and position lists are empty */
+    /* dense route: one SDP block with dense constraints; Ad (host) holds the A_i as read in full
+     * symmetric storage, column i = vec(A_i) (n*n x m), scaled by d_i on the device after the
+     * upload; the entry and position lists are empty */
     int dense;
     double *Ad;
+    size_t ad_cap;          /* doubles allocated at Ad */
     int want_hsd;           /* the solve will run the self-dual embedding (its arrays are counted) */
 } HProb;
 

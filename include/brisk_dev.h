@@ -181,6 +181,10 @@ int bk_equil(const MItem *it, int n_it, int prec, int64_t reg_off);
 int bk_splitk_reduce(int prec, int64_t part_off, int nchunk, int64_t stride, int m, int64_t M_off);
 /* f32 copy: floats at dst_off_f32 = doubles at src_off (n elements) */
 int bk_d2f(int64_t n, int64_t src_off, int64_t dst_off_f32);
+/* column scaling with f32 copy: column c (rows elements, c < cols) of the doubles at off is
+ * multiplied by the double at d_off + c in place, and the result copied to the floats at
+ * dst_off_f32 (the dense route's constraint columns scaled by 1 / ||A_i||) */
+int bk_colscale_d2f(int64_t rows, int cols, int64_t off, int64_t dst_off_f32, int64_t d_off);
 
 /* ---- constraint operators (route 0: sparse entries) ---- */
 typedef struct { int k, r, c, pad; double v; } SEnt;   /* upper entry (r <= c) of block k */
