@@ -497,41 +497,52 @@ static const size_t F_V[2] = { offsetof(Solve, X), offsetof(Solve, Z) }, F_L[2] 
                     F_dV[2] = { offsetof(Solve, dX), offsetof(Solve, dZ) };
 
 /* ---- Lanczos step-length bound (reference solver.c:6083-6225) ----------------------------- */
-/* smallest eigenvalue of the k x k tridiagonal (a, b) and the last component of its eigenvector,
- * by cyclic Jacobi on the dense tridiagonal (k <= 30) */
+/* smallest eigenvalue of the k x k tridiagonal (a, b), b >= 0, by Sturm bisection to full
+ * precision, and the last entry of its unit eigenvector by two inverse iterations just below it
+ * (the LDL' solve of the positive definite T - sigma I), as the device's Ritz step for blocks of
+ * order <= 256 takes them (k <= 30) */
 static void tri_min(int k, const double *a, const double *b, double *theta, double *zlast) {
-    double Am[BK_LZ_K * BK_LZ_K], V[BK_LZ_K * BK_LZ_K];
-    memset(Am, 0, sizeof(Am)); memset(V, 0, sizeof(V));
-    for (int i = 0; i < k; i++) { Am[i + i * k] = a[i]; V[i + i * k] = 1; if (i + 1 < k) Am[i + (i + 1) * k] = Am[(i + 1) + i * k] = b[i]; }
-    for (int sweep = 0; sweep < 60; sweep++) {
-        double off = 0, tot = 0;
-        for (int j = 0; j < k; j++) for (int i = 0; i < k; i++) { const double x = Am[i + j * k] * Am[i + j * k]; tot += x; if (i != j) off += x; }
-        if (off <= 1e-30 * tot || off == 0) break;
-        for (int p = 0; p < k; p++)
-            for (int r = p + 1; r < k; r++) {
-                const double apr = Am[p + r * k];
-                if (apr == 0) continue;
-                const double tau = (Am[r + r * k] - Am[p + p * k]) / (2 * apr);
-                const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
-                const double c = 1 / sqrt(1 + t * t), s = t * c;
-                for (int i = 0; i < k; i++) {
-                    const double x = Am[i + p * k], y = Am[i + r * k];
-                    Am[i + p * k] = c * x - s * y; Am[i + r * k] = s * x + c * y;
-                }
-                for (int i = 0; i < k; i++) {
-                    const double x = Am[p + i * k], y = Am[r + i * k];
-                    Am[p + i * k] = c * x - s * y; Am[r + i * k] = s * x + c * y;
-                }
-                for (int i = 0; i < k; i++) {
-                    const double x = V[i + p * k], y = V[i + r * k];
-                    V[i + p * k] = c * x - s * y; V[i + r * k] = s * x + c * y;
-                }
-            }
+    double lo = INFINITY, hi = -INFINITY, scale = 0;
+    for (int u = 0; u < k; u++) {
+        const double r = (u > 0 ? fabs(b[u - 1]) : 0.0) + (u + 1 < k ? fabs(b[u]) : 0.0);
+        lo = fmin(lo, a[u] - r); hi = fmax(hi, a[u] + r);
+        scale = fmax(scale, fabs(a[u]));
     }
-    int im = 0;
-    for (int i = 1; i < k; i++) if (Am[i + i * k] < Am[im + im * k]) im = i;
-    *theta = Am[im + im * k];
-    *zlast = V[(k - 1) + im * k];
+    for (int it = 0; it < 200; it++) {
+        const double w = hi - lo;
+        if (!(w > 2e-16 * fmax(fabs(lo), fabs(hi))) || w <= 1e-300) break;
+        const double x = lo + 0.5 * w;
+        int cnt = 0;
+        double q = a[0] - x;
+        if (q < 0) cnt++;
+        for (int u = 1; u < k; u++) {
+            if (fabs(q) < 1e-300) q = q < 0 ? -1e-300 : 1e-300;
+            q = a[u] - x - b[u - 1] * b[u - 1] / q;
+            if (q < 0) cnt++;
+        }
+        if (cnt >= 1) hi = x; else lo = x;
+    }
+    *theta = 0.5 * (lo + hi);
+    const double sig = lo - (hi - lo) - 1e-15 * (scale + 1e-300);
+    double x[BK_LZ_K], dd[BK_LZ_K];
+    for (int u = 0; u < k; u++) x[u] = 1.0;
+    for (int iter = 0; iter < 2; iter++) {
+        dd[0] = a[0] - sig;
+        for (int u = 1; u < k; u++) {
+            if (fabs(dd[u - 1]) < 1e-300) dd[u - 1] = 1e-300;
+            const double l = b[u - 1] / dd[u - 1];
+            dd[u] = a[u] - sig - l * b[u - 1];
+            x[u] -= l * x[u - 1];
+        }
+        if (fabs(dd[k - 1]) < 1e-300) dd[k - 1] = 1e-300;
+        x[k - 1] /= dd[k - 1];
+        for (int u = k - 2; u >= 0; u--) x[u] = (x[u] - b[u] * x[u + 1]) / dd[u];
+        double nx = 0;
+        for (int u = 0; u < k; u++) nx += x[u] * x[u];
+        nx = 1.0 / sqrt(nx);
+        for (int u = 0; u < k; u++) x[u] *= nx;
+    }
+    *zlast = x[k - 1];
 }
 /* Lanczos on T = Li V Li' (operator form: three products per step, Li and its transpose
  * stored) or on T = V (fLi == 0), per block of every active solve, for nside independent sides
@@ -596,10 +607,10 @@ static int lanczos_min_(Solve **S, int ns, const int *act, int nside, const size
     if (nf && bk_lanczos_fused(fz, nf)) goto out;
     if (nrest && bk_lanczos_start(zz, nrest)) goto out;
     for (int j = 0; j < kmax; j++) {
-        /* w = T q_j for the live blocks */
+        /* w = T q_j for the live blocks: the stages' items in one list (stage-major), one staging */
         const int nstage = fLi[0] ? 3 : 1;
+        GemvL_clear(&g_vl);
         for (int stage = 0; stage < nstage; stage++) {
-            GemvL_clear(&g_vl);
             for (int c = 0; c < nz; c++) {
                 const LzItem *z = &g_zl.v[c];
                 if (!live[c] || j >= z->K) continue;
@@ -618,8 +629,8 @@ static int lanczos_min_(Solve **S, int ns, const int *act, int nside, const size
                 else { g.a = F_(s, fLiT[sd]) + bo; g.x = t2; g.y = z->w; g.tri = 2; }                  /* w = Li t2 */
                 GemvL_add(&g_vl, g);
             }
-            if (bk_gemv_f64(1, 1.0, 0.0, H, H, H, g_vl.v, g_vl.n)) goto out;
         }
+        if (bk_gemv_chain_f64(H, g_vl.v, g_vl.n / nstage, nstage)) goto out;
         int nl = 0;
         for (int c = 0; c < nz; c++) if (live[c] && j < g_zl.v[c].K) zz[nl++] = g_zl.v[c];
         if (bk_lanczos_step(zz, nl, j)) goto out;

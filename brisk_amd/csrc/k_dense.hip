@@ -1019,6 +1019,43 @@ static int gemv_run(int trans, TACC alpha, TACC beta, const TA *A, const TX *x, 
 extern "C" int bk_gemv_f64(int trans, double alpha, double beta, const double *A, const double *x, double *y, const GemvItem *it, int n_it) {
     return gemv_run<double, double, double, double>(trans, alpha, beta, A, x, y, it, n_it);
 }
+// nstage products y = A' x in sequence from one staging (stage s: the items it[s n_it ..
+// s n_it + n_it), offsets into the heap H): the three factors of the Lanczos operator per step
+extern "C" int bk_gemv_chain_f64(double *H, const GemvItem *it, int n_it, int nstage) {
+    if (n_it <= 0 || nstage <= 0) return 0;
+    std::vector<char> buf;
+    std::vector<size_t> io(nstage), po(nstage);
+    std::vector<int64_t> nb(nstage);
+    for (int st = 0; st < nstage; st++) {
+        io[st] = buf_align(buf);
+        buf.insert(buf.end(), (const char *)(it + (size_t)st * n_it), (const char *)(it + (size_t)(st + 1) * n_it));
+        po[st] = buf_align(buf);
+        buf.resize(po[st] + sizeof(int) * (n_it + 1));
+        int64_t tot = 0;
+        for (int q = 0; q < n_it; q++) {
+            ((int *)(buf.data() + po[st]))[q] = (int)tot;
+            tot += (it[(size_t)st * n_it + q].N + 3) / 4;
+        }
+        ((int *)(buf.data() + po[st]))[n_it] = (int)tot;
+        if (tot > 0x7fffffff) return -1;
+        nb[st] = tot;
+    }
+    const char *d = (const char *)bk_stage(buf.data(), buf.size());
+    if (!d) return -1;
+    for (int st = 0; st < nstage; st++) {
+        if (nb[st] == 0) continue;
+        const int tok = bk_prof_begin();
+        hipLaunchKernelGGL((k_gemv_t<double, double, double, double>), dim3((unsigned)nb[st]), dim3(256), 0, g_bk_stream,
+                           (const GemvItem *)(d + io[st]), (const int *)(d + po[st]), n_it, H, H, H, 1.0, 0.0);
+        if (tok >= 0) {
+            double fl = 0, by = 0;
+            for (int q = 0; q < n_it; q++) { const GemvItem &g = it[(size_t)st * n_it + q]; fl += 2.0 * g.M * g.N; by += 8.0 * g.M * g.N + 8.0 * (g.M + g.N); }
+            bk_prof_end(tok, 9, fl, by);
+        }
+        if (BK_LAUNCH_CHECK("k_gemv")) return -1;
+    }
+    return 0;
+}
 extern "C" int bk_gemv_f32(int trans, float alpha, float beta, const float *A, const float *x, float *y, const GemvItem *it, int n_it) {
     return gemv_run<float, float, float, float>(trans, alpha, beta, A, x, y, it, n_it);
 }

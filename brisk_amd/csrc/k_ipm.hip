@@ -736,37 +736,30 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 // ---- fused Lanczos for blocks of order <= 256: one workgroup per block, one row per thread -----
 #define LZF 256
-// sum over the workgroup of np per-thread values (p <= j), fixed order: lanes, then the 4 waves
-__device__ __forceinline__ void lzf_sums(double (&c)[BK_LZK], int j, double (*red)[BK_LZK], double *cf) {
-    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-#pragma unroll
-    for (int p = 0; p < BK_LZK; p++) {
-        if (p <= j && wv < 4) {                     // the row threads (the first 4 waves)
-            const double v = wave_sum(c[p]);
-            if (lane == 0) red[wv][p] = v;
-        }
-    }
-    __syncthreads();
-    if (t <= j) cf[t] = (red[0][t] + red[1][t]) + (red[2][t] + red[3][t]);
-    __syncthreads();
-}
-// number of eigenvalues below x of the m x m tridiagonal (a, b) (Sturm sequence)
+// number of eigenvalues below x of the m x m tridiagonal (a, b) (Sturm sequence: the pivots of
+// the LDL' factor of T - x I). 1 / q is the hardware reciprocal refined by one Newton step
+// (relative error below 1e-13, against the 2e-16 of a division, at a third of its latency): a
+// pivot's sign is that of the exact recurrence of a matrix within about 1e-13 ||T|| of T
 __device__ __forceinline__ int sturm_count(const double *a, const double *b, int m, double x) {
     int cnt = 0;
     double q = a[0] - x;
     if (q < 0) cnt++;
     for (int k = 1; k < m; k++) {
+        const double bb = b[k - 1] * b[k - 1], d = a[k] - x;
         if (fabs(q) < 1e-300) q = q < 0 ? -1e-300 : 1e-300;
-        q = a[k] - x - b[k - 1] * b[k - 1] / q;
+        double r = __builtin_amdgcn_rcp(q);
+        r = __builtin_fma(r, __builtin_fma(-q, r, 1.0), r);
+        q = __builtin_fma(-bb, r, d);
         if (q < 0) cnt++;
     }
     return cnt;
 }
 // the smallest Ritz value of the leading m x m tridiagonal (al, be) after kk of K steps of a block
 // of order n: bisection with 64 Sturm counts per round (one wave, lane = lane index), then two
-// inverse iterations below it for the last eigenvector entry. out = theta - |beta z_last| (or
-// -1e300 for coefficients that are not finite); stop when converged to tol, at a breakdown or at K.
-__device__ void lz_ritz(const double *al, const double *be, int kk, int K, int n, double tol, int lane, double &out_, int &stop_) {
+// inverse iterations below it for the last eigenvector entry (lane 0, in the LDS workspace ws of
+// 2 BK_LZK doubles). out = theta - |beta z_last| (or -1e300 for coefficients that are not
+// finite); stop when converged to tol, at a breakdown or at K; lane 0's out and stop are final.
+__device__ void lz_ritz(const double *al, const double *be, int kk, int K, int n, double tol, int lane, double *ws, double &out_, int &stop_) {
     int m = kk, brk = 0;
     double scale = 0;
     for (int u = 0; u < kk; u++) {
@@ -801,12 +794,12 @@ __device__ void lz_ritz(const double *al, const double *be, int kk, int K, int n
         }
         const double th = 0.5 * (lo + hi);
         double res = 0;
-        if (!brk && m < n) {
-            // (T - sigma I) x = r, sigma just below lambda_min (an SPD tridiagonal solve)
+        if (!brk && m < n && lane == 0) {
+            // (T - sigma I) x = r, sigma just below lambda_min (an SPD tridiagonal solve), in the
+            // workspace ws (2 BK_LZK doubles) by lane 0
             const double sig = lo - (hi - lo) - 1e-15 * (scale + 1e-300);
-            double x[BK_LZK], dd[BK_LZK];
-#pragma unroll
-            for (int u = 0; u < BK_LZK; u++) x[u] = 1.0;
+            double *x = ws, *dd = ws + BK_LZK;
+            for (int u = 0; u < m; u++) x[u] = 1.0;
             for (int iter = 0; iter < 2; iter++) {
                 dd[0] = al[0] - sig;
                 for (int u = 1; u < m; u++) {
@@ -854,6 +847,19 @@ __device__ __forceinline__ double lzf_mv(const double *__restrict__ M, int n, in
     __syncthreads();
     return v;
 }
+// the squared norm of a vector held one entry per row thread (t < LZF), fixed order: across
+// each of the 4 row waves, then their four sums
+__device__ __forceinline__ double lzf_norm2(double v, double *red4) {
+    const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    if (wv < 4) {
+        const double s = wave_sum(t < LZF ? v * v : 0.0);
+        if (lane == 0) red4[wv] = s;
+    }
+    __syncthreads();
+    const double r = (red4[0] + red4[1]) + (red4[2] + red4[3]);
+    __syncthreads();
+    return r;
+}
 __global__ __launch_bounds__(LZF * LZP) void k_lz_fused(const LzfItem *__restrict__ it, int nit, double *__restrict__ H) {
     const int qi = blockIdx.x;
     if (qi >= nit) return;
@@ -861,32 +867,26 @@ __global__ __launch_bounds__(LZF * LZP) void k_lz_fused(const LzfItem *__restric
     const int n = L.n, K = L.K, t = threadIdx.x, lane = t & 63;
     __shared__ double xs[LZF];
     __shared__ double pr[LZP][LZF];
-    __shared__ double red[4][BK_LZK];
-    __shared__ double cf[BK_LZK];
-    __shared__ double al[BK_LZK], be[BK_LZK];
+    __shared__ double Qs[BK_LZK][LZF];    // the basis, q_p(i) at Qs[p][i]
+    __shared__ double ws[LZF];
+    __shared__ double cs[32];
+    __shared__ double red4[4];
+    __shared__ double al[BK_LZK], be[BK_LZK], rw[2 * BK_LZK];
     __shared__ double lam_sh;
     __shared__ int stop_sh;
     const double *Li = H + L.Li, *LiT = H + L.LiT, *V = H + L.V;
-    double Q[BK_LZK];
-#pragma unroll
-    for (int p = 0; p < BK_LZK; p++) Q[p] = 0;
     // q_0: the reference's start vector, normalised
     double q0 = t < n ? 1.0 + 0.5 * sin(1.7 * t + 0.3) : 0.0;
-    {
-        double c[BK_LZK];
-        c[0] = q0 * q0;
-        lzf_sums(c, 0, red, cf);
-        q0 *= 1.0 / sqrt(cf[0]);
-    }
-    Q[0] = q0;
+    q0 *= 1.0 / sqrt(lzf_norm2(q0, red4));
+    if (t < LZF) Qs[0][t] = q0;
+    // the coefficient groups: 32 groups of 32 lanes, group p summing q_p(i) w(i) over the rows
+    // i = l, l + 32, ... (fixed order: eight rows at most, then across the group's lanes)
+    const int gp = t >> 5, gl = t & 31;
     double lam = 0;
     for (int j = 0; j < K; j++) {
         // w = T q_j: the current basis vector through LDS, one output row per thread
-        double qj = 0;
-#pragma unroll
-        for (int p = 0; p < BK_LZK; p++) if (p == j) qj = Q[p];
         __syncthreads();
-        if (t < LZF) xs[t] = qj;
+        if (t < LZF) xs[t] = Qs[j][t];
         __syncthreads();
         double w;
         if (L.mode == 0) w = lzf_mv(V, n, 0, xs, pr);
@@ -902,26 +902,25 @@ __global__ __launch_bounds__(LZF * LZP) void k_lz_fused(const LzfItem *__restric
         // classical Gram-Schmidt twice; the first pass's coefficient of q_j is alpha_j
         double alpha = 0;
         for (int pass = 0; pass < 2; pass++) {
-            double c[BK_LZK];
+            if (t < LZF) ws[t] = t < n ? w : 0.0;
+            __syncthreads();
+            if (gp <= j) {
+                double c = 0;
+                for (int i = gl; i < n; i += 32) c += Qs[gp][i] * ws[i];
 #pragma unroll
-            for (int p = 0; p < BK_LZK; p++) c[p] = (p <= j) ? Q[p] * w : 0.0;
-            lzf_sums(c, j, red, cf);
-            if (pass == 0) alpha = cf[j];
-#pragma unroll
-            for (int p = 0; p < BK_LZK; p++) if (p <= j) w -= cf[p] * Q[p];
+                for (int o = 16; o > 0; o >>= 1) c += __shfl_xor(c, o, 32);
+                if (gl == 0) cs[gp] = c;
+            }
+            __syncthreads();
+            if (pass == 0) alpha = cs[j];
+            if (t < n)
+                for (int p = 0; p <= j; p++) w -= cs[p] * Qs[p][t];
         }
-        double beta;
-        {
-            double c[BK_LZK];
-            c[0] = w * w;
-            lzf_sums(c, 0, red, cf);
-            beta = sqrt(cf[0]);
-        }
+        const double beta = sqrt(lzf_norm2(t < n ? w : 0.0, red4));
         if (t == 0) { al[j] = alpha; be[j] = beta; }
-        if (j + 1 < K) {
+        if (j + 1 < K && t < LZF) {
             const double inv = beta > 1e-300 ? 1.0 / beta : 0.0;
-#pragma unroll
-            for (int p = 0; p < BK_LZK; p++) if (p == j + 1) Q[p] = w * inv;
+            Qs[j + 1][t] = t < n ? w * inv : 0.0;
         }
         const int kk = j + 1;
         if (!(kk == K || (kk >= 6 && kk % 3 == 0))) continue;
@@ -930,7 +929,7 @@ __global__ __launch_bounds__(LZF * LZP) void k_lz_fused(const LzfItem *__restric
         if (t < 64) {
             double out;
             int stop;
-            lz_ritz(al, be, kk, K, n, L.tol, lane, out, stop);
+            lz_ritz(al, be, kk, K, n, L.tol, lane, rw, out, stop);
             if (t == 0) { lam_sh = out; stop_sh = stop; }
         }
         __syncthreads();
@@ -1070,15 +1069,94 @@ __global__ __launch_bounds__(256) void k_lzm_fin(const LzItem *__restrict__ item
         H[L.Q + (int64_t)(j + 1) * L.n + i] = H[L.w + i] * inv;
     }
 }
+// ---- the same step in one workgroup per block, for blocks of order <= LZC_N ----------------------
+// w is staged in LDS; the coefficients of a pass come from 32 groups of 32 lanes (group p sums
+// q_p(i) w(i) over the rows i = lane, lane + 32, ... with four accumulators, then across its 32
+// lanes; fixed order), the updates from one thread per row. One launch replaces the four of the
+// workgroup-slot form, and the basis is read from L2 by one CU (n (j + 1) doubles per pass).
+#define LZC_N 4096
+#define LZC_T 1024
+__global__ __launch_bounds__(LZC_T) void k_lzm_cgs(const LzItem *__restrict__ items, const int *__restrict__ sel, int j,
+                                                  double *__restrict__ H) {
+    const LzItem L = items[sel[blockIdx.x]];
+    if (j >= L.K) return;
+    __shared__ double ws[LZC_N];
+    __shared__ double cs[32];
+    __shared__ double red[LZC_T / 64];
+    const int n = L.n, t = threadIdx.x, lane = t & 63, wv = t >> 6, p = t >> 5, l = t & 31;
+    double *Q = H + L.Q, *w = H + L.w;
+    for (int i = t; i < n; i += LZC_T) ws[i] = w[i];
+    __syncthreads();
+    for (int pass = 0; pass < 2; pass++) {
+        if (p <= j) {
+            const double *qp = Q + (int64_t)p * n;
+            double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            int i = l;
+            for (; i + 96 < n; i += 128) {
+                a0 += qp[i] * ws[i]; a1 += qp[i + 32] * ws[i + 32]; a2 += qp[i + 64] * ws[i + 64]; a3 += qp[i + 96] * ws[i + 96];
+            }
+            for (; i < n; i += 32) a0 += qp[i] * ws[i];
+            double c = (a0 + a1) + (a2 + a3);
+#pragma unroll
+            for (int o = 16; o > 0; o >>= 1) c += __shfl_xor(c, o, 32);
+            if (l == 0) cs[p] = c;
+        }
+        __syncthreads();
+        if (pass == 0 && t == 0) H[L.ab + j] = cs[j];
+        for (int i = t; i < n; i += LZC_T) {
+            double v = ws[i];
+            for (int q = 0; q <= j; q++) v -= cs[q] * Q[(int64_t)q * n + i];
+            ws[i] = v;
+        }
+        __syncthreads();
+    }
+    double s2 = 0;
+    for (int i = t; i < n; i += LZC_T) s2 += ws[i] * ws[i];
+    s2 = wave_sum(s2);
+    if (lane == 0) red[wv] = s2;
+    __syncthreads();
+    double tot = 0;
+#pragma unroll
+    for (int u = 0; u < LZC_T / 64; u++) tot += red[u];
+    const double beta = sqrt(tot);
+    if (t == 0) H[L.ab + 32 + j] = beta;
+    const double inv = beta > 1e-300 ? 1.0 / beta : 0.0;
+    for (int i = t; i < n; i += LZC_T) {
+        w[i] = ws[i];
+        if (j + 1 < L.K) Q[(int64_t)(j + 1) * n + i] = ws[i] * inv;
+    }
+}
 static double *g_lzm_part = nullptr;
 static int64_t g_lzm_cap = 0;
-extern "C" int bk_lanczos_step(const LzItem *it, int n_it, int j) {
-    if (n_it <= 0) return 0;
-    std::vector<int64_t> cnt(n_it);
-    for (int q = 0; q < n_it; q++) {
-        if (it[q].K > 32) return -1;
-        cnt[q] = it[q].n > 0 ? (it[q].n + 255) / 256 : 0;
+extern "C" int bk_lanczos_step(const LzItem *it_all, int n_all, int j) {
+    if (n_all <= 0) return 0;
+    // blocks of order <= LZC_N: one workgroup each; the rest in the workgroup-slot form below
+    std::vector<LzItem> big;
+    std::vector<int> small;
+    for (int q = 0; q < n_all; q++) {
+        if (it_all[q].K > 32) return -1;
+        if (it_all[q].n <= LZC_N) small.push_back(q);
+        else big.push_back(it_all[q]);
     }
+    if (!small.empty()) {
+        std::vector<char> buf(sizeof(LzItem) * n_all + sizeof(int) * small.size() + 16);
+        memcpy(buf.data(), it_all, sizeof(LzItem) * n_all);
+        const size_t so = (sizeof(LzItem) * n_all + 15) & ~(size_t)15;
+        buf.resize(so + sizeof(int) * small.size());
+        memcpy(buf.data() + so, small.data(), sizeof(int) * small.size());
+        const char *d = (const char *)bk_stage(buf.data(), buf.size());
+        if (!d) return -1;
+        const int tok = bk_prof_begin();
+        hipLaunchKernelGGL(k_lzm_cgs, dim3((unsigned)small.size()), dim3(LZC_T), 0, g_bk_stream, (const LzItem *)d, (const int *)(d + so), j,
+                           (double *)g_heap);
+        if (tok >= 0) bk_prof_end(tok, 12, 0, 0);
+        if (BK_LAUNCH_CHECK("k_lzm_cgs")) return -1;
+    }
+    if (big.empty()) return 0;
+    const LzItem *it = big.data();
+    const int n_it = (int)big.size();
+    std::vector<int64_t> cnt(n_it);
+    for (int q = 0; q < n_it; q++) cnt[q] = it[q].n > 0 ? (it[q].n + 255) / 256 : 0;
     const LzItem *d;
     const int *pre;
     const int64_t nb = flat_stage_items(it, sizeof(LzItem) * n_it, cnt, (const void **)&d, &pre);

@@ -116,6 +116,9 @@ int bk_trsv_f32(const TrsvItem *it, int n_it, const float *A, double *V);
 /* ---- batched matrix-vector: y_it = alpha op(A_it) x_it + beta y_it ---- */
 typedef struct { int M, N, lda; int64_t a, x, y; int tri; } GemvItem;   /* tri (trans = 1 only): 1 A lower (rows >= j), 2 A upper (rows <= j) */
 int bk_gemv_f64(int trans, double alpha, double beta, const double *A, const double *x, double *y, const GemvItem *it, int n_it);
+/* nstage products y = A' x (alpha 1, beta 0) in sequence from one staging; stage s takes the
+ * items it[s n_it .. s n_it + n_it) (offsets into the heap H) */
+int bk_gemv_chain_f64(double *H, const GemvItem *it, int n_it, int nstage);
 int bk_gemv_f32(int trans, float alpha, float beta, const float *A, const float *x, float *y, const GemvItem *it, int n_it);
 /* mixed: y (f64) = alpha op(A) x + beta y with A in f32, x in f64 (products and sums in f64) */
 int bk_gemv_mixed(int trans, double alpha, double beta, const float *A, const double *x, double *y, const GemvItem *it, int n_it);
