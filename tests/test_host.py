@@ -167,7 +167,78 @@ def test_sdpa_reader_forms(lib, tmp_path):
     assert list(got["v"]) == EX1["v"] and list(got["mat"]) == EX1["mat"]
     g = tmp_path / "b.dat-s"
     g.write_text("1\n1\n2\n1\n0 1 1 1 1\n1 1 1\n")
-    assert product_read(lib, g)[0] != 0
+    rc, got = product_read(lib, g)          # an incomplete last entry is ignored (problem.c:531-535)
+    assert rc == 0 and got["mat"].tolist() == [0]
+
+
+EX1_TEXT = ("3 =mDIM\n1 =nBLOCK\n{2}\n{48, -8, 20}\n"
+            "0 1 1 1 -11\n0 1 2 2 23\n1 1 1 1 10\n1 1 1 2 4\n2 1 2 2 -8\n3 1 1 2 -8\n3 1 2 2 -2\n")
+# (file text, read as example 1: True, or rejected: False) for the reader rules of the reference's
+# read_sdpa (problem.c:426-566)
+READER_RULES = [
+    ("fortran exponents", EX1_TEXT.replace("-11", "-1.1D+01").replace("23", "2.3d1").replace(" 10", " 10.D0"), True),
+    ("separators in the data", EX1_TEXT.replace("0 1 1 1 -11", "0,1,1,1,-11").replace("1 1 1 2 4", "(1 1 1 2 4)"), True),
+    ("comment lines in the data", EX1_TEXT.replace("1 1 1 1 10\n", '* a comment\n1 1 1 1 10\n" another\n'), True),
+    ("text after the last entry", EX1_TEXT + "BEGIN.COMMENT\nsome words 1 2\nEND.COMMENT\n", True),
+    ("incomplete last entry", EX1_TEXT + "1 1 2\n", True),
+    ("zero entries", EX1_TEXT + "2 1 1 1 0\n", True),
+    ("text before more entries", EX1_TEXT.replace("2 1 2 2 -8\n", "word\n2 1 2 2 -8\n"), False),
+    ("non-integer index", EX1_TEXT.replace("1 1 1 2 4", "1 1 1.5 2 4"), False),
+    ("number glued to text", EX1_TEXT.replace("-8\n3 1 1", "-8x\n3 1 1"), False),
+    ("non-finite value", EX1_TEXT.replace(" 23", " inf"), False),
+]
+
+
+def _ref_solve(path):
+    """The reference library (oracle/_ref, built from the reference's sources): its return code and
+    primal objective on a file, or None when that library is not built here."""
+    so = os.path.join(ROOT, "oracle", "_ref", "libbrisk.so")
+    if not os.path.exists(so):
+        return None
+    R = ctypes.CDLL(so)
+    R.brisk_result_new.restype = ctypes.c_void_p
+    R.brisk_result_delete.argtypes = [ctypes.c_void_p]
+    R.brisk_set_output.argtypes = [I, ctypes.c_void_p]
+    R.brisk_solve_file.argtypes = [ctypes.c_char_p, I, ctypes.POINTER(ctypes.c_char_p), ctypes.c_void_p]
+    R.brisk_result_pobj.argtypes = [ctypes.c_void_p]
+    R.brisk_result_pobj.restype = D
+    R.brisk_set_output(1, None)
+    r = R.brisk_result_new()
+    no = (ctypes.c_char_p * 1)()
+    rc = R.brisk_solve_file(str(path).encode(), 0, no, r)
+    pobj = R.brisk_result_pobj(r)
+    R.brisk_result_delete(r)
+    return rc, pobj
+
+
+@pytest.mark.parametrize("case", READER_RULES, ids=[c[0] for c in READER_RULES])
+def test_sdpa_reader_rules(lib, tmp_path, case):
+    """The reader rules of the reference's read_sdpa: Fortran D exponents, separators and comment
+    lines anywhere, text after the last entry and an incomplete last entry ignored, zero entries
+    dropped; text before more entries, non-integer indices, numbers glued to text and non-finite
+    values rejected. Each accepted file reads as the canonical example 1; the reference library,
+    where it is built, accepts and rejects the same files and solves an accepted one to the
+    canonical file's objective."""
+    name, text, ok = case
+    f = tmp_path / "case.dat-s"
+    f.write_text(text)
+    rc, got = product_read(lib, f)
+    if not ok:
+        assert rc != 0, name
+    else:
+        assert rc == 0, name
+        assert got["m"] == 3 and got["bs"].tolist() == [2] and got["c"].tolist() == [48, -8, 20]
+        for k in ("mat", "blk", "i", "j", "v"):
+            assert np.asarray(got[k], dtype=np.float64).tolist() == [float(x) for x in EX1[k]], (name, k)
+    canon = tmp_path / "canon.dat-s"
+    canon.write_text(EX1_TEXT)
+    ref, ref0 = _ref_solve(f), _ref_solve(canon)
+    if ref is None:
+        return
+    if ok:
+        assert ref[0] == ref0[0] == 0 and abs(ref[1] - ref0[1]) <= 1e-12 * (1 + abs(ref0[1])), (name, ref, ref0)
+    else:
+        assert ref[0] != 0, (name, ref)
     assert product_read(lib, tmp_path / "missing.dat-s")[0] != 0
 
 
