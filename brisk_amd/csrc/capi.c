@@ -117,7 +117,7 @@ API int brisk_result_trace(const BriskResult *r, double *po, double *dob, double
 
 /* ---- options ---- */
 int brisk_parse_options(int nopt, const char *const *opts, SolveOpts *o, int m, int *slots, double *heapgb) {
-    o->tol = 1e-8; o->maxit = 100; o->verbose = 0; o->mixed_until = 1e-6; o->time_limit = 0; o->hsd = 0; o->apply = -1; o->tput = 0;
+    o->tol = 1e-8; o->maxit = 100; o->verbose = 0; o->mixed_until = 1e-6; o->time_limit = 0; o->hsd = 0; o->apply = -1; o->tput = 0; o->polish = 1;
     o->prec = m >= 1000 ? PREC_MIXED : PREC_F64;
     for (int k = 0; k < nopt; k++) {
         const char *a = opts[k];
@@ -136,6 +136,7 @@ int brisk_parse_options(int nopt, const char *const *opts, SolveOpts *o, int m, 
         else if (!strcmp(a, "-slots")) { NEED(); if (slots) *slots = atoi(nx); }
         else if (!strcmp(a, "-heapgb")) { NEED(); if (heapgb) *heapgb = atof(nx); }
         else if (!strcmp(a, "-hsd")) { NEED(); o->hsd = atoi(nx) > 0; }
+        else if (!strcmp(a, "-nopolish")) o->polish = 0;
         else if (!strcmp(a, "-throughput")) { NEED(); o->tput = atoi(nx) > 0; }
         else if (!strcmp(a, "-schurapply")) {
             NEED();
@@ -243,7 +244,13 @@ static int solve_many(int n, const BriskProblem *pr, int nopt, const char *const
     }
     bk_sync();
     const double t2 = bk_now();
-    if (ipm_dimacs(SP, n)) { rc = -4; goto out; }
+    {
+        int *st0 = malloc(sizeof(int) * n);
+        for (int q = 0; q < n; q++) st0[q] = S[q].status;
+        const int e = ipm_dimacs(SP, n) || ipm_polish(SP, n, st0);
+        free(st0);
+        if (e) { rc = -4; goto out; }
+    }
     bk_sync();
     const double t3 = bk_now();
     for (int q = 0; q < n; q++) if (ipm_finish(SP[q])) { rc = -4; goto out; }

@@ -1679,7 +1679,19 @@ out:
  *   A(X) - b = du o (A~(X~) - b~), <C,X> = bs cs <C~,X~>, b'y = bs cs b~'y~, <X,Z> = bs cs <X~, Z~>.
  * lambda_min(X) and lambda_min(Z) are tested by Cholesky: a factor that exists proves the block
  * positive definite (error 0); otherwise the Lanczos bound gives the violation. */
+static int dimacs_point(Solve **S, int ns);
 int ipm_dimacs(Solve **S, int ns) {
+    /* the returned point into X, y (the best iterate unless the run ended optimal) */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (S[q]->status != 0) {
+        lin_add(S[q]->P.T, S[q]->X, S[q]->Xb, 1.0, -1, 0, -1, 0, -1, 0);
+        lin_add(S[q]->P.m, S[q]->y, S[q]->yb, 1.0, -1, 0, -1, 0, -1, 0);
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) return -1;
+    return dimacs_point(S, ns);
+}
+/* the DIMACS errors, objectives and status of the point (X, y) as it stands */
+static int dimacs_point(Solve **S, int ns) {
     int rc = -1;
     int *act = calloc(ns, sizeof(int)), *okx = calloc(ns, sizeof(int)), *okz = calloc(ns, sizeof(int));
     double *lam = calloc(ns, sizeof(double));
@@ -1688,13 +1700,6 @@ int ipm_dimacs(Solve **S, int ns) {
     double *hrp = malloc(sizeof(double) * mmax);
     if (batch_scratch(ns, nsl + ns)) goto out;
     double *H = heapp();
-    /* the returned point into X, y (the best iterate unless the run ended optimal) */
-    LinL_clear(&g_ll);
-    for (int q = 0; q < ns; q++) if (S[q]->status != 0) {
-        lin_add(S[q]->P.T, S[q]->X, S[q]->Xb, 1.0, -1, 0, -1, 0, -1, 0);
-        lin_add(S[q]->P.m, S[q]->y, S[q]->yb, 1.0, -1, 0, -1, 0, -1, 0);
-    }
-    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
     /* rp = A(X) - b (scaled); W1 = C - A'y (scaled Z on the data) */
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) {
@@ -2209,8 +2214,85 @@ int ipm_end_batch(Solve **S, int ns) {
     if (t_ > 0) bk_prof_host(47, bk_now() - t_);
     return r_;
 }
+/* The best-iterate polish of the reference (polish_solution, solver.c:7012-7040, applied to
+ * the returned point at solver.c:15534-15545): X_p = X + A*(w) with (A A') w = b - A(X), the
+ * projection of X onto A(X) = b in the Euclidean metric, y kept and Z = C - A*(y) as before. The
+ * Gram matrix A A' is assembled at X = Zi = I and factored in FP64 as for the start point, and
+ * the system solved by PCG to 1e-14. The polished point is kept when its acceptance error (the
+ * measure the status is set by) falls below half the point's; else the point is put back. The
+ * projection moves the primal residual (error 1) and leaves the other errors nearly as they were,
+ * so it runs only where error 1 is at least half the acceptance error: elsewhere it cannot halve
+ * it, and a keep-if-lower rule would decide near-ties by rounding (the reference keeps any gain of
+ * its own measure, solver.c:7012-7040; the margin makes the choice stable). For the solves whose
+ * run ended optimal, in difficulties or at reduced accuracy (not at the iteration or time limit,
+ * as in the reference), unless -nopolish; in either precision, the projection being FP64 here
+ * (the reference skips its mixed-precision solves, whose end phase is its own); a self-dual
+ * embedding's point is (X, y) / tau, as returned. st0: the runs' statuses before their returned points were
+ * measured. */
+int ipm_polish(Solve **S, int ns, const int *st0) {
+    int rc = -1, na = 0;
+    int *act = calloc(ns, sizeof(int)), *pol = calloc(ns, sizeof(int)), *fail = calloc(ns, sizeof(int));
+    double *rel = calloc(ns, sizeof(double)), *old = calloc((size_t)ns * 9, sizeof(double));
+    Solve **A = malloc(sizeof(Solve *) * (ns + 1));
+    for (int q = 0; q < ns; q++) {
+        Solve *s = S[q];
+        pol[q] = act[q] = s->o.polish && (st0[q] == 0 || st0[q] == 4 || st0[q] == 5) &&
+                          s->res_err[0] >= 0.5 * acc_err(s->res_err);
+        na += act[q];
+    }
+    if (!na) { rc = 0; goto out; }
+    /* the point and its measure kept: X in Xb, its Z (W1) in Q; rhs = b - A(X) */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        Solve *s = S[q];
+        lin_add(s->P.T, s->Xb, s->X, 1.0, -1, 0, -1, 0, -1, 0);
+        lin_add(s->P.T, s->Q, s->W1, 1.0, -1, 0, -1, 0, -1, 0);
+        lin_add(s->P.m, s->rhs, s->bvec, 1.0, -1, 0, -1, 0, -1, 0);
+        for (int e = 0; e < 6; e++) old[9 * q + e] = s->res_err[e];
+        old[9 * q + 6] = s->res_pobj; old[9 * q + 7] = s->res_dobj; old[9 * q + 8] = s->status;
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (aop(S, ns, act, OFF(X), OFF(rhs), -1.0, 1.0)) goto out;
+    /* the Gram factor at X = Zi = I, w = (A A')^{-1} (b - A(X)) */
+    if (set_identity(S, ns, act, OFF(X)) || set_identity(S, ns, act, OFF(Zi))) goto out;
+    if (schur_assemble(S, ns, act, 1) || schur_factor(S, ns, act, 1, fail)) goto out;
+    for (int q = 0; q < ns; q++) if (act[q] && fail[q]) act[q] = 0;
+    if (schur_solve(S, ns, act, OFF(rhs), OFF(dy), 1e-14, rel)) goto out;
+    /* X_p = X + A*(w) (the solves whose Gram factor failed get their point back) */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (pol[q]) lin_add(S[q]->P.T, S[q]->X, S[q]->Xb, 1.0, -1, 0, -1, 0, -1, 0);
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    if (aty(S, ns, act, OFF(dy), OFF(X), 1.0)) goto out;
+    na = 0;
+    for (int q = 0; q < ns; q++) if (act[q]) { S[q]->status = st0[q]; A[na++] = S[q]; }
+    if (na && dimacs_point(A, na)) goto out;
+    /* kept when better; else the point and its measure back */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) {
+        if (!act[q]) continue;
+        Solve *s = S[q];
+        if (s->o.verbose)
+            solve_log("polish: acceptance error %.3e -> %.3e (errors %.2e %.2e %.2e %.2e %.2e %.2e), %s\n", acc_err(old + 9 * q),
+                      acc_err(s->res_err), s->res_err[0], s->res_err[1], s->res_err[2], s->res_err[3], s->res_err[4], s->res_err[5],
+                      acc_err(s->res_err) < 0.5 * acc_err(old + 9 * q) ? "kept" : "not kept");
+        if (acc_err(s->res_err) < 0.5 * acc_err(old + 9 * q)) continue;
+        lin_add(s->P.T, s->X, s->Xb, 1.0, -1, 0, -1, 0, -1, 0);
+        lin_add(s->P.T, s->W1, s->Q, 1.0, -1, 0, -1, 0, -1, 0);
+        for (int e = 0; e < 6; e++) s->res_err[e] = old[9 * q + e];
+        s->res_pobj = old[9 * q + 6]; s->res_dobj = old[9 * q + 7]; s->status = (int)old[9 * q + 8];
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    rc = 0;
+out:
+    free(act); free(pol); free(fail); free(rel); free(old); free(A);
+    return rc;
+}
 static int ipm_end_batch_(Solve **S, int ns) {
-    if (ipm_dimacs(S, ns)) return -1;
+    int *st0 = malloc(sizeof(int) * (ns + 1));
+    for (int q = 0; q < ns; q++) st0[q] = S[q]->status;
+    const int e = ipm_dimacs(S, ns) || ipm_polish(S, ns, st0);
+    free(st0);
+    if (e) return -1;
     for (int q = 0; q < ns; q++) if (ipm_finish(S[q])) return -1;
     return 0;
 }

@@ -98,6 +98,7 @@ typedef struct {
     int hsd;                /* 1: self-dual embedding, 0: infeasible start */
     int apply;              /* solves with the Schur factor: -1 a cost model on the solve's own order and precision (default), 0 explicit inverse, 1 block triangular solves */
     int tput;               /* 1: the plan choices for throughput in large batches (256-column panels from order 1024, blocked solves for f64 factors from order 1500); 0: for latency */
+    int polish;             /* 1: the FP64 solves' returned point polished (projected onto A(X) = b) when that improves it (reference Params.polish, -nopolish) */
 } SolveOpts;
 
 typedef struct Solve {
@@ -161,7 +162,8 @@ int ipm_start_batch(Solve **S, int ns);
 int ipm_iterate_batch(Solve **S, int ns);   /* one iteration; sets done on solves that end */
 int ipm_iterate_hsd(Solve **S, int ns);     /* the same for the self-dual embedding */
 int ipm_dimacs(Solve **S, int ns);          /* DIMACS errors and status of the returned points (device) */
-int ipm_end_batch(Solve **S, int ns);       /* DIMACS on the device, status, results to the host */
+int ipm_polish(Solve **S, int ns, const int *st0);  /* after ipm_dimacs: the returned points polished where that helps (st0: the runs' statuses before it) */
+int ipm_end_batch(Solve **S, int ns);       /* DIMACS on the device, polish, status, results to the host */
 int ipm_finish(Solve *S);                   /* results of one solve to the host (after ipm_end_batch's DIMACS) */
 void solve_log(const char *fmt, ...);
 void ipm_release_scratch(void);           /* frees the batch scratch (before the heap is re-sized) */

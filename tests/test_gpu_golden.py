@@ -3,6 +3,7 @@ library on the same instances): same status, objectives within 1e-7 relative, DI
 <= 1e-8, and the returned y and X within the iterate-distance bar (tests/_parity.py); the SDPA
 example files through brisk_solve_file; the full-size workloads in both precisions."""
 import os
+import numpy as np
 import pytest
 import brisk_amd
 from _ctypes_util import ROOT, golden, golden_instance
@@ -85,9 +86,9 @@ HSD_CASES = sorted(k for k in SMALL if k != "file_arch0")
 # control1 drives tau to 5e-5 with kappa -> 0: the primal residual of X / tau then carries the
 # direction's rounding times 1 / tau, and the restated embedding (no Gondzio correctors) ends
 # within a few units of 1e-8 in either precision, at 28 iterations against the reference's 14;
-# which side of 1e-8 it lands on in mixed precision depends on the last bits of the factor
-HSD_UNPINNED = {("file_control1", "mixed"): "control1 at tau ~ 5e-5: the embedding without the reference's correctors "
-                                            "ends at a primal error of 1e-8 to 2e-8 (REDUCED) in mixed precision"}
+# the polish of the returned point (projection onto A(X) = b) takes that residual to 1e-15, so
+# the case no longer depends on the last bits of the factor
+HSD_UNPINNED = {}
 
 
 @pytest.mark.parametrize("prec", ["double", "mixed"])
@@ -101,3 +102,60 @@ def test_hsd_vs_reference(name, prec, request):
     a = golden_instance(rec).arrays()
     r = brisk_amd.solve_sdpa(a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"], options=["-prec", prec, "-hsd", "1"])
     check(r, rec)
+
+
+def _polish_runs(name, *o):
+    """The run of a golden instance with options o, and the same run with -nopolish."""
+    rec = SMALL[name]
+    if "file" in rec:
+        path = os.path.join(ROOT, "tests", "golden", rec["file"])
+        run = lambda *x: brisk_amd.solve_file(path, options=[*o, *x])
+    else:
+        a = golden_instance(rec).arrays()
+        run = lambda *x: brisk_amd.solve_sdpa(a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"], options=[*o, *x])
+    return run(), run("-nopolish")
+
+
+def _acc(e):
+    """The acceptance error: the largest of DIMACS errors 1, 2, 4, |5|, |6| (brisk.h:187-189)."""
+    return max(e[0], e[1], e[3], abs(e[4]), abs(e[5]))
+
+
+@pytest.mark.parametrize("prec", ["double", "mixed"])
+@pytest.mark.parametrize("hsd", ["0", "1"])
+@pytest.mark.parametrize("name", GEN + FILES)
+def test_polish_vs_unpolished(name, hsd, prec):
+    """The polish of a solve's returned point (projection onto A(X) = b, the reference's
+    polish_solution, solver.c:7012-7040): the point returned by default is never worse by the
+    acceptance error than the same run with -nopolish, has the same status and iterations, and
+    equals it where the polish was not kept; where it was kept, the acceptance error is below
+    half the unpolished one, the primal residual (DIMACS error 1) fell and the objectives moved by
+    less than the tolerance. The standard iteration's points also still match the reference's
+    outputs."""
+    print("case", name, "hsd", hsd, prec, flush=True)
+    r, r0 = _polish_runs(name, "-hsd", hsd, "-prec", prec)
+    print("  polished acc %.3e err1 %.3e, unpolished acc %.3e err1 %.3e" % (_acc(r.dimacs), r.dimacs[0], _acc(r0.dimacs), r0.dimacs[0]))
+    assert r.iterations == r0.iterations
+    assert r.status == r0.status or (r.status == 0 and r0.status == 5)
+    assert _acc(r.dimacs) <= _acc(r0.dimacs)
+    if _acc(r.dimacs) < _acc(r0.dimacs):
+        assert _acc(r.dimacs) < 0.5 * _acc(r0.dimacs) and r.dimacs[0] < r0.dimacs[0]
+        assert abs(r.pobj - r0.pobj) <= 1e-8 * (1 + abs(r0.pobj)) and r.dobj == r0.dobj
+    else:
+        assert np.array_equal(r.dimacs, r0.dimacs) and r.pobj == r0.pobj
+    if hsd == "0" and prec == "double":
+        check(r, SMALL[name], "polished")
+
+
+def test_polish_kept():
+    """The self-dual embedding's points carry their primal residual as the largest error on
+    several files (control1, example2: all of it): the polish is kept there and cuts the
+    acceptance error at least in half."""
+    kept = []
+    for name in GEN + FILES:
+        r, r0 = _polish_runs(name, "-hsd", "1", "-prec", "double")
+        if _acc(r.dimacs) < _acc(r0.dimacs):
+            assert _acc(r.dimacs) < 0.5 * _acc(r0.dimacs)
+            kept.append((name, _acc(r0.dimacs), _acc(r.dimacs)))
+    print("kept:", kept)
+    assert len(kept) >= 2
