@@ -1,5 +1,5 @@
-// chol_diag.h - factor and invert one 64 x 64 diagonal block in a workgroup of 512 threads
-// (included by k_dense.hip).
+// chol_diag.h - factor and invert one 64 x 64 diagonal block in a workgroup of 512 threads, and
+// optionally multiply a 64-row panel tile by the inverse's transpose (included by k_dense.hip).
 //
 // Layout: thread (lane r, wave w), eight waves (two per SIMD, whose dependency chains
 // interleave), holds row r of the block at the 8 columns c = 8u + w in registers. The factor runs
@@ -106,12 +106,16 @@ __device__ __forceinline__ void diag_inv_mfma(T (*Ls)[65], T (*Xs)[65], T (*Ts)[
     __syncthreads();
 }
 
-// Factor the nb x nb block at Ab (leading dimension n, lower triangle read) in place as L L'
-// (strict upper part zeroed) and write L^{-1} (64 x 64, zero outside the nb x nb lower part) to
-// Di. Rows and columns beyond nb are factored as the identity. Returns 0, or the 1-based column
-// whose pivot is not positive; pm_out is the smallest squared pivot.
-template <typename T>
-__device__ int diag_factor_inverse(T *Ab, int n, int nb, T *Di, T &pm_out) {
+// Factor the nb x nb block at Ab (leading dimension n, lower triangle read) as L L', write L
+// (strict upper part zeroed) to Lo (leading dimension ldo; Ab itself for the factor in place, or
+// nullptr) and L^{-1} (64 x 64, zero outside the nb x nb lower part) to Di (or nullptr). Rows and
+// columns beyond nb are factored as the identity. With PANEL, the 64-column tile at P (leading
+// dimension ldp, prow <= 64 rows) is then multiplied in place by L^{-T} on MFMA: the panel step
+// of a blocked factorization, L_I = A_I L^{-T}. Returns 0, or the 1-based column whose pivot is
+// not positive (then nothing is written); pm_out is the smallest squared pivot.
+template <typename T, bool PANEL = false>
+__device__ int diag_factor_inverse(const T *Ab, int n, int nb, T *Lo, int ldo, T *Di, T &pm_out, T *P = nullptr, int ldp = 0,
+                                   int prow = 0) {
     __shared__ T Ls[64][65];       // Ls[c][r] = L(r, c)
     __shared__ T Xs[64][65];       // Xs[c][r] = X(r, c), X = L^{-1}
     __shared__ T Ts[64][65];
@@ -237,9 +241,32 @@ __device__ int diag_factor_inverse(T *Ab, int n, int nb, T *Di, T &pm_out) {
 #pragma unroll
     for (int u = 0; u < 8; u++) {
         const int c = 8 * u + w;
-        if (r < nb && c < nb) Ab[r + (int64_t)c * n] = v[u];
-        Di[r + 64 * c] = (r < nb && c < nb && r >= c) ? Xs[c][r] : (T)0;
+        if (Lo && r < nb && c < nb) Lo[r + (int64_t)c * ldo] = v[u];
+        if (Di) Di[r + 64 * c] = (r < nb && c < nb && r >= c) ? Xs[c][r] : (T)0;
     }
     pm_out = pm;
+    if (PANEL) {
+        // the tile into Ts (Ts[c][r] = P(r, c), rows past prow zero), then P := P X' with wave w
+        // forming the 16 x 16 output tiles (w & 3, 2 (w >> 2)) and (w & 3, 2 (w >> 2) + 1)
+        typedef typename DiagMF<T>::v4 v4;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int c = 8 * u + w;
+            Ts[c][r] = r < prow ? P[r + (int64_t)c * ldp] : (T)0;
+        }
+        __syncthreads();
+        const int lane = tid & 63, l16 = lane & 15, g4 = lane >> 4, row0 = 16 * (w & 3);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int col0 = 16 * (2 * (w >> 2) + h);
+            v4 acc = {0, 0, 0, 0};
+#pragma unroll
+            for (int kk = 0; kk < 64; kk += 4) acc = DiagMF<T>::mma(Xs[kk + g4][col0 + l16], Ts[kk + g4][row0 + l16], acc);
+            if (row0 + l16 < prow) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) P[(row0 + l16) + (int64_t)(col0 + DiagMF<T>::out(g4, q)) * ldp] = acc[q];
+            }
+        }
+    }
     return 0;
 }

@@ -335,9 +335,10 @@ extern "C" int bk_gemm_f32(int ta, int tb, float alpha, float beta, const float 
 }
 
 // ---- Cholesky: the diagonal block ------------------------------------------------------------
-// One workgroup per matrix: factors the nb x nb diagonal block at (k0, k0) in LDS (right-looking,
-// unblocked), records the first failing column and the smallest squared pivot, writes L_kk back
-// (strict upper part of the block zeroed) and its inverse to Dinv.
+// One workgroup per matrix whose last 64-column step this is (no rows below the block): factors
+// the nb x nb diagonal block at (k0, k0) in LDS, records the first failing column and the
+// smallest squared pivot, writes L_kk back (strict upper part of the block zeroed) and its inverse
+// to Dinv.
 template <typename T>
 __global__ __launch_bounds__(512) void k_potrf_diag(const MItem *__restrict__ items, int nit, int k0, T *__restrict__ A,
                                                     T *__restrict__ Dinv, int *__restrict__ info, T *__restrict__ pmin) {
@@ -345,16 +346,58 @@ __global__ __launch_bounds__(512) void k_potrf_diag(const MItem *__restrict__ it
     if (q >= nit) return;
     const MItem it = items[q];
     const int n = it.n;
-    if (k0 >= n || info[it.slot] != 0) return;
-    const int nb = min(64, n - k0);
+    if (k0 >= n || k0 + 64 < n || info[it.slot] != 0) return;
+    const int nb = n - k0;
     T *Ab = A + it.off + k0 + (int64_t)k0 * n;
     T *Di = Dinv + it.off2 + (int64_t)(k0 / 64) * 64 * 64;
     T pm;
-    const int f = diag_factor_inverse<T>(Ab, n, nb, Di, pm);
+    const int f = diag_factor_inverse<T>(Ab, n, nb, Ab, n, Di, pm);
     if (threadIdx.x == 0) {
         if (f) info[it.slot] = k0 + f;
         else pmin[it.slot] = (k0 == 0) ? pm : min(pmin[it.slot], pm);
     }
+}
+// The diagonal block and the panel below it in one launch, for the matrices with rows below the
+// block at step k0: workgroup t of a matrix factors and inverts the 64 x 64 block itself (every
+// workgroup the same arithmetic on the same input, so all agree bit for bit) and multiplies the
+// panel tile of rows k0 + 64 (t + 1) .. by L_kk^{-T} in place. Workgroup 0 writes L_kk to the
+// staging area W (64 x 64 per step at the matrix's scratch, copied into the matrix by
+// k_potrf_lcopy after the last step: the block is read by every workgroup of this launch and must
+// not change under them), Dinv, the failing column and the smallest pivot. The panel's rows are
+// read whole before they are written, by the one workgroup that owns them.
+template <typename T>
+__global__ __launch_bounds__(512) void k_potrf_panel(const MItem *__restrict__ items, const int *__restrict__ pre, int nit, int k0,
+                                                     T *__restrict__ A, T *__restrict__ Dinv, T *__restrict__ W,
+                                                     int *__restrict__ info, T *__restrict__ pmin) {
+    const int q = flat_item(pre, nit, blockIdx.x);
+    const MItem it = items[q];
+    const int n = it.n, t = blockIdx.x - pre[q];
+    if (k0 + 64 >= n || info[it.slot] != 0) return;
+    const T *Ab = A + it.off + k0 + (int64_t)k0 * n;
+    const int r0 = k0 + 64 * (t + 1), prow = min(64, n - r0);
+    T *Pt = A + it.off + r0 + (int64_t)k0 * n;
+    T *Lo = t == 0 ? W + it.off3 + (int64_t)(k0 / 64) * 64 * 64 : nullptr;
+    T *Di = t == 0 ? Dinv + it.off2 + (int64_t)(k0 / 64) * 64 * 64 : nullptr;
+    T pm;
+    const int f = diag_factor_inverse<T, true>(Ab, n, 64, Lo, 64, Di, pm, Pt, n, prow);
+    if (t == 0 && threadIdx.x == 0) {
+        if (f) info[it.slot] = k0 + f;
+        else pmin[it.slot] = (k0 == 0) ? pm : min(pmin[it.slot], pm);
+    }
+}
+// the staged diagonal factors L_kk of the steps with a panel into the matrices' diagonal blocks
+// (after the factorization; 64 x 64 each, one workgroup per block; a matrix whose factorization
+// failed keeps whatever its steps left)
+template <typename T>
+__global__ void k_potrf_lcopy(const MItem *__restrict__ items, const int *__restrict__ pre, int nit, T *__restrict__ A,
+                              const T *__restrict__ W) {
+    const int q = flat_item(pre, nit, blockIdx.x);
+    const MItem it = items[q];
+    const int n = it.n, k0 = 64 * (blockIdx.x - pre[q]);
+    if (k0 + 64 >= n) return;
+    const T *src = W + it.off3 + (int64_t)(k0 / 64) * 64 * 64;
+    T *dst = A + it.off + k0 + (int64_t)k0 * n;
+    for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) dst[(e & 63) + (int64_t)(e >> 6) * n] = src[e];
 }
 
 // the strict upper triangle of every matrix set to zero
@@ -455,9 +498,15 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
     int64_t ztot = 0;
     for (int q = 0; q < n_it; q++) { ((int *)(buf.data() + zpre_off))[q] = (int)ztot; ztot += zc[q]; }
     ((int *)(buf.data() + zpre_off))[n_it] = (int)ztot;
-    struct Step { int k0; GemmPlan upd, panel, trail; int has_trail; };
+    // the staged diagonal factors' copy: one workgroup per 64-column step with a panel
+    const size_t lc_off = buf_align(buf);
+    buf.resize(lc_off + sizeof(int) * (n_it + 1));
+    int64_t lc_tot = 0;
+    for (int q = 0; q < n_it; q++) { ((int *)(buf.data() + lc_off))[q] = (int)lc_tot; lc_tot += it[q].n > 64 ? (it[q].n - 1) / 64 : 0; }
+    ((int *)(buf.data() + lc_off))[n_it] = (int)lc_tot;
+    struct Step { int k0; GemmPlan upd, trail; int has_trail; size_t ppre; int64_t npanel; int last; };
     std::vector<Step> st;
-    std::vector<GemmItem> gu, gp, gt;
+    std::vector<GemmItem> gu, gt;
     // panel width, chosen per matrix (its item's opt, else its order) so that a matrix is factored
     // the same way in any batch: 128 columns for latency (two n = 2000 blocks in FP64: 1.37 ->
     // 1.30 ms against 256), 256 where the trailing updates run at higher throughput with K = 256
@@ -465,7 +514,21 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
     // 1024: 365 -> 374 solves/s)
     auto panel_width = [](const MItem &m) { return m.opt == 128 || m.opt == 256 ? m.opt : m.n < POTRF_PW256_N ? 128 : 256; };
     for (int k0 = 0; k0 < nmax; k0 += 64) {
-        gu.clear(); gp.clear(); gt.clear();
+        gu.clear(); gt.clear();
+        Step S;
+        S.k0 = k0;
+        // the fused diagonal-block and panel launch: one workgroup per 64-row panel tile of the
+        // matrices with rows below the block (flat prefix per item); the rest take their last step
+        S.ppre = buf_align(buf);
+        buf.resize(S.ppre + sizeof(int) * (n_it + 1));
+        S.npanel = 0; S.last = 0;
+        for (int q = 0; q < n_it; q++) {
+            const int n = it[q].n;
+            ((int *)(buf.data() + S.ppre))[q] = (int)S.npanel;
+            if (k0 + 64 < n) S.npanel += (n - k0 - 64 + 63) / 64;
+            else if (k0 < n) S.last = 1;
+        }
+        ((int *)(buf.data() + S.ppre))[n_it] = (int)S.npanel;
         for (int q = 0; q < n_it; q++) {
             const int n = it[q].n;
             if (k0 >= n) continue;
@@ -480,18 +543,6 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
                 g.guard = it[q].slot; g.tri = 0;
                 gu.push_back(g);
             }
-            if (k0 + 64 < n) {
-                // panel, in place: A[k0+64:, k0:k0+64] := A[k0+64:, k0:k0+64] Dinv_k' (Dinv_k'
-                // upper triangular). The 64 x 64 tiles span the panel's full width, so every
-                // workgroup reads its rows completely before it writes them.
-                g.M = n - k0 - 64; g.N = 64; g.K = 64;
-                g.lda = n; g.ldb = 64; g.ldc = n;
-                g.a = it[q].off + (k0 + 64) + (int64_t)k0 * n;
-                g.b = it[q].off2 + (int64_t)(k0 / 64) * 64 * 64;
-                g.c = g.a;
-                g.guard = it[q].slot; g.tri = 16;
-                gp.push_back(g);
-            }
             const int e = K0 + PW;
             if (sub == PW / 64 - 1 && e < n) {
                 // trailing: A[e:, e:] -= L[e:, K0:e] L[e:, K0:e]' (lower triangle)
@@ -503,10 +554,7 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
                 gt.push_back(g);
             }
         }
-        Step S;
-        S.k0 = k0;
         S.upd = gemm_plan<T>(buf, gu.data(), (int)gu.size());
-        S.panel = gemm_plan<T>(buf, gp.data(), (int)gp.size());
         S.trail = gemm_plan<T>(buf, gt.data(), (int)gt.size());
         S.has_trail = !gt.empty();
         st.push_back(S);
@@ -522,11 +570,29 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
             const int k0 = st[u].k0;
             if (gemm_exec<T>(d, st[u].upd, 0, 1, (T)-1, (T)1, A, A, A, info)) return -1;
             const int tok = bk_prof_begin();
-            hipLaunchKernelGGL((k_potrf_diag<T>), dim3(n_it), dim3(512), 0, g_bk_stream, d_it, n_it, k0, A, Dinv, info, pmin);
-            if (tok >= 0) { double fl = 0; for (int q = 0; q < n_it; q++) { const double b = it[q].n > k0 ? fmin(64.0, it[q].n - k0) : 0; fl += b * b * b / 3.0 + b * b * b / 3.0; } bk_prof_end(tok, 8, fl, 0); }
-            if (BK_LAUNCH_CHECK("k_potrf_diag")) return -1;
-            if (gemm_exec<T>(d, st[u].panel, 0, 1, (T)1, (T)0, A, Dinv, A, info)) return -1;
+            if (st[u].npanel > 0) {
+                hipLaunchKernelGGL((k_potrf_panel<T>), dim3((unsigned)st[u].npanel), dim3(512), 0, g_bk_stream, d_it,
+                                   (const int *)(d + st[u].ppre), n_it, k0, A, Dinv, W, info, pmin);
+                if (BK_LAUNCH_CHECK("k_potrf_panel")) return -1;
+            }
+            if (st[u].last) {
+                hipLaunchKernelGGL((k_potrf_diag<T>), dim3(n_it), dim3(512), 0, g_bk_stream, d_it, n_it, k0, A, Dinv, info, pmin);
+                if (BK_LAUNCH_CHECK("k_potrf_diag")) return -1;
+            }
+            if (tok >= 0) {
+                double fl = 0;
+                for (int q = 0; q < n_it; q++) {
+                    const double b = it[q].n > k0 ? fmin(64.0, it[q].n - k0) : 0, rows = it[q].n > k0 + 64 ? it[q].n - k0 - 64 : 0;
+                    fl += 2.0 * b * b * b / 3.0 + 2.0 * rows * 64.0 * 64.0;
+                }
+                bk_prof_end(tok, 8, fl, 0);
+            }
             if (st[u].has_trail && gemm_exec<T>(d, st[u].trail, 0, 1, (T)-1, (T)1, A, A, A, info)) return -1;
+        }
+        if (lc_tot > 0) {
+            hipLaunchKernelGGL((k_potrf_lcopy<T>), dim3((unsigned)lc_tot), dim3(256), 0, g_bk_stream, d_it, (const int *)(d + lc_off), n_it, A,
+                               (const T *)W);
+            if (BK_LAUNCH_CHECK("k_potrf_lcopy")) return -1;
         }
         return 0;
     };
@@ -589,6 +655,12 @@ static int trtri_run(const MItem *it, int n_it, const T *L, const T *Dinv, T *Li
         ztot += flat_cap(((int64_t)it[q].n * it[q].n + 255) / 256, 1024);
     }
     ((int *)(buf.data() + zpre_off))[n_it] = (int)ztot;
+    // the staged diagonal factors' copy: one workgroup per 64-column step with a panel
+    const size_t lc_off = buf_align(buf);
+    buf.resize(lc_off + sizeof(int) * (n_it + 1));
+    int64_t lc_tot = 0;
+    for (int q = 0; q < n_it; q++) { ((int *)(buf.data() + lc_off))[q] = (int)lc_tot; lc_tot += it[q].n > 64 ? (it[q].n - 1) / 64 : 0; }
+    ((int *)(buf.data() + lc_off))[n_it] = (int)lc_tot;
     std::vector<GemmPlan> p1, p2;
     std::vector<GemmItem> g1, g2;
     for (int s = 64; s < nmax; s *= 2) {
