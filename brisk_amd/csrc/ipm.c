@@ -420,7 +420,8 @@ static int chol_sides_(Solve **S, int ns, int nside, const int *act, const size_
             const HProb *P = &S[q]->P;
             SAux *A = aux_of(S[q]);
             for (int k = 0; k < P->nb; k++) {
-                MItem it = { P->bn[k], slot++, F_(S[q], fL[sd]) + P->boff[k], F_(S[q], fD[sd]) + A->dboff[k], F_(S[q], fW[sd]) + P->boff[k], 0 };
+                MItem it = { P->bn[k], slot++, F_(S[q], fL[sd]) + P->boff[k], F_(S[q], fD[sd]) + A->dboff[k], F_(S[q], fW[sd]) + P->boff[k], 0,
+                             S[q]->o.tput ? BK_POTRF_SPLIT : 0, 0 };
                 MItL_add(&g_ml, it);
             }
         }
@@ -868,7 +869,7 @@ static int schur_factor_(Solve **S, int ns, const int *act, int force64, int *fa
                 const int64_t mul = prec ? 1 : 2;
                 fi[t].n = s->P.m; fi[t].slot = q; fi[t].off = mul * s->MF; fi[t].off2 = mul * s->MDinv; fi[t].off3 = mul * s->Mwork;
                 fi[t].off4 = 0; fi[t].pad = 0;
-                fi[t].opt = s->o.tput ? (s->P.m >= 1024 ? 256 : 128) : 0;
+                fi[t].opt = s->o.tput ? (s->P.m >= 1024 ? 256 : 128) | BK_POTRF_SPLIT : 0;
             }
             int rc = prec ? bk_potrf_f64(fi, g_ml.n, H, H, H, dint(), pm64)
                           : bk_potrf_f32(fi, g_ml.n, (float *)H, (float *)H, (float *)H, dint(), pm32);

@@ -335,10 +335,10 @@ extern "C" int bk_gemm_f32(int ta, int tb, float alpha, float beta, const float 
 }
 
 // ---- Cholesky: the diagonal block ------------------------------------------------------------
-// One workgroup per matrix whose last 64-column step this is (no rows below the block): factors
-// the nb x nb diagonal block at (k0, k0) in LDS, records the first failing column and the
-// smallest squared pivot, writes L_kk back (strict upper part of the block zeroed) and its inverse
-// to Dinv.
+// One workgroup per matrix whose last 64-column step this is (no rows below the block), or whose
+// steps are split (BK_POTRF_SPLIT: the panel then by a GEMM with Dinv'): factors the nb x nb
+// diagonal block at (k0, k0) in LDS, records the first failing column and the smallest squared
+// pivot, writes L_kk back (strict upper part of the block zeroed) and its inverse to Dinv.
 template <typename T>
 __global__ __launch_bounds__(512) void k_potrf_diag(const MItem *__restrict__ items, int nit, int k0, T *__restrict__ A,
                                                     T *__restrict__ Dinv, int *__restrict__ info, T *__restrict__ pmin) {
@@ -346,8 +346,8 @@ __global__ __launch_bounds__(512) void k_potrf_diag(const MItem *__restrict__ it
     if (q >= nit) return;
     const MItem it = items[q];
     const int n = it.n;
-    if (k0 >= n || k0 + 64 < n || info[it.slot] != 0) return;
-    const int nb = n - k0;
+    if (k0 >= n || (k0 + 64 < n && !(it.opt & BK_POTRF_SPLIT)) || info[it.slot] != 0) return;
+    const int nb = min(64, n - k0);
     T *Ab = A + it.off + k0 + (int64_t)k0 * n;
     T *Di = Dinv + it.off2 + (int64_t)(k0 / 64) * 64 * 64;
     T pm;
@@ -372,7 +372,7 @@ __global__ __launch_bounds__(512) void k_potrf_panel(const MItem *__restrict__ i
     const int q = flat_item(pre, nit, blockIdx.x);
     const MItem it = items[q];
     const int n = it.n, t = blockIdx.x - pre[q];
-    if (k0 + 64 >= n || info[it.slot] != 0) return;
+    if (k0 + 64 >= n || (it.opt & BK_POTRF_SPLIT) || info[it.slot] != 0) return;
     const T *Ab = A + it.off + k0 + (int64_t)k0 * n;
     const int r0 = k0 + 64 * (t + 1), prow = min(64, n - r0);
     T *Pt = A + it.off + r0 + (int64_t)k0 * n;
@@ -394,7 +394,7 @@ __global__ void k_potrf_lcopy(const MItem *__restrict__ items, const int *__rest
     const int q = flat_item(pre, nit, blockIdx.x);
     const MItem it = items[q];
     const int n = it.n, k0 = 64 * (blockIdx.x - pre[q]);
-    if (k0 + 64 >= n) return;
+    if (k0 + 64 >= n || (it.opt & BK_POTRF_SPLIT)) return;
     const T *src = W + it.off3 + (int64_t)(k0 / 64) * 64 * 64;
     T *dst = A + it.off + k0 + (int64_t)k0 * n;
     for (int e = threadIdx.x; e < 64 * 64; e += blockDim.x) dst[(e & 63) + (int64_t)(e >> 6) * n] = src[e];
@@ -502,31 +502,35 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
     const size_t lc_off = buf_align(buf);
     buf.resize(lc_off + sizeof(int) * (n_it + 1));
     int64_t lc_tot = 0;
-    for (int q = 0; q < n_it; q++) { ((int *)(buf.data() + lc_off))[q] = (int)lc_tot; lc_tot += it[q].n > 64 ? (it[q].n - 1) / 64 : 0; }
+    for (int q = 0; q < n_it; q++) {
+        ((int *)(buf.data() + lc_off))[q] = (int)lc_tot;
+        lc_tot += it[q].n > 64 && !(it[q].opt & BK_POTRF_SPLIT) ? (it[q].n - 1) / 64 : 0;
+    }
     ((int *)(buf.data() + lc_off))[n_it] = (int)lc_tot;
-    struct Step { int k0; GemmPlan upd, trail; int has_trail; size_t ppre; int64_t npanel; int last; };
+    struct Step { int k0; GemmPlan upd, panel, trail; int has_trail; size_t ppre; int64_t npanel; int diag; };
     std::vector<Step> st;
-    std::vector<GemmItem> gu, gt;
+    std::vector<GemmItem> gu, gp, gt;
     // panel width, chosen per matrix (its item's opt, else its order) so that a matrix is factored
     // the same way in any batch: 128 columns for latency (two n = 2000 blocks in FP64: 1.37 ->
     // 1.30 ms against 256), 256 where the trailing updates run at higher throughput with K = 256
     // (four n = 5000 blocks in FP32: 4.87 -> 4.60 ms; the serving stream with 256 from order
     // 1024: 365 -> 374 solves/s)
-    auto panel_width = [](const MItem &m) { return m.opt == 128 || m.opt == 256 ? m.opt : m.n < POTRF_PW256_N ? 128 : 256; };
+    auto panel_width = [](const MItem &m) { const int o = m.opt & 1023; return o == 128 || o == 256 ? o : m.n < POTRF_PW256_N ? 128 : 256; };
     for (int k0 = 0; k0 < nmax; k0 += 64) {
-        gu.clear(); gt.clear();
+        gu.clear(); gp.clear(); gt.clear();
         Step S;
         S.k0 = k0;
         // the fused diagonal-block and panel launch: one workgroup per 64-row panel tile of the
-        // matrices with rows below the block (flat prefix per item); the rest take their last step
+        // matrices with rows below the block (flat prefix per item); the one-workgroup diagonal
+        // launch for the rest (their last step, or split steps)
         S.ppre = buf_align(buf);
         buf.resize(S.ppre + sizeof(int) * (n_it + 1));
-        S.npanel = 0; S.last = 0;
+        S.npanel = 0; S.diag = 0;
         for (int q = 0; q < n_it; q++) {
-            const int n = it[q].n;
+            const int n = it[q].n, split = (it[q].opt & BK_POTRF_SPLIT) != 0;
             ((int *)(buf.data() + S.ppre))[q] = (int)S.npanel;
-            if (k0 + 64 < n) S.npanel += (n - k0 - 64 + 63) / 64;
-            else if (k0 < n) S.last = 1;
+            if (k0 + 64 < n && !split) S.npanel += (n - k0 - 64 + 63) / 64;
+            else if (k0 < n) S.diag = 1;
         }
         ((int *)(buf.data() + S.ppre))[n_it] = (int)S.npanel;
         for (int q = 0; q < n_it; q++) {
@@ -543,6 +547,18 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
                 g.guard = it[q].slot; g.tri = 0;
                 gu.push_back(g);
             }
+            if (k0 + 64 < n && (it[q].opt & BK_POTRF_SPLIT)) {
+                // split step: the panel, in place, A[k0+64:, k0:k0+64] := A[k0+64:, k0:k0+64]
+                // Dinv_k' (Dinv_k' upper triangular); the 64 x 64 tiles span the panel's full
+                // width, so every workgroup reads its rows completely before it writes them
+                g.M = n - k0 - 64; g.N = 64; g.K = 64;
+                g.lda = n; g.ldb = 64; g.ldc = n;
+                g.a = it[q].off + (k0 + 64) + (int64_t)k0 * n;
+                g.b = it[q].off2 + (int64_t)(k0 / 64) * 64 * 64;
+                g.c = g.a;
+                g.guard = it[q].slot; g.tri = 16;
+                gp.push_back(g);
+            }
             const int e = K0 + PW;
             if (sub == PW / 64 - 1 && e < n) {
                 // trailing: A[e:, e:] -= L[e:, K0:e] L[e:, K0:e]' (lower triangle)
@@ -555,6 +571,7 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
             }
         }
         S.upd = gemm_plan<T>(buf, gu.data(), (int)gu.size());
+        S.panel = gemm_plan<T>(buf, gp.data(), (int)gp.size());
         S.trail = gemm_plan<T>(buf, gt.data(), (int)gt.size());
         S.has_trail = !gt.empty();
         st.push_back(S);
@@ -575,7 +592,7 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
                                    (const int *)(d + st[u].ppre), n_it, k0, A, Dinv, W, info, pmin);
                 if (BK_LAUNCH_CHECK("k_potrf_panel")) return -1;
             }
-            if (st[u].last) {
+            if (st[u].diag) {
                 hipLaunchKernelGGL((k_potrf_diag<T>), dim3(n_it), dim3(512), 0, g_bk_stream, d_it, n_it, k0, A, Dinv, info, pmin);
                 if (BK_LAUNCH_CHECK("k_potrf_diag")) return -1;
             }
@@ -587,6 +604,7 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
                 }
                 bk_prof_end(tok, 8, fl, 0);
             }
+            if (gemm_exec<T>(d, st[u].panel, 0, 1, (T)1, (T)0, A, Dinv, A, info)) return -1;
             if (st[u].has_trail && gemm_exec<T>(d, st[u].trail, 0, 1, (T)-1, (T)1, A, A, A, info)) return -1;
         }
         if (lc_tot > 0) {
@@ -659,7 +677,10 @@ static int trtri_run(const MItem *it, int n_it, const T *L, const T *Dinv, T *Li
     const size_t lc_off = buf_align(buf);
     buf.resize(lc_off + sizeof(int) * (n_it + 1));
     int64_t lc_tot = 0;
-    for (int q = 0; q < n_it; q++) { ((int *)(buf.data() + lc_off))[q] = (int)lc_tot; lc_tot += it[q].n > 64 ? (it[q].n - 1) / 64 : 0; }
+    for (int q = 0; q < n_it; q++) {
+        ((int *)(buf.data() + lc_off))[q] = (int)lc_tot;
+        lc_tot += it[q].n > 64 && !(it[q].opt & BK_POTRF_SPLIT) ? (it[q].n - 1) / 64 : 0;
+    }
     ((int *)(buf.data() + lc_off))[n_it] = (int)lc_tot;
     std::vector<GemmPlan> p1, p2;
     std::vector<GemmItem> g1, g2;

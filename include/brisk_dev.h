@@ -80,8 +80,14 @@ typedef struct {
     int64_t off2;            /* potrf/trtri: diagonal-block inverses (64*64 per 64-wide panel) */
     int64_t off3;            /* potrf/trtri: scratch of n*n elements */
     int64_t off4;            /* trtri: the inverse; transpose: the destination */
-    int opt, pad;            /* potrf: the panel width, 128 or 256 (0: 256 from order 3685, else 128) */
+    int opt, pad;            /* potrf: the panel width, 128 or 256 (0: 256 from order 3685, else 128), plus
+                                BK_POTRF_SPLIT for the diagonal block and the panel below it in two launches */
 } MItem;
+/* potrf: a matrix whose 64-column steps factor the diagonal block in one workgroup and multiply
+ * the panel by its inverse in a GEMM (the least work, for batches that fill the GPU), instead of
+ * the one launch in which every panel tile's workgroup factors the block itself (the shortest
+ * chain, for latency) */
+#define BK_POTRF_SPLIT 1024
 
 /* Cholesky of a set of matrices, blocked right-looking (panels of 128 or 256 columns, chosen per
  * matrix by its item's opt; each factored in 64-column steps), in place on A (its

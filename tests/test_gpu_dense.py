@@ -86,8 +86,14 @@ def _pack(mats, dt):
     return buf, offs
 
 
+SPLIT = 1024      # BK_POTRF_SPLIT (include/brisk_dev.h): the diagonal block and the panel in two launches
+
+
+@pytest.mark.parametrize("split", [0, SPLIT], ids=["fused", "split"])
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
-def test_potrf_trtri_batched(dt):
+def test_potrf_trtri_batched(dt, split):
+    """The blocked Cholesky (both forms of its 64-column step: the diagonal block and its panel in
+    one launch, or in two) and the triangular inverse of a ragged batch against numpy."""
     rng = np.random.default_rng(3)
     sizes = [1, 5, 63, 64, 65, 130, 200, 300]
     mats = [_spd(n, rng, 1e2) for n in sizes]
@@ -97,7 +103,7 @@ def test_potrf_trtri_batched(dt):
     nb = [((n + 63) // 64) * 4096 for n in sizes]
     doff = np.concatenate([[0], np.cumsum(nb)[:-1]]).astype(np.int64)
     woff = np.array(offs, np.int64)
-    rows = [(n, k, offs[k], int(doff[k]), int(woff[k]), offs[k]) for k, n in enumerate(sizes)]
+    rows = [(n, k, offs[k], int(doff[k]), int(woff[k]), offs[k], split) for k, n in enumerate(sizes)]
     it = items_array(MItem, rows)
     dA = DBuf(buf); dD = DBuf(nbytes=int(sum(nb)) * np.dtype(dt).itemsize, dtype=dt)
     dW = DBuf(nbytes=buf.nbytes, dtype=dt)
@@ -127,8 +133,9 @@ def test_potrf_trtri_batched(dt):
         assert err < (1e-10 if dt == np.float64 else 5e-3), (n, err)
 
 
+@pytest.mark.parametrize("split", [0, SPLIT], ids=["fused", "split"])
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
-def test_potrf_graph_replay(dt):
+def test_potrf_graph_replay(dt, split):
     """The same factorization (plan and operands) four times: the first call runs eagerly, the
     second captures its launch sequence into a graph and the later ones replay it. Unrelated
     eager work (a GEMM) and a synchronization run between the calls; every call's factor, info
@@ -139,7 +146,7 @@ def test_potrf_graph_replay(dt):
     buf, offs = _pack(mats, dt)
     nb = [((n + 63) // 64) * 4096 for n in sizes]
     doff = np.concatenate([[0], np.cumsum(nb)[:-1]]).astype(np.int64)
-    rows = [(n, k, offs[k], int(doff[k]), offs[k], offs[k]) for k, n in enumerate(sizes)]
+    rows = [(n, k, offs[k], int(doff[k]), offs[k], offs[k], split) for k, n in enumerate(sizes)]
     it = items_array(MItem, rows)
     dA = DBuf(buf); dD = DBuf(nbytes=int(sum(nb)) * np.dtype(dt).itemsize, dtype=dt)
     dW = DBuf(nbytes=buf.nbytes, dtype=dt)
