@@ -334,23 +334,31 @@ static int blk_gemm(Solve **S, int ns, const int *act, int ta, int tb, int tri, 
  * lies in the span of I and the A_i, so it is diagonal in a block whose constraint entries all
  * are (max-cut, LP variables): there Out = Zi (Diag(V) X), a row scaling into W1 and one GEMM;
  * elsewhere W1 = Zi V and Out = W1 X, and W1 keeps Zi V (the corrector's Zi dZ dX reuses it). */
-static int zi_v_x(Solve **S, int ns, const int *act, size_t fV, size_t fX, size_t fOut) {
+/* dgo: only the diagonal of Out is wanted in the blocks where V is diagonal (their constraint
+ * entries are all diagonal, so A(Out) reads nothing else there): Out(r, r) = sum_t Zi(t, r) V(t, t)
+ * X(t, r) by one reduction per row instead of the product, W1 not written for them */
+static int zi_v_x_(Solve **S, int ns, const int *act, size_t fV, size_t fX, size_t fOut, int dgo) {
     double *H = heapp();
+    static MItL dl;
     GemmL_clear(&g_gl);
     MItL_clear(&g_ml);
+    MItL_clear(&dl);
     for (int q = 0; q < ns; q++) {
         if (act && !act[q]) continue;
         const HProb *P = &S[q]->P;
-        const int64_t v = F_(S[q], fV), x = F_(S[q], fX), w = S[q]->W1, zi = S[q]->Zi;
+        const int64_t v = F_(S[q], fV), x = F_(S[q], fX), w = S[q]->W1, zi = S[q]->Zi, out = F_(S[q], fOut);
         for (int k = 0; k < P->nb; k++) {
             const int n = P->bn[k];
             const int64_t o = P->boff[k];
-            if (P->bdiag && P->bdiag[k]) { MItem it = { n, 0, v + o, x + o, w + o, 0 }; MItL_add(&g_ml, it); }
-            else { GemmItem g = { n, n, n, n, n, n, zi + o, v + o, w + o, -1, 0 }; GemmL_add(&g_gl, g); }
+            if (P->bdiag && P->bdiag[k]) {
+                if (dgo) { MItem it = { n, 0, zi + o, v + o, x + o, out + o }; MItL_add(&dl, it); }
+                else { MItem it = { n, 0, v + o, x + o, w + o, 0 }; MItL_add(&g_ml, it); }
+            } else { GemmItem g = { n, n, n, n, n, n, zi + o, v + o, w + o, -1, 0 }; GemmL_add(&g_gl, g); }
         }
     }
     if (g_gl.n && bk_gemm_f64(0, 0, 1.0, 0.0, H, H, H, g_gl.v, g_gl.n, NULL)) return -1;
     if (g_ml.n && bk_rowscale(g_ml.v, g_ml.n)) return -1;
+    if (dl.n && bk_zvx_diag(dl.v, dl.n)) return -1;
     GemmL_clear(&g_gl);
     for (int q = 0; q < ns; q++) {
         if (act && !act[q]) continue;
@@ -360,11 +368,15 @@ static int zi_v_x(Solve **S, int ns, const int *act, size_t fV, size_t fX, size_
             const int n = P->bn[k];
             const int64_t o = P->boff[k];
             const int dg = P->bdiag && P->bdiag[k];
+            if (dg && dgo) continue;
             GemmItem g = { n, n, n, n, n, n, dg ? zi + o : w + o, dg ? w + o : x + o, out + o, -1, 0 };
             GemmL_add(&g_gl, g);
         }
     }
-    return bk_gemm_f64(0, 0, 1.0, 0.0, H, H, H, g_gl.v, g_gl.n, NULL);
+    return g_gl.n ? bk_gemm_f64(0, 0, 1.0, 0.0, H, H, H, g_gl.v, g_gl.n, NULL) : 0;
+}
+static int zi_v_x(Solve **S, int ns, const int *act, size_t fV, size_t fX, size_t fOut) {
+    return zi_v_x_(S, ns, act, fV, fX, fOut, 0);
 }
 /* the corrector's second-order term Q = Zi dZ dX of the affine directions: W1 dX with W1 = Zi dZ
  * left by the predictor's zi_v_x, and Zi (Diag(dZ) dX) in the blocks where dZ is diagonal */
@@ -1494,7 +1506,7 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
         else lin_add(S[q]->P.m, S[q]->h, -1, 0, -1, 0, -1, 0, -1, 0);
     }
     if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
-    if (zi_v_x(S, ns, hasH, OFF(dZ), OFF(X), OFF(G))) goto out;
+    if (zi_v_x_(S, ns, hasH, OFF(dZ), OFF(X), OFF(G), 1)) goto out;
     if (aop(S, ns, hasH, OFF(G), OFF(h), 1.0, 0.0)) goto out;
     /* predictor: M dy = b + h (tol 1e-7); dZ = theta R0 - A*(dy); dX = -X - sym(Zi dZ X) */
     LinL_clear(&g_ll);

@@ -734,6 +734,35 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// G(r, r) = (Zi V X)(r, r) = sum_t Zi(t, r) V(t, t) X(t, r) per block, V diagonal (Zi at off, V at
+// off2, X at off3, G at off4): one wave per row, the columns r of Zi and X read contiguously
+__global__ void k_zvx_diag(const MItem *__restrict__ it, const int *__restrict__ pre, int nit, double *__restrict__ H) {
+    FLAT_MITEM
+    const int n = m.n, lane = threadIdx.x & 63;
+    for (int64_t r = lb * 4 + (threadIdx.x >> 6); r < n; r += nb * 4) {
+        const double *zi = H + m.off + r * n, *x = H + m.off3 + r * n, *v = H + m.off2;
+        double s0 = 0, s1 = 0;
+        int t = lane;
+        for (; t + 64 < n; t += 128) {
+            s0 += zi[t] * v[t + (int64_t)t * n] * x[t];
+            s1 += zi[t + 64] * v[(t + 64) + (int64_t)(t + 64) * n] * x[t + 64];
+        }
+        if (t < n) s0 += zi[t] * v[t + (int64_t)t * n] * x[t];
+        const double sum = wave_sum(s0 + s1);
+        if (lane == 0) H[m.off4 + r + r * n] = sum;
+    }
+}
+extern "C" int bk_zvx_diag(const MItem *it, int n_it) {
+    if (n_it <= 0) return 0;
+    std::vector<int64_t> cnt(n_it);
+    for (int q = 0; q < n_it; q++) cnt[q] = flat_cap((it[q].n + 3) / 4, 1024);
+    const MItem *d = nullptr;
+    const int *pre = nullptr;
+    const int64_t nb = flat_stage_items(it, sizeof(MItem) * n_it, cnt, (const void **)&d, &pre);
+    if (nb <= 0) return (int)nb;
+    hipLaunchKernelGGL(k_zvx_diag, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, (double *)g_heap);
+    return BK_LAUNCH_CHECK("k_zvx_diag");
+}
 // ---- fused Lanczos for blocks of order <= 256: one workgroup per block, one row per thread -----
 #define LZF 256
 // number of eigenvalues below x of the m x m tridiagonal (a, b) (Sturm sequence: the pivots of

@@ -177,6 +177,9 @@ int bk_transpose_f32(const MItem *it, int n_it);
 /* matrix at off3 = Diag(matrix at off) (matrix at off2): each row r scaled by the first matrix's
  * diagonal entry (r, r); offsets in doubles from the heap base */
 int bk_rowscale(const MItem *it, int n_it);
+/* matrix at off4, its diagonal only: G(r, r) = sum_t Zi(t, r) V(t, t) X(t, r), with Zi at off, V
+ * (diagonal) at off2, X at off3, Zi symmetric; the rest of G is not written */
+int bk_zvx_diag(const MItem *it, int n_it);
 /* A += s[slot] I, with s read from the heap at offset s_off (doubles) */
 int bk_add_diag(const MItem *it, int n_it, int64_t s_off);
 /* equilibration of M (full f64 at off): D[off2 + i] = 1/sqrt(M_ii) (1 when M_ii <= 0);
