@@ -111,18 +111,24 @@ __device__ __forceinline__ void diag_inv_mfma(T (*Ls)[65], T (*Xs)[65], T (*Ts)[
 // nullptr) and L^{-1} (64 x 64, zero outside the nb x nb lower part) to Di (or nullptr). Rows and
 // columns beyond nb are factored as the identity. With PANEL, the 64-column tile at P (leading
 // dimension ldp, prow <= 64 rows) is then multiplied in place by L^{-T} on MFMA: the panel step
-// of a blocked factorization, L_I = A_I L^{-T}. Returns 0, or the 1-based column whose pivot is
+// of a blocked factorization, L_I = A_I L^{-T}. With K > 0 (nb = 64) the block and the tile are
+// first brought up to date with K earlier columns of the factor, on MFMA: the block takes
+// -Lk Lk' and the tile -Lp Lk', Lk (64 x K) and Lp (prow x K) at leading dimension n (the
+// left-looking update of a step inside a panel). Returns 0, or the 1-based column whose pivot is
 // not positive (then nothing is written); pm_out is the smallest squared pivot.
 template <typename T, bool PANEL = false>
 __device__ int diag_factor_inverse(const T *Ab, int n, int nb, T *Lo, int ldo, T *Di, T &pm_out, T *P = nullptr, int ldp = 0,
-                                   int prow = 0) {
+                                   int prow = 0, const T *Lk = nullptr, const T *Lp = nullptr, int K = 0) {
     __shared__ T Ls[64][65];       // Ls[c][r] = L(r, c)
     __shared__ T Xs[64][65];       // Xs[c][r] = X(r, c), X = L^{-1}
     __shared__ T Ts[64][65];
     __shared__ T colb[2][4][64];   // the group's four columns (double-buffered by group parity)
     __shared__ T lb[8][64][4];     // per wave: l(r, j0..j0+3) of every row r
     __shared__ T rdg[64];          // 1 / L(j, j)
+    typedef typename DiagMF<T>::v4 v4;
     const int tid = threadIdx.x, r = tid & 63, w = tid >> 6;
+    // the MFMA tiles of wave w: rows 16 (w & 3) .., columns 16 (2 (w >> 2) + h) .., h = 0, 1
+    const int l16 = r & 15, g4 = r >> 4, row0 = 16 * (w & 3);
     // thread (r, w) holds row r at the 8 columns c = 8 u + w; group g's columns 4 g + t are held
     // by the waves 4 (g & 1) + t, at u = g / 2
     T v[8];
@@ -130,6 +136,52 @@ __device__ int diag_factor_inverse(const T *Ab, int n, int nb, T *Lo, int ldo, T
     for (int u = 0; u < 8; u++) {
         const int c = 8 * u + w;
         v[u] = (r < nb && c < nb) ? (r >= c ? Ab[r + (int64_t)c * n] : (T)0) : (r == c ? (T)1 : (T)0);
+    }
+    v4 pu[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};       // the tile's update Lp Lk' (PANEL, K > 0)
+    if (K > 0) {
+        // 64 columns of Lk at a time into Ls (Ls[k][r] = Lk(r, k)) and of Lp into Xs, every load
+        // of a chunk in flight at once; then Ts[c][r] = (Lk Lk')(r, c) on MFMA with operands
+        // a = Lk(column, k), b = Lk(row, k) (Lp(row, k) for the tile): the result's row index is
+        // lane % 16
+        v4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+        for (int kc = 0; kc < K; kc += 64) {
+            T la[8], lp[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int64_t e = r + (int64_t)(kc + 8 * u + w) * n;
+                la[u] = Lk[e];
+                if (PANEL) lp[u] = r < prow ? Lp[e] : (T)0;
+            }
+            if (kc > 0) __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                Ls[8 * u + w][r] = la[u];
+                if (PANEL) Xs[8 * u + w][r] = lp[u];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int col0 = 16 * (2 * (w >> 2) + h);
+#pragma unroll
+                for (int kk = 0; kk < 64; kk += 4) {
+                    const T a = Ls[kk + g4][col0 + l16];
+                    acc[h] = DiagMF<T>::mma(a, Ls[kk + g4][row0 + l16], acc[h]);
+                    if (PANEL) pu[h] = DiagMF<T>::mma(a, Xs[kk + g4][row0 + l16], pu[h]);
+                }
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int col0 = 16 * (2 * (w >> 2) + h);
+#pragma unroll
+            for (int q = 0; q < 4; q++) Ts[col0 + DiagMF<T>::out(g4, q)][row0 + l16] = acc[h][q];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int c = 8 * u + w;
+            if (r >= c) v[u] -= Ts[c][r];
+        }
     }
     T pm = (T)1e30;
     int fail = 0;
@@ -246,16 +298,23 @@ __device__ int diag_factor_inverse(const T *Ab, int n, int nb, T *Lo, int ldo, T
     }
     pm_out = pm;
     if (PANEL) {
-        // the tile into Ts (Ts[c][r] = P(r, c), rows past prow zero), then P := P X' with wave w
-        // forming the 16 x 16 output tiles (w & 3, 2 (w >> 2)) and (w & 3, 2 (w >> 2) + 1)
-        typedef typename DiagMF<T>::v4 v4;
+        // the tile into Ts (Ts[c][r] = P(r, c), rows past prow zero), less its update, then
+        // P := P X' with wave w forming its two 16 x 16 output tiles
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             const int c = 8 * u + w;
             Ts[c][r] = r < prow ? P[r + (int64_t)c * ldp] : (T)0;
         }
         __syncthreads();
-        const int lane = tid & 63, l16 = lane & 15, g4 = lane >> 4, row0 = 16 * (w & 3);
+        if (K > 0) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int col0 = 16 * (2 * (w >> 2) + h);
+#pragma unroll
+                for (int q = 0; q < 4; q++) Ts[col0 + DiagMF<T>::out(g4, q)][row0 + l16] -= pu[h][q];
+            }
+            __syncthreads();
+        }
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int col0 = 16 * (2 * (w >> 2) + h);

@@ -400,18 +400,26 @@ static int zi_dz_dx(Solve **S, int ns, const int *act) {
     if (g_ml.n && bk_rowscale(g_ml.v, g_ml.n)) return -1;
     return bk_gemm_f64(0, 0, 1.0, 0.0, H, H, H, g_gl.v, g_gl.n, NULL);
 }
-static int blk_sym(Solve **S, int ns, const int *act, size_t f) {
+static int blk_sym_(Solve **S, int ns, const int *act, size_t f, int mirror) {
     MItL_clear(&g_ml);
     for (int q = 0; q < ns; q++) {
         if (act && !act[q]) continue;
         const HProb *P = &S[q]->P;
         for (int k = 0; k < P->nb; k++) {
             if (P->bn[k] < 2) continue;
-            MItem it = { P->bn[k], 0, F_(S[q], f) + P->boff[k], 0, 0 };
+            MItem it = { P->bn[k], 0, F_(S[q], f) + P->boff[k], 0, 0, 0, mirror, 0 };
             MItL_add(&g_ml, it);
         }
     }
     return bk_symmetrize(g_ml.v, g_ml.n);
+}
+/* G = (G + G')/2 per block */
+static int blk_sym(Solve **S, int ns, const int *act, size_t f) { return blk_sym_(S, ns, act, f, 0); }
+/* Zi = LZi' LZi per block: the product's lower triangle (K range k >= max(i, j); the tiles above
+ * the diagonal skipped), mirrored into the upper one */
+static int form_zi(Solve **S, int ns, const int *act) {
+    if (blk_gemm(S, ns, act, 1, 0, 1 | 2 | 8, 1.0, 0.0, OFF(LZi), OFF(LZi), OFF(Zi))) return -1;
+    return blk_sym_(S, ns, act, OFF(Zi), 1);
 }
 /* Cholesky of every block of the active (solve, side) pairs in one batched factorisation:
  * L = V (copied), factored in place; side s uses the fields fV[s], fL[s], fD[s] and the scratch
@@ -1394,9 +1402,8 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
     for (int q = 0; q < ns; q++) nsl += S[q]->P.nb;
     if (batch_scratch(ns, nsl + ns)) goto out;
 
-    /* Zi = LZi' LZi (K range trimmed to k >= max(i, j)), symmetrised */
-    if (blk_gemm(S, ns, act, 1, 0, 2 | 8, 1.0, 0.0, OFF(LZi), OFF(LZi), OFF(Zi))) goto out;
-    if (blk_sym(S, ns, act, OFF(Zi))) goto out;
+    /* Zi = LZi' LZi */
+    if (form_zi(S, ns, act)) goto out;
     /* residuals and objectives (solver.c:14426-14482) */
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) if (act[q]) {
@@ -1883,8 +1890,7 @@ static int ipm_iterate_hsd_(Solve **S, int ns) {
     for (int q = 0; q < ns; q++) nsl += S[q]->P.nb;
     if (batch_scratch(ns, nsl + ns)) goto out;
     /* Zi = LZi' LZi */
-    if (blk_gemm(S, ns, act, 1, 0, 2 | 8, 1.0, 0.0, OFF(LZi), OFF(LZi), OFF(Zi))) goto out;
-    if (blk_sym(S, ns, act, OFF(Zi))) goto out;
+    if (form_zi(S, ns, act)) goto out;
     /* residuals: rp = tau b - A(X), Rd = tau C - Z - A*(y) */
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) if (act[q]) {
@@ -2323,7 +2329,7 @@ int ipm_probe_schur(Solve *s, int prec) {
     if (batch_scratch(1, s->P.nb + 2)) return -1;
     if (chol_sides(S, 1, 2, NULL, F_V, F_L, F_D, F_W, ok) || !ok[0] || !ok[1]) return -1;
     if (inv_sides(S, 1, 2, a2, F_L, F_D, F_Li, F_LiT, F_W)) return -1;
-    if (blk_gemm(S, 1, NULL, 1, 0, 2 | 8, 1.0, 0.0, OFF(LZi), OFF(LZi), OFF(Zi)) || blk_sym(S, 1, NULL, OFF(Zi))) return -1;
+    if (form_zi(S, 1, NULL)) return -1;
     return schur_assemble(S, 1, NULL, 0) || bk_sync() ? -1 : 0;
 }
 /* s->rp = A(D) with D in s->W1 */

@@ -232,6 +232,7 @@ __global__ __launch_bounds__(GEMM_NT(BM)) void k_gemm(const GemmItem *__restrict
 // algorithmic flops of one item: 2 per multiply-add where both operands can be nonzero
 static double gemm_flops(const GemmItem &g) {
     const double M = g.M, N = g.N, K = g.K;
+    if ((g.tri & 1) && (g.tri & 2) && (g.tri & 8)) return K * (K + 1) * (K + 2) / 3.0;   // C lower, k >= i (sum of 2 (i + 1)(n - i))
     if (g.tri & 1) return M * (M + 1) * K;                                   // C lower only (M == N)
     if ((g.tri & 2) && (g.tri & 8)) return 2.0 * K * K * K / 3.0 + K * K;   // k >= max(i, j), M == N == K
     if (g.tri & 4) return N * M * (M + 1);                                   // op(A) lower, K == M
@@ -335,6 +336,13 @@ extern "C" int bk_gemm_f32(int ta, int tb, float alpha, float beta, const float 
 }
 
 // ---- Cholesky: the diagonal block ------------------------------------------------------------
+#define POTRF_PW256_N 3685                 // by default 256-column panels from this order (n^3 >= 5e10)
+// the panel width of a matrix: its item's opt (128 or 256), else by its order, so that a matrix
+// is factored the same way in any batch
+__host__ __device__ inline int potrf_pw(const MItem &m) {
+    const int o = m.opt & 1023;
+    return o == 128 || o == 256 ? o : m.n < POTRF_PW256_N ? 128 : 256;
+}
 // One workgroup per matrix whose last 64-column step this is (no rows below the block), or whose
 // steps are split (BK_POTRF_SPLIT: the panel then by a GEMM with Dinv'): factors the nb x nb
 // diagonal block at (k0, k0) in LDS, records the first failing column and the smallest squared
@@ -358,9 +366,11 @@ __global__ __launch_bounds__(512) void k_potrf_diag(const MItem *__restrict__ it
     }
 }
 // The diagonal block and the panel below it in one launch, for the matrices with rows below the
-// block at step k0: workgroup t of a matrix factors and inverts the 64 x 64 block itself (every
-// workgroup the same arithmetic on the same input, so all agree bit for bit) and multiplies the
-// panel tile of rows k0 + 64 (t + 1) .. by L_kk^{-T} in place. Workgroup 0 writes L_kk to the
+// block at step k0: workgroup t of a matrix brings the 64 x 64 block and the panel tile of rows
+// k0 + 64 (t + 1) .. up to date with the panel's earlier steps (columns K0 .. k0 of the factor, K0
+// the panel's first column), factors and inverts the block itself (every workgroup the same
+// arithmetic on the same input, so all agree bit for bit) and multiplies the tile by L_kk^{-T} in
+// place. Workgroup 0 writes L_kk to the
 // staging area W (64 x 64 per step at the matrix's scratch, copied into the matrix by
 // k_potrf_lcopy after the last step: the block is read by every workgroup of this launch and must
 // not change under them), Dinv, the failing column and the smallest pivot. The panel's rows are
@@ -374,12 +384,13 @@ __global__ __launch_bounds__(512) void k_potrf_panel(const MItem *__restrict__ i
     const int n = it.n, t = blockIdx.x - pre[q];
     if (k0 + 64 >= n || (it.opt & BK_POTRF_SPLIT) || info[it.slot] != 0) return;
     const T *Ab = A + it.off + k0 + (int64_t)k0 * n;
-    const int r0 = k0 + 64 * (t + 1), prow = min(64, n - r0);
+    const int r0 = k0 + 64 * (t + 1), prow = min(64, n - r0), K = k0 % potrf_pw(it), K0 = k0 - K;
     T *Pt = A + it.off + r0 + (int64_t)k0 * n;
     T *Lo = t == 0 ? W + it.off3 + (int64_t)(k0 / 64) * 64 * 64 : nullptr;
     T *Di = t == 0 ? Dinv + it.off2 + (int64_t)(k0 / 64) * 64 * 64 : nullptr;
     T pm;
-    const int f = diag_factor_inverse<T, true>(Ab, n, 64, Lo, 64, Di, pm, Pt, n, prow);
+    const int f = diag_factor_inverse<T, true>(Ab, n, 64, Lo, 64, Di, pm, Pt, n, prow, A + it.off + k0 + (int64_t)K0 * n,
+                                               A + it.off + r0 + (int64_t)K0 * n, K);
     if (t == 0 && threadIdx.x == 0) {
         if (f) info[it.slot] = k0 + f;
         else pmin[it.slot] = (k0 == 0) ? pm : min(pmin[it.slot], pm);
@@ -475,13 +486,13 @@ static int graph_capture(GraphEnt *g, const std::vector<char> &buf, F &&launch) 
     return bk_check(hipGraphLaunch(g->exec, g_bk_stream), "hipGraphLaunch");
 }
 
-#define POTRF_PW256_N 3685                 // by default 256-column panels from this order (n^3 >= 5e10)
 // Blocked right-looking Cholesky in panels of 128 or 256 columns, each factored left-looking in
 // 64-column steps: the 64 columns are first brought up to date with the panel's earlier steps
-// (a GEMM with K <= 192), then the diagonal block is factored and inverted in LDS and the rows
-// below are multiplied by its inverse; the trailing matrix takes the whole panel's update as one
-// lower-triangle GEMM with K = the panel width (a half or a quarter of the launches and of the
-// trailing traffic of 64-wide steps).
+// (K <= 192: inside the fused diagonal-block and panel launch, else a GEMM before it), then the
+// diagonal block is factored and inverted in LDS and the rows below are multiplied by its
+// inverse; the trailing matrix takes the whole panel's update as one lower-triangle GEMM with
+// K = the panel width (a half or a quarter of the launches and of the trailing traffic of
+// 64-wide steps).
 template <typename T>
 static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, T *pmin) {
     if (n_it <= 0) return 0;
@@ -510,12 +521,10 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
     struct Step { int k0; GemmPlan upd, panel, trail; int has_trail; size_t ppre; int64_t npanel; int diag; };
     std::vector<Step> st;
     std::vector<GemmItem> gu, gp, gt;
-    // panel width, chosen per matrix (its item's opt, else its order) so that a matrix is factored
-    // the same way in any batch: 128 columns for latency (two n = 2000 blocks in FP64: 1.37 ->
-    // 1.30 ms against 256), 256 where the trailing updates run at higher throughput with K = 256
-    // (four n = 5000 blocks in FP32: 4.87 -> 4.60 ms; the serving stream with 256 from order
-    // 1024: 365 -> 374 solves/s)
-    auto panel_width = [](const MItem &m) { const int o = m.opt & 1023; return o == 128 || o == 256 ? o : m.n < POTRF_PW256_N ? 128 : 256; };
+    // panel width (potrf_pw): 128 columns for latency (two n = 2000 blocks in FP64: 1.37 -> 1.30
+    // ms against 256), 256 where the trailing updates run at higher throughput with K = 256 (four
+    // n = 5000 blocks in FP32: 4.87 -> 4.60 ms; the serving stream with 256 from order 1024: 365
+    // -> 374 solves/s)
     for (int k0 = 0; k0 < nmax; k0 += 64) {
         gu.clear(); gp.clear(); gt.clear();
         Step S;
@@ -536,9 +545,10 @@ static int potrf_run(const MItem *it, int n_it, T *A, T *Dinv, T *W, int *info, 
         for (int q = 0; q < n_it; q++) {
             const int n = it[q].n;
             if (k0 >= n) continue;
-            const int PW = panel_width(it[q]), K0 = k0 - k0 % PW, sub = (k0 - K0) / 64;
+            const int PW = potrf_pw(it[q]), K0 = k0 - k0 % PW, sub = (k0 - K0) / 64;
+            const int fused = k0 + 64 < n && !(it[q].opt & BK_POTRF_SPLIT);
             GemmItem g;
-            if (sub > 0) {
+            if (sub > 0 && !fused) {
                 // columns k0 .. k0+64 (rows k0 ..) -= L[k0.., K0..k0] L[k0..k0+64, K0..k0]'
                 g.M = n - k0; g.N = min(64, n - k0); g.K = k0 - K0;
                 g.lda = n; g.ldb = n; g.ldc = n;

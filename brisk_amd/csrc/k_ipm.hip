@@ -237,7 +237,8 @@ extern "C" int bk_colscale_d2f(int64_t rows, int cols, int64_t off, int64_t dst_
     const MItem m = it[q];                                                  \
     const int64_t lb = blockIdx.x - pre[q], nb = pre[q + 1] - pre[q];
 // G = (G + G')/2 over pairs of 32 x 32 tiles (I, J), I >= J, both staged in LDS so that every
-// read and write is coalesced; each entry pair gets 0.5 (g_rc + g_cr)
+// read and write is coalesced; each entry pair gets 0.5 (g_rc + g_cr). With opt = 1 the strict
+// upper triangle takes the lower one instead (G(c, r) = G(r, c), r > c; the lower tile only read)
 __global__ void k_symmetrize(const MItem *__restrict__ it, const int *__restrict__ pre, int nit, double *__restrict__ H) {
     FLAT_MITEM
     const int n = m.n, tiles = (n + 31) / 32;
@@ -252,9 +253,15 @@ __global__ void k_symmetrize(const MItem *__restrict__ it, const int *__restrict
         const int r0 = 32 * I, c0 = 32 * J;
         for (int y = ly; y < 32; y += 8) {
             if (r0 + lx < n && c0 + y < n) ta[y][lx] = G[(r0 + lx) + (int64_t)(c0 + y) * n];    // G(r0 + lx, c0 + y)
-            if (c0 + lx < n && r0 + y < n) tb[y][lx] = G[(c0 + lx) + (int64_t)(r0 + y) * n];    // G(c0 + lx, r0 + y)
+            if (!m.opt && c0 + lx < n && r0 + y < n) tb[y][lx] = G[(c0 + lx) + (int64_t)(r0 + y) * n];    // G(c0 + lx, r0 + y)
         }
         __syncthreads();
+        if (m.opt) {
+            for (int y = ly; y < 32; y += 8)
+                if (c0 + lx < n && r0 + y < n && (I > J || y > lx)) G[(c0 + lx) + (int64_t)(r0 + y) * n] = ta[lx][y];
+            __syncthreads();
+            continue;
+        }
         for (int y = ly; y < 32; y += 8) {
             // G(r0 + lx, c0 + y) and its mirror G(c0 + y, r0 + lx) = tb[lx][y]
             if (r0 + lx < n && c0 + y < n && (I > J || lx > y))

@@ -171,7 +171,7 @@ int bk_lincomb(const LinItem *it, int n_it);
 int bk_vmul(const LinItem *it, int n_it);
 
 /* per square matrix at heap offset off (n = order): */
-int bk_symmetrize(const MItem *it, int n_it);                       /* G = (G + G')/2 */
+int bk_symmetrize(const MItem *it, int n_it);                       /* G = (G + G')/2; opt 1: upper := lower' */
 int bk_transpose_f64(const MItem *it, int n_it);                    /* matrix at off4 = (matrix at off)' */
 int bk_transpose_f32(const MItem *it, int n_it);
 /* matrix at off3 = Diag(matrix at off) (matrix at off2): each row r scaled by the first matrix's

@@ -63,6 +63,15 @@ def test_gemm_triangular_flags(dt, n):
     ref = Li.T @ Li
     tol = 1e-12 if dt == np.float64 else 1e-4
     assert np.allclose(got, ref, rtol=tol, atol=tol * np.abs(ref).max())
+    # the same product's lower triangle only (1|2|8): the upper triangle left untouched
+    C0 = rng.standard_normal((n, n))
+    dC1 = DBuf(C0.flatten(order="F").astype(dt))
+    it = items_array(GemmItem, [(n, n, n, n, n, n, 0, 0, 0, -1, 1 | 2 | 8)])
+    assert f(1, 0, 1.0, 0.0, dA.p, dA.p, dC1.p, it, 1, None) == 0
+    got = dC1.get().reshape((n, n), order="F")
+    low = np.tril(np.ones((n, n), bool))
+    assert np.allclose(got[low], ref[low], rtol=tol, atol=tol * np.abs(ref).max())
+    assert np.array_equal(got[~low], C0.astype(dt)[~low])
     # lower-only: C = -W W' + C, upper triangle left untouched
     W = rng.standard_normal((n, 64))
     C0 = rng.standard_normal((n, n))
@@ -89,11 +98,14 @@ def _pack(mats, dt):
 SPLIT = 1024      # BK_POTRF_SPLIT (include/brisk_dev.h): the diagonal block and the panel in two launches
 
 
+@pytest.mark.parametrize("pw", [128, 256], ids=["pw128", "pw256"])
 @pytest.mark.parametrize("split", [0, SPLIT], ids=["fused", "split"])
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
-def test_potrf_trtri_batched(dt, split):
+def test_potrf_trtri_batched(dt, split, pw):
     """The blocked Cholesky (both forms of its 64-column step: the diagonal block and its panel in
-    one launch, or in two) and the triangular inverse of a ragged batch against numpy."""
+    one launch, with the step's update from the panel's earlier columns, K = 64 .. 192, inside it,
+    or in two launches after a GEMM) at both panel widths, and the triangular inverse of a ragged
+    batch against numpy."""
     rng = np.random.default_rng(3)
     sizes = [1, 5, 63, 64, 65, 130, 200, 300]
     mats = [_spd(n, rng, 1e2) for n in sizes]
@@ -103,7 +115,7 @@ def test_potrf_trtri_batched(dt, split):
     nb = [((n + 63) // 64) * 4096 for n in sizes]
     doff = np.concatenate([[0], np.cumsum(nb)[:-1]]).astype(np.int64)
     woff = np.array(offs, np.int64)
-    rows = [(n, k, offs[k], int(doff[k]), int(woff[k]), offs[k], split) for k, n in enumerate(sizes)]
+    rows = [(n, k, offs[k], int(doff[k]), int(woff[k]), offs[k], split | pw) for k, n in enumerate(sizes)]
     it = items_array(MItem, rows)
     dA = DBuf(buf); dD = DBuf(nbytes=int(sum(nb)) * np.dtype(dt).itemsize, dtype=dt)
     dW = DBuf(nbytes=buf.nbytes, dtype=dt)
