@@ -138,7 +138,7 @@ int64_t solve_heap_bytes(const HProb *P) {
     b += (m + 1) * 4 + (int64_t)P->nent * (sizeof(SEnt) + 12) + (int64_t)P->nb * sizeof(BlkRef) + (P->npos + 1) * 12 + 2048;
     b += (m + 1) * 4 + (int64_t)(P->ngrp + 1) * sizeof(GrpRef) + 2 * (P->gp_tot * 8 + 256) + 512;
     b += P->npair * 8 + 256;
-    if (P->want_hsd) b += 6 * (T * 8 + 256) + 5 * (m * 8 + 256);
+    if (P->want_hsd) b += 9 * (T * 8 + 256) + 6 * (m * 8 + 256) + 4 * (BK_LZ_K + 3) * lz * 8;   /* (and four more Lanczos sides) */
     if (P->dense) {
         const int64_t nn = T;
         b += nn * m * (8 + 4 + 8 + 8) + 4 * 256;      /* Ad, Af, P, B */
@@ -167,7 +167,9 @@ int solve_setup(Solve *S) {
     ALLOC(dX, T * 8); ALLOC(dZ, T * 8); ALLOC(G, T * 8); ALLOC(Q, T * 8); ALLOC(R0, T * 8); ALLOC(F, T * 8); ALLOC(W1, T * 8);
     ALLOC(W2, T * 8); ALLOC(C, T * 8); ALLOC(Xb, T * 8);
     ALLOC(DinvX, dinv * 8); ALLOC(DinvZ, dinv * 8);
-    ALLOC(lzQ, 2 * (int64_t)BK_LZ_K * lz * 8); ALLOC(lzw, 2 * lz * 8); ALLOC(lzt1, 2 * lz * 8); ALLOC(lzt2, 2 * lz * 8);
+    /* Lanczos bases for two sides (primal, dual), six for the embedding's sigma search */
+    const int64_t nsd = S->o.hsd ? 6 : 2;
+    ALLOC(lzQ, nsd * (int64_t)BK_LZ_K * lz * 8); ALLOC(lzw, nsd * lz * 8); ALLOC(lzt1, nsd * lz * 8); ALLOC(lzt2, nsd * lz * 8);
     ALLOC(M, m * m * 8); ALLOC(MF, m * m * 8); ALLOC(MLi, m * m * 8); ALLOC(MLiT, m * m * 8); ALLOC(Mwork, mwork_elems(m) * 8);
     ALLOC(MDinv, ((m + 63) / 64) * 4096 * 8); ALLOC(Dm, m * 8);
     ALLOC(y, m * 8); ALLOC(dy, m * 8); ALLOC(rp, m * 8); ALLOC(rhs, m * 8); ALLOC(a0, m * 8); ALLOC(h, m * 8); ALLOC(qv, m * 8);
@@ -183,6 +185,7 @@ int solve_setup(Solve *S) {
     if (S->hsd) {
         ALLOC(Rd, T * 8); ALLOC(CX, T * 8); ALLOC(dX0, T * 8); ALLOC(dZ0, T * 8); ALLOC(dX1, T * 8); ALLOC(dZ1, T * 8);
         ALLOC(hv, m * 8); ALLOC(hg, m * 8); ALLOC(ht, m * 8); ALLOC(hy0, m * 8); ALLOC(hy1, m * 8);
+        ALLOC(Qp, T * 8); ALLOC(dX0s, T * 8); ALLOC(dZ0s, T * 8); ALLOC(hy0s, m * 8);
     }
     S->tau = 1.0; S->kap = 0.0;
     if (P->dense) {
@@ -1877,14 +1880,230 @@ int ipm_finish(Solve *s) {
  * the predicted mu reduction alpha (1 - sigma) (solver.c:11195-11228); one step length for all
  * variables; dkappa = (sigma mu - tau kappa - kappa dtau) / tau (solver.c:11127); tau normalised
  * to 1 when it passes 10 (solver.c:12054-12069). The solution is (X, y) / tau. */
+/* ---- the self-dual embedding's direction family ------------------------------------------- */
+/* per-solve scalars of an embedding iteration (HSD_NV doubles each) */
+#define HSD_NV 64
+#define HV(q, k) V[HSD_NV * (size_t)(q) + (k)]
+enum { MU, XZ, CXv, BY, RG, C1, C2, CZ, BV, GV, DEN, SIGM, DTA, DKA, DT0, DK0, CK, CTK, BESTS, BESTA, BESTM, BESTAM,
+       DDT, DDK, DTC, DKC, TK2P, QDP, M0, HSIG, DT0S, DK0S, S0S, A0S, AM0S, DTF, DKF, T0, T1, T2, T3, T4 };
+#define HSD_PASSES 4        /* second-order passes an iteration at most (reference Params.hsd_hoc, solver.c:179) */
+/* b'u and A(X C Zi)'u into scalars 0 and 1 of the active solves */
+static int hsd_tau_dots(Solve **S, int ns, const int *a, size_t fu, int unused) {
+    (void)unused;
+    DotL_clear(&g_dl);
+    for (int q = 0; q < ns; q++) if (a[q]) {
+        Solve *s = S[q];
+        dot_add(s->P.m, s->bvec, F_(s, fu), SC(q, 0));
+        dot_add(s->P.m, s->hg, F_(s, fu), SC(q, 1));
+    }
+    return (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) ? -1 : 0;
+}
+/* a member of the family from its u (at fy) with the scalars T0..T4 of each solve:
+ *   dy = u + T0 v,  dZ = T1 Rd + T0 C - A*(dy),  dX = -sym(Zi dZ X) + T2 X + T3 Zi + T4 Q
+ * (T0 the member's dtau; Q only with withq) */
+static int hsd_form(Solve **S, int ns, const int *a, const double *V, size_t fy, size_t fdZ, size_t fdX, int withq) {
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (a[q]) {
+        Solve *s = S[q];
+        lin_add(s->P.m, F_(s, fy), F_(s, fy), 1.0, s->hv, HV(q, T0), -1, 0, -1, 0);
+        lin_add(s->P.T, F_(s, fdZ), s->Rd, HV(q, T1), s->C, HV(q, T0), -1, 0, -1, 0);
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n) || aty(S, ns, a, fy, fdZ, -1.0)) return -1;
+    /* (full products: dZ holds tau C and Rd, which are not diagonal where the constraints are) */
+    if (blk_gemm(S, ns, a, 0, 0, 0, 1.0, 0.0, OFF(Zi), fdZ, OFF(W1)) || blk_gemm(S, ns, a, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G)) ||
+        blk_sym(S, ns, a, OFF(G))) return -1;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (a[q]) {
+        Solve *s = S[q];
+        lin_add(s->P.T, F_(s, fdX), s->G, -1.0, s->X, HV(q, T2), HV(q, T3) != 0 ? s->Zi : -1, HV(q, T3), withq ? s->Q : -1, HV(q, T4));
+    }
+    return bk_lincomb(g_ll.v, g_ll.n);
+}
+/* the sigma search on the family (reference HSD_SIGSEARCH, solver.c:11199-11231): the members at
+ * sigma_m, 0.2 sigma_m and 3 sigma_m formed in (dX, dZ), (G, F), (W1, W2), the six step bounds in
+ * one Lanczos launch; the candidate with the largest alpha (1 - sigma) goes to BESTS, BESTA
+ * (the damped step), BESTAM (the step to the boundary), BESTM */
+static int hsd_search(Solve **S, int ns, const int *a, double *V, double *apx, double *apz, double *sgc, double *lo6) {
+    static const size_t fX[3] = { OFF(dX), OFF(G), OFF(W1) }, fZ[3] = { OFF(dZ), OFF(F), OFF(W2) };
+    (void)apx; (void)apz;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (a[q]) {
+        Solve *s = S[q];
+        const double sm = HV(q, SIGM);
+        sgc[3 * q] = sm; sgc[3 * q + 1] = fmin(0.95, fmax(1e-4, 0.2 * sm)); sgc[3 * q + 2] = fmin(0.95, fmax(1e-3, 3.0 * sm));
+        for (int c = 0; c < 3; c++) {
+            lin_add(s->P.T, F_(s, fX[c]), s->dX0, 1.0, s->dX1, sgc[3 * q + c], -1, 0, -1, 0);
+            lin_add(s->P.T, F_(s, fZ[c]), s->dZ0, 1.0, s->dZ1, sgc[3 * q + c], -1, 0, -1, 0);
+        }
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) return -1;
+    const size_t fLi[6] = { OFF(LXi), OFF(LZi), OFF(LXi), OFF(LZi), OFF(LXi), OFF(LZi) };
+    const size_t fLiT[6] = { OFF(LXiT), OFF(LZiT), OFF(LXiT), OFF(LZiT), OFF(LXiT), OFF(LZiT) };
+    const size_t fV[6] = { fX[0], fZ[0], fX[1], fZ[1], fX[2], fZ[2] };
+    if (lanczos_min(S, ns, a, 6, fLi, fLiT, fV, 1, 0.02, lo6)) return -1;
+    for (int q = 0; q < ns; q++) if (a[q]) {
+        Solve *s = S[q];
+        HV(q, BESTM) = -1; HV(q, BESTS) = HV(q, SIGM); HV(q, BESTA) = 0; HV(q, BESTAM) = 0;
+        for (int c = 0; c < 3; c++) {
+            const double sg = sgc[3 * q + c];
+            const double dtau = HV(q, DT0) + sg * HV(q, DDT), dkap = HV(q, DK0) + sg * HV(q, DDK);
+            double am = fmin(lo6[(2 * c) * ns + q], lo6[(2 * c + 1) * ns + q]);
+            if (dtau < 0) am = fmin(am, -s->tau / dtau);
+            if (dkap < 0) am = fmin(am, -s->kap / dkap);
+            const double gam = 0.9 + (0.99 - 0.9) * fmin(1.0, am);
+            const double al = fmin(1.0, gam * am), merit = al * (1.0 - sg);
+            if (merit > HV(q, BESTM)) { HV(q, BESTM) = merit; HV(q, BESTS) = sg; HV(q, BESTA) = al; HV(q, BESTAM) = am; }
+        }
+    }
+    return 0;
+}
+/* lambda_min((X + al dX)(Z + al dZ)) >= thr per solve (al, thr in ald[q], thr[q]): L = chol(X +
+ * al dX) in W1, then L'(Z + al dZ) L - thr I must factor (in G); the reference's sdp_nb_ok
+ * (solver.c:6470-6504). LP variables are blocks of order 1, where this is the product test.
+ * DinvX, DinvZ, F and G serve as scratch (the step's factorizations rebuild the first two). */
+static int hsd_nb_ok(Solve **S, int ns, const int *a, const double *ald, const double *thr, int *ok) {
+    int *ok1 = calloc(ns, sizeof(int)), *a2 = calloc(ns, sizeof(int)), rc = -1, any = 0;
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) if (a[q]) {
+        Solve *s = S[q];
+        lin_add(s->P.T, s->W1, s->X, 1.0, s->dX, ald[q], -1, 0, -1, 0);
+        lin_add(s->P.T, s->W2, s->Z, 1.0, s->dZ, ald[q], -1, 0, -1, 0);
+    }
+    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    {
+        const size_t fa[1] = { OFF(W1) }, fd[1] = { OFF(DinvX) }, fw[1] = { OFF(G) };
+        if (chol_sides(S, ns, 1, a, fa, fa, fd, fw, ok1)) goto out;
+    }
+    for (int q = 0; q < ns; q++) { ok[q] = 0; if (a[q] && ok1[q]) { a2[q] = 1; any = 1; } }
+    if (!any) { rc = 0; goto out; }
+    if (blk_gemm(S, ns, a2, 0, 0, 0, 1.0, 0.0, OFF(W2), OFF(W1), OFF(F)) || blk_gemm(S, ns, a2, 1, 0, 0, 1.0, 0.0, OFF(W1), OFF(F), OFF(G)) ||
+        blk_sym(S, ns, a2, OFF(G))) goto out;
+    MItL_clear(&g_ml);
+    for (int q = 0; q < ns; q++) {
+        g_hdbl[q] = a2[q] ? -thr[q] : 0.0;
+        if (!a2[q]) continue;
+        for (int k = 0; k < S[q]->P.nb; k++) { MItem it = { S[q]->P.bn[k], q, S[q]->G + S[q]->P.boff[k], 0, 0, 0 }; MItL_add(&g_ml, it); }
+    }
+    if (bk_h2d(ddbl(), g_hdbl, sizeof(double) * ns) || bk_add_diag(g_ml.v, g_ml.n, g_bdbl / 8)) goto out;
+    {
+        const size_t fa[1] = { OFF(G) }, fl[1] = { OFF(G) }, fd[1] = { OFF(DinvZ) }, fw[1] = { OFF(F) };
+        if (chol_sides(S, ns, 1, a2, fa, fl, fd, fw, ok1)) goto out;
+    }
+    for (int q = 0; q < ns; q++) if (a2[q]) ok[q] = ok1[q];
+    rc = 0;
+out:
+    free(ok1); free(a2);
+    return rc;
+}
+/* the long step after passes that stood (reference solver.c:11818-11855): where the step to the
+ * boundary is at least 0.9, 0.992 of it is tested first; inside the neighbourhood lambda_min(XZ)
+ * >= 0.3 mu_new (and tau kappa >= 0.3 mu_new), 0.9995 and then 0.998 are tried, the first one
+ * inside taken (0.992 if neither is) */
+static int hsd_long_step(Solve **S, int ns, const int *act, double *V, const int *kept, double *alpha) {
+    static const double gs[3] = { 0.9995, 0.998, 0.992 };
+    int *stage = calloc(ns, sizeof(int)), *t = calloc(ns, sizeof(int)), *ok = calloc(ns, sizeof(int)), rc = -1;
+    double *al = calloc(ns, sizeof(double)), *thr = calloc(ns, sizeof(double)), *sf = calloc(ns, sizeof(double));
+    /* stage: 0 not taking part, 1 the short test, 2 and 3 the tests at 0.9995 and 0.998 */
+    for (int q = 0; q < ns; q++) stage[q] = act[q] && kept[q] > 0 && HV(q, BESTAM) >= 0.9 ? 1 : 0;
+    for (int round = 0; round < 3; round++) {
+        int any = 0;
+        for (int q = 0; q < ns; q++) {
+            t[q] = 0;
+            if (!stage[q]) continue;
+            Solve *s = S[q];
+            const double am = HV(q, BESTAM), sg = HV(q, BESTS), dt = HV(q, DTF), dk = HV(q, DKF), mu = HV(q, MU);
+            for (;;) {
+                const double a = fmin(1.0, (stage[q] == 1 ? gs[2] : gs[stage[q] - 2]) * am);
+                if (stage[q] == 1 && !(a > alpha[q] && s->tau + a * dt > 0 && s->kap + a * dk > 0)) { stage[q] = 2; continue; }
+                if (stage[q] == 4) { stage[q] = 0; break; }
+                if (stage[q] >= 2 && !(a > alpha[q])) { stage[q] = 0; break; }
+                if (stage[q] >= 2 && (s->tau + a * dt <= 0 || s->kap + a * dk <= 0)) { stage[q]++; continue; }
+                const double mun = mu * (1.0 - a * (1.0 - sg));
+                if (!((s->tau + a * dt) * (s->kap + a * dk) >= 0.3 * mun)) {
+                    if (stage[q] == 1) { stage[q] = 0; break; }
+                    stage[q]++; continue;
+                }
+                al[q] = a; thr[q] = 0.3 * mun; t[q] = 1; any = 1;
+                break;
+            }
+        }
+        if (!any) break;
+        if (hsd_nb_ok(S, ns, t, al, thr, ok)) goto out;
+        for (int q = 0; q < ns; q++) {
+            if (!t[q]) continue;
+            if (stage[q] == 1) {
+                /* the short step inside: the longer ones are tried (0.992 stands if neither is inside) */
+                if (ok[q]) { stage[q] = 2; sf[q] = al[q]; } else stage[q] = 0;
+            } else if (ok[q]) { alpha[q] = al[q]; stage[q] = 0; }
+            else if (stage[q] == 3) { if (sf[q] > alpha[q]) alpha[q] = sf[q]; stage[q] = 0; }
+            else stage[q]++;
+        }
+    }
+    /* a short test that stood with no longer step inside (or none left to try) */
+    for (int q = 0; q < ns; q++) if (stage[q] >= 2 && sf[q] > alpha[q]) alpha[q] = sf[q];
+    rc = 0;
+out:
+    free(stage); free(t); free(ok); free(al); free(thr); free(sf);
+    return rc;
+}
+/* the neighbourhood safeguard (reference solver.c:11856-11882): tau kappa and the products of the
+ * LP variables at least 1e-3 of the new mu, else the step x0.7 (at most ten times) */
+static int hsd_safeguard(Solve **S, int ns, const int *act, double *V, double *alpha) {
+    DotL_clear(&g_dl);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        Solve *s = S[q];
+        dot_add(s->P.T, s->X, s->dZ, SC(q, 0));
+        dot_add(s->P.T, s->dX, s->Z, SC(q, 1));
+        dot_add(s->P.T, s->dX, s->dZ, SC(q, 2));
+    }
+    if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) return -1;
+    double *H = heapp();
+    for (int q = 0; q < ns; q++) {
+        if (!act[q]) continue;
+        Solve *s = S[q];
+        const HProb *P = &s->P;
+        /* the LP variables (blocks of order 1) in runs of consecutive blocks */
+        int nlp = 0;
+        for (int k = 0; k < P->nb; k++) nlp += P->bn[k] == 1;
+        double *lp = NULL;
+        if (nlp) {
+            lp = malloc(sizeof(double) * 4 * nlp);
+            int c = 0;
+            for (int k = 0; k < P->nb;) {
+                if (P->bn[k] != 1) { k++; continue; }
+                int e = k;
+                while (e < P->nb && P->bn[e] == 1) e++;
+                const int64_t o = P->boff[k], len = e - k;
+                if (bk_d2h(lp + c, H + s->X + o, 8 * len) || bk_d2h(lp + nlp + c, H + s->dX + o, 8 * len) ||
+                    bk_d2h(lp + 2 * nlp + c, H + s->Z + o, 8 * len) || bk_d2h(lp + 3 * nlp + c, H + s->dZ + o, 8 * len)) { free(lp); return -1; }
+                c += (int)len; k = e;
+            }
+        }
+        const double dt = HV(q, DTF), dk = HV(q, DKF);
+        double a = alpha[q];
+        for (int t2 = 0; t2 < 10; t2++) {
+            const double tk = (s->tau + a * dt) * (s->kap + a * dk);
+            const double munew = (HV(q, XZ) + a * (HS(q, 0) + HS(q, 1)) + a * a * HS(q, 2) + tk) / (P->Ntot + 1.0);
+            double lo = tk;
+            for (int i = 0; i < nlp; i++) {
+                const double w = (lp[i] + a * lp[nlp + i]) * (lp[2 * nlp + i] + a * lp[3 * nlp + i]);
+                if (w < lo) lo = w;
+            }
+            if (!(munew > 0) || lo >= 1e-3 * munew || a < 1e-3) break;
+            a *= 0.7;
+        }
+        alpha[q] = a;
+        free(lp);
+    }
+    return 0;
+}
+
 static int ipm_iterate_hsd_(Solve **S, int ns) {
     int rc = -1;
     int *act = calloc(ns, sizeof(int)), *fail = calloc(ns, sizeof(int)), *okp = calloc(ns, sizeof(int)), *okd = calloc(ns, sizeof(int));
     int *nrm = calloc(ns, sizeof(int));
-    double *V = calloc(32 * (size_t)ns, sizeof(double));       /* per-solve scalars, 32 each */
+    double *V = calloc(HSD_NV * (size_t)ns, sizeof(double));  /* per-solve scalars */
     double *rel = calloc(ns, sizeof(double)), *apx = calloc(ns, sizeof(double)), *apz = calloc(ns, sizeof(double));
-#define HV(q, k) V[32 * (size_t)(q) + (k)]
-    enum { MU, XZ, CXv, BY, RG, C1, C2, CZ, BV, GV, DEN, SIGM, DTA, DKA, DT0, DK0, DT1, DK1, CK, BESTS, BESTA, BESTM, CTK };
     for (int q = 0; q < ns; q++) act[q] = !S[q]->done;
     int nsl = 1;
     for (int q = 0; q < ns; q++) nsl += S[q]->P.nb;
@@ -2005,66 +2224,42 @@ static int ipm_iterate_hsd_(Solve **S, int ns) {
         HV(q, C2) = HS(q, 0); HV(q, C1) = HS(q, 1); HV(q, CZ) = HS(q, 2); HV(q, BV) = HS(q, 3); HV(q, GV) = HS(q, 4);
         HV(q, DEN) = HV(q, BV) + HV(q, C2) - HV(q, GV) + S[q]->kap / S[q]->tau;
     }
-    /* three directions: 0 affine (sigma 0, no K) into dy/dX/dZ; 1 sigma = 0 with K into hy0/dX0/dZ0;
-     * 2 sigma = 1 with K into hy1/dX1/dZ1 */
-    for (int pass = 0; pass < 3; pass++) {
-        const size_t fy = pass == 0 ? OFF(dy) : pass == 1 ? OFF(hy0) : OFF(hy1);
-        const size_t fdX = pass == 0 ? OFF(dX) : pass == 1 ? OFF(dX0) : OFF(dX1);
-        const size_t fdZ = pass == 0 ? OFF(dZ) : pass == 1 ? OFF(dZ0) : OFF(dZ1);
-        /* w1 = eta rp - smu a0 + (tau b - rp) + eta A(Zi Rd X) [+ A(Q)] */
+    /* The direction family of the embedding (reference solver.c:11240-11480). With w1(sigma) =
+     * [tau b + A(X Rd Zi) + A(Q)] + sigma [-rp - mu A(Zi) - A(X Rd Zi)], M u = w1 has the solution
+     * u0 + sigma u1 with u1 independent of the second-order term Q, and every part of the
+     * direction is affine in sigma: the family is kept as its base (sigma 0: hy0, dX0, dZ0) and
+     * its difference (hy1, dX1, dZ1), which is formed once an iteration.
+     * 1. The affine direction (sigma 0, no Q) into dy, dX, dZ: its step and the Mehrotra sigma_m.
+     * 2. The difference.
+     * 3. Second-order passes: Q = sym(Zi dZ dX) from the current direction (the affine one first,
+     *    then the family's member at the sigma the first search chose), u0 solved again, the base
+     *    formed and sigma searched among sigma_m, 0.2 sigma_m and 3 sigma_m by the largest
+     *    predicted reduction alpha (1 - sigma) of mu (the three candidates' step bounds in one
+     *    Lanczos launch). A pass stands while Q contracts (|Q_k - Q_k-1| <= 0.9 |Q_k-1 - Q_k-2|)
+     *    and its predicted reduction is at least the first family's, else the first family
+     *    stands; at most four passes, none within a factor 100 of the tolerance and none for the
+     *    rest of the solve after three iterations whose passes fell behind the first family
+     *    (solver.c:11236-11247). */
+    {
+        double *sgc = calloc(3 * (size_t)ns, sizeof(double)), *lo6 = calloc(6 * (size_t)ns, sizeof(double));
+        int *hl = calloc(ns, sizeof(int)), *nhoc = calloc(ns, sizeof(int)), *kept = calloc(ns, sizeof(int));
+        int e = 0;
+        /* 1. affine: u = M^-1 (tau b + A(X Rd Zi)) */
         LinL_clear(&g_ll);
-        for (int q = 0; q < ns; q++) if (act[q]) {
+        for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->rhs, S[q]->bvec, S[q]->tau, S[q]->ht, 1.0, -1, 0, -1, 0);
+        if (bk_lincomb(g_ll.v, g_ll.n) || schur_solve(S, ns, act, OFF(rhs), OFF(dy), 1e-7, rel)) e = 1;
+        if (!e) e = hsd_tau_dots(S, ns, act, OFF(dy), -1);
+        for (int q = 0; q < ns && !e; q++) if (act[q]) {
             Solve *s = S[q];
-            if (pass == 0) lin_add(s->P.m, s->rhs, s->bvec, s->tau, s->ht, 1.0, -1, 0, -1, 0);
-            else if (pass == 1) lin_add(s->P.m, s->rhs, s->bvec, s->tau, s->ht, 1.0, s->qv, 1.0, -1, 0);
-            else lin_add(s->P.m, s->rhs, s->bvec, s->tau, s->rp, -1.0, s->a0, -HV(q, MU), s->qv, 1.0);
+            const double tau = s->tau, kap = s->kap;
+            const double num = HV(q, RG) - HS(q, 0) - HV(q, CXv) - HV(q, C1) + HS(q, 1) - kap;
+            HV(q, DTA) = num / HV(q, DEN);
+            HV(q, DKA) = (-tau * kap - kap * HV(q, DTA)) / tau;
+            HV(q, T0) = HV(q, DTA); HV(q, T1) = 1.0; HV(q, T2) = -1.0; HV(q, T3) = 0.0; HV(q, T4) = 0.0;
         }
-        if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
-        if (schur_solve(S, ns, act, OFF(rhs), fy, pass == 0 ? 1e-7 : 0.0, rel)) goto out;
-        DotL_clear(&g_dl);
-        for (int q = 0; q < ns; q++) if (act[q]) {
-            Solve *s = S[q];
-            dot_add(s->P.m, s->bvec, F_(s, fy), SC(q, 0));
-            dot_add(s->P.m, s->hg, F_(s, fy), SC(q, 1));
-            if (pass == 1) dot_add(s->P.T, s->C, s->Q, SC(q, 2));
-        }
-        if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
-        LinL_clear(&g_ll);
-        for (int q = 0; q < ns; q++) {
-            if (!act[q]) continue;
-            Solve *s = S[q];
-            const double tau = s->tau, kap = s->kap, mu = HV(q, MU);
-            const double sg = pass == 2 ? 1.0 : 0.0, eta = 1.0 - sg, smu = sg * mu;
-            if (pass == 1) { HV(q, CK) = -HS(q, 2); HV(q, CTK) = -HV(q, DTA) * HV(q, DKA); }
-            const double cK = pass ? HV(q, CK) : 0.0, ctk = pass ? HV(q, CTK) : 0.0;
-            const double c0 = smu * HV(q, CZ) - HV(q, CXv);
-            const double num = eta * HV(q, RG) - HS(q, 0) + c0 - eta * HV(q, C1) + HS(q, 1) + cK + (smu - tau * kap + ctk) / tau;
-            const double dtau = num / HV(q, DEN), dkap = (smu - tau * kap + ctk - kap * dtau) / tau;
-            if (pass == 0) { HV(q, DTA) = dtau; HV(q, DKA) = dkap; }
-            else if (pass == 1) { HV(q, DT0) = dtau; HV(q, DK0) = dkap; }
-            else { HV(q, DT1) = dtau; HV(q, DK1) = dkap; }
-            /* dy = u + dtau v; dZ = eta Rd + dtau C (then - A*(dy)) */
-            lin_add(s->P.m, F_(s, fy), F_(s, fy), 1.0, s->hv, dtau, -1, 0, -1, 0);
-            if (pass < 2) lin_add(s->P.T, F_(s, fdZ), s->Rd, eta, s->C, dtau, -1, 0, -1, 0);
-            else lin_add(s->P.T, F_(s, fdZ), s->C, dtau, -1, 0, -1, 0, -1, 0);
-        }
-        if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
-        if (aty(S, ns, act, fy, fdZ, -1.0)) goto out;
-        /* dX = smu Zi - X - sym(Zi dZ X) [- Q] */
-        if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(Zi), fdZ, OFF(W1))) goto out;
-        if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G))) goto out;
-        if (blk_sym(S, ns, act, OFF(G))) goto out;
-        LinL_clear(&g_ll);
-        for (int q = 0; q < ns; q++) if (act[q]) {
-            Solve *s = S[q];
-            if (pass == 0) lin_add(s->P.T, F_(s, fdX), s->G, -1.0, s->X, -1.0, -1, 0, -1, 0);
-            else if (pass == 1) lin_add(s->P.T, F_(s, fdX), s->G, -1.0, s->X, -1.0, s->Q, -1.0, -1, 0);
-            else lin_add(s->P.T, F_(s, fdX), s->G, -1.0, s->X, -1.0, s->Q, -1.0, s->Zi, HV(q, MU));
-        }
-        if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
-        if (pass == 0) {
-            /* the affine step, mu after it, sigma_m; then Q = sym(Zi dZa dXa) (W1 = Zi dZa) */
-            if (maxstep2(S, ns, act, apx, apz)) goto out;
+        if (!e) e = hsd_form(S, ns, act, V, OFF(dy), OFF(dZ), OFF(dX), 0);
+        if (!e) e = maxstep2(S, ns, act, apx, apz);
+        if (!e) {
             DotL_clear(&g_dl);
             for (int q = 0; q < ns; q++) if (act[q]) {
                 Solve *s = S[q];
@@ -2072,60 +2267,141 @@ static int ipm_iterate_hsd_(Solve **S, int ns) {
                 dot_add(s->P.T, s->dX, s->Z, SC(q, 1));
                 dot_add(s->P.T, s->dX, s->dZ, SC(q, 2));
             }
-            if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+            if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) e = 1;
+        }
+        for (int q = 0; q < ns && !e; q++) if (act[q]) {
+            Solve *s = S[q];
+            const double tau = s->tau, kap = s->kap, dtau = HV(q, DTA), dkap = HV(q, DKA);
+            double am = fmin(apx[q], apz[q]);
+            if (dtau < 0) am = fmin(am, -tau / dtau);
+            if (dkap < 0) am = fmin(am, -kap / dkap);
+            const double aa = fmin(1.0, am);
+            const double mua = (HV(q, XZ) + aa * (HS(q, 0) + HS(q, 1)) + aa * aa * HS(q, 2) + (tau + aa * dtau) * (kap + aa * dkap)) / (s->P.Ntot + 1.0);
+            const double expon = s->relgap > 1e-6 ? fmax(1.0, 3.0 * aa * aa) : 3.0;
+            HV(q, SIGM) = fmin(1.0, pow(fmax(mua / HV(q, MU), 0.0), expon));
+            const int late = fmax(s->pinf, fmax(s->relgap, s->dinf)) <= 100.0 * s->o.tol;
+            nhoc[q] = (s->hoc_nfail >= 3 || late) ? 0 : HSD_PASSES;
+            HV(q, DTC) = dtau; HV(q, DKC) = dkap;
+            hl[q] = 1;
+        }
+        /* 2. the difference: u1 = M^-1 (-rp - mu A(Zi) - A(X Rd Zi)) */
+        if (!e) {
+            LinL_clear(&g_ll);
+            for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->rhs, S[q]->rp, -1.0, S[q]->a0, -HV(q, MU), S[q]->ht, -1.0, -1, 0);
+            if (bk_lincomb(g_ll.v, g_ll.n) || schur_solve(S, ns, act, OFF(rhs), OFF(hy1), 0.0, rel)) e = 1;
+        }
+        if (!e) e = hsd_tau_dots(S, ns, act, OFF(hy1), -1);
+        for (int q = 0; q < ns && !e; q++) if (act[q]) {
+            Solve *s = S[q];
+            const double tau = s->tau, kap = s->kap, mu = HV(q, MU);
+            const double num = -HV(q, RG) - HS(q, 0) + mu * HV(q, CZ) + HV(q, C1) + HS(q, 1) + mu / tau;
+            HV(q, DDT) = num / HV(q, DEN);
+            HV(q, DDK) = (mu - kap * HV(q, DDT)) / tau;
+            HV(q, T0) = HV(q, DDT); HV(q, T1) = -1.0; HV(q, T2) = 0.0; HV(q, T3) = mu; HV(q, T4) = 0.0;
+        }
+        if (!e) e = hsd_form(S, ns, act, V, OFF(hy1), OFF(dZ1), OFF(dX1), 0);
+        /* 3. the passes */
+        for (int hoc = 0; !e; hoc++) {
+            int any = 0;
+            for (int q = 0; q < ns; q++) any |= hl[q];
+            if (!any) break;
+            /* Q = sym(Zi dZ dX) at the current direction, A(Q), <C, Q>; the change against the last pass */
+            if (blk_gemm(S, ns, hl, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(dZ), OFF(W1)) || blk_gemm(S, ns, hl, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(dX), OFF(Q)) ||
+                blk_sym(S, ns, hl, OFF(Q))) { e = 1; break; }
+            LinL_clear(&g_ll);
+            for (int q = 0; q < ns; q++) if (hl[q] && nhoc[q] > 0 && hoc > 0) lin_add(S[q]->P.T, S[q]->G, S[q]->Q, 1.0, S[q]->Qp, -1.0, -1, 0, -1, 0);
+            if (g_ll.n && bk_lincomb(g_ll.v, g_ll.n)) { e = 1; break; }
+            if (aop(S, ns, hl, OFF(Q), OFF(qv), 1.0, 0.0)) { e = 1; break; }
+            DotL_clear(&g_dl);
+            for (int q = 0; q < ns; q++) if (hl[q]) {
+                Solve *s = S[q];
+                dot_add(s->P.T, s->C, s->Q, SC(q, 2));
+                if (nhoc[q] > 0) { if (hoc == 0) dot_add(s->P.T, s->Q, s->Q, SC(q, 3)); else dot_add(s->P.T, s->G, s->G, SC(q, 3)); }
+            }
+            if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) { e = 1; break; }
+            LinL_clear(&g_ll);
+            for (int q = 0; q < ns; q++) if (hl[q]) {
+                Solve *s = S[q];
+                const double tk2 = HV(q, DTC) * HV(q, DKC);
+                if (nhoc[q] > 0) {
+                    const double d = sqrt(HS(q, 3) + (hoc == 0 ? tk2 * tk2 : (tk2 - HV(q, TK2P)) * (tk2 - HV(q, TK2P))));
+                    if (hoc > 0 && !(d <= 0.9 * HV(q, QDP))) { hl[q] = 0; continue; }   /* the last pass that stood keeps its family */
+                    HV(q, QDP) = d; HV(q, TK2P) = tk2;
+                    lin_add(s->P.T, s->Qp, s->Q, 1.0, -1, 0, -1, 0, -1, 0);
+                }
+                HV(q, CK) = -HS(q, 2); HV(q, CTK) = -tk2;
+                lin_add(s->P.m, s->rhs, s->bvec, s->tau, s->ht, 1.0, s->qv, 1.0, -1, 0);
+            }
+            if (bk_lincomb(g_ll.v, g_ll.n)) { e = 1; break; }
+            any = 0;
+            for (int q = 0; q < ns; q++) any |= hl[q];
+            if (!any) break;
+            if (schur_solve(S, ns, hl, OFF(rhs), OFF(hy0), 0.0, rel) || hsd_tau_dots(S, ns, hl, OFF(hy0), -1)) { e = 1; break; }
+            for (int q = 0; q < ns; q++) if (hl[q]) {
+                Solve *s = S[q];
+                const double tau = s->tau, kap = s->kap;
+                const double num = HV(q, RG) - HS(q, 0) - HV(q, CXv) - HV(q, C1) + HS(q, 1) + HV(q, CK) + (-tau * kap + HV(q, CTK)) / tau;
+                HV(q, DT0) = num / HV(q, DEN);
+                HV(q, DK0) = (-tau * kap + HV(q, CTK) - kap * HV(q, DT0)) / tau;
+                HV(q, T0) = HV(q, DT0); HV(q, T1) = 1.0; HV(q, T2) = -1.0; HV(q, T3) = 0.0; HV(q, T4) = -1.0;
+            }
+            if (hsd_form(S, ns, hl, V, OFF(hy0), OFF(dZ0), OFF(dX0), 1)) { e = 1; break; }
+            /* the sigma search: the three candidates' directions in (dX, dZ), (G, F), (W1, W2) */
+            if (hsd_search(S, ns, hl, V, apx, apz, sgc, lo6)) { e = 1; break; }
+            LinL_clear(&g_ll);
+            int restore = 0;
+            for (int q = 0; q < ns; q++) if (hl[q]) {
+                Solve *s = S[q];
+                if (nhoc[q] == 0) { kept[q] = 0; hl[q] = 0; continue; }
+                if (hoc == 0) {
+                    HV(q, M0) = HV(q, BESTM); HV(q, HSIG) = HV(q, BESTS);
+                    HV(q, DT0S) = HV(q, DT0); HV(q, DK0S) = HV(q, DK0);
+                    HV(q, S0S) = HV(q, BESTS); HV(q, A0S) = HV(q, BESTA); HV(q, AM0S) = HV(q, BESTAM);
+                    lin_add(s->P.T, s->dX0s, s->dX0, 1.0, -1, 0, -1, 0, -1, 0);
+                    lin_add(s->P.T, s->dZ0s, s->dZ0, 1.0, -1, 0, -1, 0, -1, 0);
+                    lin_add(s->P.m, s->hy0s, s->hy0, 1.0, -1, 0, -1, 0, -1, 0);
+                }
+                if (hoc == 0 || HV(q, BESTM) >= HV(q, M0)) kept[q] = hoc;
+                else {
+                    /* behind the first family: back to it */
+                    lin_add(s->P.T, s->dX0, s->dX0s, 1.0, -1, 0, -1, 0, -1, 0);
+                    lin_add(s->P.T, s->dZ0, s->dZ0s, 1.0, -1, 0, -1, 0, -1, 0);
+                    lin_add(s->P.m, s->hy0, s->hy0s, 1.0, -1, 0, -1, 0, -1, 0);
+                    HV(q, DT0) = HV(q, DT0S); HV(q, DK0) = HV(q, DK0S);
+                    HV(q, BESTS) = HV(q, S0S); HV(q, BESTA) = HV(q, A0S); HV(q, BESTAM) = HV(q, AM0S);
+                    kept[q] = 0; s->hoc_nfail++; hl[q] = 0; restore = 1;
+                    continue;
+                }
+                if (hoc >= nhoc[q]) { hl[q] = 0; continue; }
+                /* the next pass's direction: the family's member at the first search's sigma */
+                const double sg = HV(q, HSIG);
+                lin_add(s->P.T, s->dX, s->dX0, 1.0, s->dX1, sg, -1, 0, -1, 0);
+                lin_add(s->P.T, s->dZ, s->dZ0, 1.0, s->dZ1, sg, -1, 0, -1, 0);
+                HV(q, DTC) = HV(q, DT0) + sg * HV(q, DDT); HV(q, DKC) = HV(q, DK0) + sg * HV(q, DDK);
+            }
+            (void)restore;
+            if (g_ll.n && bk_lincomb(g_ll.v, g_ll.n)) { e = 1; break; }
+        }
+        /* the chosen member: dy, dX, dZ; dtau, dkappa; the step */
+        if (!e) {
+            LinL_clear(&g_ll);
             for (int q = 0; q < ns; q++) if (act[q]) {
                 Solve *s = S[q];
-                const double tau = s->tau, kap = s->kap, dtau = HV(q, DTA), dkap = HV(q, DKA);
-                double am = fmin(apx[q], apz[q]);
-                if (dtau < 0) am = fmin(am, -tau / dtau);
-                if (dkap < 0) am = fmin(am, -kap / dkap);
-                const double aa = fmin(1.0, am);
-                const double mua = (HV(q, XZ) + aa * (HS(q, 0) + HS(q, 1)) + aa * aa * HS(q, 2) + (tau + aa * dtau) * (kap + aa * dkap)) / (s->P.Ntot + 1.0);
-                const double expon = s->relgap > 1e-6 ? fmax(1.0, 3.0 * aa * aa) : 3.0;
-                HV(q, SIGM) = fmin(1.0, pow(fmax(mua / HV(q, MU), 0.0), expon));
+                const double sg = HV(q, BESTS);
+                lin_add(s->P.T, s->dX, s->dX0, 1.0, s->dX1, sg, -1, 0, -1, 0);
+                lin_add(s->P.T, s->dZ, s->dZ0, 1.0, s->dZ1, sg, -1, 0, -1, 0);
+                lin_add(s->P.m, s->dy, s->hy0, 1.0, s->hy1, sg, -1, 0, -1, 0);
+                HV(q, DTF) = HV(q, DT0) + sg * HV(q, DDT); HV(q, DKF) = HV(q, DK0) + sg * HV(q, DDK);
+                apx[q] = apz[q] = HV(q, BESTA);
             }
-            if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(dX), OFF(Q))) goto out;
-            if (blk_sym(S, ns, act, OFF(Q))) goto out;
-            if (aop(S, ns, act, OFF(Q), OFF(qv), 1.0, 0.0)) goto out;
+            if (bk_lincomb(g_ll.v, g_ll.n)) e = 1;
         }
+        if (!e) e = hsd_long_step(S, ns, act, V, kept, apx);
+        if (!e) e = hsd_safeguard(S, ns, act, V, apx);
+        for (int q = 0; q < ns; q++) if (act[q]) apz[q] = apx[q];
+        free(sgc); free(lo6); free(hl); free(nhoc); free(kept);
+        if (e) goto out;
     }
-    /* the sigma search on d0 + sigma (d1 - d0) */
-    for (int q = 0; q < ns; q++) if (act[q]) { HV(q, BESTM) = -1; HV(q, BESTS) = HV(q, SIGM); HV(q, BESTA) = 0; }
-    for (int c = 0; c < 3; c++) {
-        LinL_clear(&g_ll);
-        for (int q = 0; q < ns; q++) if (act[q]) {
-            Solve *s = S[q];
-            const double sm = HV(q, SIGM);
-            const double sg = c == 0 ? sm : c == 1 ? fmin(0.95, fmax(1e-4, 0.2 * sm)) : fmin(0.95, fmax(1e-3, 3.0 * sm));
-            HV(q, 24 + c) = sg;
-            lin_add(s->P.T, s->dX, s->dX0, 1.0 - sg, s->dX1, sg, -1, 0, -1, 0);
-            lin_add(s->P.T, s->dZ, s->dZ0, 1.0 - sg, s->dZ1, sg, -1, 0, -1, 0);
-        }
-        if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
-        if (maxstep2(S, ns, act, apx, apz)) goto out;
-        for (int q = 0; q < ns; q++) if (act[q]) {
-            Solve *s = S[q];
-            const double sg = HV(q, 24 + c);
-            const double dtau = HV(q, DT0) + sg * (HV(q, DT1) - HV(q, DT0)), dkap = HV(q, DK0) + sg * (HV(q, DK1) - HV(q, DK0));
-            double am = fmin(apx[q], apz[q]);
-            if (dtau < 0) am = fmin(am, -s->tau / dtau);
-            if (dkap < 0) am = fmin(am, -s->kap / dkap);
-            const double gam = 0.9 + (0.99 - 0.9) * fmin(1.0, am);
-            const double al = fmin(1.0, gam * am), merit = al * (1.0 - sg);
-            if (merit > HV(q, BESTM)) { HV(q, BESTM) = merit; HV(q, BESTS) = sg; HV(q, BESTA) = al; }
-        }
-    }
-    /* the chosen direction; X and Z positive definite at one common step (x0.8 on failure) */
-    LinL_clear(&g_ll);
-    for (int q = 0; q < ns; q++) if (act[q]) {
-        Solve *s = S[q];
-        const double sg = HV(q, BESTS);
-        lin_add(s->P.T, s->dX, s->dX0, 1.0 - sg, s->dX1, sg, -1, 0, -1, 0);
-        lin_add(s->P.T, s->dZ, s->dZ0, 1.0 - sg, s->dZ1, sg, -1, 0, -1, 0);
-        lin_add(s->P.m, s->dy, s->hy0, 1.0 - sg, s->hy1, sg, -1, 0, -1, 0);
-        apx[q] = apz[q] = HV(q, BESTA);
-    }
-    if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
     {
         int *ta = calloc(2 * ns, sizeof(int)), *okv = calloc(2 * ns, sizeof(int)), *ok = calloc(2 * ns, sizeof(int));
         double *st = calloc(2 * ns, sizeof(double));
@@ -2176,8 +2452,8 @@ static int ipm_iterate_hsd_(Solve **S, int ns) {
                 if (!act[q]) continue;
                 Solve *s = S[q];
                 if (!okp[q] || !okd[q]) { s->status = 4; s->done = 1; continue; }
-                const double a = fmin(apx[q], apz[q]), sg = HV(q, BESTS);
-                const double dtau = HV(q, DT0) + sg * (HV(q, DT1) - HV(q, DT0)), dkap = HV(q, DK0) + sg * (HV(q, DK1) - HV(q, DK0));
+                const double a = fmin(apx[q], apz[q]);
+                const double dtau = HV(q, DTF), dkap = HV(q, DKF);
                 lin_add(s->P.m, s->y, s->y, 1.0, s->dy, a, -1, 0, -1, 0);
                 s->tau += a * dtau; s->kap += a * dkap;
                 s->alp = s->ald = a;
@@ -2211,7 +2487,6 @@ static int ipm_iterate_hsd_(Solve **S, int ns) {
     for (int q = 0; q < ns; q++) if (act[q] && !S[q]->done) S[q]->it++;
     rc = 0;
 out:
-#undef HV
     free(act); free(fail); free(okp); free(okd); free(nrm); free(V); free(rel); free(apx); free(apz);
     return rc;
 }

@@ -115,7 +115,9 @@ typedef struct Solve {
     int64_t cptr, ent, blk, pptr, ppos, pcon, pval, dinvoff; /* problem data on the device */
     int64_t gptr, grp, Pg, Tg;      /* constraint groups (bytes), heavy-group products P and T (doubles) */
     int64_t pairs;                  /* the constraint pairs sharing a block (bytes) */
-    int64_t Rd, CX, dX0, dZ0, dX1, dZ1, hv, hg, ht, hy0, hy1;   /* self-dual embedding: residual, Zi C X, the corrector family, vectors */
+    int64_t Rd, CX, dX0, dZ0, dX1, dZ1, hv, hg, ht, hy0, hy1;   /* self-dual embedding: residual, Zi C X, the direction family (base, difference), vectors */
+    int64_t Qp, dX0s, dZ0s, hy0s;   /* self-dual embedding: the last second-order term that stood, the first family's base */
+    int hoc_nfail;          /* self-dual embedding: iterations whose second-order passes fell behind the first family */
     int64_t Ad, Af, Pm, Bm, Mpart, LXf, LZif, Om1, Om2;  /* dense route: A f64 / f32 (n^2 x m), P, B, split-K partials, f32 factors, M v scratch */
     int nsplit, kchunk;     /* dense route: split-K of M = P'P */
     int m_exact;            /* the stored M is exact in f64 (else M v uses the operator form) */

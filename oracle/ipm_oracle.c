@@ -916,6 +916,75 @@ int oracle_schur(const OracleData *Dd, const double *X, const double *Zi, double
  * dtau) / tau (solver.c:11127), one step length for all variables, and tau normalised back to 1
  * when it grows past 10 (solver.c:12054-12069). Mehrotra predictor-corrector with the standard
  * loop's sigma and gamma rules; the solution is (X, y, Z) / tau. */
+/* one endpoint of the embedding's direction family (solver.c:11356-11375, 11410-11440): with
+ * u = M^-1 w1(sigma) given, dtau from the gap row, dkappa from the tau-kappa row, dy = u + dtau v,
+ * dZ = eta Rd + dtau C - A'(dy), dX = sigma mu Zi - X - sym(Zi dZ X) - Q (Q the second-order
+ * term, NULL for none; cK = -<C, Q> and ctk = -dtau_a dkappa_a its scalar parts) */
+typedef struct { double tau, kap, mu, rg, cx, c1, cz, den; } HsdScal;
+static void hsd_endpoint(const OProb *P, const HsdScal *h, double sg, const double *u, const double *v, const double *hg,
+                         const double *Rd, const double *Zi, const double *X, const double *Q, double cK, double ctk,
+                         double *dy, double *dZ, double *dX, double *dtau_o, double *dkap_o, double *F, double *G, double *W) {
+    const int m = P->m;
+    const size_t T = P->tot;
+    const double eta = 1.0 - sg, smu = sg * h->mu, tau = h->tau, kap = h->kap;
+    const double c0 = smu * h->cz - h->cx;
+    const double num = eta * h->rg - dot(m, P->b, u) + c0 - eta * h->c1 + dot(m, hg, u) + cK + (smu - tau * kap + ctk) / tau;
+    const double dtau = num / h->den, dkap = (smu - tau * kap + ctk - kap * dtau) / tau;
+    for (int i = 0; i < m; i++) dy[i] = u[i] + dtau * v[i];
+    ATy(P, dy, F);
+    for (size_t q = 0; q < T; q++) dZ[q] = eta * Rd[q] - F[q] + P->C[q] * dtau;
+    prod3(P, Zi, dZ, X, G, W);
+    for (int k = 0; k < P->nb; k++) {
+        const OBlk *B = &P->B[k];
+        const size_t len = B->lp ? (size_t)B->n : (size_t)B->n * B->n;
+        double *dXk = dX + B->off;
+        const double *Gk = G + B->off, *Zik = Zi + B->off, *Xk = X + B->off;
+        for (size_t q = 0; q < len; q++) dXk[q] = -Gk[q];
+        if (!B->lp) symmetrize(B->n, dXk);
+        for (size_t q = 0; q < len; q++) dXk[q] += smu * Zik[q] - Xk[q] - (Q ? Q[B->off + q] : 0.0);
+    }
+    *dtau_o = dtau; *dkap_o = dkap;
+}
+/* the step to the boundary of the cone and of tau, kappa >= 0 along (dX, dZ, dtau, dkappa) */
+static double hsd_amax(const OProb *P, const double *X, const double *LX, const double *dX, const double *Z, const double *LZ,
+                       const double *dZ, double tau, double dtau, double kap, double dkap, double *work) {
+    double am = fmin(maxstep(P, X, LX, dX, work), maxstep(P, Z, LZ, dZ, work));
+    if (dtau < 0) am = fmin(am, -tau / dtau);
+    if (dkap < 0) am = fmin(am, -kap / dkap);
+    return am;
+}
+/* lambda_min((X + a dX)(Z + a dZ)) >= thr on every block: the neighbourhood test of the long step
+ * (reference sdp_nb_ok, solver.c:6470-6504): L = chol(X + a dX), then L'(Z + a dZ) L - thr I must
+ * factor; LP variables: the products themselves */
+static int hsd_nb_ok(const OProb *P, const double *X, const double *dX, const double *Z, const double *dZ, double a, double thr,
+                     double *L, double *T, double *W) {
+    for (int k = 0; k < P->nb; k++) {
+        const OBlk *B = &P->B[k];
+        const int n = B->n;
+        const double *Xk = X + B->off, *dXk = dX + B->off, *Zk = Z + B->off, *dZk = dZ + B->off;
+        if (B->lp) { for (int i = 0; i < n; i++) if (!((Xk[i] + a * dXk[i]) * (Zk[i] + a * dZk[i]) >= thr)) return 0; continue; }
+        const size_t n2 = (size_t)n * n;
+        for (size_t q = 0; q < n2; q++) { L[q] = Xk[q] + a * dXk[q]; T[q] = Zk[q] + a * dZk[q]; }
+        if (chol(n, L)) return 0;
+        gemm_nn(n, T, L, W);                                     /* W = (Z + a dZ) L */
+        for (int j = 0; j < n; j++)                              /* T = L' W - thr I */
+            for (int i = 0; i < n; i++) {
+                double s = 0;
+                for (int r = i; r < n; r++) s += L[r + (size_t)i * n] * W[r + (size_t)j * n];
+                T[i + (size_t)j * n] = s - (i == j ? thr : 0.0);
+            }
+        symmetrize(n, T);
+        if (chol(n, T)) return 0;
+    }
+    return 1;
+}
+/* sum of <X + a dX, Z + a dZ> over the blocks */
+static double hsd_xz_at(size_t T, const double *X, const double *dX, const double *Z, const double *dZ, double a) {
+    double s = 0;
+    for (size_t q = 0; q < T; q++) s += (X[q] + a * dX[q]) * (Z[q] + a * dZ[q]);
+    return s;
+}
+
 static int oracle_solve_hsd(const OracleData *Dd, const OracleOptions *opt, OracleResult *res) {
     memset(res, 0, sizeof(*res));
     res->status = -1;
@@ -940,6 +1009,10 @@ static int oracle_solve_hsd(const OracleData *Dd, const OracleOptions *opt, Orac
     double *t1 = calloc(m + 1, sizeof(double)), *yb = calloc(m + 1, sizeof(double)), *aq = calloc(m + 1, sizeof(double));
     double *ev0 = calloc(m + 1, sizeof(double)), *ev1 = calloc(m + 1, sizeof(double));
     double *dX0 = calloc(T + 1, sizeof(double)), *dX1 = calloc(T + 1, sizeof(double)), *dZ0 = calloc(T + 1, sizeof(double)), *dZ1 = calloc(T + 1, sizeof(double));
+    double *hg = calloc(m + 1, sizeof(double)), *u1 = calloc(m + 1, sizeof(double)), *ev0s = calloc(m + 1, sizeof(double)), *ev1s = calloc(m + 1, sizeof(double));
+    double *Qp = calloc(T + 1, sizeof(double)), *dX0s = calloc(T + 1, sizeof(double)), *dX1s = calloc(T + 1, sizeof(double)), *dZ0s = calloc(T + 1, sizeof(double)), *dZ1s = calloc(T + 1, sizeof(double));
+    double *nbL = malloc(sizeof(double) * (size_t)maxn * maxn + 8);
+    int hoc_nfail = 0;
     OSchur S = { m, calloc((size_t)m * m + 1, sizeof(double)), calloc((size_t)m * m + 1, sizeof(double)), calloc(m + 1, sizeof(double)), 0, 0, 0 };
     int status = 3, it = 0, it_best = 0, nfail = 0, small = 0;
     double best_score = 1e300;
@@ -1039,88 +1112,160 @@ static int oracle_solve_hsd(const OracleData *Dd, const OracleOptions *opt, Orac
         memset(a0, 0, sizeof(double) * m);
         Aop(&P, Zi, a0, 1.0);
         const double cz = dot(T, P.C, Zi);
-        /* the direction at (sigma, eta = 1 - sigma, second-order term K): dy = u + dtau v, dtau from
-         * the gap row, dkappa from the tau-kappa row; pass 0 is the affine direction (sigma = 0,
-         * no K); the corrector family is affine in sigma (solver.c:11356-11375), evaluated at
-         * sigma = 0 and 1 with K from the affine direction, and sigma is chosen among the Mehrotra
-         * value sigma_m, 0.2 sigma_m and 3 sigma_m by the largest predicted reduction alpha (1 - sigma)
-         * of mu (solver.c:11195-11228, 11476-11484) */
-        double sigma_m = 0, dta = 0, dka = 0, dtv[2] = { 0, 0 }, dkv[2] = { 0, 0 };
-        for (int pass = 0; pass < 3; pass++) {
-            const double sg = pass == 2 ? 1.0 : 0.0, eta = 1.0 - sg, smu = sg * mu;
-            for (int i = 0; i < m; i++) w1[i] = eta * rp[i] - smu * a0[i] + (P.b[i] * tau - rp[i]) + eta * t1[i];
-            double cK = 0, ctk = 0;
-            if (pass >= 1) {
-                if (pass == 1) {
-                    prod3(&P, Zi, dZa, dXa, Q, W);         /* K = -sym(Zi dZa dXa), from the affine direction */
-                    sym_all(&P, Q);
-                    memset(gv, 0, sizeof(double) * m);
-                    Aop(&P, Q, gv, 1.0);
-                    memcpy(aq, gv, sizeof(double) * m);    /* A(Q) */
-                }
-                for (int i = 0; i < m; i++) w1[i] += aq[i];
-                cK = -dot(T, P.C, Q);
-                ctk = -dta * dka;
-            }
-            sc_solve(&S, w1, u, 0);
-            memset(gv, 0, sizeof(double) * m);
-            Aop(&P, CX, gv, 1.0);
-            const double c0 = smu * cz - cx;
-            const double den = dot(m, P.b, v) + c2 - dot(m, gv, v) + kap / tau;
-            const double num = eta * rg - dot(m, P.b, u) + c0 - eta * c1 + dot(m, gv, u) + cK + (smu - tau * kap + ctk) / tau;
-            const double dtau = num / den;
-            const double dkap = (smu - tau * kap + ctk - kap * dtau) / tau;
-            double *dyp = pass == 0 ? dy : (pass == 1 ? ev0 : ev1);
-            for (int i = 0; i < m; i++) dyp[i] = u[i] + dtau * v[i];
-            ATy(&P, dyp, F);
-            double *dZp = pass == 0 ? dZ : (pass == 1 ? dZ0 : dZ1), *dXp = pass == 0 ? dX : (pass == 1 ? dX0 : dX1);
-            for (size_t q = 0; q < T; q++) dZp[q] = eta * Rd[q] - F[q] + P.C[q] * dtau;
-            prod3(&P, Zi, dZp, X, G, W);
-            for (int k = 0; k < P.nb; k++) {
-                const OBlk *B = &P.B[k];
-                const size_t len = B->lp ? (size_t)B->n : (size_t)B->n * B->n;
-                double *dXk = dXp + B->off;
-                const double *Gk = G + B->off, *Zik = Zi + B->off, *Xk = X + B->off, *Qk = Q + B->off;
-                for (size_t q = 0; q < len; q++) dXk[q] = -Gk[q];
-                if (!B->lp) symmetrize(B->n, dXk);
-                for (size_t q = 0; q < len; q++) dXk[q] += smu * Zik[q] - Xk[q] + (pass >= 1 ? -Qk[q] : 0.0);
-            }
-            if (pass == 0) {
-                double amax = fmin(maxstep(&P, X, LX, dX, work), maxstep(&P, Z, LZ, dZ, work));
-                if (dtau < 0) amax = fmin(amax, -tau / dtau);
-                if (dkap < 0) amax = fmin(amax, -kap / dkap);
-                const double aa = fmin(1.0, amax);
-                const double xdz = dot(T, X, dZ), dxz = dot(T, dX, Z), dxdz = dot(T, dX, dZ);
-                const double mua = (xz + aa * (dxz + xdz) + aa * aa * dxdz + (tau + aa * dtau) * (kap + aa * dkap)) / nu1;
-                const double expon = relgap > 1e-6 ? fmax(1.0, 3.0 * aa * aa) : 3.0;
-                sigma_m = fmin(1.0, pow(fmax(mua / mu, 0.0), expon));
-                memcpy(dXa, dX, sizeof(double) * T); memcpy(dZa, dZ, sizeof(double) * T);
-                dta = dtau; dka = dkap;
-                (void)anysdp;
-            } else { dtv[pass - 1] = dtau; dkv[pass - 1] = dkap; }
-        }
-        /* the sigma search on d0 + sigma (d1 - d0) */
-        double best_m = -1, best_s = sigma_m, best_a = 0;
+        /* The direction family of the embedding (solver.c:11240-11480). With w1(sigma) = [tau b +
+         * A(X Rd Zi) + A(Q)] + sigma [-rp - mu A(Zi) - A(X Rd Zi)], M u = w1 has the solution u0 +
+         * sigma u1 with u1 independent of the second-order term Q, and every part of the direction is
+         * affine in sigma: two endpoints (sigma 0 and 1) give it for any sigma.
+         * 1. The affine direction (sigma 0, no Q): its step aa and the Mehrotra sigma_m.
+         * 2. Second-order passes: Q = sym(Zi dZ dX) from the current direction (the affine one
+         *    first, then the family's member at the sigma the first search chose), u0 solved again,
+         *    the endpoints formed and sigma searched among sigma_m, 0.2 sigma_m and 3 sigma_m by the
+         *    largest predicted reduction alpha (1 - sigma) of mu (solver.c:11195-11228). A pass
+         *    stands while Q contracts (|Q_k - Q_k-1| <= 0.9 |Q_k-1 - Q_k-2|) and its predicted
+         *    reduction is at least the first family's, else the first family stands; at most four
+         *    passes, none within a factor 100 of the tolerance and none for the rest of the solve
+         *    after three iterations whose passes fell behind the first family (solver.c:11236-11247). */
+        HsdScal hs = { tau, kap, mu, rg, cx, c1, cz, 0 };
+        memset(hg, 0, sizeof(double) * m);
+        Aop(&P, CX, hg, 1.0);                              /* A(X C Zi) */
+        hs.den = dot(m, P.b, v) + c2 - dot(m, hg, v) + kap / tau;
+        for (int i = 0; i < m; i++) w1[i] = P.b[i] * tau + t1[i];
+        sc_solve(&S, w1, u, 0);
+        double dta = 0, dka = 0;
+        hsd_endpoint(&P, &hs, 0.0, u, v, hg, Rd, Zi, X, NULL, 0, 0, dy, dZ, dX, &dta, &dka, F, G, W);
+        const double aa = fmin(1.0, hsd_amax(&P, X, LX, dX, Z, LZ, dZ, tau, dta, kap, dka, work));
+        double sigma_m;
         {
-            const double cand[3] = { sigma_m, fmin(0.95, fmax(1e-4, 0.2 * sigma_m)), fmin(0.95, fmax(1e-3, 3.0 * sigma_m)) };
-            for (int c = 0; c < 3; c++) {
-                const double sg = cand[c];
-                for (size_t q = 0; q < T; q++) { dX[q] = dX0[q] + sg * (dX1[q] - dX0[q]); dZ[q] = dZ0[q] + sg * (dZ1[q] - dZ0[q]); }
-                const double dtau = dtv[0] + sg * (dtv[1] - dtv[0]), dkap = dkv[0] + sg * (dkv[1] - dkv[0]);
-                double am = fmin(maxstep(&P, X, LX, dX, work), maxstep(&P, Z, LZ, dZ, work));
-                if (dtau < 0) am = fmin(am, -tau / dtau);
-                if (dkap < 0) am = fmin(am, -kap / dkap);
-                const double gam = 0.9 + (0.99 - 0.9) * fmin(1.0, am);
-                const double al = fmin(1.0, gam * am), merit = al * (1.0 - sg);
-                if (merit > best_m) { best_m = merit; best_s = sg; best_a = al; }
-            }
+            const double xdz = dot(T, X, dZ), dxz = dot(T, dX, Z), dxdz = dot(T, dX, dZ);
+            const double mua = (xz + aa * (dxz + xdz) + aa * aa * dxdz + (tau + aa * dta) * (kap + aa * dka)) / nu1;
+            const double expon = relgap > 1e-6 ? fmax(1.0, 3.0 * aa * aa) : 3.0;
+            sigma_m = fmin(1.0, pow(fmax(mua / mu, 0.0), expon));
         }
+        for (int i = 0; i < m; i++) w1[i] = -rp[i] - mu * a0[i] - t1[i];
+        sc_solve(&S, w1, u1, 0);
+        const int late = fmax(pinf, fmax(relgap, dinf)) <= 100.0 * tol;
+        const int nhoc = (hoc_nfail >= 3 || late) ? 0 : 4;
+        double dtc = dta, dkc = dka, qd_prev = 0, tk2p = 0;
+        double m0 = -1, best_m = -1, best_s = sigma_m, best_a = 0, best_am = 0, hsig = sigma_m;
+        double dtv[2] = { 0, 0 }, dkv[2] = { 0, 0 }, dtv0[2] = { 0, 0 }, dkv0[2] = { 0, 0 };
+        int kept = 0;
+        for (int hoc = 0; ; hoc++) {
+            prod3(&P, Zi, dZ, dX, Q, W);                   /* Q = sym(Zi dZ dX) at the current direction */
+            sym_all(&P, Q);
+            const double tk2 = dtc * dkc;
+            if (nhoc > 0) {
+                double d = 0;
+                if (hoc == 0) d = dot(T, Q, Q) + tk2 * tk2;
+                else { for (size_t q = 0; q < T; q++) d += (Q[q] - Qp[q]) * (Q[q] - Qp[q]); d += (tk2 - tk2p) * (tk2 - tk2p); }
+                d = sqrt(d);
+                if (hoc > 0 && !(d <= 0.9 * qd_prev)) break;   /* the last pass that stood keeps its family */
+                qd_prev = d; tk2p = tk2;
+                memcpy(Qp, Q, sizeof(double) * T);
+            }
+            memset(aq, 0, sizeof(double) * m);
+            Aop(&P, Q, aq, 1.0);
+            const double cK = -dot(T, P.C, Q), ctk = -tk2;
+            for (int i = 0; i < m; i++) w1[i] = P.b[i] * tau + t1[i] + aq[i];
+            sc_solve(&S, w1, u, 0);
+            for (int i = 0; i < m; i++) w2[i] = u[i] + u1[i];
+            hsd_endpoint(&P, &hs, 0.0, u, v, hg, Rd, Zi, X, Q, cK, ctk, ev0, dZ0, dX0, &dtv[0], &dkv[0], F, G, W);
+            hsd_endpoint(&P, &hs, 1.0, w2, v, hg, Rd, Zi, X, Q, cK, ctk, ev1, dZ1, dX1, &dtv[1], &dkv[1], F, G, W);
+            double mer = -1, sgb = sigma_m, alb = 0, amb = 0;
+            {
+                const double cand[3] = { sigma_m, fmin(0.95, fmax(1e-4, 0.2 * sigma_m)), fmin(0.95, fmax(1e-3, 3.0 * sigma_m)) };
+                for (int c = 0; c < 3; c++) {
+                    const double sg = cand[c];
+                    for (size_t q = 0; q < T; q++) { dX[q] = dX0[q] + sg * (dX1[q] - dX0[q]); dZ[q] = dZ0[q] + sg * (dZ1[q] - dZ0[q]); }
+                    const double dtau = dtv[0] + sg * (dtv[1] - dtv[0]), dkap = dkv[0] + sg * (dkv[1] - dkv[0]);
+                    const double am = hsd_amax(&P, X, LX, dX, Z, LZ, dZ, tau, dtau, kap, dkap, work);
+                    const double gam = 0.9 + (0.99 - 0.9) * fmin(1.0, am);
+                    const double al = fmin(1.0, gam * am), merit = al * (1.0 - sg);
+                    if (merit > mer) { mer = merit; sgb = sg; alb = al; amb = am; }
+                }
+            }
+            if (nhoc == 0) { best_m = mer; best_s = sgb; best_a = alb; best_am = amb; break; }
+            if (hoc == 0) {
+                m0 = mer; hsig = sgb;
+                memcpy(ev0s, ev0, sizeof(double) * m); memcpy(ev1s, ev1, sizeof(double) * m);
+                memcpy(dX0s, dX0, sizeof(double) * T); memcpy(dX1s, dX1, sizeof(double) * T);
+                memcpy(dZ0s, dZ0, sizeof(double) * T); memcpy(dZ1s, dZ1, sizeof(double) * T);
+                dtv0[0] = dtv[0]; dtv0[1] = dtv[1]; dkv0[0] = dkv[0]; dkv0[1] = dkv[1];
+            }
+            if (hoc == 0 || mer >= m0) { best_m = mer; best_s = sgb; best_a = alb; best_am = amb; kept = hoc; }
+            else {
+                /* behind the first family: back to it */
+                memcpy(ev0, ev0s, sizeof(double) * m); memcpy(ev1, ev1s, sizeof(double) * m);
+                memcpy(dX0, dX0s, sizeof(double) * T); memcpy(dX1, dX1s, sizeof(double) * T);
+                memcpy(dZ0, dZ0s, sizeof(double) * T); memcpy(dZ1, dZ1s, sizeof(double) * T);
+                dtv[0] = dtv0[0]; dtv[1] = dtv0[1]; dkv[0] = dkv0[0]; dkv[1] = dkv0[1];
+                kept = 0; hoc_nfail++;
+                {   /* the first family's search result */
+                    const double cand[3] = { sigma_m, fmin(0.95, fmax(1e-4, 0.2 * sigma_m)), fmin(0.95, fmax(1e-3, 3.0 * sigma_m)) };
+                    best_m = -1;
+                    for (int c = 0; c < 3; c++) {
+                        const double sg = cand[c];
+                        for (size_t q = 0; q < T; q++) { dX[q] = dX0[q] + sg * (dX1[q] - dX0[q]); dZ[q] = dZ0[q] + sg * (dZ1[q] - dZ0[q]); }
+                        const double dtau = dtv[0] + sg * (dtv[1] - dtv[0]), dkap = dkv[0] + sg * (dkv[1] - dkv[0]);
+                        const double am = hsd_amax(&P, X, LX, dX, Z, LZ, dZ, tau, dtau, kap, dkap, work);
+                        const double gam = 0.9 + (0.99 - 0.9) * fmin(1.0, am);
+                        const double al = fmin(1.0, gam * am), merit = al * (1.0 - sg);
+                        if (merit > best_m) { best_m = merit; best_s = sg; best_a = al; best_am = am; }
+                    }
+                }
+                break;
+            }
+            if (hoc >= nhoc) break;
+            for (size_t q = 0; q < T; q++) { dX[q] = dX0[q] + hsig * (dX1[q] - dX0[q]); dZ[q] = dZ0[q] + hsig * (dZ1[q] - dZ0[q]); }
+            dtc = dtv[0] + hsig * (dtv[1] - dtv[0]); dkc = dkv[0] + hsig * (dkv[1] - dkv[0]);
+        }
+        (void)best_m;
         {
             const double sg = best_s;
             for (size_t q = 0; q < T; q++) { dX[q] = dX0[q] + sg * (dX1[q] - dX0[q]); dZ[q] = dZ0[q] + sg * (dZ1[q] - dZ0[q]); }
             for (int i = 0; i < m; i++) dy[i] = ev0[i] + sg * (ev1[i] - ev0[i]);
             const double dtau = dtv[0] + sg * (dtv[1] - dtv[0]), dkap = dkv[0] + sg * (dkv[1] - dkv[0]);
             double a = best_a;
+            /* the long step after passes that stood (solver.c:11818-11855): 0.992, then 0.9995 and
+             * 0.998 of the way to the boundary when the new point stays in the neighbourhood
+             * lambda_min(XZ) >= 0.3 mu_new (and tau kappa >= 0.3 mu_new) */
+            if (kept > 0 && best_am >= 0.9) {
+                const double gs[3] = { 0.9995, 0.998, 0.992 };
+                int short_ok = 1, short_tested = 0;
+                {
+                    const double al = fmin(1.0, gs[2] * best_am);
+                    if (al > a && tau + al * dtau > 0 && kap + al * dkap > 0) {
+                        const double mun = mu * (1.0 - al * (1.0 - sg));
+                        short_ok = (tau + al * dtau) * (kap + al * dkap) >= 0.3 * mun && hsd_nb_ok(&P, X, dX, Z, dZ, al, 0.3 * mun, nbL, work, W);
+                        short_tested = 1;
+                    }
+                }
+                for (int c = 0; c < 3 && short_ok; c++) {
+                    const double al = fmin(1.0, gs[c] * best_am);
+                    if (short_tested && c == 2) { a = al; break; }
+                    if (al <= a) break;
+                    if (tau + al * dtau <= 0 || kap + al * dkap <= 0) continue;
+                    const double mun = mu * (1.0 - al * (1.0 - sg));
+                    if ((tau + al * dtau) * (kap + al * dkap) >= 0.3 * mun && hsd_nb_ok(&P, X, dX, Z, dZ, al, 0.3 * mun, nbL, work, W)) { a = al; break; }
+                }
+            }
+            /* neighbourhood safeguard (solver.c:11856-11882): tau kappa and the LP products stay at
+             * least 1e-3 of the new mu (x0.7, at most ten times) */
+            for (int t2 = 0; t2 < 10; t2++) {
+                const double tk = (tau + a * dtau) * (kap + a * dkap);
+                const double munew = (hsd_xz_at(T, X, dX, Z, dZ, a) + tk) / nu1;
+                double lo = tk;
+                for (int k = 0; k < P.nb; k++) {
+                    const OBlk *B = &P.B[k];
+                    if (!B->lp) continue;
+                    for (int i = 0; i < B->n; i++) {
+                        const size_t o = B->off + i;
+                        const double w = (X[o] + a * dX[o]) * (Z[o] + a * dZ[o]);
+                        if (w < lo) lo = w;
+                    }
+                }
+                if (!(munew > 0) || lo >= 1e-3 * munew || a < 1e-3) break;
+                a *= 0.7;
+            }
             /* X and Z positive definite at the common step (x0.8 on failure, solver.c:6393-6446) */
             double ax = a, az = a;
             if (!try_step(&P, X, LX, dX, &ax)) { status = 4; goto fin; }
@@ -1190,6 +1335,7 @@ done:
     free(X); free(Z); free(Zi); free(LX); free(LZ); free(dX); free(dZ); free(F); free(G); free(Q); free(Rd); free(Xb);
     free(CX); free(dXa); free(dZa); free(W); free(work); free(y); free(dy); free(rp); free(w1); free(w2); free(u); free(v);
     free(gv); free(a0); free(t1); free(yb); free(aq); free(ev0); free(ev1); free(dX0); free(dX1); free(dZ0); free(dZ1);
+    free(hg); free(u1); free(ev0s); free(ev1s); free(Qp); free(dX0s); free(dX1s); free(dZ0s); free(dZ1s); free(nbL);
     free(S.M); free(S.L); free(S.D);
     prob_free(&P);
     return 0;
