@@ -117,7 +117,7 @@ API int brisk_result_trace(const BriskResult *r, double *po, double *dob, double
 
 /* ---- options ---- */
 int brisk_parse_options(int nopt, const char *const *opts, SolveOpts *o, int m, int *slots, double *heapgb) {
-    o->tol = 1e-8; o->maxit = 100; o->verbose = 0; o->mixed_until = 1e-6; o->time_limit = 0; o->hsd = 0; o->apply = -1; o->tput = 0; o->polish = 1;
+    o->tol = 1e-8; o->maxit = 100; o->verbose = 0; o->mixed_until = 1e-6; o->time_limit = 0; o->hsd = -1; o->hsd_first = 0; o->hsd_first_c = 12.0; o->hsd_first_asm = 0.5; o->apply = -1; o->tput = 0; o->polish = 1;
     o->prec = m >= 1000 ? PREC_MIXED : PREC_F64;
     for (int k = 0; k < nopt; k++) {
         const char *a = opts[k];
@@ -135,7 +135,14 @@ int brisk_parse_options(int nopt, const char *const *opts, SolveOpts *o, int m, 
         else if (!strcmp(a, "-mixeduntil")) { NEED(); o->mixed_until = atof(nx); }
         else if (!strcmp(a, "-slots")) { NEED(); if (slots) *slots = atoi(nx); }
         else if (!strcmp(a, "-heapgb")) { NEED(); if (heapgb) *heapgb = atof(nx); }
-        else if (!strcmp(a, "-hsd")) { NEED(); o->hsd = atoi(nx) > 0; }
+        else if (!strcmp(a, "-hsd")) {
+            /* the reference's flag (-hsd: the embedding), or -hsd 0 | 1 */
+            if (nx && (!strcmp(nx, "0") || !strcmp(nx, "1"))) { o->hsd = atoi(nx); k++; } else o->hsd = 1;
+        }
+        else if (!strcmp(a, "-nohsd")) o->hsd = 0;
+        else if (!strcmp(a, "-hsdfirst")) { NEED(); o->hsd_first = atoi(nx) > 0; }
+        else if (!strcmp(a, "-hsdfirstc")) { NEED(); o->hsd_first_c = atof(nx); }
+        else if (!strcmp(a, "-hsdfirstasm")) { NEED(); o->hsd_first_asm = atof(nx); }
         else if (!strcmp(a, "-nopolish")) o->polish = 0;
         else if (!strcmp(a, "-throughput")) { NEED(); o->tput = atoi(nx) > 0; }
         else if (!strcmp(a, "-schurapply")) {
@@ -217,6 +224,7 @@ static int solve_many(int n, const BriskProblem *pr, int nopt, const char *const
         const int e = hprob_build(&S[q].P, p->m, p->nblk, p->bs, p->c, p->nnz, p->mat, p->blk, p->i, p->j, p->v);
         built = q + 1;
         if (e) { rc = e; goto out; }
+        if (S[q].o.hsd < 0) S[q].o.hsd = hsd_first_rule(&S[q].P, &S[q].o);
         S[q].P.want_hsd = S[q].o.hsd;
         need += (size_t)solve_heap_bytes(&S[q].P);
         SP[q] = &S[q];

@@ -122,9 +122,40 @@ static SAux *aux_of(Solve *S) {
 /* Mwork: the factorization's n x n scratch, and between factorizations the partial sums of the
  * symmetric products with M (T x T slots of 64, T = ceil(m / 64)) */
 static int64_t mwork_elems(int64_t m) {
-    const int64_t t = (m + 63) / 64, a = m * m, b = t * t * 64;
+    const int64_t t = (m + 63) / 64, a = m * m, b = 3 * t * t * 64;   /* (three sets of partial-sum slots: schur_solve_k) */
     return a > b ? a : b;
 }
+/* The automatic choice of the method (reference brisk_solve_body, solver.c:13519-13568): the
+ * self-dual embedding first unless its extra dense work per iteration, hsd_first_c (12) sum n^3
+ * over the SDP blocks (the C and residual products, the second endpoint), is large next to the
+ * factorization of the Schur complement (m^3/3): it is first when that work is at most m^3/3, or
+ * at most hsd_first_asm (0.5) of m^3/3 plus the assembly, or when the two together stay below 1e7
+ * flops. The assembly's flops are this build's own count of its routes: on the dense route
+ * 4 m n^3 + m^2 n^2 (the products P_i = L' A_i L and M = P'P); elsewhere 2 n^3 + n^2 cnt per
+ * heavy group (its product X A Zi) and two per pair of entries that meet in a block. */
+int hsd_first_rule(const HProb *P, const SolveOpts *o) {
+    if (!o->hsd_first) return 0;
+    double n3 = 0, asm_fl = 0;
+    const double mm = P->m;
+    for (int k = 0; k < P->nb; k++) if (P->bn[k] > 1) { const double n = P->bn[k]; n3 += n * n * n; }
+    if (P->dense) {
+        const double n = P->bn[0];
+        asm_fl = 4.0 * mm * n * n * n + mm * mm * n * n;
+    } else {
+        double *per = calloc(P->nb + 1, sizeof(double));
+        for (int g = 0; g < P->ngrp; g++) {
+            const GrpRef *r = &P->grp[g];
+            const double n = P->bn[r->k];
+            per[r->k] += r->cnt;
+            if (r->poff >= 0) asm_fl += 2.0 * n * n * n + n * n * r->cnt;
+        }
+        for (int k = 0; k < P->nb; k++) asm_fl += 2.0 * per[k] * per[k];
+        free(per);
+    }
+    const double fac = mm * mm * mm / 3.0, dense = o->hsd_first_c * n3;
+    return dense <= fac || dense <= o->hsd_first_asm * (fac + asm_fl) || fac + dense < 1e7;
+}
+
 int64_t solve_heap_bytes(const HProb *P) {
     const int64_t T = P->T, m = P->m;
     int64_t dinv = 0, lz = 0;
@@ -138,7 +169,7 @@ int64_t solve_heap_bytes(const HProb *P) {
     b += (m + 1) * 4 + (int64_t)P->nent * (sizeof(SEnt) + 12) + (int64_t)P->nb * sizeof(BlkRef) + (P->npos + 1) * 12 + 2048;
     b += (m + 1) * 4 + (int64_t)(P->ngrp + 1) * sizeof(GrpRef) + 2 * (P->gp_tot * 8 + 256) + 512;
     b += P->npair * 8 + 256;
-    if (P->want_hsd) b += 9 * (T * 8 + 256) + 6 * (m * 8 + 256) + 4 * (BK_LZ_K + 3) * lz * 8;   /* (and four more Lanczos sides) */
+    if (P->want_hsd) b += 9 * (T * 8 + 256) + 8 * (m * 8 + 256) + 16 * m * 8 + 256 + 4 * (BK_LZ_K + 3) * lz * 8 + (P->dense ? 4 * T * 8 + 256 : 0);   /* (and four more Lanczos sides) */
     if (P->dense) {
         const int64_t nn = T;
         b += nn * m * (8 + 4 + 8 + 8) + 4 * 256;      /* Ad, Af, P, B */
@@ -186,6 +217,8 @@ int solve_setup(Solve *S) {
         ALLOC(Rd, T * 8); ALLOC(CX, T * 8); ALLOC(dX0, T * 8); ALLOC(dZ0, T * 8); ALLOC(dX1, T * 8); ALLOC(dZ1, T * 8);
         ALLOC(hv, m * 8); ALLOC(hg, m * 8); ALLOC(ht, m * 8); ALLOC(hy0, m * 8); ALLOC(hy1, m * 8);
         ALLOC(Qp, T * 8); ALLOC(dX0s, T * 8); ALLOC(dZ0s, T * 8); ALLOC(hy0s, m * 8);
+        ALLOC(rhs2, m * 8); ALLOC(rhs3, m * 8); ALLOC(xvec, 2 * 8 * m * 8);
+        if (P->dense) ALLOC(xom, 4 * T * 8);
     }
     S->tau = 1.0; S->kap = 0.0;
     if (P->dense) {
@@ -1248,6 +1281,46 @@ static int schur_solve(Solve **S, int ns, const int *act0, size_t frhs, size_t f
     return r_;
 }
 
+/* k right-hand sides per solve (k <= 3) in one batched PCG: the solves are entered k times, as
+ * copies whose PCG vectors, preconditioner scratch, partial-sum slots of the symmetric product
+ * (the j-th set of ceil(m/64)^2 slots in Mwork) and operator-form scratch are their own (the
+ * solve's extra vectors), so that every launch covers all k systems; right-hand side j at field
+ * fr[j], solution at fx[j]. The PCG counts and the switch to f64 go back to the solves. */
+static int schur_solve_k(Solve **S, int ns, const int *act, int k, const size_t *fr, const size_t *fx, double tol_rel) {
+    Solve *C = malloc(sizeof(Solve) * (size_t)ns * k);
+    Solve **SS = malloc(sizeof(Solve *) * (size_t)ns * k);
+    int *a = calloc((size_t)ns * k, sizeof(int));
+    double *rel = calloc((size_t)ns * k, sizeof(double));
+    for (int j = 0; j < k; j++)
+        for (int q = 0; q < ns; q++) {
+            const int u = j * ns + q;
+            Solve *c = &C[u];
+            *c = *S[q];
+            SS[u] = c;
+            a[u] = !act || act[q];
+            if (!a[u]) continue;
+            c->rhs = F_(S[q], fr[j]); c->wv = F_(S[q], fx[j]);
+            if (j > 0) {
+                const int64_t m = S[q]->P.m, base = S[q]->xvec + (int64_t)(j - 1) * 8 * m, tt = (m + 63) / 64;
+                c->pr = base; c->pz = base + m; c->pp = base + 2 * m; c->pq = base + 3 * m; c->ps = base + 4 * m; c->pxb = base + 5 * m;
+                c->t1 = base + 6 * m; c->t2 = base + 7 * m;
+                c->Mwork = S[q]->Mwork + (int64_t)j * tt * tt * 64;
+                if (S[q]->P.dense) { c->Om1 = S[q]->xom + (int64_t)(j - 1) * 2 * S[q]->P.T; c->Om2 = c->Om1 + S[q]->P.T; }
+            }
+        }
+    const int rc = schur_solve(SS, ns * k, a, OFF(rhs), OFF(wv), tol_rel, rel);
+    for (int q = 0; q < ns; q++) {
+        if (act && !act[q]) continue;
+        const int p0 = S[q]->pcg_total;          /* (every copy started from it) */
+        for (int j = 0; j < k; j++) {
+            S[q]->pcg_total += C[j * ns + q].pcg_total - p0;
+            if (C[j * ns + q].want_double) S[q]->want_double = 1;
+        }
+    }
+    free(C); free(SS); free(a); free(rel);
+    return rc;
+}
+
 /* ---- start point (reference solver.c:13940-14210) ----------------------------------------- */
 static int set_identity(Solve **S, int ns, const int *act, size_t f) {
     LinL_clear(&g_ll);
@@ -1995,6 +2068,68 @@ out:
     free(ok1); free(a2);
     return rc;
 }
+/* refinement of the chosen direction (reference solver.c:11599-11695): two passes enforcing the
+ * primal rows A(dX) - b dtau = eta rp (eta = 1 - sigma). The miss e is corrected by the
+ * embedding's Newton step for that residual alone: M w = e, dtau_e = -(b - hg)'w / denom, dy_e =
+ * w + dtau_e v, dZ_e = dtau_e C - A*(dy_e), dX_e = -sym(Zi dZ_e X), dkappa_e = -kappa dtau_e / tau;
+ * a pass is taken when it at least halves the miss and stops when the miss is below 1e-2 of
+ * eta |rp| */
+static int hsd_refine(Solve **S, int ns, const int *act, double *V) {
+    int *ra = calloc(ns, sizeof(int)), rc = -1;
+    double *ne = calloc(ns, sizeof(double)), *rel = calloc(ns, sizeof(double));
+    for (int q = 0; q < ns; q++) ra[q] = act[q];
+    for (int pass = 0; pass < 2; pass++) {
+        int any = 0;
+        for (int q = 0; q < ns; q++) any |= ra[q];
+        if (!any) break;
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) if (ra[q]) lin_add(S[q]->P.m, S[q]->ev, S[q]->rp, 1.0 - HV(q, BESTS), S[q]->bvec, HV(q, DTF), -1, 0, -1, 0);
+        if (bk_lincomb(g_ll.v, g_ll.n) || aop(S, ns, ra, OFF(dX), OFF(ev), -1.0, 1.0)) goto out;
+        DotL_clear(&g_dl);
+        for (int q = 0; q < ns; q++) if (ra[q]) { dot_add(S[q]->P.m, S[q]->ev, S[q]->ev, SC(q, 0)); dot_add(S[q]->P.m, S[q]->rp, S[q]->rp, SC(q, 1)); }
+        if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+        any = 0;
+        for (int q = 0; q < ns; q++) if (ra[q]) {
+            ne[q] = sqrt(HS(q, 0));
+            if (!(ne[q] > 1e-2 * (1.0 - HV(q, BESTS)) * sqrt(HS(q, 1)) && ne[q] > 0)) ra[q] = 0; else any = 1;
+        }
+        if (!any) break;
+        if (schur_solve(S, ns, ra, OFF(ev), OFF(wv), 0.0, rel) || hsd_tau_dots(S, ns, ra, OFF(wv), -1)) goto out;
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) if (ra[q]) {
+            Solve *s = S[q];
+            const double dte = -(HS(q, 0) - HS(q, 1)) / HV(q, DEN);
+            HV(q, T0) = dte;
+            lin_add(s->P.m, s->wv, s->wv, 1.0, s->hv, dte, -1, 0, -1, 0);
+            lin_add(s->P.T, s->F, s->C, dte, -1, 0, -1, 0, -1, 0);
+            lin_add(s->P.m, s->qv, s->rp, 1.0 - HV(q, BESTS), s->bvec, HV(q, DTF) + dte, -1, 0, -1, 0);
+        }
+        if (bk_lincomb(g_ll.v, g_ll.n) || aty(S, ns, ra, OFF(wv), OFF(F), -1.0)) goto out;
+        if (blk_gemm(S, ns, ra, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(F), OFF(W1)) || blk_gemm(S, ns, ra, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G)) ||
+            blk_sym(S, ns, ra, OFF(G))) goto out;
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) if (ra[q]) lin_add(S[q]->P.T, S[q]->Q, S[q]->dX, 1.0, S[q]->G, -1.0, -1, 0, -1, 0);
+        if (bk_lincomb(g_ll.v, g_ll.n) || aop(S, ns, ra, OFF(Q), OFF(qv), -1.0, 1.0)) goto out;
+        DotL_clear(&g_dl);
+        for (int q = 0; q < ns; q++) if (ra[q]) dot_add(S[q]->P.m, S[q]->qv, S[q]->qv, SC(q, 0));
+        if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) if (ra[q]) {
+            Solve *s = S[q];
+            if (!(sqrt(HS(q, 0)) < 0.5 * ne[q])) { ra[q] = 0; continue; }
+            const double dte = HV(q, T0);
+            lin_add(s->P.T, s->dX, s->Q, 1.0, -1, 0, -1, 0, -1, 0);
+            lin_add(s->P.T, s->dZ, s->dZ, 1.0, s->F, 1.0, -1, 0, -1, 0);
+            lin_add(s->P.m, s->dy, s->dy, 1.0, s->wv, 1.0, -1, 0, -1, 0);
+            HV(q, DTF) += dte; HV(q, DKF) -= s->kap * dte / s->tau;
+        }
+        if (g_ll.n && bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    }
+    rc = 0;
+out:
+    free(ra); free(ne); free(rel);
+    return rc;
+}
 /* the long step after passes that stood (reference solver.c:11818-11855): where the step to the
  * boundary is at least 0.9, 0.992 of it is tested first; inside the neighbourhood lambda_min(XZ)
  * >= 0.3 mu_new (and tau kappa >= 0.3 mu_new), 0.9995 and then 0.998 are tried, the first one
@@ -2161,8 +2296,8 @@ static int ipm_iterate_hsd_(Solve **S, int ns) {
         const int it = s->it;
         if (it < 128) { s->tr_pobj[it] = po; s->tr_dobj[it] = dob; s->tr_mu[it] = sc * s->mu; }
         if (s->o.verbose)
-            solve_log("%4d %+.10e %+.10e %.1e %.1e %.1e %.1e %.3f tau %.2e kap %.2e %s %.3fs\n", it, po, dob, s->pinf, s->dinf, s->relgap,
-                      sc * s->mu, s->alp, tau, kap, s->f32 ? "f32" : "f64", bk_now() - s->t_start);
+            solve_log("%4d %+.10e %+.10e %.1e %.1e %.1e %.1e %.3f tau %.2e kap %.2e %s retries %d %.3fs\n", it, po, dob, s->pinf, s->dinf, s->relgap,
+                      sc * s->mu, s->alp, tau, kap, s->f32 ? "f32" : "f64", s->nretry, bk_now() - s->t_start);
         int st = -1;
         if (!isfinite(po) || !isfinite(dob)) st = 4;
         else {
@@ -2206,10 +2341,21 @@ static int ipm_iterate_hsd_(Solve **S, int ns) {
     if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(Zi), OFF(Rd), OFF(W1))) goto out;
     if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(X), OFF(G))) goto out;
     if (aop(S, ns, act, OFF(CX), OFF(hg), 1.0, 0.0) || aop(S, ns, act, OFF(G), OFF(ht), 1.0, 0.0) || aop(S, ns, act, OFF(Zi), OFF(a0), 1.0, 0.0)) goto out;
+    /* the three solves of the iteration whose right-hand sides do not depend on the second-order
+     * term, in one PCG: v = M^-1 (b + hg) (the tau column), the affine direction's u =
+     * M^-1 (tau b + ht) into dy, and the family's difference u1 = M^-1 (-rp - mu a0 - ht) into hy1 */
     LinL_clear(&g_ll);
-    for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->rhs, S[q]->bvec, 1.0, S[q]->hg, 1.0, -1, 0, -1, 0);
+    for (int q = 0; q < ns; q++) if (act[q]) {
+        Solve *s = S[q];
+        lin_add(s->P.m, s->rhs, s->bvec, 1.0, s->hg, 1.0, -1, 0, -1, 0);
+        lin_add(s->P.m, s->rhs2, s->bvec, s->tau, s->ht, 1.0, -1, 0, -1, 0);
+        lin_add(s->P.m, s->rhs3, s->rp, -1.0, s->a0, -HV(q, MU), s->ht, -1.0, -1, 0);
+    }
     if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
-    if (schur_solve(S, ns, act, OFF(rhs), OFF(hv), 0.0, rel)) goto out;
+    {
+        const size_t fr[3] = { OFF(rhs), OFF(rhs2), OFF(rhs3) }, fx[3] = { OFF(hv), OFF(dy), OFF(hy1) };
+        if (schur_solve_k(S, ns, act, 3, fr, fx, 0.0)) goto out;
+    }
     DotL_clear(&g_dl);
     for (int q = 0; q < ns; q++) if (act[q]) {
         Solve *s = S[q];
@@ -2244,11 +2390,8 @@ static int ipm_iterate_hsd_(Solve **S, int ns) {
         double *sgc = calloc(3 * (size_t)ns, sizeof(double)), *lo6 = calloc(6 * (size_t)ns, sizeof(double));
         int *hl = calloc(ns, sizeof(int)), *nhoc = calloc(ns, sizeof(int)), *kept = calloc(ns, sizeof(int));
         int e = 0;
-        /* 1. affine: u = M^-1 (tau b + A(X Rd Zi)) */
-        LinL_clear(&g_ll);
-        for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->rhs, S[q]->bvec, S[q]->tau, S[q]->ht, 1.0, -1, 0, -1, 0);
-        if (bk_lincomb(g_ll.v, g_ll.n) || schur_solve(S, ns, act, OFF(rhs), OFF(dy), 1e-7, rel)) e = 1;
-        if (!e) e = hsd_tau_dots(S, ns, act, OFF(dy), -1);
+        /* 1. affine: u = M^-1 (tau b + A(X Rd Zi)) (in dy) */
+        e = hsd_tau_dots(S, ns, act, OFF(dy), -1);
         for (int q = 0; q < ns && !e; q++) if (act[q]) {
             Solve *s = S[q];
             const double tau = s->tau, kap = s->kap;
@@ -2284,12 +2427,7 @@ static int ipm_iterate_hsd_(Solve **S, int ns) {
             HV(q, DTC) = dtau; HV(q, DKC) = dkap;
             hl[q] = 1;
         }
-        /* 2. the difference: u1 = M^-1 (-rp - mu A(Zi) - A(X Rd Zi)) */
-        if (!e) {
-            LinL_clear(&g_ll);
-            for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->rhs, S[q]->rp, -1.0, S[q]->a0, -HV(q, MU), S[q]->ht, -1.0, -1, 0);
-            if (bk_lincomb(g_ll.v, g_ll.n) || schur_solve(S, ns, act, OFF(rhs), OFF(hy1), 0.0, rel)) e = 1;
-        }
+        /* 2. the difference: u1 = M^-1 (-rp - mu A(Zi) - A(X Rd Zi)) (in hy1) */
         if (!e) e = hsd_tau_dots(S, ns, act, OFF(hy1), -1);
         for (int q = 0; q < ns && !e; q++) if (act[q]) {
             Solve *s = S[q];
@@ -2396,6 +2534,7 @@ static int ipm_iterate_hsd_(Solve **S, int ns) {
             }
             if (bk_lincomb(g_ll.v, g_ll.n)) e = 1;
         }
+        if (!e) e = hsd_refine(S, ns, act, V);
         if (!e) e = hsd_long_step(S, ns, act, V, kept, apx);
         if (!e) e = hsd_safeguard(S, ns, act, V, apx);
         for (int q = 0; q < ns; q++) if (act[q]) apz[q] = apx[q];
@@ -2439,7 +2578,7 @@ static int ipm_iterate_hsd_(Solve **S, int ns) {
                 for (int t = 0; t < 2 * ns; t++) {
                     if (!ta[t]) continue;
                     if (ok[t]) { ta[t] = 0; okv[t] = 1; }
-                    else { okv[t] = 0; if (t / ns) apz[t % ns] *= 0.8; else apx[t % ns] *= 0.8; }
+                    else { okv[t] = 0; S[t % ns]->nretry++; if (t / ns) apz[t % ns] *= 0.8; else apx[t % ns] *= 0.8; }
                 }
             }
         }

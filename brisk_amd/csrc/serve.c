@@ -72,6 +72,7 @@ API int brisk_serve_submit(BriskServe *sv, const BriskProblem *p) {
     Solve *s = calloc(1, sizeof(Solve));
     if (brisk_parse_options(sv->nopt, (const char *const *)sv->opts, &s->o, p->m, NULL, NULL)) { free(s); return -1; }
     if (hprob_build(&s->P, p->m, p->nblk, p->bs, p->c, p->nnz, p->mat, p->blk, p->i, p->j, p->v)) { hprob_free(&s->P); free(s); return -2; }
+    if (s->o.hsd < 0) s->o.hsd = hsd_first_rule(&s->P, &s->o);
     s->P.want_hsd = s->o.hsd;
     if (sv->nreq == sv->capreq) { sv->capreq = sv->capreq ? 2 * sv->capreq : 256; sv->req = realloc(sv->req, sizeof(Req) * sv->capreq); }
     const int id = sv->nreq++;

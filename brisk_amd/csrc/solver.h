@@ -95,7 +95,9 @@ typedef struct {
     int verbose;
     double mixed_until;     /* mixed: f32 factor while max(relgap, pinf, dinf) exceeds this (1e-6) */
     double time_limit;      /* s, <= 0 none */
-    int hsd;                /* 1: self-dual embedding, 0: infeasible start */
+    int hsd;                /* 1: self-dual embedding, 0: infeasible start, -1: chosen per problem (hsd_first_rule) */
+    int hsd_first;          /* automatic method: the embedding first where its extra dense work is small (1), the infeasible start first (0) */
+    double hsd_first_c, hsd_first_asm;   /* the rule's constants: dense work c sum n^3 against m^3/3, or against asm (m^3/3 + the assembly) */
     int apply;              /* solves with the Schur factor: -1 a cost model on the solve's own order and precision (default), 0 explicit inverse, 1 block triangular solves */
     int tput;               /* 1: the plan choices for throughput in large batches (256-column panels from order 1024, blocked solves for f64 factors from order 1500); 0: for latency */
     int polish;             /* 1: the FP64 solves' returned point polished (projected onto A(X) = b) when that improves it (reference Params.polish, -nopolish) */
@@ -117,7 +119,9 @@ typedef struct Solve {
     int64_t pairs;                  /* the constraint pairs sharing a block (bytes) */
     int64_t Rd, CX, dX0, dZ0, dX1, dZ1, hv, hg, ht, hy0, hy1;   /* self-dual embedding: residual, Zi C X, the direction family (base, difference), vectors */
     int64_t Qp, dX0s, dZ0s, hy0s;   /* self-dual embedding: the last second-order term that stood, the first family's base */
+    int64_t rhs2, rhs3, xvec, xom;  /* self-dual embedding: right-hand sides and the PCG vectors (8 m each) and operator scratch (2 T each) of two more systems */
     int hoc_nfail;          /* self-dual embedding: iterations whose second-order passes fell behind the first family */
+    int nretry;             /* step-length factorizations that failed (x0.8 retries), all iterations */
     int64_t Ad, Af, Pm, Bm, Mpart, LXf, LZif, Om1, Om2;  /* dense route: A f64 / f32 (n^2 x m), P, B, split-K partials, f32 factors, M v scratch */
     int nsplit, kchunk;     /* dense route: split-K of M = P'P */
     int m_exact;            /* the stored M is exact in f64 (else M v uses the operator form) */
@@ -148,6 +152,7 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
                 const int *ii, const int *jj, const double *v);
 void hprob_free(HProb *P);
 int64_t solve_heap_bytes(const HProb *P);   /* device memory a solve needs */
+int hsd_first_rule(const HProb *P, const SolveOpts *o);   /* the method of an automatic choice: 1 the embedding, 0 the infeasible start */
 
 /* heap allocator (host-managed, first fit) */
 int heap_setup(size_t bytes);

@@ -1223,8 +1223,34 @@ static int oracle_solve_hsd(const OracleData *Dd, const OracleOptions *opt, Orac
             const double sg = best_s;
             for (size_t q = 0; q < T; q++) { dX[q] = dX0[q] + sg * (dX1[q] - dX0[q]); dZ[q] = dZ0[q] + sg * (dZ1[q] - dZ0[q]); }
             for (int i = 0; i < m; i++) dy[i] = ev0[i] + sg * (ev1[i] - ev0[i]);
-            const double dtau = dtv[0] + sg * (dtv[1] - dtv[0]), dkap = dkv[0] + sg * (dkv[1] - dkv[0]);
+            double dtau = dtv[0] + sg * (dtv[1] - dtv[0]), dkap = dkv[0] + sg * (dkv[1] - dkv[0]);
             double a = best_a;
+            /* refinement (solver.c:11599-11695): two passes enforcing A(dX) - b dtau = eta rp, each the
+             * embedding's Newton step for the miss e alone (M w = e, dtau_e = -(b - hg)'w / den, dy_e =
+             * w + dtau_e v, dZ_e = dtau_e C - A'(dy_e), dX_e = -sym(Zi dZ_e X)), taken when it at
+             * least halves the miss; none when the miss is below 1e-2 of eta |rp| */
+            for (int pass = 0; pass < 2; pass++) {
+                const double eta = 1.0 - sg;
+                for (int i = 0; i < m; i++) w1[i] = eta * rp[i] + P.b[i] * dtau;
+                Aop(&P, dX, w1, -1.0);
+                const double ne = sqrt(dot(m, w1, w1)), nrp = sqrt(dot(m, rp, rp));
+                if (!(ne > 1e-2 * eta * nrp && ne > 0)) break;
+                sc_solve(&S, w1, u, 0);
+                const double dte = -(dot(m, P.b, u) - dot(m, hg, u)) / hs.den;
+                for (int i = 0; i < m; i++) u[i] += dte * v[i];
+                ATy(&P, u, F);
+                for (size_t q = 0; q < T; q++) F[q] = P.C[q] * dte - F[q];
+                prod3(&P, Zi, F, X, G, W);
+                sym_all(&P, G);
+                for (size_t q = 0; q < T; q++) Qp[q] = dX[q] - G[q];
+                for (int i = 0; i < m; i++) w2[i] = eta * rp[i] + P.b[i] * (dtau + dte);
+                Aop(&P, Qp, w2, -1.0);
+                if (!(sqrt(dot(m, w2, w2)) < 0.5 * ne)) break;
+                memcpy(dX, Qp, sizeof(double) * T);
+                for (size_t q = 0; q < T; q++) dZ[q] += F[q];
+                for (int i = 0; i < m; i++) dy[i] += u[i];
+                dtau += dte; dkap -= kap * dte / tau;
+            }
             /* the long step after passes that stood (solver.c:11818-11855): 0.992, then 0.9995 and
              * 0.998 of the way to the boundary when the new point stays in the neighbourhood
              * lambda_min(XZ) >= 0.3 mu_new (and tau kappa >= 0.3 mu_new) */

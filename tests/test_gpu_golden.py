@@ -104,6 +104,46 @@ def test_hsd_vs_reference(name, prec, request):
     check(r, rec)
 
 
+# The embedding's iteration count against the reference's default run (its automatic rule takes
+# the embedding first on these: 12 sum n^3 <= m^3 / 3, or a tiny problem), with the second-order
+# passes, the long step and the refinement restated; control1 is left out (tau falls to 5e-5 and
+# the restated embedding, without the reference's shifted tau column and correctors, takes 24
+# iterations against 14).
+HSD_ITER_CASES = ["theta_30_0.3_s3", "theta_50_0.2_s0", "theta_80_0.1_s5", "file_theta1", "file_theta3", "dense_12_30_s0",
+                  "dense_40_120_s1", "multi_4_10_20_60_s0", "multi_6_30_40_300_s2", "file_truss1", "file_example2"]
+
+
+@pytest.mark.parametrize("name", HSD_ITER_CASES)
+def test_hsd_iterations_vs_reference(name):
+    """The self-dual embedding takes at most two iterations more than the reference's default run."""
+    rec = SMALL[name]
+    print("case", name, "hsd iterations", flush=True)
+    if "file" in rec:
+        r = brisk_amd.solve_file(os.path.join(ROOT, "tests", "golden", rec["file"]), options=["-hsd", "1"])
+    else:
+        a = golden_instance(rec).arrays()
+        r = brisk_amd.solve_sdpa(a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"], options=["-hsd", "1"])
+    print("gpu", r.status_str, r.iterations, "ref", rec["iterations"], flush=True)
+    assert r.status == 0
+    assert r.iterations <= rec["iterations"] + 2
+
+
+@pytest.mark.parametrize("name,hsd", [("theta_50_0.2_s0", "1"), ("dense_40_120_s1", "1"), ("maxcut_200_0.05_s1", "0"),
+                                      ("maxcut_cycle_8", "1"), ("multi_6_30_40_300_s2", "1")])
+def test_hsd_first_rule(name, hsd):
+    """-hsdfirst 1 (the reference's automatic rule, solver.c:13519-13568) runs the embedding where its
+    extra dense work 12 sum n^3 is at most m^3 / 3, or the problem is tiny, and the infeasible start
+    elsewhere: its result is bit for bit the run with that method forced."""
+    rec = SMALL[name]
+    print("case", name, "hsdfirst", flush=True)
+    a = golden_instance(rec).arrays()
+    run = lambda *o: brisk_amd.solve_sdpa(a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"], options=list(o))
+    r, f = run("-hsdfirst", "1"), run("-hsd", hsd)
+    print("rule", r.status_str, r.iterations, repr(r.pobj), "forced", f.status_str, f.iterations, repr(f.pobj), flush=True)
+    assert r.status == f.status == 0 and r.iterations == f.iterations
+    assert r.pobj == f.pobj and r.dobj == f.dobj
+
+
 def _polish_runs(name, *o):
     """The run of a golden instance with options o, and the same run with -nopolish."""
     rec = SMALL[name]
