@@ -54,7 +54,7 @@ int bk_prof_on(void);                            /* profiling enabled */
 void bk_prof_host(int cls, double seconds);      /* add a host-side interval to class cls (40-59) */
 /* classes: 0-3 k_gemm<double, ta, tb> (2 ta + tb), 4-7 k_gemm<float, ta, tb>, 8 k_potrf_diag,
  * 9 k_gemv, 10 Schur assembly, 11 constraint operators, 12 Lanczos steps, 13 dot products,
- * 14 linear combinations, 15 k_symv (tiles and reduction), 16 k_trsv (both directions) */
+ * 14 linear combinations, 15 k_symv (tiles and reduction), 16 k_trsv (both directions), 17 k_syev */
 
 /* ---- batched GEMM: C_it = alpha op(A_it) op(B_it) + beta C_it ---- */
 typedef struct {
@@ -249,6 +249,13 @@ int bk_lanczos_start(const LzItem *it, int n_it);
  * at K. Writes theta - res to out (doubles), -1e300 when the coefficients are not finite. */
 typedef struct { int n, K, mode, pad; int64_t Li, LiT, V, out; double tol; } LzfItem;
 int bk_lanczos_fused(const LzfItem *it, int n_it);
+
+/* ---- symmetric eigendecomposition (parallel cyclic Jacobi, one workgroup per matrix) ----
+ * A (n x n at a, both triangles) = V diag(w) V': V (n x n, column-major) to v, the eigenvalues
+ * (unsorted, in the order of V's columns) to w; offsets in doubles from H. The matrix at a is
+ * overwritten for n > 64 (transformed in place); n <= 4096. */
+typedef struct { int n, pad; int64_t a, v, w; } EigItem;
+int bk_syev_f64(const EigItem *it, int n_it, double *H);
 
 #ifdef __cplusplus
 }

@@ -25,6 +25,10 @@ class SymvItem(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("ld", ctypes.c_int), ("a", I64), ("x", I64), ("y", I64), ("p", I64)]
 
 
+class EigItem(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("pad", ctypes.c_int), ("a", I64), ("v", I64), ("w", I64)]
+
+
 class TrsvItem(ctypes.Structure):
     _fields_ = [("m", ctypes.c_int), ("c", ctypes.c_int), ("pre", I64), ("L", I64), ("Di", I64), ("DiT", I64), ("v0", I64), ("v1", I64)]
 
@@ -64,6 +68,7 @@ def lib():
             sc = ctypes.c_float if nm == "bk_gemv_f32" else ctypes.c_double
             getattr(L, nm).argtypes = [ctypes.c_int, sc, sc, P, P, P, P, ctypes.c_int]
         L.bk_symv_f64.argtypes = [ctypes.c_double, ctypes.c_double, P, P, P, P, P, ctypes.c_int]
+        L.bk_syev_f64.argtypes = [P, ctypes.c_int, P]
         if L.bk_init() != 0:
             raise RuntimeError("bk_init failed: %s" % L.bk_error())
         _lib = L

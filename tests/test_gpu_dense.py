@@ -2,7 +2,7 @@
 triangular inverse, matrix-vector products) against numpy/LAPACK on the same inputs."""
 import numpy as np
 import pytest
-from _gpu_util import lib, DBuf, GemmItem, MItem, GemvItem, SymvItem, TrsvItem, items_array
+from _gpu_util import lib, DBuf, GemmItem, MItem, GemvItem, SymvItem, TrsvItem, EigItem, items_array
 
 pytestmark = pytest.mark.gpu
 
@@ -311,3 +311,59 @@ def test_symv_lower_batched():
     out = dy.get()
     for (o, ref) in refs:
         assert np.allclose(out[o:o + ref.size], ref, rtol=1e-12, atol=1e-11)
+
+
+def _sym_cases(rng):
+    """Symmetric test matrices: random of several orders (the in-LDS form up to 64, in place above),
+    indefinite, a tight cluster around mu (the central path's X Z), a graded positive definite one,
+    repeated eigenvalues, and orders 1 and 2."""
+    out = []
+    for n in (1, 2, 3, 17, 63, 64, 65, 130, 301):
+        B = rng.standard_normal((n, n))
+        out.append((B + B.T) / 2)
+    n = 60
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    out.append((Q * (1.0 + 1e-6 * rng.standard_normal(n))) @ Q.T)          # cluster at 1
+    out.append((Q * np.logspace(-10, 2, n)) @ Q.T)                          # graded, condition 1e12
+    out.append((Q * np.repeat([-2.0, 0.5, 3.0], n // 3)) @ Q.T)             # three repeated eigenvalues
+    n = 200
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    out.append((Q * np.linspace(-1, 1, n) ** 3) @ Q.T)
+    return out
+
+
+def test_syev_jacobi():
+    """The batched Jacobi eigendecomposition against LAPACK (numpy.linalg.eigvalsh): eigenvalues to
+    a few units of eps |A|, V orthogonal, A V = V diag(w); every matrix in one call."""
+    rng = np.random.default_rng(11)
+    mats = _sym_cases(rng)
+    parts, rows, off = [], [], 0
+    for A in mats:
+        n = A.shape[0]
+        parts.append(A.flatten(order="F"))
+        rows.append((n, 0, off, 0, 0))
+        off += n * n
+    vo = off
+    for k, A in enumerate(mats):
+        n = A.shape[0]
+        rows[k] = (n, 0, rows[k][2], vo, 0)
+        vo += n * n
+    wo = vo
+    for k, A in enumerate(mats):
+        n = A.shape[0]
+        rows[k] = (n, 0, rows[k][2], rows[k][3], wo)
+        wo += n
+    H = DBuf(np.concatenate(parts + [np.zeros(wo - off)]))
+    assert lib().bk_syev_f64(items_array(EigItem, rows), len(rows), H.p) == 0
+    out = H.get()
+    for k, A in enumerate(mats):
+        n = A.shape[0]
+        V = out[rows[k][3]:rows[k][3] + n * n].reshape((n, n), order="F")
+        w = out[rows[k][4]:rows[k][4] + n]
+        nrm = np.abs(np.linalg.eigvalsh(A)).max() if n else 1.0
+        ref = np.linalg.eigvalsh(A)
+        print("n", n, "eig err %.1e" % (np.abs(np.sort(w) - ref).max() / nrm), "orth %.1e" % np.abs(V.T @ V - np.eye(n)).max(),
+              "resid %.1e" % (np.abs(A @ V - V * w).max() / nrm), flush=True)
+        assert np.abs(np.sort(w) - ref).max() <= 1e-13 * n * nrm + 1e-300
+        assert np.abs(V.T @ V - np.eye(n)).max() <= 1e-13 * max(n, 8)
+        assert np.abs(A @ V - V * w).max() <= 1e-13 * max(n, 8) * nrm
