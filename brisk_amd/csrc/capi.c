@@ -117,7 +117,7 @@ API int brisk_result_trace(const BriskResult *r, double *po, double *dob, double
 
 /* ---- options ---- */
 int brisk_parse_options(int nopt, const char *const *opts, SolveOpts *o, int m, int *slots, double *heapgb) {
-    o->tol = 1e-8; o->maxit = 100; o->verbose = 0; o->mixed_until = 1e-6; o->time_limit = 0; o->hsd = -1; o->hsd_first = 0; o->hsd_first_c = 12.0; o->hsd_first_asm = 0.5; o->apply = -1; o->tput = 0; o->polish = 1;
+    o->tol = 1e-8; o->maxit = 100; o->verbose = 0; o->mixed_until = 1e-6; o->time_limit = 0; o->hsd = -1; o->hsd_first = 0; o->hsd_first_c = 12.0; o->hsd_first_asm = 0.5; o->dir = 0; o->apply = -1; o->tput = 0; o->polish = 1;
     o->prec = m >= 1000 ? PREC_MIXED : PREC_F64;
     for (int k = 0; k < nopt; k++) {
         const char *a = opts[k];
@@ -140,6 +140,13 @@ int brisk_parse_options(int nopt, const char *const *opts, SolveOpts *o, int m, 
             if (nx && (!strcmp(nx, "0") || !strcmp(nx, "1"))) { o->hsd = atoi(nx); k++; } else o->hsd = 1;
         }
         else if (!strcmp(a, "-nohsd")) o->hsd = 0;
+        else if (!strcmp(a, "-dir")) {
+            NEED();
+            if (!strcmp(nx, "nt")) o->dir = 1;
+            else if (!strcmp(nx, "hkm")) o->dir = 0;
+            else if (!strcmp(nx, "auto")) o->dir = -1;
+            else { say(1, "brisk: -dir must be auto, hkm or nt\n"); return 1; }
+        }
         else if (!strcmp(a, "-hsdfirst")) { NEED(); o->hsd_first = atoi(nx) > 0; }
         else if (!strcmp(a, "-hsdfirstc")) { NEED(); o->hsd_first_c = atof(nx); }
         else if (!strcmp(a, "-hsdfirstasm")) { NEED(); o->hsd_first_asm = atof(nx); }
@@ -226,6 +233,8 @@ static int solve_many(int n, const BriskProblem *pr, int nopt, const char *const
         if (e) { rc = e; goto out; }
         if (S[q].o.hsd < 0) S[q].o.hsd = hsd_first_rule(&S[q].P, &S[q].o);
         S[q].P.want_hsd = S[q].o.hsd;
+        if (S[q].o.dir < 0) S[q].o.dir = nt_rule(&S[q].P);
+        S[q].P.want_nt = !S[q].o.hsd && S[q].o.dir == 1;
         need += (size_t)solve_heap_bytes(&S[q].P);
         SP[q] = &S[q];
         S[q].id = q;

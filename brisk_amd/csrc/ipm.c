@@ -156,6 +156,18 @@ int hsd_first_rule(const HProb *P, const SolveOpts *o) {
     return dense <= fac || dense <= o->hsd_first_asm * (fac + asm_fl) || fac + dense < 1e7;
 }
 
+/* The automatic choice of the direction of the infeasible-start method (reference
+ * brisk_solve_body, solver.c:13582-13592): Nesterov-Todd, whose scaling costs about 15 n^3 more
+ * dense work per SDP block and iteration than HKM, where that is small next to the Schur
+ * factorization and HKM's own dense work (15 sum n^3 < 0.25 (m^3/3 + 5 sum n^3)) or at most 1e8
+ * flops in all. */
+int nt_rule(const HProb *P) {
+    double n3 = 0;
+    const double mm = P->m;
+    for (int k = 0; k < P->nb; k++) if (P->bn[k] > 1) { const double n = P->bn[k]; n3 += n * n * n; }
+    return 15.0 * n3 < 0.25 * (mm * mm * mm / 3.0 + 5.0 * n3) || 15.0 * n3 <= 1e8;
+}
+
 int64_t solve_heap_bytes(const HProb *P) {
     const int64_t T = P->T, m = P->m;
     int64_t dinv = 0, lz = 0;
@@ -170,6 +182,7 @@ int64_t solve_heap_bytes(const HProb *P) {
     b += (m + 1) * 4 + (int64_t)(P->ngrp + 1) * sizeof(GrpRef) + 2 * (P->gp_tot * 8 + 256) + 512;
     b += P->npair * 8 + 256;
     if (P->want_hsd) b += 9 * (T * 8 + 256) + 8 * (m * 8 + 256) + 16 * m * 8 + 256 + 4 * (BK_LZ_K + 3) * lz * 8 + (P->dense ? 4 * T * 8 + 256 : 0);   /* (and four more Lanczos sides) */
+    if (P->want_nt) b += 6 * (T * 8 + 256) + 4 * lz * 8 + 256;
     if (P->dense) {
         const int64_t nn = T;
         b += nn * m * (8 + 4 + 8 + 8) + 4 * 256;      /* Ad, Af, P, B */
@@ -220,6 +233,12 @@ int solve_setup(Solve *S) {
         ALLOC(rhs2, m * 8); ALLOC(rhs3, m * 8); ALLOC(xvec, 2 * 8 * m * 8);
         if (P->dense) ALLOC(xom, 4 * T * 8);
     }
+    S->nt = !S->hsd && S->o.dir == 1;
+    if (S->nt) {
+        ALLOC(Wn, T * 8); ALLOC(Gn, T * 8); ALLOC(GnT, T * 8); ALLOC(Gi, T * 8); ALLOC(ntV, T * 8); ALLOC(ntA, T * 8);
+        ALLOC(lam, 4 * lz * 8);
+    }
+    S->SX = S->X; S->SZi = S->Zi; S->SLX = S->LX; S->SLZi = S->LZi;
     S->tau = 1.0; S->kap = 0.0;
     if (P->dense) {
         const int64_t nn = T;
@@ -365,7 +384,7 @@ static int blk_gemm(Solve **S, int ns, const int *act, int ta, int tb, int tri, 
     double *H = heapp();
     return bk_gemm_f64(ta, tb, alpha, beta, H, H, H, g_gl.v, g_gl.n, NULL);
 }
-/* Out = Zi V X per block of the active solves, V a dual-side direction: dZ = theta R0 - A*(dy)
+/* Out = Zi V X per block of the active solves (Zi, X the scaling pair: W V W for the Nesterov-Todd direction), V a dual-side direction: dZ = theta R0 - A*(dy)
  * (R0 = -shift I at the start, C - Z - A*(y) after a re-base), or the refinement's A*(w). Each
  * lies in the span of I and the A_i, so it is diagonal in a block whose constraint entries all
  * are (max-cut, LP variables): there Out = Zi (Diag(V) X), a row scaling into W1 and one GEMM;
@@ -382,7 +401,7 @@ static int zi_v_x_(Solve **S, int ns, const int *act, size_t fV, size_t fX, size
     for (int q = 0; q < ns; q++) {
         if (act && !act[q]) continue;
         const HProb *P = &S[q]->P;
-        const int64_t v = F_(S[q], fV), x = F_(S[q], fX), w = S[q]->W1, zi = S[q]->Zi, out = F_(S[q], fOut);
+        const int64_t v = F_(S[q], fV), x = F_(S[q], fX), w = S[q]->W1, zi = S[q]->SZi, out = F_(S[q], fOut);
         for (int k = 0; k < P->nb; k++) {
             const int n = P->bn[k];
             const int64_t o = P->boff[k];
@@ -399,7 +418,7 @@ static int zi_v_x_(Solve **S, int ns, const int *act, size_t fV, size_t fX, size
     for (int q = 0; q < ns; q++) {
         if (act && !act[q]) continue;
         const HProb *P = &S[q]->P;
-        const int64_t x = F_(S[q], fX), w = S[q]->W1, zi = S[q]->Zi, out = F_(S[q], fOut);
+        const int64_t x = F_(S[q], fX), w = S[q]->W1, zi = S[q]->SZi, out = F_(S[q], fOut);
         for (int k = 0; k < P->nb; k++) {
             const int n = P->bn[k];
             const int64_t o = P->boff[k];
@@ -774,18 +793,21 @@ static int schur_dense(Solve *s, int prec, int gram) {
     GemmL_clear(&g_gl);
     if (gram) Pb = prec ? s->Ad : 2 * s->Af;
     else {
-        int64_t lx = s->LX, lz = s->LZi;
+        /* HKM: X = LX LX', Zi = LZi' LZi (triangular: the products' K ranges trimmed); NT: W = G G'
+         * with SLX = G and SLZi = G' (full) */
+        int64_t lx = s->SLX, lz = s->SLZi;
+        const int ta = s->SLX == s->LX ? 2 : 0, tb = s->SLZi == s->LZi ? 16 : 0;
         if (!prec) {
-            if (bk_d2f(nn, s->LX, 2 * s->LXf) || bk_d2f(nn, s->LZi, 2 * s->LZif)) return -1;
+            if (bk_d2f(nn, s->SLX, 2 * s->LXf) || bk_d2f(nn, s->SLZi, 2 * s->LZif)) return -1;
             lx = 2 * s->LXf; lz = 2 * s->LZif;
         }
         const int64_t A = prec ? s->Ad : 2 * s->Af, B = mul * s->Bm;
         Pb = mul * s->Pm;
-        for (int i = 0; i < m; i++) { GemmItem g = { n, n, n, n, n, n, lx, A + i * nn, B + i * nn, -1, 2 }; GemmL_add(&g_gl, g); }
+        for (int i = 0; i < m; i++) { GemmItem g = { n, n, n, n, n, n, lx, A + i * nn, B + i * nn, -1, ta }; GemmL_add(&g_gl, g); }
         if (prec ? bk_gemm_f64(1, 0, 1.0, 0.0, H, H, H, g_gl.v, g_gl.n, NULL)
                  : bk_gemm_f32(1, 0, 1.0f, 0.0f, (float *)H, (float *)H, (float *)H, g_gl.v, g_gl.n, NULL)) return -1;
         GemmL_clear(&g_gl);
-        for (int i = 0; i < m; i++) { GemmItem g = { n, n, n, n, n, n, B + i * nn, lz, Pb + i * nn, -1, 16 }; GemmL_add(&g_gl, g); }
+        for (int i = 0; i < m; i++) { GemmItem g = { n, n, n, n, n, n, B + i * nn, lz, Pb + i * nn, -1, tb }; GemmL_add(&g_gl, g); }
         if (prec ? bk_gemm_f64(0, 1, 1.0, 0.0, H, H, H, g_gl.v, g_gl.n, NULL)
                  : bk_gemm_f32(0, 1, 1.0f, 0.0f, (float *)H, (float *)H, (float *)H, g_gl.v, g_gl.n, NULL)) return -1;
     }
@@ -804,7 +826,7 @@ static int schur_dense(Solve *s, int prec, int gram) {
     s->m_exact = prec;
     return 0;
 }
-/* the products of the heavy groups at the current X, Zi: T = A Zi_k, then P = X_k T (MFMA) */
+/* the products of the heavy groups at the current scaling pair (X, Zi, or W, W): T = A Zi_k, then P = X_k T (MFMA) */
 static int heavy_products(Solve **S, int ns, const int *act) {
     int ngi = 0;
     for (int q = 0; q < ns; q++) if ((!act || act[q]) && !S[q]->P.dense && S[q]->P.gp_tot > 0) ngi = 1;
@@ -822,12 +844,12 @@ static int heavy_products(Solve **S, int ns, const int *act) {
             if (n == cap) { cap = cap ? 2 * cap : 256; gi = realloc(gi, sizeof(GrpItem) * cap); gm = realloc(gm, sizeof(GemmItem) * cap); }
             const int nk = P->bn[G->k];
             const int64_t boff = P->boff[G->k];
-            GrpItem it = { nk, G->cnt, s->ent + (int64_t)G->beg * (int64_t)sizeof(SEnt), s->Zi + boff, s->Tg + G->poff };
+            GrpItem it = { nk, G->cnt, s->ent + (int64_t)G->beg * (int64_t)sizeof(SEnt), s->SZi + boff, s->Tg + G->poff };
             gi[n] = it;
             GemmItem m;
             memset(&m, 0, sizeof(m));
             m.M = nk; m.N = nk; m.K = nk; m.lda = nk; m.ldb = nk; m.ldc = nk;
-            m.a = s->X + boff; m.b = s->Tg + G->poff; m.c = s->Pg + G->poff;
+            m.a = s->SX + boff; m.b = s->Tg + G->poff; m.c = s->Pg + G->poff;
             m.guard = -1; m.tri = 0;
             gm[n] = m;
             n++;
@@ -852,7 +874,7 @@ static int schur_assemble_(Solve **S, int ns, const int *act, int gram) {
         }
         S[q]->m_exact = 1;
         ConItem c = con_item(S[q]);
-        c.X = S[q]->X; c.Zi = S[q]->Zi; c.M = S[q]->M;
+        c.X = S[q]->SX; c.Zi = S[q]->SZi; c.M = S[q]->M;
         ConL_add(&g_cl, c);
     }
     return bk_schur(g_cl.v, g_cl.n);
@@ -1108,8 +1130,8 @@ static int schur_mv(Solve **S, int ns, const int *act, size_t fx, size_t fy, dou
     for (int q = 0; q < ns; q++) if (op[q]) lin_add(S[q]->P.T, S[q]->Om1, -1, 0, -1, 0, -1, 0, -1, 0);
     int rc = bk_lincomb(g_ll.v, g_ll.n);
     rc = rc || aty(S, ns, op, fx, OFF(Om1), 1.0);
-    rc = rc || blk_gemm(S, ns, op, 0, 0, 0, 1.0, 0.0, OFF(X), OFF(Om1), OFF(Om2));
-    rc = rc || blk_gemm(S, ns, op, 0, 0, 0, 1.0, 0.0, OFF(Om2), OFF(Zi), OFF(Om1));
+    rc = rc || blk_gemm(S, ns, op, 0, 0, 0, 1.0, 0.0, OFF(SX), OFF(Om1), OFF(Om2));
+    rc = rc || blk_gemm(S, ns, op, 0, 0, 0, 1.0, 0.0, OFF(Om2), OFF(SZi), OFF(Om1));
     rc = rc || aop(S, ns, op, OFF(Om1), fy, alpha, beta);
     free(op);
     return rc ? -1 : 0;
@@ -1466,6 +1488,94 @@ int ipm_iterate_batch(Solve **S, int ns) {
     return r_;
 }
 int ipm_iterate_hsd(Solve **S, int ns) { return ipm_iterate_hsd_(S, ns); }
+/* ---- the Nesterov-Todd direction (reference nt_scaling, nt_toX / nt_toZ / nt_fromS,
+ * solver.c:7813-8068) ---------------------------------------------------------------------- */
+/* per block with X = L_X L_X' and Z = L_Z L_Z': L_X' Z L_X = (L_Z' L_X)' (L_Z' L_X) = V diag(lambda^2)
+ * V' (the eigenvalues are those of X Z), G = L_X V diag(lambda^-1/2), W = G G' (W Z W = X),
+ * G^-1 = diag(lambda^1/2) V' L_X^-1; LP variables (blocks of order 1) come out as W = sqrt(x / z).
+ * The Schur complement and the direction's products then take (W, W) in the place of (X, Zi)
+ * (the scaling pair SX, SZi; SLX = G, SLZi = G' for the dense route). A solve whose eigenvalues
+ * are not all positive keeps HKM for the iteration. */
+static int nt_scaling(Solve **S, int ns, const int *act) {
+    int *a = calloc(ns, sizeof(int)), any = 0, rc = -1;
+    for (int q = 0; q < ns; q++) { a[q] = act[q] && S[q]->nt; any |= a[q]; }
+    if (!any) { free(a); return 0; }
+    double *H = heapp();
+    EigItem *ei = NULL;
+    ScaleItem *si = NULL;
+    int ne = 0, cap = 0;
+    if (blk_gemm(S, ns, a, 1, 0, 0, 1.0, 0.0, OFF(LZ), OFF(LX), OFF(Gn)) || blk_gemm(S, ns, a, 1, 0, 0, 1.0, 0.0, OFF(Gn), OFF(Gn), OFF(ntA)) ||
+        blk_sym(S, ns, a, OFF(ntA))) goto out;
+    for (int q = 0; q < ns; q++) if (a[q]) {
+        Solve *s = S[q];
+        SAux *A = aux_of(s);
+        for (int k = 0; k < s->P.nb; k++) {
+            if (ne == cap) { cap = cap ? 2 * cap : 256; ei = realloc(ei, sizeof(EigItem) * cap); si = realloc(si, sizeof(ScaleItem) * cap); }
+            const int64_t o = s->P.boff[k];
+            EigItem e = { s->P.bn[k], 0, s->ntA + o, s->ntV + o, s->lam + A->lzoff[k] };
+            ScaleItem c = { s->P.bn[k], 3, s->lam + A->lzoff[k], s->lam + (int64_t)s->P.Ntot + 3 * A->lzoff[k] };
+            ei[ne] = e; si[ne] = c; ne++;
+        }
+    }
+    if (bk_syev_f64(ei, ne, H) || bk_nt_scale(si, ne, H)) goto out;
+    /* an eigenvalue <= 0 leaves a NaN among the solve's lambda^-1/2 */
+    DotL_clear(&g_dl);
+    for (int q = 0; q < ns; q++) if (a[q]) { const int64_t nt = (int64_t)S[q]->P.Ntot; dot_add(3 * nt, S[q]->lam + nt, S[q]->lam + nt, SC(q, 0)); }
+    if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+    any = 0;
+    for (int q = 0; q < ns; q++) if (a[q]) { if (!isfinite(HS(q, 0))) a[q] = 0; else any = 1; }
+    if (!any) { rc = 0; goto out; }
+    /* G = L_X V diag(lambda^-1/2); W = G G'; G'; G^-1 = diag(lambda^1/2) V' L_X^-1 */
+    if (blk_gemm(S, ns, a, 0, 0, 4, 1.0, 0.0, OFF(LX), OFF(ntV), OFF(Gn)) || blk_gemm(S, ns, a, 1, 0, 8, 1.0, 0.0, OFF(ntV), OFF(LXi), OFF(Gi))) goto out;
+    ne = 0;
+    MItL_clear(&g_ml);
+    for (int q = 0; q < ns; q++) if (a[q]) {
+        Solve *s = S[q];
+        SAux *A = aux_of(s);
+        for (int k = 0; k < s->P.nb; k++) {
+            if (ne + 2 > cap) { cap = 2 * cap + 2; ei = realloc(ei, sizeof(EigItem) * cap); si = realloc(si, sizeof(ScaleItem) * cap); }
+            const int n = s->P.bn[k];
+            const int64_t o = s->P.boff[k], u = s->lam + (int64_t)s->P.Ntot + 3 * A->lzoff[k];
+            ScaleItem c0 = { n, 0, s->Gn + o, u + n }, c1 = { n, 1, s->Gi + o, u + 2 * n };
+            si[ne++] = c0; si[ne++] = c1;
+            MItem it = { n, 0, s->Gn + o, 0, 0, s->GnT + o };
+            MItL_add(&g_ml, it);
+        }
+    }
+    if (bk_nt_scale(si, ne, H) || bk_transpose_f64(g_ml.v, g_ml.n)) goto out;
+    if (blk_gemm(S, ns, a, 0, 1, 0, 1.0, 0.0, OFF(Gn), OFF(Gn), OFF(Wn)) || blk_sym(S, ns, a, OFF(Wn))) goto out;
+    for (int q = 0; q < ns; q++) if (a[q]) { Solve *s = S[q]; s->SX = s->Wn; s->SZi = s->Wn; s->SLX = s->Gn; s->SLZi = s->GnT; }
+    rc = 0;
+out:
+    free(a); free(ei); free(si);
+    return rc;
+}
+/* the corrector's second-order term in the Nesterov-Todd frame (reference solver.c:9518-9524,
+ * 7990-8068): dXt = G^-1 dX G^-T, dZt = G' dZ G, R = sym(dXt dZt) with R(i, j) scaled by
+ * 2 / (lambda_i + lambda_j) (the inverse of the Lyapunov operator of diag(lambda)), Q = G R G' */
+static int nt_corr(Solve **S, int ns, const int *a) {
+    double *H = heapp();
+    if (blk_gemm(S, ns, a, 0, 0, 0, 1.0, 0.0, OFF(Gi), OFF(dX), OFF(W1)) || blk_gemm(S, ns, a, 0, 1, 0, 1.0, 0.0, OFF(W1), OFF(Gi), OFF(W2)) ||
+        blk_gemm(S, ns, a, 0, 0, 0, 1.0, 0.0, OFF(GnT), OFF(dZ), OFF(W1)) || blk_gemm(S, ns, a, 0, 0, 0, 1.0, 0.0, OFF(W1), OFF(Gn), OFF(F)) ||
+        blk_gemm(S, ns, a, 0, 0, 0, 1.0, 0.0, OFF(W2), OFF(F), OFF(Q)) || blk_sym(S, ns, a, OFF(Q))) return -1;
+    ScaleItem *si = NULL;
+    int ne = 0, cap = 0;
+    for (int q = 0; q < ns; q++) if (a[q]) {
+        Solve *s = S[q];
+        SAux *A = aux_of(s);
+        for (int k = 0; k < s->P.nb; k++) {
+            if (ne == cap) { cap = cap ? 2 * cap : 256; si = realloc(si, sizeof(ScaleItem) * cap); }
+            ScaleItem c = { s->P.bn[k], 2, s->Q + s->P.boff[k], s->lam + (int64_t)s->P.Ntot + 3 * A->lzoff[k] };
+            si[ne++] = c;
+        }
+    }
+    const int rc = bk_nt_scale(si, ne, H);
+    free(si);
+    if (rc) return -1;
+    if (blk_gemm(S, ns, a, 0, 0, 0, 1.0, 0.0, OFF(Gn), OFF(Q), OFF(W1)) || blk_gemm(S, ns, a, 0, 1, 0, 1.0, 0.0, OFF(W1), OFF(Gn), OFF(Q))) return -1;
+    return 0;
+}
+
 static int ipm_iterate_batch_(Solve **S, int ns) {
     int rc = -1;
     int *act = calloc(ns, sizeof(int)), *fail = calloc(ns, sizeof(int)), *okp = calloc(ns, sizeof(int)), *okd = calloc(ns, sizeof(int));
@@ -1478,8 +1588,9 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
     for (int q = 0; q < ns; q++) nsl += S[q]->P.nb;
     if (batch_scratch(ns, nsl + ns)) goto out;
 
-    /* Zi = LZi' LZi */
+    /* Zi = LZi' LZi; the Nesterov-Todd scaling where that direction is taken */
     if (form_zi(S, ns, act)) goto out;
+    if (nt_scaling(S, ns, act)) goto out;
     /* residuals and objectives (solver.c:14426-14482) */
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) if (act[q]) {
@@ -1589,7 +1700,7 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
         else lin_add(S[q]->P.m, S[q]->h, -1, 0, -1, 0, -1, 0, -1, 0);
     }
     if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
-    if (zi_v_x_(S, ns, hasH, OFF(dZ), OFF(X), OFF(G), 1)) goto out;
+    if (zi_v_x_(S, ns, hasH, OFF(dZ), OFF(SX), OFF(G), 1)) goto out;
     if (aop(S, ns, hasH, OFF(G), OFF(h), 1.0, 0.0)) goto out;
     /* predictor: M dy = b + h (tol 1e-7); dZ = theta R0 - A*(dy); dX = -X - sym(Zi dZ X) */
     LinL_clear(&g_ll);
@@ -1600,7 +1711,7 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
     if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
     if (schur_solve(S, ns, act, OFF(rhs), OFF(dy), 1e-7, rel)) goto out;
     if (aty(S, ns, act, OFF(dy), OFF(dZ), -1.0)) goto out;
-    if (zi_v_x(S, ns, act, OFF(dZ), OFF(X), OFF(G))) goto out;
+    if (zi_v_x(S, ns, act, OFF(dZ), OFF(SX), OFF(G))) goto out;
     if (blk_sym(S, ns, act, OFF(G))) goto out;
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.T, S[q]->dX, S[q]->G, -1.0, S[q]->X, -1.0, -1, 0, -1, 0);
@@ -1630,7 +1741,13 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
     }
     /* corrector (solver.c:14997-15055): Q = Zi dZ dX (W1 = Zi dZ still holds from the predictor
      * in the blocks where dZ is not diagonal); rhs = b - smu a0 + A(Q) + h */
-    if (zi_dz_dx(S, ns, act)) goto out;
+    {
+        int *hk = calloc(ns, sizeof(int)), *nt = calloc(ns, sizeof(int)), anyh = 0, anyn = 0;
+        for (int q = 0; q < ns; q++) if (act[q]) { if (S[q]->SX == S[q]->Wn && S[q]->nt) { nt[q] = 1; anyn = 1; } else { hk[q] = 1; anyh = 1; } }
+        const int e = (anyh && zi_dz_dx(S, ns, hk)) || (anyn && nt_corr(S, ns, nt));
+        free(hk); free(nt);
+        if (e) goto out;
+    }
     if (aop(S, ns, act, OFF(Q), OFF(qv), 1.0, 0.0)) goto out;
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) if (act[q]) {
@@ -1640,7 +1757,7 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
     if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
     if (schur_solve(S, ns, act, OFF(rhs), OFF(dy), 0.0, rel)) goto out;
     if (aty(S, ns, act, OFF(dy), OFF(dZ), -1.0)) goto out;
-    if (zi_v_x(S, ns, act, OFF(dZ), OFF(X), OFF(G))) goto out;
+    if (zi_v_x(S, ns, act, OFF(dZ), OFF(SX), OFF(G))) goto out;
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.T, S[q]->dX, S[q]->G, -1.0, S[q]->Q, -1.0, -1, 0, -1, 0);
     if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
@@ -1678,7 +1795,7 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
         }
         if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
         if (aty(S, ns, ract, OFF(wv), OFF(Q), 1.0)) goto out;
-        if (zi_v_x(S, ns, ract, OFF(Q), OFF(X), OFF(G))) goto out;
+        if (zi_v_x(S, ns, ract, OFF(Q), OFF(SX), OFF(G))) goto out;
         if (blk_sym(S, ns, ract, OFF(G))) goto out;
         if (aop(S, ns, ract, OFF(G), OFF(qv), -1.0, 1.0)) goto out;
         DotL_clear(&g_dl);
@@ -1765,6 +1882,7 @@ static int ipm_iterate_batch_(Solve **S, int ns) {
     for (int q = 0; q < ns; q++) if (act[q] && !S[q]->done) S[q]->it++;
     rc = 0;
 out:
+    for (int q = 0; q < ns; q++) { S[q]->SX = S[q]->X; S[q]->SZi = S[q]->Zi; S[q]->SLX = S[q]->LX; S[q]->SLZi = S[q]->LZi; }
     free(act); free(fail); free(okp); free(okd); free(hasH); free(ract); free(tact); free(rel); free(ap); free(ad); free(smu); free(gam);
     free(apr); free(adr); free(ne); free(nrp); free(stepped);
     return rc;

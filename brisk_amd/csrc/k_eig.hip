@@ -153,3 +153,41 @@ extern "C" int bk_syev_f64(const EigItem *it, int n_it, double *H) {
     }
     return 0;
 }
+
+// ---- scalings of the Nesterov-Todd frame --------------------------------------------------------
+// mode 0: A(i, j) *= u(j); 1: A(i, j) *= u(i); 2: A(i, j) *= 2 / (u(i) + u(j)); 3 (vector, n
+// entries of w at a): u(i) = sqrt(w(i)), u(n + i) = w(i)^(-1/4), u(2n + i) = w(i)^(1/4), NaN where
+// w(i) <= 0 or is not finite. One thread per entry over a flattened grid.
+__global__ __launch_bounds__(256) void k_nt_scale(const ScaleItem *__restrict__ items, const int *__restrict__ pre, int nit,
+                                                  double *__restrict__ H) {
+    const int q = flat_item(pre, nit, blockIdx.x);
+    const ScaleItem it = items[q];
+    const int64_t e = (int64_t)(blockIdx.x - pre[q]) * 256 + threadIdx.x, n = it.n;
+    if (it.mode == 3) {
+        if (e >= n) return;
+        const double w = H[it.a + e];
+        const bool ok = w > 0 && w < INFINITY;
+        const double r = ok ? sqrt(w) : NAN, r4 = ok ? sqrt(r) : NAN;
+        H[it.u + e] = r; H[it.u + n + e] = ok ? 1.0 / r4 : NAN; H[it.u + 2 * n + e] = r4;
+        return;
+    }
+    if (e >= n * n) return;
+    const int i = (int)(e % n), j = (int)(e / n);
+    double *a = H + it.a + e;
+    const double *u = H + it.u;
+    if (it.mode == 0) *a *= u[j];
+    else if (it.mode == 1) *a *= u[i];
+    else *a *= 2.0 / (u[i] + u[j]);
+}
+extern "C" int bk_nt_scale(const ScaleItem *it, int n_it, double *H) {
+    if (n_it <= 0) return 0;
+    std::vector<int64_t> cnt(n_it);
+    for (int q = 0; q < n_it; q++) cnt[q] = ((it[q].mode == 3 ? (int64_t)it[q].n : (int64_t)it[q].n * it[q].n) + 255) / 256;
+    const ScaleItem *d;
+    const int *pre;
+    const int64_t nb = flat_stage_items(it, sizeof(ScaleItem) * n_it, cnt, (const void **)&d, &pre);
+    if (nb < 0) return -1;
+    if (nb == 0) return 0;
+    hipLaunchKernelGGL(k_nt_scale, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, d, pre, n_it, H);
+    return BK_LAUNCH_CHECK("k_nt_scale");
+}

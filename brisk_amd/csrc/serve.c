@@ -74,6 +74,8 @@ API int brisk_serve_submit(BriskServe *sv, const BriskProblem *p) {
     if (hprob_build(&s->P, p->m, p->nblk, p->bs, p->c, p->nnz, p->mat, p->blk, p->i, p->j, p->v)) { hprob_free(&s->P); free(s); return -2; }
     if (s->o.hsd < 0) s->o.hsd = hsd_first_rule(&s->P, &s->o);
     s->P.want_hsd = s->o.hsd;
+    if (s->o.dir < 0) s->o.dir = nt_rule(&s->P);
+    s->P.want_nt = !s->o.hsd && s->o.dir == 1;
     if (sv->nreq == sv->capreq) { sv->capreq = sv->capreq ? 2 * sv->capreq : 256; sv->req = realloc(sv->req, sizeof(Req) * sv->capreq); }
     const int id = sv->nreq++;
     Req *r = &sv->req[id];

@@ -84,6 +84,7 @@ typedef struct {
     double *Ad;
     size_t ad_cap;          /* doubles allocated at Ad */
     int want_hsd;           /* the solve will run the self-dual embedding (its arrays are counted) */
+    int want_nt;            /* the solve will take the Nesterov-Todd direction (its arrays are counted) */
 } HProb;
 
 enum { PREC_F64 = 0, PREC_MIXED = 1 };
@@ -100,6 +101,7 @@ typedef struct {
     double hsd_first_c, hsd_first_asm;   /* the rule's constants: dense work c sum n^3 against m^3/3, or against asm (m^3/3 + the assembly) */
     int apply;              /* solves with the Schur factor: -1 a cost model on the solve's own order and precision (default), 0 explicit inverse, 1 block triangular solves */
     int tput;               /* 1: the plan choices for throughput in large batches (256-column panels from order 1024, blocked solves for f64 factors from order 1500); 0: for latency */
+    int dir;                /* direction of the infeasible-start method: 0 HKM, 1 Nesterov-Todd, -1 chosen per problem (nt_rule) */
     int polish;             /* 1: the FP64 solves' returned point polished (projected onto A(X) = b) when that improves it (reference Params.polish, -nopolish) */
 } SolveOpts;
 
@@ -119,6 +121,10 @@ typedef struct Solve {
     int64_t pairs;                  /* the constraint pairs sharing a block (bytes) */
     int64_t Rd, CX, dX0, dZ0, dX1, dZ1, hv, hg, ht, hy0, hy1;   /* self-dual embedding: residual, Zi C X, the direction family (base, difference), vectors */
     int64_t Qp, dX0s, dZ0s, hy0s;   /* self-dual embedding: the last second-order term that stood, the first family's base */
+    int64_t SX, SZi, SLX, SLZi;     /* the scaling pair of the Schur complement and the direction's products: (X, Zi) and
+                                       their factors for HKM, (W, W) and (G, G') for the Nesterov-Todd direction */
+    int nt;                         /* the solve takes the Nesterov-Todd direction */
+    int64_t Wn, Gn, GnT, Gi, ntV, ntA, lam;   /* Nesterov-Todd: W = G G', G, G', G^-1, eigenvectors, L_X' Z L_X, the eigenvalue vectors (4 per block order) */
     int64_t rhs2, rhs3, xvec, xom;  /* self-dual embedding: right-hand sides and the PCG vectors (8 m each) and operator scratch (2 T each) of two more systems */
     int hoc_nfail;          /* self-dual embedding: iterations whose second-order passes fell behind the first family */
     int nretry;             /* step-length factorizations that failed (x0.8 retries), all iterations */
@@ -153,6 +159,7 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
 void hprob_free(HProb *P);
 int64_t solve_heap_bytes(const HProb *P);   /* device memory a solve needs */
 int hsd_first_rule(const HProb *P, const SolveOpts *o);   /* the method of an automatic choice: 1 the embedding, 0 the infeasible start */
+int nt_rule(const HProb *P);                /* the direction of an automatic choice: 1 Nesterov-Todd, 0 HKM */
 
 /* heap allocator (host-managed, first fit) */
 int heap_setup(size_t bytes);

@@ -8,6 +8,7 @@ not).  Usage:
     python tests/golden/make_golden.py small          # -> tests/golden/small.json (+ sdpa/*.dat-s)
     python tests/golden/make_golden.py large c2 c4    # -> tests/golden/large.json (entries merged)
     python tests/golden/make_golden.py stream         # -> tests/golden/stream.json (serving requests)
+    python tests/golden/make_golden.py methods        # -> small.json: the infeasible-start method with NT / HKM
 Each record: rc, status, pobj, dobj, iterations, DIMACS errors, y (for m <= 2000), X (blocks
 concatenated, SDP column-major, when it has at most 10000 numbers), the options
 and the seconds the reference took on this container's host; under "dd" the same problem solved
@@ -161,6 +162,23 @@ def main():
             out[name]["request"] = k
             print(name, out[name]["m"], out[name]["status"], out[name]["pobj"], out[name]["seconds"], flush=True)
         dst = os.path.join(HERE, "stream.json")
+    elif which == "methods":
+        # the reference's infeasible-start method in either direction (-nohsd -dir nt | hkm) on the
+        # small set, added to small.json under "std_nt" / "std_hkm": status, objectives, iterations
+        dst = os.path.join(HERE, "small.json")
+        out = json.load(open(dst))
+        L = lib()
+        for key, rec in sorted(out.items()):
+            for d in ("nt", "hkm"):
+                opts = ["-threads", "8", "-nohsd", "-dir", d]
+                if "file" in rec:
+                    path = os.path.join(HERE, rec["file"])
+                    r = record(lambda n, o, r_: L.brisk_solve_file(path.encode(), n, o, r_), opts)
+                else:
+                    inst = Instance(rec["generator"], *rec["args"])
+                    r = record(lambda n, o, r_: L.brisk_solve_data(*inst.data_args(), n, o, r_), opts)
+                rec["std_" + d] = {k: r[k] for k in ("status", "pobj", "dobj", "iterations", "dimacs", "options")}
+            print(key, rec["std_nt"]["iterations"], rec["std_hkm"]["iterations"], flush=True)
     else:
         dst = os.path.join(HERE, "large.json")
         out = json.load(open(dst)) if os.path.exists(dst) else {}

@@ -144,6 +144,58 @@ def test_hsd_first_rule(name, hsd):
     assert r.pobj == f.pobj and r.dobj == f.dobj
 
 
+@pytest.mark.parametrize("prec", ["double", "mixed"])
+@pytest.mark.parametrize("name", GEN + FILES)
+def test_nt_direction_vs_reference(name, prec):
+    """The infeasible-start method with the Nesterov-Todd direction (-dir nt: the scaling W = G G'
+    from the eigendecomposition of L_X' Z L_X, the Schur complement at (W, W), the corrector in the
+    frame of lambda) against the reference's outputs."""
+    rec = SMALL[name]
+    print("case", name, prec, "nt", flush=True)
+    opts = ["-prec", prec, "-hsd", "0", "-dir", "nt"]
+    if "file" in rec:
+        r = brisk_amd.solve_file(os.path.join(ROOT, "tests", "golden", rec["file"]), options=opts)
+    else:
+        a = golden_instance(rec).arrays()
+        r = brisk_amd.solve_sdpa(a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"], options=opts)
+    check(r, rec)
+    # the reference's own infeasible-start NT run (-nohsd -dir nt) takes the same number of
+    # iterations to within one
+    print("iterations", r.iterations, "reference -nohsd -dir nt", rec["std_nt"]["iterations"], flush=True)
+    assert abs(r.iterations - rec["std_nt"]["iterations"]) <= 1
+
+
+@pytest.mark.parametrize("name", GEN + FILES)
+def test_hkm_iterations_vs_reference(name):
+    """The infeasible-start method with the HKM direction (the default here) takes the iterations of
+    the reference's own run of that method and direction (-nohsd -dir hkm), to within one."""
+    rec = SMALL[name]
+    print("case", name, "hkm", flush=True)
+    opts = ["-prec", "double", "-hsd", "0", "-dir", "hkm"]
+    if "file" in rec:
+        r = brisk_amd.solve_file(os.path.join(ROOT, "tests", "golden", rec["file"]), options=opts)
+    else:
+        a = golden_instance(rec).arrays()
+        r = brisk_amd.solve_sdpa(a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"], options=opts)
+    print("gpu", r.status_str, r.iterations, "reference -nohsd -dir hkm", rec["std_hkm"]["status"], rec["std_hkm"]["iterations"], flush=True)
+    assert r.status == rec["std_hkm"]["status"] == 0
+    assert abs(r.iterations - rec["std_hkm"]["iterations"]) <= 1
+
+
+@pytest.mark.parametrize("name,d", [("theta_50_0.2_s0", "nt"), ("maxcut_200_0.05_s1", "hkm"), ("multi_6_30_40_300_s2", "nt"),
+                                    ("dense_40_120_s1", "nt")])
+def test_nt_rule(name, d):
+    """-dir auto (the reference's rule, solver.c:13582-13592) takes NT where 15 sum n^3 < 0.25 (m^3 / 3
+    + 5 sum n^3) or 15 sum n^3 <= 1e8, HKM elsewhere: bit for bit the run with that direction forced."""
+    rec = SMALL[name]
+    print("case", name, "dir auto", flush=True)
+    a = golden_instance(rec).arrays()
+    run = lambda *o: brisk_amd.solve_sdpa(a["bs"], a["c"], a["mat"], a["blk"], a["i"], a["j"], a["v"], options=["-hsd", "0"] + list(o))
+    r, f = run("-dir", "auto"), run("-dir", d)
+    print("auto", r.iterations, repr(r.pobj), "forced", d, f.iterations, repr(f.pobj), flush=True)
+    assert r.status == f.status == 0 and r.iterations == f.iterations and r.pobj == f.pobj and r.dobj == f.dobj
+
+
 def _polish_runs(name, *o):
     """The run of a golden instance with options o, and the same run with -nopolish."""
     rec = SMALL[name]
