@@ -1006,11 +1006,11 @@ static int schur_factor_(Solve **S, int ns, const int *act, int force64, int *fa
             const double m = s->P.m;
             const double t_inv = m * m * m / 3.0 / (prec ? 1.5e13 : 3.0e13) + 3e-4;
             const double t_sv = (prec ? 3 : 6) * 2.0 * ((s->P.m + 255) / 256) * 15e-6;
-            /* with -throughput 1 (the serving loop's default) an f64 factor of order 1500 or more
-             * takes the solves: in a large batch their launches are shared by every solve in flight
-             * while the inverses' flops add up (the serving stream: 374 -> 402 solves/s) */
-            const int model = prec ? (s->o.tput ? s->P.m >= 1500 : t_inv > 3e-3 && t_sv < t_inv / 2)
-                                   : t_inv > 15e-3 && t_sv < t_inv / 3;
+            /* with -throughput 1 (the serving loop's default) every factor takes the solves: in a
+             * large batch their launches are shared by every solve in flight while the inverses'
+             * flops add up (the serving stream, 1024 requests: 411 solves/s with the f64 factors
+             * of order 1500 or more on the solves, 439 with all of them, 371 with none) */
+            const int model = s->o.tput ? 1 : prec ? t_inv > 3e-3 && t_sv < t_inv / 2 : t_inv > 15e-3 && t_sv < t_inv / 3;
             s->trsv = s->o.apply < 0 ? model : s->o.apply;
         }
         bk_zero(dint(), sizeof(int) * ns);
