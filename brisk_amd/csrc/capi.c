@@ -117,7 +117,7 @@ API int brisk_result_trace(const BriskResult *r, double *po, double *dob, double
 
 /* ---- options ---- */
 int brisk_parse_options(int nopt, const char *const *opts, SolveOpts *o, int m, int *slots, double *heapgb) {
-    o->tol = 1e-8; o->maxit = 100; o->verbose = 0; o->mixed_until = 1e-6; o->time_limit = 0; o->hsd = -1; o->hsd_first = 0; o->hsd_first_c = 12.0; o->hsd_first_asm = 0.5; o->dir = 0; o->apply = -1; o->tput = 0; o->polish = 1;
+    o->tol = 1e-8; o->maxit = 100; o->verbose = 0; o->mixed_until = 1e-6; o->time_limit = 0; o->hsd = -1; o->hsd_first = 0; o->hsd_first_c = 12.0; o->hsd_first_asm = 0.5; o->dir = 0; o->apply = -1; o->tput = 0; o->polish = 1; o->polish_r = 1; o->polish_x = 1;
     o->prec = m >= 1000 ? PREC_MIXED : PREC_F64;
     for (int k = 0; k < nopt; k++) {
         const char *a = opts[k];
@@ -151,6 +151,8 @@ int brisk_parse_options(int nopt, const char *const *opts, SolveOpts *o, int m, 
         else if (!strcmp(a, "-hsdfirstc")) { NEED(); o->hsd_first_c = atof(nx); }
         else if (!strcmp(a, "-hsdfirstasm")) { NEED(); o->hsd_first_asm = atof(nx); }
         else if (!strcmp(a, "-nopolish")) o->polish = 0;
+        else if (!strcmp(a, "-nopolishr")) o->polish_r = 0;
+        else if (!strcmp(a, "-nopolishx")) o->polish_x = 0;
         else if (!strcmp(a, "-throughput")) { NEED(); o->tput = atoi(nx) > 0; }
         else if (!strcmp(a, "-schurapply")) {
             NEED();
@@ -264,7 +266,7 @@ static int solve_many(int n, const BriskProblem *pr, int nopt, const char *const
     {
         int *st0 = malloc(sizeof(int) * n);
         for (int q = 0; q < n; q++) st0[q] = S[q].status;
-        const int e = ipm_dimacs(SP, n) || ipm_polish(SP, n, st0);
+        const int e = ipm_dimacs(SP, n) || ipm_end_corrections(SP, n, st0) || ipm_polish(SP, n, st0);
         free(st0);
         if (e) { rc = -4; goto out; }
     }

@@ -2780,6 +2780,211 @@ int ipm_end_batch(Solve **S, int ns) {
  * (the reference skips its mixed-precision solves, whose end phase is its own); a self-dual
  * embedding's point is (X, y) / tau, as returned. st0: the runs' statuses before their returned points were
  * measured. */
+/* ---- the end phase's metric corrections (reference sqrt_restore, solver.c:7551-7655, and
+ * polish_xmetric, 7657-7760) ------------------------------------------------------------------ */
+#define RST_MARGIN 0.01
+#define RST_MAXN 512        /* the square-root metric's eigendecomposition on blocks up to this order */
+/* X <- X + a dX with dX = D W D or sym(X W Zi), W = A*(w) and M w = b - A(X) for the Schur
+ * complement M = A (D (x) D) A' or A (X (x) Zi) A', assembled (the routes of the iteration),
+ * factored in FP64 and solved to 1e-14:
+ *  - half = 1, the restoration of primal feasibility in the square-root metric: D = X^{1/2} by
+ *    the eigendecomposition (eigenvalues below 0 taken as 0), D = G G' with G = V diag(w^{1/4}),
+ *    so the dense route's P_i = G' A_i G. The step a (from 1, halved up to eight times) keeps
+ *    (1 - 0.01) I + a W positive definite: X^{1/2} (I + a W) X^{1/2} stays in the cone and no
+ *    eigenvalue shrinks by more than 0.01. Three passes, until the residual falls below 1e-6
+ *    of its first value.
+ *  - half = 0, the polish in the metric of X: Zi the inverse of the point's Z = C - A*(y) (the
+ *    primal half of an HKM step); a (halved up to six times) keeps X + a dX positive definite.
+ *    Two passes.
+ * One step per solve, the smallest over its blocks (the reference steps each block by its own).
+ * Solves whose Z has no factor (half = 0), whose Schur factorization fails or whose step never
+ * passes drop out with X as the last pass left it. X is corrected in place (the caller keeps a
+ * copy); W1, dX, dZ, F, G, R0, W2, LX, LZ, LXi, Zi, LZi and the Schur arrays are scratch. steps
+ * (NULL, or 3 per solve): the step of each pass taken, 0 for none. */
+static int metric_correct(Solve **S, int ns, const int *act0, int half, double *steps) {
+    int rc = -1;
+    int *act = calloc(ns, sizeof(int)), *ok = calloc(ns, sizeof(int)), *tst = calloc(ns, sizeof(int)), *pd = calloc(2 * ns, sizeof(int));
+    int *wd = calloc(ns, sizeof(int)), *a2 = calloc(2 * ns, sizeof(int));
+    double *nr0 = calloc(ns, sizeof(double)), *al = calloc(ns, sizeof(double)), *rel = calloc(ns, sizeof(double));
+    EigItem *ei = NULL;
+    ScaleItem *s4 = NULL, *s0 = NULL;
+    double *H = heapp();
+    for (int q = 0; q < ns; q++) { act[q] = act0[q]; wd[q] = S[q]->want_double; S[q]->want_double = 1; }
+    if (steps) for (int q = 0; q < 3 * ns; q++) steps[q] = 0;
+    if (!half) {
+        /* Zi = (C - A*(y))^-1 of the point (in W1 after the measure): LZ LZ' = Z, LZi = LZ^-1 */
+        const size_t fV[1] = { OFF(W1) }, fL[1] = { OFF(LZ) }, fD[1] = { OFF(DinvZ) }, fW[1] = { OFF(W2) };
+        if (chol_sides(S, ns, 1, act, fV, fL, fD, fW, pd)) goto out;
+        for (int q = 0; q < ns; q++) if (act[q] && !pd[q]) act[q] = 0;
+        const size_t fLi[1] = { OFF(LZi) }, fLiT[1] = { OFF(LZiT) };
+        if (inv_sides(S, ns, 1, act, fL, fD, fLi, fLiT, fW) || form_zi(S, ns, act)) goto out;
+    }
+    for (int pass = 0; pass < (half ? 3 : 2); pass++) {
+        int any = 0;
+        for (int q = 0; q < ns; q++) any |= act[q];
+        if (!any) break;
+        /* rhs = b - A(X) */
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.m, S[q]->rhs, S[q]->bvec, 1.0, -1, 0, -1, 0, -1, 0);
+        if (bk_lincomb(g_ll.v, g_ll.n) || aop(S, ns, act, OFF(X), OFF(rhs), -1.0, 1.0)) goto out;
+        DotL_clear(&g_dl);
+        for (int q = 0; q < ns; q++) if (act[q]) dot_add(S[q]->P.m, S[q]->rhs, S[q]->rhs, SC(q, 0));
+        if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
+        any = 0;
+        for (int q = 0; q < ns; q++) if (act[q]) {
+            const double nr = sqrt(HS(q, 0));
+            if (pass == 0) nr0[q] = nr; else if (nr <= 1e-6 * nr0[q]) { act[q] = 0; continue; }
+            any = 1;
+        }
+        if (!any) break;
+        if (half) {
+            /* X = V diag(w) V' (a copy of X in dZ; V into G, w into lzw), G <- V diag(max(w, 0)^{1/4}),
+             * D = G G' into dX, G' into R0; the scaling pair (D, D) with the factors (G, G') */
+            int ne = 0, cap = 0;
+            LinL_clear(&g_ll);
+            MItL_clear(&g_ml);
+            for (int q = 0; q < ns; q++) if (act[q]) {
+                Solve *s = S[q];
+                SAux *A = aux_of(s);
+                lin_add(s->P.T, s->dZ, s->X, 1.0, -1, 0, -1, 0, -1, 0);
+                for (int k = 0; k < s->P.nb; k++) {
+                    if (ne == cap) {
+                        cap = 2 * cap + 64;
+                        ei = realloc(ei, sizeof(EigItem) * cap); s4 = realloc(s4, sizeof(ScaleItem) * cap); s0 = realloc(s0, sizeof(ScaleItem) * cap);
+                    }
+                    const int n = s->P.bn[k];
+                    const int64_t o = s->P.boff[k], w = s->lzw + A->lzoff[k], u = s->lzw + (int64_t)s->P.Ntot + A->lzoff[k];
+                    EigItem e = { n, 0, s->dZ + o, s->G + o, w };
+                    ScaleItem c4 = { n, 4, w, u }, c0 = { n, 0, s->G + o, u };
+                    ei[ne] = e; s4[ne] = c4; s0[ne] = c0; ne++;
+                    MItem it = { n, 0, s->G + o, 0, 0, s->R0 + o };
+                    MItL_add(&g_ml, it);
+                }
+            }
+            if (bk_lincomb(g_ll.v, g_ll.n) || bk_syev_f64(ei, ne, H) || bk_nt_scale(s4, ne, H) || bk_nt_scale(s0, ne, H) ||
+                bk_transpose_f64(g_ml.v, g_ml.n) || blk_gemm(S, ns, act, 0, 1, 0, 1.0, 0.0, OFF(G), OFF(G), OFF(dX)) || blk_sym(S, ns, act, OFF(dX)))
+                goto out;
+            for (int q = 0; q < ns; q++) if (act[q]) { Solve *s = S[q]; s->SX = s->dX; s->SZi = s->dX; s->SLX = s->G; s->SLZi = s->R0; }
+        } else {
+            /* the factor of X as it stands (the dense route's P_i = LX' A_i LZi') */
+            const size_t fV[1] = { OFF(X) }, fL[1] = { OFF(LX) }, fD[1] = { OFF(DinvX) }, fW[1] = { OFF(W2) };
+            if (chol_sides(S, ns, 1, act, fV, fL, fD, fW, pd)) goto out;
+            for (int q = 0; q < ns; q++) if (act[q] && !pd[q]) act[q] = 0;
+            for (int q = 0; q < ns; q++) if (act[q]) { Solve *s = S[q]; s->SX = s->X; s->SZi = s->Zi; s->SLX = s->LX; s->SLZi = s->LZi; }
+        }
+        /* w = M^-1 (b - A(X)) into dy, W = A*(w) into F */
+        if (schur_assemble(S, ns, act, 0) || schur_factor(S, ns, act, 1, pd)) goto out;
+        for (int q = 0; q < ns; q++) if (act[q] && pd[q]) act[q] = 0;
+        if (schur_solve(S, ns, act, OFF(rhs), OFF(dy), 1e-14, rel)) goto out;
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) if (act[q]) lin_add(S[q]->P.T, S[q]->F, -1, 0, -1, 0, -1, 0, -1, 0);
+        if (bk_lincomb(g_ll.v, g_ll.n) || aty(S, ns, act, OFF(dy), OFF(F), 1.0)) goto out;
+        /* dX: D W D into dZ, or sym(Zi W X) (= sym(X W Zi)) into G */
+        const size_t fdx = half ? OFF(dZ) : OFF(G);
+        if (half) {
+            if (blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(F), OFF(dX), OFF(LXi)) || blk_gemm(S, ns, act, 0, 0, 0, 1.0, 0.0, OFF(dX), OFF(LXi), OFF(dZ)) ||
+                blk_sym(S, ns, act, OFF(dZ))) goto out;
+        } else if (zi_v_x(S, ns, act, OFF(F), OFF(X), OFF(G)) || blk_sym(S, ns, act, OFF(G))) goto out;
+        /* the step: (1 - margin) I + a W, or X + a dX, positive definite (Cholesky in LZ) */
+        for (int q = 0; q < ns; q++) { al[q] = 1.0; tst[q] = act[q]; ok[q] = 0; }
+        for (int h = 0; h < (half ? 8 : 6); h++) {
+            int nt = 0;
+            LinL_clear(&g_ll);
+            MItL_clear(&g_ml);
+            for (int q = 0; q < ns; q++) {
+                g_hdbl[q] = 0;
+                if (!tst[q]) continue;
+                Solve *s = S[q];
+                nt++;
+                if (half) {
+                    lin_add(s->P.T, s->LZ, s->F, al[q], -1, 0, -1, 0, -1, 0);
+                    g_hdbl[q] = 1.0 - RST_MARGIN;
+                    for (int k = 0; k < s->P.nb; k++) { MItem it = { s->P.bn[k], q, s->LZ + s->P.boff[k], 0, 0, 0 }; MItL_add(&g_ml, it); }
+                } else lin_add(s->P.T, s->LZ, s->X, 1.0, F_(s, fdx), al[q], -1, 0, -1, 0);
+            }
+            if (!nt) break;
+            if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
+            if (half && (bk_h2d(ddbl(), g_hdbl, sizeof(double) * ns) || bk_add_diag(g_ml.v, g_ml.n, g_bdbl / 8))) goto out;
+            const size_t fa[1] = { OFF(LZ) }, fd[1] = { OFF(DinvZ) }, fw[1] = { OFF(W2) };
+            if (chol_sides(S, ns, 1, tst, fa, fa, fd, fw, pd)) goto out;
+            for (int q = 0; q < ns; q++) if (tst[q]) { if (pd[q]) { ok[q] = 1; tst[q] = 0; } else al[q] *= 0.5; }
+        }
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) if (act[q]) {
+            if (S[q]->o.verbose)
+                solve_log("%s, pass %d: residual %.3e, solve residual %.1e, step %.3g%s\n", half ? "restoration (square-root metric)" : "polish (X metric)",
+                          pass, sqrt(HS(q, 0)), rel[q], ok[q] ? al[q] : 0.0, ok[q] ? "" : " (no step keeps X in the cone)");
+            if (!ok[q]) { act[q] = 0; continue; }
+            lin_add(S[q]->P.T, S[q]->X, S[q]->X, 1.0, F_(S[q], fdx), al[q], -1, 0, -1, 0);
+            if (steps) steps[3 * q + pass] = al[q];
+        }
+        if (g_ll.n && bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    }
+    rc = 0;
+out:
+    for (int q = 0; q < ns; q++) { Solve *s = S[q]; s->SX = s->X; s->SZi = s->Zi; s->SLX = s->LX; s->SLZi = s->LZi; s->want_double = wd[q]; }
+    free(act); free(ok); free(tst); free(pd); free(wd); free(a2); free(nr0); free(al); free(rel); free(ei); free(s4); free(s0);
+    return rc;
+}
+/* The end phase's metric corrections of points that missed the tolerance (reference
+ * solver.c:15574-15631): the restoration of primal feasibility in the square-root metric when
+ * the primal residual is at least a quarter of the acceptance error (a restoration moves X in
+ * proportion to that residual, so it cannot help an error that is the complementarity of a
+ * feasible point) and every block has order <= 512; the polish in the metric of X when no
+ * restoration was taken. Each is kept when it lowers the acceptance error; otherwise the point
+ * and its measure go back. */
+int ipm_end_corrections(Solve **S, int ns, const int *st0) {
+    int rc = -1;
+    int *cur = calloc(ns, sizeof(int)), *taken = calloc(ns, sizeof(int));
+    double *old = calloc((size_t)ns * 9, sizeof(double));
+    Solve **A = malloc(sizeof(Solve *) * (ns + 1));
+    for (int half = 1; half >= 0; half--) {
+        int na = 0;
+        for (int q = 0; q < ns; q++) {
+            Solve *s = S[q];
+            const double acc = acc_err(s->res_err);
+            int small = 1;
+            for (int k = 0; k < s->P.nb; k++) if (s->P.bn[k] > RST_MAXN) small = 0;
+            cur[q] = s->o.polish && (half ? s->o.polish_r : s->o.polish_x) && (st0[q] == 0 || st0[q] == 4 || st0[q] == 5) && !taken[q] && isfinite(acc) && acc > s->o.tol &&
+                     (half ? small && s->res_err[0] >= 0.25 * acc : 1);
+            na += cur[q];
+        }
+        if (!na) continue;
+        /* the point and its measure kept: X in Xb, its Z (W1) in Q */
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) if (cur[q]) {
+            Solve *s = S[q];
+            lin_add(s->P.T, s->Xb, s->X, 1.0, -1, 0, -1, 0, -1, 0);
+            lin_add(s->P.T, s->Q, s->W1, 1.0, -1, 0, -1, 0, -1, 0);
+            for (int e = 0; e < 6; e++) old[9 * q + e] = s->res_err[e];
+            old[9 * q + 6] = s->res_pobj; old[9 * q + 7] = s->res_dobj; old[9 * q + 8] = s->status;
+        }
+        if (bk_lincomb(g_ll.v, g_ll.n) || metric_correct(S, ns, cur, half, NULL)) goto out;
+        na = 0;
+        for (int q = 0; q < ns; q++) if (cur[q]) { S[q]->status = st0[q]; A[na++] = S[q]; }
+        if (dimacs_point(A, na)) goto out;
+        LinL_clear(&g_ll);
+        for (int q = 0; q < ns; q++) {
+            if (!cur[q]) continue;
+            Solve *s = S[q];
+            const int keep = acc_err(s->res_err) < acc_err(old + 9 * q);
+            if (s->o.verbose)
+                solve_log("%s: acceptance error %.3e -> %.3e (errors %.2e %.2e %.2e %.2e %.2e %.2e), %s\n",
+                          half ? "restoration (square-root metric)" : "polish (X metric)", acc_err(old + 9 * q), acc_err(s->res_err),
+                          s->res_err[0], s->res_err[1], s->res_err[2], s->res_err[3], s->res_err[4], s->res_err[5], keep ? "kept" : "not kept");
+            if (keep) { taken[q] = 1; continue; }
+            lin_add(s->P.T, s->X, s->Xb, 1.0, -1, 0, -1, 0, -1, 0);
+            lin_add(s->P.T, s->W1, s->Q, 1.0, -1, 0, -1, 0, -1, 0);
+            for (int e = 0; e < 6; e++) s->res_err[e] = old[9 * q + e];
+            s->res_pobj = old[9 * q + 6]; s->res_dobj = old[9 * q + 7]; s->status = (int)old[9 * q + 8];
+        }
+        if (g_ll.n && bk_lincomb(g_ll.v, g_ll.n)) goto out;
+    }
+    rc = 0;
+out:
+    free(cur); free(taken); free(old); free(A);
+    return rc;
+}
 int ipm_polish(Solve **S, int ns, const int *st0) {
     int rc = -1, na = 0;
     int *act = calloc(ns, sizeof(int)), *pol = calloc(ns, sizeof(int)), *fail = calloc(ns, sizeof(int));
@@ -2841,7 +3046,7 @@ out:
 static int ipm_end_batch_(Solve **S, int ns) {
     int *st0 = malloc(sizeof(int) * (ns + 1));
     for (int q = 0; q < ns; q++) st0[q] = S[q]->status;
-    const int e = ipm_dimacs(S, ns) || ipm_polish(S, ns, st0);
+    const int e = ipm_dimacs(S, ns) || ipm_end_corrections(S, ns, st0) || ipm_polish(S, ns, st0);
     free(st0);
     if (e) return -1;
     for (int q = 0; q < ns; q++) if (ipm_finish(S[q])) return -1;
@@ -2891,6 +3096,14 @@ int ipm_probe_lammin(Solve *s, int mode, double *out) {
     return lanczos_min(S, 1, NULL, 1, fLi, fLiT, fV, 0, 0.02, out);
 }
 /* the DIMACS errors of (X, y) in s->X, s->y (scaled), as for a returned point */
+/* the square-root-metric restoration (half 1, three passes) or the X-metric polish (half 0, two
+ * passes) of the point (X, y) as the end phase runs it; steps[3]: the step of each pass */
+int ipm_probe_correct(Solve *s, int half, double *steps) {
+    Solve *S[1] = { s };
+    const int one = 1;
+    s->status = 0;
+    return ipm_dimacs(S, 1) || metric_correct(S, 1, &one, half, steps) ? -1 : 0;
+}
 int ipm_probe_dimacs(Solve *s) {
     Solve *S[1] = { s };
     s->status = 0;

@@ -157,7 +157,8 @@ extern "C" int bk_syev_f64(const EigItem *it, int n_it, double *H) {
 // ---- scalings of the Nesterov-Todd frame --------------------------------------------------------
 // mode 0: A(i, j) *= u(j); 1: A(i, j) *= u(i); 2: A(i, j) *= 2 / (u(i) + u(j)); 3 (vector, n
 // entries of w at a): u(i) = sqrt(w(i)), u(n + i) = w(i)^(-1/4), u(2n + i) = w(i)^(1/4), NaN where
-// w(i) <= 0 or is not finite. One thread per entry over a flattened grid.
+// w(i) <= 0 or is not finite; 4 (vector): u(i) = w(i)^(1/4), 0 where w(i) <= 0. One thread per entry
+// over a flattened grid.
 __global__ __launch_bounds__(256) void k_nt_scale(const ScaleItem *__restrict__ items, const int *__restrict__ pre, int nit,
                                                   double *__restrict__ H) {
     const int q = flat_item(pre, nit, blockIdx.x);
@@ -171,6 +172,12 @@ __global__ __launch_bounds__(256) void k_nt_scale(const ScaleItem *__restrict__ 
         H[it.u + e] = r; H[it.u + n + e] = ok ? 1.0 / r4 : NAN; H[it.u + 2 * n + e] = r4;
         return;
     }
+    if (it.mode == 4) {
+        if (e >= n) return;
+        const double w = H[it.a + e];
+        H[it.u + e] = w > 0 && w < INFINITY ? sqrt(sqrt(w)) : 0.0;
+        return;
+    }
     if (e >= n * n) return;
     const int i = (int)(e % n), j = (int)(e / n);
     double *a = H + it.a + e;
@@ -182,7 +189,7 @@ __global__ __launch_bounds__(256) void k_nt_scale(const ScaleItem *__restrict__ 
 extern "C" int bk_nt_scale(const ScaleItem *it, int n_it, double *H) {
     if (n_it <= 0) return 0;
     std::vector<int64_t> cnt(n_it);
-    for (int q = 0; q < n_it; q++) cnt[q] = ((it[q].mode == 3 ? (int64_t)it[q].n : (int64_t)it[q].n * it[q].n) + 255) / 256;
+    for (int q = 0; q < n_it; q++) cnt[q] = ((it[q].mode >= 3 ? (int64_t)it[q].n : (int64_t)it[q].n * it[q].n) + 255) / 256;
     const ScaleItem *d;
     const int *pre;
     const int64_t nb = flat_stage_items(it, sizeof(ScaleItem) * n_it, cnt, (const void **)&d, &pre);

@@ -4,7 +4,7 @@
  * groups; solve_setup: the device heap layout), places caller data in the solve's arrays, runs
  * one kernel family of the iteration (ipm_probe_*) and returns the result on the data as read:
  * the Schur complement M_ij = <A_i, X A_j Z^-1>, the operators A(D) and A*(y), the Lanczos
- * bound of the step-length test, the DIMACS errors. The solver scales A_i by d_i = 1/||A_i||_F,
+ * bound of the step-length test, the DIMACS errors, the end phase's metric corrections. The solver scales A_i by d_i = 1/||A_i||_F,
  * X by 1/bs, y_i by 1/(cs d_i) (problem.c); the probes apply and undo those factors.
  *
  * Block data is packed as the result accessors return it: per input block, n x n column-major
@@ -23,6 +23,7 @@ int ipm_probe_aop(Solve *s);
 int ipm_probe_aty(Solve *s);
 int ipm_probe_lammin(Solve *s, int mode, double *out);
 int ipm_probe_dimacs(Solve *s);
+int ipm_probe_correct(Solve *s, int half, double *steps);
 
 static int64_t packed_len(const HProb *P) {
     int64_t t = 0;
@@ -149,6 +150,23 @@ API int brisk_probe_dimacs(const BriskProblem *p, const double *X, const double 
         obj[0] = s.res_pobj; obj[1] = s.res_dobj;
     }
     free(ys);
+    probe_close(&s);
+    return rc;
+}
+
+API int brisk_probe_correct(const BriskProblem *p, const double *X, const double *y, int half, double *Xout, double *steps) {
+    Solve s;
+    int rc = probe_open(&s, p);
+    if (rc) return rc;
+    double *ys = malloc(sizeof(double) * (s.P.m + 1)), *ar = malloc(sizeof(double) * (size_t)(s.P.T + 1));
+    for (int i = 0; i < s.P.m; i++) ys[i] = y[i] / (s.P.cs * s.P.d[i]);
+    if (upload_blocks(&s, s.X, X, 1.0 / s.P.bs) || bk_h2d((double *)bk_heap() + s.y, ys, sizeof(double) * s.P.m) || ipm_probe_correct(&s, half, steps) ||
+        bk_d2h(ar, (double *)bk_heap() + s.X, sizeof(double) * (size_t)s.P.T)) rc = -4;
+    else {
+        for (int64_t q = 0; q < s.P.T; q++) ar[q] *= s.P.bs;
+        from_arena(&s.P, ar, Xout);
+    }
+    free(ys); free(ar);
     probe_close(&s);
     return rc;
 }

@@ -103,6 +103,7 @@ typedef struct {
     int tput;               /* 1: the plan choices for throughput in large batches (256-column panels from order 1024, blocked solves for f64 factors from order 1500); 0: for latency */
     int dir;                /* direction of the infeasible-start method: 0 HKM, 1 Nesterov-Todd, -1 chosen per problem (nt_rule) */
     int polish;             /* 1: the FP64 solves' returned point polished (projected onto A(X) = b) when that improves it (reference Params.polish, -nopolish) */
+    int polish_r, polish_x; /* the square-root-metric restoration and the X-metric polish before it (reference Params.polish_r, polish_x: -nopolishr, -nopolishx) */
 } SolveOpts;
 
 typedef struct Solve {
@@ -176,6 +177,7 @@ int ipm_start_batch(Solve **S, int ns);
 int ipm_iterate_batch(Solve **S, int ns);   /* one iteration; sets done on solves that end */
 int ipm_iterate_hsd(Solve **S, int ns);     /* the same for the self-dual embedding */
 int ipm_dimacs(Solve **S, int ns);          /* DIMACS errors and status of the returned points (device) */
+int ipm_end_corrections(Solve **S, int ns, const int *st0);  /* after ipm_dimacs: the square-root-metric restoration and the X-metric polish of points that missed the tolerance */
 int ipm_polish(Solve **S, int ns, const int *st0);  /* after ipm_dimacs: the returned points polished where that helps (st0: the runs' statuses before it) */
 int ipm_end_batch(Solve **S, int ns);       /* DIMACS on the device, polish, status, results to the host */
 int ipm_finish(Solve *S);                   /* results of one solve to the host (after ipm_end_batch's DIMACS) */

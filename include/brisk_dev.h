@@ -259,7 +259,7 @@ int bk_syev_f64(const EigItem *it, int n_it, double *H);
 /* the Nesterov-Todd frame's scalings of n x n matrices at a (offsets in doubles from H): mode 0
  * A(i, j) *= u(j) (columns), 1 A(i, j) *= u(i) (rows), 2 A(i, j) *= 2 / (u(i) + u(j)); mode 3 on the
  * n eigenvalues w at a: u(i) = sqrt(w(i)), u(n + i) = w(i)^(-1/4), u(2n + i) = w(i)^(1/4) (NaN where
- * w(i) <= 0) */
+ * w(i) <= 0); mode 4: u(i) = w(i)^(1/4), 0 where w(i) <= 0 */
 typedef struct { int n, mode; int64_t a, u; } ScaleItem;
 int bk_nt_scale(const ScaleItem *it, int n_it, double *H);
 

@@ -31,6 +31,12 @@ int brisk_probe_lammin(const BriskProblem *p, const double *X, const double *dX,
 /* the six DIMACS errors and the objectives (obj[0] = <C,X>, obj[1] = b'y) of the point (X, y),
  * as the solve measures a returned point (reference ps_measure_xc, postsolve.c:3037-3112) */
 int brisk_probe_dimacs(const BriskProblem *p, const double *X, const double *y, double *err, double *obj);
+/* the end phase's metric correction of the point (X, y) into Xout (packed like X), with the step
+ * of each pass in steps[3] (0: none taken): half 1 the restoration of A(X) = b in the square-root
+ * metric, X <- X + a D A*(w) D with D = X^{1/2} (reference sqrt_restore, solver.c:7551-7655);
+ * half 0 the polish in the metric of X, X <- X + a sym(X A*(w) Z^-1) with Z = C - A*(y)
+ * (reference polish_xmetric, solver.c:7657-7760) */
+int brisk_probe_correct(const BriskProblem *p, const double *X, const double *y, int half, double *Xout, double *steps);
 /* the packed length of a problem's block arrays */
 int64_t brisk_probe_packed_len(const BriskProblem *p);
 

@@ -22,8 +22,10 @@
  *   catastrophe guard  solver.c:15143-15160, try_step solver.c:6393-6446, update 15185-15188
  *   DIMACS errors      postsolve.c:3037-3112, status rule main.c:2349-2359, brisk.h:187-189
  *
- * Deliberately absent (documented as out of scope): presolve, HSD embedding, NT direction,
- * correctors, polish/restoration and the double-double endgame.
+ *   metric corrections solver.c:7551-7760 (sqrt_restore, polish_xmetric: oracle_correct)
+ *
+ * Deliberately absent (documented as out of scope): presolve, the NT direction and the
+ * double-double endgame.
  */
 #include <math.h>
 #include <stdint.h>
@@ -902,6 +904,165 @@ int oracle_schur(const OracleData *Dd, const double *X, const double *Zi, double
     schur(&P, X, Zi, M);
     prob_free(&P);
     return 0;
+}
+
+/* ---- the end phase's metric corrections (reference sqrt_restore, solver.c:7551-7655, and
+ * polish_xmetric, solver.c:7657-7760) ---------------------------------------------------------- */
+/* A = V diag(w) V' by the row-cyclic Jacobi method (Golub and Van Loan, Matrix Computations,
+ * 4th ed., algorithm 8.5.3): sweeps over (p, q), p < q, each rotation zeroing A(p, q), until the
+ * off-diagonal norm is below 1e-15 of the Frobenius norm. A is destroyed. */
+static void jacobi_eig(int n, double *A, double *V, double *w) {
+    for (int j = 0; j < n; j++) for (int i = 0; i < n; i++) V[i + (size_t)j * n] = i == j;
+    double fro = 0;
+    for (size_t e = 0; e < (size_t)n * n; e++) fro += A[e] * A[e];
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0;
+        for (int j = 0; j < n; j++) for (int i = 0; i < n; i++) if (i != j) off += A[i + (size_t)j * n] * A[i + (size_t)j * n];
+        if (off <= 1e-30 * fro) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = A[p + (size_t)q * n];
+                if (apq == 0) continue;
+                const double tau = (A[q + (size_t)q * n] - A[p + (size_t)p * n]) / (2 * apq);
+                const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
+                const double c = 1 / sqrt(1 + t * t), s = t * c;
+                for (int k = 0; k < n; k++) {       /* A <- A J (columns p, q) */
+                    const double akp = A[k + (size_t)p * n], akq = A[k + (size_t)q * n];
+                    A[k + (size_t)p * n] = c * akp - s * akq; A[k + (size_t)q * n] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {       /* A <- J' A (rows p, q) */
+                    const double apk = A[p + (size_t)k * n], aqk = A[q + (size_t)k * n];
+                    A[p + (size_t)k * n] = c * apk - s * aqk; A[q + (size_t)k * n] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {       /* V <- V J */
+                    const double vkp = V[k + (size_t)p * n], vkq = V[k + (size_t)q * n];
+                    V[k + (size_t)p * n] = c * vkp - s * vkq; V[k + (size_t)q * n] = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < n; i++) w[i] = A[i + (size_t)i * n];
+}
+/* X <- X + a dX with M w = b - A(X), W = A*(w) (scaled problem) and
+ *   half = 1: dX = D W D, D = X^{1/2} (eigenvalues below 0 taken as 0), M = A (D (x) D) A';
+ *             a, from 1 and halved up to eight times, keeps (1 - 0.01) I + a W positive definite
+ *             in every block; at most three passes, ending when the residual falls to 1e-6 of
+ *             its first value;
+ *   half = 0: dX = sym(Zi W X), Zi = (C - A*(y))^-1, M = A (X (x) Zi) A'; a, halved up to six
+ *             times, keeps X + a dX positive definite; at most two passes.
+ * One step per pass for all blocks (the GPU path's rule; the reference steps each block by its
+ * own and takes the same steps when every block accepts a = 1). M is factored and solved by the
+ * oracle's restatement of the FP64 factor and PCG, to 1e-14. X and y on the data as read (X
+ * packed); Xout the corrected X, steps[pass] the step of each pass taken (0: none). Returns 0, or
+ * 3 when a pass found no factor of M, no step, or (half = 0) no factor of Z or X. */
+int oracle_correct(const OracleData *Dd, const double *Xin, const double *yin, int half, int maxpass, double *Xout, double *steps) {
+    OProb P;
+    int rc = prob_build(Dd, &P);
+    if (rc) return rc;
+    const int m = P.m;
+    const size_t T = P.tot;
+    int maxn = 1;
+    for (int k = 0; k < P.nb; k++) if (P.B[k].n > maxn) maxn = P.B[k].n;
+    const size_t nn = (size_t)maxn * maxn;
+    double *X = malloc(sizeof(double) * (T + 1)), *Dm = malloc(sizeof(double) * (T + 1)), *Zi = malloc(sizeof(double) * (T + 1));
+    double *W = malloc(sizeof(double) * (T + 1)), *dX = malloc(sizeof(double) * (T + 1)), *tmp = malloc(sizeof(double) * (T + 1));
+    double *w1 = malloc(sizeof(double) * (nn + 1)), *w2 = malloc(sizeof(double) * (nn + 1)), *ev = malloc(sizeof(double) * (maxn + 1));
+    double *r = malloc(sizeof(double) * (m + 1)), *wv = malloc(sizeof(double) * (m + 1)), *ys = malloc(sizeof(double) * (m + 1));
+    OSchur S = { m, malloc(sizeof(double) * (size_t)m * m + 8), malloc(sizeof(double) * (size_t)m * m + 8), malloc(sizeof(double) * (m + 1)), 0, 0, 0 };
+    for (size_t q = 0; q < T; q++) X[q] = Xin[q] / P.bs;
+    for (int i = 0; i < m; i++) ys[i] = yin[i] / (P.cs * P.d[i]);
+    for (int p = 0; p < maxpass; p++) steps[p] = 0;
+    rc = 3;
+    if (!half) {
+        /* Zi = (C - A*(y))^-1 per block */
+        ATy(&P, ys, Zi);
+        for (size_t q = 0; q < T; q++) Zi[q] = P.C[q] - Zi[q];
+        for (int k = 0; k < P.nb; k++) {
+            const OBlk *B = &P.B[k];
+            double *Zk = Zi + B->off;
+            if (B->lp) { for (int i = 0; i < B->n; i++) { if (!(Zk[i] > 0)) goto out; Zk[i] = 1 / Zk[i]; } continue; }
+            memcpy(w1, Zk, sizeof(double) * B->n * B->n);
+            if (chol(B->n, w1)) goto out;
+            inv_from_chol(B->n, w1, Zk);
+        }
+    }
+    double nr0 = 0;
+    for (int pass = 0; pass < maxpass; pass++) {
+        memcpy(r, P.b, sizeof(double) * m);
+        Aop(&P, X, r, -1.0);
+        const double nr = sqrt(dot(m, r, r));
+        if (pass == 0) nr0 = nr; else if (nr <= 1e-6 * nr0) break;
+        if (half) {
+            for (int k = 0; k < P.nb; k++) {
+                const OBlk *B = &P.B[k];
+                const int n = B->n;
+                double *Dk = Dm + B->off;
+                if (B->lp) { for (int i = 0; i < n; i++) Dk[i] = X[B->off + i] > 0 ? sqrt(X[B->off + i]) : 0.0; continue; }
+                memcpy(w1, X + B->off, sizeof(double) * n * n);
+                jacobi_eig(n, w1, w2, ev);
+                for (int j = 0; j < n; j++) {
+                    const double f = ev[j] > 0 && ev[j] < INFINITY ? sqrt(sqrt(ev[j])) : 0.0;
+                    for (int i = 0; i < n; i++) w2[i + (size_t)j * n] *= f;
+                }
+                for (int j = 0; j < n; j++)
+                    for (int i = 0; i < n; i++) {
+                        double s_ = 0;
+                        for (int t = 0; t < n; t++) s_ += w2[i + (size_t)t * n] * w2[j + (size_t)t * n];
+                        Dk[i + (size_t)j * n] = s_;
+                    }
+                symmetrize(n, Dk);
+            }
+            schur(&P, Dm, Dm, S.M);
+        } else {
+            for (int k = 0; k < P.nb; k++) {
+                const OBlk *B = &P.B[k];
+                if (B->lp) { for (int i = 0; i < B->n; i++) if (!(X[B->off + i] > 0)) goto out; continue; }
+                memcpy(w1, X + B->off, sizeof(double) * B->n * B->n);
+                if (chol(B->n, w1)) goto out;
+            }
+            schur(&P, X, Zi, S.M);
+        }
+        if (sc_factor(&S, NULL)) goto out;
+        memset(wv, 0, sizeof(double) * m);
+        sc_solve(&S, r, wv, 1e-14);
+        ATy(&P, wv, W);
+        for (int k = 0; k < P.nb; k++) {
+            const OBlk *B = &P.B[k];
+            const int n = B->n;
+            const size_t o = B->off;
+            if (B->lp) {
+                for (int i = 0; i < n; i++) dX[o + i] = half ? Dm[o + i] * W[o + i] * Dm[o + i] : Zi[o + i] * W[o + i] * X[o + i];
+                continue;
+            }
+            if (half) { gemm_nn(n, W + o, Dm + o, w1); gemm_nn(n, Dm + o, w1, dX + o); }
+            else { gemm_nn(n, Zi + o, W + o, w1); gemm_nn(n, w1, X + o, dX + o); }
+            symmetrize(n, dX + o);
+        }
+        double a = 1;
+        int ok = 0;
+        for (int h = 0; h < (half ? 8 : 6) && !ok; h++) {
+            ok = 1;
+            for (int k = 0; k < P.nb && ok; k++) {
+                const OBlk *B = &P.B[k];
+                const int n = B->n;
+                const size_t o = B->off, len = B->lp ? (size_t)n : (size_t)n * n;
+                for (size_t e = 0; e < len; e++) tmp[o + e] = half ? a * W[o + e] : X[o + e] + a * dX[o + e];
+                if (half) for (int i = 0; i < n; i++) tmp[o + (B->lp ? (size_t)i : i + (size_t)i * n)] += 1.0 - 0.01;
+                if (B->lp) { for (int i = 0; i < n; i++) if (!(tmp[o + i] > 0)) ok = 0; }
+                else if (chol(n, tmp + o)) ok = 0;
+            }
+            if (!ok) a *= 0.5;
+        }
+        if (!ok) goto out;
+        for (size_t q = 0; q < T; q++) X[q] += a * dX[q];
+        steps[pass] = a;
+    }
+    rc = 0;
+out:
+    for (size_t q = 0; q < T; q++) Xout[q] = X[q] * P.bs;
+    free(X); free(Dm); free(Zi); free(W); free(dX); free(tmp); free(w1); free(w2); free(ev); free(r); free(wv); free(ys);
+    free(S.M); free(S.L); free(S.D);
+    prob_free(&P);
+    return rc;
 }
 
 /* ---- the homogeneous self-dual embedding (reference hsd_run, solver.c:9803-12211) ----------

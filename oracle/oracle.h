@@ -31,4 +31,5 @@ int oracle_schur(const OracleData *d, const double *X, const double *Zi, double 
 double oracle_lammin(int n, double *B);
 double oracle_lanczos(int n, const double *L, const double *D, int K, double tol, double *info3);
 double oracle_step_lambda(int n, const double *L, const double *D);
+int oracle_correct(const OracleData *d, const double *X, const double *y, int half, int maxpass, double *Xout, double *steps);
 #endif

@@ -48,6 +48,7 @@ def oracle_lib():
         _orc.oracle_lanczos.restype = D
         _orc.oracle_step_lambda.argtypes = [I, PD, PD]
         _orc.oracle_step_lambda.restype = D
+        _orc.oracle_correct.argtypes = [pO, PD, PD, I, I, PD, PD]
     return _orc
 
 
@@ -69,6 +70,19 @@ def oracle_solve(inst, tol=1e-8, maxit=100, method=0):
                trace_mu=list(r.trace_mu[:r.iters + 1]))
     lib.oracle_result_free(ctypes.byref(r))
     return out
+
+
+def oracle_correct(inst, X, y, half, maxpass):
+    """The oracle's end-phase metric correction of (X, y): (return code, corrected X, steps)."""
+    lib = oracle_lib()
+    d = oracle_data(inst)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros_like(X)
+    steps = np.zeros(maxpass)
+    rc = lib.oracle_correct(ctypes.byref(d), X.ctypes.data_as(PD), y.ctypes.data_as(PD), half, maxpass,
+                            out.ctypes.data_as(PD), steps.ctypes.data_as(PD))
+    return rc, out, steps
 
 
 def oracle_dimacs(inst, X, y):

@@ -2,14 +2,15 @@
 libbrisk_amd.so (include/brisk_probe.h), on random points of real problems: the Schur
 complement assembly of every route against the oracle's (pinned by brute force in
 test_oracle.py), the constraint operators A(D) and A*(y) against dense numpy, the Lanczos bound of
-the step-length test against eigenvalues, and the DIMACS errors (including the cone errors of
-non-PSD points) against the oracle's, whose cone errors are eigenvalues."""
+the step-length test against eigenvalues, the DIMACS errors (including the cone errors of
+non-PSD points) against the oracle's, whose cone errors are eigenvalues, and the end phase's
+metric corrections against the oracle's (pinned by their definitions in test_oracle.py)."""
 import ctypes
 import numpy as np
 import pytest
 import brisk_amd
 from _ctypes_util import (Instance, ArrInstance, oracle_lib, oracle_data, oracle_dimacs, oracle_solve, oracle_lanczos,
-                          oracle_step_lambda, PD)
+                          oracle_step_lambda, oracle_correct, PD)
 from _dense import dense_constraints, random_spd, random_sym, pack, unpack
 
 pytestmark = pytest.mark.gpu
@@ -29,6 +30,7 @@ def _lib():
     L.brisk_probe_aty.argtypes = [P, PD, PD]
     L.brisk_probe_lammin.argtypes = [P, PD, PD, ctypes.c_int, PD]
     L.brisk_probe_dimacs.argtypes = [P, PD, PD, PD, PD]
+    L.brisk_probe_correct.argtypes = [P, PD, PD, ctypes.c_int, PD, PD]
     return L
 
 
@@ -200,3 +202,65 @@ def test_dimacs_vs_oracle(case, shift):
         floor = 2e-12
         assert err[e] >= want[e] * (1 - 1e-9) - floor
         assert err[e] <= want[e] * (1 + 1.1e-4) + floor
+
+
+# every assembly route at the correction's scaling pairs: gathers (theta), Hadamard (max-cut),
+# blocks + LP with heavy groups (multi), the dense route with full factors (dense)
+CORRECT_CASES = [("theta", (9, 0.4, 1)), ("theta", (30, 0.3, 3)), ("maxcut", (60, 0.1, 2)), ("multi", (2, 6, 5, 12, 4)),
+                 ("multi", (4, 10, 20, 60, 0)), ("dense", (6, 10, 2)), ("dense", (12, 30, 0))]
+
+
+@pytest.mark.parametrize("point", [(1e-3, 1.0), (1.0, 30.0), (0.0, 1.001)], ids=["near", "far", "singular"])
+@pytest.mark.parametrize("case", CORRECT_CASES, ids=["%s%r" % c for c in CORRECT_CASES])
+def test_sqrt_restore_vs_oracle(case, point):
+    """The square-root-metric restoration (three passes) of X = scale X* + shift I at the oracle's
+    optimum X* (1.001 X*: the numerically singular X the restoration is for) through the GPU's
+    eigendecomposition, scalings, assembly, FP64 factor and PCG, against the oracle's: the same
+    steps, X within 1e-9 of |X| plus 1e-6 of the correction (near-zero eigenvalues of a
+    singular X enter D through their square roots)."""
+    inst = Instance(case[0], *case[1])
+    A, C, bs, lp, _ = dense_constraints(inst, scaled=False)
+    o = oracle_solve(inst)
+    assert o["status"] == 0
+    shift, scale = point
+    X = [scale * B + shift * np.eye(B.shape[0]) for B in unpack(o["X"], bs, lp)]
+    xv = pack(X, lp)
+    rc, want, wst = oracle_correct(inst, xv, o["y"], 1, 3)
+    assert rc == 0
+    xp, xpp = arr(xv)
+    yp, ypp = arr(o["y"])
+    got = np.zeros_like(xv)
+    st = np.zeros(3)
+    print("case", case, point, flush=True)
+    assert _lib().brisk_probe_correct(ctypes.byref(bp(inst)), xpp, ypp, 1, got.ctypes.data_as(PD), st.ctypes.data_as(PD)) == 0
+    print("steps gpu", st, "oracle", wst, "diff", np.abs(got - want).max(), "corr", np.abs(want - xv).max(), flush=True)
+    assert list(st) == list(wst)
+    assert np.abs(got - want).max() <= 1e-9 * np.abs(xv).max() + 1e-6 * np.abs(want - xv).max()
+
+
+@pytest.mark.parametrize("point", [(1e-3, 1.0), (0.1, 3.0)], ids=["near", "far"])
+@pytest.mark.parametrize("case", [("theta", (30, 0.3, 3)), ("maxcut", (60, 0.1, 2))], ids=["theta", "maxcut"])
+def test_xmetric_polish_vs_oracle(case, point):
+    """The X-metric polish (two passes) at y = y* - c with A*(c) = 0.05 I (Z = C - A*(y)
+    positive definite), against the oracle's: the same steps, X within 1e-9 of |X| plus 1e-7 of
+    the correction."""
+    inst = Instance(case[0], *case[1])
+    A, C, bs, lp, _ = dense_constraints(inst, scaled=False)
+    o = oracle_solve(inst)
+    shift, scale = point
+    X = [scale * B + shift * np.eye(B.shape[0]) for B in unpack(o["X"], bs, lp)]
+    basis = np.array([np.concatenate([Ai[k].ravel() for k in range(len(X))]) for Ai in A]).T
+    eye = np.concatenate([np.eye(n).ravel() for n in bs])
+    y = o["y"] - np.linalg.lstsq(basis, 0.05 * eye, rcond=None)[0]
+    xv = pack(X, lp)
+    rc, want, wst = oracle_correct(inst, xv, y, 0, 2)
+    assert rc == 0
+    xp, xpp = arr(xv)
+    yp, ypp = arr(y)
+    got = np.zeros_like(xv)
+    st = np.zeros(3)
+    print("case", case, point, flush=True)
+    assert _lib().brisk_probe_correct(ctypes.byref(bp(inst)), xpp, ypp, 0, got.ctypes.data_as(PD), st.ctypes.data_as(PD)) == 0
+    print("steps gpu", st, "oracle", wst, "diff", np.abs(got - want).max(), "corr", np.abs(want - xv).max(), flush=True)
+    assert list(st[:2]) == list(wst)
+    assert np.abs(got - want).max() <= 1e-9 * np.abs(xv).max() + 1e-7 * np.abs(want - xv).max()

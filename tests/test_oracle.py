@@ -1,15 +1,15 @@
 """The oracle pinned on the CPU: against the reference's own outputs (golden fixtures written
 by the reference library compiled from its sources), against closed-form optima, and against
 brute-force restatements of its building blocks (Schur complement, smallest eigenvalue, DIMACS
-errors) from their definitions."""
+errors, the end phase's metric corrections) from their definitions."""
 import math
 import numpy as np
 import pytest
 
-from _dense import dense_constraints, random_spd, pack  # noqa: F401
+from _dense import dense_constraints, random_spd, pack, unpack  # noqa: F401
 from _ctypes_util import oracle_lanczos, oracle_step_lambda  # noqa: E402
 from _ctypes_util import (Instance, ArrInstance, golden, golden_instance, oracle_solve, oracle_dimacs,
-                          oracle_lib, oracle_data, PD, D)
+                          oracle_lib, oracle_data, oracle_correct, PD, D)
 import ctypes
 
 GOLD = golden()
@@ -296,3 +296,91 @@ def test_oracle_lanczos_conservative(n):
         G = rng.standard_normal((m, m))
         D_ = (G + G.T) / 2
         assert abs(oracle_step_lambda(D_) - np.linalg.eigvalsh(D_)[0]) <= 1e-9 * (1 + np.abs(D_).max())
+
+
+# ---- the end phase's metric corrections -------------------------------------------------------
+
+def _sqrtm(B):
+    w, V = np.linalg.eigh(B)
+    return (V * np.sqrt(np.maximum(w, 0))) @ V.T
+
+
+def _correct_setup(case, shift, scale):
+    """(instance, dense A_i, C, orders, LP flags, b, the point X = scale X* + shift I, y = y*)"""
+    inst = Instance(case[0], *case[1])
+    A, C, bs, lp, _ = dense_constraints(inst, scaled=False)
+    b = np.array(inst.arrays()["c"], dtype=np.float64)
+    o = oracle_solve(inst)
+    assert o["status"] == 0
+    X = [scale * B + shift * np.eye(B.shape[0]) for B in unpack(o["X"], bs, lp)]
+    return inst, A, C, bs, lp, b, X, o["y"]
+
+
+def _aop(A, X):
+    return np.array([sum(float(np.sum(Ai[k] * X[k])) for k in range(len(X))) for Ai in A])
+
+
+CORRECT_CASES = [("theta", (9, 0.4, 1)), ("maxcut", (10, 0.3, 2)), ("multi", (2, 6, 5, 12, 4)), ("dense", (6, 10, 2))]
+
+
+@pytest.mark.parametrize("point", [(1e-3, 1.0), (0.1, 3.0), (1.0, 30.0)])
+@pytest.mark.parametrize("case", CORRECT_CASES, ids=["%s%r" % c for c in CORRECT_CASES])
+def test_oracle_sqrt_restore_definition(case, point):
+    """One pass of the square-root-metric restoration against what defines it: with D = X^{1/2}
+    (numpy's eigendecomposition) and a the step taken, dX = (X' - X) / a satisfies A(dX) = b - A(X)
+    (so A(X') - b = (1 - a)(A(X) - b)) and D^-1 dX D^-1 lies in the span of the A_i; a is the
+    largest of 1, 1/2, ... for which 0.99 I + a D^-1 dX D^-1 is positive definite in every block.
+    Those conditions fix dX and a uniquely (A (D (x) D) A' is positive definite)."""
+    inst, A, C, bs, lp, b, X, y = _correct_setup(case, *point)
+    rc, xo, steps = oracle_correct(inst, pack(X, lp), y, 1, 1)
+    assert rc == 0 and steps[0] > 0
+    a = steps[0]
+    Xo = unpack(xo, bs, lp)
+    r0 = b - _aop(A, X)
+    dX = [(Xo[k] - X[k]) / a for k in range(len(X))]
+    assert np.linalg.norm(_aop(A, dX) - r0) <= 1e-9 * np.linalg.norm(r0)
+    Dh = [_sqrtm(B) for B in X]
+    E = [np.linalg.solve(Dh[k], np.linalg.solve(Dh[k], dX[k]).T).T for k in range(len(X))]
+    basis = np.array([np.concatenate([Ai[k].ravel() for k in range(len(X))]) for Ai in A]).T
+    e = np.concatenate([B.ravel() for B in E])
+    coef = np.linalg.lstsq(basis, e, rcond=None)[0]
+    assert np.linalg.norm(basis @ coef - e) <= 1e-8 * np.linalg.norm(e)
+    lam = lambda t: min(np.linalg.eigvalsh(0.99 * np.eye(B.shape[0]) + t * B)[0] for B in E)
+    assert lam(a) > 0
+    assert a == 1.0 or lam(2 * a) <= 1e-12
+    assert a in [0.5 ** h for h in range(8)]
+
+
+@pytest.mark.parametrize("point", [(1e-3, 1.0), (0.1, 3.0)])
+@pytest.mark.parametrize("case", [("theta", (9, 0.4, 1)), ("maxcut", (10, 0.3, 2))], ids=["theta", "maxcut"])
+def test_oracle_xmetric_polish_definition(case, point):
+    """One pass of the X-metric polish against what defines it: at y = y* - c (A*(c) = 0.05 I, so
+    Z = C - A*(y) is positive definite), dX = (X' - X) / a satisfies A(dX) = b - A(X) and is
+    sym(Zi W X) for some W in the span of the A_i; a is the largest of 1, 1/2, ... keeping X + a dX
+    positive definite."""
+    inst, A, C, bs, lp, b, X, y = _correct_setup(case, *point)
+    basis = np.array([np.concatenate([Ai[k].ravel() for k in range(len(X))]) for Ai in A]).T
+    eye = np.concatenate([np.eye(n).ravel() for n in bs])
+    c = np.linalg.lstsq(basis, 0.05 * eye, rcond=None)[0]
+    assert np.linalg.norm(basis @ c - 0.05 * eye) <= 1e-10
+    y = y - c
+    Z = [C[k] - sum(y[i] * A[i][k] for i in range(len(A))) for k in range(len(X))]
+    assert min(np.linalg.eigvalsh(B)[0] for B in Z) > 0
+    rc, xo, steps = oracle_correct(inst, pack(X, lp), y, 0, 1)
+    assert rc == 0 and steps[0] > 0
+    a = steps[0]
+    Xo = unpack(xo, bs, lp)
+    r0 = b - _aop(A, X)
+    dX = [(Xo[k] - X[k]) / a for k in range(len(X))]
+    assert np.linalg.norm(_aop(A, dX) - r0) <= 1e-9 * np.linalg.norm(r0)
+    Zi = [np.linalg.inv(B) for B in Z]
+    cols = []
+    for Ai in A:
+        G = [Zi[k] @ Ai[k] @ X[k] for k in range(len(X))]
+        cols.append(np.concatenate([((B + B.T) / 2).ravel() for B in G]))
+    cols = np.array(cols).T
+    e = np.concatenate([B.ravel() for B in dX])
+    coef = np.linalg.lstsq(cols, e, rcond=None)[0]
+    assert np.linalg.norm(cols @ coef - e) <= 1e-8 * np.linalg.norm(e)
+    lam = lambda t: min(np.linalg.eigvalsh(X[k] + t * dX[k])[0] for k in range(len(X)))
+    assert lam(a) > 0 and (a == 1.0 or lam(2 * a) <= 1e-12)
