@@ -1009,8 +1009,9 @@ extern "C" int bk_trsv_f64(const TrsvItem *it, int n_it, const double *A, double
 extern "C" int bk_trsv_f32(const TrsvItem *it, int n_it, const float *A, double *V) { return trsv_run<float>(it, n_it, A, V); }
 
 // ---- batched matrix-vector products -------------------------------------------------------
-// trans = 1: y_j = alpha sum_i A[i, j] x_i + beta y_j, one wave per output (column of A read
-// contiguously), fixed-order lane partials + butterfly: deterministic.
+// trans = 1: y_j = alpha sum_i A[i, j] x_i + beta y_j, one to four waves per output (column of A
+// read contiguously, split in segments), fixed-order lane partials + butterfly, segments summed
+// in order: deterministic.
 // trans = 0: y_i = alpha sum_j A[i, j] x_j + beta y_i, one thread per row (workgroups of one
 // wave), GVN_U partial sums over the columns (rows read coalesced across the wave).
 // loads in flight per lane (gemv_t: columns of a few thousand rows are common, and 8 serve them
@@ -1028,46 +1029,69 @@ __device__ __forceinline__ T gv_sum(const T (&p)[U]) {
         for (int u = 0; u < h; u++) s[u] = s[u] + s[u + h];
     return s[0];
 }
+// waves of a workgroup per output of the transposed form: a column of a thousand rows or more is
+// split over 4 waves (each of its segments then takes one or two rounds of GVT_U loads per lane,
+// not several in sequence), of 512 or more over 2; by the item's own rows, so that a product is
+// summed the same way in any batch
+__host__ __device__ inline int gvt_wpc(int M) { return M >= 1024 ? 4 : M >= 512 ? 2 : 1; }
 template <typename TA, typename TX, typename TY, typename TACC>
 __global__ __launch_bounds__(256) void k_gemv_t(const GemvItem *__restrict__ items, const int *__restrict__ pre, int nit,
                                                 const TA *__restrict__ A, const TX *__restrict__ x, TY *__restrict__ y, TACC alpha,
                                                 TACC beta) {
     const int q = flat_item(pre, nit, blockIdx.x);
     const GemvItem it = items[q];
-    const int lane = threadIdx.x & 63;
-    const int j = (blockIdx.x - pre[q]) * 4 + (threadIdx.x >> 6);
-    if (j >= it.N) return;
-    const TA *col = A + it.a + (int64_t)j * it.lda;
-    const TX *xv = x + it.x;
-    const int i0 = it.tri == 1 ? j : 0, i1 = it.tri == 2 ? j + 1 : it.M;     // triangular A: only its nonzero rows
-    // GVT_U loads of the column in flight per lane (independent partial sums, added in a fixed
-    // order): a long column streams at HBM speed instead of at the latency of a few loads
-    TACC p[GVT_U];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wpc = gvt_wpc(it.M), seg = wave % wpc;                      // uniform in the workgroup
+    const int j = (blockIdx.x - pre[q]) * (4 / wpc) + wave / wpc;
+    __shared__ TACC part[4];
+    const bool live = j < it.N;
+    TACC s = 0;
+    if (live) {
+        const TA *col = A + it.a + (int64_t)j * it.lda;
+        const TX *xv = x + it.x;
+        const int r0 = it.tri == 1 ? j : 0, r1 = it.tri == 2 ? j + 1 : it.M;     // triangular A: only its nonzero rows
+        // this wave's segment of the rows (segments of a multiple of 64 rows)
+        const int per = (((r1 - r0) + wpc - 1) / wpc + 63) & ~63;
+        const int i0 = min(r1, r0 + seg * per), i1 = min(r1, i0 + per);
+        // GVT_U loads of the segment in flight per lane (independent partial sums, added in a
+        // fixed order)
+        TACC p[GVT_U];
 #pragma unroll
-    for (int u = 0; u < GVT_U; u++) p[u] = 0;
-    int i = i0 + lane;
-    for (; i + 64 * (GVT_U - 1) < i1; i += 64 * GVT_U) {
+        for (int u = 0; u < GVT_U; u++) p[u] = 0;
+        int i = i0 + lane;
+        for (; i + 64 * (GVT_U - 1) < i1; i += 64 * GVT_U) {
 #pragma unroll
-        for (int u = 0; u < GVT_U; u++) p[u] += (TACC)col[i + 64 * u] * (TACC)xv[i + 64 * u];
+            for (int u = 0; u < GVT_U; u++) p[u] += (TACC)col[i + 64 * u] * (TACC)xv[i + 64 * u];
+        }
+        if (i < i1) {       // the tail (< 64 GVT_U rows) as one batch, addresses clamped to i1 - 1 and masked
+            TA a[GVT_U];
+            TX b[GVT_U];
+#pragma unroll
+            for (int u = 0; u < GVT_U; u++) { const int ii = min(i + 64 * u, i1 - 1); a[u] = col[ii]; b[u] = xv[ii]; }
+#pragma unroll
+            for (int u = 0; u < GVT_U; u++) p[u] += i + 64 * u < i1 ? (TACC)a[u] * (TACC)b[u] : (TACC)0;
+        }
+        s = gv_sum<TACC, GVT_U>(p);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
     }
-    if (i < i1) {       // the tail (< 64 GVT_U rows) as one batch, addresses clamped to i1 - 1 and masked
-        TA a[GVT_U];
-        TX b[GVT_U];
-#pragma unroll
-        for (int u = 0; u < GVT_U; u++) { const int ii = min(i + 64 * u, i1 - 1); a[u] = col[ii]; b[u] = xv[ii]; }
-#pragma unroll
-        for (int u = 0; u < GVT_U; u++) p[u] += i + 64 * u < i1 ? (TACC)a[u] * (TACC)b[u] : (TACC)0;
+    if (wpc > 1) {          // the segments' sums in order, by the column's first wave
+        if (lane == 0) part[wave] = s;
+        __syncthreads();
+        if (seg == 0)
+            for (int u = 1; u < wpc; u++) s += part[wave + u];
     }
-    TACC s = gv_sum<TACC, GVT_U>(p);
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) {
+    if (live && seg == 0 && lane == 0) {
         TY *yp = y + it.y + j;
         TACC v = alpha * s;
         if (beta != (TACC)0) v += beta * (TACC)*yp;
         *yp = (TY)v;
     }
 }
+// the workgroups of the transposed form for an item, and its algorithmic bytes and flops (a
+// triangular matrix: its nonzero half)
+static inline int64_t gvt_blocks(const GemvItem &g) { const int cpb = 4 / gvt_wpc(g.M); return (g.N + cpb - 1) / cpb; }
+static inline double gv_elems(const GemvItem &g) { return g.tri ? 0.5 * (double)g.N * (g.N + 1) : (double)g.M * g.N; }
 template <typename TA, typename TX, typename TY, typename TACC>
 __global__ __launch_bounds__(64) void k_gemv_n(const GemvItem *__restrict__ items, const int *__restrict__ pre, int nit,
                                                 const TA *__restrict__ A, const TX *__restrict__ x, TY *__restrict__ y, TACC alpha,
@@ -1102,7 +1126,7 @@ template <typename TA, typename TX, typename TY, typename TACC>
 static int gemv_run(int trans, TACC alpha, TACC beta, const TA *A, const TX *x, TY *y, const GemvItem *it, int n_it) {
     if (n_it <= 0) return 0;
     std::vector<int64_t> cnt(n_it);
-    for (int q = 0; q < n_it; q++) cnt[q] = trans ? (it[q].N + 3) / 4 : (it[q].M + 63) / 64;
+    for (int q = 0; q < n_it; q++) cnt[q] = trans ? gvt_blocks(it[q]) : (it[q].M + 63) / 64;
     const GemvItem *d = nullptr;
     const int *pre = nullptr;
     const int64_t nb = flat_stage_items(it, sizeof(GemvItem) * n_it, cnt, (const void **)&d, &pre);
@@ -1114,7 +1138,7 @@ static int gemv_run(int trans, TACC alpha, TACC beta, const TA *A, const TX *x, 
     else hipLaunchKernelGGL((k_gemv_n<TA, TX, TY, TACC>), grid, dim3(64), 0, g_bk_stream, d, pre, n_it, A, x, y, alpha, beta);
     if (tok >= 0) {
         double fl = 0, by = 0;
-        for (int q = 0; q < n_it; q++) { fl += 2.0 * it[q].M * it[q].N; by += (double)sizeof(TA) * it[q].M * it[q].N + sizeof(TX) * (double)(trans ? it[q].M : it[q].N) + sizeof(TY) * (double)(trans ? it[q].N : it[q].M) * (beta != (TACC)0 ? 2 : 1); }
+        for (int q = 0; q < n_it; q++) { fl += 2.0 * gv_elems(it[q]); by += (double)sizeof(TA) * gv_elems(it[q]) + sizeof(TX) * (double)(trans ? it[q].M : it[q].N) + sizeof(TY) * (double)(trans ? it[q].N : it[q].M) * (beta != (TACC)0 ? 2 : 1); }
         bk_prof_end(tok, 9, fl, by);
     }
     return BK_LAUNCH_CHECK("k_gemv");
@@ -1137,7 +1161,7 @@ extern "C" int bk_gemv_chain_f64(double *H, const GemvItem *it, int n_it, int ns
         int64_t tot = 0;
         for (int q = 0; q < n_it; q++) {
             ((int *)(buf.data() + po[st]))[q] = (int)tot;
-            tot += (it[(size_t)st * n_it + q].N + 3) / 4;
+            tot += gvt_blocks(it[(size_t)st * n_it + q]);
         }
         ((int *)(buf.data() + po[st]))[n_it] = (int)tot;
         if (tot > 0x7fffffff) return -1;
@@ -1152,7 +1176,7 @@ extern "C" int bk_gemv_chain_f64(double *H, const GemvItem *it, int n_it, int ns
                            (const GemvItem *)(d + io[st]), (const int *)(d + po[st]), n_it, H, H, H, 1.0, 0.0);
         if (tok >= 0) {
             double fl = 0, by = 0;
-            for (int q = 0; q < n_it; q++) { const GemvItem &g = it[(size_t)st * n_it + q]; fl += 2.0 * g.M * g.N; by += 8.0 * g.M * g.N + 8.0 * (g.M + g.N); }
+            for (int q = 0; q < n_it; q++) { const GemvItem &g = it[(size_t)st * n_it + q]; fl += 2.0 * gv_elems(g); by += 8.0 * gv_elems(g) + 8.0 * (g.M + g.N); }
             bk_prof_end(tok, 9, fl, by);
         }
         if (BK_LAUNCH_CHECK("k_gemv")) return -1;

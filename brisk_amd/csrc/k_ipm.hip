@@ -1107,11 +1107,12 @@ __global__ __launch_bounds__(256) void k_lzm_fin(const LzItem *__restrict__ item
 }
 // ---- the same step in one workgroup per block, for blocks of order <= LZC_N ----------------------
 // w is staged in LDS; the coefficients of a pass come from 32 groups of 32 lanes (group p sums
-// q_p(i) w(i) over the rows i = lane, lane + 32, ... with four accumulators, then across its 32
-// lanes; fixed order), the updates from one thread per row. One launch replaces the four of the
+// q_p(i) w(i) over the rows i = lane, lane + 32, ... with LZC_U accumulators, then across its 32
+// lanes; fixed order), the updates from one thread per row, in the order q = 0, 1, ... One launch replaces the four of the
 // workgroup-slot form, and the basis is read from L2 by one CU (n (j + 1) doubles per pass).
 #define LZC_N 4096
 #define LZC_T 1024
+#define LZC_U 16          // loads in flight per lane in the coefficients' sums
 __global__ __launch_bounds__(LZC_T) void k_lzm_cgs(const LzItem *__restrict__ items, const int *__restrict__ sel, int j,
                                                   double *__restrict__ H) {
     const LzItem L = items[sel[blockIdx.x]];
@@ -1125,23 +1126,40 @@ __global__ __launch_bounds__(LZC_T) void k_lzm_cgs(const LzItem *__restrict__ it
     __syncthreads();
     for (int pass = 0; pass < 2; pass++) {
         if (p <= j) {
+            // LZC_U loads of the basis vector in flight per lane (LZC_U accumulators, summed pairwise)
             const double *qp = Q + (int64_t)p * n;
-            double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+            double a[LZC_U];
+#pragma unroll
+            for (int u = 0; u < LZC_U; u++) a[u] = 0;
             int i = l;
-            for (; i + 96 < n; i += 128) {
-                a0 += qp[i] * ws[i]; a1 += qp[i + 32] * ws[i + 32]; a2 += qp[i + 64] * ws[i + 64]; a3 += qp[i + 96] * ws[i + 96];
+            for (; i + 32 * (LZC_U - 1) < n; i += 32 * LZC_U) {
+#pragma unroll
+                for (int u = 0; u < LZC_U; u++) a[u] += qp[i + 32 * u] * ws[i + 32 * u];
             }
-            for (; i < n; i += 32) a0 += qp[i] * ws[i];
-            double c = (a0 + a1) + (a2 + a3);
+            for (; i < n; i += 32) a[0] += qp[i] * ws[i];
+#pragma unroll
+            for (int h = LZC_U / 2; h > 0; h /= 2)
+#pragma unroll
+                for (int u = 0; u < h; u++) a[u] += a[u + h];
+            double c = a[0];
 #pragma unroll
             for (int o = 16; o > 0; o >>= 1) c += __shfl_xor(c, o, 32);
             if (l == 0) cs[p] = c;
         }
         __syncthreads();
         if (pass == 0 && t == 0) H[L.ab + j] = cs[j];
+        // the update, the basis read eight vectors at a time (indices past j clamped to j, their
+        // terms not taken)
         for (int i = t; i < n; i += LZC_T) {
             double v = ws[i];
-            for (int q = 0; q <= j; q++) v -= cs[q] * Q[(int64_t)q * n + i];
+            for (int q0 = 0; q0 <= j; q0 += 8) {
+                double b[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) b[u] = Q[(int64_t)min(q0 + u, j) * n + i];
+#pragma unroll
+                for (int u = 0; u < 8; u++)
+                    if (q0 + u <= j) v -= cs[q0 + u] * b[u];
+            }
             ws[i] = v;
         }
         __syncthreads();
