@@ -137,6 +137,10 @@ __device__ int diag_factor_inverse(const T *Ab, int n, int nb, T *Lo, int ldo, T
         const int c = 8 * u + w;
         v[u] = (r < nb && c < nb) ? (r >= c ? Ab[r + (int64_t)c * n] : (T)0) : (r == c ? (T)1 : (T)0);
     }
+    // the panel tile's rows (PANEL), loaded now so that their latency overlaps the factorization
+    T pt[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) pt[u] = PANEL && r < prow ? P[r + (int64_t)(8 * u + w) * ldp] : (T)0;
     v4 pu[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};       // the tile's update Lp Lk' (PANEL, K > 0)
     if (K > 0) {
         // 64 columns of Lk at a time into Ls (Ls[k][r] = Lk(r, k)) and of Lp into Xs, every load
@@ -301,10 +305,7 @@ __device__ int diag_factor_inverse(const T *Ab, int n, int nb, T *Lo, int ldo, T
         // the tile into Ts (Ts[c][r] = P(r, c), rows past prow zero), less its update, then
         // P := P X' with wave w forming its two 16 x 16 output tiles
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
-            const int c = 8 * u + w;
-            Ts[c][r] = r < prow ? P[r + (int64_t)c * ldp] : (T)0;
-        }
+        for (int u = 0; u < 8; u++) Ts[8 * u + w][r] = pt[u];
         __syncthreads();
         if (K > 0) {
 #pragma unroll

@@ -11,6 +11,7 @@
  * elements at boff[k] of every block arena. */
 #include <math.h>
 #include <stdlib.h>
+#include <pthread.h>
 #include <string.h>
 #include "par.h"
 #include "solver.h"
@@ -31,18 +32,32 @@ typedef struct {
 /* the entries valid (indices within their blocks, LP entries diagonal, finite values); the
  * count of nonzero constraint entries; where the matrix number changes (the first m + 1 places,
  * nchg counting all of them) */
+/* an entry that names no matrix or block, lies outside its block, is off the diagonal of an LP
+ * block, or is not finite */
+static inline int entry_bad(int m, int nblk, const int *bs, int t, int blk1, int i, int j, double v) {
+    const int k = blk1 - 1;
+    if (t < 0 || t > m || k < 0 || k >= nblk) return 1;
+    const int n = abs(bs[k]);
+    if (i < 1 || j < 1 || i > n || j > n) return 1;
+    if (bs[k] < 0 && i != j) return 1;
+    return !isfinite(v);
+}
+int hprob_check(int m, int nblk, const int *bs, const double *c, int64_t nnz, const int *mat, const int *blk, const int *ii,
+                const int *jj, const double *v) {
+    if (m < 0 || nblk < 1 || !bs || (m > 0 && !c) || nnz < 0) return 2;
+    if (nnz > 0 && (!mat || !blk || !ii || !jj || !v)) return 2;
+    for (int k = 0; k < nblk; k++) if (bs[k] == 0) return 2;
+    for (int64_t q = 0; q < nnz; q++) if (entry_bad(m, nblk, bs, mat[q], blk[q], ii[q], jj[q], v[q])) return 2;
+    return 0;
+}
 static void check_range(void *c_, int64_t lo, int64_t hi, int part) {
     SetupCtx *c = c_;
     int bad = 0;
     int64_t ncon = 0, nchg = 0;
     int64_t *chg = malloc(sizeof(int64_t) * ((size_t)c->m + 2));
     for (int64_t q = lo; q < hi; q++) {
-        const int t = c->mat[q], k = c->blk[q] - 1;
-        if (t < 0 || t > c->m || k < 0 || k >= c->nblk) { bad = 1; break; }
-        const int n = abs(c->bs[k]);
-        if (c->ii[q] < 1 || c->jj[q] < 1 || c->ii[q] > n || c->jj[q] > n) { bad = 1; break; }
-        if (c->bs[k] < 0 && c->ii[q] != c->jj[q]) { bad = 1; break; }
-        if (!isfinite(c->v[q])) { bad = 1; break; }
+        if (entry_bad(c->m, c->nblk, c->bs, c->mat[q], c->blk[q], c->ii[q], c->jj[q], c->v[q])) { bad = 1; break; }
+        const int t = c->mat[q];
         if (c->v[q] != 0 && t > 0) ncon++;
         if (q > 0 && t != c->mat[q - 1]) {
             if (chg && nchg <= c->m) chg[nchg] = q;
@@ -55,26 +70,35 @@ static void check_range(void *c_, int64_t lo, int64_t hi, int part) {
 /* the dense route's host columns: the largest buffer seen (at most 32 GiB) is kept when its
  * problem is freed and handed to the next dense problem that fits in it, so that a repeated
  * solve of that size writes memory that is already mapped (the first touch of a fresh 2.5 GB
- * buffer costs more than building the columns). The library is called from one thread. */
+ * buffer costs more than building the columns). Problems are built on the serving loop's worker
+ * threads too: the kept buffer is taken and given under a lock. */
 #define AD_KEEP_MAX ((size_t)4 << 30)           /* doubles */
 static double *g_ad_keep;
 static size_t g_ad_keep_n;
+static pthread_mutex_t g_ad_lock = PTHREAD_MUTEX_INITIALIZER;
 static double *ad_take(size_t n, size_t *cap) {
+    pthread_mutex_lock(&g_ad_lock);
     if (g_ad_keep && g_ad_keep_n >= n) {
         double *p = g_ad_keep;
         *cap = g_ad_keep_n;
         g_ad_keep = NULL; g_ad_keep_n = 0;
+        pthread_mutex_unlock(&g_ad_lock);
         return p;
     }
+    pthread_mutex_unlock(&g_ad_lock);
     *cap = n;
     return malloc(sizeof(double) * n);
 }
 static void ad_give(double *p, size_t cap) {
     if (!p) return;
+    double *drop = p;
+    pthread_mutex_lock(&g_ad_lock);
     if (cap <= AD_KEEP_MAX && (!g_ad_keep || cap > g_ad_keep_n)) {
-        free(g_ad_keep);
+        drop = g_ad_keep;
         g_ad_keep = p; g_ad_keep_n = cap;
-    } else free(p);
+    }
+    pthread_mutex_unlock(&g_ad_lock);
+    free(drop);
 }
 /* 1 / ||A||_F of an n x n column of Ad, or 1 for a zero column */
 static double column_dinv(const double *A, int64_t nn) {
@@ -314,7 +338,7 @@ static int build_pairs(HProb *P) {
  * constraints costs 2 |A_light| loads instead of 4 |A_a| |A_b|. The heaviest groups are chosen
  * first, within a budget of device memory for their P and T arrays (the larger of 256 MB and
  * 16 m^2 bytes). */
-static const GrpRef *g_sort_grp;      /* the groups being ranked (set around the qsort) */
+static __thread const GrpRef *g_sort_grp;   /* the groups being ranked (set around the qsort; per thread) */
 static int cmp_grp_cnt(const void *a, const void *b) {
     const GrpRef *G = g_sort_grp;
     const int x = *(const int *)a, y = *(const int *)b;

@@ -155,6 +155,9 @@ typedef struct Solve {
 } Solve;
 
 /* problem.c */
+/* the checks of hprob_build on the data alone (0, or 2: bad data), without building anything */
+int hprob_check(int m, int nblk, const int *bs, const double *c, int64_t nnz, const int *mat, const int *blk, const int *ii,
+                const int *jj, const double *v);
 int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64_t nnz, const int *mat, const int *blk,
                 const int *ii, const int *jj, const double *v);
 void hprob_free(HProb *P);
