@@ -205,6 +205,27 @@ def profile_solve(a, prec, name):
                     traffic=measured_traffic(name, KCLASS[dom]), kernel=KCLASS[dom], launches=int(d["launches"]),
                     avg_us=round(avg_s * 1e6, 2), bytes_per_launch=d["bytes"] / max(d["launches"], 1),
                     share_of_kernel_time=round(d["ms"] / max(tot, 1e-9), 3))
+    if dom == 8:
+        roof["note"] = ("latency-bound: each launch is one 64-column step of the blocked Cholesky (the diagonal block "
+                        "factored and inverted in LDS by every workgroup, its panel tile multiplied on MFMA); the "
+                        "diagonal block's 64 pivots form a dependency chain (an in-kernel timer puts it at 13 of "
+                        "the step's 22 us), so the step's time does not shrink with its flops")
+    # the next classes by time, each against its own roofline
+    nxt = []
+    for c in sorted(counted, key=lambda c: -prof[c]["ms"]):
+        if c == dom or len(nxt) == 2:
+            continue
+        p = prof[c]
+        if c <= 8 and p["flops"] > 0:
+            pk = PEAK["f32" if c in F32_CLASSES else "f64"]
+            a_ = p["flops"] / max(p["ms"], 1e-9) / 1e9
+            nxt.append(dict(kernel=KCLASS[c], bound="mfma", achieved=round(a_, 3), peak=pk, unit="TFLOP/s", frac=round(a_ / pk, 4),
+                            share_of_kernel_time=round(p["ms"] / max(tot, 1e-9), 3)))
+        else:
+            a_ = p["bytes"] / max(p["ms"], 1e-9) / 1e6
+            nxt.append(dict(kernel=KCLASS[c], bound="hbm", achieved=round(a_, 1), peak=HBM_PEAK, unit="GB/s", frac=round(a_ / HBM_PEAK, 4),
+                            share_of_kernel_time=round(p["ms"] / max(tot, 1e-9), 3)))
+    roof["next_classes"] = nxt
     f64 = sum(prof[c]["flops"] for c in prof if c <= 3 or c == 8)
     f32 = sum(prof[c]["flops"] for c in F32_CLASSES)
     return roof, dict(flops_f64=f64, flops_f32=f32, kernel_ms=tot)

@@ -1136,7 +1136,14 @@ __global__ __launch_bounds__(LZC_T) void k_lzm_cgs(const LzItem *__restrict__ it
 #pragma unroll
                 for (int u = 0; u < LZC_U; u++) a[u] += qp[i + 32 * u] * ws[i + 32 * u];
             }
-            for (; i < n; i += 32) a[0] += qp[i] * ws[i];
+            if (i < n) {        // the tail (< 32 LZC_U rows) as one batch, indices clamped and masked
+                double b[LZC_U];
+#pragma unroll
+                for (int u = 0; u < LZC_U; u++) b[u] = qp[min(i + 32 * u, n - 1)];
+#pragma unroll
+                for (int u = 0; u < LZC_U; u++)
+                    if (i + 32 * u < n) a[u] += b[u] * ws[i + 32 * u];
+            }
 #pragma unroll
             for (int h = LZC_U / 2; h > 0; h /= 2)
 #pragma unroll
@@ -1148,18 +1155,10 @@ __global__ __launch_bounds__(LZC_T) void k_lzm_cgs(const LzItem *__restrict__ it
         }
         __syncthreads();
         if (pass == 0 && t == 0) H[L.ab + j] = cs[j];
-        // the update, the basis read eight vectors at a time (indices past j clamped to j, their
-        // terms not taken)
+        // the update, in the order q = 0, 1, ..., j
         for (int i = t; i < n; i += LZC_T) {
             double v = ws[i];
-            for (int q0 = 0; q0 <= j; q0 += 8) {
-                double b[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) b[u] = Q[(int64_t)min(q0 + u, j) * n + i];
-#pragma unroll
-                for (int u = 0; u < 8; u++)
-                    if (q0 + u <= j) v -= cs[q0 + u] * b[u];
-            }
+            for (int q = 0; q <= j; q++) v -= cs[q] * Q[(int64_t)q * n + i];
             ws[i] = v;
         }
         __syncthreads();
