@@ -27,6 +27,8 @@ struct BriskResult {
     int ntrace;
     double tr[3][128];
     char cause[160];
+    int n_resolves;             /* re-solves (none here: one method per solve) and their time */
+    double t_resolves;
 };
 
 static const char *g_status_str[] = { "OPTIMAL", "PRIMAL INFEASIBLE", "DUAL INFEASIBLE", "ITERATION LIMIT",
@@ -105,7 +107,10 @@ API const double *brisk_result_y(const BriskResult *r) { return r ? r->y : NULL;
 API const double *brisk_result_X(const BriskResult *r, int k) { return r && r->have_x && r->X && k >= 0 && k < r->nblk ? r->X[k] : NULL; }
 API const double *brisk_result_Z(const BriskResult *r, int k) { return r && r->Z && k >= 0 && k < r->nblk ? r->Z[k] : NULL; }
 API const char *brisk_result_cause(const BriskResult *r) { return r ? r->cause : ""; }
-API int brisk_result_resolves(const BriskResult *r, double *t) { (void)r; if (t) *t = 0; return 0; }
+/* re-solves of a result (reference capi.c:121: the method retries and fallbacks, brisk.h:122):
+ * the GPU path runs one method per solve and retries none, so the count and its time are 0 for
+ * every result, solved or not */
+API int brisk_result_resolves(const BriskResult *r, double *t) { if (t) *t = r ? r->t_resolves : 0; return r ? r->n_resolves : 0; }
 API void brisk_result_phases(const BriskResult *r, double *out) { for (int k = 0; k < 4; k++) out[k] = r ? r->phases[k] : 0; }
 API void brisk_result_counts(const BriskResult *r, int *out) { for (int k = 0; k < 3; k++) out[k] = r ? r->counts[k] : 0; }
 API int brisk_result_trace(const BriskResult *r, double *po, double *dob, double *mu, int n) {
