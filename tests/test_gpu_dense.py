@@ -105,9 +105,10 @@ def test_potrf_trtri_batched(dt, split, pw):
     """The blocked Cholesky (both forms of its 64-column step: the diagonal block and its panel in
     one launch, with the step's update from the panel's earlier columns, K = 64 .. 192, inside it,
     or in two launches after a GEMM) at both panel widths, and the triangular inverse of a ragged
-    batch against numpy."""
+    batch against numpy; orders 700 and 1030 take several panels' trailing updates at both
+    widths."""
     rng = np.random.default_rng(3)
-    sizes = [1, 5, 63, 64, 65, 130, 200, 300]
+    sizes = [1, 5, 63, 64, 65, 130, 200, 300, 700, 1030]
     mats = [_spd(n, rng, 1e2) for n in sizes]
     mats.append(-np.eye(7))                                # not positive definite: fails at column 1
     sizes.append(7)
