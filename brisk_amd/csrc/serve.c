@@ -34,6 +34,7 @@ typedef struct {
     int64_t nnz;
     int *bs, *mat, *blk, *ii, *jj;
     double *c, *v;
+    int fault;                  /* BRISK_FAULT=build at submission: the build fails (tests of that path) */
     int state;
 } Build;
 
@@ -75,6 +76,7 @@ static void build_free_data(Build *b) {
 /* the problem's host structures and the method and direction rules (as brisk_solve_data does) */
 static int build_run(Build *b) {
     Solve *s = b->s;
+    if (b->fault) return -1;
     if (hprob_build(&s->P, b->m, b->nblk, b->bs, b->c, b->nnz, b->mat, b->blk, b->ii, b->jj, b->v)) return -1;
     if (s->o.hsd < 0) s->o.hsd = hsd_first_rule(&s->P, &s->o);
     s->P.want_hsd = s->o.hsd;
@@ -156,6 +158,7 @@ API int brisk_serve_submit(BriskServe *sv, const BriskProblem *p) {
     Build *b = calloc(1, sizeof(Build));
     const size_t nz = (size_t)p->nnz;
     b->s = s; b->m = p->m; b->nblk = p->nblk; b->nnz = p->nnz;
+    b->fault = brisk_fault("build");    /* read here: the workers do not read the environment */
     b->bs = dup_mem(p->bs, sizeof(int) * p->nblk);
     b->c = dup_mem(p->c, sizeof(double) * p->m);
     b->mat = dup_mem(p->mat, sizeof(int) * nz); b->blk = dup_mem(p->blk, sizeof(int) * nz);

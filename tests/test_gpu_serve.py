@@ -171,3 +171,29 @@ def test_serving_setup_failure_ends_unsolved(monkeypatch):
     r, _ = sv.take(good)
     assert r.status == 0
     sv.close()
+
+
+def test_serving_bad_data_and_build_failure(monkeypatch):
+    """Submission checks the data itself (an entry outside its block is refused at once, as the
+    solve refuses it); a request whose host build fails on the worker threads (forced with
+    BRISK_FAULT=build) ends unsolved with its cause while the requests around it are solved."""
+    sv = brisk_amd.Server(["-prec", "double", "-slots", "4"])
+    bs, c, mat, blk, i, j, v = tup(Instance("theta", 30, 0.3, 3))
+    i_bad = np.array(i, copy=True)
+    i_bad[3] = 31
+    with pytest.raises(ValueError):
+        sv.submit(bs, c, mat, blk, i_bad, j, v)
+    first = sv.submit(*tup(Instance("maxcut", 100, 0.1, 0)))
+    monkeypatch.setenv("BRISK_FAULT", "build")
+    bad = sv.submit(*tup(Instance("theta", 30, 0.3, 3)))
+    monkeypatch.delenv("BRISK_FAULT")
+    last = sv.submit(*tup(Instance("theta", 50, 0.2, 0)))
+    for _ in range(400):
+        if sv.step() == 0:
+            break
+    r, _ = sv.take(bad)
+    assert r.status == -1 and "host structures" in r.cause
+    for rid in (first, last):
+        r, _ = sv.take(rid)
+        assert r.status == 0
+    sv.close()
