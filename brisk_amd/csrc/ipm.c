@@ -257,7 +257,17 @@ int solve_setup(Solve *S) {
     if (fail) { solve_release(S); return -1; }
     char *H = heapp();
     int rc = 0;
-    rc |= bk_h2d_async(H + S->C * 8, P->C, T * 8);
+    if (P->C) rc |= bk_h2d_async(H + S->C * 8, P->C, T * 8);
+    else {
+        /* C by its nonzeros: the arena zeroed, positions and values staged in W2 and W1 (free until
+         * the first iteration; nc <= T / 8), then scattered */
+        rc |= bk_zero(H + S->C * 8, (size_t)T * 8);
+        if (P->nc > 0) {
+            rc |= bk_h2d_async(H + S->W2 * 8, P->cpos, P->nc * 8);
+            rc |= bk_h2d_async(H + S->W1 * 8, P->cval, P->nc * 8);
+            rc |= bk_scatter_f64((const int64_t *)(H + S->W2 * 8), (const double *)(H + S->W1 * 8), P->nc, (double *)(H + S->C * 8));
+        }
+    }
     rc |= bk_h2d_async(H + S->bvec * 8, P->b, m * 8);
     rc |= bk_h2d_async(H + S->duv * 8, P->du, m * 8);
     rc |= bk_h2d_async(H + S->cptr, P->cptr, (m + 1) * 4);

@@ -140,6 +140,19 @@ __global__ __launch_bounds__(256) void k_lincomb(const LinItem *__restrict__ it,
         H[L.out + e] = v;
     }
 }
+// out[pos[i]] = val[i]: a sparse array into a zeroed dense one (C's nonzeros at setup)
+__global__ __launch_bounds__(256) void k_scatter(const int64_t *__restrict__ pos, const double *__restrict__ val, int64_t n,
+                                                 double *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[pos[i]] = val[i];
+}
+extern "C" int bk_scatter_f64(const int64_t *pos, const double *val, int64_t n, double *out) {
+    if (n <= 0) return 0;
+    const int64_t nb = (n + 255) / 256;
+    if (nb > 0x7fffffff) return bk_fail("scatter: too many entries");
+    hipLaunchKernelGGL(k_scatter, dim3((unsigned)nb), dim3(256), 0, g_bk_stream, pos, val, n, out);
+    return BK_LAUNCH_CHECK("k_scatter");
+}
 static int64_t lin_stage(const LinItem *it, int n_it, const LinItem **d, const int **pre) {
     std::vector<int64_t> cnt(n_it);
     for (int q = 0; q < n_it; q++) cnt[q] = flat_cap((it[q].len + 255) / 256, 2048);

@@ -155,7 +155,7 @@ static int cmp_pos(const void *a, const void *b) {
 
 void hprob_free(HProb *P) {
     free(P->bs_in); free(P->in_first); free(P->bn); free(P->boff);
-    free(P->b); free(P->b0); free(P->d); free(P->du); free(P->C);
+    free(P->b); free(P->b0); free(P->d); free(P->du); free(P->C); free(P->cpos); free(P->cval);
     free(P->cptr); free(P->ent); free(P->gptr); free(P->grp); free(P->pptr); free(P->ppos); free(P->pcon); free(P->pval);
     ad_give(P->Ad, P->ad_cap);
     free(P->bdiag); free(P->pairs);
@@ -385,6 +385,26 @@ static int build_groups(HProb *P) {
     return 0;
 }
 
+/* C's nonzeros sorted by position (stable: the entries of one position keep their order, so
+ * that their sum is the dense accumulation's) and merged; zero sums kept (as the dense arena
+ * keeps them) */
+typedef struct { int64_t pos, seq; double v; } CEnt;
+static int cmp_cent(const void *a_, const void *b_) {
+    const CEnt *a = a_, *b = b_;
+    return a->pos < b->pos ? -1 : a->pos > b->pos ? 1 : (a->seq < b->seq ? -1 : a->seq > b->seq);
+}
+static void c_merge(HProb *P) {
+    CEnt *e = malloc(sizeof(CEnt) * (P->nc + 1));
+    for (int64_t q = 0; q < P->nc; q++) { e[q].pos = P->cpos[q]; e[q].seq = q; e[q].v = P->cval[q]; }
+    qsort(e, P->nc, sizeof(CEnt), cmp_cent);
+    int64_t w = 0;
+    for (int64_t q = 0; q < P->nc; q++) {
+        if (w > 0 && P->cpos[w - 1] == e[q].pos) P->cval[w - 1] += e[q].v;
+        else { P->cpos[w] = e[q].pos; P->cval[w] = 0.0 + e[q].v; w++; }
+    }
+    P->nc = w;
+    free(e);
+}
 int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64_t nnz, const int *mat, const int *blk,
                 const int *ii, const int *jj, const double *v) {
     memset(P, 0, sizeof(*P));
@@ -414,7 +434,6 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
         }
     }
     P->T = off;
-    P->C = calloc(off + 1, sizeof(double));
     P->b = malloc(sizeof(double) * (m + 1)); P->b0 = malloc(sizeof(double) * (m + 1));
     P->d = malloc(sizeof(double) * (m + 1)); P->du = malloc(sizeof(double) * (m + 1));
     for (int i = 0; i < m; i++) P->b[i] = P->b0[i] = c[i];
@@ -428,10 +447,18 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
     int bad = 0;
     for (int p = 0; p < nparts; p++) { bad |= sc.bad[p]; ncon += sc.ncon[p]; }
     if (bad) { for (int p = 0; p < nparts; p++) free(sc.chg[p]); return 2; }
+    /* C: dense in the block arena, or by its nonzeros where they are few (their count with the
+     * mirrored off-diagonal entries at most an eighth of the arena; the dense route keeps it dense) */
+    int64_t ncin = 0;
+    for (int64_t q = 0; q < nnz; q++) if (mat[q] == 0 && v[q] != 0) ncin += ii[q] == jj[q] ? 1 : 2;
     /* dense route: one SDP block whose constraints fill at least a quarter of their upper
      * triangles (the reference's route 2, solver.c:2968-3016, taken for every constraint) */
-    if (nblk == 1 && bs[0] > 1 && m > 0 && (double)ncon >= 0.25 * (double)m * bs[0] * (bs[0] + 1) / 2.0 &&
-        (double)m * bs[0] * bs[0] * 8.0 < 64e9) {
+    const int dense_route = nblk == 1 && bs[0] > 1 && m > 0 && (double)ncon >= 0.25 * (double)m * bs[0] * (bs[0] + 1) / 2.0 &&
+                            (double)m * bs[0] * bs[0] * 8.0 < 64e9;
+    const int c_sparse = !dense_route && 8 * ncin <= off;
+    if (!c_sparse) P->C = calloc(off + 1, sizeof(double));
+    else { P->cpos = malloc(sizeof(int64_t) * (ncin + 1)); P->cval = malloc(sizeof(double) * (ncin + 1)); }
+    if (dense_route) {
         int64_t *rb = malloc(sizeof(int64_t) * (m + 1)), *re = malloc(sizeof(int64_t) * (m + 1));
         const int grouped = !matrix_ranges(&sc, nnz, rb, re);
         for (int p = 0; p < nparts; p++) free(sc.chg[p]);
@@ -452,12 +479,19 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
         else { k = P->in_first[k0] + i; r = 0; cc = 0; }
         if (t == 0) {
             const int n = P->bn[k];
-            P->C[P->boff[k] + r + (int64_t)cc * n] += -v[q];
-            if (r != cc) P->C[P->boff[k] + cc + (int64_t)r * n] += -v[q];
+            const int64_t p0 = P->boff[k] + r + (int64_t)cc * n, p1 = P->boff[k] + cc + (int64_t)r * n;
+            if (!c_sparse) {
+                P->C[p0] += -v[q];
+                if (r != cc) P->C[p1] += -v[q];
+            } else {
+                P->cpos[P->nc] = p0; P->cval[P->nc++] = -v[q];
+                if (r != cc) { P->cpos[P->nc] = p1; P->cval[P->nc++] = -v[q]; }
+            }
             continue;
         }
         tr[nt].con = t - 1; tr[nt].k = k; tr[nt].r = r; tr[nt].c = cc; tr[nt].v = v[q]; nt++;
     }
+    if (c_sparse) c_merge(P);
     qsort(tr, nt, sizeof(Trip), cmp_con);
     /* merge duplicate positions of one constraint */
     int64_t w = 0;
@@ -467,9 +501,10 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
     }
     nt = w;
     if (nt > 0x7fffffff) { free(tr); return 2; }
-    /* norms on the data as read */
+    /* norms on the data as read (the nonzeros in the arena's order: the same sums) */
     double c2 = 0, c1 = 0;
-    for (int64_t q = 0; q < P->T; q++) { c2 += P->C[q] * P->C[q]; c1 += fabs(P->C[q]); }
+    if (!c_sparse) for (int64_t q = 0; q < P->T; q++) { c2 += P->C[q] * P->C[q]; c1 += fabs(P->C[q]); }
+    else for (int64_t q = 0; q < P->nc; q++) { c2 += P->cval[q] * P->cval[q]; c1 += fabs(P->cval[q]); }
     P->normC2 = sqrt(c2); P->normC1 = c1;
     for (int i = 0; i < m; i++) { P->normb1 += fabs(P->b0[i]); P->normb2 += P->b0[i] * P->b0[i]; }
     P->normb2 = sqrt(P->normb2);
@@ -486,7 +521,8 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
     for (int i = 0; i < m; i++) { P->b[i] *= P->d[i]; P->du[i] = P->bs / P->d[i]; }
     P->cs = P->normC2 > 1 ? P->normC2 : 1.0;
     for (int i = 0; i < m; i++) P->b[i] /= P->bs;
-    for (int64_t q = 0; q < P->T; q++) P->C[q] /= P->cs;
+    if (!c_sparse) for (int64_t q = 0; q < P->T; q++) P->C[q] /= P->cs;
+    else for (int64_t q = 0; q < P->nc; q++) P->cval[q] /= P->cs;
     double nb2 = 0;
     for (int i = 0; i < m; i++) nb2 += P->b[i] * P->b[i];
     P->nbs = sqrt(nb2);

@@ -58,6 +58,10 @@ typedef struct {
     double Ntot;            /* sum of block orders (the barrier's dimension) */
     /* scaled data (problem.c:1866-1909) */
     double *b, *b0, *d, *du, *C;
+    /* C given by its nonzeros instead (C NULL): positions in the block arena, ascending, and values;
+     * the device arena is zeroed and these scattered into it (sparse C: max-cut's Laplacian) */
+    int64_t nc, *cpos;
+    double *cval;
     double normb1, normb2, normC1, normC2, bs, cs, nbs;
     /* constraint entries, upper (r <= c), CSR by constraint, grouped by block */
     int *cptr;

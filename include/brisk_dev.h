@@ -166,6 +166,9 @@ void bk_dot_mirror(int64_t off, int64_t len, double *host);
 /* out[q] = c0 in0[q] + c1 in1[q] + c2 in2[q] + c3 in3[q] for q < len (an offset < 0 means the
  * term is absent; out may alias any input) */
 typedef struct { int64_t len, out, in[4]; double c[4]; } LinItem;
+/* out[pos[i]] = val[i] for i < n (device arrays; positions in elements): a sparse array into a
+ * dense one that the caller zeroed */
+int bk_scatter_f64(const int64_t *pos, const double *val, int64_t n, double *out);
 int bk_lincomb(const LinItem *it, int n_it);
 /* out[q] = c0 in0[q] in1[q] (+ c1 in2[q] when in[2] >= 0) */
 int bk_vmul(const LinItem *it, int n_it);
