@@ -2814,7 +2814,7 @@ int ipm_end_batch(Solve **S, int ns) {
 static int metric_correct(Solve **S, int ns, const int *act0, int half, double *steps) {
     int rc = -1;
     int *act = calloc(ns, sizeof(int)), *ok = calloc(ns, sizeof(int)), *tst = calloc(ns, sizeof(int)), *pd = calloc(2 * ns, sizeof(int));
-    int *wd = calloc(ns, sizeof(int)), *a2 = calloc(2 * ns, sizeof(int));
+    int *wd = calloc(ns, sizeof(int));
     double *nr0 = calloc(ns, sizeof(double)), *al = calloc(ns, sizeof(double)), *rel = calloc(ns, sizeof(double));
     EigItem *ei = NULL;
     ScaleItem *s4 = NULL, *s0 = NULL;
@@ -2933,7 +2933,7 @@ static int metric_correct(Solve **S, int ns, const int *act0, int half, double *
     rc = 0;
 out:
     for (int q = 0; q < ns; q++) { Solve *s = S[q]; s->SX = s->X; s->SZi = s->Zi; s->SLX = s->LX; s->SLZi = s->LZi; s->want_double = wd[q]; }
-    free(act); free(ok); free(tst); free(pd); free(wd); free(a2); free(nr0); free(al); free(rel); free(ei); free(s4); free(s0);
+    free(act); free(ok); free(tst); free(pd); free(wd); free(nr0); free(al); free(rel); free(ei); free(s4); free(s0);
     return rc;
 }
 /* The end phase's metric corrections of points that missed the tolerance (reference
