@@ -472,7 +472,7 @@ def main():
     ap.add_argument("--workload", default="all", choices=["all"] + sorted(WORKLOADS) + ["c5"],
                     help="all: the c2 headline with c1, c3, c4, c5 beside it; else one workload as the headline")
     ap.add_argument("--requests", type=int, default=0, help="c5: requests of the stream to serve (default 4096)")
-    ap.add_argument("--slots", type=int, default=1024, help="c5: most solves in flight")
+    ap.add_argument("--slots", type=int, default=2048, help="c5: most solves in flight (admission also waits on device memory)")
     ap.add_argument("--load", type=float, default=0.8, help="c5: Poisson arrival rate as a fraction of the saturation rate")
     ap.add_argument("--cpu-stride", type=int, default=16, help="c5: the CPU baseline solves every k-th request")
     ap.add_argument("--cpu-serving-sample", type=int, default=0, help=argparse.SUPPRESS)
