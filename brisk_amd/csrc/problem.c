@@ -449,12 +449,13 @@ int hprob_build(HProb *P, int m, int nblk, const int *bs, const double *c, int64
     if (bad) { for (int p = 0; p < nparts; p++) free(sc.chg[p]); return 2; }
     /* C: dense in the block arena, or by its nonzeros where they are few (their count with the
      * mirrored off-diagonal entries at most an eighth of the arena; the dense route keeps it dense) */
-    int64_t ncin = 0;
-    for (int64_t q = 0; q < nnz; q++) if (mat[q] == 0 && v[q] != 0) ncin += ii[q] == jj[q] ? 1 : 2;
     /* dense route: one SDP block whose constraints fill at least a quarter of their upper
      * triangles (the reference's route 2, solver.c:2968-3016, taken for every constraint) */
     const int dense_route = nblk == 1 && bs[0] > 1 && m > 0 && (double)ncon >= 0.25 * (double)m * bs[0] * (bs[0] + 1) / 2.0 &&
                             (double)m * bs[0] * bs[0] * 8.0 < 64e9;
+    int64_t ncin = 0;
+    if (!dense_route)
+        for (int64_t q = 0; q < nnz; q++) if (mat[q] == 0 && v[q] != 0) ncin += ii[q] == jj[q] ? 1 : 2;
     const int c_sparse = !dense_route && 8 * ncin <= off;
     if (!c_sparse) P->C = calloc(off + 1, sizeof(double));
     else { P->cpos = malloc(sizeof(int64_t) * (ncin + 1)); P->cval = malloc(sizeof(double) * (ncin + 1)); }
