@@ -277,7 +277,7 @@ static int solve_many(int n, const BriskProblem *pr, int nopt, const char *const
     }
     bk_sync();
     const double t3 = bk_now();
-    for (int q = 0; q < n; q++) if (ipm_finish(SP[q])) { rc = -4; goto out; }
+    if (ipm_finish_batch(SP, n)) { rc = -4; goto out; }
     const double t4 = bk_now();
     for (int q = 0; q < n; q++) if (res[q]) brisk_fill_result(res[q], &S[q], t1 - t0, t2 - t1, t3 - t2, (t4 - t3) / n);
     if (bk_failed()) rc = -4;

@@ -1919,11 +1919,9 @@ static int dimacs_point(Solve **S, int ns) {
     int rc = -1;
     int *act = calloc(ns, sizeof(int)), *okx = calloc(ns, sizeof(int)), *okz = calloc(ns, sizeof(int));
     double *lam = calloc(ns, sizeof(double));
-    int mmax = 1, nsl = 1;
-    for (int q = 0; q < ns; q++) { act[q] = 1; if (S[q]->P.m > mmax) mmax = S[q]->P.m; nsl += S[q]->P.nb; }
-    double *hrp = malloc(sizeof(double) * mmax);
+    int nsl = 1;
+    for (int q = 0; q < ns; q++) { act[q] = 1; nsl += S[q]->P.nb; }
     if (batch_scratch(ns, nsl + ns)) goto out;
-    double *H = heapp();
     /* rp = A(X) - b (scaled); W1 = C - A'y (scaled Z on the data) */
     LinL_clear(&g_ll);
     for (int q = 0; q < ns; q++) {
@@ -1932,12 +1930,17 @@ static int dimacs_point(Solve **S, int ns) {
     }
     if (bk_lincomb(g_ll.v, g_ll.n)) goto out;
     if (aop(S, ns, act, OFF(X), OFF(rp), 1.0, 1.0) || aty(S, ns, act, OFF(y), OFF(W1), -1.0)) goto out;
+    /* t1 = du o rp: the residual on the data as read, its squared norm among the dots */
+    LinL_clear(&g_ll);
+    for (int q = 0; q < ns; q++) lin_add(S[q]->P.m, S[q]->t1, S[q]->rp, 1.0, S[q]->duv, 0, -1, 0, -1, 0);
+    if (bk_vmul(g_ll.v, g_ll.n)) goto out;
     DotL_clear(&g_dl);
     for (int q = 0; q < ns; q++) {
         Solve *s = S[q];
         dot_add(s->P.T, s->C, s->X, SC(q, 0));
         dot_add(s->P.T, s->X, s->W1, SC(q, 1));
         dot_add(s->P.m, s->bvec, s->y, SC(q, 2));
+        dot_add(s->P.m, s->t1, s->t1, SC(q, 3));
     }
     if (bk_dots(g_dl.v, g_dl.n) || read_scalars(ns)) goto out;
     for (int q = 0; q < ns; q++) {
@@ -1947,9 +1950,7 @@ static int dimacs_point(Solve **S, int ns) {
         s->res_pobj = sc * HS(q, 0);
         s->res_dobj = sc * HS(q, 2);
         const double xz = sc * HS(q, 1);
-        if (bk_d2h(hrp, H + s->rp, sizeof(double) * P->m)) goto out;
-        double r2 = 0;
-        for (int i = 0; i < P->m; i++) { const double v = hrp[i] * P->du[i]; r2 += v * v; }
+        const double r2 = HS(q, 3);
         const double den = 1 + fabs(s->res_pobj) + fabs(s->res_dobj);
         s->res_err[0] = sqrt(r2) / (1 + P->normb1);
         s->res_err[1] = 0; s->res_err[2] = 0;
@@ -2022,7 +2023,7 @@ static int dimacs_point(Solve **S, int ns) {
     for (int q = 0; q < ns; q++) S[q]->status = status_from_errors(S[q]->status, S[q]->res_err, S[q]->o.tol);
     rc = 0;
 out:
-    free(act); free(okx); free(okz); free(lam); free(hrp);
+    free(act); free(okx); free(okz); free(lam);
     return rc;
 }
 
@@ -2031,46 +2032,68 @@ out:
  * into G and Q (free after the evaluation) and downloaded through the pinned staging buffers:
  * straight into the result arrays when the problem is one SDP block (the arena's layout is the
  * result's), else into a host copy that is packed per input block. y_i = cs d_i y~_i on the host. */
-int ipm_finish(Solve *s) {
-    const HProb *P = &s->P;
-    const int m = P->m;
+/* the results of the solves to the host: X = bs X~, Z = cs W1 (the measure's Z on the data as
+ * read), y_i = cs d_i y~_i; one linear-combination launch and one batch of reads for all */
+int ipm_finish_batch(Solve **S, int ns) {
+    if (ns <= 0) return 0;
     double *H = heapp();
+    int rc = -1;
     LinL_clear(&g_ll);
-    lin_add(P->T, s->G, s->X, P->bs, -1, 0, -1, 0, -1, 0);
-    lin_add(P->T, s->Q, s->W1, P->cs, -1, 0, -1, 0, -1, 0);
+    for (int q = 0; q < ns; q++) {
+        Solve *s = S[q];
+        lin_add(s->P.T, s->G, s->X, s->P.bs, -1, 0, -1, 0, -1, 0);
+        lin_add(s->P.T, s->Q, s->W1, s->P.cs, -1, 0, -1, 0, -1, 0);
+    }
     if (bk_lincomb(g_ll.v, g_ll.n)) return -1;
-    double *y = malloc(sizeof(double) * (m + 1));
-    if (bk_d2h(y, H + s->y, sizeof(double) * m)) { free(y); return -1; }
-    for (int i = 0; i < m; i++) y[i] *= P->cs * P->d[i];                 /* y_i = cs d_i y~_i */
-    int64_t tot = 0;
-    for (int k = 0; k < P->nblk_in; k++) tot += P->bs_in[k] > 0 ? (int64_t)P->bs_in[k] * P->bs_in[k] : -P->bs_in[k];
-    free(s->res_X); free(s->res_Z); free(s->res_y);
-    s->res_X = malloc(sizeof(double) * (tot + 1));
-    s->res_Z = malloc(sizeof(double) * (tot + 1));
-    s->res_y = y;
-    if (P->nblk_in == 1 && P->bs_in[0] > 0 && tot == P->T) {
-        if (bk_d2h(s->res_X, H + s->G, sizeof(double) * P->T) || bk_d2h(s->res_Z, H + s->Q, sizeof(double) * P->T)) return -1;
-    } else {
-        double *X = malloc(sizeof(double) * (P->T + 1)), *Zint = malloc(sizeof(double) * (P->T + 1));
-        if (bk_d2h(X, H + s->G, sizeof(double) * P->T) || bk_d2h(Zint, H + s->Q, sizeof(double) * P->T)) { free(X); free(Zint); return -1; }
-        int64_t o = 0;
-        for (int k = 0; k < P->nblk_in; k++) {
-            const int kb = P->in_first[k];
-            if (P->bs_in[k] > 0) {
-                const int64_t len = (int64_t)P->bs_in[k] * P->bs_in[k];
-                memcpy(s->res_X + o, X + P->boff[kb], sizeof(double) * len);
-                memcpy(s->res_Z + o, Zint + P->boff[kb], sizeof(double) * len);
-                o += len;
-            } else {
-                for (int i = 0; i < -P->bs_in[k]; i++) { s->res_X[o + i] = X[P->boff[kb + i]]; s->res_Z[o + i] = Zint[P->boff[kb + i]]; }
-                o += -P->bs_in[k];
+    void **dst = malloc(sizeof(void *) * 3 * ns);
+    const void **src = malloc(sizeof(void *) * 3 * ns);
+    size_t *by = malloc(sizeof(size_t) * 3 * ns);
+    double **tx = calloc(ns, sizeof(double *)), **tz = calloc(ns, sizeof(double *));
+    for (int q = 0; q < ns; q++) {
+        Solve *s = S[q];
+        const HProb *P = &s->P;
+        int64_t tot = 0;
+        for (int k = 0; k < P->nblk_in; k++) tot += P->bs_in[k] > 0 ? (int64_t)P->bs_in[k] * P->bs_in[k] : -P->bs_in[k];
+        free(s->res_X); free(s->res_Z); free(s->res_y);
+        s->res_X = malloc(sizeof(double) * (tot + 1));
+        s->res_Z = malloc(sizeof(double) * (tot + 1));
+        s->res_y = malloc(sizeof(double) * (P->m + 1));
+        /* one SDP block: read straight into the results; else through the block arena */
+        const int direct = P->nblk_in == 1 && P->bs_in[0] > 0 && tot == P->T;
+        if (!direct) { tx[q] = malloc(sizeof(double) * (P->T + 1)); tz[q] = malloc(sizeof(double) * (P->T + 1)); }
+        dst[3 * q] = s->res_y; src[3 * q] = H + s->y; by[3 * q] = sizeof(double) * P->m;
+        dst[3 * q + 1] = direct ? s->res_X : tx[q]; src[3 * q + 1] = H + s->G; by[3 * q + 1] = sizeof(double) * P->T;
+        dst[3 * q + 2] = direct ? s->res_Z : tz[q]; src[3 * q + 2] = H + s->Q; by[3 * q + 2] = sizeof(double) * P->T;
+    }
+    if (bk_d2h_many(3 * ns, dst, src, by)) goto out;
+    for (int q = 0; q < ns; q++) {
+        Solve *s = S[q];
+        const HProb *P = &s->P;
+        for (int i = 0; i < P->m; i++) s->res_y[i] *= P->cs * P->d[i];
+        if (tx[q]) {
+            int64_t o = 0;
+            for (int k = 0; k < P->nblk_in; k++) {
+                const int kb = P->in_first[k];
+                if (P->bs_in[k] > 0) {
+                    const int64_t len = (int64_t)P->bs_in[k] * P->bs_in[k];
+                    memcpy(s->res_X + o, tx[q] + P->boff[kb], sizeof(double) * len);
+                    memcpy(s->res_Z + o, tz[q] + P->boff[kb], sizeof(double) * len);
+                    o += len;
+                } else {
+                    for (int i = 0; i < -P->bs_in[k]; i++) { s->res_X[o + i] = tx[q][P->boff[kb + i]]; s->res_Z[o + i] = tz[q][P->boff[kb + i]]; }
+                    o += -P->bs_in[k];
+                }
             }
         }
-        free(X); free(Zint);
+        s->t_solve = bk_now() - s->t_start;
     }
-    s->t_solve = bk_now() - s->t_start;
-    return 0;
+    rc = 0;
+out:
+    for (int q = 0; q < ns; q++) { free(tx[q]); free(tz[q]); }
+    free(tx); free(tz); free(dst); free(src); free(by);
+    return rc;
 }
+int ipm_finish(Solve *s) { return ipm_finish_batch(&s, 1); }
 
 /* ---- the homogeneous self-dual embedding (reference hsd_run, solver.c:9803-12211) ------------
  * Variables X, y, Z, tau, kappa with rp = tau b - A(X), Rd = tau C - A*(y) - Z, rg = kappa - b'y +
@@ -3059,8 +3082,7 @@ static int ipm_end_batch_(Solve **S, int ns) {
     const int e = ipm_dimacs(S, ns) || ipm_end_corrections(S, ns, st0) || ipm_polish(S, ns, st0);
     free(st0);
     if (e) return -1;
-    for (int q = 0; q < ns; q++) if (ipm_finish(S[q])) return -1;
-    return 0;
+    return ipm_finish_batch(S, ns);
 }
 
 /* ---- kernel-level probes: one kernel family of the iteration on caller data, through the same

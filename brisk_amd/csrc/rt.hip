@@ -24,6 +24,7 @@ static char *g_ring_h = nullptr, *g_ring_d = nullptr;
 // far more idle time than the few microseconds of such a kernel
 #define KCOPY_MAX ((size_t)1 << 20)
 static char *g_bounce = nullptr;      // pinned: destination of small device-to-host reads
+#define BOUNCE_BYTES ((size_t)32 << 20)   // its size: a batch of small reads lands in it before one wait
 // larger transfers go through two pinned buffers of XFER_CHUNK bytes in turn, so that the host's
 // copy of one chunk (to or from pageable memory) overlaps the DMA of the next: a DMA from
 // pageable memory runs at a few GB/s, from pinned memory at the link's rate
@@ -90,7 +91,7 @@ extern "C" int bk_init(void) {
     g_ring_pos = 0;
     if (bk_check(hipHostMalloc((void **)&g_cache_h, CACHE_CAP, hipHostMallocDefault), "hipHostMalloc cache")) return -1;
     if (bk_check(hipMalloc((void **)&g_cache_d, CACHE_CAP), "hipMalloc cache")) return -1;
-    if (bk_check(hipHostMalloc((void **)&g_bounce, KCOPY_MAX, hipHostMallocDefault), "hipHostMalloc bounce")) return -1;
+    if (bk_check(hipHostMalloc((void **)&g_bounce, BOUNCE_BYTES, hipHostMallocDefault), "hipHostMalloc bounce")) return -1;
     for (int b = 0; b < 2; b++) {
         if (bk_check(hipHostMalloc((void **)&g_xfer[b], XFER_CHUNK, hipHostMallocDefault), "hipHostMalloc transfer")) return -1;
         if (bk_check(hipEventCreateWithFlags(&g_xfer_ev[b], hipEventDisableTiming), "hipEventCreate transfer")) return -1;
@@ -233,6 +234,35 @@ extern "C" int bk_d2h(void *dst, const void *src, size_t bytes) {
     if (bk_check(hipEventSynchronize(g_xfer_ev[lb]), "hipEventSynchronize D2H")) return -1;
     pcopy((char *)dst + prev_off, g_xfer[lb], prev_len);
     return bk_sync();
+}
+// n device-to-host reads with one wait per 32 MB: the small ones (up to KCOPY_MAX each) are copied
+// by kernels into the pinned bounce buffer one after another, then the stream is waited for once
+// and each is copied out; a larger one goes through bk_d2h on its own
+extern "C" int bk_d2h_many(int n, void *const *dst, const void *const *src, const size_t *bytes) {
+    int i = 0;
+    while (i < n) {
+        size_t used = 0;
+        int j = i;
+        for (; j < n; j++) {
+            const size_t al = (bytes[j] + 255) & ~(size_t)255;
+            if (bytes[j] > KCOPY_MAX || used + al > BOUNCE_BYTES) break;
+            if (bytes[j] && kcopy(g_bounce + used, src[j], bytes[j], "d2h")) return -1;
+            used += al;
+        }
+        if (j == i) {
+            if (bk_d2h(dst[i], src[i], bytes[i])) return -1;
+            i++;
+            continue;
+        }
+        if (bk_sync()) return -1;
+        size_t off = 0;
+        for (int t = i; t < j; t++) {
+            if (bytes[t]) memcpy(dst[t], g_bounce + off, bytes[t]);
+            off += (bytes[t] + 255) & ~(size_t)255;
+        }
+        i = j;
+    }
+    return 0;
 }
 extern "C" int bk_d2d(void *dst, const void *src, size_t bytes) {
     if (!bytes) return 0;

@@ -187,6 +187,7 @@ int ipm_dimacs(Solve **S, int ns);          /* DIMACS errors and status of the r
 int ipm_end_corrections(Solve **S, int ns, const int *st0);  /* after ipm_dimacs: the square-root-metric restoration and the X-metric polish of points that missed the tolerance */
 int ipm_polish(Solve **S, int ns, const int *st0);  /* after ipm_dimacs: the returned points polished where that helps (st0: the runs' statuses before it) */
 int ipm_end_batch(Solve **S, int ns);       /* DIMACS on the device, polish, status, results to the host */
+int ipm_finish_batch(Solve **S, int ns);   /* results of the solves to the host, one batch of reads */
 int ipm_finish(Solve *S);                   /* results of one solve to the host (after ipm_end_batch's DIMACS) */
 void solve_log(const char *fmt, ...);
 void ipm_release_scratch(void);           /* frees the batch scratch (before the heap is re-sized) */

@@ -25,6 +25,8 @@ void bk_host_free(void *p);
 int bk_h2d(void *dst, const void *src, size_t bytes);        /* synchronous copy */
 int bk_h2d_async(void *dst, const void *src, size_t bytes);  /* copy ordered on the stream; src may be reused at once */
 int bk_d2h(void *dst, const void *src, size_t bytes);        /* synchronous (drains the stream) */
+/* n device-to-host reads (dst[i] <- src[i], bytes[i]) with one wait per 32 MB of small reads */
+int bk_d2h_many(int n, void *const *dst, const void *const *src, const size_t *bytes);
 int bk_d2d(void *dst, const void *src, size_t bytes);        /* async device copy */
 int bk_zero(void *dst, size_t bytes);                         /* async fill kernel (graph safe) */
 int bk_sync(void);
