@@ -965,8 +965,12 @@ static int schur_factor_(Solve **S, int ns, const int *act, int force64, int *fa
             if (rc) { bk_join(); goto err; }
         }
         if (bk_join()) goto err;
-        if (bk_d2h(g_hint, dint(), sizeof(int) * ns)) goto err;
-        if (bk_d2h(g_hdbl + ns, pm64, sizeof(double) * 2 * ns)) goto err;
+        {   /* the factors' failure columns and smallest pivots, one wait for both */
+            void *dst[2] = { g_hint, g_hdbl + ns };
+            const void *src[2] = { dint(), pm64 };
+            const size_t by[2] = { sizeof(int) * ns, sizeof(double) * 2 * ns };
+            if (bk_d2h_many(2, dst, src, by)) goto err;
+        }
         const float *hp32 = (const float *)(g_hdbl + 2 * ns);
         for (int q = 0; q < ns; q++) {
             if (ran[q] < 0) continue;
