@@ -255,13 +255,14 @@ def test_trsv_batched(dt, sizes):
         assert np.abs(zk - ref).max() <= (1e-9 if f64 else 2e-3) * np.abs(ref).max(), n
 
 
-def test_gemv_triangular_ranges():
-    """trans = 1 with tri 1 (A lower: rows >= j) and tri 2 (A upper: rows <= j)."""
+@pytest.mark.parametrize("n", [333, 700, 1500])
+def test_gemv_triangular_ranges(n):
+    """trans = 1 with tri 1 (A lower: rows >= j) and tri 2 (A upper: rows <= j); orders 700 and
+    1500 split each column over two and four waves (segments of the triangular row range)."""
     rng = np.random.default_rng(9)
-    n = 333
     Lw = np.tril(rng.standard_normal((n, n)))
     x = rng.standard_normal(n)
-    for tri, A in ((1, Lw), (2, Lw.T.copy())):
+    for tri, A in ((0, Lw), (1, Lw), (2, Lw.T.copy())):
         dA = DBuf(A.flatten(order="F")); dx = DBuf(x); dy = DBuf(np.zeros(n))
         it = items_array(GemvItem, [(n, n, n, 0, 0, 0, tri)])
         assert lib().bk_gemv_f64(1, 1.0, 0.0, dA.p, dx.p, dy.p, it, 1) == 0
@@ -271,7 +272,7 @@ def test_gemv_triangular_ranges():
 @pytest.mark.parametrize("trans", [0, 1])
 def test_gemv_batched(trans):
     rng = np.random.default_rng(5)
-    shapes = [(1, 1), (100, 37), (257, 513), (64, 64)]
+    shapes = [(1, 1), (100, 37), (257, 513), (64, 64), (600, 9), (1100, 45), (5000, 3)]
     As, xs, ys, rows, refs = [], [], [], [], []
     oa = ox = oy = 0
     for (M, N) in shapes:
